@@ -1,0 +1,203 @@
+"""MI355X-native CrystalScatter per-pixel 2D scattering path.
+
+Python mirror of the reference engine interface (SasCalc_GENERIC_calculation,
+Source/sc_calc_generic_gpu.h:77-1000) over the C ABI of libcsx.so
+(include/csx.h).  Every pixel is evaluated by the HIP kernels; there is no
+CPU fallback.
+
+    from crystalscatter_amd import SasCalc
+    calc = SasCalc(device=0, params={"LType": 5, "EditRadius": 8, ...})
+    img = calc.doCalculation()            # IDX order, like data() of the reference
+    calc.image2d(img)                     # rows = i (qx), columns = ihex (qy)
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import (CSX_PREC_MIXED, CSX_PREC_REF64, CsxError, Grid, Mask, NCOEF, NIM2, Scalars,
+                   lib)
+
+__all__ = ["SasCalc", "Context", "CsxError", "CSX_PREC_MIXED", "CSX_PREC_REF64", "PRECISIONS"]
+
+PRECISIONS = {"ref64": CSX_PREC_REF64, "mixed": CSX_PREC_MIXED, CSX_PREC_REF64: CSX_PREC_REF64,
+              CSX_PREC_MIXED: CSX_PREC_MIXED}
+
+
+def _prec(p):
+    if p not in PRECISIONS:
+        raise ValueError("precision must be 'ref64' or 'mixed'")
+    return PRECISIONS[p]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class SasCalc:
+    """Engine with the reference's key=value parameters (Source/sc_calc_generic.cpp:36-138).
+
+    device < 0 gives a host-only engine (preparation and tables, no GPU work).
+    """
+
+    def __init__(self, device=0, params=None):
+        self._L = lib()
+        self._h = self._L.csx_engine_new(int(device))
+        if not self._h:
+            raise CsxError("could not create the engine on device %d (no HIP device?)" % device)
+        self.device = device
+        if params:
+            self.set_params(params)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.csx_engine_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _err(self):
+        b = ctypes.create_string_buffer(512)
+        self._L.csx_engine_last_error(self._h, b, 512)
+        return b.value.decode(errors="replace")
+
+    def set(self, key, value):
+        if self._L.csx_engine_set(self._h, key.encode(), float(value)) != 0:
+            raise KeyError("unknown parameter key %r" % key)
+
+    def set_params(self, params):
+        for k, v in params.items():
+            self.set(k, v)
+
+    def get(self, key):
+        return self._L.csx_engine_get(self._h, key.encode())
+
+    def prepare(self):
+        rc = self._L.csx_engine_prepare(self._h)
+        if rc != 0:
+            raise CsxError("prepare failed (%d): %s" % (rc, self._err()))
+
+    def scalars(self):
+        return self._L.csx_engine_scalars(self._h).contents
+
+    def bounds(self):
+        s = self.scalars()
+        return s.xmin, s.xmax, s.ymin, s.ymax
+
+    def doCalculation(self, precision="mixed"):
+        """Image in the reference IDX order ((ihex-xmin) + W*(i-ymin)), like data()."""
+        self.prepare()
+        x0, x1, y0, y1 = self.bounds()
+        out = np.empty((x1 - x0) * (y1 - y0), dtype=np.float64)
+        b = (ctypes.c_int32 * 4)()
+        rc = self._L.csx_engine_calc(self._h, _prec(precision), _dp(out), b)
+        if rc != 0:
+            raise CsxError("doCalculation failed (%d): %s" % (rc, self._err()))
+        return out
+
+    def doCalculationDevice(self, d_out_ptr, precision="mixed"):
+        """Image into caller-owned device memory (e.g. torch tensor .data_ptr())."""
+        b = (ctypes.c_int32 * 4)()
+        rc = self._L.csx_engine_calc_device(self._h, _prec(precision), ctypes.c_void_p(d_out_ptr), b)
+        if rc != 0:
+            raise CsxError("doCalculation failed (%d): %s" % (rc, self._err()))
+        return tuple(b)
+
+    def image2d(self, flat):
+        x0, x1, y0, y1 = self.bounds()
+        return np.asarray(flat).reshape(y1 - y0, x1 - x0)
+
+    def setFitData(self, w, h, data):
+        data = np.ascontiguousarray(data, dtype=np.float64).ravel()
+        if data.size != w * h:
+            raise ValueError("fit data must hold w*h values")
+        rc = self._L.csx_engine_set_fit_data(self._h, int(w), int(h), _dp(data))
+        if rc != 0:
+            raise CsxError("setFitData failed (%d): %s" % (rc, self._err()))
+
+    def setNoFitRect(self, idx, x0, y0, x1, y1):
+        if self._L.csx_engine_set_nofit_rect(self._h, idx, x0, y0, x1, y1) != 0:
+            raise ValueError("rectangle index must be 0..3")
+
+    def doFitCalculation(self, bstop=0, border=0, precision="mixed"):
+        """Returns (sum, cnt, nancnt) like doFitCalculation(numThreads=GPU, bstop, border, cnt, nancnt)."""
+        s = ctypes.c_double()
+        c, n = ctypes.c_int64(), ctypes.c_int64()
+        rc = self._L.csx_engine_fit(self._h, _prec(precision), int(bstop), int(border), ctypes.byref(s),
+                                    ctypes.byref(c), ctypes.byref(n))
+        if rc != 0:
+            raise CsxError("doFitCalculation failed (%d): %s" % (rc, self._err()))
+        return s.value, c.value, n.value
+
+    def table(self, name):
+        n, kind = ctypes.c_int64(), ctypes.c_int()
+        p = self._L.csx_engine_table(self._h, name.encode(), ctypes.byref(n), ctypes.byref(kind))
+        if not p:
+            raise KeyError(name)
+        ct = {0: ctypes.c_double, 1: ctypes.c_int32, 2: ctypes.c_float}[kind.value]
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct * n.value)).contents).copy()
+
+    def last_kernel_ms(self):
+        return self._L.csx_last_kernel_ms(self._L.csx_engine_ctx(self._h))
+
+
+class Context:
+    """Device context of the C ABI (csx_create): upload / render / batched objective."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.csx_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise CsxError("csx_create failed (%d)" % rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.csx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _err(self):
+        b = ctypes.create_string_buffer(512)
+        self._L.csx_last_error(self._h, b, 512)
+        return b.value.decode(errors="replace")
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise CsxError("%s failed (%d): %s" % (what, rc, self._err()))
+
+    def set_stream(self, stream_ptr):
+        self._chk(self._L.csx_set_stream(self._h, ctypes.c_void_p(stream_ptr)), "csx_set_stream")
+
+    def upload(self, scalars, carr, carr11pm=None, latpar1=None, latpar2=None, latpar3=None):
+        carr = np.ascontiguousarray(carr, dtype=np.float64)
+        args = [self._h, ctypes.byref(scalars), _dp(carr)]
+        keep = [carr]
+        if carr11pm is not None and len(carr11pm):
+            c11 = np.ascontiguousarray(carr11pm, dtype=np.float64)
+            keep.append(c11)
+            args.append(_dp(c11))
+        else:
+            args.append(None)
+        for a, ct in ((latpar1, ctypes.c_int32), (latpar2, ctypes.c_int32), (latpar3, ctypes.c_float)):
+            if a is None:
+                args.append(None)
+            else:
+                arr = np.ascontiguousarray(a, dtype=np.int32 if ct is ctypes.c_int32 else np.float32)
+                keep.append(arr)
+                args.append(arr.ctypes.data_as(ctypes.POINTER(ct)))
+        self._chk(self._L.csx_upload(*args), "csx_upload")
+
+    def render(self, grid, d_out_ptr, precision="mixed"):
+        g = Grid(*grid)
+        self._chk(self._L.csx_render(self._h, ctypes.byref(g), _prec(precision), ctypes.c_void_p(d_out_ptr)),
+                  "csx_render")
+
+    def last_kernel_ms(self):
+        return self._L.csx_last_kernel_ms(self._h)
+
+    def synchronize(self):
+        self._chk(self._L.csx_synchronize(self._h), "csx_synchronize")

@@ -1,0 +1,101 @@
+"""Loader of libcsx.so (the HIP kernels + C ABI declared in include/csx.h).
+
+There is no fallback: if the in-tree library is missing or fails to load,
+every entry point raises.  Build it with ``python -m crystalscatter_amd.build``.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcsx.so")
+
+CSX_PREC_REF64 = 0
+CSX_PREC_MIXED = 1
+NCOEF = 151
+NIM2 = 131
+
+
+class CsxError(RuntimeError):
+    pass
+
+
+class Scalars(ctypes.Structure):
+    """Mirror of csx_scalars (include/csx.h)."""
+    _fields_ = [
+        ("combo_key", ctypes.c_uint32), ("zmax", ctypes.c_int32), ("centerbeam", ctypes.c_int32),
+        ("orcase", ctypes.c_int32), ("fix_orcase", ctypes.c_int32), ("peakmax1", ctypes.c_int32),
+        ("peakmax2", ctypes.c_int32), ("dsch", ctypes.c_int32), ("para", ctypes.c_int32),
+        ("twin", ctypes.c_int32), ("ordis", ctypes.c_int32),
+    ] + [(n, ctypes.c_double) for n in (
+        "qmax", "pixnox", "pixnoy", "pixx_m", "pixy_m", "det", "wave", "beamx", "beamy",
+        "radius", "sigma", "length", "sigmal", "dbeta", "polTheta", "polPhi",
+        "cosphi", "sinphi", "costheta", "sintheta", "cosphic", "sinphic")] + [
+        ("limq", ctypes.c_double * 18), ("norm", ctypes.c_double),
+    ] + [(n, ctypes.c_double) for n in (
+        "base", "i0", "iso", "ifluc", "rfluc", "dwfactor", "uca", "ucb", "domainsize",
+        "width_zuf", "reldis", "dist", "phiwidth", "ceff", "sphno_cubevol")] + [
+        ("xmin", ctypes.c_int32), ("xmax", ctypes.c_int32), ("ymin", ctypes.c_int32),
+        ("ymax", ctypes.c_int32), ("iimin", ctypes.c_int32), ("pad0", ctypes.c_int32),
+    ]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("xmin", ctypes.c_int32), ("xmax", ctypes.c_int32), ("ymin", ctypes.c_int32),
+                ("ymax", ctypes.c_int32)]
+
+
+class Mask(ctypes.Structure):
+    _fields_ = [("rects", ctypes.c_int32 * 16), ("bstop", ctypes.c_int32), ("border", ctypes.c_int32)]
+
+
+_LIB = None
+
+# (name, restype, argtypes) of every exported entry point of include/csx.h.
+_VP, _CP, _D, _I = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int
+_PD, _PI32, _PF = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = [
+    ("csx_create", _I, [_I, ctypes.POINTER(_VP)]),
+    ("csx_destroy", None, [_VP]),
+    ("csx_set_stream", _I, [_VP, _VP]),
+    ("csx_synchronize", _I, [_VP]),
+    ("csx_upload", _I, [_VP, ctypes.POINTER(Scalars), _PD, _PD, _PI32, _PI32, _PF]),
+    ("csx_render", _I, [_VP, ctypes.POINTER(Grid), _I, _VP]),
+    ("csx_qgrid", _I, [_VP, ctypes.POINTER(Grid), _VP]),
+    ("csx_fit_fqs", _I, [_VP, _I, _I, ctypes.POINTER(Mask), _I, _VP, _PD, _PI64, _PI64, _VP]),
+    ("csx_fit_batch_begin", _I, [_VP]),
+    ("csx_fit_batch_stage", _I, [_VP, ctypes.POINTER(Scalars), _PD, _PI32, _PI32, _PF, _I, _I,
+                                 ctypes.POINTER(Mask), _VP]),
+    ("csx_fit_batch_run", _I, [_VP, _I, _PD, _PI64]),
+    ("csx_last_error", _I, [_VP, _CP, _I]),
+    ("csx_last_kernel_ms", _D, [_VP]),
+    ("csx_engine_new", _VP, [_I]),
+    ("csx_engine_free", None, [_VP]),
+    ("csx_engine_set", _I, [_VP, _CP, _D]),
+    ("csx_engine_get", _D, [_VP, _CP]),
+    ("csx_engine_prepare", _I, [_VP]),
+    ("csx_engine_calc", _I, [_VP, _I, _PD, _PI32]),
+    ("csx_engine_calc_device", _I, [_VP, _I, _VP, _PI32]),
+    ("csx_engine_set_fit_data", _I, [_VP, _I, _I, _PD]),
+    ("csx_engine_set_nofit_rect", _I, [_VP, _I, _I, _I, _I, _I]),
+    ("csx_engine_fit", _I, [_VP, _I, _I, _I, _PD, _PI64, _PI64]),
+    ("csx_engine_table", _VP, [_VP, _CP, _PI64, ctypes.POINTER(_I)]),
+    ("csx_engine_scalars", ctypes.POINTER(Scalars), [_VP]),
+    ("csx_engine_ctx", _VP, [_VP]),
+    ("csx_engine_last_error", _I, [_VP, _CP, _I]),
+]
+
+
+def lib():
+    """Load libcsx.so; raises CsxError if it is missing (no fallback path exists)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CsxError("libcsx.so is not built: run `python -m crystalscatter_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB

@@ -1,0 +1,844 @@
+// csx_device.hip -- HIP kernels (gfx950) and the device half of the C ABI.
+//
+// One thread per detector pixel, 16x16-pixel tiles per 256-thread block
+// (4 wave64s, each a 16x4 strip), coalesced FP64 stores in the reference's IDX
+// order.  Per-pixel form factors run in FP64 with the reference formulas; the
+// Gaussian peak sums run either in FP64 exactly as the reference writes them
+// (CSX_PREC_REF64) or in FP32 over Friedel pairs (CSX_PREC_MIXED): +hkl and
+// -hkl share |q^.u| and |q_hkl|, so one acos/sqrt serves both, their angular
+// and radial Gaussians fold into one exp2 each, and partial sums go to an FP64
+// accumulator.  Pairs whose |q^.u| is within 1e-6 of 1 are evaluated in FP64
+// exactly as the reference does (its acos clamp maps out-of-range cosines to
+// swapped angles, which decides those pixels).  The pair table is read with
+// wave-uniform addresses, i.e. through the scalar cache.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "csx.h"
+#include "csx_physics.h"
+
+namespace csx {
+
+constexpr int kTile = 16;         // pixels per tile edge
+constexpr int kBlock = kTile * kTile;
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr double kLog2e = 1.4426950408889634074;
+
+enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4 };
+
+inline int kind_of(uint32_t key) {
+    switch (key) {
+    case CSX_KEY_SPHERE_NONE_ISO: return K_SPHERE;
+    case CSX_KEY_SPHERE_FCC_GAUSS: return K_FCC;
+    case CSX_KEY_CYL_NONE_GAUSS: return K_CYL_GAUSS;
+    case CSX_KEY_CYL_NONE_ZDIR: return K_CYL_ZDIR;
+    case CSX_KEY_CYL_BCT_ZDIR: return K_BCT;
+    default: return -1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian peak sums
+// ---------------------------------------------------------------------------
+
+// The reference's acos with its out-of-range mapping (Source/sc_calc_generic_gpu.cu:31).
+__device__ __forceinline__ double ref_acos(double x) { return (x < -1) ? 0.0 : ((x > +1) ? kPiD : acos(x)); }
+
+// One anisotropic reflection exactly as the reference evaluates it
+// (Source/sc_gpu_pSph_pGau_lFCC_oGau.h:85-142), without the 1/(pi*width/2) factor.
+__device__ double aniso_term_ref(const DevSet& d, int ii, double qx, double qy, double qz, double q,
+                                 double width, double x2phi) {
+    const float* L3 = d.latpar3 + ii * 17;
+    const int32_t* L2 = d.latpar2 + ii * 6;
+    const double qhkl = L3[5];
+    const int mf = L2[4] * L2[5];
+    if (!(qhkl > 0 && mf != 0)) return 0.0;
+    const double yphi = ref_acos((qx * (double)L3[2] + qy * (double)L3[3] + qz * (double)L3[4]) / (q * qhkl));
+    const double x2 = (q - qhkl) * (q - qhkl) / (width * width);
+    const double sq = exp(-4 * x2 / kPiD) / (kPiD * width / 2.0);
+    const double psi = (double)L3[10] * exp(-x2phi * yphi * yphi) / (double)L3[11];
+    return sq * mf * psi;
+}
+
+// Reference-order FP64 sums: intensity over the anisotropic list (stopping at the first
+// inf, :131-140) and radintensity over the isotropic list (reading the anisotropic q_hkl
+// column, :69-83).
+__device__ void lattice_ref64(const DevSet& d, double qx, double qy, double qz, double q, double width,
+                              double& inten, double& radint) {
+    const csx_scalars& s = d.s;
+    const double widthiso = 1.0 / s.uca;
+    radint = 0.0;
+    for (int ii = 1; ii <= s.peakmax1; ++ii) {
+        const int mhkl = d.latpar1[ii * 6 + 4], fhkl = d.latpar1[ii * 6 + 5];
+        if (mhkl == 0 || fhkl == 0) continue;
+        const double qhkl = d.latpar3[ii * 17 + 5];
+        const double x2 = sqd(q - qhkl) / sqd(widthiso);
+        const double sq = exp(-4 * x2 / kPiD) / (kPiD * widthiso / 2.0);
+        radint += sq * mhkl * fhkl;
+    }
+    const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));
+    inten = 0.0;
+    for (int ii = 1; ii <= s.peakmax2; ++ii) {
+        const double t = aniso_term_ref(d, ii, qx, qy, qz, q, width, x2phi);
+        const double sav = inten;
+        inten += t;
+        if (isinf(inten)) { inten = sav; break; }
+    }
+}
+
+// acos on [0,1] as sqrt(1-x)*P7(x), Abramowitz & Stegun 4.4.46 (|error| <= 2e-8).
+__device__ __forceinline__ float acos01(float x) {
+    float p = -0.0012624911f;
+    p = __builtin_fmaf(p, x, 0.0066700901f);
+    p = __builtin_fmaf(p, x, -0.0170881256f);
+    p = __builtin_fmaf(p, x, 0.0308918810f);
+    p = __builtin_fmaf(p, x, -0.0501743046f);
+    p = __builtin_fmaf(p, x, 0.0889789874f);
+    p = __builtin_fmaf(p, x, -0.2145988016f);
+    p = __builtin_fmaf(p, x, 1.5707963050f);
+    return __builtin_amdgcn_sqrtf(1.0f - x) * p;
+}
+
+// Mixed-precision sums.  Pair record (2 x float4): (ux,uy,uz,qhkl) and (A, idx+, idx-, 0)
+// with A = mhkl*fhkl*g3/peaknorm1; the caller multiplies intensity by 2/(pi*width).
+__device__ void lattice_mixed(const DevSet& d, double qx, double qy, double qz, double q, double width,
+                              double& inten_scaled, double& inten_exact, double& radint) {
+    const csx_scalars& s = d.s;
+    const double rq = 1.0 / q;
+    const float hx = (float)(qx * rq), hy = (float)(qy * rq), hz = (float)(qz * rq);
+    // q split into two floats: t = (qhi - qhkl) + qlo keeps t accurate to float rounding
+    // of t itself (the radial Gaussian is narrow, so its exponent is steep in t).
+    const float qf = (float)q;
+    const float qlo = (float)(q - (double)qf);
+    const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));
+    const float kr = (float)(-4.0 / (kPiD * width * width) * kLog2e);
+    const float kp = (float)(-x2phi * kLog2e);
+    double acc = 0.0, exact = 0.0;
+    float part = 0.0f;
+    const float4* __restrict__ pr = d.pairs;
+    const int np = d.npairs;
+    for (int p = 0; p < np; ++p) {
+        const float4 a = pr[2 * p];
+        const float4 b = pr[2 * p + 1];
+        const float dt = __builtin_fmaf(hz, a.z, __builtin_fmaf(hy, a.y, hx * a.x));
+        const float ad = fabsf(dt);
+        if (ad > 0.999999f) {
+            exact += aniso_term_ref(d, __float_as_int(b.y), qx, qy, qz, q, width, x2phi);
+            exact += aniso_term_ref(d, __float_as_int(b.z), qx, qy, qz, q, width, x2phi);
+            continue;
+        }
+        const float y = acos01(ad);
+        const float ym = kPiF - y;
+        const float t = (qf - a.w) + qlo;
+        const float r = (t * kr) * t;
+        const float e1 = __builtin_fmaf(kp * y, y, r);
+        const float e2 = __builtin_fmaf(kp * ym, ym, r);
+        const float w = __builtin_amdgcn_exp2f(e1) + __builtin_amdgcn_exp2f(e2);
+        part = __builtin_fmaf(b.x, w, part);
+        if ((p & 15) == 15) { acc += (double)part; part = 0.0f; }
+    }
+    acc += (double)part;
+    // Reflections without a Friedel partner (not produced by FCC/BCT tables).
+    const float4* sg = pr + 2 * np;
+    for (int k = 0; k < d.nsingle; ++k) {
+        exact += aniso_term_ref(d, __float_as_int(sg[k].x), qx, qy, qz, q, width, x2phi);
+    }
+    inten_scaled = acc;
+    inten_exact = exact;
+    radint = 0.0;
+    if (s.iso != 0) {
+        const double wi = 1.0 / s.uca;
+        const float ki = (float)(-4.0 / (kPiD * wi * wi) * kLog2e);
+        float ra = 0.0f;
+        for (int k = 0; k < d.niso; ++k) {
+            const float2 e = d.iso[k];
+            const float t = qf - e.x;
+            ra = __builtin_fmaf(e.y, __builtin_amdgcn_exp2f((t * ki) * t), ra);
+        }
+        radint = (double)ra / (kPiD * wi / 2.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pixel value of each combination
+// ---------------------------------------------------------------------------
+template <int KIND, int PREC>
+__device__ double pixel_value(const DevSet& d, double qx, double qy, double qz) {
+    const csx_scalars& s = d.s;
+    const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
+    if constexpr (KIND == K_SPHERE) {
+        const double pq = pq_sphere(d, q);
+        return s.base + s.i0 * (pq + s.iso * pq) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+    } else if constexpr (KIND == K_CYL_ZDIR) {
+        double pq;
+        switch (s.orcase) {
+        case 0: pq = pq_cyl_long_zdir(d, qx * s.cosphi * s.sintheta, qy * s.sinphi * s.sintheta, q); break;
+        case 1: pq = pq_cyl_long_zdir(d, qx, 0, q); break;
+        case 2: pq = pq_cyl_long_zdir(d, 0, qy, q); break;
+        default: pq = pq_cyl_long_zdir(d, qx, qy, q); break;
+        }
+        pq *= pq_cyl_transverse(d, q);
+        return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+    } else if constexpr (KIND == K_CYL_GAUSS) {
+        // Gaussian orientation, general case in reference mode: outside the small-q series
+        // the reference's Romberg integrand is identically zero (orientation-case numbering),
+        // inside it the orientation-case branches leave the longitudinal factor at 1.
+        const double qxs = qx * s.cosphic - qy * s.sinphic, qys = qx * s.sinphic + qy * s.cosphic;
+        (void)qxs; (void)qys;
+        double pql = 0.0;
+        if (q < (0.4 * s.limq[0])) pql = 1.0;
+        const double pq = pql * pq_cyl_transverse(d, q);
+        return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+    } else {
+        double pq, fq;
+        if constexpr (KIND == K_FCC) {
+            pq = pq_sphere(d, q);
+            fq = fq_sphere(d, q);
+        } else {
+            switch (s.orcase) {
+            case 0:
+                pq = pq_cyl_long_zdir(d, qx * s.cosphi * s.sintheta, qy * s.sinphi * s.sintheta, q);
+                fq = fq_cyl_zdir(d, sqrt(s.cosphi * qx * qx + s.sinphi * qy * qy + 0.000000001),
+                                 qx * s.cosphi * s.sintheta, qy * s.sinphi * s.sintheta, q);
+                break;
+            case 1: pq = pq_cyl_long_zdir(d, qx, 0, q); fq = fq_cyl_zdir(d, fabs(qx), qx, 0, q); break;
+            case 2: pq = pq_cyl_long_zdir(d, 0, qy, q); fq = fq_cyl_zdir(d, fabs(qy), 0, qy, q); break;
+            default: pq = pq_cyl_long_zdir(d, qx, qy, q); fq = fq_cyl_zdir(d, q, qx, qy, q); break;
+            }
+            pq *= pq_cyl_transverse(d, q);
+        }
+        double width = 1;
+        if (s.dsch) width = 4.0 / s.domainsize;
+        if (s.para) width = (4.0 / s.domainsize) + sqd(s.reldis) * s.dist * sqd(q);
+        width *= s.width_zuf;
+        double inten, radint;
+        const double dwe = exp(-s.dwfactor * q * q);
+        if constexpr (PREC == CSX_PREC_REF64) {
+            lattice_ref64(d, qx, qy, qz, q, width, inten, radint);
+        } else {
+            double scaled, exact;
+            lattice_mixed(d, qx, qy, qz, q, width, scaled, exact, radint);
+            inten = scaled / (kPiD * width / 2.0) + exact;
+            // Conditioning guard: szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
+            // The FP32 peak sum carries ~3e-6 relative error, which szq amplifies by |B|/|szq|;
+            // pixels where that could exceed 1e-5 are re-summed in FP64 (reference order).
+            const double B = (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol) * dwe;
+            const double szq_est = 1 + (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe;
+            if (!(fabs(B) * 3.0e-6 <= 1.0e-5 * fabs(szq_est)))
+                lattice_ref64(d, qx, qy, qz, q, width, inten, radint);
+        }
+        const double szqiso = (1 + (1) * (8 * kPiD * kPiD * kPiD * radint / (4 * kPiD * s.sphno_cubevol) - 1) * exp(-s.ucb * q * q));
+        const double szq = (1 + (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
+        const double pqiso = (KIND == K_FCC) ? pq : 1.0;
+        return s.base + s.i0 * (szq * pq + s.iso * szqiso * pqiso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out) {
+    const int ihex = g.xmin + blockIdx.x * kTile + (threadIdx.x & (kTile - 1));
+    const int i = g.ymin + blockIdx.y * kTile + (threadIdx.x / kTile);
+    if (ihex >= g.xmax || i >= g.ymax) return;
+    double qx, qy, qz;
+    pixel_q(d.s, ihex, i, qx, qy, qz);
+    const double v = pixel_value<KIND, PREC>(d, qx, qy, qz);
+    const size_t W = (size_t)(g.xmax - g.xmin);
+    out[(size_t)(ihex - g.xmin) + W * (size_t)(i - g.ymin)] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_qgrid(const csx_scalars s, const csx_grid g, double* __restrict__ out) {
+    const int ihex = g.xmin + blockIdx.x * kTile + (threadIdx.x & (kTile - 1));
+    const int i = g.ymin + blockIdx.y * kTile + (threadIdx.x / kTile);
+    if (ihex >= g.xmax || i >= g.ymax) return;
+    double qx, qy, qz;
+    pixel_q(s, ihex, i, qx, qy, qz);
+    const size_t k = (size_t)(ihex - g.xmin) + (size_t)(g.xmax - g.xmin) * (size_t)(i - g.ymin);
+    out[3 * k] = qx;
+    out[3 * k + 1] = qy;
+    out[3 * k + 2] = qz;
+}
+
+// checkFitRanges (Source/sc_calc_generic_gpu.cu:293-347): true if pixel (x,y) of the fit
+// array is excluded.
+__device__ __forceinline__ bool fit_masked(const csx_scalars& s, const csx_mask& m, const double* data, int w,
+                                           int x, int y) {
+    for (int r = 0; r < 4; ++r) {
+        const int* R = m.rects + 4 * r;
+        if (R[0] < 0) continue;
+        if (x < R[0] || R[2] < x || y < R[1] || R[3] < y) continue;
+        return true;
+    }
+    if (m.border > 0 || m.bstop > 0) {
+        if (x < s.xmin + m.border || x >= s.xmax - m.border || y < s.ymin + m.border || y >= s.ymax - m.border)
+            return true;
+        if (x >= s.beamx - m.bstop && x < s.beamx + m.bstop && y >= s.beamy - m.bstop && y < s.beamy + m.bstop)
+            return true;
+    } else if (m.bstop == -1) {
+        if (data[(size_t)x + (size_t)w * y] <= data[0]) return true;
+    }
+    return false;
+}
+
+struct FitSet {
+    DevSet d;
+    csx_mask m;
+    const double* data;
+    int32_t w, h;
+    int32_t tiles_x, first_tile;  // tile range of this set in the batch
+};
+
+struct TileSum {
+    double sum;
+    int32_t cnt, nan;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One fit tile: per-pixel FQSVERGL terms, NaN counted, block-reduced in FP64.
+template <int KIND, int PREC>
+__device__ void fit_tile(const FitSet& f, int tile, double* __restrict__ fqs_out, TileSum* __restrict__ out) {
+    __shared__ double s_sum[kBlock / 64];
+    __shared__ int s_cnt[kBlock / 64], s_nan[kBlock / 64];
+    const int tx = tile % f.tiles_x, ty = tile / f.tiles_x;
+    const int x = tx * kTile + (threadIdx.x & (kTile - 1));
+    const int y = ty * kTile + (threadIdx.x / kTile);
+    double term = 0.0;
+    int nz = 0, isnan_ = 0;
+    if (x < f.w && y < f.h) {
+        const csx_scalars& s = f.d.s;
+        if (!fit_masked(s, f.m, f.data, f.w, x, y)) {
+            const int ihex = x + s.iimin, i = y + s.iimin;
+            double qx, qy, qz;
+            pixel_q(s, ihex, i, qx, qy, qz);
+            const double v = pixel_value<KIND, PREC>(f.d, qx, qy, qz);
+            const size_t idx = (size_t)(ihex - s.xmin) + (size_t)(s.xmax - s.xmin) * (size_t)(i - s.ymin);
+            const double dv = f.data[idx];
+            if (v > 0 && dv > 0) {
+                const double a = log(v) / log(10.) - log(dv) / log(10.);
+                term = a * a;
+            }
+            if (fqs_out) fqs_out[idx] = term;
+        } else if (fqs_out) {
+            fqs_out[(size_t)x + (size_t)f.w * y] = 0.0;
+        }
+        if (isnan(term)) { isnan_ = 1; term = 0.0; }
+        else if (term != 0) nz = 1;
+    }
+    const double ws = wave_sum(term);
+    const int wc = wave_sum_i(nz), wn = wave_sum_i(isnan_);
+    const int wid = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) { s_sum[wid] = ws; s_cnt[wid] = wc; s_nan[wid] = wn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        int c = 0, n = 0;
+        for (int k = 0; k < kBlock / 64; ++k) { t += s_sum[k]; c += s_cnt[k]; n += s_nan[k]; }
+        out->sum = t; out->cnt = c; out->nan = n;
+    }
+    __syncthreads();
+}
+
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) k_fit_one(const FitSet f, double* __restrict__ fqs_out, TileSum* __restrict__ out) {
+    fit_tile<KIND, PREC>(f, blockIdx.x, fqs_out, out + blockIdx.x);
+}
+
+// Persistent batched objective: blocks stride over the flat tile list of all sets.
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) k_fit_batch(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
+                                                      int ntiles, TileSum* __restrict__ out) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 st = tiles[t];  // (set, tile within set)
+        const FitSet& f = sets[st.x];
+        fit_tile<KIND, PREC>(f, st.y, nullptr, out + t);
+    }
+}
+
+// Per-set sum of the tile partials (deterministic order).
+__global__ void k_reduce_sets(const FitSet* __restrict__ sets, int nsets, const TileSum* __restrict__ part,
+                              const int* __restrict__ ntiles_of, double* __restrict__ sums, long long* __restrict__ cnts) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsets) return;
+    const int a = sets[s].first_tile, n = ntiles_of[s];
+    double t = 0.0;
+    long long c = 0;
+    for (int k = 0; k < n; ++k) { t += part[a + k].sum; c += part[a + k].cnt; }
+    sums[s] = t;
+    cnts[s] = c;
+}
+
+}  // namespace csx
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace csx;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+// Host-side fast tables of one parameter set (Friedel pairs and isotropic list).
+struct FastTables {
+    std::vector<float4> pairs;  // 2 per pair, then 1 per unpaired reflection
+    std::vector<float2> iso;
+    int npairs = 0, nsingle = 0;
+};
+
+void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L2, const float* L3, FastTables& ft) {
+    ft.pairs.clear();
+    ft.iso.clear();
+    ft.npairs = ft.nsingle = 0;
+    const int n2 = s.peakmax2;
+    std::vector<char> used(n2 + 1, 0);
+    std::vector<int> singles;
+    auto valid = [&](int ii) {
+        const double qh = L3[ii * 17 + 5];
+        return qh > 0 && L2[ii * 6 + 4] * L2[ii * 6 + 5] != 0;
+    };
+    for (int ii = 1; ii <= n2; ++ii) {
+        if (used[ii] || !valid(ii)) continue;
+        used[ii] = 1;
+        const int h = L2[ii * 6 + 1], k = L2[ii * 6 + 2], l = L2[ii * 6 + 3];
+        int jj = 0;
+        for (int c = 1; c <= n2 && !jj; ++c) {
+            if (!used[c] && L2[c * 6 + 1] == -h && L2[c * 6 + 2] == -k && L2[c * 6 + 3] == -l && valid(c) &&
+                L3[c * 17 + 5] == L3[ii * 17 + 5] && L3[c * 17 + 10] == L3[ii * 17 + 10] &&
+                L3[c * 17 + 11] == L3[ii * 17 + 11] && L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5])
+                jj = c;
+        }
+        if (!jj) { singles.push_back(ii); continue; }
+        used[jj] = 1;
+        const double qh = L3[ii * 17 + 5];
+        const double A = (double)(L2[ii * 6 + 4] * L2[ii * 6 + 5]) * (double)L3[ii * 17 + 10] / (double)L3[ii * 17 + 11];
+        float4 a, b;
+        a.x = (float)((double)L3[ii * 17 + 2] / qh);
+        a.y = (float)((double)L3[ii * 17 + 3] / qh);
+        a.z = (float)((double)L3[ii * 17 + 4] / qh);
+        a.w = L3[ii * 17 + 5];
+        b.x = (float)A;
+        int ia = ii, ib = jj;
+        std::memcpy(&b.y, &ia, 4);
+        std::memcpy(&b.z, &ib, 4);
+        b.w = 0.0f;
+        ft.pairs.push_back(a);
+        ft.pairs.push_back(b);
+        ft.npairs++;
+    }
+    for (int ii : singles) {
+        float4 r{};
+        std::memcpy(&r.x, &ii, 4);
+        ft.pairs.push_back(r);
+    }
+    ft.nsingle = (int)singles.size();
+    for (int ii = 1; ii <= s.peakmax1 && ii <= n2; ++ii) {
+        const int m = L1[ii * 6 + 4], f = L1[ii * 6 + 5];
+        if (m == 0 || f == 0) continue;
+        ft.iso.push_back(make_float2(L3[ii * 17 + 5], (float)(m * f)));
+    }
+}
+
+}  // namespace
+
+struct csx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::string err;
+    bool uploaded = false;
+    DevSet set{};
+    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, iso, tiles, fqs_tmp;
+    // batch staging
+    std::vector<FitSet> bsets;
+    std::vector<int> bntiles;
+    std::vector<int2> btiles;
+    std::vector<char> barena;  // host image of the device arena (tables of all sets)
+    struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_iso; };
+    std::vector<Pending> bpend;
+    DevBuf barena_d, bsets_d, btiles_d, bpart_d, bntiles_d, bsums_d, bcnts_d;
+};
+
+static int fail(csx_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(ctx, CSX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <int K, int P>
+static void launch_image(const DevSet& d, const csx_grid& g, double* out, hipStream_t st) {
+    dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
+    hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out);
+}
+
+int csx_qgrid(csx_ctx* c, const csx_grid* g, double* d_q3) {
+    if (!c || !g || !d_q3) return CSX_E_ARG;
+    if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_qgrid before csx_upload");
+    if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
+    HIPCHK(c, hipSetDevice(c->device));
+    dim3 grid((g->xmax - g->xmin + kTile - 1) / kTile, (g->ymax - g->ymin + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_qgrid, grid, dim3(kBlock), 0, c->stream, c->set.s, *g, d_q3);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CSX_OK;
+}
+
+template <int K, int P>
+static void launch_fit_one(const FitSet& f, int ntiles, double* fqs, TileSum* part, hipStream_t st) {
+    hipLaunchKernelGGL((k_fit_one<K, P>), dim3(ntiles), dim3(kBlock), 0, st, f, fqs, part);
+}
+
+template <int K, int P>
+static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, TileSum* part, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_fit_batch<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
+}
+
+#define DISPATCH_KIND(kind, prec, FN, ...)                                          \
+    switch (kind) {                                                                 \
+    case K_SPHERE: if (prec) FN<K_SPHERE, 1>(__VA_ARGS__); else FN<K_SPHERE, 0>(__VA_ARGS__); break; \
+    case K_FCC: if (prec) FN<K_FCC, 1>(__VA_ARGS__); else FN<K_FCC, 0>(__VA_ARGS__); break; \
+    case K_CYL_GAUSS: if (prec) FN<K_CYL_GAUSS, 1>(__VA_ARGS__); else FN<K_CYL_GAUSS, 0>(__VA_ARGS__); break; \
+    case K_CYL_ZDIR: if (prec) FN<K_CYL_ZDIR, 1>(__VA_ARGS__); else FN<K_CYL_ZDIR, 0>(__VA_ARGS__); break; \
+    default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
+    }
+
+extern "C" {
+
+int csx_create(int device, csx_ctx** out) {
+    if (!out) return CSX_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CSX_E_HIP;
+    if (device < 0 || device >= n) return CSX_E_ARG;
+    csx_ctx* c = new csx_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return CSX_E_HIP;
+    }
+    c->own_stream = true;
+    *out = c;
+    return CSX_OK;
+}
+
+void csx_destroy(csx_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->iso, &c->tiles,
+                      &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
+                      &c->bsums_d, &c->bcnts_d})
+        b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int csx_set_stream(csx_ctx* c, void* s) {
+    if (!c) return CSX_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    if (s) { c->stream = (hipStream_t)s; c->own_stream = false; }
+    else {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return CSX_OK;
+}
+
+int csx_synchronize(csx_ctx* c) {
+    if (!c) return CSX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CSX_OK;
+}
+
+int csx_last_error(csx_ctx* c, char* buf, int len) {
+    if (!c || !buf || len <= 0) return CSX_E_ARG;
+    std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
+    return CSX_OK;
+}
+
+double csx_last_kernel_ms(csx_ctx* c) { return c ? c->last_ms : -1.0; }
+
+static int check_scalars(csx_ctx* c, const csx_scalars* s) {
+    if (!s) return fail(c, CSX_E_ARG, "null scalars");
+    if (kind_of(s->combo_key) < 0) return fail(c, CSX_E_ARG, "combination key not supported by the device path");
+    if (s->peakmax1 < 0 || s->peakmax2 < 0 || s->peakmax1 >= CSX_LATLEN || s->peakmax2 >= CSX_LATLEN)
+        return fail(c, CSX_E_ARG, "peakmax out of range");
+    if (s->peakmax1 > s->peakmax2 && s->peakmax1 > 0)
+        return fail(c, CSX_E_ARG, "isotropic list longer than the anisotropic table");
+    if (s->zmax <= 0) return fail(c, CSX_E_ARG, "zmax must be positive");
+    return CSX_OK;
+}
+
+int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const double* h_carr11pm,
+               const int32_t* h_l1, const int32_t* h_l2, const float* h_l3) {
+    if (!c) return CSX_E_ARG;
+    int rc = check_scalars(c, s);
+    if (rc) return rc;
+    if (!h_carr) return fail(c, CSX_E_ARG, "null coefficient table");
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
+    HIPCHK(c, c->carr.ensure(18 * CSX_NCOEF * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->carr.p, h_carr, 18 * CSX_NCOEF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->set = DevSet{};
+    c->set.s = *s;
+    c->set.carr = (const double*)c->carr.p;
+    if (h_carr11pm) {
+        HIPCHK(c, c->carr11pm.ensure(CSX_NIM2 * CSX_NIM2 * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(c->carr11pm.p, h_carr11pm, CSX_NIM2 * CSX_NIM2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        c->set.carr11pm = (const double*)c->carr11pm.p;
+    }
+    FastTables ft;
+    if (lattice) {
+        if (!h_l1 || !h_l2 || !h_l3) return fail(c, CSX_E_ARG, "null reflection tables");
+        const size_t b1 = (size_t)(s->peakmax1 + 1) * 6 * 4, b2 = (size_t)(s->peakmax2 + 1) * 6 * 4,
+                     b3 = (size_t)(s->peakmax2 + 1) * 17 * 4;
+        HIPCHK(c, c->lat1.ensure(b1));
+        HIPCHK(c, c->lat2.ensure(b2));
+        HIPCHK(c, c->lat3.ensure(b3));
+        HIPCHK(c, hipMemcpyAsync(c->lat1.p, h_l1, b1, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->lat2.p, h_l2, b2, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->lat3.p, h_l3, b3, hipMemcpyHostToDevice, c->stream));
+        build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
+        const size_t bp = std::max<size_t>(ft.pairs.size(), 1) * sizeof(float4);
+        const size_t bi = std::max<size_t>(ft.iso.size(), 1) * sizeof(float2);
+        HIPCHK(c, c->pairs.ensure(bp));
+        HIPCHK(c, c->iso.ensure(bi));
+        if (!ft.pairs.empty())
+            HIPCHK(c, hipMemcpyAsync(c->pairs.p, ft.pairs.data(), ft.pairs.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        if (!ft.iso.empty())
+            HIPCHK(c, hipMemcpyAsync(c->iso.p, ft.iso.data(), ft.iso.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+        c->set.latpar1 = (const int32_t*)c->lat1.p;
+        c->set.latpar2 = (const int32_t*)c->lat2.p;
+        c->set.latpar3 = (const float*)c->lat3.p;
+        c->set.pairs = (const float4*)c->pairs.p;
+        c->set.iso = (const float2*)c->iso.p;
+        c->set.npairs = ft.npairs;
+        c->set.nsingle = ft.nsingle;
+        c->set.niso = (int)ft.iso.size();
+    }
+    // The host staging memory is reused: wait until the copies have consumed it.
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    return CSX_OK;
+}
+
+
+
+int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
+    if (!c || !g || !d_out) return CSX_E_ARG;
+    if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_render before csx_upload");
+    if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
+    if (precision != CSX_PREC_REF64 && precision != CSX_PREC_MIXED) return fail(c, CSX_E_ARG, "bad precision");
+    if (c->set.s.fix_orcase && kind_of(c->set.s.combo_key) == K_CYL_GAUSS)
+        return fail(c, CSX_E_ARG, "orientation-averaged cylinders in physics mode are not available");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int kind = kind_of(c->set.s.combo_key);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return CSX_OK;
+}
+
+
+int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, const double* d_data,
+                double* h_sum, int64_t* h_cnt, int64_t* h_nan, double* d_fqs) {
+    if (!c || !mask || !d_data || !h_sum || w <= 0 || h <= 0) return CSX_E_ARG;
+    if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_fit_fqs before csx_upload");
+    const csx_scalars& s = c->set.s;
+    if ((long long)(s.xmax - s.xmin) * (s.ymax - s.ymin) < (long long)w * h || (s.xmax - s.xmin) != w)
+        return fail(c, CSX_E_ARG, "fit array must match the image width and fit inside the image");
+    if (s.iimin != s.xmin || s.iimin != s.ymin)
+        return fail(c, CSX_E_ARG, "fit needs the full (Q4) or first-quadrant (Q1) image layout");
+    HIPCHK(c, hipSetDevice(c->device));
+    FitSet f{};
+    f.d = c->set;
+    f.m = *mask;
+    f.data = d_data;
+    f.w = w;
+    f.h = h;
+    f.tiles_x = (w + kTile - 1) / kTile;
+    const int ntiles = f.tiles_x * ((h + kTile - 1) / kTile);
+    HIPCHK(c, c->tiles.ensure((size_t)ntiles * sizeof(TileSum)));
+    const int kind = kind_of(s.combo_key);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    DISPATCH_KIND(kind, precision, launch_fit_one, f, ntiles, d_fqs, (TileSum*)c->tiles.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    std::vector<TileSum> part(ntiles);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->tiles.p, ntiles * sizeof(TileSum), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    double sum = 0.0;
+    int64_t cnt = 0, nn = 0;
+    for (const auto& p : part) { sum += p.sum; cnt += p.cnt; nn += p.nan; }
+    *h_sum = sum;
+    if (h_cnt) *h_cnt = cnt;
+    if (h_nan) *h_nan = nn;
+    return CSX_OK;
+}
+
+// ---- batched objective ------------------------------------------------------
+int csx_fit_batch_begin(csx_ctx* c) {
+    if (!c) return CSX_E_ARG;
+    c->bsets.clear();
+    c->bntiles.clear();
+    c->btiles.clear();
+    c->barena.clear();
+    c->bpend.clear();
+    return CSX_OK;
+}
+
+static size_t arena_put(std::vector<char>& a, const void* p, size_t bytes) {
+    const size_t off = (a.size() + 255) & ~size_t(255);
+    a.resize(off + bytes);
+    if (bytes) std::memcpy(a.data() + off, p, bytes);
+    return off;
+}
+
+int csx_fit_batch_stage(csx_ctx* c, const csx_scalars* s, const double* h_carr, const int32_t* h_l1,
+                        const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
+                        const double* d_data) {
+    if (!c || !h_carr || !mask || !d_data || w <= 0 || h <= 0) return CSX_E_ARG;
+    int rc = check_scalars(c, s);
+    if (rc) return rc;
+    if (!c->bsets.empty() && kind_of(c->bsets[0].d.s.combo_key) != kind_of(s->combo_key))
+        return fail(c, CSX_E_ARG, "all sets of a batch must share the combination");
+    if ((s->xmax - s->xmin) != w || (long long)(s->xmax - s->xmin) * (s->ymax - s->ymin) < (long long)w * h)
+        return fail(c, CSX_E_ARG, "fit array must match the image width and fit inside the image");
+    if (s->iimin != s->xmin || s->iimin != s->ymin)
+        return fail(c, CSX_E_ARG, "fit needs the full (Q4) or first-quadrant (Q1) image layout");
+    const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
+    FitSet f{};
+    f.d.s = *s;
+    f.m = *mask;
+    f.data = d_data;
+    f.w = w;
+    f.h = h;
+    f.tiles_x = (w + kTile - 1) / kTile;
+    const int nt = f.tiles_x * ((h + kTile - 1) / kTile);
+    f.first_tile = (int)c->btiles.size();
+    csx_ctx::Pending pd{};
+    pd.off_carr = arena_put(c->barena, h_carr, 18 * CSX_NCOEF * sizeof(double));
+    if (lattice) {
+        if (!h_l1 || !h_l2 || !h_l3) return fail(c, CSX_E_ARG, "null reflection tables");
+        pd.off_l1 = arena_put(c->barena, h_l1, (size_t)(s->peakmax1 + 1) * 6 * 4);
+        pd.off_l2 = arena_put(c->barena, h_l2, (size_t)(s->peakmax2 + 1) * 6 * 4);
+        pd.off_l3 = arena_put(c->barena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
+        FastTables ft;
+        build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
+        pd.off_pairs = arena_put(c->barena, ft.pairs.data(), ft.pairs.size() * sizeof(float4));
+        pd.off_iso = arena_put(c->barena, ft.iso.data(), ft.iso.size() * sizeof(float2));
+        f.d.npairs = ft.npairs;
+        f.d.nsingle = ft.nsingle;
+        f.d.niso = (int)ft.iso.size();
+    }
+    for (int k = 0; k < nt; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
+    c->bsets.push_back(f);
+    c->bntiles.push_back(nt);
+    c->bpend.push_back(pd);
+    return CSX_OK;
+}
+
+
+int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts) {
+    if (!c || !h_sums) return CSX_E_ARG;
+    const int nsets = (int)c->bsets.size();
+    if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->barena_d.ensure(c->barena.size()));
+    char* base = (char*)c->barena_d.p;
+    for (int i = 0; i < nsets; ++i) {
+        FitSet& f = c->bsets[i];
+        const auto& pd = c->bpend[i];
+        f.d.carr = (const double*)(base + pd.off_carr);
+        if ((f.d.s.combo_key >> 8 & 0xff) != 12) {
+            f.d.latpar1 = (const int32_t*)(base + pd.off_l1);
+            f.d.latpar2 = (const int32_t*)(base + pd.off_l2);
+            f.d.latpar3 = (const float*)(base + pd.off_l3);
+            f.d.pairs = (const float4*)(base + pd.off_pairs);
+            f.d.iso = (const float2*)(base + pd.off_iso);
+        }
+    }
+    const int ntiles = (int)c->btiles.size();
+    HIPCHK(c, c->bsets_d.ensure(nsets * sizeof(FitSet)));
+    HIPCHK(c, c->btiles_d.ensure(ntiles * sizeof(int2)));
+    HIPCHK(c, c->bpart_d.ensure(ntiles * sizeof(TileSum)));
+    HIPCHK(c, c->bntiles_d.ensure(nsets * sizeof(int)));
+    HIPCHK(c, c->bsums_d.ensure(nsets * sizeof(double)));
+    HIPCHK(c, c->bcnts_d.ensure(nsets * sizeof(long long)));
+    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.data(), c->barena.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bsets_d.p, c->bsets.data(), nsets * sizeof(FitSet), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->btiles_d.p, c->btiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bntiles_d.p, c->bntiles.data(), nsets * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int nblocks = std::min(ntiles, cus * 8);
+    const int kind = kind_of(c->bsets[0].d.s.combo_key);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    DISPATCH_KIND(kind, precision, launch_fit_batch, (const FitSet*)c->bsets_d.p, (const int2*)c->btiles_d.p,
+                  ntiles, (TileSum*)c->bpart_d.p, nblocks, c->stream);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_sets, dim3((nsets + 255) / 256), dim3(256), 0, c->stream, (const FitSet*)c->bsets_d.p,
+                       nsets, (const TileSum*)c->bpart_d.p, (const int*)c->bntiles_d.p, (double*)c->bsums_d.p,
+                       (long long*)c->bcnts_d.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h_sums, c->bsums_d.p, nsets * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<long long> cn(nsets);
+    HIPCHK(c, hipMemcpyAsync(cn.data(), c->bcnts_d.p, nsets * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = cn[i];
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return CSX_OK;
+}
+
+}  // extern "C"

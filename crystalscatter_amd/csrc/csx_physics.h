@@ -1,0 +1,218 @@
+// Per-pixel physics of the five specialised kernels, FP64, reference formulas.
+//
+// Device functions (HIP, gfx950).  Compiled with -ffp-contract=off so the FP64
+// arithmetic rounds operation by operation like the reference's CPU build;
+// transcendental functions are the device math library's (within an ulp of
+// glibc), so REF64 results agree with the CPU oracle to ~1e-15 relative.
+//
+// Reference functions restated (paths relative to the reference checkout):
+//   pq_sphere     formpq_partSphere cs=0     Source/sc_lib_formpq_partSphere.h:11-75
+//   fq_sphere     formfq_partSphere cs=0     Source/sc_lib_formfq_partSphere.h:8-52
+//   pq_cylinder   formpq_partCylinder        Source/sc_lib_formpq_partCylinder.h:8-257
+//   fq_cylinder   formfq_partCylinder ord 6  Source/sc_lib_formfq_partCylinder.h:8-206
+//   ref_gamma_d   gamma                      Source/sc_calc_generic_gpu.cu:613-673
+//   pixel_q       convert_idx2q              Source/sc_calc_generic_gpu.cu:249-287
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "csx.h"
+
+namespace csx {
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// Device copy of one parameter set: scalars by value, tables in HBM.
+struct DevSet {
+    csx_scalars s;
+    const double* carr;      // 18 x CSX_NCOEF
+    const double* carr11pm;  // CSX_NIM2^2 or null
+    const int32_t* latpar1;  // (peakmax1+1) x 6
+    const int32_t* latpar2;  // (peakmax2+1) x 6
+    const float* latpar3;    // (peakmax2+1) x 17
+    const float4* pairs;     // fast path: 2 float4 per Friedel pair (see csx_device.hip)
+    const float2* iso;       // fast path: isotropic peaks (qhkl, mhkl*fhkl)
+    int32_t npairs, nsingle, niso, pad;
+};
+
+__device__ __forceinline__ const double* carr_of(const DevSet& d, int which) {
+    return d.carr + which * CSX_NCOEF;  // 0..8 = carr1p..9p, 9..17 = carr1f..9f
+}
+
+__device__ __forceinline__ double sqd(double x) { return x * x; }
+
+// q of pixel (ihex, i), convert_idx2q (.cu:249-287).
+__device__ __forceinline__ void pixel_q(const csx_scalars& s, int ihex, int i, double& qx, double& qy, double& qz) {
+    if (s.centerbeam) {
+        const double mdet = (ihex) * s.pixnoy / (2.0 * s.zmax);
+        const double ndet = (i) * s.pixnox / (2.0 * s.zmax);
+        const double xdet = s.pixx_m * (ndet - s.beamx);
+        const double ydet = s.pixy_m * (mdet - s.beamy);
+        const double rdet = sqrt(xdet * xdet + ydet * ydet);
+        const double phidet = atan2(ydet, xdet);
+        const double thetadet = atan2(rdet, s.det);
+        qx = 2 * kPiD * cos(phidet) * sin(thetadet) / s.wave;
+        qy = 2 * kPiD * sin(phidet) * sin(thetadet) / s.wave;
+        qz = 2 * kPiD * (1 - cos(thetadet)) / s.wave;
+    } else {
+        qx = s.qmax * i / (double)(s.zmax);
+        qy = s.qmax * ihex / (double)(s.zmax);
+        qz = 1e-20;
+    }
+}
+
+// Power series sum_n c[n] x^(2n) with the reference's exit rule (relative change < 1e-4),
+// the power built as (xn*x)*x.
+__device__ __forceinline__ double series_x2(const double* c, double x, int nmax) {
+    double sum = 1.0, old = 0.0, xn = 1.0;
+    for (int n = 1; n <= nmax; ++n) {
+        xn = xn * x * x;
+        sum = sum + c[n] * xn;
+        const double d = fabs((sum - old) / sum);
+        if (d < 0.0001) break;
+        old = sum;
+    }
+    return sum;
+}
+
+// Same series in the longitudinal projection p = qxs+qys: the power is built as (xn*p)*p.
+__device__ __forceinline__ double series_proj(const double* c, double p, int nmax) {
+    double sum = 1.0, old = 0.0, xn = 1.0;
+    for (int n = 1; n <= nmax; ++n) {
+        xn = xn * p * p;
+        sum += c[n] * xn;
+        const double d = fabs((sum - old) / sum);
+        if (d < 0.0001) break;
+        old = sum;
+    }
+    return sum;
+}
+
+__device__ inline double pq_sphere(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    double zr = (1 - sqd(s.sigma)) / (sqd(s.sigma));
+    if (q < 0.4 * s.limq[3]) return series_x2(carr_of(d, 3), q, 100);
+    const double argq = q * s.radius / (zr + 1);
+    if (zr * (zr - 1) * (zr - 2) * (zr - 3) * (zr - 4) * (zr - 5) == 0) zr += 0.001;
+    const double pqr = (1 / (2.0 * zr * (zr - 1) * (zr - 2) * (zr - 3))) * pow(argq, -4);
+    const double at = atan(2.0 * argq), b = 1.0 + 4 * argq * argq;
+    const double pq1 = pqr * (1 + cos((zr - 3) * at) / pow(b, (zr - 3) / 2.0));
+    const double pq2 = (pqr / ((zr - 4) * argq)) * sin((zr - 4) * at) / pow(b, (zr - 4) / 2.0);
+    // exponent zr-5 and a trailing /2.0 on the ratio, as the reference writes it
+    const double pq3 = (pqr / ((zr - 4) * (zr - 5) * argq * argq)) * (1 - cos((zr - 5) * at) / pow(b, zr - 5) / 2.0);
+    return 9.0 * (pq1 - 2.0 * pq2 + pq3);
+}
+
+__device__ inline double fq_sphere(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
+    if (q < (0.4 * s.limq[12])) return series_x2(carr_of(d, 12), q, 100);
+    const double argq = q * s.radius / (zr + 1);
+    const double pqr = (1 / (zr * (zr - 1)) * pow(argq, -2));
+    const double at = atan(argq), b = 1.0 + argq * argq;
+    const double pq1 = pqr * cos((zr - 1) * at) / pow(b, (zr - 1) / 2.0);
+    const double pq2 = (pqr / ((zr - 2) * argq)) * sin((zr - 2) * at) / pow(b, (zr - 2) / 2.0);
+    const double pq3 = 3 * (pq2 - pq1);
+    return pq3 * pq3;
+}
+
+__device__ inline double ref_gamma_d(double z) {
+    constexpr double bm[9] = {0.0, -0.577191652, 0.988205891, -0.897056937, 0.918206857,
+                              -0.756704078, 0.482199394, -0.193527818, 0.035868343};
+    constexpr double cm[27] = {
+        0.0, 1.0000000000000000, 0.5772156649015329, -0.6558780715202538, -0.0420026350340952,
+        0.1665386113822915, -0.0421977345555443, -0.0096219715278770, 0.00721894322466630,
+        -0.0011651675918591, -0.0002152416741149, 0.0001280502823882, -0.0000201348547807,
+        -0.0000012504934821, 0.0000011330272320, -0.0000002056338417, 0.0000000061160950,
+        0.0000000050020075, -0.0000000011812746, 0.0000000001043427, 0.0000000000077823,
+        -0.0000000000036968, 0.0000000000005100, -0.0000000000000206, -0.0000000000000054,
+        0.0000000000000014, 0.0000000000000001};
+    bool neg;
+    double ip;
+    if (fabs(z) < 0.000000001) z = z + 0.000000001;
+    if (z > 0) neg = false;
+    else { neg = true; if (fabs(modf(z, &ip)) < 0.000000001) z = z + 0.000000001; }
+    const double az = fabs(z);
+    if (az > 0.0 && az <= 1) {
+        double sum = 0;
+        for (int i = 1; i <= 26; ++i) sum += cm[i] * pow(az, (double)i);
+        return neg ? -kPiD * sum / (az * sin(kPiD * az)) : 1. / sum;
+    }
+    if (az > 1 && az <= 1000) {
+        if (fabs(modf(az, &ip)) < 0.000000001) {
+            double f = 1;
+            for (int i = 1; i <= trunc(az) - 1; ++i) f = f * i;
+            return neg ? -kPiD / (az * f * sin(kPiD * az)) : f;
+        }
+        const double y = az - 1, di = trunc(y), x = az - trunc(az);
+        double fak = y;
+        if (di == 0) fak = 1;
+        else for (int i = 1; i < di; ++i) fak = fak * (y - i);
+        double f = 1;
+        for (int i = 1; i <= 8; ++i) f = f + bm[i] * pow(x, (double)i);
+        return neg ? -kPiD / (az * fak * f * sin(kPiD * az)) : fak * f;
+    }
+    const double g = exp(-az) * exp((az - 0.5) * log(az)) * sqrt(2 * kPiD);
+    return neg ? -kPiD / (az * g * sin(kPiD * az)) : g;
+}
+
+// Schulz-averaged longitudinal P of a rod, argument (qxs+qys+eps9)*L/(zl+1).
+__device__ __forceinline__ double rod_p(double zl, double arg) {
+    return (1 / (2.0 * zl * (zl - 1))) * (1 / (arg * arg)) *
+           (1 - cos((zl - 1) * atan(2.0 * arg)) / pow(1.0 + 4 * arg * arg, (zl - 1) / 2.0));
+}
+
+// Orientation case as the form factors compare it: the reference passes its 0-based enum
+// where the comparisons use the 1-based numbering; fix_orcase maps it (physics mode).
+__device__ __forceinline__ int orc_cmp(const csx_scalars& s) { return s.fix_orcase ? s.orcase + 1 : s.orcase; }
+
+// Transverse part of the homogeneous cylinder P(q) (:16246-16279).
+__device__ inline double pq_cyl_transverse(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    const double zr = s.sigma == 1.0 ? 0.000000001 : (1 - sqd(s.sigma)) / (sqd(s.sigma));
+    if ((s.length / s.radius) < 2) return 1;
+    if (q < (0.65 * s.limq[3])) return series_x2(carr_of(d, 3), q, 120);
+    const double argpq = q * s.radius / (zr + 1);
+    const double at = atan(2.0 * argpq), b = 1.0 + 4 * argpq * argpq;
+    const double pqr1 = (1 / (zr * (zr - 1) * (zr - 2))) * pow(argpq, -3);
+    const double pqr2 = (1 / (zr * (zr - 1) * (zr - 2))) * pow(argpq, -3) * sin((zr - 2) * at) / pow(b, (zr - 2) / 2.0);
+    const double pqr3 = (1 / (zr * (zr - 1) * (zr - 2) * (zr - 3))) * pow(argpq, -4) * cos((zr - 3) * at) / pow(b, (zr - 3) / 2.0);
+    return (4. / kPiD) * (pqr1 - pqr2 - (9. / 8.0) * pqr3);
+}
+
+// Longitudinal part for perfect (z-dir) orientation, ordis 6 (:16143-16168).
+__device__ inline double pq_cyl_long_zdir(const DevSet& d, double qxs, double qys, double q) {
+    const csx_scalars& s = d.s;
+    if (orc_cmp(s) == 4) return 1.0;
+    const double zl = s.sigmal == 1.0 ? 0.000000001 : (1 - sqd(s.sigmal)) / sqd(s.sigmal);
+    if (q < (0.6 * s.limq[0])) return series_proj(carr_of(d, 0), qxs + qys, 120);
+    return rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
+}
+
+// formfq_partCylinder, ordis 6, cs 0 (Source/sc_lib_formfq_partCylinder.h:8-206).
+__device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, double qys, double q) {
+    const csx_scalars& s = d.s;
+    const double zl = (1 - sqd(s.sigmal)) / sqd(s.sigmal);
+    const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
+    double pql;
+    if (orc_cmp(s) == 4) pql = 1.0;
+    else if (limql < (0.6 * s.limq[9])) pql = series_proj(carr_of(d, 9), qxs + qys, 120);
+    else pql = rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
+    double pqr;
+    if (q < (1.0 * s.limq[12])) {
+        pqr = series_x2(carr_of(d, 12), q, 120);
+    } else {
+        const double argpq = q * s.radius / (zr + 1);
+        const double at = atan(argpq), b = 1.0 + argpq * argpq;
+        const double g0 = ref_gamma_d(zr + 1);
+        const double pqr1 = (ref_gamma_d(zr - 1 / 2.0) / g0) * pow(argpq, -3 / 2.0) *
+                            (sin((zr - 1 / 2.0) * at) - cos((zr - 1 / 2.0) * at)) / pow(b, (zr - 1 / 2.0) / 2.0);
+        const double pqr2 = (ref_gamma_d(zr - 3 / 2.0) / g0) * pow(argpq, -5 / 2.0) *
+                            (sin((zr - 3 / 2.0) * at) + cos((zr - 3 / 2.0) * at)) / pow(b, (zr - 3 / 2.0) / 2.0);
+        const double pqr3 = (2 / sqrt(kPiD)) * (pqr1 + (9 / 16.0) * pqr2);
+        pqr = pqr3 * pqr3;
+    }
+    return pql * pqr;
+}
+
+}  // namespace csx

@@ -1,0 +1,63 @@
+// Host-side preparation of one parameter set for the device path.
+//
+// Produces exactly what the reference's prepareCalculation leaves for the
+// per-pixel code (Source/sc_calc_generic_cpu.cpp:319-988): the series
+// coefficient tables and their switch points (coefficients(), :2099-8090),
+// the FCC/BCT reflection tables (ButtonHKLClick :8246, fhkl_c :10299,
+// corotations :1487, table fill :838-954) and the per-pixel scalars, in the
+// flat layout of include/csx.h.  Runs in FP64 with x87 long double where the
+// reference's recurrences use it, so the tables agree with the reference's.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "csx.h"
+
+namespace csx {
+
+// The reference key table (Source/sc_calc_generic.cpp:36-138) with its defaults.
+struct ParamSet {
+    int ltype = 12, particle = 0, ordis = 7, interior = 0, peakidx = 1, hklmax = 5, gridpoints = 64;
+    int radq1 = 0, radq2 = 0, radq4 = 1, expand = 0, centerbeam = 0, twinned = 0, dsch = 0, para = 0;
+    int waxs = 0, fix_orcase = 0;
+    double uca = 30, ucb = 30, ucc = 30, ucalpha = 90, ucbeta = 90, ucgamma = 90, ceff = 1;
+    double ceffcyl = 0, reff = 0, acpl = 0, bcpl = 0, ifluc = 0, rfluc = 0;
+    double radius = 1.0, radiusi = 0.5, sigma = 0.07, dbeta = 0.4, length = 1, sigmal = 0.06;
+    double alphash = 0, rho = 0, rotalpha = 0;
+    double peakpar = 0, displacement = 2, azi = 180, domainsize = 400;
+    double ucpsi = 0, ucn1 = 1, ucn2 = 1, ucn3 = 1, theta = 90, phi = 0, rottheta = 90, rotphi = 90;
+    double qmax = 1, qmin = 0, pixnox = 2048, pixnoy = 2048, pixx_mm = 1, pixy_mm = 1, det = 1;
+    double wave = 0.09499, beamx = 0, beamy = 0, bfactor = 1, p1 = 0, iso = 0, i0 = 10000, base = 0;
+    double width_zuf = 1;
+
+    // Returns false for an unknown key.
+    bool set(const std::string& key, double v);
+    bool get(const std::string& key, double* v) const;
+    uint32_t combo_key() const;
+};
+
+struct PreparedSet {
+    csx_scalars s{};
+    std::array<double, 18 * CSX_NCOEF> carr{};  // carr1p..carr9p, carr1f..carr9f
+    std::vector<double> carr11pm;               // CSX_NIM2*CSX_NIM2 (Gaussian cylinders)
+    std::vector<int32_t> latpar1, latpar2;      // (peakmax+1) x 6
+    std::vector<float> latpar3;                 // (peakmax2+1) x 17
+    double order = 0, ucvol = 0, sphno = 1;
+    int xmin = 0, xmax = 0, ymin = 0, ymax = 0; // image bounds (ihex, i)
+    int cdim = 3;
+};
+
+// checkAllParameters (Source/sc_calc_generic_cpu.cpp:198-317): empty string when valid.
+std::string check_parameters(const ParamSet& p, bool ignore_cbs);
+
+// prepareCalculation.  Returns empty string on success, else the reason.
+std::string prepare(const ParamSet& p, PreparedSet& out);
+
+// Reference Gamma (Source/sc_calc_generic_gpu.cu:613-673), needed on host and device.
+double ref_gamma(double z);
+// gaussnorm3 (Source/sc_calc_generic_gpu.cu:838-854).
+double gauss_norm3(double a);
+
+}  // namespace csx

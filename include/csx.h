@@ -1,0 +1,174 @@
+/*
+ * csx.h -- C ABI of the MI355X-native CrystalScatter per-pixel scattering path.
+ *
+ * Plain C types only: host pointers are marked h_, device (HBM) pointers d_.
+ * Every function returns 0 on success or a negative CSX_E* code; nothing
+ * throws across this boundary.  One csx_ctx per host thread; all device work
+ * of a context is ordered on its HIP stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   csx_create / csx_destroy   SasCalc_GENERIC_calculation::initMemory, createMemory,
+ *                              memcleanup   Source/sc_calc_generic_gpu.cu:374, :495, :589
+ *   csx_upload                 the pinned zero-copy tables params.CR->carr*, latpar1/2/3
+ *                              Source/sc_calc_generic_gpu.h:412-415, :443-485
+ *   csx_render                 kernel_selection(Nx,Ny,dofit=false)   Source/sc_calc_generic_gpu.cu:87-156
+ *                              with the image store setXYIntensity   Source/sc_calc_generic_gpu.cu:353-370
+ *   csx_fit_fqs                kernel_selection(_fitWidth,_fitHeight,dofit=true) plus the host
+ *                              sum of doFitCalculation  Source/sc_calc_generic_cpu.cpp:1352-1434
+ *   csx_fit_batch_*            (new) every simplex vertex of many independent fits in one launch;
+ *                              replaces the sequential fehlerquadratsumme calls of
+ *                              SasCalc_SimplexFit2D   Source/sc_simplexfit2d.cpp:19-194
+ *   csx_engine_*               the SasCalc_GENERIC_calculation facade used by sc_calcCons and
+ *                              sc_simplexfit2d: key=value setters (Source/sc_calc_generic.cpp:142-238),
+ *                              doCalculation (Source/sc_calc_generic_cpu.cpp:1151),
+ *                              doFitCalculation (:1334), setFitData (:8220), setNoFitRect (.h:365)
+ */
+#ifndef CSX_H
+#define CSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CSX_OK = 0,
+    CSX_E_ARG = -1,        /* bad argument / unsupported combination */
+    CSX_E_HIP = -2,        /* HIP runtime error */
+    CSX_E_NOMEM = -3,      /* device allocation failed */
+    CSX_E_STATE = -4,      /* call out of order (e.g. render before upload) */
+    CSX_E_PARAM = -5       /* parameter check failed (checkAllParameters) */
+};
+
+/* Arithmetic of the per-pixel evaluation. */
+enum {
+    CSX_PREC_REF64 = 0,    /* FP64 throughout, the reference formulas and order */
+    CSX_PREC_MIXED = 1     /* FP32 peak/quadrature loops with FP64 series, phases and sums */
+};
+
+/* Combination keys of kernel_selection (Source/sc_calc_generic_gpu.cu:113-136):
+ * (particle<<24) | (ltype==12 ? 0 : shp<<16) | (ltype<<8) | ordis. */
+#define CSX_KEY_SPHERE_NONE_ISO   0x00000C07u
+#define CSX_KEY_SPHERE_FCC_GAUSS  0x00020500u
+#define CSX_KEY_CYL_NONE_GAUSS    0x01000C00u
+#define CSX_KEY_CYL_NONE_ZDIR     0x01000C06u
+#define CSX_KEY_CYL_BCT_ZDIR      0x01020806u
+
+#define CSX_NCOEF 151   /* coeffarray_len */
+#define CSX_NIM2 131    /* imax2d_len */
+#define CSX_LATLEN 2500 /* latparlen */
+
+/* Per-parameter-set scalars the device path reads (the by-value engine copy of
+ * the reference, reduced to what the five kernels use).  Filled by the host
+ * preparation (csx_engine_* or the caller's own prepareCalculation). */
+typedef struct csx_scalars {
+    uint32_t combo_key;
+    int32_t zmax;                       /* GridPoints */
+    int32_t centerbeam;                 /* q from detector geometry (useBeamStop) */
+    int32_t orcase;                     /* 0 general, 1 x, 2 y, 3 z (reference enum) */
+    int32_t fix_orcase;                 /* 1: compare orientation cases 1-based (physics mode) */
+    int32_t peakmax1, peakmax2;
+    int32_t dsch, para, twin;           /* Debye-Scherrer width, paracrystal width, twinning */
+    int32_t ordis;
+    double qmax, pixnox, pixnoy, pixx_m, pixy_m, det, wave, beamx, beamy;
+    double radius, sigma, length, sigmal;
+    double dbeta;                       /* degrees, as the key table */
+    double polTheta, polPhi;            /* degrees, after the prep's phi quirk */
+    double cosphi, sinphi, costheta, sintheta, cosphic, sinphic;
+    double limq[18];                    /* limq1..9, limq1f..9f */
+    double norm;
+    double base, i0, iso, ifluc, rfluc;
+    double dwfactor, uca, ucb, domainsize, width_zuf, reldis, dist, phiwidth, ceff;
+    double sphno_cubevol;               /* latpar[1]*latpar[2] */
+    int32_t xmin, xmax, ymin, ymax;     /* image bounds _xmin.._ymax (ihex, i) */
+    int32_t iimin;                      /* offset of fit coordinates (checkFitRanges) */
+    int32_t pad0;
+} csx_scalars;
+
+/* Block of pixels to evaluate: ihex in [xmin,xmax), i in [ymin,ymax); output in the
+ * reference IDX order (ihex - xmin) + (xmax - xmin) * (i - ymin). */
+typedef struct csx_grid {
+    int32_t xmin, xmax, ymin, ymax;
+} csx_grid;
+
+/* Fit masks of checkFitRanges (Source/sc_calc_generic_gpu.cu:293-347): up to four
+ * inclusive no-fit rectangles (x0<0 = unused), border and beam-stop pixels, with the
+ * reference's coordinate conventions. */
+typedef struct csx_mask {
+    int32_t rects[16];                  /* x0,y0,x1,y1 per rectangle */
+    int32_t bstop, border;
+} csx_mask;
+
+typedef struct csx_ctx csx_ctx;
+
+int csx_create(int device, csx_ctx** out);
+void csx_destroy(csx_ctx* ctx);
+/* Use an existing HIP stream (hipStream_t) for all work of this context; NULL = own stream. */
+int csx_set_stream(csx_ctx* ctx, void* hip_stream);
+int csx_synchronize(csx_ctx* ctx);
+
+/* Copy one parameter set's tables into HBM (once per parameter set).
+ * carr: 18 x CSX_NCOEF doubles in the order carr1p..carr9p, carr1f..carr9f;
+ * carr11pm: CSX_NIM2 x CSX_NIM2 doubles (may be NULL when the key does not use it);
+ * latpar1/latpar2: (peakmax+1) x 6 ints; latpar3: (peakmax2+1) x 17 floats (row 0 unused). */
+int csx_upload(csx_ctx* ctx, const csx_scalars* s, const double* h_carr, const double* h_carr11pm,
+               const int32_t* h_latpar1, const int32_t* h_latpar2, const float* h_latpar3);
+
+/* Evaluate the pixels of `grid` into d_out (device, (xmax-xmin)*(ymax-ymin) doubles). */
+int csx_render(csx_ctx* ctx, const csx_grid* grid, int precision, double* d_out);
+
+/* q vectors of the pixels of `grid` as the kernels generate them (convert_idx2q,
+ * Source/sc_calc_generic_gpu.cu:249-287): d_q3 receives 3 doubles per pixel, IDX order. */
+int csx_qgrid(csx_ctx* ctx, const csx_grid* grid, double* d_q3);
+
+/* Fit objective of one parameter set: sum over fit pixels of
+ * (log10 I_calc - log10 I_data)^2 for I_calc>0 and I_data>0 (FQSVERGL, Source/sc_globalConfig.h:32),
+ * NaN terms counted and skipped, nonzero terms counted.  d_data: w*h doubles (device).
+ * d_fqs (optional, device): per-pixel terms. */
+int csx_fit_fqs(csx_ctx* ctx, int w, int h, const csx_mask* mask, int precision, const double* d_data,
+                double* h_sum, int64_t* h_cnt, int64_t* h_nancnt, double* d_fqs);
+
+/* Batched objective: many parameter sets, each with its own tables, detector size, data and
+ * mask, evaluated in one persistent launch.  Sets are staged with csx_fit_batch_stage after
+ * csx_fit_batch_begin; csx_fit_batch_run evaluates all staged sets and writes h_sums[nsets]. */
+int csx_fit_batch_begin(csx_ctx* ctx);
+int csx_fit_batch_stage(csx_ctx* ctx, const csx_scalars* s, const double* h_carr,
+                        const int32_t* h_latpar1, const int32_t* h_latpar2, const float* h_latpar3,
+                        int w, int h, const csx_mask* mask, const double* d_data);
+int csx_fit_batch_run(csx_ctx* ctx, int precision, double* h_sums, int64_t* h_cnts);
+
+int csx_last_error(csx_ctx* ctx, char* buf, int len);
+/* Kernel time of the last csx_render / csx_fit_fqs / csx_fit_batch_run, measured with HIP
+ * events on the context's stream (milliseconds). */
+double csx_last_kernel_ms(csx_ctx* ctx);
+
+/* ---- SasCalc_GENERIC_calculation facade ------------------------------------------- */
+typedef struct csx_engine csx_engine;
+/* device < 0 creates a host-only engine: preparation and tables, no GPU work. */
+csx_engine* csx_engine_new(int device);
+void csx_engine_free(csx_engine* e);
+/* Set a parameter by its reference key (Source/sc_calc_generic.cpp:36-138). */
+int csx_engine_set(csx_engine* e, const char* key, double value);
+double csx_engine_get(csx_engine* e, const char* key);
+/* prepareCalculation on the host + csx_upload; returns 0 or CSX_E_PARAM (message in last_error). */
+int csx_engine_prepare(csx_engine* e);
+/* doCalculation: image into h_out ((xmax-xmin)*(ymax-ymin) doubles, IDX order), bounds in b4. */
+int csx_engine_calc(csx_engine* e, int precision, double* h_out, int32_t* b4);
+/* Same, into device memory. */
+int csx_engine_calc_device(csx_engine* e, int precision, double* d_out, int32_t* b4);
+/* setFitData + doFitCalculation: returns the objective in *h_sum (1e20 on failure). */
+int csx_engine_set_fit_data(csx_engine* e, int w, int h, const double* h_data);
+int csx_engine_set_nofit_rect(csx_engine* e, int id, int x0, int y0, int x1, int y1);
+int csx_engine_fit(csx_engine* e, int precision, int bstop, int border, double* h_sum,
+                   int64_t* h_cnt, int64_t* h_nancnt);
+/* Host-side prepared tables (for checks): name as in the reference ("carr4p", "latpar3", ...). */
+const void* csx_engine_table(csx_engine* e, const char* name, int64_t* n, int* kind);
+const csx_scalars* csx_engine_scalars(csx_engine* e);
+csx_ctx* csx_engine_ctx(csx_engine* e);
+int csx_engine_last_error(csx_engine* e, char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
