@@ -61,6 +61,7 @@ SIGNATURES = [
     ("csx_synchronize", _I, [_VP]),
     ("csx_upload", _I, [_VP, ctypes.POINTER(Scalars), _PD, _PD, _PI32, _PI32, _PF]),
     ("csx_render", _I, [_VP, ctypes.POINTER(Grid), _I, _VP]),
+    ("csx_render_stats", _I, [_VP, ctypes.POINTER(Grid), _I, _VP, ctypes.POINTER(ctypes.c_uint64)]),
     ("csx_qgrid", _I, [_VP, ctypes.POINTER(Grid), _VP]),
     ("csx_fit_fqs", _I, [_VP, _I, _I, ctypes.POINTER(Mask), _I, _VP, _PD, _PI64, _PI64, _VP]),
     ("csx_fit_batch_begin", _I, [_VP]),
@@ -92,6 +93,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise CsxError("libcsx.so is not built: run `python -m crystalscatter_amd.build`")
+        # One HIP runtime per process.  PyTorch ships its own libamdhip64 with the same
+        # soname as ROCm's: loaded first, it also serves libcsx.so; loaded after libcsx.so,
+        # a second runtime comes up and torch finds no GPU.  So torch, when present, goes first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

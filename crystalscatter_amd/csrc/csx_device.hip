@@ -91,84 +91,222 @@ __device__ void lattice_ref64(const DevSet& d, double qx, double qy, double qz, 
     }
 }
 
-// acos on [0,1] as sqrt(1-x)*P7(x), Abramowitz & Stegun 4.4.46 (|error| <= 2e-8).
-__device__ __forceinline__ float acos01(float x) {
-    float p = -0.0012624911f;
-    p = __builtin_fmaf(p, x, 0.0066700901f);
-    p = __builtin_fmaf(p, x, -0.0170881256f);
-    p = __builtin_fmaf(p, x, 0.0308918810f);
-    p = __builtin_fmaf(p, x, -0.0501743046f);
-    p = __builtin_fmaf(p, x, 0.0889789874f);
-    p = __builtin_fmaf(p, x, -0.2145988016f);
-    p = __builtin_fmaf(p, x, 1.5707963050f);
-    return __builtin_amdgcn_sqrtf(1.0f - x) * p;
+// acos(c)^2 / (1 - c^2) for c in [0, 1], as a degree-9 polynomial in x = 1 - c fitted for
+// relative error (max 1.8e-7 in FP32 Horner).  The function is analytic on [0, 1] (its only
+// singularity is the pole at c = -1), equal to 1 at c = 1 and (pi/2)^2 at c = 0.
+__device__ __forceinline__ float acos2_over_sin2(float c) {
+    const float x = 1.0f - c;
+    float p = 9.084462374e-02f;
+    p = __builtin_fmaf(p, x, -2.716867626e-01f);
+    p = __builtin_fmaf(p, x, 4.152069092e-01f);
+    p = __builtin_fmaf(p, x, -2.917656898e-01f);
+    p = __builtin_fmaf(p, x, 2.068143934e-01f);
+    p = __builtin_fmaf(p, x, 6.417080760e-02f);
+    p = __builtin_fmaf(p, x, 2.098862827e-01f);
+    p = __builtin_fmaf(p, x, 3.772477508e-01f);
+    p = __builtin_fmaf(p, x, 6.666826606e-01f);
+    p = __builtin_fmaf(p, x, 9.999999404e-01f);
+    return p;
 }
 
-// Mixed-precision sums.  Pair record (2 x float4): (ux,uy,uz,qhkl) and (A, idx+, idx-, 0)
-// with A = mhkl*fhkl*g3/peaknorm1; the caller multiplies intensity by 2/(pi*width).
-__device__ void lattice_mixed(const DevSet& d, double qx, double qy, double qz, double q, double width,
-                              double& inten_scaled, double& inten_exact, double& radint) {
+__device__ __forceinline__ float wave_min_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// |q^.u| above this can put the reference's FP64 cosine above 1, where its acos mapping
+// returns pi (Source/sc_calc_generic_gpu.cu:31): such pairs take the FP64 formula.
+constexpr float kNearOne = 0.999998f;
+
+// Per-pixel constants of the FP32 pair sum (base-2 exponents).
+struct PixelConsts {
+    float hx, hy, hz;      // q^
+    float qf, qlo;         // q = qf + qlo
+    float kr;              // radial exponent = kr * (q - q_hkl)^2
+    float kp;              // angular exponent = kp * y^2, y = angle(q^, +-u)
+    float c1, c2;          // second Friedel term: e2 = e1 + c1 + c2 * y
+};
+
+// Sum over pairs [p0, p1) (p0, p1 multiples of 4, wave-uniform) of A*(2^e1 [+ 2^e2]).
+// y^2 = sin^2(y) * acos^2(c)/(1-c^2), with sin^2 y = |q^ x u|^2 taken from the cross
+// product: accurate where the Gaussian is narrow (y small), where 1-|q^.u| is not.
+// Pairs with |q^.u| > kNearOne contribute 0 here and set `flag`.
+template <bool PLANAR, bool TWO>
+__device__ __forceinline__ void pair_sum(const PairRec* __restrict__ pr, int p0, int p1, const PixelConsts& L,
+                                         double& acc, bool& flag) {
+    for (int p = p0; p < p1; p += 4) {
+        float part = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const PairRec P = pr[p + j];
+            float dt = __builtin_fmaf(L.hy, P.uy, L.hx * P.ux);
+            float ss;
+            if constexpr (PLANAR) {
+                const float k = __builtin_fmaf(L.hx, P.uy, -(L.hy * P.ux));
+                ss = __builtin_fmaf(k, k, P.uz2);
+            } else {
+                dt = __builtin_fmaf(L.hz, P.uz, dt);
+                const float k1 = __builtin_fmaf(L.hy, P.uz, -(L.hz * P.uy));
+                const float k2 = __builtin_fmaf(L.hz, P.ux, -(L.hx * P.uz));
+                const float k3 = __builtin_fmaf(L.hx, P.uy, -(L.hy * P.ux));
+                ss = __builtin_fmaf(k1, k1, __builtin_fmaf(k2, k2, k3 * k3));
+            }
+            const float c = fabsf(dt);
+            const float y2 = ss * acos2_over_sin2(c);
+            const float t = (L.qf - P.qh) + L.qlo;
+            const float r = (t * L.kr) * t;
+            const float e1 = __builtin_fmaf(L.kp, y2, r);
+            float w = __builtin_amdgcn_exp2f(e1);
+            if constexpr (TWO) {
+                const float y = __builtin_amdgcn_sqrtf(y2);
+                w += __builtin_amdgcn_exp2f(__builtin_fmaf(L.c2, y, e1 + L.c1));
+            }
+            const bool fl = c > kNearOne;
+            flag = flag || fl;
+            part = __builtin_fmaf(P.A, fl ? 0.0f : w, part);
+        }
+        acc += (double)part;
+    }
+}
+
+// FP64 terms (reference formula) of the pairs that pair_sum left out, for this lane.
+template <bool PLANAR>
+__device__ double near_one_terms(const DevSet& d, int p0, int p1, const PixelConsts& L, double qx, double qy,
+                                 double qz, double q, double width, double x2phi) {
+    double add = 0.0;
+    for (int p = p0; p < p1; ++p) {
+        const PairRec P = d.pairs[p];
+        float dt = __builtin_fmaf(L.hy, P.uy, L.hx * P.ux);
+        if constexpr (!PLANAR) dt = __builtin_fmaf(L.hz, P.uz, dt);
+        if (fabsf(dt) > kNearOne)
+            add += aniso_term_ref(d, P.ia, qx, qy, qz, q, width, x2phi) +
+                   aniso_term_ref(d, P.ib, qx, qy, qz, q, width, x2phi);
+    }
+    return add;
+}
+
+// Pair-window bookkeeping of one wave: |q| bins [b0, b1] and |u_z| bands [band0, band1].
+struct Window {
+    int b0, b1, band0, band1;
+};
+
+// Mixed-precision anisotropic peak sum (sum over reflections of sq*mhkl*fhkl*psi).  Must be
+// called by all 64 lanes of a wave: the wave's union of per-lane windows selects the pairs.
+// Pairs outside the window, and second Friedel terms where they cannot reach 2^cut, are
+// dropped: every dropped term is below 2^cut * A.  need_exact: unpaired reflections exist.
+template <bool PLANAR>
+__device__ double lattice_mixed(const DevSet& d, double qx, double qy, double qz, double q, double width,
+                                bool& need_exact, double& dropped, unsigned long long* stats) {
     const csx_scalars& s = d.s;
+    PixelConsts L;
     const double rq = 1.0 / q;
-    const float hx = (float)(qx * rq), hy = (float)(qy * rq), hz = (float)(qz * rq);
-    // q split into two floats: t = (qhi - qhkl) + qlo keeps t accurate to float rounding
-    // of t itself (the radial Gaussian is narrow, so its exponent is steep in t).
+    L.hx = (float)(qx * rq);
+    L.hy = (float)(qy * rq);
+    L.hz = (float)(qz * rq);
+    L.qf = (float)q;
+    L.qlo = (float)(q - (double)L.qf);
+    const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));
+    const double krd = -4.0 / (kPiD * width * width) * kLog2e;
+    const double kpd = -x2phi * kLog2e;
+    L.kr = (float)krd;
+    L.kp = (float)kpd;
+    L.c1 = (float)(kpd * kPiD * kPiD);
+    L.c2 = (float)(-2.0 * kPiD * kpd);
+    const float cut = d.cut;
+    // Window edges are bounds, computed in FP32 with a relative margin of 1e-4.
+    // radial window: |q - q_hkl| <= dq  <=>  radial exponent >= cut
+    const float dq = __builtin_amdgcn_sqrtf(cut / L.kr) * 1.0001f + 1e-6f;
+    // angular window: y <= ymax  <=>  angular exponent >= cut (y <= pi/2 always)
+    const float hpi = 1.5707963f;
+    const float ymax = (L.kp < 0.0f) ? fminf(__builtin_amdgcn_sqrtf(cut / L.kp) * 1.0001f + 1e-6f, hpi) : hpi;
+    float uz_lo = 0.0f, uz_hi;
+    if constexpr (PLANAR) {
+        uz_hi = (ymax < hpi) ? sinf(ymax) * 1.0001f + 1e-6f : 1.0f;
+    } else {
+        // angle(q^, +-u) >= | |asin q^_z| - |asin u_z| |
+        const float eh = asinf(fminf(fabsf(L.hz), 1.0f));
+        uz_lo = (eh - ymax > 0.0f) ? sinf(eh - ymax) * 0.9999f - 1e-6f : 0.0f;
+        uz_hi = (eh + ymax < hpi) ? sinf(eh + ymax) * 1.0001f + 1e-6f : 1.0f;
+    }
+    // second Friedel term: e2 <= kp*(pi/2)^2 for every pair
+    const bool two = L.kp * (hpi * hpi) > cut;
+    Window W;
+    {
+        const float lo = wave_min_f(L.qf - dq), hi = wave_max_f(L.qf + dq);
+        const float ulo = wave_min_f(uz_lo), uhi = wave_max_f(uz_hi);
+        // Bound of this lane's dropped mass (sum of A * 2^e over dropped terms).  Pairs left
+        // out by the |q| window lie below lo or above hi, and inside [qh_min, qh_max].
+        double dmin = HUGE_VAL;
+        if (d.qh_min < lo) dmin = fmin(dmin, q - fmin((double)lo, (double)d.qh_max));
+        if (d.qh_max > hi) dmin = fmin(dmin, fmax((double)hi, (double)d.qh_min) - q);
+        dmin = fmax(dmin, 0.0);
+        dropped = (dmin < HUGE_VAL) ? d.asum2 * exp2(krd * dmin * dmin) : 0.0;
+        W.b0 = uniform_i(max(0, min(d.nlut, (int)floorf(lo * d.lut_inv_dq))));
+        W.b1 = uniform_i(max(0, min(d.nlut, (int)ceilf(hi * d.lut_inv_dq))));
+        W.band0 = uniform_i(max(0, min(kBands - 1, (int)floorf(ulo * kBands))));
+        W.band1 = uniform_i(max(0, min(kBands - 1, (int)floorf(uhi * kBands))));
+    }
+    const bool any_two = __builtin_amdgcn_readfirstlane((int)(__ballot(two) != 0ull)) != 0;
+    if (W.band0 > 0 || W.band1 < kBands - 1) {
+        dropped += d.asum2 * exp2((double)cut);
+    } else {
+        W.band0 = W.band1 = kBands;  // every band: the merged list, in one pass
+    }
+    if (!any_two) dropped += 0.5 * d.asum2 * exp2(kpd * 0.25 * kPiD * kPiD);
+    double acc = 0.0;
+    bool flag = false;
+    long long evals = 0;
+    for (int band = W.band0; band <= W.band1; ++band) {
+        const int32_t* lb = d.lut + band * (d.nlut + 1);
+        const int p0 = uniform_i(lb[W.b0] & ~3);
+        const int p1 = uniform_i((lb[W.b1] + 3) & ~3);
+        if (any_two) pair_sum<PLANAR, true>(d.pairs, p0, p1, L, acc, flag);
+        else pair_sum<PLANAR, false>(d.pairs, p0, p1, L, acc, flag);
+        evals += p1 - p0;
+    }
+    double inten = acc / (kPiD * width / 2.0);
+    if (flag) {
+        for (int band = W.band0; band <= W.band1; ++band) {
+            const int32_t* lb = d.lut + band * (d.nlut + 1);
+            const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
+            inten += near_one_terms<PLANAR>(d, p0, p1, L, qx, qy, qz, q, width, x2phi);
+        }
+    }
+    if (stats && (threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)(64 * evals));
+        atomicAdd(&stats[3], (unsigned long long)(any_two ? 1 : 0));
+    }
+    if (stats && flag) atomicAdd(&stats[2], 1ull);
+    need_exact = d.nsingle > 0;
+    return inten;
+}
+
+// Isotropic (radial) peak sum in FP32, only needed when iso != 0.
+__device__ double iso_mixed(const DevSet& d, double q) {
+    const double wi = 1.0 / d.s.uca;
+    const float ki = (float)(-4.0 / (kPiD * wi * wi) * kLog2e);
     const float qf = (float)q;
     const float qlo = (float)(q - (double)qf);
-    const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));
-    const float kr = (float)(-4.0 / (kPiD * width * width) * kLog2e);
-    const float kp = (float)(-x2phi * kLog2e);
-    double acc = 0.0, exact = 0.0;
-    float part = 0.0f;
-    const float4* __restrict__ pr = d.pairs;
-    const int np = d.npairs;
-    for (int p = 0; p < np; ++p) {
-        const float4 a = pr[2 * p];
-        const float4 b = pr[2 * p + 1];
-        const float dt = __builtin_fmaf(hz, a.z, __builtin_fmaf(hy, a.y, hx * a.x));
-        const float ad = fabsf(dt);
-        if (ad > 0.999999f) {
-            exact += aniso_term_ref(d, __float_as_int(b.y), qx, qy, qz, q, width, x2phi);
-            exact += aniso_term_ref(d, __float_as_int(b.z), qx, qy, qz, q, width, x2phi);
-            continue;
-        }
-        const float y = acos01(ad);
-        const float ym = kPiF - y;
-        const float t = (qf - a.w) + qlo;
-        const float r = (t * kr) * t;
-        const float e1 = __builtin_fmaf(kp * y, y, r);
-        const float e2 = __builtin_fmaf(kp * ym, ym, r);
-        const float w = __builtin_amdgcn_exp2f(e1) + __builtin_amdgcn_exp2f(e2);
-        part = __builtin_fmaf(b.x, w, part);
-        if ((p & 15) == 15) { acc += (double)part; part = 0.0f; }
+    float ra = 0.0f;
+    for (int k = 0; k < d.niso; ++k) {
+        const float2 e = d.iso[k];
+        const float t = (qf - e.x) + qlo;
+        ra = __builtin_fmaf(e.y, __builtin_amdgcn_exp2f((t * ki) * t), ra);
     }
-    acc += (double)part;
-    // Reflections without a Friedel partner (not produced by FCC/BCT tables).
-    const float4* sg = pr + 2 * np;
-    for (int k = 0; k < d.nsingle; ++k) {
-        exact += aniso_term_ref(d, __float_as_int(sg[k].x), qx, qy, qz, q, width, x2phi);
-    }
-    inten_scaled = acc;
-    inten_exact = exact;
-    radint = 0.0;
-    if (s.iso != 0) {
-        const double wi = 1.0 / s.uca;
-        const float ki = (float)(-4.0 / (kPiD * wi * wi) * kLog2e);
-        float ra = 0.0f;
-        for (int k = 0; k < d.niso; ++k) {
-            const float2 e = d.iso[k];
-            const float t = qf - e.x;
-            ra = __builtin_fmaf(e.y, __builtin_amdgcn_exp2f((t * ki) * t), ra);
-        }
-        radint = (double)ra / (kPiD * wi / 2.0);
-    }
+    return (double)ra / (kPiD * wi / 2.0);
 }
 
 // ---------------------------------------------------------------------------
 // Pixel value of each combination
 // ---------------------------------------------------------------------------
 template <int KIND, int PREC>
-__device__ double pixel_value(const DevSet& d, double qx, double qy, double qz) {
+__device__ double pixel_value(const DevSet& d, double qx, double qy, double qz,
+                              unsigned long long* stats = nullptr) {
     const csx_scalars& s = d.s;
     const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
     if constexpr (KIND == K_SPHERE) {
@@ -197,8 +335,13 @@ __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz) 
     } else {
         double pq, fq;
         if constexpr (KIND == K_FCC) {
-            pq = pq_sphere(d, q);
-            fq = fq_sphere(d, q);
+            if constexpr (PREC == CSX_PREC_MIXED) {
+                pq = pq_sphere_fast(d, q);
+                fq = fq_sphere_fast(d, q);
+            } else {
+                pq = pq_sphere(d, q);
+                fq = fq_sphere(d, q);
+            }
         } else {
             switch (s.orcase) {
             case 0:
@@ -221,16 +364,23 @@ __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz) 
         if constexpr (PREC == CSX_PREC_REF64) {
             lattice_ref64(d, qx, qy, qz, q, width, inten, radint);
         } else {
-            double scaled, exact;
-            lattice_mixed(d, qx, qy, qz, q, width, scaled, exact, radint);
-            inten = scaled / (kPiD * width / 2.0) + exact;
-            // Conditioning guard: szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
-            // The FP32 peak sum carries ~3e-6 relative error, which szq amplifies by |B|/|szq|;
-            // pixels where that could exceed 1e-5 are re-summed in FP64 (reference order).
-            const double B = (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol) * dwe;
+            bool need_exact = false;
+            double dropped = 0.0;
+            inten = s.centerbeam ? lattice_mixed<false>(d, qx, qy, qz, q, width, need_exact, dropped, stats)
+                                 : lattice_mixed<true>(d, qx, qy, qz, q, width, need_exact, dropped, stats);
+            radint = (s.iso != 0) ? iso_mixed(d, q) : 0.0;
+            // Conditioning guard.  szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
+            // The FP32 peak sum I carries up to ~1e-5 relative error plus this lane's dropped
+            // mass; szq amplifies an error of I by |B|/|szq|.  Pixels where szq could move by
+            // more than 3e-5 relative, and by more than 2^-60 absolute (below the rounding of
+            // szq's own FP64 terms), are re-summed in FP64 in the reference's order.
+            const double bs = (fq / pq) * (2 * kPiD / s.sphno_cubevol) * dwe;
             const double szq_est = 1 + (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe;
-            if (!(fabs(B) * 3.0e-6 <= 1.0e-5 * fabs(szq_est)))
+            const double dsz = fabs(bs) * (1.0e-5 * fabs(inten) + dropped / (kPiD * width / 2.0));
+            if (need_exact || !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60))) {
                 lattice_ref64(d, qx, qy, qz, q, width, inten, radint);
+                if (stats) atomicAdd(&stats[1], 1ull);
+            }
         }
         const double szqiso = (1 + (1) * (8 * kPiD * kPiD * kPiD * radint / (4 * kPiD * s.sphno_cubevol) - 1) * exp(-s.ucb * q * q));
         const double szq = (1 + (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
@@ -244,16 +394,33 @@ __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz) 
 // ---------------------------------------------------------------------------
 
 // Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.
+// Pixel (px, py) of a 16x16 tile handled by lane threadIdx.x: each wave covers an 8x8
+// quadrant, the most compact 64-pixel footprint (narrowest |q| and angle ranges per wave).
+__device__ __forceinline__ void tile_pixel(int& px, int& py) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    px = (w & 1) * 8 + (l & 7);
+    py = (w >> 1) * 8 + (l >> 3);
+}
+
+// Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.
 template <int KIND, int PREC>
-__global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out) {
-    const int ihex = g.xmin + blockIdx.x * kTile + (threadIdx.x & (kTile - 1));
-    const int i = g.ymin + blockIdx.y * kTile + (threadIdx.x / kTile);
-    if (ihex >= g.xmax || i >= g.ymax) return;
+__global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out,
+                                                  unsigned long long* __restrict__ stats) {
+    // Every lane computes (the mixed path needs all 64 lanes of a wave); lanes past the grid
+    // edge evaluate the last valid pixel and do not store.
+    int px, py;
+    tile_pixel(px, py);
+    const int ihex0 = g.xmin + blockIdx.x * kTile + px;
+    const int i0 = g.ymin + blockIdx.y * kTile + py;
+    const bool valid = ihex0 < g.xmax && i0 < g.ymax;
+    const int ihex = min(ihex0, g.xmax - 1), i = min(i0, g.ymax - 1);
     double qx, qy, qz;
     pixel_q(d.s, ihex, i, qx, qy, qz);
-    const double v = pixel_value<KIND, PREC>(d, qx, qy, qz);
-    const size_t W = (size_t)(g.xmax - g.xmin);
-    out[(size_t)(ihex - g.xmin) + W * (size_t)(i - g.ymin)] = v;
+    const double v = pixel_value<KIND, PREC>(d, qx, qy, qz, stats);
+    if (valid) {
+        const size_t W = (size_t)(g.xmax - g.xmin);
+        out[(size_t)(ihex - g.xmin) + W * (size_t)(i - g.ymin)] = v;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_qgrid(const csx_scalars s, const csx_grid g, double* __restrict__ out) {
@@ -317,18 +484,25 @@ __device__ void fit_tile(const FitSet& f, int tile, double* __restrict__ fqs_out
     __shared__ double s_sum[kBlock / 64];
     __shared__ int s_cnt[kBlock / 64], s_nan[kBlock / 64];
     const int tx = tile % f.tiles_x, ty = tile / f.tiles_x;
-    const int x = tx * kTile + (threadIdx.x & (kTile - 1));
-    const int y = ty * kTile + (threadIdx.x / kTile);
+    int px, py;
+    tile_pixel(px, py);
+    const int x = tx * kTile + px;
+    const int y = ty * kTile + py;
     double term = 0.0;
     int nz = 0, isnan_ = 0;
-    if (x < f.w && y < f.h) {
-        const csx_scalars& s = f.d.s;
-        if (!fit_masked(s, f.m, f.data, f.w, x, y)) {
-            const int ihex = x + s.iimin, i = y + s.iimin;
-            double qx, qy, qz;
-            pixel_q(s, ihex, i, qx, qy, qz);
-            const double v = pixel_value<KIND, PREC>(f.d, qx, qy, qz);
-            const size_t idx = (size_t)(ihex - s.xmin) + (size_t)(s.xmax - s.xmin) * (size_t)(i - s.ymin);
+    // All lanes evaluate a pixel (wave-wide |q| ranges need all 64 lanes); lanes outside the
+    // fit array or masked discard the value.
+    const bool inside = x < f.w && y < f.h;
+    const int xc = min(x, f.w - 1), yc = min(y, f.h - 1);
+    const csx_scalars& s = f.d.s;
+    const bool masked = !inside || fit_masked(s, f.m, f.data, f.w, xc, yc);
+    const int ihex = xc + s.iimin, i = yc + s.iimin;
+    double qx, qy, qz;
+    pixel_q(s, ihex, i, qx, qy, qz);
+    const double v = pixel_value<KIND, PREC>(f.d, qx, qy, qz);
+    if (inside) {
+        const size_t idx = (size_t)(ihex - s.xmin) + (size_t)(s.xmax - s.xmin) * (size_t)(i - s.ymin);
+        if (!masked) {
             const double dv = f.data[idx];
             if (v > 0 && dv > 0) {
                 const double a = log(v) / log(10.) - log(dv) / log(10.);
@@ -408,24 +582,35 @@ struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
-// Host-side fast tables of one parameter set (Friedel pairs and isotropic list).
+// Base-2 exponent below which the mixed path drops a pair term (2^-40 ~ 9e-13 of the
+// pair's weight); each pixel's bound of the dropped mass enters its conditioning guard.
+constexpr float kDropExponent = -40.0f;
+constexpr int kLutBins = 256;
+
+// Host-side fast tables of one parameter set: Friedel pairs in |u_z| bands sorted by
+// |q_hkl|, the bin table, unpaired reflections and the isotropic list.
 struct FastTables {
-    std::vector<float4> pairs;  // 2 per pair, then 1 per unpaired reflection
+    std::vector<PairRec> pairs;
+    std::vector<int32_t> lut;
+    std::vector<int32_t> singles;
     std::vector<float2> iso;
-    int npairs = 0, nsingle = 0;
+    float lut_inv_dq = 0.0f;
+    float qh_min = 0.0f, qh_max = 0.0f;
+    double asum2 = 0.0;
 };
 
 void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L2, const float* L3, FastTables& ft) {
-    ft.pairs.clear();
-    ft.iso.clear();
-    ft.npairs = ft.nsingle = 0;
+    ft = FastTables{};
     const int n2 = s.peakmax2;
     std::vector<char> used(n2 + 1, 0);
-    std::vector<int> singles;
+    std::vector<PairRec> found;
+    std::vector<int> band_of;
     auto valid = [&](int ii) {
         const double qh = L3[ii * 17 + 5];
         return qh > 0 && L2[ii * 6 + 4] * L2[ii * 6 + 5] != 0;
     };
+    double asum = 0.0;
+    bool negative = false;
     for (int ii = 1; ii <= n2; ++ii) {
         if (used[ii] || !valid(ii)) continue;
         used[ii] = 1;
@@ -434,38 +619,82 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
         for (int c = 1; c <= n2 && !jj; ++c) {
             if (!used[c] && L2[c * 6 + 1] == -h && L2[c * 6 + 2] == -k && L2[c * 6 + 3] == -l && valid(c) &&
                 L3[c * 17 + 5] == L3[ii * 17 + 5] && L3[c * 17 + 10] == L3[ii * 17 + 10] &&
-                L3[c * 17 + 11] == L3[ii * 17 + 11] && L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5])
+                L3[c * 17 + 11] == L3[ii * 17 + 11] && L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5] &&
+                L3[c * 17 + 2] == -L3[ii * 17 + 2] && L3[c * 17 + 3] == -L3[ii * 17 + 3] &&
+                L3[c * 17 + 4] == -L3[ii * 17 + 4])
                 jj = c;
         }
-        if (!jj) { singles.push_back(ii); continue; }
+        if (!jj) { ft.singles.push_back(ii); continue; }
         used[jj] = 1;
         const double qh = L3[ii * 17 + 5];
         const double A = (double)(L2[ii * 6 + 4] * L2[ii * 6 + 5]) * (double)L3[ii * 17 + 10] / (double)L3[ii * 17 + 11];
-        float4 a, b;
-        a.x = (float)((double)L3[ii * 17 + 2] / qh);
-        a.y = (float)((double)L3[ii * 17 + 3] / qh);
-        a.z = (float)((double)L3[ii * 17 + 4] / qh);
-        a.w = L3[ii * 17 + 5];
-        b.x = (float)A;
-        int ia = ii, ib = jj;
-        std::memcpy(&b.y, &ia, 4);
-        std::memcpy(&b.z, &ib, 4);
-        b.w = 0.0f;
-        ft.pairs.push_back(a);
-        ft.pairs.push_back(b);
-        ft.npairs++;
+        const double ux = (double)L3[ii * 17 + 2], uy = (double)L3[ii * 17 + 3], uz = (double)L3[ii * 17 + 4];
+        const double un = std::sqrt(ux * ux + uy * uy + uz * uz);
+        PairRec r;
+        r.ux = (float)(ux / un);
+        r.uy = (float)(uy / un);
+        r.uz = (float)(uz / un);
+        r.qh = L3[ii * 17 + 5];
+        r.uz2 = (float)((uz / un) * (uz / un));
+        r.A = (float)A;
+        r.ia = ii;
+        r.ib = jj;
+        found.push_back(r);
+        band_of.push_back(std::min(kBands - 1, (int)(std::fabs(uz / un) * kBands)));
+        asum += 2.0 * std::fabs(A);
+        negative = negative || A < 0;
     }
-    for (int ii : singles) {
-        float4 r{};
-        std::memcpy(&r.x, &ii, 4);
-        ft.pairs.push_back(r);
+    float qtop = 0.0f, qbot = found.empty() ? 0.0f : found[0].qh;
+    for (const PairRec& r : found) {
+        qtop = std::max(qtop, r.qh);
+        qbot = std::min(qbot, r.qh);
     }
-    ft.nsingle = (int)singles.size();
+    ft.qh_min = qbot;
+    ft.qh_max = qtop;
+    const double span = (double)qtop * 1.001 + 1e-3;
+    ft.lut_inv_dq = (float)(kLutBins / span);
+    const double dqb = 1.0 / (double)ft.lut_inv_dq;
+    ft.lut.assign((size_t)(kBands + 1) * (kLutBins + 1), 0);
+    const PairRec inert{0.0f, 0.0f, 0.0f, 1.0e30f, 0.0f, 0.0f, 0, 0};
+    // Segments 0..kBands-1: one |u_z| band each; segment kBands: every pair.
+    for (int b = 0; b <= kBands; ++b) {
+        std::vector<PairRec> band;
+        for (size_t k = 0; k < found.size(); ++k)
+            if (b == kBands || band_of[k] == b) band.push_back(found[k]);
+        std::stable_sort(band.begin(), band.end(), [](const PairRec& x, const PairRec& y) { return x.qh < y.qh; });
+        const int start = (int)ft.pairs.size();  // a multiple of 4
+        for (const PairRec& r : band) ft.pairs.push_back(r);
+        const int end = (int)ft.pairs.size();
+        while (ft.pairs.size() % 4) ft.pairs.push_back(inert);
+        int32_t* lb = ft.lut.data() + (size_t)b * (kLutBins + 1);
+        int idx = start;
+        for (int k = 0; k < kLutBins; ++k) {
+            const double edge = k * dqb;
+            while (idx < end && (double)ft.pairs[idx].qh < edge) ++idx;
+            lb[k] = idx;
+        }
+        lb[kLutBins] = end;
+    }
+    // With a negative weight the relative error of the FP32 sum is unbounded: force the
+    // conditioning guard to take the FP64 sum everywhere.
+    ft.asum2 = negative ? HUGE_VAL : asum;
     for (int ii = 1; ii <= s.peakmax1 && ii <= n2; ++ii) {
         const int m = L1[ii * 6 + 4], f = L1[ii * 6 + 5];
         if (m == 0 || f == 0) continue;
         ft.iso.push_back(make_float2(L3[ii * 17 + 5], (float)(m * f)));
     }
+}
+
+void set_fast(DevSet& d, const FastTables& ft) {
+    d.npairs = (int)ft.pairs.size();
+    d.nsingle = (int)ft.singles.size();
+    d.niso = (int)ft.iso.size();
+    d.nlut = kLutBins;
+    d.lut_inv_dq = ft.lut_inv_dq;
+    d.cut = kDropExponent;
+    d.qh_min = ft.qh_min;
+    d.qh_max = ft.qh_max;
+    d.asum2 = ft.asum2;
 }
 
 }  // namespace
@@ -476,16 +705,17 @@ struct csx_ctx {
     bool own_stream = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    bool timing_pending = false;
     std::string err;
     bool uploaded = false;
     DevSet set{};
-    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, iso, tiles, fqs_tmp;
+    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, lut, singles, iso, tiles, fqs_tmp, stats;
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
     std::vector<int2> btiles;
     std::vector<char> barena;  // host image of the device arena (tables of all sets)
-    struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_iso; };
+    struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_lut, off_singles, off_iso; };
     std::vector<Pending> bpend;
     DevBuf barena_d, bsets_d, btiles_d, bpart_d, bntiles_d, bsums_d, bcnts_d;
 };
@@ -502,9 +732,10 @@ static int fail(csx_ctx* c, int code, const std::string& msg) {
     } while (0)
 
 template <int K, int P>
-static void launch_image(const DevSet& d, const csx_grid& g, double* out, hipStream_t st) {
+static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
+                         hipStream_t st) {
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
-    hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out);
+    hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats);
 }
 
 int csx_qgrid(csx_ctx* c, const csx_grid* g, double* d_q3) {
@@ -527,6 +758,13 @@ static void launch_fit_one(const FitSet& f, int ntiles, double* fqs, TileSum* pa
 template <int K, int P>
 static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, TileSum* part, int nblocks, hipStream_t st) {
     hipLaunchKernelGGL((k_fit_batch<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
+}
+
+template <class T>
+static int put_table(csx_ctx* c, DevBuf& b, const std::vector<T>& v) {
+    HIPCHK(c, b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T)));
+    if (!v.empty()) HIPCHK(c, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return CSX_OK;
 }
 
 #define DISPATCH_KIND(kind, prec, FN, ...)                                          \
@@ -562,7 +800,8 @@ void csx_destroy(csx_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->iso, &c->tiles,
+    for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->lut, &c->singles,
+                      &c->iso, &c->stats, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
                       &c->bsums_d, &c->bcnts_d})
         b->release();
@@ -596,7 +835,16 @@ int csx_last_error(csx_ctx* c, char* buf, int len) {
     return CSX_OK;
 }
 
-double csx_last_kernel_ms(csx_ctx* c) { return c ? c->last_ms : -1.0; }
+double csx_last_kernel_ms(csx_ctx* c) {
+    if (!c) return -1.0;
+    if (c->timing_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(c->ev1) == hipSuccess && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
+            c->last_ms = ms;
+        c->timing_pending = false;
+    }
+    return c->last_ms;
+}
 
 static int check_scalars(csx_ctx* c, const csx_scalars* s) {
     if (!s) return fail(c, CSX_E_ARG, "null scalars");
@@ -639,22 +887,19 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
         HIPCHK(c, hipMemcpyAsync(c->lat2.p, h_l2, b2, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->lat3.p, h_l3, b3, hipMemcpyHostToDevice, c->stream));
         build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
-        const size_t bp = std::max<size_t>(ft.pairs.size(), 1) * sizeof(float4);
-        const size_t bi = std::max<size_t>(ft.iso.size(), 1) * sizeof(float2);
-        HIPCHK(c, c->pairs.ensure(bp));
-        HIPCHK(c, c->iso.ensure(bi));
-        if (!ft.pairs.empty())
-            HIPCHK(c, hipMemcpyAsync(c->pairs.p, ft.pairs.data(), ft.pairs.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
-        if (!ft.iso.empty())
-            HIPCHK(c, hipMemcpyAsync(c->iso.p, ft.iso.data(), ft.iso.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream));
         c->set.latpar1 = (const int32_t*)c->lat1.p;
         c->set.latpar2 = (const int32_t*)c->lat2.p;
         c->set.latpar3 = (const float*)c->lat3.p;
-        c->set.pairs = (const float4*)c->pairs.p;
+        rc = put_table(c, c->pairs, ft.pairs);
+        if (!rc) rc = put_table(c, c->lut, ft.lut);
+        if (!rc) rc = put_table(c, c->singles, ft.singles);
+        if (!rc) rc = put_table(c, c->iso, ft.iso);
+        if (rc) return rc;
+        c->set.pairs = (const PairRec*)c->pairs.p;
+        c->set.lut = (const int32_t*)c->lut.p;
+        c->set.singles = (const int32_t*)c->singles.p;
         c->set.iso = (const float2*)c->iso.p;
-        c->set.npairs = ft.npairs;
-        c->set.nsingle = ft.nsingle;
-        c->set.niso = (int)ft.iso.size();
+        set_fast(c->set, ft);
     }
     // The host staging memory is reused: wait until the copies have consumed it.
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -674,13 +919,29 @@ int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
     HIPCHK(c, hipSetDevice(c->device));
     const int kind = kind_of(c->set.s.combo_key);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, c->stream);
+    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, nullptr, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_ms = ms;
+    c->timing_pending = true;  // asynchronous: csx_last_kernel_ms waits for the events
+    return CSX_OK;
+}
+
+int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out, uint64_t* h_stats) {
+    if (!c || !g || !d_out || !h_stats) return CSX_E_ARG;
+    if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_render_stats before csx_upload");
+    if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
+    if (precision != CSX_PREC_REF64 && precision != CSX_PREC_MIXED) return fail(c, CSX_E_ARG, "bad precision");
+    if (c->set.s.fix_orcase && kind_of(c->set.s.combo_key) == K_CYL_GAUSS)
+        return fail(c, CSX_E_ARG, "orientation-averaged cylinders in physics mode are not available");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->stats.ensure(CSX_NSTATS * sizeof(uint64_t)));
+    const std::vector<uint64_t> zero(CSX_NSTATS, 0);
+    HIPCHK(c, hipMemcpyAsync(c->stats.p, zero.data(), CSX_NSTATS * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    const int kind = kind_of(c->set.s.combo_key);
+    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, (unsigned long long*)c->stats.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h_stats, c->stats.p, CSX_NSTATS * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return CSX_OK;
 }
 
@@ -715,6 +976,7 @@ int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, c
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->timing_pending = false;
     double sum = 0.0;
     int64_t cnt = 0, nn = 0;
     for (const auto& p : part) { sum += p.sum; cnt += p.cnt; nn += p.nan; }
@@ -773,11 +1035,11 @@ int csx_fit_batch_stage(csx_ctx* c, const csx_scalars* s, const double* h_carr, 
         pd.off_l3 = arena_put(c->barena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
         FastTables ft;
         build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
-        pd.off_pairs = arena_put(c->barena, ft.pairs.data(), ft.pairs.size() * sizeof(float4));
+        pd.off_pairs = arena_put(c->barena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec));
+        pd.off_lut = arena_put(c->barena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
+        pd.off_singles = arena_put(c->barena, ft.singles.data(), ft.singles.size() * sizeof(int32_t));
         pd.off_iso = arena_put(c->barena, ft.iso.data(), ft.iso.size() * sizeof(float2));
-        f.d.npairs = ft.npairs;
-        f.d.nsingle = ft.nsingle;
-        f.d.niso = (int)ft.iso.size();
+        set_fast(f.d, ft);
     }
     for (int k = 0; k < nt; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
     c->bsets.push_back(f);
@@ -802,7 +1064,9 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
             f.d.latpar1 = (const int32_t*)(base + pd.off_l1);
             f.d.latpar2 = (const int32_t*)(base + pd.off_l2);
             f.d.latpar3 = (const float*)(base + pd.off_l3);
-            f.d.pairs = (const float4*)(base + pd.off_pairs);
+            f.d.pairs = (const PairRec*)(base + pd.off_pairs);
+            f.d.lut = (const int32_t*)(base + pd.off_lut);
+            f.d.singles = (const int32_t*)(base + pd.off_singles);
             f.d.iso = (const float2*)(base + pd.off_iso);
         }
     }
@@ -838,6 +1102,7 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->timing_pending = false;
     return CSX_OK;
 }
 

@@ -54,7 +54,7 @@ void csx_engine_free(csx_engine* e) {
 
 int csx_engine_set(csx_engine* e, const char* key, double v) {
     if (!e || !key) return CSX_E_ARG;
-    if (std::strcmp(key, "ignore_cbs") == 0) { e->ignore_cbs = v != 0; return CSX_OK; }
+    if (std::strcmp(key, "ignore_cbs") == 0) { e->ignore_cbs = v != 0; e->prepared = false; return CSX_OK; }
     if (!e->p.set(key, v)) { e->err = std::string("unknown key ") + key; return CSX_E_ARG; }
     e->prepared = false;
     return CSX_OK;
@@ -92,7 +92,7 @@ int csx_engine_prepare(csx_engine* e) { return e ? engine_prepare(e, true) : CSX
 int csx_engine_calc_device(csx_engine* e, int precision, double* d_out, int32_t* b4) {
     if (!e || !d_out) return CSX_E_ARG;
     if (!e->ctx) { e->err = "host-only engine has no device"; return CSX_E_STATE; }
-    int rc = engine_prepare(e, true);
+    int rc = e->prepared ? CSX_OK : engine_prepare(e, true);  // tables stay in HBM until a parameter changes
     if (rc != CSX_OK) return rc;
     const csx_scalars& s = e->P.s;
     csx_grid g{s.xmin, s.xmax, s.ymin, s.ymax};
@@ -109,7 +109,7 @@ int csx_engine_calc_device(csx_engine* e, int precision, double* d_out, int32_t*
 int csx_engine_calc(csx_engine* e, int precision, double* h_out, int32_t* b4) {
     if (!e || !h_out) return CSX_E_ARG;
     if (!e->ctx) { e->err = "host-only engine has no device"; return CSX_E_STATE; }
-    int rc = engine_prepare(e, true);
+    int rc = e->prepared ? CSX_OK : engine_prepare(e, true);
     if (rc != CSX_OK) return rc;
     const csx_scalars& s = e->P.s;
     const size_t n = (size_t)(s.xmax - s.xmin) * (size_t)(s.ymax - s.ymin);
@@ -122,7 +122,8 @@ int csx_engine_calc(csx_engine* e, int precision, double* h_out, int32_t* b4) {
     }
     rc = csx_engine_calc_device(e, precision, e->d_img, b4);
     if (rc != CSX_OK) return rc;
-    if (hipMemcpy(h_out, e->d_img, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (csx_synchronize(e->ctx) != CSX_OK ||
+        hipMemcpy(h_out, e->d_img, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
         e->err = "copy image to host";
         return CSX_E_HIP;
     }
@@ -157,7 +158,7 @@ int csx_engine_fit(csx_engine* e, int precision, int bstop, int border, double* 
     if (!e || !h_sum) return CSX_E_ARG;
     *h_sum = 1e20;
     if (!e->d_fit) { e->err = "no fit data"; return CSX_E_STATE; }
-    int rc = engine_prepare(e, false);  // doFitCalculation does not run checkAllParameters
+    int rc = e->prepared ? CSX_OK : engine_prepare(e, false);  // no checkAllParameters in the fit path
     if (rc != CSX_OK) return rc;
     csx_mask m{};
     std::memcpy(m.rects, e->rects, sizeof m.rects);

@@ -23,6 +23,15 @@ namespace csx {
 constexpr double kPiD = 3.14159265358979323846;
 
 // Device copy of one parameter set: scalars by value, tables in HBM.
+// One Friedel pair (+hkl, -hkl) of the anisotropic list: unit direction u of q_hkl,
+// |q_hkl|, u_z^2, the common weight A = mhkl*fhkl*g3/peaknorm1 and both latpar indices.
+struct alignas(32) PairRec {
+    float ux, uy, uz, qh;
+    float uz2, A;
+    int32_t ia, ib;
+};
+constexpr int kBands = 8;
+
 struct DevSet {
     csx_scalars s;
     const double* carr;      // 18 x CSX_NCOEF
@@ -30,9 +39,20 @@ struct DevSet {
     const int32_t* latpar1;  // (peakmax1+1) x 6
     const int32_t* latpar2;  // (peakmax2+1) x 6
     const float* latpar3;    // (peakmax2+1) x 17
-    const float4* pairs;     // fast path: 2 float4 per Friedel pair (see csx_device.hip)
-    const float2* iso;       // fast path: isotropic peaks (qhkl, mhkl*fhkl)
-    int32_t npairs, nsingle, niso, pad;
+    // Mixed-precision tables (built at upload, see csx_device.hip).  Friedel pairs in kBands+1
+    // segments: segment b < kBands holds the pairs with b/kBands <= |u_z| < (b+1)/kBands,
+    // segment kBands all pairs; each is sorted by |q_hkl|, starts at a multiple of 4 and is
+    // padded to one with inert records.  lut[b*(nlut+1)+k] is the index of the first pair of
+    // segment b with |q_hkl| >= k/lut_inv_dq (the segment end for k = nlut).
+    const struct PairRec* pairs;
+    const int32_t* lut;
+    const int32_t* singles;  // latpar indices of unpaired reflections
+    const float2* iso;       // (|q_hkl|, mhkl*fhkl)
+    int32_t npairs, nsingle, niso, nlut;
+    float lut_inv_dq;
+    float cut;               // base-2 exponent below which a pair term is dropped
+    float qh_min, qh_max;    // |q_hkl| range of the pairs
+    double asum2;            // sum over pairs of 2|A| (infinite when a weight is negative)
 };
 
 __device__ __forceinline__ const double* carr_of(const DevSet& d, int which) {
@@ -112,6 +132,55 @@ __device__ inline double fq_sphere(const DevSet& d, double q) {
     const double at = atan(argq), b = 1.0 + argq * argq;
     const double pq1 = pqr * cos((zr - 1) * at) / pow(b, (zr - 1) / 2.0);
     const double pq2 = (pqr / ((zr - 2) * argq)) * sin((zr - 2) * at) / pow(b, (zr - 2) / 2.0);
+    const double pq3 = 3 * (pq2 - pq1);
+    return pq3 * pq3;
+}
+
+// Same closed forms with fewer FP64 library calls, for the mixed-precision path: the three
+// powers of one base b come from one log and one exp (b^((zr-3)/2) = P*b, b^((zr-4)/2) =
+// P*sqrt(b), b^(zr-5) = P^2 with P = b^((zr-5)/2)), and the cosines at (zr-3)*at and
+// (zr-5)*at from the sine and cosine at (zr-4)*at by angle addition, with cos(at) =
+// 1/sqrt(b) and sin(at) = 2*argq/sqrt(b) for at = atan(2*argq).  Differences from the
+// reference's evaluation are rounding-level (phases agree to ~1e-14 rad).
+__device__ inline double pq_sphere_fast(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    double zr = (1 - sqd(s.sigma)) / (sqd(s.sigma));
+    if (q < 0.4 * s.limq[3]) return series_x2(carr_of(d, 3), q, 100);
+    const double argq = q * s.radius / (zr + 1);
+    if (zr * (zr - 1) * (zr - 2) * (zr - 3) * (zr - 4) * (zr - 5) == 0) zr += 0.001;
+    const double a2 = argq * argq;
+    const double pqr = (1 / (2.0 * zr * (zr - 1) * (zr - 2) * (zr - 3))) / (a2 * a2);
+    const double u = 2.0 * argq;
+    const double at = atan(u), b = 1.0 + 4 * a2;
+    const double sb = sqrt(b);
+    const double P = exp(0.5 * (zr - 5) * log(b));
+    double s4, c4;
+    sincos((zr - 4) * at, &s4, &c4);
+    const double ca = 1.0 / sb, sa = u / sb;
+    const double cos3 = c4 * ca - s4 * sa, cos5 = c4 * ca + s4 * sa;
+    const double pq1 = pqr * (1 + cos3 / (P * b));
+    const double pq2 = (pqr / ((zr - 4) * argq)) * s4 / (P * sb);
+    const double pq3 = (pqr / ((zr - 4) * (zr - 5) * a2)) * (1 - cos5 / (P * P) / 2.0);
+    return 9.0 * (pq1 - 2.0 * pq2 + pq3);
+}
+
+// fq_sphere with one log/exp for both powers and the cosine at (zr-1)*at from the sine
+// and cosine at (zr-2)*at (at = atan(argq), cos(at) = 1/sqrt(b), sin(at) = argq/sqrt(b)).
+__device__ inline double fq_sphere_fast(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
+    if (q < (0.4 * s.limq[12])) return series_x2(carr_of(d, 12), q, 100);
+    const double argq = q * s.radius / (zr + 1);
+    const double a2 = argq * argq;
+    const double pqr = (1 / (zr * (zr - 1))) / a2;
+    const double at = atan(argq), b = 1.0 + a2;
+    const double sb = sqrt(b);
+    const double P = exp(0.5 * (zr - 2) * log(b));  // b^((zr-2)/2)
+    double s2, c2;
+    sincos((zr - 2) * at, &s2, &c2);
+    const double cos1 = c2 / sb - s2 * (argq / sb);
+    const double pq1 = pqr * cos1 / (P * sb);
+    const double pq2 = (pqr / ((zr - 2) * argq)) * s2 / P;
     const double pq3 = 3 * (pq2 - pq1);
     return pq3 * pq3;
 }

@@ -115,8 +115,16 @@ int csx_synchronize(csx_ctx* ctx);
 int csx_upload(csx_ctx* ctx, const csx_scalars* s, const double* h_carr, const double* h_carr11pm,
                const int32_t* h_latpar1, const int32_t* h_latpar2, const float* h_latpar3);
 
-/* Evaluate the pixels of `grid` into d_out (device, (xmax-xmin)*(ymax-ymin) doubles). */
+/* Evaluate the pixels of `grid` into d_out (device, (xmax-xmin)*(ymax-ymin) doubles).
+ * Asynchronous on the context's stream. */
 int csx_render(csx_ctx* ctx, const csx_grid* grid, int precision, double* d_out);
+
+/* csx_render plus work counters (synchronous; for measurement, no reference counterpart):
+ * h_stats[0] pair evaluations (lanes x pairs iterated, mixed path), [1] pixels re-summed in
+ * FP64 by the conditioning guard, [2] pixels with FP64 terms for |q^.u| near 1,
+ * [3] waves that evaluated both Friedel terms. */
+#define CSX_NSTATS 4
+int csx_render_stats(csx_ctx* ctx, const csx_grid* grid, int precision, double* d_out, uint64_t* h_stats);
 
 /* q vectors of the pixels of `grid` as the kernels generate them (convert_idx2q,
  * Source/sc_calc_generic_gpu.cu:249-287): d_q3 receives 3 doubles per pixel, IDX order. */
@@ -140,7 +148,7 @@ int csx_fit_batch_run(csx_ctx* ctx, int precision, double* h_sums, int64_t* h_cn
 
 int csx_last_error(csx_ctx* ctx, char* buf, int len);
 /* Kernel time of the last csx_render / csx_fit_fqs / csx_fit_batch_run, measured with HIP
- * events on the context's stream (milliseconds). */
+ * events on the context's stream (milliseconds); waits for a pending render. */
 double csx_last_kernel_ms(csx_ctx* ctx);
 
 /* ---- SasCalc_GENERIC_calculation facade ------------------------------------------- */

@@ -152,3 +152,27 @@ def gamma(z):
 
 def gaussnorm3(a):
     return lib().csxo_gaussnorm3(float(a))
+
+
+_OBJ = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
+
+
+def simplex(f, x0, numfit, minv, maxv, stp=0.1, maxit=200, ftol=1e-6):
+    """Nelder-Mead as doSimplexFit2D runs it (returns x, fbest, iterations, evaluations)."""
+    L = lib()
+    L.csxo_simplex.restype = ctypes.c_int
+    L.csxo_simplex.argtypes = [_OBJ, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                               ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+    m = len(x0)
+    x = np.array(x0, dtype=np.float64)
+    lo = np.array(minv, dtype=np.float64)
+    hi = np.array(maxv, dtype=np.float64)
+    cb = _OBJ(lambda user, p: float(f(np.ctypeslib.as_array(p, shape=(m,)).copy())))
+    it, fb, nev = ctypes.c_int(), ctypes.c_double(), ctypes.c_long()
+    rc = L.csxo_simplex(cb, None, numfit, m, _dp(x), _dp(lo), _dp(hi), stp, maxit, ftol, ctypes.byref(it),
+                        ctypes.byref(fb), ctypes.byref(nev))
+    if rc != 0:
+        raise RuntimeError("simplex failed")
+    return x, fb.value, it.value, nev.value

@@ -161,6 +161,7 @@ int main(int argc, char** argv) {
         const auto* CR = P->CR;
         w.d("carr1p", CR->carr1p, coeffarray_len);
         w.d("carr2p", CR->carr2p, coeffarray_len);
+        w.d("carr3p", CR->carr3p, coeffarray_len);
         w.d("carr4p", CR->carr4p, coeffarray_len);
         w.d("carr1f", CR->carr1f, coeffarray_len);
         w.d("carr2f", CR->carr2f, coeffarray_len);

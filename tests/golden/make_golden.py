@@ -1,0 +1,165 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own engine.
+
+Runs oracle/_ref/ref_driver (the reference's sc_calc_generic_cpu.cpp and
+sc_calc_generic_gpu.cu compiled unchanged by `make -C oracle ref`) on the
+parameter sets below and stores inputs and outputs as .npz data, plus the
+reference's only golden image (Documentation/Vitess-Sphere.dat) decoded.
+Needs the reference checkout at /root/reference; the fixtures themselves are
+self-contained and are what the tests read.
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+from refbin import read_refbin  # noqa: E402
+
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+REF_DOC = "/root/reference/Documentation"
+
+# Parameter sets (reference key names).  Config numbers follow the benchmark brief.
+CFG1 = dict(LType=12, ComboBoxParticle=0, Ordis=7, ComboBoxInterior=0, EditRadius=5, EditSigma=0.1,
+            GridPoints=64, EditQmax=2, I0=1, Base=0, iso=0)
+CFG2 = dict(LType=12, ComboBoxParticle=1, Ordis=0, ComboBoxInterior=0, EditRadius=3, Length=40,
+            EditSigma=0.06, SigmaL=0.06, EditDbeta=10, theta=30, phi=45, GridPoints=16, EditQmax=1.5,
+            I0=1, Base=0)
+CFG3 = dict(LType=5, ComboBoxParticle=0, Ordis=0, ComboBoxPeak=1, ComboBoxInterior=0, EditRadius=8,
+            EditSigma=0.08, uca=30, ucb=30, ucc=30, EditDomainSize=200, RadButDebyeScherrer=1,
+            EditDebyeWaller=1.5, EditAzi=5, HKLmax=5, GridPoints=512, EditQmax=1, I0=1, Base=0)
+CFG4_BASE = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, ComboBoxInterior=0, theta=0,
+                 phi=0, EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1, Base=0,
+                 HKLmax=5)
+VITESS = dict(LType=12, ComboBoxParticle=0, Ordis=7, EditRadius=16.7, EditSigma=0.0639, EditQmax=2,
+              GridPoints=128, I0=1, Base=0, CenterBeam=1, EditPixelX=2.5, EditPixelY=2.5,
+              EditPixelNoX=256, EditPixelNoY=256, EditDet=10, EditWavelength=0.154)
+
+
+def cfg4_params(rng):
+    """One config-4 parameter draw: R, L, a, c/a, domain size, angular peak width (EditAzi)."""
+    R = rng.uniform(2, 6)
+    L = rng.uniform(10, 60)
+    a = rng.uniform(15, 40)
+    ca = rng.uniform(1, 2)
+    dom = rng.uniform(100, 400)
+    azi = rng.uniform(2.0, 10.0)
+    p = dict(CFG4_BASE)
+    p.update(EditRadius=R, Length=L, uca=a, ucb=a, ucc=a * ca, EditDomainSize=dom, EditAzi=azi)
+    return p
+
+
+def run_ref(params, extra=None):
+    with tempfile.TemporaryDirectory() as td:
+        pf = os.path.join(td, "p.txt")
+        with open(pf, "w") as fh:
+            for k, v in list(params.items()) + list((extra or {}).items()):
+                fh.write("%s=%r\n" % (k, v) if not isinstance(v, str) else "%s=%s\n" % (k, v))
+        out = os.path.join(td, "o.bin")
+        subprocess.run([REF_DRIVER, pf, out], check=True, capture_output=True)
+        return read_refbin(out)
+
+
+TABLES = ["carr1p", "carr2p", "carr4p", "carr1f", "carr4f", "limq"]
+
+
+def table_dict(r, with_lattice):
+    d = {t: r[t] for t in TABLES}
+    for s in ("norm", "orcase", "ucvol", "cosphic", "sinphic", "polPhi"):
+        d[s] = r[s]
+    if with_lattice:
+        d["latpar1"] = r["latpar1"]
+        d["latpar2"] = r["latpar2"]
+        # columns 1..13 are the ones the reference fills (14..16 are never written)
+        d["latpar3"] = r["latpar3"].reshape(-1, 17)[:, :14].copy()
+        d["peakmax"] = np.array([r["peakmax1"][0], r["peakmax2"][0]])
+    return d
+
+
+def main():
+    if not os.path.exists(REF_DRIVER):
+        sys.exit("build the reference first: make -C oracle ref")
+    manifest = {"generator": "tests/golden/make_golden.py", "driver": "oracle/ref_driver.cpp",
+                "compiler": subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
+                "libc": platform.libc_ver(), "fixtures": {}}
+
+    # Reference golden image, decoded (Documentation/Vitess-Sphere.dat: 19-byte header + f64).
+    raw = open(os.path.join(REF_DOC, "Vitess-Sphere.dat"), "rb").read()
+    np.savez_compressed(os.path.join(HERE, "vitess_sphere.npz"), header=np.frombuffer(raw[:19], np.uint8),
+                        image=np.frombuffer(raw[19:], "<f8"))
+    manifest["fixtures"]["vitess_sphere.npz"] = {"source": "Documentation/Vitess-Sphere.dat",
+                                                 "params": VITESS}
+
+    # Config 1: sphere, 128x128, FP64.
+    r = run_ref(CFG1)
+    np.savez_compressed(os.path.join(HERE, "cfg1_sphere.npz"), image=r["image"], bounds=r["bounds"],
+                        **table_dict(r, False))
+    manifest["fixtures"]["cfg1_sphere.npz"] = {"params": CFG1}
+
+    # Config 2: Gaussian-oriented cylinders (reference mode), 32x32 + tables.
+    r = run_ref(CFG2)
+    c11 = r["carr11pm"].reshape(131, 131)[:121, :121].copy()
+    np.savez_compressed(os.path.join(HERE, "cfg2_cylgauss.npz"), image=r["image"], carr11pm=c11,
+                        carr3p=r["carr3p"] if "carr3p" in r else np.zeros(0), **table_dict(r, False))
+    manifest["fixtures"]["cfg2_cylgauss.npz"] = {"params": CFG2}
+
+    # Config 3: FCC spheres.  64x64 full image at the same q range, and 4096 sampled pixels of
+    # the full 1024x1024 image.
+    small = dict(CFG3, GridPoints=32)
+    r = run_ref(small)
+    np.savez_compressed(os.path.join(HERE, "cfg3_fcc_g32.npz"), image=r["image"], **table_dict(r, True))
+    manifest["fixtures"]["cfg3_fcc_g32.npz"] = {"params": small}
+    r = run_ref(CFG3, {"threads": 8})
+    rng = np.random.default_rng(20261202)
+    idx = np.sort(rng.choice(r["image"].size, 4096, replace=False))
+    np.savez_compressed(os.path.join(HERE, "cfg3_fcc_1024_samples.npz"), idx=idx.astype(np.int64),
+                        values=r["image"][idx], bounds=r["bounds"])
+    manifest["fixtures"]["cfg3_fcc_1024_samples.npz"] = {"params": CFG3, "sample": "4096 pixels, seed 20261202",
+                                                        "calc_ms_8threads": float(r["calc_ms"][0])}
+
+    # Config 4: BCT cylinders, six parameter draws at GridPoints 16 (32x32), plus the fit
+    # objective with every mask mode on one of them.
+    rng = np.random.default_rng(1234)
+    sets = []
+    arrays = {}
+    for k in range(6):
+        p = cfg4_params(rng)
+        p["GridPoints"] = 16
+        r = run_ref(p)
+        sets.append(p)
+        arrays["image_%d" % k] = r["image"]
+        for t, v in table_dict(r, True).items():
+            arrays["%s_%d" % (t, k)] = v
+    p = dict(sets[0], BeamPosX=16, BeamPosY=16)
+    data = np.random.default_rng(5678).uniform(0.01, 10.0, 32 * 32)
+    fits = []
+    with tempfile.TemporaryDirectory() as td:
+        fd = os.path.join(td, "fit.bin")
+        data.astype("<f8").tofile(fd)
+        for bstop, border, rect in [(0, 0, None), (4, 0, None), (0, 2, None), (-1, 0, None), (0, 0, (3, 4, 9, 12))]:
+            extra = dict(mode="fit", fitdata=fd, fitw=32, fith=32, bstop=bstop, border=border)
+            if rect:
+                extra["nofit0"] = ",".join(map(str, rect))
+            r = run_ref(p, extra)
+            fits.append(dict(bstop=bstop, border=border, rect=rect, fqs=float(r["fqs"][0]),
+                             cnt=int(r["cnt"][0]), nancnt=int(r["nancnt"][0])))
+            arrays["fitfqs_b%d_r%d_%s" % (bstop, border, "rect" if rect else "none")] = r["fitfqs"]
+    arrays["fitdata"] = data
+    np.savez_compressed(os.path.join(HERE, "cfg4_bct.npz"), **arrays)
+    manifest["fixtures"]["cfg4_bct.npz"] = {"params": sets, "fit_params": p, "fits": fits,
+                                            "fitdata": "uniform(0.01,10) seed 5678, 32x32"}
+
+    with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

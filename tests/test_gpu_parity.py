@@ -1,0 +1,200 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the reference's golden
+fixtures and the CPU oracle.  REF64 must agree to FP64 rounding (device libm differs from
+glibc by <= 1 ulp); MIXED to 1e-4 relative (the benchmark tolerance)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import CFG1, CFG3, fixture, manifest, params, rel_err
+
+from crystalscatter_amd import CsxError, SasCalc, _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _calc(p, prec, ignore_cbs=True):
+    e = SasCalc(device=0, params=p)
+    e.set("ignore_cbs", 1 if ignore_cbs else 0)
+    img = e.doCalculation(prec)
+    return e, img
+
+
+def _report(tag, r):
+    print("%s: max rel %.3e  rms rel %.3e" % (tag, np.nanmax(r), np.sqrt(np.nanmean(r * r))), flush=True)
+
+
+@pytest.mark.parametrize("centerbeam", [0, 1])
+def test_qgrid_bit_exact(oracle_lib, centerbeam):
+    import torch
+    p = manifest()["fixtures"]["vitess_sphere.npz"]["params"] if centerbeam else dict(CFG1)
+    e = SasCalc(device=0, params=p)
+    e.prepare()
+    x0, x1, y0, y1 = e.bounds()
+    n = (x1 - x0) * (y1 - y0)
+    d = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+    g = _lib.Grid(x0, x1, y0, y1)
+    L = _lib.lib()
+    assert L.csx_qgrid(L.csx_engine_ctx(e._h), ctypes.byref(g), ctypes.c_void_p(d.data_ptr())) == 0
+    q = d.cpu().numpy().reshape(-1, 3)
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    W = x1 - x0
+    k = np.arange(n)
+    ref = np.array([o.idx2q(j % W + x0, j // W + y0) for j in k])
+    if centerbeam:
+        # detector geometry goes through atan2/sin/cos (device libm within an ulp or two of
+        # glibc) and 1-cos(theta) for qz: compare against |q| of the pixel
+        qn = np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+        assert (np.abs(q - ref).max(axis=1) / qn).max() < 1e-14
+    else:
+        np.testing.assert_array_equal(q, ref)
+
+
+@pytest.mark.parametrize("prec", ["ref64", "mixed"])
+def test_cfg1_sphere_fp64(prec):
+    f = fixture("cfg1_sphere.npz")
+    e, img = _calc(params("cfg1_sphere.npz"), prec)
+    assert tuple(e.bounds()) == tuple(int(b) for b in f["bounds"])
+    r = rel_err(img, f["image"])
+    _report("cfg1 " + prec, r)
+    assert r.max() < 1e-13
+
+
+def test_vitess_golden_geometry():
+    f = fixture("vitess_sphere.npz")
+    p = manifest()["fixtures"]["vitess_sphere.npz"]["params"]
+    e, img = _calc(p, "ref64")
+    r = rel_err(img, f["image"])
+    _report("vitess golden", r)
+    assert np.median(r) < 1e-14 and r.max() < 5e-3 and (r > 1e-12).sum() <= 11499
+
+
+@pytest.mark.parametrize("prec", ["ref64", "mixed"])
+def test_cfg2_reference_mode_exact(prec):
+    f = fixture("cfg2_cylgauss.npz")
+    e, img = _calc(params("cfg2_cylgauss.npz"), prec)
+    np.testing.assert_array_equal(img, f["image"])
+
+
+@pytest.mark.parametrize("prec,tol", [("ref64", 1e-13), ("mixed", 1e-4)])
+def test_cfg3_fcc_small_grid(prec, tol):
+    f = fixture("cfg3_fcc_g32.npz")
+    e, img = _calc(params("cfg3_fcc_g32.npz"), prec)
+    r = rel_err(img, f["image"])
+    _report("cfg3 64x64 " + prec, r)
+    assert r.max() < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("ref64", 1e-12), ("mixed", 1e-4)])
+def test_cfg3_fcc_full_size(prec, tol):
+    f = fixture("cfg3_fcc_1024_samples.npz")
+    e, img = _calc(CFG3, prec)
+    assert img.size == 1024 * 1024
+    r = rel_err(img[f["idx"]], f["values"])
+    _report("cfg3 1024^2 sampled " + prec, r)
+    assert r.max() < tol
+    # size-independent invariant: I(-q) = I(q).  FP64: Friedel partners are summed in the
+    # same order up to rounding.  Mixed: mirrored pixels sit in different waves, whose pair
+    # windows drop different terms below 2^-40 of a pair weight, so the two sides agree to
+    # the dropped mass, far below the 1e-4 bound.
+    a = img.reshape(1024, 1024)[1:, 1:]
+    assert rel_err(a, a[::-1, ::-1]).max() < (1e-13 if prec == "ref64" else 1e-8)
+
+
+def test_cfg3_full_size_against_oracle_random_pixels(oracle_lib):
+    e, img = _calc(CFG3, "mixed")
+    o = oracle_lib.OracleEngine(CFG3)
+    o.prepare()
+    rng = np.random.default_rng(7)
+    idx = rng.choice(img.size, 6000, replace=False)
+    ref = o.image_points(idx % 1024 - 512, idx // 1024 - 512, nthreads=16)
+    r = rel_err(img[idx], ref)
+    _report("cfg3 1024^2 vs oracle", r)
+    assert r.max() < 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [("ref64", 1e-11), ("mixed", 1e-4)])
+def test_cfg4_bct_sets(prec, tol):
+    f = fixture("cfg4_bct.npz")
+    worst = 0.0
+    for k, p in enumerate(manifest()["fixtures"]["cfg4_bct.npz"]["params"]):
+        e, img = _calc(p, prec)
+        r = rel_err(img, f["image_%d" % k])
+        worst = max(worst, r.max())
+    print("cfg4 %s worst rel %.3e" % (prec, worst))
+    assert worst < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("ref64", 1e-12), ("mixed", 1e-6)])
+def test_cfg4_fit_objective_and_masks(prec, tol):
+    import torch
+    f = fixture("cfg4_bct.npz")
+    m = manifest()["fixtures"]["cfg4_bct.npz"]
+    e = SasCalc(device=0, params=m["fit_params"])
+    e.setFitData(32, 32, f["fitdata"])
+    for fit in m["fits"]:
+        for i in range(4):
+            e.setNoFitRect(i, -1, -1, -1, -1)
+        if fit["rect"]:
+            e.setNoFitRect(0, *fit["rect"])
+        s, c, n = e.doFitCalculation(fit["bstop"], fit["border"], prec)
+        assert abs(s / fit["fqs"] - 1) < tol, (fit, s)
+        assert c == fit["cnt"] and n == fit["nancnt"]
+
+
+def test_batched_objective_equals_sequential():
+    import torch
+    from crystalscatter_amd import Context
+    m = manifest()["fixtures"]["cfg4_bct.npz"]
+    f = fixture("cfg4_bct.npz")
+    seq = []
+    sets = []
+    data = torch.tensor(f["fitdata"], dtype=torch.float64, device="cuda")
+    for p in m["params"]:
+        e = SasCalc(device=0, params=p)
+        e.setFitData(32, 32, f["fitdata"])
+        s, c, n = e.doFitCalculation(4, 0, "mixed")
+        seq.append(s)
+        h = SasCalc(device=-1, params=p)
+        h.set("ignore_cbs", 1)
+        h.prepare()
+        sets.append(h)
+    ctx = Context(0)
+    L = _lib.lib()
+    assert L.csx_fit_batch_begin(ctx._h) == 0
+    mask = _lib.Mask()
+    for i in range(16):
+        mask.rects[i] = -1
+    mask.bstop, mask.border = 4, 0
+    keep = []
+    for h in sets:
+        carr = np.concatenate([h.table(n) for n in ("carr1p", "carr2p", "carr3p", "carr4p", "carr5p", "carr6p", "carr7p",
+                                                    "carr8p", "carr9p", "carr1f", "carr2f", "carr3f", "carr4f", "carr5f",
+                                                    "carr6f", "carr7f", "carr8f", "carr9f")])
+        l1, l2, l3 = h.table("latpar1"), h.table("latpar2"), h.table("latpar3")
+        keep += [carr, l1, l2, l3]
+        rc = L.csx_fit_batch_stage(ctx._h, ctypes.byref(h.scalars()), carr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   l1.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   l2.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   l3.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 32, 32, ctypes.byref(mask),
+                                   ctypes.c_void_p(data.data_ptr()))
+        assert rc == 0
+    sums = np.zeros(len(sets))
+    cnts = np.zeros(len(sets), dtype=np.int64)
+    assert L.csx_fit_batch_run(ctx._h, 1, sums.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               cnts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))) == 0
+    np.testing.assert_array_equal(sums, np.array(seq))
+
+
+def test_error_paths():
+    from crystalscatter_amd import Context
+    ctx = Context(0)
+    L = _lib.lib()
+    g = _lib.Grid(0, 8, 0, 8)
+    assert L.csx_render(ctx._h, ctypes.byref(g), 1, ctypes.c_void_p(1)) == -4   # before upload
+    e = SasCalc(device=0, params=dict(CFG1))
+    with pytest.raises(ValueError):
+        e.doCalculation("fp16")
+    with pytest.raises(CsxError):
+        SasCalc(device=0, params=dict(CFG1, uca=0.001)).doCalculation()

@@ -1,0 +1,75 @@
+"""Product host preparation (libcsx.so, host-only engine: no GPU needed) against the
+reference's golden tables, and the parameter interface."""
+import numpy as np
+import pytest
+
+from helpers import fixture, manifest, params
+
+from crystalscatter_amd import CsxError, SasCalc
+
+TABLES = ["carr1p", "carr2p", "carr4p", "carr1f", "carr4f", "limq"]
+
+
+def _host(p, ignore_cbs=True):
+    e = SasCalc(device=-1, params=p)
+    e.set("ignore_cbs", 1 if ignore_cbs else 0)
+    e.prepare()
+    return e
+
+
+@pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz"])
+def test_tables_match_reference(name):
+    f = fixture(name)
+    e = _host(params(name))
+    for t in TABLES:
+        np.testing.assert_array_equal(e.table(t), f[t], err_msg=t)
+    s = e.scalars()
+    assert s.norm == f["norm"][0] and s.orcase == f["orcase"][0]
+    assert s.cosphic == f["cosphic"][0] and s.sinphic == f["sinphic"][0]
+    if "latpar3" in f:
+        assert (s.peakmax1, s.peakmax2) == tuple(f["peakmax"])
+        np.testing.assert_array_equal(e.table("latpar1")[6:], f["latpar1"][6:])
+        np.testing.assert_array_equal(e.table("latpar2")[6:], f["latpar2"][6:])
+        np.testing.assert_array_equal(e.table("latpar3").reshape(-1, 17)[1:, 1:14], f["latpar3"][1:, 1:14])
+        assert s.sphno_cubevol == f["ucvol"][0] * 4
+    if name == "cfg2_cylgauss.npz":
+        np.testing.assert_array_equal(e.table("carr11pm").reshape(131, 131)[:121, :121], f["carr11pm"])
+
+
+def test_bct_sets_match_reference():
+    f = fixture("cfg4_bct.npz")
+    for k, p in enumerate(manifest()["fixtures"]["cfg4_bct.npz"]["params"]):
+        e = _host(p)
+        for t in TABLES:
+            np.testing.assert_array_equal(e.table(t), f["%s_%d" % (t, k)], err_msg="%s/%d" % (t, k))
+        np.testing.assert_array_equal(e.table("latpar3").reshape(-1, 17)[1:, 1:14], f["latpar3_%d" % k][1:, 1:14])
+        assert e.scalars().sphno_cubevol == f["ucvol_%d" % k][0] * 2
+
+
+def test_key_table_defaults_and_unknown_key():
+    e = SasCalc(device=-1)
+    # defaults of the reference key table (Source/sc_calc_generic.cpp:36-138)
+    assert e.get("uca") == 30 and e.get("EditAzi") == 180 and e.get("I0") == 10000
+    assert e.get("GridPoints") == 64 and e.get("HKLmax") == 5 and e.get("SigmaL") == 0.06
+    with pytest.raises(KeyError):
+        e.set("NoSuchKey", 1)
+
+
+def test_parameter_checks_like_the_reference():
+    p = params("cfg3_fcc_g32.npz")
+    e = SasCalc(device=-1, params=dict(p, uca=0.005))
+    with pytest.raises(CsxError, match="uca<=0.01"):
+        e.prepare()
+    # cylinder + BCT is rejected by isCbsValid unless the check is skipped
+    p4 = manifest()["fixtures"]["cfg4_bct.npz"]["params"][0]
+    e = SasCalc(device=-1, params=p4)
+    with pytest.raises(CsxError, match="INVALID CBParticle"):
+        e.prepare()
+    e.set("ignore_cbs", 1)
+    e.prepare()
+
+
+def test_unsupported_combination_is_an_error():
+    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=4, Ordis=7))  # cube
+    with pytest.raises(CsxError):
+        e.prepare()
