@@ -29,20 +29,46 @@ CFG3 = dict(LType=5, ComboBoxParticle=0, Ordis=0, ComboBoxPeak=1, ComboBoxInteri
             EditSigma=0.08, uca=30, ucb=30, ucc=30, EditDomainSize=200, RadButDebyeScherrer=1,
             EditDebyeWaller=1.5, EditAzi=5, HKLmax=5, GridPoints=512, EditQmax=1, I0=1, Base=0)
 
-# Roofline accounting of the dominant kernel (the mixed-precision image kernel).  Its hot
-# loop evaluates Friedel pairs: one pair evaluation covers reflections +hkl and -hkl for one
-# pixel.  FP32 operations per pair evaluation in the in-plane single-term loop, FMA = 2:
-# q^.u 3, |q^ x u|^2 5, acos^2/sin^2 polynomial 18, y^2 1, radial exponent 4, angular
-# exponent 2, exp2 1, weighting 2 = 36 FLOP (+ a select).  The pair count per launch comes
-# from the kernel's own counters (csx_render_stats): pairs outside a wave's |q| and angle
-# window are never evaluated and are not counted.  The per-pixel FP64 form factors and
-# the rare FP64 re-sums are left out of the count (it is a lower bound).
-FLOP_PER_PAIR_EVAL = 36
-# The reference evaluates every reflection at every pixel: per reflection ~29 FLOP + 3
-# libm calls (Source/sc_gpu_pSph_pGau_lFCC_oGau.h:85-142).  Reported beside the roofline as
-# the reference-equivalent rate.
-REF_FLOP_PER_REFL = 29
+# Roofline accounting of the dominant kernel, k_image<K_FCC, mixed> (with its radial
+# pre-pass k_radial, bracketed by the same events).
+#
+# roofline.achieved counts the work the kernel's algorithm executes in its FP32 pair loop:
+# one pair evaluation covers reflections +hkl and -hkl for one pixel, at 40 FP32 FLOP
+# (FMA = 2): q^.u 3, |q^ x u|^2 5, acos^2/sin^2 polynomial 19, y^2 1, radial exponent 4,
+# first angular exponent 2, second exponent 3, sum of the two exp2 1, weighting 2; plus
+# 3 transcendentals (2 exp2, 1 sqrt).  The number of pair evaluations per launch comes
+# from the kernel's own counters (csx_render_stats).  The per-pixel FP64 work (q, sphere
+# factors, S(q)) is not counted, so achieved is a lower bound.
+EXEC_FLOP_PER_PAIR_EVAL, EXEC_T_PER_PAIR_EVAL = 40, 3
+#
+# Reference-equivalent rate (reported beside it): the reference formula's operations per
+# pixel (add/sub/mul/div = 1 FLOP; exp, log, sin, cos, atan, acos, sqrt = 1 transcendental,
+# pow = log + exp = 2), closed-form branch of the sphere factors, iso = 0:
+#   per anisotropic reflection (Source/sc_gpu_pSph_pGau_lFCC_oGau.h:103-125): cosine 7,
+#     radial Gaussian 10, angular factor 9, accumulation 3 = 29 FLOP; acos + 2 exp = 3 T
+#   per pixel outside the loop: q 10 FLOP + 1 T, sphere P(q) 65 + 12 T
+#     (Source/sc_lib_formpq_partSphere.h:58-74), <F>^2 31 + 9 T
+#     (Source/sc_lib_formfq_partSphere.h:8-52), width, S(q) and the affine map 36 + 2 T
+# = 340 * 29 + 142 = 10002 FLOP and 340 * 3 + 24 = 1044 T per pixel.  The kernel skips
+# reflections whose terms are below 2^-40 of their weight, so this rate can exceed the
+# FP32 peak: it measures the speed-up over the reference algorithm, not ALU use.
+ALG_FLOP_PER_REFL, ALG_T_PER_REFL = 29, 3
+ALG_FLOP_PER_PX_EXTRA, ALG_T_PER_PX_EXTRA = 142, 24
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector, dense (packed dual-issue counted)
+PEAK_TRANS_TOPS = 9.83        # 256 CU x 64 lanes x 2.4 GHz / 4 (quarter-rate transcendentals)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fcc1024_mixed.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(threads):
@@ -67,6 +93,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "ref64"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("CSX_BENCH_CPU_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, copy 256 MiB device to device once (a plain copy of "
+                         "known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pixels shard across GPUs "
@@ -92,6 +121,7 @@ def main():
     nrefl = s.peakmax2
     prec = {"mixed": 1, "ref64": 0}[args.precision]
 
+    assert L.csx_set_timing(ctx, 0) == 0   # no library events inside the timed region
     for _ in range(args.warmup):
         eng.doCalculationDevice(out.data_ptr(), args.precision)
     torch.cuda.synchronize()
@@ -105,8 +135,25 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
-    kernel_ms = ev0.elapsed_time(ev1) / args.steps   # HIP events on the launch stream
-    last_launch_ms = eng.last_kernel_ms()
+    region_ms = ev0.elapsed_time(ev1) / args.steps   # back-to-back launches, HIP events on the stream
+
+    # Average duration of one launch (roofline): the same K launches, each bracketed by its
+    # own pair of HIP events on the launch stream.
+    pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+             for _ in range(args.steps)]
+    for a, b in pairs:
+        a.record(stream)
+        eng.doCalculationDevice(out.data_ptr(), args.precision)
+        b.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = sum(a.elapsed_time(b) for a, b in pairs) / args.steps
+
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # Work counters of one launch (separate, untimed run of the same kernel).
     stats = (ctypes.c_uint64 * 4)()
@@ -130,9 +177,12 @@ def main():
         max_rel = "unavailable: %s" % exc
 
     px = 1024 * 1024
-    flops = pair_evals * FLOP_PER_PAIR_EVAL
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
-    ref_equiv = px * nrefl * REF_FLOP_PER_REFL / (kernel_ms * 1e-3) / 1e12
+    sec = kernel_ms * 1e-3
+    alg_flop = px * (nrefl * ALG_FLOP_PER_REFL + ALG_FLOP_PER_PX_EXTRA)
+    alg_t = px * (nrefl * ALG_T_PER_REFL + ALG_T_PER_PX_EXTRA)
+    exec_tflops = pair_evals * EXEC_FLOP_PER_PAIR_EVAL / sec / 1e12
+    exec_tops = pair_evals * EXEC_T_PER_PAIR_EVAL / sec / 1e12
+    achieved = exec_tflops
     line = {
         "metric": METRIC,
         "value": args.steps / wall,
@@ -153,15 +203,17 @@ def main():
         "max_rel_err_vs_fp64": max_rel,
         "rms_rel_err_vs_fp64": rms_rel,
         "kernel_ms": kernel_ms,
-        "last_launch_ms": last_launch_ms,
+        "region_ms_per_launch": region_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_image<K_FCC, CSX_PREC_MIXED>",
-                     "pair_evals_per_launch": pair_evals, "flop_per_pair_eval": FLOP_PER_PAIR_EVAL,
-                     "fp64_guard_pixels": guard_px, "near_one_pixels": near_one_px,
-                     "two_term_waves": two_waves,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
+                     "kernel": "k_radial<K_FCC, CSX_PREC_MIXED> + k_image<K_FCC, CSX_PREC_MIXED>",
+                     "pair_evals_per_launch": pair_evals, "flop_per_pair_eval": EXEC_FLOP_PER_PAIR_EVAL,
                      "pair_evals_uncut": px * ((nrefl + 1) // 2),
-                     "reference_equivalent_tflops": ref_equiv},
+                     "transcendental_rate_tops": exec_tops, "transcendental_peak_tops": PEAK_TRANS_TOPS,
+                     "reference_equivalent_tflops": alg_flop / sec / 1e12,
+                     "reference_equivalent_transcendental_tops": alg_t / sec / 1e12,
+                     "fp64_guard_pixels": guard_px, "near_one_pixels": near_one_px,
+                     "two_term_waves": two_waves},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads)

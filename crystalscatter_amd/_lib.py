@@ -70,6 +70,7 @@ SIGNATURES = [
     ("csx_fit_batch_run", _I, [_VP, _I, _PD, _PI64]),
     ("csx_last_error", _I, [_VP, _CP, _I]),
     ("csx_last_kernel_ms", _D, [_VP]),
+    ("csx_set_timing", _I, [_VP, _I]),
     ("csx_engine_new", _VP, [_I]),
     ("csx_engine_free", None, [_VP]),
     ("csx_engine_set", _I, [_VP, _CP, _D]),

@@ -306,11 +306,11 @@ __device__ double iso_mixed(const DevSet& d, double q) {
 // ---------------------------------------------------------------------------
 template <int KIND, int PREC>
 __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz,
-                              unsigned long long* stats = nullptr) {
+                              unsigned long long* stats = nullptr, const double2* ff = nullptr) {
     const csx_scalars& s = d.s;
     const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
     if constexpr (KIND == K_SPHERE) {
-        const double pq = pq_sphere(d, q);
+        const double pq = ff ? ff->x : pq_sphere(d, q);
         return s.base + s.i0 * (pq + s.iso * pq) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
     } else if constexpr (KIND == K_CYL_ZDIR) {
         double pq;
@@ -335,7 +335,10 @@ __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz,
     } else {
         double pq, fq;
         if constexpr (KIND == K_FCC) {
-            if constexpr (PREC == CSX_PREC_MIXED) {
+            if (ff) {
+                pq = ff->x;
+                fq = ff->y;
+            } else if constexpr (PREC == CSX_PREC_MIXED) {
                 pq = pq_sphere_fast(d, q);
                 fq = fq_sphere_fast(d, q);
             } else {
@@ -402,10 +405,47 @@ __device__ __forceinline__ void tile_pixel(int& px, int& py) {
     py = (w >> 1) * 8 + (l >> 3);
 }
 
-// Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.
+// Kinds whose form factors depend on |q| only.
+constexpr bool radial_kind(int kind) { return kind == K_SPHERE || kind == K_FCC; }
+
+// Index of (a, b), 0 <= a <= b, in the triangle table of radial values.
+__device__ __forceinline__ int tri_index(int a, int b) { return b * (b + 1) / 2 + a; }
+
+// Radial pre-pass (midpoint q grid): in that grid |q| of pixel (ihex, i) depends only on
+// {|i|, |ihex|} and is bitwise the same for all eight sign/swap images, so P(q) and <F>^2
+// are evaluated once per unordered pair and shared.  Entry (a, b) uses pixel (ihex=b, i=a).
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, double2* __restrict__ tab) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    const int total = (m + 1) * (m + 2) / 2;
+    if (n >= total) return;
+    int b = (int)((sqrt(8.0 * n + 1.0) - 1.0) * 0.5);
+    while (b * (b + 1) / 2 > n) --b;
+    while ((b + 1) * (b + 2) / 2 <= n) ++b;
+    const int a = n - b * (b + 1) / 2;
+    double qx, qy, qz;
+    pixel_q(d.s, b, a, qx, qy, qz);
+    const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
+    double2 r;
+    if constexpr (KIND == K_SPHERE) {
+        r.x = pq_sphere(d, q);
+        r.y = 0.0;
+    } else if constexpr (PREC == CSX_PREC_MIXED) {
+        r.x = pq_sphere_fast(d, q);
+        r.y = fq_sphere_fast(d, q);
+    } else {
+        r.x = pq_sphere(d, q);
+        r.y = fq_sphere(d, q);
+    }
+    tab[n] = r;
+}
+
+// Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.  rtab: radial
+// table of k_radial (midpoint grid, radial kinds) or null.
 template <int KIND, int PREC>
 __global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out,
-                                                  unsigned long long* __restrict__ stats) {
+                                                  unsigned long long* __restrict__ stats,
+                                                  const double2* __restrict__ rtab) {
     // Every lane computes (the mixed path needs all 64 lanes of a wave); lanes past the grid
     // edge evaluate the last valid pixel and do not store.
     int px, py;
@@ -416,7 +456,12 @@ __global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid
     const int ihex = min(ihex0, g.xmax - 1), i = min(i0, g.ymax - 1);
     double qx, qy, qz;
     pixel_q(d.s, ihex, i, qx, qy, qz);
-    const double v = pixel_value<KIND, PREC>(d, qx, qy, qz, stats);
+    double2 ff;
+    if (rtab) {
+        const int a = abs(i), b = abs(ihex);
+        ff = rtab[a <= b ? tri_index(a, b) : tri_index(b, a)];
+    }
+    const double v = pixel_value<KIND, PREC>(d, qx, qy, qz, stats, rtab ? &ff : nullptr);
     if (valid) {
         const size_t W = (size_t)(g.xmax - g.xmin);
         out[(size_t)(ihex - g.xmin) + W * (size_t)(i - g.ymin)] = v;
@@ -706,10 +751,11 @@ struct csx_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     bool timing_pending = false;
+    bool timing = false;  // bracket csx_render launches with events (costs a drain per launch)
     std::string err;
     bool uploaded = false;
     DevSet set{};
-    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, lut, singles, iso, tiles, fqs_tmp, stats;
+    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, lut, singles, iso, tiles, fqs_tmp, stats, radial;
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
@@ -733,9 +779,18 @@ static int fail(csx_ctx* c, int code, const std::string& msg) {
 
 template <int K, int P>
 static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
-                         hipStream_t st) {
+                         double2* rtab, hipStream_t st) {
+    // Radial pre-pass for kinds whose form factors depend on |q| only, in the midpoint grid.
+    if (rtab && radial_kind(K) && !d.s.centerbeam) {
+        const int m = std::max(std::max(std::abs(g.xmin), std::abs(g.xmax - 1)),
+                               std::max(std::abs(g.ymin), std::abs(g.ymax - 1)));
+        const int total = (m + 1) * (m + 2) / 2;
+        hipLaunchKernelGGL((k_radial<K, P>), dim3((total + 255) / 256), dim3(256), 0, st, d, m, rtab);
+    } else {
+        rtab = nullptr;
+    }
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
-    hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats);
+    hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
 }
 
 int csx_qgrid(csx_ctx* c, const csx_grid* g, double* d_q3) {
@@ -764,6 +819,18 @@ template <class T>
 static int put_table(csx_ctx* c, DevBuf& b, const std::vector<T>& v) {
     HIPCHK(c, b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T)));
     if (!v.empty()) HIPCHK(c, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return CSX_OK;
+}
+
+// Radial table of a render (midpoint grid, radial kinds), sized for the grid; null otherwise.
+static int radial_table(csx_ctx* c, const csx_grid& g, double2** out) {
+    *out = nullptr;
+    if (!radial_kind(kind_of(c->set.s.combo_key)) || c->set.s.centerbeam) return CSX_OK;
+    const long long m = std::max(std::max(std::llabs(g.xmin), std::llabs((long long)g.xmax - 1)),
+                                 std::max(std::llabs(g.ymin), std::llabs((long long)g.ymax - 1)));
+    if (m > 46000) return CSX_OK;  // triangle index would overflow int: evaluate per pixel
+    HIPCHK(c, c->radial.ensure((size_t)((m + 1) * (m + 2) / 2) * sizeof(double2)));
+    *out = (double2*)c->radial.p;
     return CSX_OK;
 }
 
@@ -801,7 +868,7 @@ void csx_destroy(csx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->lut, &c->singles,
-                      &c->iso, &c->stats, &c->tiles,
+                      &c->iso, &c->stats, &c->radial, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
                       &c->bsums_d, &c->bcnts_d})
         b->release();
@@ -832,6 +899,12 @@ int csx_synchronize(csx_ctx* c) {
 int csx_last_error(csx_ctx* c, char* buf, int len) {
     if (!c || !buf || len <= 0) return CSX_E_ARG;
     std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
+    return CSX_OK;
+}
+
+int csx_set_timing(csx_ctx* c, int enable) {
+    if (!c) return CSX_E_ARG;
+    c->timing = enable != 0;
     return CSX_OK;
 }
 
@@ -918,11 +991,15 @@ int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
         return fail(c, CSX_E_ARG, "orientation-averaged cylinders in physics mode are not available");
     HIPCHK(c, hipSetDevice(c->device));
     const int kind = kind_of(c->set.s.combo_key);
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, nullptr, c->stream);
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    double2* rtab = nullptr;
+    if (int rc = radial_table(c, *g, &rtab)) return rc;
+    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, nullptr, rtab, c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    c->timing_pending = true;  // asynchronous: csx_last_kernel_ms waits for the events
+    if (c->timing) {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        c->timing_pending = true;  // asynchronous: csx_last_kernel_ms waits for the events
+    }
     return CSX_OK;
 }
 
@@ -938,7 +1015,9 @@ int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out
     const std::vector<uint64_t> zero(CSX_NSTATS, 0);
     HIPCHK(c, hipMemcpyAsync(c->stats.p, zero.data(), CSX_NSTATS * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
     const int kind = kind_of(c->set.s.combo_key);
-    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, (unsigned long long*)c->stats.p, c->stream);
+    double2* rtab = nullptr;
+    if (int rc = radial_table(c, *g, &rtab)) return rc;
+    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, (unsigned long long*)c->stats.p, rtab, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(h_stats, c->stats.p, CSX_NSTATS * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

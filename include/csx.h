@@ -147,9 +147,13 @@ int csx_fit_batch_stage(csx_ctx* ctx, const csx_scalars* s, const double* h_carr
 int csx_fit_batch_run(csx_ctx* ctx, int precision, double* h_sums, int64_t* h_cnts);
 
 int csx_last_error(csx_ctx* ctx, char* buf, int len);
-/* Kernel time of the last csx_render / csx_fit_fqs / csx_fit_batch_run, measured with HIP
- * events on the context's stream (milliseconds); waits for a pending render. */
+/* Kernel time of the last csx_fit_fqs / csx_fit_batch_run, and of the last csx_render when
+ * timing is on, measured with HIP events on the context's stream (milliseconds); waits for
+ * a pending render.  Stands in for higResTimerElapsedCalc (Source/sc_calc_generic_gpu.h:376). */
 double csx_last_kernel_ms(csx_ctx* ctx);
+/* Bracket every csx_render launch with HIP events (default off: the event pair keeps
+ * back-to-back launches from overlapping, about 10 us per 1024^2 render). */
+int csx_set_timing(csx_ctx* ctx, int enable);
 
 /* ---- SasCalc_GENERIC_calculation facade ------------------------------------------- */
 typedef struct csx_engine csx_engine;
