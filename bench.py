@@ -50,7 +50,7 @@ EXEC_FLOP_PER_PAIR_EVAL, EXEC_T_PER_PAIR_EVAL = 40, 3
 #     (Source/sc_lib_formpq_partSphere.h:58-74), <F>^2 31 + 9 T
 #     (Source/sc_lib_formfq_partSphere.h:8-52), width, S(q) and the affine map 36 + 2 T
 # = 340 * 29 + 142 = 10002 FLOP and 340 * 3 + 24 = 1044 T per pixel.  The kernel skips
-# reflections whose terms are below 2^-40 of their weight, so this rate can exceed the
+# reflections whose terms are below 2^-32 of their weight, so this rate can exceed the
 # FP32 peak: it measures the speed-up over the reference algorithm, not ALU use.
 ALG_FLOP_PER_REFL, ALG_T_PER_REFL = 29, 3
 ALG_FLOP_PER_PX_EXTRA, ALG_T_PER_PX_EXTRA = 142, 24

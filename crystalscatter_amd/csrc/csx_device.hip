@@ -132,6 +132,19 @@ struct PixelConsts {
     float c1, c2;          // second Friedel term: e2 = e1 + c1 + c2 * y
 };
 
+// A pixel's position and, on demand, its exact FP64 q vector (convert_idx2q).
+struct PixQ {
+    int ihex, i;
+    double qx, qy, qz;
+    bool have;
+};
+__device__ __forceinline__ void need_q(const csx_scalars& s, PixQ& p) {
+    if (!p.have) {
+        pixel_q(s, p.ihex, p.i, p.qx, p.qy, p.qz);
+        p.have = true;
+    }
+}
+
 // Sum over pairs [p0, p1) (p0, p1 multiples of 4, wave-uniform) of A*(2^e1 [+ 2^e2]).
 // y^2 = sin^2(y) * acos^2(c)/(1-c^2), with sin^2 y = |q^ x u|^2 taken from the cross
 // product: accurate where the Gaussian is narrow (y small), where 1-|q^.u| is not.
@@ -176,8 +189,9 @@ __device__ __forceinline__ void pair_sum(const PairRec* __restrict__ pr, int p0,
 
 // FP64 terms (reference formula) of the pairs that pair_sum left out, for this lane.
 template <bool PLANAR>
-__device__ double near_one_terms(const DevSet& d, int p0, int p1, const PixelConsts& L, double qx, double qy,
-                                 double qz, double q, double width, double x2phi) {
+__device__ double near_one_terms(const DevSet& d, int p0, int p1, const PixelConsts& L, const PixQ& P3, double q,
+                                 double width, double x2phi) {
+    const double qx = P3.qx, qy = P3.qy, qz = P3.qz;
     double add = 0.0;
     for (int p = p0; p < p1; ++p) {
         const PairRec P = d.pairs[p];
@@ -200,19 +214,27 @@ struct Window {
 // Pairs outside the window, and second Friedel terms where they cannot reach 2^cut, are
 // dropped: every dropped term is below 2^cut * A.  need_exact: unpaired reflections exist.
 template <bool PLANAR>
-__device__ double lattice_mixed(const DevSet& d, double qx, double qy, double qz, double q, double width,
-                                bool& need_exact, double& dropped, unsigned long long* stats) {
+__device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, double q, double width, bool& need_exact,
+                                double& dropped, unsigned long long* stats) {
     const csx_scalars& s = d.s;
     PixelConsts L;
-    const double rq = 1.0 / q;
-    L.hx = (float)(qx * rq);
-    L.hy = (float)(qy * rq);
-    L.hz = (float)(qz * rq);
+    if (P3.have) {
+        const double rq = 1.0 / q;
+        L.hx = (float)(P3.qx * rq);
+        L.hy = (float)(P3.qy * rq);
+        L.hz = (float)(P3.qz * rq);
+    } else {
+        // midpoint grid: q^ = (i, ihex) * (qmax/zmax) / |q| in FP32 (q_z = 1e-20 drops out)
+        const float c = (float)(s.qmax / s.zmax) * __builtin_amdgcn_rcpf((float)q);
+        L.hx = (float)P3.i * c;
+        L.hy = (float)P3.ihex * c;
+        L.hz = 0.0f;
+    }
     L.qf = (float)q;
     L.qlo = (float)(q - (double)L.qf);
-    const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));
-    const double krd = -4.0 / (kPiD * width * width) * kLog2e;
-    const double kpd = -x2phi * kLog2e;
+    const double kpd = d.kp_coef * q * q;
+    const double krd = s.para ? -4.0 / (kPiD * width * width) * kLog2e : d.kr_const;
+    const double inv_hpw = s.para ? 1.0 / (kPiD * width / 2.0) : d.inv_hpw;
     L.kr = (float)krd;
     L.kp = (float)kpd;
     L.c1 = (float)(kpd * kPiD * kPiD);
@@ -221,31 +243,40 @@ __device__ double lattice_mixed(const DevSet& d, double qx, double qy, double qz
     // Window edges are bounds, computed in FP32 with a relative margin of 1e-4.
     // radial window: |q - q_hkl| <= dq  <=>  radial exponent >= cut
     const float dq = __builtin_amdgcn_sqrtf(cut / L.kr) * 1.0001f + 1e-6f;
-    // angular window: y <= ymax  <=>  angular exponent >= cut (y <= pi/2 always)
+    // angular window: y <= ymax  <=>  angular exponent >= cut (y <= pi/2 always); only
+    // evaluated when some lane's angular factor can drop below 2^cut
     const float hpi = 1.5707963f;
-    const float ymax = (L.kp < 0.0f) ? fminf(__builtin_amdgcn_sqrtf(cut / L.kp) * 1.0001f + 1e-6f, hpi) : hpi;
-    float uz_lo = 0.0f, uz_hi;
-    if constexpr (PLANAR) {
-        uz_hi = (ymax < hpi) ? sinf(ymax) * 1.0001f + 1e-6f : 1.0f;
-    } else {
-        // angle(q^, +-u) >= | |asin q^_z| - |asin u_z| |
-        const float eh = asinf(fminf(fabsf(L.hz), 1.0f));
-        uz_lo = (eh - ymax > 0.0f) ? sinf(eh - ymax) * 0.9999f - 1e-6f : 0.0f;
-        uz_hi = (eh + ymax < hpi) ? sinf(eh + ymax) * 1.0001f + 1e-6f : 1.0f;
+    const bool narrow = L.kp * (hpi * hpi) < cut;
+    const bool any_narrow = __builtin_amdgcn_readfirstlane((int)(__ballot(narrow) != 0ull)) != 0;
+    float uz_lo = 0.0f, uz_hi = 1.0f;
+    if (any_narrow) {
+        const float ymax = narrow ? __builtin_amdgcn_sqrtf(cut / L.kp) * 1.0001f + 1e-6f : hpi;
+        if constexpr (PLANAR) {
+            uz_hi = (ymax < hpi) ? sinf(ymax) * 1.0001f + 1e-6f : 1.0f;
+        } else {
+            // angle(q^, +-u) >= | |asin q^_z| - |asin u_z| |
+            const float eh = asinf(fminf(fabsf(L.hz), 1.0f));
+            uz_lo = (eh - ymax > 0.0f) ? sinf(eh - ymax) * 0.9999f - 1e-6f : 0.0f;
+            uz_hi = (eh + ymax < hpi) ? sinf(eh + ymax) * 1.0001f + 1e-6f : 1.0f;
+        }
     }
     // second Friedel term: e2 <= kp*(pi/2)^2 for every pair
     const bool two = L.kp * (hpi * hpi) > cut;
     Window W;
     {
         const float lo = wave_min_f(L.qf - dq), hi = wave_max_f(L.qf + dq);
-        const float ulo = wave_min_f(uz_lo), uhi = wave_max_f(uz_hi);
+        const float ulo = any_narrow ? wave_min_f(uz_lo) : 0.0f;
+        const float uhi = any_narrow ? wave_max_f(uz_hi) : 1.0f;
         // Bound of this lane's dropped mass (sum of A * 2^e over dropped terms).  Pairs left
         // out by the |q| window lie below lo or above hi, and inside [qh_min, qh_max].
         double dmin = HUGE_VAL;
         if (d.qh_min < lo) dmin = fmin(dmin, q - fmin((double)lo, (double)d.qh_max));
         if (d.qh_max > hi) dmin = fmin(dmin, fmax((double)hi, (double)d.qh_min) - q);
         dmin = fmax(dmin, 0.0);
-        dropped = (dmin < HUGE_VAL) ? d.asum2 * exp2(krd * dmin * dmin) : 0.0;
+        // 2^max(e, -126) in FP32: an upper bound of 2^e, cheap, never denormal
+        dropped = (dmin < HUGE_VAL)
+                      ? d.asum2 * (double)__builtin_amdgcn_exp2f(fmaxf((float)(krd * dmin * dmin), -126.0f))
+                      : 0.0;
         W.b0 = uniform_i(max(0, min(d.nlut, (int)floorf(lo * d.lut_inv_dq))));
         W.b1 = uniform_i(max(0, min(d.nlut, (int)ceilf(hi * d.lut_inv_dq))));
         W.band0 = uniform_i(max(0, min(kBands - 1, (int)floorf(ulo * kBands))));
@@ -253,11 +284,12 @@ __device__ double lattice_mixed(const DevSet& d, double qx, double qy, double qz
     }
     const bool any_two = __builtin_amdgcn_readfirstlane((int)(__ballot(two) != 0ull)) != 0;
     if (W.band0 > 0 || W.band1 < kBands - 1) {
-        dropped += d.asum2 * exp2((double)cut);
+        dropped += d.asum2 * (double)__builtin_amdgcn_exp2f(cut);
     } else {
         W.band0 = W.band1 = kBands;  // every band: the merged list, in one pass
     }
-    if (!any_two) dropped += 0.5 * d.asum2 * exp2(kpd * 0.25 * kPiD * kPiD);
+    if (!any_two)
+        dropped += 0.5 * d.asum2 * (double)__builtin_amdgcn_exp2f(fmaxf(L.kp * (hpi * hpi), -126.0f));
     double acc = 0.0;
     bool flag = false;
     long long evals = 0;
@@ -269,12 +301,14 @@ __device__ double lattice_mixed(const DevSet& d, double qx, double qy, double qz
         else pair_sum<PLANAR, false>(d.pairs, p0, p1, L, acc, flag);
         evals += p1 - p0;
     }
-    double inten = acc / (kPiD * width / 2.0);
+    double inten = acc * inv_hpw;
     if (flag) {
+        need_q(s, P3);
+        const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));  // the reference's expression
         for (int band = W.band0; band <= W.band1; ++band) {
             const int32_t* lb = d.lut + band * (d.nlut + 1);
             const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
-            inten += near_one_terms<PLANAR>(d, p0, p1, L, qx, qy, qz, q, width, x2phi);
+            inten += near_one_terms<PLANAR>(d, p0, p1, L, P3, q, width, x2phi);
         }
     }
     if (stats && (threadIdx.x & 63) == 0) {
@@ -304,13 +338,18 @@ __device__ double iso_mixed(const DevSet& d, double q) {
 // ---------------------------------------------------------------------------
 // Pixel value of each combination
 // ---------------------------------------------------------------------------
+// P3: the pixel (its exact q vector is computed here unless the caller has it); rv: its
+// radial-table entry (midpoint grid, sphere kinds) or null.
 template <int KIND, int PREC>
-__device__ double pixel_value(const DevSet& d, double qx, double qy, double qz,
-                              unsigned long long* stats = nullptr, const double2* ff = nullptr) {
+__device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigned long long* stats = nullptr,
+                              const Radial* rv = nullptr) {
     const csx_scalars& s = d.s;
-    const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
+    // The mixed FCC path with a radial entry needs only |q| and the pixel indices.
+    if (!(rv && KIND == K_FCC && PREC == CSX_PREC_MIXED)) need_q(s, P3);
+    const double qx = P3.qx, qy = P3.qy, qz = P3.qz;
+    const double q = rv ? rv->q : sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
     if constexpr (KIND == K_SPHERE) {
-        const double pq = ff ? ff->x : pq_sphere(d, q);
+        const double pq = rv ? rv->pq : pq_sphere(d, q);
         return s.base + s.i0 * (pq + s.iso * pq) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
     } else if constexpr (KIND == K_CYL_ZDIR) {
         double pq;
@@ -335,9 +374,9 @@ __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz,
     } else {
         double pq, fq;
         if constexpr (KIND == K_FCC) {
-            if (ff) {
-                pq = ff->x;
-                fq = ff->y;
+            if (rv) {
+                pq = rv->pq;
+                fq = rv->fq;
             } else if constexpr (PREC == CSX_PREC_MIXED) {
                 pq = pq_sphere_fast(d, q);
                 fq = fq_sphere_fast(d, q);
@@ -363,32 +402,41 @@ __device__ double pixel_value(const DevSet& d, double qx, double qy, double qz,
         if (s.para) width = (4.0 / s.domainsize) + sqd(s.reldis) * s.dist * sqd(q);
         width *= s.width_zuf;
         double inten, radint;
-        const double dwe = exp(-s.dwfactor * q * q);
+        const double dwe = rv ? rv->dwe : exp(-s.dwfactor * q * q);
+        const double fqpq = fq / pq;
         if constexpr (PREC == CSX_PREC_REF64) {
             lattice_ref64(d, qx, qy, qz, q, width, inten, radint);
         } else {
             bool need_exact = false;
             double dropped = 0.0;
-            inten = s.centerbeam ? lattice_mixed<false>(d, qx, qy, qz, q, width, need_exact, dropped, stats)
-                                 : lattice_mixed<true>(d, qx, qy, qz, q, width, need_exact, dropped, stats);
+            inten = s.centerbeam ? lattice_mixed<false>(d, P3, q, width, need_exact, dropped, stats)
+                                 : lattice_mixed<true>(d, P3, q, width, need_exact, dropped, stats);
             radint = (s.iso != 0) ? iso_mixed(d, q) : 0.0;
             // Conditioning guard.  szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
             // The FP32 peak sum I carries up to ~1e-5 relative error plus this lane's dropped
             // mass; szq amplifies an error of I by |B|/|szq|.  Pixels where szq could move by
             // more than 3e-5 relative, and by more than 2^-60 absolute (below the rounding of
             // szq's own FP64 terms), are re-summed in FP64 in the reference's order.
-            const double bs = (fq / pq) * (2 * kPiD / s.sphno_cubevol) * dwe;
-            const double szq_est = 1 + (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe;
-            const double dsz = fabs(bs) * (1.0e-5 * fabs(inten) + dropped / (kPiD * width / 2.0));
+            const double bs = fqpq * d.two_pi_over_v * dwe;
+            const double szq_est = 1 + fqpq * (inten * d.two_pi_over_v - 1) * dwe;
+            const double dsz = fabs(bs) * (1.0e-5 * fabs(inten) + dropped * (s.para ? 1.0 / (kPiD * width / 2.0) : d.inv_hpw));
             if (need_exact || !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60))) {
-                lattice_ref64(d, qx, qy, qz, q, width, inten, radint);
+                need_q(s, P3);
+                lattice_ref64(d, P3.qx, P3.qy, P3.qz, q, width, inten, radint);
                 if (stats) atomicAdd(&stats[1], 1ull);
             }
         }
-        const double szqiso = (1 + (1) * (8 * kPiD * kPiD * kPiD * radint / (4 * kPiD * s.sphno_cubevol) - 1) * exp(-s.ucb * q * q));
-        const double szq = (1 + (fq / pq) * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
+        const double szq = (1 + fqpq * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
         const double pqiso = (KIND == K_FCC) ? pq : 1.0;
-        return s.base + s.i0 * (szq * pq + s.iso * szqiso * pqiso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+        // iso == 0 and ifluc == 0 make their terms exactly +0 (szqiso is finite): skipped.
+        double isot = 0.0, fluct = 0.0;
+        if (s.iso != 0) {
+            const double szqiso = (1 + (1) * (8 * kPiD * kPiD * kPiD * radint / (4 * kPiD * s.sphno_cubevol) - 1) *
+                                           exp(-s.ucb * q * q));
+            isot = s.iso * szqiso * pqiso;
+        }
+        if (s.ifluc != 0) fluct = s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+        return s.base + s.i0 * (szq * pq + isot) + fluct;
     }
 }
 
@@ -412,11 +460,16 @@ constexpr bool radial_kind(int kind) { return kind == K_SPHERE || kind == K_FCC;
 __device__ __forceinline__ int tri_index(int a, int b) { return b * (b + 1) / 2 + a; }
 
 // Radial pre-pass (midpoint q grid): in that grid |q| of pixel (ihex, i) depends only on
-// {|i|, |ihex|} and is bitwise the same for all eight sign/swap images, so P(q) and <F>^2
-// are evaluated once per unordered pair and shared.  Entry (a, b) uses pixel (ihex=b, i=a).
+// {|i|, |ihex|} and is bitwise the same for all eight sign/swap images, so |q|, P(q), <F>^2
+// and the Debye-Waller factor are evaluated once per unordered pair and shared.  Entry
+// (a, b) uses pixel (ihex=b, i=a).  The grid has two halves of `half` threads each (a
+// multiple of the wave size, so no wave mixes them): the first evaluates q, P(q) and dwe,
+// the second <F>^2 -- independent FP64 chains, twice the waves in flight.
 template <int KIND, int PREC>
-__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, double2* __restrict__ tab) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, int half, Radial* __restrict__ tab) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const bool second = g >= half;
+    const int n = second ? g - half : g;
     const int total = (m + 1) * (m + 2) / 2;
     if (n >= total) return;
     int b = (int)((sqrt(8.0 * n + 1.0) - 1.0) * 0.5);
@@ -426,18 +479,22 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, double2* 
     double qx, qy, qz;
     pixel_q(d.s, b, a, qx, qy, qz);
     const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
-    double2 r;
-    if constexpr (KIND == K_SPHERE) {
-        r.x = pq_sphere(d, q);
-        r.y = 0.0;
-    } else if constexpr (PREC == CSX_PREC_MIXED) {
-        r.x = pq_sphere_fast(d, q);
-        r.y = fq_sphere_fast(d, q);
+    double* e = reinterpret_cast<double*>(tab + n);
+    if (!second) {
+        double pq;
+        if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
+        else pq = pq_sphere_fast(d, q);
+        e[0] = q;
+        e[1] = pq;
+        e[3] = exp(-d.s.dwfactor * q * q);
     } else {
-        r.x = pq_sphere(d, q);
-        r.y = fq_sphere(d, q);
+        double fq = 0.0;
+        if constexpr (KIND == K_FCC) {
+            if constexpr (PREC == CSX_PREC_REF64) fq = fq_sphere(d, q);
+            else fq = fq_sphere_fast(d, q);
+        }
+        e[2] = fq;
     }
-    tab[n] = r;
 }
 
 // Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.  rtab: radial
@@ -445,26 +502,31 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, double2* 
 template <int KIND, int PREC>
 __global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out,
                                                   unsigned long long* __restrict__ stats,
-                                                  const double2* __restrict__ rtab) {
+                                                  const Radial* __restrict__ rtab) {
     // Every lane computes (the mixed path needs all 64 lanes of a wave); lanes past the grid
-    // edge evaluate the last valid pixel and do not store.
+    // edge evaluate the last valid pixel, and only in-grid pixels are stored.
     int px, py;
     tile_pixel(px, py);
     const int ihex0 = g.xmin + blockIdx.x * kTile + px;
     const int i0 = g.ymin + blockIdx.y * kTile + py;
-    const bool valid = ihex0 < g.xmax && i0 < g.ymax;
     const int ihex = min(ihex0, g.xmax - 1), i = min(i0, g.ymax - 1);
-    double qx, qy, qz;
-    pixel_q(d.s, ihex, i, qx, qy, qz);
-    double2 ff;
+    PixQ P3{ihex, i, 0.0, 0.0, 0.0, false};
+    const Radial* rv = nullptr;
     if (rtab) {
         const int a = abs(i), b = abs(ihex);
-        ff = rtab[a <= b ? tri_index(a, b) : tri_index(b, a)];
+        rv = rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
     }
-    const double v = pixel_value<KIND, PREC>(d, qx, qy, qz, stats, rtab ? &ff : nullptr);
-    if (valid) {
+    const double v = pixel_value<KIND, PREC>(d, P3, stats, rv);
+    // Stage the tile in LDS and store it row by row: each wave then writes four whole rows of
+    // 16 doubles (full 128-byte lines) instead of half lines from its 8x8 quadrant.
+    __shared__ double tile[kTile][kTile + 1];
+    tile[py][px] = v;
+    __syncthreads();
+    const int r = threadIdx.x / kTile, cx = threadIdx.x % kTile;
+    const int oh = g.xmin + blockIdx.x * kTile + cx, oi = g.ymin + blockIdx.y * kTile + r;
+    if (oh < g.xmax && oi < g.ymax) {
         const size_t W = (size_t)(g.xmax - g.xmin);
-        out[(size_t)(ihex - g.xmin) + W * (size_t)(i - g.ymin)] = v;
+        out[(size_t)(oh - g.xmin) + W * (size_t)(oi - g.ymin)] = tile[r][cx];
     }
 }
 
@@ -542,9 +604,8 @@ __device__ void fit_tile(const FitSet& f, int tile, double* __restrict__ fqs_out
     const csx_scalars& s = f.d.s;
     const bool masked = !inside || fit_masked(s, f.m, f.data, f.w, xc, yc);
     const int ihex = xc + s.iimin, i = yc + s.iimin;
-    double qx, qy, qz;
-    pixel_q(s, ihex, i, qx, qy, qz);
-    const double v = pixel_value<KIND, PREC>(f.d, qx, qy, qz);
+    PixQ P3{ihex, i, 0.0, 0.0, 0.0, false};
+    const double v = pixel_value<KIND, PREC>(f.d, P3);
     if (inside) {
         const size_t idx = (size_t)(ihex - s.xmin) + (size_t)(s.xmax - s.xmin) * (size_t)(i - s.ymin);
         if (!masked) {
@@ -627,9 +688,9 @@ struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
-// Base-2 exponent below which the mixed path drops a pair term (2^-40 ~ 9e-13 of the
+// Base-2 exponent below which the mixed path drops a pair term (2^-32 ~ 2.3e-10 of the
 // pair's weight); each pixel's bound of the dropped mass enters its conditioning guard.
-constexpr float kDropExponent = -40.0f;
+constexpr float kDropExponent = -32.0f;
 constexpr int kLutBins = 256;
 
 // Host-side fast tables of one parameter set: Friedel pairs in |u_z| bands sorted by
@@ -731,6 +792,14 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
 }
 
 void set_fast(DevSet& d, const FastTables& ft) {
+    const csx_scalars& s = d.s;
+    double width = 1;
+    if (s.dsch) width = 4.0 / s.domainsize;
+    width *= s.width_zuf;
+    d.kp_coef = -4.0 / (kPiD * s.phiwidth * s.phiwidth) * kLog2e;
+    d.kr_const = -4.0 / (kPiD * width * width) * kLog2e;
+    d.inv_hpw = 1.0 / (kPiD * width / 2.0);
+    d.two_pi_over_v = 2 * kPiD / s.sphno_cubevol;
     d.npairs = (int)ft.pairs.size();
     d.nsingle = (int)ft.singles.size();
     d.niso = (int)ft.iso.size();
@@ -779,13 +848,14 @@ static int fail(csx_ctx* c, int code, const std::string& msg) {
 
 template <int K, int P>
 static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
-                         double2* rtab, hipStream_t st) {
+                         Radial* rtab, hipStream_t st) {
     // Radial pre-pass for kinds whose form factors depend on |q| only, in the midpoint grid.
     if (rtab && radial_kind(K) && !d.s.centerbeam) {
         const int m = std::max(std::max(std::abs(g.xmin), std::abs(g.xmax - 1)),
                                std::max(std::abs(g.ymin), std::abs(g.ymax - 1)));
         const int total = (m + 1) * (m + 2) / 2;
-        hipLaunchKernelGGL((k_radial<K, P>), dim3((total + 255) / 256), dim3(256), 0, st, d, m, rtab);
+        const int half = (total + 255) / 256 * 256;
+        hipLaunchKernelGGL((k_radial<K, P>), dim3(2 * half / 256), dim3(256), 0, st, d, m, half, rtab);
     } else {
         rtab = nullptr;
     }
@@ -823,14 +893,14 @@ static int put_table(csx_ctx* c, DevBuf& b, const std::vector<T>& v) {
 }
 
 // Radial table of a render (midpoint grid, radial kinds), sized for the grid; null otherwise.
-static int radial_table(csx_ctx* c, const csx_grid& g, double2** out) {
+static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out) {
     *out = nullptr;
     if (!radial_kind(kind_of(c->set.s.combo_key)) || c->set.s.centerbeam) return CSX_OK;
     const long long m = std::max(std::max(std::llabs(g.xmin), std::llabs((long long)g.xmax - 1)),
                                  std::max(std::llabs(g.ymin), std::llabs((long long)g.ymax - 1)));
     if (m > 46000) return CSX_OK;  // triangle index would overflow int: evaluate per pixel
-    HIPCHK(c, c->radial.ensure((size_t)((m + 1) * (m + 2) / 2) * sizeof(double2)));
-    *out = (double2*)c->radial.p;
+    HIPCHK(c, c->radial.ensure((size_t)((m + 1) * (m + 2) / 2) * sizeof(Radial)));
+    *out = (Radial*)c->radial.p;
     return CSX_OK;
 }
 
@@ -992,7 +1062,7 @@ int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
     HIPCHK(c, hipSetDevice(c->device));
     const int kind = kind_of(c->set.s.combo_key);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    double2* rtab = nullptr;
+    Radial* rtab = nullptr;
     if (int rc = radial_table(c, *g, &rtab)) return rc;
     DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, nullptr, rtab, c->stream);
     HIPCHK(c, hipGetLastError());
@@ -1015,7 +1085,7 @@ int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out
     const std::vector<uint64_t> zero(CSX_NSTATS, 0);
     HIPCHK(c, hipMemcpyAsync(c->stats.p, zero.data(), CSX_NSTATS * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
     const int kind = kind_of(c->set.s.combo_key);
-    double2* rtab = nullptr;
+    Radial* rtab = nullptr;
     if (int rc = radial_table(c, *g, &rtab)) return rc;
     DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, (unsigned long long*)c->stats.p, rtab, c->stream);
     HIPCHK(c, hipGetLastError());

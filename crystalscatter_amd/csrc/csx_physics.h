@@ -32,6 +32,11 @@ struct alignas(32) PairRec {
 };
 constexpr int kBands = 8;
 
+// Per-|q| values shared by the eight sign/swap images of a midpoint-grid pixel.
+struct alignas(32) Radial {
+    double q, pq, fq, dwe;  // |q|, P(q), <F>^2, exp(-dwfactor q^2)
+};
+
 struct DevSet {
     csx_scalars s;
     const double* carr;      // 18 x CSX_NCOEF
@@ -53,6 +58,10 @@ struct DevSet {
     float cut;               // base-2 exponent below which a pair term is dropped
     float qh_min, qh_max;    // |q_hkl| range of the pairs
     double asum2;            // sum over pairs of 2|A| (infinite when a weight is negative)
+    double kp_coef;          // -4/(pi*phiwidth^2)*log2(e): angular exponent = kp_coef*q^2*y^2
+    double kr_const;         // -4/(pi*width^2)*log2(e) when the width does not depend on q
+    double inv_hpw;          // 1/(pi*width/2) when the width does not depend on q
+    double two_pi_over_v;    // 2*pi/sphno_cubevol
 };
 
 __device__ __forceinline__ const double* carr_of(const DevSet& d, int which) {

@@ -96,10 +96,10 @@ def test_cfg3_fcc_full_size(prec, tol):
     assert r.max() < tol
     # size-independent invariant: I(-q) = I(q).  FP64: Friedel partners are summed in the
     # same order up to rounding.  Mixed: mirrored pixels sit in different waves, whose pair
-    # windows drop different terms below 2^-40 of a pair weight, so the two sides agree to
-    # the dropped mass, far below the 1e-4 bound.
+    # windows drop different terms below 2^-32 of a pair weight, so the two sides differ by
+    # at most the dropped mass (about 1e-8 relative here), far below the 1e-4 bound.
     a = img.reshape(1024, 1024)[1:, 1:]
-    assert rel_err(a, a[::-1, ::-1]).max() < (1e-13 if prec == "ref64" else 1e-8)
+    assert rel_err(a, a[::-1, ::-1]).max() < (1e-13 if prec == "ref64" else 1e-6)
 
 
 def test_cfg3_full_size_against_oracle_random_pixels(oracle_lib):
