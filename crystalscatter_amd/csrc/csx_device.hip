@@ -29,7 +29,9 @@ constexpr int kBlock = kTile * kTile;
 constexpr float kPiF = 3.14159265358979323846f;
 constexpr double kLog2e = 1.4426950408889634074;
 
-enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4 };
+// K_CYL_GAUSS_Q: Gaussian cylinders in physics mode with the FP32 orientation quadrature
+// (mixed precision only; chosen by csx_render, never by the key alone).
+enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4, K_CYL_GAUSS_Q = 5 };
 
 inline int kind_of(uint32_t key) {
     switch (key) {
@@ -336,13 +338,253 @@ __device__ double iso_mixed(const DevSet& d, double q) {
 }
 
 // ---------------------------------------------------------------------------
+// Gaussian-distributed cylinders: the longitudinal orientation average
+// ---------------------------------------------------------------------------
+
+// Rod factor as the chi integrand evaluates it (Source/sc_calc_generic_gpu.cu:1640-1652).
+__device__ __forceinline__ double chi_rod(double z, double arg) {
+    const double a1 = 1 / (2.0 * z * (z - 1));
+    return (a1 / (arg * arg)) * (1 - cos((z - 1) * atan(2.0 * arg)) / pow(1.0 + 4 * arg * arg, (z - 1) / 2.0));
+}
+
+struct ChiCtx {
+    double l, z;
+    float p11, p12, p13, p21, p22, p23;  // float-rounded rotation entries (.cu:1402)
+    double qx, qy;
+    int i1;
+};
+
+// trapzdchid integrand for the cylinder cases i1 = 1..4 (.cu:1634-1720); other cases give 0.
+__device__ double chi_integrand(const ChiCtx& c, double x, double delta) {
+    const double eps = 0.0000000001;
+    double arg;
+    switch (c.i1) {
+    case 1: {
+        const double ml1 = c.p11 * cos(x) * sin(delta) + c.p12 * sin(x) * sin(delta) + c.p13 * cos(delta);
+        const double ml2 = c.p21 * sin(x) * sin(delta) + c.p22 * cos(x) * sin(delta) + c.p23 * cos(delta);
+        arg = (c.qx * ml1 + c.qy * ml2 + eps) * c.l / (c.z + 1);
+        break;
+    }
+    case 2: arg = (c.qx * cos(delta) - c.qy * sin(x) * sin(delta) + eps) * c.l / (c.z + 1); break;
+    case 3: arg = (c.qx * sin(x) * sin(delta) + c.qy * cos(delta) + eps) * c.l / (c.z + 1); break;
+    case 4: arg = (c.qx * sin(x) * sin(delta) - c.qy * cos(x) * sin(delta) + eps) * c.l / (c.z + 1); break;
+    default: return 0.0;
+    }
+    return chi_rod(c.z, arg);
+}
+
+// Neville extrapolation to h = 0 through k = 5 points (polint, .cu:1280-1335); 1-based.
+__device__ void neville5(const double* xa, const double* ya, double& y, double& dy) {
+    double c[6], d[6];
+    int ns = 1;
+    double dif = fabs(xa[1]);
+    for (int i = 1; i <= 5; ++i) {
+        const double dift = fabs(xa[i]);
+        if (dift < dif) { ns = i; dif = dift; }
+        c[i] = ya[i];
+        d[i] = ya[i];
+    }
+    y = ya[ns];
+    ns = ns - 1;
+    for (int m = 1; m <= 4; ++m) {
+        for (int i = 1; i <= 5 - m; ++i) {
+            const double ho = xa[i], hp = xa[i + m];
+            const double w = c[i + 1] - d[i];
+            double den = ho - hp;
+            if (den != 0) den = w / den;
+            d[i] = hp * den;
+            c[i] = ho * den;
+        }
+        if (2 * ns < 5 - m) dy = c[ns + 1];
+        else { dy = d[ns]; ns = ns - 1; }
+        y = y + dy;
+    }
+}
+
+// Romberg over chi in [0, float(2 pi)] (qrombchid .cu:1495-1583 with trapzdchid's
+// midpoint refinement): eps 1e-4, jmax 20, k 5.
+__device__ double romb_chi(const ChiCtx& c, double delta) {
+    const double eps = 1.0e-4;
+    double hp[22], sp[22];
+    const double a = 0.0, b = (double)(float)(2 * kPiD);
+    double pq = 0, dssp = 0;
+    int cnt = 1;
+    hp[1] = 1.0;
+    for (int j = 1; j <= 20; ++j) {
+        if (j == 1) {
+            sp[j] = 0.5 * (b - a) * (chi_integrand(c, a, delta) + chi_integrand(c, b, delta));
+            cnt = 1;
+        } else {
+            const double tnm = cnt, del = (b - a) / tnm;
+            double x = a + 0.5 * del, sum = 0.0;
+            for (int i = 1; i <= cnt; ++i) { sum += chi_integrand(c, x, delta); x = x + del; }
+            sp[j] = 0.5 * (sp[j] + (b - a) * sum / tnm);
+            cnt = 2 * cnt;
+        }
+        if (j >= 5) {
+            neville5(hp + j - 5, sp + j - 5, pq, dssp);
+            if (fabs(dssp) < (eps * fabs(pq))) break;
+            if (fabs(pq) < 1.0e-300) break;
+        }
+        sp[j + 1] = sp[j];
+        hp[j + 1] = 0.25 * hp[j];
+    }
+    return pq;
+}
+
+// Romberg over delta (qrombdeltac .cu:1370-1488, i0 = 4, Gaussian weight) of
+// sin(d) exp(-d^2/dbeta^2) * romb_chi / (2 pi), in [0, float(min(dbeta sqrt(ln 1e5), pi/2))].
+__device__ double romb_delta_cyl(const DevSet& d, double qx, double qy, int i1) {
+    const csx_scalars& s = d.s;
+    const double eps = 1.0e-4, prmin = 0.00001;
+    double hp[22], sp[22];
+    const double dbeta = s.dbeta * kPiD / 180.0;
+    const double th = s.polTheta * kPiD / 180.0, ph = 0.0;
+    double delmax = dbeta * sqrt(log(1.0 / prmin));
+    if (delmax > kPiD / 2.0) delmax = kPiD / 2.0;
+    ChiCtx c;
+    c.l = s.length;
+    c.z = (1 - s.sigmal * s.sigmal) / (s.sigmal * s.sigmal);
+    c.p11 = (float)(-cos(ph) * cos(th));
+    c.p12 = (float)sin(ph);
+    c.p13 = (float)(cos(ph) * sin(th));
+    c.p21 = (float)(-cos(ph));
+    c.p22 = (float)(-sin(ph) * cos(th));
+    c.p23 = (float)(sin(ph) * sin(th));
+    c.qx = qx;
+    c.qy = qy;
+    c.i1 = i1;
+    const double a = 0.0, b = (double)(float)delmax;
+    auto f = [&](double x) {
+        return sin(x) * exp(-x * x / (dbeta * dbeta)) * romb_chi(c, x) / (2.0 * kPiD);
+    };
+    double pq = 0, dssp = 0;
+    int cnt = 1;
+    hp[1] = 1.0;
+    for (int j = 1; j <= 20; ++j) {
+        if (j == 1) {
+            sp[j] = 0.5 * (b - a) * (f(a) + f(b));
+            cnt = 1;
+        } else {
+            const double tnm = cnt, del = (b - a) / tnm;
+            double x = a + 0.5 * del, sum = 0.0;
+            for (int i = 1; i <= cnt; ++i) { sum = sum + f(x); x = x + del; }
+            sp[j] = 0.5 * (sp[j] + (b - a) * sum / tnm);
+            cnt = 2 * cnt;
+        }
+        if (j >= 5) {
+            neville5(hp + j - 5, sp + j - 5, pq, dssp);
+            if (fabs(dssp) < (eps * fabs(pq))) break;
+            if (fabs(pq) < 1.0e-100) break;
+        }
+        sp[j + 1] = sp[j];
+        hp[j + 1] = 0.25 * hp[j];
+    }
+    return pq;
+}
+
+// Small-q double series of the Gaussian orientation average (formpq_partCylinder ordis 0,
+// Source/sc_lib_formpq_partCylinder.h:133-203), orientation cases 1..3:
+//   pql = 1 + sum_n c1p[n] sum_m c11pm[n-m][m] X^m Y^(n-m)
+// with (X, Y) = (qxs^2, qys^2), swapped for case 3.  The inner sum is evaluated by Horner in
+// the ratio of the smaller to the larger of X and Y (no power tables; the reference builds
+// X^m and Y^(n-m) as running products, so the two agree to rounding).
+__device__ double cyl_gauss_series(const DevSet& d, double qxs, double qys, int oc) {
+    const double* c1p = carr_of(d, 0);
+    const double* c11 = d.carr11pm;  // [n][m], CSX_NIM2 columns
+    const double X = (oc == 3) ? qys * qys : qxs * qxs, Y = (oc == 3) ? qxs * qxs : qys * qys;
+    const bool xs = X <= Y;            // Horner variable r = small/large <= 1
+    const double big = xs ? Y : X, r = xs ? (Y > 0 ? X / Y : 0.0) : (X > 0 ? Y / X : 0.0);
+    double pql = 1.0, old = 0.0, bn = 1.0;
+#pragma unroll 1
+    for (int n = 1; n <= 120; ++n) {
+        bn *= big;
+        // sum_m c11[n-m][m] X^m Y^(n-m) = big^n * sum_k a_k r^k, with k = m (X small) or n-m
+        double h = xs ? c11[0 * CSX_NIM2 + n] : c11[n * CSX_NIM2 + 0];
+#pragma unroll 1
+        for (int k = n - 1; k >= 0; --k) {
+            const double a = xs ? c11[(n - k) * CSX_NIM2 + k] : c11[k * CSX_NIM2 + (n - k)];
+            h = h * r + a;
+        }
+        pql += c1p[n] * (h * bn);
+        const double dd = fabs((pql - old) / pql);
+        if (dd < 0.0001) break;
+        old = pql;
+    }
+    return pql;
+}
+
+// LDS copies of the physics-mode orientation quadrature tables (filled by k_image for the
+// Gaussian-cylinder kernels; 33.9 KB).  File-scope LDS, so that every access compiles to a
+// ds_read with a 32-bit offset.
+__shared__ float2 g_cq_tab[kCqTab + 1];
+__shared__ float4 g_cq_dl[kCqDelta];
+__shared__ float g_cq_tt[kCqPsi / 2];
+
+// Rod factor P(|a|) from the LDS table: cubic Hermite on the knot interval (value and
+// h*slope at both knots).  |a| is below amax by construction; the clamp only keeps the
+// index in range.
+__device__ __forceinline__ float cq_rod(float invh, float a) {
+    const float u = fminf(fabsf(a) * invh, (float)kCqTab - 0.001f);
+    const int i = (int)u;
+    const float t = __builtin_amdgcn_fractf(u);
+    const float2 A = g_cq_tab[i], B = g_cq_tab[i + 1];
+    const float df = B.x - A.x;
+    const float c2 = __builtin_fmaf(3.0f, df, -(2.0f * A.y + B.y));
+    const float c3 = (A.y + B.y) - 2.0f * df;
+    return __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, c3, c2), A.y), A.x);
+}
+
+// Physics-mode orientation average of the longitudinal factor, FP32 with a compensated sum
+// over the polar nodes.  For a fixed polar angle delta the chi integrand depends on chi only
+// through arg = alpha + R cos(psi) (psi = chi - chi0; the integral is shift-invariant over a
+// period), so (1/2pi) int_0^2pi dchi = (1/pi) int_0^pi dpsi, taken as a kCqPsi-point midpoint
+// rule whose nodes come in pairs +-t.  Polar: kCqDelta-point Gauss-Legendre with the weight
+// sin(d) exp(-d^2/dbeta^2), normalised to 1 on the host.
+__device__ double cyl_gauss_quad(const DevSet& d, double qa, double qb, int oc) {
+    double c1, rq;
+    if (oc == 1) {
+        const double p11 = (double)(float)(-cos(d.s.polTheta * kPiD / 180.0));
+        const double p13 = (double)(float)(sin(d.s.polTheta * kPiD / 180.0));
+        // phi = 0 in the reference's call: p12 = p22 = p23 = 0, p21 = -1
+        c1 = qa * p13;
+        rq = sqrt(sqd(qa * p11) + sqd(qb * -1.0));
+    } else if (oc == 2) {
+        c1 = qa;
+        rq = fabs(qb);
+    } else {
+        c1 = qb;
+        rq = fabs(qa);
+    }
+    const float lz = d.cq_lz, invh = d.cq_invh;
+    const float fc1 = (float)c1, frq = (float)rq;
+    float sum = 0.0f, comp = 0.0f;
+    for (int k = 0; k < kCqDelta; ++k) {
+        const float4 nd = g_cq_dl[k];  // cos, sin, weight
+        const float al = __builtin_fmaf(nd.x, fc1, 1.0e-10f) * lz;
+        const float rr = nd.y * frq * lz;
+        float sk = 0.0f;
+#pragma unroll 4
+        for (int j = 0; j < kCqPsi / 2; ++j) {
+            const float t = g_cq_tt[j];
+            sk += cq_rod(invh, __builtin_fmaf(rr, t, al)) + cq_rod(invh, __builtin_fmaf(-rr, t, al));
+        }
+        const float y = __builtin_fmaf(nd.z, sk, -comp);
+        const float tnew = sum + y;
+        comp = (tnew - sum) - y;
+        sum = tnew;
+    }
+    return (double)sum;
+}
+
+// ---------------------------------------------------------------------------
 // Pixel value of each combination
 // ---------------------------------------------------------------------------
 // P3: the pixel (its exact q vector is computed here unless the caller has it); rv: its
 // radial-table entry (midpoint grid, sphere kinds) or null.
 template <int KIND, int PREC>
 __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigned long long* stats = nullptr,
-                              const Radial* rv = nullptr) {
+                                              const Radial* rv = nullptr) {
     const csx_scalars& s = d.s;
     // The mixed FCC path with a radial entry needs only |q| and the pixel indices.
     if (!(rv && KIND == K_FCC && PREC == CSX_PREC_MIXED)) need_q(s, P3);
@@ -361,14 +603,27 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
         }
         pq *= pq_cyl_transverse(d, q);
         return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
-    } else if constexpr (KIND == K_CYL_GAUSS) {
-        // Gaussian orientation, general case in reference mode: outside the small-q series
-        // the reference's Romberg integrand is identically zero (orientation-case numbering),
-        // inside it the orientation-case branches leave the longitudinal factor at 1.
-        const double qxs = qx * s.cosphic - qy * s.sinphic, qys = qx * s.sinphic + qy * s.cosphic;
-        (void)qxs; (void)qys;
-        double pql = 0.0;
-        if (q < (0.4 * s.limq[0])) pql = 1.0;
+    } else if constexpr (KIND == K_CYL_GAUSS || KIND == K_CYL_GAUSS_Q) {
+        // Gaussian orientation (Source/sc_gpu_pCyl_lNon_oGau.h:36-89, formpq_partCylinder
+        // ordis 0).  The general case uses q rotated by (cosphic, sinphic).
+        double qa = qx, qb = qy;
+        if (s.orcase == 0) {
+            qa = qx * s.cosphic - qy * s.sinphic;
+            qb = qx * s.sinphic + qy * s.cosphic;
+        }
+        const int oc = orc_cmp(s);
+        double pql;
+        if (oc == 4) {
+            pql = 1.0;
+        } else if (q < (0.4 * s.limq[0])) {
+            pql = (oc >= 1 && oc <= 3) ? cyl_gauss_series(d, qa, qb, oc) : 1.0;
+        } else if constexpr (KIND == K_CYL_GAUSS_Q) {
+            pql = cyl_gauss_quad(d, qa, qb, oc);  // physics mode: oc in 1..3, tables in LDS
+        } else {
+            // the reference's nested Romberg; in reference mode the general case (oc = 0)
+            // has an identically zero integrand
+            pql = (oc >= 1 && oc <= 4) ? romb_delta_cyl(d, qa, qb, oc) / s.norm : 0.0;
+        }
         const double pq = pql * pq_cyl_transverse(d, q);
         return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
     } else {
@@ -511,6 +766,12 @@ __global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid
     const int i0 = g.ymin + blockIdx.y * kTile + py;
     const int ihex = min(ihex0, g.xmax - 1), i = min(i0, g.ymax - 1);
     PixQ P3{ihex, i, 0.0, 0.0, 0.0, false};
+    if constexpr (KIND == K_CYL_GAUSS_Q) {
+        for (int k = threadIdx.x; k <= kCqTab; k += kBlock) g_cq_tab[k] = d.cq_tab[k];
+        if (threadIdx.x < kCqDelta) g_cq_dl[threadIdx.x] = d.cq_delta[threadIdx.x];
+        if (threadIdx.x < kCqPsi / 2) g_cq_tt[threadIdx.x] = d.cq_t[threadIdx.x];
+        __syncthreads();
+    }
     const Radial* rv = nullptr;
     if (rtab) {
         const int a = abs(i), b = abs(ihex);
@@ -811,6 +1072,93 @@ void set_fast(DevSet& d, const FastTables& ft) {
     d.asum2 = ft.asum2;
 }
 
+// Physics-mode orientation quadrature tables of Gaussian-distributed cylinders.
+struct CylQuad {
+    std::vector<float4> dl;   // kCqDelta x (cos d, sin d, weight, 0)
+    std::vector<float> tt;    // kCqPsi/2 x cos(psi)
+    std::vector<float2> tab;  // kCqTab+1 x (P(a), h*P'(a))
+    float invh = 0.0f, lz = 0.0f;
+};
+
+// Gauss-Legendre nodes and weights on [-1, 1]: Newton iteration on the three-term
+// recurrence of P_n from the asymptotic first guesses cos(pi (i + 3/4) / (n + 1/2)).
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = std::cos(kPiD * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;  // P_0, P_1
+            for (int k = 2; k <= n; ++k) {
+                const double pk = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = pk;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1);  // P_n'
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[i] = -z;
+        x[n - 1 - i] = z;
+        w[i] = w[n - 1 - i] = 2.0 / ((1 - z * z) * dp * dp);
+    }
+}
+
+// Schulz-averaged rod factor P(a) = (1 - e^A cos B) / (2 n (n+1) a^2) with n = z-1,
+// A = -(n/2) ln(1+4a^2), B = n atan(2a), written as -expm1(A) + 2 e^A sin^2(B/2) (no
+// cancellation at small a), and its slope; long double on the host.
+void rod_value_slope(double z, long double a, long double& P, long double& dP) {
+    if (a == 0) { P = 1; dP = 0; return; }
+    const long double n = z - 1, q = 1 + 4 * a * a;
+    const long double A = -(n / 2) * log1pl(4 * a * a), B = n * atanl(2 * a);
+    const long double eA = expl(A), sB = sinl(B / 2);
+    const long double g = -expm1l(A) + 2 * eA * sB * sB;
+    const long double Ap = -4 * n * a / q, Bp = 2 * n / q;
+    const long double gp = -eA * (Ap * cosl(B) - Bp * sinl(B));
+    const long double den = 2 * n * (n + 1);
+    P = g / (den * a * a);
+    dP = (gp * a - 2 * g) / (den * a * a * a);
+}
+
+void build_cyl_quad(const csx_scalars& s, CylQuad& cq) {
+    const double zl = (1 - s.sigmal * s.sigmal) / (s.sigmal * s.sigmal);
+    const double dbeta = s.dbeta * kPiD / 180.0;
+    double delmax = dbeta * std::sqrt(std::log(1.0 / 0.00001));
+    if (delmax > kPiD / 2.0) delmax = kPiD / 2.0;
+    const double dmax = (double)(float)delmax;  // the reference's float limit
+    std::vector<double> x, w;
+    gauss_legendre(kCqDelta, x, w);
+    std::vector<double> wd(kCqDelta);
+    double wsum = 0.0;
+    for (int k = 0; k < kCqDelta; ++k) {
+        const double dl = 0.5 * dmax * (x[k] + 1);
+        wd[k] = 0.5 * dmax * w[k] * std::sin(dl) * std::exp(-dl * dl / (dbeta * dbeta));
+        wsum += wd[k];
+    }
+    cq.dl.resize(kCqDelta);
+    for (int k = 0; k < kCqDelta; ++k) {
+        const double dl = 0.5 * dmax * (x[k] + 1);
+        cq.dl[k] = make_float4((float)std::cos(dl), (float)std::sin(dl), (float)(wd[k] / (wsum * kCqPsi)), 0.0f);
+    }
+    cq.tt.resize(kCqPsi / 2);
+    for (int j = 0; j < kCqPsi / 2; ++j) cq.tt[j] = (float)std::cos(kPiD * (j + 0.5) / kCqPsi);
+    // |arg| <= (|q| + 1e-10) L/(z+1): bound |q| over the grid (midpoint: sqrt(2) qmax;
+    // detector: 4 pi / lambda)
+    const double lz = s.length / (zl + 1);
+    const double qb = s.centerbeam ? 4 * kPiD / s.wave : std::sqrt(2.0) * s.qmax;
+    const double amax = (qb * 1.001 + 1e-9) * lz;
+    const double h = amax / kCqTab;
+    cq.tab.resize(kCqTab + 1);
+    for (int i = 0; i <= kCqTab; ++i) {
+        long double P, dP;
+        rod_value_slope(zl, (long double)i * h, P, dP);
+        cq.tab[i] = make_float2((float)P, (float)(dP * h));
+    }
+    cq.invh = (float)(kCqTab / amax);
+    cq.lz = (float)lz;
+}
+
 }  // namespace
 
 struct csx_ctx {
@@ -824,7 +1172,8 @@ struct csx_ctx {
     std::string err;
     bool uploaded = false;
     DevSet set{};
-    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, lut, singles, iso, tiles, fqs_tmp, stats, radial;
+    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, lut, singles, iso, tiles, fqs_tmp, stats, radial, cqdl, cqtt,
+        cqtab;
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
@@ -892,6 +1241,14 @@ static int put_table(csx_ctx* c, DevBuf& b, const std::vector<T>& v) {
     return CSX_OK;
 }
 
+// Kernel kind of a render: the key's kind, or the physics-mode quadrature kernel for
+// Gaussian cylinders in mixed precision.
+static int render_kind(const csx_ctx* c, int precision) {
+    const int k = kind_of(c->set.s.combo_key);
+    if (k == K_CYL_GAUSS && c->set.s.fix_orcase && precision == CSX_PREC_MIXED && c->set.cq_tab) return K_CYL_GAUSS_Q;
+    return k;
+}
+
 // Radial table of a render (midpoint grid, radial kinds), sized for the grid; null otherwise.
 static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out) {
     *out = nullptr;
@@ -910,6 +1267,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out) {
     case K_FCC: if (prec) FN<K_FCC, 1>(__VA_ARGS__); else FN<K_FCC, 0>(__VA_ARGS__); break; \
     case K_CYL_GAUSS: if (prec) FN<K_CYL_GAUSS, 1>(__VA_ARGS__); else FN<K_CYL_GAUSS, 0>(__VA_ARGS__); break; \
     case K_CYL_ZDIR: if (prec) FN<K_CYL_ZDIR, 1>(__VA_ARGS__); else FN<K_CYL_ZDIR, 0>(__VA_ARGS__); break; \
+    case K_CYL_GAUSS_Q: FN<K_CYL_GAUSS_Q, 1>(__VA_ARGS__); break; \
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
@@ -938,7 +1296,7 @@ void csx_destroy(csx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->lut, &c->singles,
-                      &c->iso, &c->stats, &c->radial, &c->tiles,
+                      &c->iso, &c->stats, &c->radial, &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
                       &c->bsums_d, &c->bcnts_d})
         b->release();
@@ -1044,6 +1402,19 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
         c->set.iso = (const float2*)c->iso.p;
         set_fast(c->set, ft);
     }
+    CylQuad cq;
+    if (kind_of(s->combo_key) == K_CYL_GAUSS && s->fix_orcase) {
+        build_cyl_quad(*s, cq);
+        rc = put_table(c, c->cqdl, cq.dl);
+        if (!rc) rc = put_table(c, c->cqtt, cq.tt);
+        if (!rc) rc = put_table(c, c->cqtab, cq.tab);
+        if (rc) return rc;
+        c->set.cq_delta = (const float4*)c->cqdl.p;
+        c->set.cq_t = (const float*)c->cqtt.p;
+        c->set.cq_tab = (const float2*)c->cqtab.p;
+        c->set.cq_invh = cq.invh;
+        c->set.cq_lz = cq.lz;
+    }
     // The host staging memory is reused: wait until the copies have consumed it.
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->uploaded = true;
@@ -1057,10 +1428,8 @@ int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
     if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_render before csx_upload");
     if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
     if (precision != CSX_PREC_REF64 && precision != CSX_PREC_MIXED) return fail(c, CSX_E_ARG, "bad precision");
-    if (c->set.s.fix_orcase && kind_of(c->set.s.combo_key) == K_CYL_GAUSS)
-        return fail(c, CSX_E_ARG, "orientation-averaged cylinders in physics mode are not available");
     HIPCHK(c, hipSetDevice(c->device));
-    const int kind = kind_of(c->set.s.combo_key);
+    const int kind = render_kind(c, precision);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     Radial* rtab = nullptr;
     if (int rc = radial_table(c, *g, &rtab)) return rc;
@@ -1078,13 +1447,11 @@ int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out
     if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_render_stats before csx_upload");
     if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
     if (precision != CSX_PREC_REF64 && precision != CSX_PREC_MIXED) return fail(c, CSX_E_ARG, "bad precision");
-    if (c->set.s.fix_orcase && kind_of(c->set.s.combo_key) == K_CYL_GAUSS)
-        return fail(c, CSX_E_ARG, "orientation-averaged cylinders in physics mode are not available");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, c->stats.ensure(CSX_NSTATS * sizeof(uint64_t)));
     const std::vector<uint64_t> zero(CSX_NSTATS, 0);
     HIPCHK(c, hipMemcpyAsync(c->stats.p, zero.data(), CSX_NSTATS * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
-    const int kind = kind_of(c->set.s.combo_key);
+    const int kind = render_kind(c, precision);
     Radial* rtab = nullptr;
     if (int rc = radial_table(c, *g, &rtab)) return rc;
     DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, (unsigned long long*)c->stats.p, rtab, c->stream);

@@ -31,6 +31,9 @@ struct alignas(32) PairRec {
     int32_t ia, ib;
 };
 constexpr int kBands = 8;
+constexpr int kCqDelta = 64;  // polar nodes
+constexpr int kCqPsi = 64;    // azimuth nodes on [0, pi] (symmetric pairs: kCqPsi/2 values of t)
+constexpr int kCqTab = 4096;  // intervals of the rod-factor table
 
 // Per-|q| values shared by the eight sign/swap images of a midpoint-grid pixel.
 struct alignas(32) Radial {
@@ -62,6 +65,15 @@ struct DevSet {
     double kr_const;         // -4/(pi*width^2)*log2(e) when the width does not depend on q
     double inv_hpw;          // 1/(pi*width/2) when the width does not depend on q
     double two_pi_over_v;    // 2*pi/sphno_cubevol
+    // Orientation quadrature of Gaussian-distributed cylinders in physics mode (built at
+    // upload, see csx_device.hip): kCqDelta Gauss-Legendre polar nodes (cos, sin, weight),
+    // kCqPsi/2 azimuth nodes t = cos(psi) of the midpoint rule on [0, pi], and the
+    // longitudinal rod factor P(a) on [0, amax] as (value, h*slope) at kCqTab+1 knots.
+    const float4* cq_delta;
+    const float* cq_t;
+    const float2* cq_tab;
+    float cq_invh;           // kCqTab / amax
+    float cq_lz;             // L/(z_L+1)
 };
 
 __device__ __forceinline__ const double* carr_of(const DevSet& d, int which) {

@@ -77,6 +77,90 @@ def test_cfg2_reference_mode_exact(prec):
     np.testing.assert_array_equal(img, f["image"])
 
 
+def _tight_orientation_average(qx, qy, p):
+    """Longitudinal orientation average of the Gaussian-cylinder rod factor by adaptive
+    scipy quadrature (epsrel 1e-11), independent of the kernels and of the oracle."""
+    import math
+    from scipy import integrate
+    sl, L, db, th = p["SigmaL"], p["Length"], p["EditDbeta"] * np.pi / 180, p["theta"] * np.pi / 180
+    zl = (1 - sl * sl) / (sl * sl)
+    dmax = float(np.float32(min(db * math.sqrt(math.log(1e5)), np.pi / 2)))
+    p11, p13, p21 = float(np.float32(-math.cos(th))), float(np.float32(math.sin(th))), -1.0
+
+    def rod(arg):
+        a1 = 1 / (2 * zl * (zl - 1))
+        return (a1 / arg ** 2) * (1 - np.cos((zl - 1) * np.arctan(2 * arg)) / (1 + 4 * arg * arg) ** ((zl - 1) / 2))
+
+    def inner(d):
+        f = lambda c: rod((qx * (p11 * np.cos(c) * np.sin(d) + p13 * np.cos(d)) + qy * (p21 * np.sin(c) * np.sin(d))
+                           + 1e-10) * L / (zl + 1))
+        return integrate.quad(f, 0, 2 * np.pi, limit=800, epsrel=1e-11)[0] / (2 * np.pi)
+
+    w = lambda d: np.sin(d) * np.exp(-d * d / db ** 2)
+    norm = integrate.quad(w, 0, dmax, epsrel=1e-13)[0]
+    return integrate.quad(lambda d: w(d) * inner(d), 0, dmax, limit=400, epsrel=1e-10)[0] / norm
+
+
+def test_cfg2_physics_mode_ref64_matches_oracle(oracle_lib):
+    """Physics mode (fix_orcase=1): the GPU's FP64 nested Romberg against the oracle's."""
+    from helpers import CFG2
+    p = dict(CFG2, GridPoints=16, fix_orcase=1)
+    e, img = _calc(p, "ref64")
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    r = rel_err(img, o.image(nthreads=16))
+    _report("cfg2 physics ref64 vs oracle", r)
+    assert r.max() < 1e-12
+
+
+def test_cfg2_physics_mode_mixed_vs_tight_quadrature(oracle_lib):
+    """Physics mode, FP32 quadrature kernel at full size: against an independent adaptive
+    quadrature of the same integral at pixels across the detector (the reference's own
+    Romberg, eps 1e-4, is off by up to a few percent at large q, so it is not the bar here)."""
+    from helpers import CFG2
+    p = dict(CFG2, fix_orcase=1)
+    e, img = _calc(p, "mixed")
+    x0, x1, y0, y1 = e.bounds()
+    W = x1 - x0
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    ot = oracle_lib.OracleEngine(dict(p, Ordis=6, theta=0, phi=0))  # transverse factor (pql = 1)
+    ot.prepare()
+    worst = 0.0
+    for ih, ii in ((255, 255), (-256, 100), (40, -200), (0, 255), (120, 7), (-30, -40)):
+        qx, qy, qz = o.idx2q(ih, ii)
+        ref = _tight_orientation_average(qx, qy, p) * ot.pixel(qx, qy, qz)
+        worst = max(worst, abs(img[(ih - x0) + W * (ii - y0)] / ref - 1))
+    print("cfg2 physics mixed vs tight quadrature: max rel %.3e" % worst)
+    assert worst < 2e-6
+
+
+def test_cfg2_physics_small_q_series_same_in_both_precisions():
+    """Below 0.4*limq1 both precisions take the FP64 double series."""
+    from helpers import CFG2
+    p = dict(CFG2, GridPoints=16, fix_orcase=1)
+    e, a = _calc(p, "mixed")
+    _, b = _calc(p, "ref64")
+    s = e.scalars()
+    x0, x1, y0, y1 = e.bounds()
+    ih, ii = np.meshgrid(np.arange(x0, x1), np.arange(y0, y1))
+    q = np.hypot(ih, ii).ravel() * s.qmax / s.zmax
+    small = q < 0.4 * s.limq[0] * 0.999
+    assert small.sum() > 0
+    np.testing.assert_array_equal(a[small], b[small])
+
+
+@pytest.mark.parametrize("theta,phi", [(60, 20), (10, 80)])
+def test_cylinders_reference_mode_other_orientations(oracle_lib, theta, phi):
+    from helpers import CFG2
+    p = dict(CFG2, GridPoints=16, theta=theta, phi=phi)
+    e, img = _calc(p, "ref64")
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    r = rel_err(img, o.image(nthreads=16))
+    assert r.max() < 1e-13
+
+
 @pytest.mark.parametrize("prec,tol", [("ref64", 1e-13), ("mixed", 1e-4)])
 def test_cfg3_fcc_small_grid(prec, tol):
     f = fixture("cfg3_fcc_g32.npz")
