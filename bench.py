@@ -71,6 +71,100 @@ def measured_traffic():
     return None
 
 
+# Batched 2D fits: BCT lattice of cylinders, z-axis orientation (key 0x01020806).
+CFG4_BASE = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, ComboBoxInterior=0, theta=0, phi=0,
+                 EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1, Base=0, HKLmax=5)
+FIT_KEYS = ["EditRadius", "Length", "uca,ucb", "ucc", "EditDomainSize", "EditAzi"]
+FIT_LO = np.array([2.0, 10.0, 15.0, 15.0, 100.0, 2.0])
+FIT_HI = np.array([6.0, 60.0, 40.0, 80.0, 400.0, 10.0])
+
+
+def fit_problems(nfits, seed=1234, noise_seed=5678):
+    """The fit workload: hidden true parameters drawn uniformly (R in [2,6] nm, L in [10,60] nm,
+    a in [15,40] nm, c/a in [1,2], domain in [100,400] nm, angular peak width EditAzi in
+    [2,10] deg), detector N in {128,256,384,512}, q in [-1,1] nm^-1; target = FP64 image
+    at the truth scaled to 1e4 counts at the brightest unmasked pixel (the fits know this
+    scale as I0), Poisson noise, a circular beam stop of radius 8 px (data 0: excluded by
+    the FQS rule); start = truth perturbed by up to +-10 % per parameter."""
+    import torch
+    from crystalscatter_amd import SasCalc
+    rng = np.random.default_rng(seed)
+    nrng = np.random.default_rng(noise_seed)
+    probs = []
+    for k in range(nfits):
+        R, L, a, ca = rng.uniform(2, 6), rng.uniform(10, 60), rng.uniform(15, 40), rng.uniform(1, 2)
+        dom, azi = rng.uniform(100, 400), rng.uniform(2, 10)
+        n = int(rng.choice([128, 256, 384, 512]))
+        p = dict(CFG4_BASE, EditRadius=R, Length=L, uca=a, ucb=a, ucc=a * ca, EditDomainSize=dom, EditAzi=azi,
+                 GridPoints=n // 2)
+        e = SasCalc(device=0, params=p)
+        e.set("ignore_cbs", 1)
+        img = e.image2d(e.doCalculation("ref64"))
+        x0, x1, y0, y1 = e.bounds()
+        ii, jj = np.meshgrid(np.arange(y0, y1), np.arange(x0, x1), indexing="ij")
+        stop = ii * ii + jj * jj < 8 * 8
+        good = np.isfinite(img) & (img > 0) & ~stop
+        scale = 1e4 / img[good].max()
+        lam = np.where(good, img * scale, 0.0)  # NaN / non-positive model pixels carry no data
+        data = nrng.poisson(lam).astype(np.float64)
+        data[stop] = 0.0
+        truth = np.array([R, L, a, a * ca, dom, azi])
+        start = np.clip(truth * (1 + rng.uniform(-0.1, 0.1, 6)), FIT_LO * 1.001, FIT_HI * 0.999)
+        base = SasCalc(device=-1, params=dict(p, I0=scale))  # the count scale is known
+        base.set("ignore_cbs", 1)
+        probs.append(dict(engine=base, data=torch.tensor(data.ravel(), dtype=torch.float64, device="cuda"),
+                          w=x1 - x0, h=y1 - y0, truth=truth, x0=start, n=n))
+    return probs
+
+
+def bench_fits(args):
+    import torch
+    from crystalscatter_amd import Context, nm_batch
+    t0 = time.perf_counter()
+    probs = fit_problems(args.nfits)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    fits = [dict(engine=p["engine"], d_data=p["data"].data_ptr(), w=p["w"], h=p["h"], x0=p["x0"], lo=FIT_LO,
+                 hi=FIT_HI) for p in probs]
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    res, st = nm_batch(ctx, fits, FIT_KEYS, stp=0.05, maxit=args.maxit, ftol=1e-6, precision=args.precision,
+                       threads=args.cpu_threads)
+    rounds = st["rounds"]
+    wall = time.perf_counter() - t0
+    px = sum(p["w"] * p["h"] for p in probs)
+    evals = np.array([r[3] for r in res])
+    iters = np.array([r[2] for r in res])
+    rel0 = np.array([np.abs(p["x0"] / p["truth"] - 1) for p in probs])
+    rel1 = np.array([np.abs(r[0] / p["truth"] - 1) for p, r in zip(probs, res)])
+    line = {
+        "metric": "2D fits/s (batched Nelder-Mead, exact kernel)",
+        "value": len(probs) / wall,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": int(rounds),
+        "warmup": 0,
+        "ms_per_step": 1e3 * wall / max(1, rounds),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d independent fits, BCT cylinders pCyl_pGau_lBCT_oZDi (key 0x01020806), "
+                               "6 free parameters, detectors 128..512, Poisson targets, 8 px beam stop, "
+                               "maxit %d" % (len(probs), args.maxit),
+                   "precision": args.precision, "pixels_per_round_mean": px},
+        "rounds": int(rounds),
+        "evals_mean": float(evals.mean()),
+        "iters_mean": float(iters.mean()),
+        "median_rel_param_err_start": float(np.median(rel0)),
+        "median_rel_param_err_final": float(np.median(rel1)),
+        "target_generation_s": t_gen,
+        "host_prep_ms": st["prep_ms"], "staging_ms": st["stage_ms"], "batched_run_ms": st["run_ms"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(threads):
     """FP64 oracle (bit-exact restatement of the reference CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -93,6 +187,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "ref64"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("CSX_BENCH_CPU_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits"],
+                    help="fcc: the 1024^2 FCC pattern benchmark (default); fits: batched 2D fits")
+    ap.add_argument("--nfits", type=int, default=512)
+    ap.add_argument("--maxit", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 256 MiB device to device once (a plain copy of "
                          "known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -101,6 +199,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (pixels shard across GPUs "
               "without exchange; see the sharding notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "fits":
+        bench_fits(args)
+        return
 
     import ctypes
     import torch
@@ -156,10 +258,10 @@ def main():
         del src, dst
 
     # Work counters of one launch (separate, untimed run of the same kernel).
-    stats = (ctypes.c_uint64 * 4)()
+    stats = (ctypes.c_uint64 * 5)()
     g = _lib.Grid(s.xmin, s.xmax, s.ymin, s.ymax)
     assert L.csx_render_stats(ctx, ctypes.byref(g), prec, ctypes.c_void_p(out.data_ptr()), stats) == 0
-    pair_evals, guard_px, near_one_px, two_waves = (int(v) for v in stats)
+    pair_evals, guard_px, near_one_px, two_waves, full_px = (int(v) for v in stats)
 
     img = out.cpu().numpy()
     max_rel = rms_rel = None
@@ -212,7 +314,7 @@ def main():
                      "transcendental_rate_tops": exec_tops, "transcendental_peak_tops": PEAK_TRANS_TOPS,
                      "reference_equivalent_tflops": alg_flop / sec / 1e12,
                      "reference_equivalent_transcendental_tops": alg_t / sec / 1e12,
-                     "fp64_guard_pixels": guard_px, "near_one_pixels": near_one_px,
+                     "fp64_window_pixels": guard_px, "fp64_full_pixels": full_px, "near_one_pixels": near_one_px,
                      "two_term_waves": two_waves},
     }
     if not args.no_cpu_baseline:

@@ -201,3 +201,52 @@ class Context:
 
     def synchronize(self):
         self._chk(self._L.csx_synchronize(self._h), "csx_synchronize")
+
+
+def simplex(f, x0, lo, hi, stp=0.1, maxit=200, ftol=1e-6):
+    """Nelder-Mead on a Python objective f(x) -> float with the simplex of doSimplexFit2D
+    (Source/sc_simplexfit2d.cpp:254-611).  Returns (x, f, iters, evals)."""
+    L = _lib.lib()
+    n = len(x0)
+    if n > _lib.NM_MAXPAR:
+        raise ValueError("at most %d parameters" % _lib.NM_MAXPAR)
+    cb = _lib.OBJECTIVE(lambda user, p: float(f(np.ctypeslib.as_array(p, shape=(n,)).copy())))
+    arr = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+    x0, lo, hi = arr(x0), arr(lo), arr(hi)
+    r = _lib.NMResult()
+    rc = L.csx_nm_run_cb(n, cb, None, _dp(x0), _dp(lo), _dp(hi), stp, maxit, ftol, ctypes.byref(r))
+    if rc != 0:
+        raise CsxError("csx_nm_run_cb failed (%d)" % rc)
+    return np.array(r.x[:n]), r.f, r.iters, r.evals
+
+
+def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", threads=0):
+    """Batched Nelder-Mead (csx_nm_batch).  fits: list of dicts with 'engine' (SasCalc, may be
+    host-only), 'd_data' (device pointer), 'w', 'h', 'x0', 'lo', 'hi', optional 'bstop',
+    'border', 'rects'.  Returns (list of (x, f, iters, evals), stats dict: rounds, evals,
+    prep_ms, stage_ms, run_ms)."""
+    L = _lib.lib()
+    n = len(keys)
+    arr = (_lib.NMFit * len(fits))()
+    for i, fd in enumerate(fits):
+        a = arr[i]
+        a.base = fd["engine"]._h
+        a.d_data = fd["d_data"]
+        a.w, a.h = fd["w"], fd["h"]
+        for k in range(16):
+            a.mask.rects[k] = -1
+        for r, rect in enumerate(fd.get("rects", [])[:4]):
+            for k in range(4):
+                a.mask.rects[4 * r + k] = rect[k]
+        a.mask.bstop, a.mask.border = fd.get("bstop", 0), fd.get("border", 0)
+        for j in range(n):
+            a.x0[j], a.lo[j], a.hi[j] = fd["x0"][j], fd["lo"][j], fd["hi"][j]
+    ckeys = (ctypes.c_char_p * n)(*[k.encode() for k in keys])
+    res = (_lib.NMResult * len(fits))()
+    st = _lib.NMStats()
+    rc = L.csx_nm_batch(ctx._h if isinstance(ctx, Context) else ctx, len(fits), arr, n, ckeys, stp, maxit, ftol,
+                        _prec(precision), threads, res, ctypes.byref(st))
+    if rc != 0:
+        raise CsxError("csx_nm_batch failed (%d)" % rc)
+    stats = {k: getattr(st, k) for k, _ in _lib.NMStats._fields_}
+    return [(np.array(r.x[:n]), r.f, r.iters, r.evals) for r in res], stats

@@ -48,6 +48,29 @@ class Mask(ctypes.Structure):
     _fields_ = [("rects", ctypes.c_int32 * 16), ("bstop", ctypes.c_int32), ("border", ctypes.c_int32)]
 
 
+NM_MAXPAR = 16
+OBJECTIVE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
+
+
+class NMResult(ctypes.Structure):
+    """Mirror of csx_nm_result."""
+    _fields_ = [("x", ctypes.c_double * NM_MAXPAR), ("f", ctypes.c_double), ("iters", ctypes.c_int32),
+                ("evals", ctypes.c_int32)]
+
+
+class NMStats(ctypes.Structure):
+    """Mirror of csx_nm_stats."""
+    _fields_ = [("rounds", ctypes.c_int64), ("evals", ctypes.c_int64), ("prep_ms", ctypes.c_double),
+                ("stage_ms", ctypes.c_double), ("run_ms", ctypes.c_double)]
+
+
+class NMFit(ctypes.Structure):
+    """Mirror of csx_nm_fit."""
+    _fields_ = [("base", ctypes.c_void_p), ("d_data", ctypes.c_void_p), ("w", ctypes.c_int32), ("h", ctypes.c_int32),
+                ("mask", Mask), ("x0", ctypes.c_double * NM_MAXPAR), ("lo", ctypes.c_double * NM_MAXPAR),
+                ("hi", ctypes.c_double * NM_MAXPAR)]
+
+
 _LIB = None
 
 # (name, restype, argtypes) of every exported entry point of include/csx.h.
@@ -68,6 +91,9 @@ SIGNATURES = [
     ("csx_fit_batch_stage", _I, [_VP, ctypes.POINTER(Scalars), _PD, _PI32, _PI32, _PF, _I, _I,
                                  ctypes.POINTER(Mask), _VP]),
     ("csx_fit_batch_run", _I, [_VP, _I, _PD, _PI64]),
+    ("csx_nm_run_cb", _I, [_I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
+    ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
+                          ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),
     ("csx_last_error", _I, [_VP, _CP, _I]),
     ("csx_last_kernel_ms", _D, [_VP]),
     ("csx_set_timing", _I, [_VP, _I]),

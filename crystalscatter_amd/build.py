@@ -2,9 +2,10 @@
 
     python -m crystalscatter_amd.build        (or __graft_entry__.build())
 
-Host preparation (csx_prep.cpp) is plain C++ built with g++ like the
-reference's engine (no FMA contraction on the x86-64 baseline); the device
-file and the facade are built with hipcc --offload-arch=gfx950.  Packed-FP32
+Host preparation (csx_prep.cpp) and the batched fit driver (csx_fit.cpp) are
+plain C++ built with g++ like the reference's engine (no FMA contraction on the
+x86-64 baseline); the device file and the facade are built with hipcc
+--offload-arch=gfx950.  Packed-FP32
 instruction selection is disabled for all HIP code.
 """
 import os
@@ -36,18 +37,21 @@ def build(verbose=True):
     prep_o = os.path.join(BUILD, "csx_prep.o")
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-I" + INC, "-I" + CSRC, "-c",
           os.path.join(CSRC, "csx_prep.cpp"), "-o", prep_o])
+    fit_o = os.path.join(BUILD, "csx_fit.o")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-pthread", "-I" + INC, "-I" + CSRC, "-c",
+          os.path.join(CSRC, "csx_fit.cpp"), "-o", fit_o])
     dev_o = os.path.join(BUILD, "csx_device.o")
     _run(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, "csx_device.hip"), "-o", dev_o])
     eng_o = os.path.join(BUILD, "csx_engine.o")
     _run(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, "csx_engine.cpp"), "-o", eng_o])
     extra = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.startswith("csx_mlp") and f.endswith(".hip")]
-    objs = [prep_o, dev_o, eng_o]
+    objs = [prep_o, fit_o, dev_o, eng_o]
     for src in extra:
         o = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
         _run(["hipcc"] + HIP_FLAGS + ["-c", src, "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
 

@@ -17,10 +17,12 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "csx.h"
 #include "csx_physics.h"
+#include "csx_prep.h"
 
 namespace csx {
 
@@ -217,7 +219,7 @@ struct Window {
 // dropped: every dropped term is below 2^cut * A.  need_exact: unpaired reflections exist.
 template <bool PLANAR>
 __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, double q, double width, bool& need_exact,
-                                double& dropped, unsigned long long* stats) {
+                                double& dropped, Window& W, unsigned long long* stats) {
     const csx_scalars& s = d.s;
     PixelConsts L;
     if (P3.have) {
@@ -264,7 +266,6 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
     }
     // second Friedel term: e2 <= kp*(pi/2)^2 for every pair
     const bool two = L.kp * (hpi * hpi) > cut;
-    Window W;
     {
         const float lo = wave_min_f(L.qf - dq), hi = wave_max_f(L.qf + dq);
         const float ulo = any_narrow ? wave_min_f(uz_lo) : 0.0f;
@@ -320,6 +321,25 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
     if (stats && flag) atomicAdd(&stats[2], 1ull);
     need_exact = d.nsingle > 0;
     return inten;
+}
+
+// FP64 peak sum over the pairs of a window, both reflections of each pair with the
+// reference's per-reflection formula (acos mapping included).  Differs from the reference's
+// sum only by the order of the terms and by the window's dropped mass.
+__device__ double window_sum64(const DevSet& d, const Window& W, const PixQ& P3, double q, double width) {
+    const double x2phi = 4 * q * q / (kPiD * sqd(d.s.phiwidth));
+    double acc = 0.0;
+    for (int band = W.band0; band <= W.band1; ++band) {
+        const int32_t* lb = d.lut + band * (d.nlut + 1);
+        const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
+        for (int p = p0; p < p1; ++p) {
+            const PairRec P = d.pairs[p];
+            if (P.ia == 0) continue;  // padding record
+            acc += aniso_term_ref(d, P.ia, P3.qx, P3.qy, P3.qz, q, width, x2phi) +
+                   aniso_term_ref(d, P.ib, P3.qx, P3.qy, P3.qz, q, width, x2phi);
+        }
+    }
+    return acc;
 }
 
 // Isotropic (radial) peak sum in FP32, only needed when iso != 0.
@@ -664,21 +684,34 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
         } else {
             bool need_exact = false;
             double dropped = 0.0;
-            inten = s.centerbeam ? lattice_mixed<false>(d, P3, q, width, need_exact, dropped, stats)
-                                 : lattice_mixed<true>(d, P3, q, width, need_exact, dropped, stats);
+            Window W;
+            inten = s.centerbeam ? lattice_mixed<false>(d, P3, q, width, need_exact, dropped, W, stats)
+                                 : lattice_mixed<true>(d, P3, q, width, need_exact, dropped, W, stats);
             radint = (s.iso != 0) ? iso_mixed(d, q) : 0.0;
             // Conditioning guard.  szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
             // The FP32 peak sum I carries up to ~1e-5 relative error plus this lane's dropped
-            // mass; szq amplifies an error of I by |B|/|szq|.  Pixels where szq could move by
-            // more than 3e-5 relative, and by more than 2^-60 absolute (below the rounding of
-            // szq's own FP64 terms), are re-summed in FP64 in the reference's order.
+            // mass; szq amplifies an error of I by |B|/|szq|.  A pixel where szq could move by
+            // more than 3e-5 relative (and 2^-60 absolute, below the rounding of szq's own
+            // FP64 terms) takes the same window in FP64, whose only error is the dropped mass;
+            // if that is still too coarse, the reference's full FP64 sum, in its order.
             const double bs = fqpq * d.two_pi_over_v * dwe;
-            const double szq_est = 1 + fqpq * (inten * d.two_pi_over_v - 1) * dwe;
-            const double dsz = fabs(bs) * (1.0e-5 * fabs(inten) + dropped * (s.para ? 1.0 / (kPiD * width / 2.0) : d.inv_hpw));
-            if (need_exact || !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60))) {
+            const double inv_hpw = s.para ? 1.0 / (kPiD * width / 2.0) : d.inv_hpw;
+            auto loose = [&](double I, double rel) {
+                const double szq_est = 1 + fqpq * (I * d.two_pi_over_v - 1) * dwe;
+                const double dsz = fabs(bs) * (rel * fabs(I) + dropped * inv_hpw);
+                return !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60));
+            };
+            bool full = need_exact;
+            if (!full && loose(inten, 1.0e-5)) {
+                need_q(s, P3);
+                inten = window_sum64(d, W, P3, q, width);
+                if (stats) atomicAdd(&stats[1], 1ull);
+                full = loose(inten, 1.0e-13);
+            }
+            if (full) {
                 need_q(s, P3);
                 lattice_ref64(d, P3.qx, P3.qy, P3.qz, q, width, inten, radint);
-                if (stats) atomicAdd(&stats[1], 1ull);
+                if (stats) atomicAdd(&stats[4], 1ull);
             }
         }
         const double szq = (1 + fqpq * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
@@ -978,19 +1011,27 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
     };
     double asum = 0.0;
     bool negative = false;
+    // (h, k, l) -> reflections with those indices, in list order
+    auto key = [](int h, int k, int l) { return ((int64_t)(h + 1024) << 22) | ((int64_t)(k + 1024) << 11) | (l + 1024); };
+    std::unordered_map<int64_t, std::vector<int>> byhkl;
+    byhkl.reserve(2 * n2 + 1);
+    for (int ii = 1; ii <= n2; ++ii) byhkl[key(L2[ii * 6 + 1], L2[ii * 6 + 2], L2[ii * 6 + 3])].push_back(ii);
     for (int ii = 1; ii <= n2; ++ii) {
         if (used[ii] || !valid(ii)) continue;
         used[ii] = 1;
         const int h = L2[ii * 6 + 1], k = L2[ii * 6 + 2], l = L2[ii * 6 + 3];
         int jj = 0;
-        for (int c = 1; c <= n2 && !jj; ++c) {
-            if (!used[c] && L2[c * 6 + 1] == -h && L2[c * 6 + 2] == -k && L2[c * 6 + 3] == -l && valid(c) &&
-                L3[c * 17 + 5] == L3[ii * 17 + 5] && L3[c * 17 + 10] == L3[ii * 17 + 10] &&
-                L3[c * 17 + 11] == L3[ii * 17 + 11] && L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5] &&
-                L3[c * 17 + 2] == -L3[ii * 17 + 2] && L3[c * 17 + 3] == -L3[ii * 17 + 3] &&
-                L3[c * 17 + 4] == -L3[ii * 17 + 4])
-                jj = c;
-        }
+        const auto it = byhkl.find(key(-h, -k, -l));
+        if (it != byhkl.end())
+            for (int c : it->second) {
+                if (jj) break;
+                if (!used[c] && valid(c) && L3[c * 17 + 5] == L3[ii * 17 + 5] &&
+                    L3[c * 17 + 10] == L3[ii * 17 + 10] && L3[c * 17 + 11] == L3[ii * 17 + 11] &&
+                    L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5] &&
+                    L3[c * 17 + 2] == -L3[ii * 17 + 2] && L3[c * 17 + 3] == -L3[ii * 17 + 3] &&
+                    L3[c * 17 + 4] == -L3[ii * 17 + 4])
+                    jj = c;
+            }
         if (!jj) { ft.singles.push_back(ii); continue; }
         used[jj] = 1;
         const double qh = L3[ii * 17 + 5];
@@ -1049,6 +1090,17 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
         const int m = L1[ii * 6 + 4], f = L1[ii * 6 + 5];
         if (m == 0 || f == 0) continue;
         ft.iso.push_back(make_float2(L3[ii * 17 + 5], (float)(m * f)));
+    }
+}
+
+// Per-set constants of the pixel bodies (all kinds).
+void set_consts(DevSet& d) {
+    const csx_scalars& s = d.s;
+    const double zr = (1 - s.sigma * s.sigma) / (s.sigma * s.sigma);
+    if (kind_of(s.combo_key) == K_BCT) {
+        d.cyl_g0 = csx::ref_gamma(zr + 1);
+        d.cyl_g1 = csx::ref_gamma(zr - 1 / 2.0);
+        d.cyl_g2 = csx::ref_gamma(zr - 3 / 2.0);
     }
 }
 
@@ -1370,6 +1422,7 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
     HIPCHK(c, hipMemcpyAsync(c->carr.p, h_carr, 18 * CSX_NCOEF * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->set = DevSet{};
     c->set.s = *s;
+    set_consts(c->set);
     c->set.carr = (const double*)c->carr.p;
     if (h_carr11pm) {
         HIPCHK(c, c->carr11pm.ensure(CSX_NIM2 * CSX_NIM2 * sizeof(double)));
@@ -1535,6 +1588,7 @@ int csx_fit_batch_stage(csx_ctx* c, const csx_scalars* s, const double* h_carr, 
     const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
     FitSet f{};
     f.d.s = *s;
+    set_consts(f.d);
     f.m = *mask;
     f.data = d_data;
     f.w = w;

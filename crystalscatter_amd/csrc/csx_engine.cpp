@@ -30,6 +30,8 @@ struct csx_engine {
     size_t img_n = 0;
 };
 
+const csx::ParamSet* csx_engine_paramset(const csx_engine* e) { return e ? &e->p : nullptr; }
+
 extern "C" {
 
 csx_engine* csx_engine_new(int device) {

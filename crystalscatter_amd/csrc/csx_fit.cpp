@@ -1,0 +1,377 @@
+// Batched 2D fitting: many independent Nelder-Mead fits advanced in lockstep.
+//
+// The simplex is the one of SasCalc_SimplexFit2D::doSimplexFit2D
+// (Source/sc_simplexfit2d.cpp:254-611) with its trial step amotry (:221-251) and the bounds
+// rule of fehlerquadratsumme (:25-44), written as a resumable state machine: instead of
+// calling the objective, a fit posts the points it needs and resumes when their values
+// arrive.  The driver collects the points of every fit, prepares each parameter set on host
+// threads, stages them all with csx_fit_batch_stage and evaluates them in one
+// csx_fit_batch_run launch.  Independent evaluations of one fit (the initial vertices and a
+// shrink) are posted together; the order of those evaluations does not change any value.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "csx.h"
+#include "csx_prep.h"
+
+namespace {
+
+constexpr double kOutOfBounds = 1e20;
+
+class NelderMead {
+public:
+    void init(int n, const double* x0, const double* lo, const double* hi, double stp, int maxit, double ftol) {
+        n_ = n;
+        maxit_ = maxit;
+        ftol_ = ftol;
+        std::memcpy(lo_, lo, n * sizeof(double));
+        std::memcpy(hi_, hi, n * sizeof(double));
+        std::memset(p_, 0, sizeof p_);
+        std::memset(y_, 0, sizeof y_);
+        const double rootn1 = std::sqrt(n + 1.0) - 1.0, mroot2 = 1.0 / (n * std::sqrt(2.0));
+        double pp[CSX_NM_MAXPAR], qv[CSX_NM_MAXPAR];
+        for (int i = 0; i < n; ++i) {
+            double step;
+            if (x0[i] == 0) step = (lo[i] == 0) ? stp : lo[i] * stp;
+            else step = x0[i] * stp;
+            pp[i] = step * (rootn1 + n) * mroot2;
+            qv[i] = step * rootn1 * mroot2;
+            p_[0][i] = x0[i];
+        }
+        for (int i = 1; i <= n; ++i) {
+            for (int j = 0; j < n; ++j) p_[i][j] = p_[0][j] + qv[j];
+            p_[i][i - 1] = p_[i][i - 1] + pp[i - 1];
+        }
+        phase_ = kInit0;
+        rep_ = 0;
+        evals_ = 0;
+        done_ = false;
+        post({0});
+        drain();
+    }
+
+    bool done() const { return done_; }
+    int npending() const { return npend_; }
+    const double* pending(int k) const { return pts_[k]; }
+
+    // Values of the pending points (in posting order); advances to the next request.
+    void deliver(const double* f) {
+        evals_ += npend_;
+        double v[CSX_NM_MAXPAR + 1];
+        for (int k = 0; k < nall_; ++k) v[k] = kOutOfBounds;
+        for (int w = 0; w < npend_; ++w) v[slot_[w]] = f[w];
+        step(v);
+        drain();
+    }
+
+    void result(csx_nm_result& r) const {
+        for (int j = 0; j < n_; ++j) r.x[j] = p_[ilo_][j];
+        r.f = y_[ilo_];
+        r.iters = rep_;
+        r.evals = evals_;
+    }
+
+private:
+    enum Phase { kInit0, kInitV, kReflect, kExpand, kContract, kShrink };
+
+    // Post points: rows of p_ (ids >= 0) or the trial point (id -1).  Points outside the
+    // bounds are not evaluated: they get 1e20, as the reference's objective returns.
+    void post(std::initializer_list<int> ids) { post(std::vector<int>(ids)); }
+    void post(const std::vector<int>& ids) {
+        nall_ = (int)ids.size();
+        npend_ = 0;
+        for (int k = 0; k < nall_; ++k) {
+            const double* x = ids[k] < 0 ? pt_ : p_[ids[k]];
+            if (out_of_bounds(x)) continue;
+            std::memcpy(pts_[npend_], x, n_ * sizeof(double));
+            slot_[npend_++] = k;
+        }
+        resolve_now_ = npend_ == 0;
+    }
+
+    // A post with no point in bounds is answered at once.
+    void drain() {
+        while (resolve_now_ && !done_) {
+            resolve_now_ = false;
+            double v[CSX_NM_MAXPAR + 1];
+            for (int k = 0; k < nall_; ++k) v[k] = kOutOfBounds;
+            npend_ = 0;
+            step(v);
+        }
+    }
+
+    bool out_of_bounds(const double* x) const {
+        for (int j = 0; j < n_; ++j) {
+            if (lo_[j] == 0.0 && hi_[j] == 360.0) continue;
+            if (x[j] < lo_[j] || x[j] > hi_[j]) return true;
+        }
+        return false;
+    }
+
+    // amotry (Source/sc_simplexfit2d.cpp:221-251): trial point of factor fac.
+    void trial(double fac) {
+        const double f1 = (1.0 - fac) / n_, f2 = f1 - fac;
+        for (int j = 0; j < n_; ++j) pt_[j] = psum_[j] * f1 - p_[ihi_][j] * f2;
+    }
+    void accept_if_better(double yt) {
+        if (yt < y_[ihi_]) {
+            y_[ihi_] = yt;
+            for (int j = 0; j < n_; ++j) {
+                psum_[j] += pt_[j] - p_[ihi_][j];
+                p_[ihi_][j] = pt_[j];
+            }
+        }
+    }
+    void sum_rows() {
+        for (int j = 0; j < n_; ++j) {
+            double s = 0.0;
+            for (int i = 0; i <= n_; ++i) s += p_[i][j];
+            psum_[j] = s;
+        }
+    }
+
+    // Start of one simplex iteration (the body of the reference's `rep` loop).
+    void iterate() {
+        if (rep_ >= maxit_) { finish(); return; }
+        const int mpts = n_ + 1;
+        ilo_ = 0;
+        if (y_[0] > y_[1]) { ihi_ = 0; inhi_ = 1; }
+        else { ihi_ = 1; inhi_ = 0; }
+        for (int i = 0; i < mpts; ++i) {
+            if (y_[i] < y_[ilo_]) ilo_ = i;
+            if (y_[i] > y_[ihi_]) { inhi_ = ihi_; ihi_ = i; }
+            else if (y_[i] > y_[inhi_]) { if (i != ihi_) inhi_ = i; }
+        }
+        if (std::isnan(y_[ihi_]) || std::isnan(y_[ilo_]) || std::fabs(y_[ihi_]) + std::fabs(y_[ilo_]) == 0) {
+            finish();
+            return;
+        }
+        const double rtol = 2.0 * std::fabs(y_[ihi_] - y_[ilo_]) / (std::fabs(y_[ihi_]) + std::fabs(y_[ilo_]));
+        if (rtol <= ftol_) { finish(); return; }
+        trial(-1.0);  // reflection, alpha = 1
+        phase_ = kReflect;
+        post({-1});
+    }
+
+    void next_iteration() {
+        ++rep_;
+        iterate();
+    }
+
+    void step(const double* v) {
+        switch (phase_) {
+        case kInit0: {
+            y_[0] = v[0];
+            std::vector<int> ids;
+            for (int i = 1; i <= n_; ++i) ids.push_back(i);
+            phase_ = kInitV;
+            post(ids);
+            return;
+        }
+        case kInitV:
+            for (int i = 1; i <= n_; ++i) y_[i] = v[i - 1];
+            sum_rows();
+            rep_ = 0;
+            iterate();
+            return;
+        case kReflect: {
+            const double yt = v[0];
+            accept_if_better(yt);
+            if (yt <= y_[ilo_]) {
+                trial(2.0);  // expansion, gamma = 2
+                phase_ = kExpand;
+                post({-1});
+            } else if (yt >= y_[inhi_]) {
+                ysave_ = y_[ihi_];
+                trial(0.5);  // contraction, beta = 0.5
+                phase_ = kContract;
+                post({-1});
+            } else {
+                next_iteration();
+            }
+            return;
+        }
+        case kExpand:
+            accept_if_better(v[0]);
+            next_iteration();
+            return;
+        case kContract: {
+            const double yt = v[0];
+            accept_if_better(yt);
+            if (yt >= ysave_) {
+                std::vector<int> ids;
+                for (int i = 0; i <= n_; ++i) {
+                    if (i == ilo_) continue;
+                    for (int j = 0; j < n_; ++j) p_[i][j] = 0.5 * (p_[i][j] + p_[ilo_][j]);
+                    ids.push_back(i);
+                }
+                phase_ = kShrink;
+                post(ids);
+            } else {
+                next_iteration();
+            }
+            return;
+        }
+        case kShrink: {
+            int k = 0;
+            for (int i = 0; i <= n_; ++i)
+                if (i != ilo_) y_[i] = v[k++];
+            sum_rows();
+            next_iteration();
+            return;
+        }
+        }
+    }
+
+    void finish() {
+        done_ = true;
+        npend_ = 0;
+        nall_ = 0;
+    }
+
+    int n_ = 0, maxit_ = 0;
+    double ftol_ = 0.0, lo_[CSX_NM_MAXPAR], hi_[CSX_NM_MAXPAR];
+    double p_[CSX_NM_MAXPAR + 1][CSX_NM_MAXPAR], y_[CSX_NM_MAXPAR + 1], psum_[CSX_NM_MAXPAR], pt_[CSX_NM_MAXPAR];
+    double ysave_ = 0.0;
+    int ilo_ = 0, ihi_ = 0, inhi_ = 0, rep_ = 0, evals_ = 0;
+    Phase phase_ = kInit0;
+    bool done_ = false, resolve_now_ = false;
+    int nall_ = 0, npend_ = 0, slot_[CSX_NM_MAXPAR + 1];
+    double pts_[CSX_NM_MAXPAR + 1][CSX_NM_MAXPAR];
+};
+
+// Set one free parameter: a key may name several parameters separated by commas, which
+// all take the value (a = b of a tetragonal cell, for example).
+void set_tied(csx::ParamSet& ps, const char* key, double v) {
+    std::string k;
+    for (const char* c = key;; ++c) {
+        if (*c == ',' || *c == 0) {
+            if (!k.empty()) ps.set(k, v);
+            k.clear();
+            if (*c == 0) break;
+        } else {
+            k += *c;
+        }
+    }
+}
+
+// Run f over [0, n) on up to `threads` host threads.
+template <class F>
+void parallel_for(int n, int threads, F&& f) {
+    threads = std::max(1, std::min(threads, n));
+    if (threads == 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (int i = next++; i < n; i = next++) f(i);
+        });
+    for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const double* lo, const double* hi,
+                  double stp, int maxit, double ftol, csx_nm_result* out) {
+    if (npar < 1 || npar > CSX_NM_MAXPAR || !f || !x0 || !lo || !hi || !out) return CSX_E_ARG;
+    NelderMead nm;
+    nm.init(npar, x0, lo, hi, stp, maxit, ftol);
+    std::vector<double> vals;
+    while (!nm.done()) {
+        vals.resize(nm.npending());
+        for (int k = 0; k < nm.npending(); ++k) vals[k] = f(user, nm.pending(k));
+        nm.deliver(vals.data());
+    }
+    nm.result(*out);
+    return CSX_OK;
+}
+
+int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
+                 int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats) {
+    if (!ctx || nfits < 1 || !fits || npar < 1 || npar > CSX_NM_MAXPAR || !keys || !out) return CSX_E_ARG;
+    std::vector<NelderMead> nm(nfits);
+    std::vector<const csx::ParamSet*> base(nfits);
+    for (int i = 0; i < nfits; ++i) {
+        base[i] = csx_engine_paramset(fits[i].base);
+        if (!base[i]) return CSX_E_ARG;
+        nm[i].init(npar, fits[i].x0, fits[i].lo, fits[i].hi, stp, maxit, ftol);
+    }
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    struct Req {
+        int fit, k;
+        csx::PreparedSet P;
+        std::string err;
+    };
+    std::vector<Req> reqs;
+    std::vector<double> sums;
+    std::vector<int64_t> cnts;
+    int64_t rounds = 0, evals = 0;
+    double t_prep = 0, t_stage = 0, t_run = 0;
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    for (;;) {
+        reqs.clear();
+        for (int i = 0; i < nfits; ++i)
+            for (int k = 0; k < nm[i].npending(); ++k) reqs.push_back(Req{i, k, {}, {}});
+        if (reqs.empty()) break;
+        ++rounds;
+        evals += (int64_t)reqs.size();
+        const auto t0 = clk::now();
+        parallel_for((int)reqs.size(), threads, [&](int r) {
+            Req& q = reqs[r];
+            csx::ParamSet ps = *base[q.fit];
+            const double* x = nm[q.fit].pending(q.k);
+            for (int j = 0; j < npar; ++j) set_tied(ps, keys[j], x[j]);
+            q.err = csx::prepare(ps, q.P);
+        });
+        const auto t1 = clk::now();
+        int rc = csx_fit_batch_begin(ctx);
+        if (rc) return rc;
+        for (const Req& q : reqs) {
+            if (!q.err.empty()) return CSX_E_PARAM;
+            const csx_nm_fit& F = fits[q.fit];
+            rc = csx_fit_batch_stage(ctx, &q.P.s, q.P.carr.data(), q.P.latpar1.empty() ? nullptr : q.P.latpar1.data(),
+                                     q.P.latpar2.empty() ? nullptr : q.P.latpar2.data(),
+                                     q.P.latpar3.empty() ? nullptr : q.P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
+            if (rc) return rc;
+        }
+        sums.resize(reqs.size());
+        cnts.resize(reqs.size());
+        const auto t2 = clk::now();
+        rc = csx_fit_batch_run(ctx, precision, sums.data(), cnts.data());
+        if (rc) return rc;
+        const auto t3 = clk::now();
+        t_prep += ms(t0, t1);
+        t_stage += ms(t1, t2);
+        t_run += ms(t2, t3);
+        size_t r = 0;
+        for (int i = 0; i < nfits; ++i) {
+            const int np = nm[i].npending();
+            if (np == 0) continue;
+            nm[i].deliver(sums.data() + r);
+            r += np;
+        }
+    }
+    for (int i = 0; i < nfits; ++i) nm[i].result(out[i]);
+    if (stats) {
+        stats->rounds = rounds;
+        stats->evals = evals;
+        stats->prep_ms = t_prep;
+        stats->stage_ms = t_stage;
+        stats->run_ms = t_run;
+    }
+    return CSX_OK;
+}
+
+}  // extern "C"

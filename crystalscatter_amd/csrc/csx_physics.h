@@ -10,7 +10,7 @@
 //   fq_sphere     formfq_partSphere cs=0     Source/sc_lib_formfq_partSphere.h:8-52
 //   pq_cylinder   formpq_partCylinder        Source/sc_lib_formpq_partCylinder.h:8-257
 //   fq_cylinder   formfq_partCylinder ord 6  Source/sc_lib_formfq_partCylinder.h:8-206
-//   ref_gamma_d   gamma                      Source/sc_calc_generic_gpu.cu:613-673
+//   (Gamma values of the cylinder <F>^2 come from the host: csx::ref_gamma, same formula)
 //   pixel_q       convert_idx2q              Source/sc_calc_generic_gpu.cu:249-287
 #pragma once
 
@@ -65,6 +65,9 @@ struct DevSet {
     double kr_const;         // -4/(pi*width^2)*log2(e) when the width does not depend on q
     double inv_hpw;          // 1/(pi*width/2) when the width does not depend on q
     double two_pi_over_v;    // 2*pi/sphno_cubevol
+    // Gamma(zr+1), Gamma(zr-1/2), Gamma(zr-3/2) of the transverse cylinder <F>^2 (zr from
+    // sigma): per-set constants, evaluated once on the host with the reference's Gamma.
+    double cyl_g0, cyl_g1, cyl_g2;
     // Orientation quadrature of Gaussian-distributed cylinders in physics mode (built at
     // upload, see csx_device.hip): kCqDelta Gauss-Legendre polar nodes (cos, sin, weight),
     // kCqPsi/2 azimuth nodes t = cos(psi) of the midpoint rule on [0, pi], and the
@@ -206,45 +209,6 @@ __device__ inline double fq_sphere_fast(const DevSet& d, double q) {
     return pq3 * pq3;
 }
 
-__device__ inline double ref_gamma_d(double z) {
-    constexpr double bm[9] = {0.0, -0.577191652, 0.988205891, -0.897056937, 0.918206857,
-                              -0.756704078, 0.482199394, -0.193527818, 0.035868343};
-    constexpr double cm[27] = {
-        0.0, 1.0000000000000000, 0.5772156649015329, -0.6558780715202538, -0.0420026350340952,
-        0.1665386113822915, -0.0421977345555443, -0.0096219715278770, 0.00721894322466630,
-        -0.0011651675918591, -0.0002152416741149, 0.0001280502823882, -0.0000201348547807,
-        -0.0000012504934821, 0.0000011330272320, -0.0000002056338417, 0.0000000061160950,
-        0.0000000050020075, -0.0000000011812746, 0.0000000001043427, 0.0000000000077823,
-        -0.0000000000036968, 0.0000000000005100, -0.0000000000000206, -0.0000000000000054,
-        0.0000000000000014, 0.0000000000000001};
-    bool neg;
-    double ip;
-    if (fabs(z) < 0.000000001) z = z + 0.000000001;
-    if (z > 0) neg = false;
-    else { neg = true; if (fabs(modf(z, &ip)) < 0.000000001) z = z + 0.000000001; }
-    const double az = fabs(z);
-    if (az > 0.0 && az <= 1) {
-        double sum = 0;
-        for (int i = 1; i <= 26; ++i) sum += cm[i] * pow(az, (double)i);
-        return neg ? -kPiD * sum / (az * sin(kPiD * az)) : 1. / sum;
-    }
-    if (az > 1 && az <= 1000) {
-        if (fabs(modf(az, &ip)) < 0.000000001) {
-            double f = 1;
-            for (int i = 1; i <= trunc(az) - 1; ++i) f = f * i;
-            return neg ? -kPiD / (az * f * sin(kPiD * az)) : f;
-        }
-        const double y = az - 1, di = trunc(y), x = az - trunc(az);
-        double fak = y;
-        if (di == 0) fak = 1;
-        else for (int i = 1; i < di; ++i) fak = fak * (y - i);
-        double f = 1;
-        for (int i = 1; i <= 8; ++i) f = f + bm[i] * pow(x, (double)i);
-        return neg ? -kPiD / (az * fak * f * sin(kPiD * az)) : fak * f;
-    }
-    const double g = exp(-az) * exp((az - 0.5) * log(az)) * sqrt(2 * kPiD);
-    return neg ? -kPiD / (az * g * sin(kPiD * az)) : g;
-}
 
 // Schulz-averaged longitudinal P of a rod, argument (qxs+qys+eps9)*L/(zl+1).
 __device__ __forceinline__ double rod_p(double zl, double arg) {
@@ -294,10 +258,10 @@ __device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, 
     } else {
         const double argpq = q * s.radius / (zr + 1);
         const double at = atan(argpq), b = 1.0 + argpq * argpq;
-        const double g0 = ref_gamma_d(zr + 1);
-        const double pqr1 = (ref_gamma_d(zr - 1 / 2.0) / g0) * pow(argpq, -3 / 2.0) *
+        const double g0 = d.cyl_g0;
+        const double pqr1 = (d.cyl_g1 / g0) * pow(argpq, -3 / 2.0) *
                             (sin((zr - 1 / 2.0) * at) - cos((zr - 1 / 2.0) * at)) / pow(b, (zr - 1 / 2.0) / 2.0);
-        const double pqr2 = (ref_gamma_d(zr - 3 / 2.0) / g0) * pow(argpq, -5 / 2.0) *
+        const double pqr2 = (d.cyl_g2 / g0) * pow(argpq, -5 / 2.0) *
                             (sin((zr - 3 / 2.0) * at) + cos((zr - 3 / 2.0) * at)) / pow(b, (zr - 3 / 2.0) / 2.0);
         const double pqr3 = (2 / sqrt(kPiD)) * (pqr1 + (9 / 16.0) * pqr2);
         pqr = pqr3 * pqr3;

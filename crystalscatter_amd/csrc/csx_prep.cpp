@@ -175,8 +175,19 @@ void coef_sphere(PreparedSet& P, double z, double xr2z, CoefState& cs) {
     (void)P;
 }
 
+// The O(n^2) inner sums of the <F>^2 coefficients (carr1f, carr4f) depend only on the
+// Schulz parameters z and zl, not on the sizes: kept per thread for the last (z, zl), so a
+// sequence of preparations that varies R, L or the lattice (a fit) computes them once.  The
+// cached values are the same doubles the loop below would produce.
+struct FSums {
+    double z = -1.0, zl = -1.0;
+    double acc1[kNN + 1], acc4[kNN + 1];
+};
+
 // Cylinder, perfect orientation, homogeneous: Source/sc_calc_generic_cpu.cpp:4124-4212.
 void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState& cs) {
+    static thread_local FSums fs;
+    const bool cached = fs.z == z && fs.zl == zl;
     long double fkv[kNN + 2], fk2v[kNN + 1], gam3[kNN + 2], z12vl[kNN + 1];
     double z12v[kNN + 1];
     fkv[0] = 1; fk2v[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12vl[0] = 1; z12v[0] = 1.0;
@@ -195,14 +206,18 @@ void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState
         c1p[n] = std::sqrt(kPi) * std::pow(4.0, n) * z12vl[n] * xln / (2.0 * (n + 1) * gam3[n] * fkv[n]);
         c4p[n] = 4 * (n + 1 / 2.0) * fk2v[n] * z12v[n] * xrn /
                  ((n + 2) * (n + 1) * fkv[n] * fkv[n] * (n + 1) * fkv[n]);
-        double acc = 0.0;
-        for (int m = 0; m <= n; ++m)
-            acc += z12vl[m] * z12vl[n - m] / (gam3[m] * gam3[n - m] * fkv[m] * fkv[n - m] * fkv[m] * fkv[n - m]);
-        c1f[n] = kPi * xln * acc / 4.0;
-        acc = 0.0;
-        for (int m = 0; m <= n; ++m)
-            acc += z12v[m] * z12v[n - m] / ((m + 1) * fkv[m] * (n - m + 1) * fkv[n - m] * fkv[m] * fkv[n - m]);
-        c4f[n] = xrn * acc;
+        if (!cached) {
+            double acc = 0.0;
+            for (int m = 0; m <= n; ++m)
+                acc += z12vl[m] * z12vl[n - m] / (gam3[m] * gam3[n - m] * fkv[m] * fkv[n - m] * fkv[m] * fkv[n - m]);
+            fs.acc1[n] = acc;
+            acc = 0.0;
+            for (int m = 0; m <= n; ++m)
+                acc += z12v[m] * z12v[n - m] / ((m + 1) * fkv[m] * (n - m + 1) * fkv[n - m] * fkv[m] * fkv[n - m]);
+            fs.acc4[n] = acc;
+        }
+        c1f[n] = kPi * xln * fs.acc1[n] / 4.0;
+        c4f[n] = xrn * fs.acc4[n];
         // The reference's switch-point search stops at the first coefficient below 1e-50
         // with its sign, i.e. at the first negative term (n=1).
         if (find1 && c1p[n] < 1e-50) { cs.nidx[0] = n; find1 = false; }
@@ -210,6 +225,8 @@ void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState
         if (std::fabs(c1f[n]) < 1E-30 && n < cs.nidx[9]) cs.nidx[9] = n;
         if (std::fabs(c4f[n]) < 1E-30 && n < cs.nidx[12]) cs.nidx[12] = n;
     }
+    fs.z = z;
+    fs.zl = zl;
 }
 
 // Cylinder, Gaussian orientation, general case: Source/sc_calc_generic_cpu.cpp:4648-4720 and

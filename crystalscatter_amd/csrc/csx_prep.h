@@ -61,3 +61,7 @@ double ref_gamma(double z);
 double gauss_norm3(double a);
 
 }  // namespace csx
+
+// Parameters of a facade engine (csx_engine.cpp), for the batched fit driver.
+struct csx_engine;
+const csx::ParamSet* csx_engine_paramset(const csx_engine* e);

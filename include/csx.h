@@ -120,10 +120,11 @@ int csx_upload(csx_ctx* ctx, const csx_scalars* s, const double* h_carr, const d
 int csx_render(csx_ctx* ctx, const csx_grid* grid, int precision, double* d_out);
 
 /* csx_render plus work counters (synchronous; for measurement, no reference counterpart):
- * h_stats[0] pair evaluations (lanes x pairs iterated, mixed path), [1] pixels re-summed in
- * FP64 by the conditioning guard, [2] pixels with FP64 terms for |q^.u| near 1,
- * [3] waves that evaluated both Friedel terms. */
-#define CSX_NSTATS 4
+ * h_stats[0] pair evaluations (lanes x pairs iterated, mixed path), [1] pixels whose peak sum
+ * the conditioning guard re-took in FP64 over the window, [2] pixels with FP64 terms for
+ * |q^.u| near 1, [3] waves that evaluated both Friedel terms, [4] pixels re-summed in FP64
+ * over the full reflection list. */
+#define CSX_NSTATS 5
 int csx_render_stats(csx_ctx* ctx, const csx_grid* grid, int precision, double* d_out, uint64_t* h_stats);
 
 /* q vectors of the pixels of `grid` as the kernels generate them (convert_idx2q,
@@ -145,6 +146,43 @@ int csx_fit_batch_stage(csx_ctx* ctx, const csx_scalars* s, const double* h_carr
                         const int32_t* h_latpar1, const int32_t* h_latpar2, const float* h_latpar3,
                         int w, int h, const csx_mask* mask, const double* d_data);
 int csx_fit_batch_run(csx_ctx* ctx, int precision, double* h_sums, int64_t* h_cnts);
+
+/* ---- Batched Nelder-Mead fits ------------------------------------------------------
+ * The simplex of SasCalc_SimplexFit2D::doSimplexFit2D (Source/sc_simplexfit2d.cpp:254-611,
+ * amotry :221-251) with the bounds rule of fehlerquadratsumme (:25-44: a point outside
+ * [lo, hi] scores 1e20 without evaluation, except parameters with bounds exactly 0..360),
+ * run as a resumable state machine.  csx_nm_batch advances many independent fits in
+ * lockstep: every point they need in a round is prepared on host threads and evaluated with
+ * the batched objective in one launch.  It replaces the sequential fehlerquadratsumme calls. */
+#define CSX_NM_MAXPAR 16
+typedef double (*csx_objective)(void* user, const double* x);
+typedef struct csx_nm_result {
+    double x[CSX_NM_MAXPAR];  /* best vertex, p[ilo] as the reference returns it */
+    double f;                 /* its objective */
+    int32_t iters, evals;     /* simplex iterations, objective evaluations */
+} csx_nm_result;
+struct csx_engine;
+typedef struct csx_nm_fit {
+    const struct csx_engine* base;  /* parameters of this fit (host-only engine is enough) */
+    const double* d_data;           /* device: w*h doubles in the image's IDX layout */
+    int32_t w, h;
+    csx_mask mask;
+    double x0[CSX_NM_MAXPAR], lo[CSX_NM_MAXPAR], hi[CSX_NM_MAXPAR];
+} csx_nm_fit;
+/* One fit on a host objective (the same state machine; test and scripting use). */
+int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const double* lo, const double* hi,
+                  double stp, int maxit, double ftol, csx_nm_result* out);
+/* nfits fits over the npar parameters named in keys (a key "uca,ucb" sets both names to the
+ * parameter's value); threads <= 0: all host cores. */
+typedef struct csx_nm_stats {
+    int64_t rounds;           /* lockstep rounds (one batched launch each) */
+    int64_t evals;            /* objective evaluations over all fits */
+    double prep_ms;           /* host preparation of the parameter sets (threads) */
+    double stage_ms;          /* staging of tables and fit sets */
+    double run_ms;            /* batched launch, reductions and the copy of the sums */
+} csx_nm_stats;
+int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
+                 int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats);
 
 int csx_last_error(csx_ctx* ctx, char* buf, int len);
 /* Kernel time of the last csx_fit_fqs / csx_fit_batch_run, and of the last csx_render when
