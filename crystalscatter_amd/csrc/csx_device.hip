@@ -122,6 +122,16 @@ __device__ __forceinline__ float wave_max_f(float v) {
     return v;
 }
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Butterfly sums: every lane gets the total, in the same order on every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 
 // |q^.u| above this can put the reference's FP64 cosine above 1, where its acos mapping
 // returns pi (Source/sc_calc_generic_gpu.cu:31): such pairs take the FP64 formula.
@@ -323,23 +333,80 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
     return inten;
 }
 
-// FP64 peak sum over the pairs of a window, both reflections of each pair with the
-// reference's per-reflection formula (acos mapping included).  Differs from the reference's
-// sum only by the order of the terms and by the window's dropped mass.
-__device__ double window_sum64(const DevSet& d, const Window& W, const PixQ& P3, double q, double width) {
-    const double x2phi = 4 * q * q / (kPiD * sqd(d.s.phiwidth));
-    double acc = 0.0;
-    for (int band = W.band0; band <= W.band1; ++band) {
-        const int32_t* lb = d.lut + band * (d.nlut + 1);
-        const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
-        for (int p = p0; p < p1; ++p) {
-            const PairRec P = d.pairs[p];
-            if (P.ia == 0) continue;  // padding record
-            acc += aniso_term_ref(d, P.ia, P3.qx, P3.qy, P3.qz, q, width, x2phi) +
-                   aniso_term_ref(d, P.ib, P3.qx, P3.qy, P3.qz, q, width, x2phi);
+// Wave-cooperative FP64 re-sums.  Lanes whose FP32 peak sum is too coarse are taken one at
+// a time; the 64 lanes share that lane's reflections (stride 64) and a butterfly sum gives
+// the total, so a re-sum costs the wave about 1/64 of a serial one instead of stalling it.
+// Called by all 64 lanes; `want` is the per-lane request, `inten`/`radint` are replaced
+// for the requesting lanes only.
+struct PixArgs {
+    double qx, qy, qz, q, width;
+};
+__device__ __forceinline__ PixArgs shfl_args(const PixArgs& a, int lane) {
+    return PixArgs{__shfl(a.qx, lane, 64), __shfl(a.qy, lane, 64), __shfl(a.qz, lane, 64), __shfl(a.q, lane, 64),
+                   __shfl(a.width, lane, 64)};
+}
+
+// Tier 2: both reflections of every pair in the wave's window, with the reference's
+// per-reflection formula (acos mapping included).  Differs from the reference's sum by the
+// order of the terms and by the window's dropped mass only.
+__device__ void coop_window_sum64(const DevSet& d, const Window& W, const PixArgs& me, bool want, double& inten) {
+    unsigned long long m = __ballot(want);
+    const int lane = threadIdx.x & 63;
+    const double x2c = 4 / (kPiD * sqd(d.s.phiwidth));
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const PixArgs a = shfl_args(me, l);
+        const double x2phi = x2c * a.q * a.q;
+        double acc = 0.0;
+        for (int band = W.band0; band <= W.band1; ++band) {
+            const int32_t* lb = d.lut + band * (d.nlut + 1);
+            const int p0 = uniform_i(lb[W.b0] & ~3), p1 = uniform_i((lb[W.b1] + 3) & ~3);
+            for (int p = p0 + lane; p < p1; p += 64) {
+                const PairRec P = d.pairs[p];
+                if (P.ia == 0) continue;  // padding record
+                acc += aniso_term_ref(d, P.ia, a.qx, a.qy, a.qz, a.q, a.width, x2phi) +
+                       aniso_term_ref(d, P.ib, a.qx, a.qy, a.qz, a.q, a.width, x2phi);
+            }
+        }
+        acc = wave_sum(acc);
+        if (lane == l) inten = acc;
+    }
+}
+
+// Tier 3: the full reflection lists (lattice_ref64's terms).  A non-finite total takes the
+// serial reference loop, which stops at the first infinite partial sum.
+__device__ void coop_full_sum64(const DevSet& d, const PixArgs& me, bool want, double& inten, double& radint) {
+    unsigned long long m = __ballot(want);
+    const int lane = threadIdx.x & 63;
+    const csx_scalars& s = d.s;
+    const double widthiso = 1.0 / s.uca;
+    const double x2c = 4 / (kPiD * sqd(s.phiwidth));
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const PixArgs a = shfl_args(me, l);
+        const double x2phi = x2c * a.q * a.q;
+        double acc = 0.0, rad = 0.0;
+        for (int ii = 1 + lane; ii <= s.peakmax2; ii += 64)
+            acc += aniso_term_ref(d, ii, a.qx, a.qy, a.qz, a.q, a.width, x2phi);
+        if (s.iso != 0) {
+            for (int ii = 1 + lane; ii <= s.peakmax1; ii += 64) {
+                const int mhkl = d.latpar1[ii * 6 + 4], fhkl = d.latpar1[ii * 6 + 5];
+                if (mhkl == 0 || fhkl == 0) continue;
+                const double qhkl = d.latpar3[ii * 17 + 5];
+                const double x2 = sqd(a.q - qhkl) / sqd(widthiso);
+                rad += exp(-4 * x2 / kPiD) / (kPiD * widthiso / 2.0) * mhkl * fhkl;
+            }
+        }
+        acc = wave_sum(acc);
+        rad = wave_sum(rad);
+        if (lane == l) {
+            inten = acc;
+            radint = rad;
         }
     }
-    return acc;
+    if (want && !isfinite(inten)) lattice_ref64(d, me.qx, me.qy, me.qz, me.q, me.width, inten, radint);
 }
 
 // Isotropic (radial) peak sum in FP32, only needed when iso != 0.
@@ -701,17 +768,17 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 const double dsz = fabs(bs) * (rel * fabs(I) + dropped * inv_hpw);
                 return !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60));
             };
-            bool full = need_exact;
-            if (!full && loose(inten, 1.0e-5)) {
-                need_q(s, P3);
-                inten = window_sum64(d, W, P3, q, width);
-                if (stats) atomicAdd(&stats[1], 1ull);
-                full = loose(inten, 1.0e-13);
+            const bool want2 = !need_exact && loose(inten, 1.0e-5);
+            if (__ballot(want2)) {
+                if (want2) need_q(s, P3);
+                coop_window_sum64(d, W, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want2, inten);
+                if (stats && want2) atomicAdd(&stats[1], 1ull);
             }
-            if (full) {
-                need_q(s, P3);
-                lattice_ref64(d, P3.qx, P3.qy, P3.qz, q, width, inten, radint);
-                if (stats) atomicAdd(&stats[4], 1ull);
+            const bool want3 = need_exact || (want2 && loose(inten, 1.0e-13));
+            if (__ballot(want3)) {
+                if (want3) need_q(s, P3);
+                coop_full_sum64(d, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want3, inten, radint);
+                if (stats && want3) atomicAdd(&stats[4], 1ull);
             }
         }
         const double szq = (1 + fqpq * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
@@ -788,9 +855,8 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, int half,
 // Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.  rtab: radial
 // table of k_radial (midpoint grid, radial kinds) or null.
 template <int KIND, int PREC>
-__global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out,
-                                                  unsigned long long* __restrict__ stats,
-                                                  const Radial* __restrict__ rtab) {
+__device__ __forceinline__ void image_tile(const DevSet& d, const csx_grid& g, double* __restrict__ out,
+                                           unsigned long long* __restrict__ stats, const Radial* __restrict__ rtab) {
     // Every lane computes (the mixed path needs all 64 lanes of a wave); lanes past the grid
     // edge evaluate the last valid pixel, and only in-grid pixels are stored.
     int px, py;
@@ -822,6 +888,23 @@ __global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid
         const size_t W = (size_t)(g.xmax - g.xmin);
         out[(size_t)(oh - g.xmin) + W * (size_t)(oi - g.ymin)] = tile[r][cx];
     }
+}
+
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid g, double* __restrict__ out,
+                                                  unsigned long long* __restrict__ stats,
+                                                  const Radial* __restrict__ rtab) {
+    image_tile<KIND, PREC>(d, g, out, stats, rtab);
+}
+
+// Lattice kinds: the rarely taken wave-cooperative FP64 re-sums would otherwise set the
+// register budget of the whole kernel (about 140 VGPRs, 3 waves per SIMD); capped at 128
+// (4 waves per SIMD), only those cold paths spill.
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_image_lat(const DevSet d, const csx_grid g, double* __restrict__ out, unsigned long long* __restrict__ stats,
+            const Radial* __restrict__ rtab) {
+    image_tile<KIND, PREC>(d, g, out, stats, rtab);
 }
 
 __global__ void __launch_bounds__(kBlock) k_qgrid(const csx_scalars s, const csx_grid g, double* __restrict__ out) {
@@ -870,14 +953,6 @@ struct TileSum {
     int32_t cnt, nan;
 };
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // One fit tile: per-pixel FQSVERGL terms, NaN counted, block-reduced in FP64.
 template <int KIND, int PREC>
@@ -934,15 +1009,28 @@ __global__ void __launch_bounds__(kBlock) k_fit_one(const FitSet f, double* __re
     fit_tile<KIND, PREC>(f, blockIdx.x, fqs_out, out + blockIdx.x);
 }
 
-// Persistent batched objective: blocks stride over the flat tile list of all sets.
+// Batched objective: one launch over the flat tile list of all sets, one block per tile.
+// (A grid-stride loop here makes the compiler hoist the set's fields out of the loop into
+// vector registers: about 240 VGPRs and 2 waves per SIMD instead of 128 and 4.)
+template <int KIND, int PREC>
+__device__ __forceinline__ void fit_batch_tiles(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
+                                                int ntiles, TileSum* __restrict__ out) {
+    const int t = blockIdx.x;
+    if (t < ntiles) {
+        const int2 st = tiles[t];  // (set, tile within set)
+        fit_tile<KIND, PREC>(sets[st.x], st.y, nullptr, out + t);
+    }
+}
 template <int KIND, int PREC>
 __global__ void __launch_bounds__(kBlock) k_fit_batch(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
                                                       int ntiles, TileSum* __restrict__ out) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int2 st = tiles[t];  // (set, tile within set)
-        const FitSet& f = sets[st.x];
-        fit_tile<KIND, PREC>(f, st.y, nullptr, out + t);
-    }
+    fit_batch_tiles<KIND, PREC>(sets, tiles, ntiles, out);
+}
+// Lattice kinds, register budget capped as for k_image_lat.
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_fit_batch_lat(const FitSet* __restrict__ sets, const int2* __restrict__ tiles, int ntiles, TileSum* __restrict__ out) {
+    fit_batch_tiles<KIND, PREC>(sets, tiles, ntiles, out);
 }
 
 // Per-set sum of the tile partials (deterministic order).
@@ -1261,7 +1349,10 @@ static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsign
         rtab = nullptr;
     }
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
-    hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
+    if constexpr (K == K_FCC || K == K_BCT)
+        hipLaunchKernelGGL((k_image_lat<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
+    else
+        hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
 }
 
 int csx_qgrid(csx_ctx* c, const csx_grid* g, double* d_q3) {
@@ -1283,7 +1374,10 @@ static void launch_fit_one(const FitSet& f, int ntiles, double* fqs, TileSum* pa
 
 template <int K, int P>
 static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, TileSum* part, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_fit_batch<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
+    if constexpr (K == K_FCC || K == K_BCT)
+        hipLaunchKernelGGL((k_fit_batch_lat<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
+    else
+        hipLaunchKernelGGL((k_fit_batch<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
 }
 
 template <class T>
@@ -1556,6 +1650,108 @@ int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, c
 }
 
 // ---- batched objective ------------------------------------------------------
+}  // extern "C"
+
+// Append bytes at the next 256-byte boundary; returns their offset.
+static size_t arena_put(std::vector<char>& a, const void* p, size_t bytes) {
+    const size_t off = (a.size() + 255) & ~size_t(255);
+    a.resize(off + bytes);
+    if (bytes) std::memcpy(a.data() + off, p, bytes);
+    return off;
+}
+
+struct csx_staged {
+    struct Offsets { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_lut, off_singles, off_iso; };
+    FitSet f;
+    Offsets pd;
+    std::vector<char> arena;
+    int ntiles = 0;
+    int rc = CSX_OK;
+    std::string err;
+};
+csx_staged* csx_staged_new() { return new csx_staged(); }
+void csx_staged_free(csx_staged* s) { delete s; }
+
+// Host part of staging one set: checks, fast tables and the set's bytes, into `out`.
+// Touches no context state, so many sets can be built on different threads at once.
+void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr, const int32_t* h_l1,
+                     const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
+                     const double* d_data) {
+    csx_staged& st = *out;
+    st.rc = CSX_OK;
+    st.err.clear();
+    st.arena.clear();
+    auto bad = [&](int code, const char* msg) {
+        st.rc = code;
+        st.err = msg;
+    };
+    if (!s || !h_carr || !mask || !d_data || w <= 0 || h <= 0) return bad(CSX_E_ARG, "null argument or empty fit array");
+    if (kind_of(s->combo_key) < 0) return bad(CSX_E_ARG, "combination key not supported by the device path");
+    if (s->peakmax1 < 0 || s->peakmax2 < 0 || s->peakmax1 >= CSX_LATLEN || s->peakmax2 >= CSX_LATLEN)
+        return bad(CSX_E_ARG, "peakmax out of range");
+    if (s->peakmax1 > s->peakmax2 && s->peakmax1 > 0)
+        return bad(CSX_E_ARG, "isotropic list longer than the anisotropic table");
+    if (s->zmax <= 0) return bad(CSX_E_ARG, "zmax must be positive");
+    if ((s->xmax - s->xmin) != w || (long long)(s->xmax - s->xmin) * (s->ymax - s->ymin) < (long long)w * h)
+        return bad(CSX_E_ARG, "fit array must match the image width and fit inside the image");
+    if (s->iimin != s->xmin || s->iimin != s->ymin)
+        return bad(CSX_E_ARG, "fit needs the full (Q4) or first-quadrant (Q1) image layout");
+    const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
+    if (lattice && (!h_l1 || !h_l2 || !h_l3)) return bad(CSX_E_ARG, "null reflection tables");
+    FitSet& f = st.f;
+    f = FitSet{};
+    f.d.s = *s;
+    set_consts(f.d);
+    f.m = *mask;
+    f.data = d_data;
+    f.w = w;
+    f.h = h;
+    f.tiles_x = (w + kTile - 1) / kTile;
+    st.ntiles = f.tiles_x * ((h + kTile - 1) / kTile);
+    auto& pd = st.pd;
+    pd = csx_staged::Offsets{};
+    pd.off_carr = arena_put(st.arena, h_carr, 18 * CSX_NCOEF * sizeof(double));
+    if (lattice) {
+        pd.off_l1 = arena_put(st.arena, h_l1, (size_t)(s->peakmax1 + 1) * 6 * 4);
+        pd.off_l2 = arena_put(st.arena, h_l2, (size_t)(s->peakmax2 + 1) * 6 * 4);
+        pd.off_l3 = arena_put(st.arena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
+        FastTables ft;
+        build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
+        pd.off_pairs = arena_put(st.arena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec));
+        pd.off_lut = arena_put(st.arena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
+        pd.off_singles = arena_put(st.arena, ft.singles.data(), ft.singles.size() * sizeof(int32_t));
+        pd.off_iso = arena_put(st.arena, ft.iso.data(), ft.iso.size() * sizeof(float2));
+        set_fast(f.d, ft);
+    }
+}
+
+// Append a built set to the context's pending batch (serial; cheap copies only).
+int csx_stage_commit(csx_ctx* c, const csx_staged* stp) {
+    if (!c || !stp) return CSX_E_ARG;
+    const csx_staged& st = *stp;
+    if (st.rc != CSX_OK) return fail(c, st.rc, st.err);
+    if (!c->bsets.empty() && kind_of(c->bsets[0].d.s.combo_key) != kind_of(st.f.d.s.combo_key))
+        return fail(c, CSX_E_ARG, "all sets of a batch must share the combination");
+    const size_t base = arena_put(c->barena, st.arena.data(), st.arena.size());  // 256-aligned
+    csx_ctx::Pending pd{};
+    pd.off_carr = base + st.pd.off_carr;
+    pd.off_l1 = base + st.pd.off_l1;
+    pd.off_l2 = base + st.pd.off_l2;
+    pd.off_l3 = base + st.pd.off_l3;
+    pd.off_pairs = base + st.pd.off_pairs;
+    pd.off_lut = base + st.pd.off_lut;
+    pd.off_singles = base + st.pd.off_singles;
+    pd.off_iso = base + st.pd.off_iso;
+    FitSet f = st.f;
+    f.first_tile = (int)c->btiles.size();
+    for (int k = 0; k < st.ntiles; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
+    c->bsets.push_back(f);
+    c->bntiles.push_back(st.ntiles);
+    c->bpend.push_back(pd);
+    return CSX_OK;
+}
+
+extern "C" {
 int csx_fit_batch_begin(csx_ctx* c) {
     if (!c) return CSX_E_ARG;
     c->bsets.clear();
@@ -1566,58 +1762,15 @@ int csx_fit_batch_begin(csx_ctx* c) {
     return CSX_OK;
 }
 
-static size_t arena_put(std::vector<char>& a, const void* p, size_t bytes) {
-    const size_t off = (a.size() + 255) & ~size_t(255);
-    a.resize(off + bytes);
-    if (bytes) std::memcpy(a.data() + off, p, bytes);
-    return off;
-}
 
 int csx_fit_batch_stage(csx_ctx* c, const csx_scalars* s, const double* h_carr, const int32_t* h_l1,
                         const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
                         const double* d_data) {
-    if (!c || !h_carr || !mask || !d_data || w <= 0 || h <= 0) return CSX_E_ARG;
-    int rc = check_scalars(c, s);
-    if (rc) return rc;
-    if (!c->bsets.empty() && kind_of(c->bsets[0].d.s.combo_key) != kind_of(s->combo_key))
-        return fail(c, CSX_E_ARG, "all sets of a batch must share the combination");
-    if ((s->xmax - s->xmin) != w || (long long)(s->xmax - s->xmin) * (s->ymax - s->ymin) < (long long)w * h)
-        return fail(c, CSX_E_ARG, "fit array must match the image width and fit inside the image");
-    if (s->iimin != s->xmin || s->iimin != s->ymin)
-        return fail(c, CSX_E_ARG, "fit needs the full (Q4) or first-quadrant (Q1) image layout");
-    const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
-    FitSet f{};
-    f.d.s = *s;
-    set_consts(f.d);
-    f.m = *mask;
-    f.data = d_data;
-    f.w = w;
-    f.h = h;
-    f.tiles_x = (w + kTile - 1) / kTile;
-    const int nt = f.tiles_x * ((h + kTile - 1) / kTile);
-    f.first_tile = (int)c->btiles.size();
-    csx_ctx::Pending pd{};
-    pd.off_carr = arena_put(c->barena, h_carr, 18 * CSX_NCOEF * sizeof(double));
-    if (lattice) {
-        if (!h_l1 || !h_l2 || !h_l3) return fail(c, CSX_E_ARG, "null reflection tables");
-        pd.off_l1 = arena_put(c->barena, h_l1, (size_t)(s->peakmax1 + 1) * 6 * 4);
-        pd.off_l2 = arena_put(c->barena, h_l2, (size_t)(s->peakmax2 + 1) * 6 * 4);
-        pd.off_l3 = arena_put(c->barena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
-        FastTables ft;
-        build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
-        pd.off_pairs = arena_put(c->barena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec));
-        pd.off_lut = arena_put(c->barena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
-        pd.off_singles = arena_put(c->barena, ft.singles.data(), ft.singles.size() * sizeof(int32_t));
-        pd.off_iso = arena_put(c->barena, ft.iso.data(), ft.iso.size() * sizeof(float2));
-        set_fast(f.d, ft);
-    }
-    for (int k = 0; k < nt; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
-    c->bsets.push_back(f);
-    c->bntiles.push_back(nt);
-    c->bpend.push_back(pd);
-    return CSX_OK;
+    if (!c) return CSX_E_ARG;
+    csx_staged st;
+    csx_stage_build(&st, s, h_carr, h_l1, h_l2, h_l3, w, h, mask, d_data);
+    return csx_stage_commit(c, &st);
 }
-
 
 int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts) {
     if (!c || !h_sums) return CSX_E_ARG;
@@ -1651,9 +1804,7 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
     HIPCHK(c, hipMemcpyAsync(c->bsets_d.p, c->bsets.data(), nsets * sizeof(FitSet), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->btiles_d.p, c->btiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->bntiles_d.p, c->bntiles.data(), nsets * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    const int nblocks = std::min(ntiles, cus * 8);
+    const int nblocks = ntiles;  // one block per tile
     const int kind = kind_of(c->bsets[0].d.s.combo_key);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     DISPATCH_KIND(kind, precision, launch_fit_batch, (const FitSet*)c->bsets_d.p, (const int2*)c->btiles_d.p,

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -314,6 +315,15 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, cons
         std::string err;
     };
     std::vector<Req> reqs;
+    // one staging slot per request, reused across rounds
+    struct Slot {
+        csx_staged* p = csx_staged_new();
+        ~Slot() { csx_staged_free(p); }
+        Slot() = default;
+        Slot(const Slot&) = delete;
+        Slot& operator=(const Slot&) = delete;
+    };
+    std::vector<std::unique_ptr<Slot>> slots;
     std::vector<double> sums;
     std::vector<int64_t> cnts;
     int64_t rounds = 0, evals = 0;
@@ -327,23 +337,27 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, cons
         if (reqs.empty()) break;
         ++rounds;
         evals += (int64_t)reqs.size();
+        while (slots.size() < reqs.size()) slots.push_back(std::make_unique<Slot>());
         const auto t0 = clk::now();
+        // preparation and the host part of staging, one vertex per task
         parallel_for((int)reqs.size(), threads, [&](int r) {
             Req& q = reqs[r];
             csx::ParamSet ps = *base[q.fit];
             const double* x = nm[q.fit].pending(q.k);
             for (int j = 0; j < npar; ++j) set_tied(ps, keys[j], x[j]);
             q.err = csx::prepare(ps, q.P);
+            if (!q.err.empty()) return;
+            const csx_nm_fit& F = fits[q.fit];
+            csx_stage_build(slots[r]->p, &q.P.s, q.P.carr.data(), q.P.latpar1.empty() ? nullptr : q.P.latpar1.data(),
+                            q.P.latpar2.empty() ? nullptr : q.P.latpar2.data(),
+                            q.P.latpar3.empty() ? nullptr : q.P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
         });
         const auto t1 = clk::now();
         int rc = csx_fit_batch_begin(ctx);
         if (rc) return rc;
-        for (const Req& q : reqs) {
-            if (!q.err.empty()) return CSX_E_PARAM;
-            const csx_nm_fit& F = fits[q.fit];
-            rc = csx_fit_batch_stage(ctx, &q.P.s, q.P.carr.data(), q.P.latpar1.empty() ? nullptr : q.P.latpar1.data(),
-                                     q.P.latpar2.empty() ? nullptr : q.P.latpar2.data(),
-                                     q.P.latpar3.empty() ? nullptr : q.P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
+        for (size_t r = 0; r < reqs.size(); ++r) {
+            if (!reqs[r].err.empty()) return CSX_E_PARAM;
+            rc = csx_stage_commit(ctx, slots[r]->p);
             if (rc) return rc;
         }
         sums.resize(reqs.size());

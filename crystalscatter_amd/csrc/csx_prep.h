@@ -65,3 +65,15 @@ double gauss_norm3(double a);
 // Parameters of a facade engine (csx_engine.cpp), for the batched fit driver.
 struct csx_engine;
 const csx::ParamSet* csx_engine_paramset(const csx_engine* e);
+
+// Staging of one set of a batched objective in two steps (csx_device.hip):
+// csx_stage_build does the host work (checks, fast peak tables, the set's bytes) and touches
+// no context, so the batched fit driver runs it on many threads; csx_stage_commit appends
+// the result to the context's pending batch.  csx_fit_batch_stage is the two in a row.
+struct csx_staged;
+csx_staged* csx_staged_new();
+void csx_staged_free(csx_staged* s);
+void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr, const int32_t* h_l1,
+                     const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
+                     const double* d_data);
+int csx_stage_commit(csx_ctx* c, const csx_staged* st);

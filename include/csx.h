@@ -177,8 +177,8 @@ int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const
 typedef struct csx_nm_stats {
     int64_t rounds;           /* lockstep rounds (one batched launch each) */
     int64_t evals;            /* objective evaluations over all fits */
-    double prep_ms;           /* host preparation of the parameter sets (threads) */
-    double stage_ms;          /* staging of tables and fit sets */
+    double prep_ms;           /* host preparation of the parameter sets and their peak tables (threads) */
+    double stage_ms;          /* appending the built sets to the batch (serial) */
     double run_ms;            /* batched launch, reductions and the copy of the sums */
 } csx_nm_stats;
 int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
