@@ -111,7 +111,21 @@ SIGNATURES = [
     ("csx_engine_scalars", ctypes.POINTER(Scalars), [_VP]),
     ("csx_engine_ctx", _VP, [_VP]),
     ("csx_engine_last_error", _I, [_VP, _CP, _I]),
+    ("csx_mlp_nparams", ctypes.c_int64, []),
+    ("csx_mlp_create", _I, [_I, _I, ctypes.c_uint64, ctypes.POINTER(_VP)]),
+    ("csx_mlp_destroy", None, [_VP]),
+    ("csx_mlp_set_stream", _I, [_VP, _VP]),
+    ("csx_mlp_last_error", _I, [_VP, _CP, _I]),
+    ("csx_mlp_get_params", _I, [_VP, _PF, _PF]),
+    ("csx_mlp_set_params", _I, [_VP, _PF, _PF]),
+    ("csx_mlp_forward", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP]),
+    ("csx_mlp_gradient", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _PF]),
+    ("csx_mlp_get_grads", _I, [_VP, _PF]),
+    ("csx_mlp_adam", _I, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("csx_mlp_train_step", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP, ctypes.c_float, _PF]),
+    ("csx_mlp_synchronize", _I, [_VP]),
 ]
+MLP_NPAR, MLP_NFREQ, MLP_IN, MLP_IN_PAD, MLP_HIDDEN, MLP_LAYERS = 6, 32, 70, 128, 256, 4
 
 
 def lib():

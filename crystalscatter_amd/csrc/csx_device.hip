@@ -45,6 +45,11 @@ inline int kind_of(uint32_t key) {
     default: return -1;
     }
 }
+// Kinds whose mixed peak sum runs the precise pair loop (FP64 exponents): their structure
+// factor 1 + (fq/pq)(2*pi*I/V - 1)*dwe cancels over much of the pattern (fq/pq reaches 1e8
+// for the z-oriented cylinder), so the FP32 loop's ~1e-5 would send a third of the pixels
+// to the FP64 re-sum.
+constexpr bool precise_kind(int kind) { return kind == K_BCT; }
 
 // ---------------------------------------------------------------------------
 // Gaussian peak sums
@@ -133,6 +138,13 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 }
 
 
+// Relative error bounds of the two pair loops' sums, used by the conditioning guard.  FP32
+// loop: an exponent e carries ~7 ulp relative error plus the FP32 direction error of the
+// angular part, about 1e-5 of a term at the 2^-32 cut.  Precise loop: the FP32 exp2 of the
+// fraction (1 ulp), its conversion and the polynomial (2.4e-11 of y^2): 5e-7 with margin.
+constexpr double kRelErrFP32 = 1.0e-5;
+constexpr double kRelErrPrecise = 5.0e-7;
+
 // |q^.u| above this can put the reference's FP64 cosine above 1, where its acos mapping
 // returns pi (Source/sc_calc_generic_gpu.cu:31): such pairs take the FP64 formula.
 constexpr float kNearOne = 0.999998f;
@@ -218,6 +230,83 @@ __device__ double near_one_terms(const DevSet& d, int p0, int p1, const PixelCon
     return add;
 }
 
+// acos(1-x)^2 / x for x in [0, 1], degree-11 polynomial (Chebyshev-node interpolant of the
+// function, which is analytic on [0, 2) with value 2 at x = 0); max relative error 2.4e-11.
+__device__ __forceinline__ double acos2_over_x(double x) {
+    double p = 5.8703930721466823e-04;
+    p = fma(p, x, -2.2269864280509148e-03);
+    p = fma(p, x, 4.3534013060061065e-03);
+    p = fma(p, x, -4.3610819883067170e-03);
+    p = fma(p, x, 3.8076579843268261e-03);
+    p = fma(p, x, 9.4124350172984761e-05);
+    p = fma(p, x, 4.2622239283164089e-03);
+    p = fma(p, x, 1.0083515220786222e-02);
+    p = fma(p, x, 2.8579409148493790e-02);
+    p = fma(p, x, 8.8888454904565387e-02);
+    p = fma(p, x, 3.3333334251157515e-01);
+    p = fma(p, x, 1.9999999999680802e+00);
+    return p;
+}
+
+// 2^e for e <= 0 in FP64 exponent arithmetic with an FP32 exp2 of the fraction: relative
+// error about 1 ulp of FP32 whatever |e| (the FP32 loop's error grows with |e|).
+__device__ __forceinline__ double exp2_split(double e) {
+    const double n = rint(e);
+    const float f = (float)(e - n);  // |f| <= 1/2
+    return (double)ldexpf(__builtin_amdgcn_exp2f(f), (int)fmax(n, -200.0));
+}
+
+// Per-pixel constants of the precise pair loop.
+struct PixelConsts64 {
+    double hx, hy, hz;  // q / |q| (|q| with the reference's +1e-9)
+    double q, kr, kp;   // |q|, radial and angular base-2 exponent coefficients
+};
+
+// The precise pair loop: the FP32 loop's sum with FP64 exponents.  The reference's own
+// cosine c = q.q_hkl / (|q| |q_hkl|) is formed in FP64 and y^2 = acos(c)^2 = x*g(x), x = 1-c,
+// needs no cancellation-free rewrite in FP64.  Same window, same near-one rule.
+template <bool PLANAR, bool TWO>
+__device__ __forceinline__ void pair_sum64(const PairRec64* __restrict__ pr, int p0, int p1, const PixelConsts64& L,
+                                           double& acc, bool& flag) {
+    constexpr double kPi = 3.14159265358979323846;
+    for (int p = p0; p < p1; ++p) {
+        const PairRec64 P = pr[p];
+        double dt = fma(L.hy, P.uy, L.hx * P.ux);
+        if constexpr (!PLANAR) dt = fma(L.hz, P.uz, dt);
+        const double c = fabs(dt);
+        const double x = 1.0 - c;
+        const double y2 = x * acos2_over_x(x);
+        const double t = L.q - (double)P.qh;
+        const double r = (t * L.kr) * t;
+        const double e1 = fma(L.kp, y2, r);
+        double w = exp2_split(e1);
+        if constexpr (TWO) {
+            // partner at pi - y: e2 = e1 + kp*pi*(pi - 2y)
+            const double y = (double)__builtin_amdgcn_sqrtf((float)y2);
+            w += exp2_split(fma(L.kp * kPi, kPi - 2.0 * y, e1));
+        }
+        const bool fl = c > (double)kNearOne;
+        flag = flag || fl;
+        acc = fma(P.A, fl ? 0.0 : w, acc);
+    }
+}
+
+// near_one_terms for the precise loop (its cosine test in FP64).
+template <bool PLANAR>
+__device__ double near_one_terms64(const DevSet& d, int p0, int p1, const PixelConsts64& L, const PixQ& P3, double q,
+                                   double width, double x2phi) {
+    double add = 0.0;
+    for (int p = p0; p < p1; ++p) {
+        const PairRec64 P = d.pairs64[p];
+        double dt = fma(L.hy, P.uy, L.hx * P.ux);
+        if constexpr (!PLANAR) dt = fma(L.hz, P.uz, dt);
+        if (fabs(dt) > (double)kNearOne)
+            add += aniso_term_ref(d, P.ia, P3.qx, P3.qy, P3.qz, q, width, x2phi) +
+                   aniso_term_ref(d, P.ib, P3.qx, P3.qy, P3.qz, q, width, x2phi);
+    }
+    return add;
+}
+
 // Pair-window bookkeeping of one wave: |q| bins [b0, b1] and |u_z| bands [band0, band1].
 struct Window {
     int b0, b1, band0, band1;
@@ -227,7 +316,7 @@ struct Window {
 // called by all 64 lanes of a wave: the wave's union of per-lane windows selects the pairs.
 // Pairs outside the window, and second Friedel terms where they cannot reach 2^cut, are
 // dropped: every dropped term is below 2^cut * A.  need_exact: unpaired reflections exist.
-template <bool PLANAR>
+template <bool PLANAR, bool PRECISE>
 __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, double q, double width, bool& need_exact,
                                 double& dropped, Window& W, unsigned long long* stats) {
     const csx_scalars& s = d.s;
@@ -303,6 +392,17 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
     }
     if (!any_two)
         dropped += 0.5 * d.asum2 * (double)__builtin_amdgcn_exp2f(fmaxf(L.kp * (hpi * hpi), -126.0f));
+    PixelConsts64 L64;
+    if constexpr (PRECISE) {
+        need_q(s, P3);
+        const double rq = 1.0 / q;
+        L64.hx = P3.qx * rq;
+        L64.hy = P3.qy * rq;
+        L64.hz = P3.qz * rq;
+        L64.q = q;
+        L64.kr = krd;
+        L64.kp = kpd;
+    }
     double acc = 0.0;
     bool flag = false;
     long long evals = 0;
@@ -310,8 +410,13 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
         const int32_t* lb = d.lut + band * (d.nlut + 1);
         const int p0 = uniform_i(lb[W.b0] & ~3);
         const int p1 = uniform_i((lb[W.b1] + 3) & ~3);
-        if (any_two) pair_sum<PLANAR, true>(d.pairs, p0, p1, L, acc, flag);
-        else pair_sum<PLANAR, false>(d.pairs, p0, p1, L, acc, flag);
+        if constexpr (PRECISE) {
+            if (any_two) pair_sum64<PLANAR, true>(d.pairs64, p0, p1, L64, acc, flag);
+            else pair_sum64<PLANAR, false>(d.pairs64, p0, p1, L64, acc, flag);
+        } else {
+            if (any_two) pair_sum<PLANAR, true>(d.pairs, p0, p1, L, acc, flag);
+            else pair_sum<PLANAR, false>(d.pairs, p0, p1, L, acc, flag);
+        }
         evals += p1 - p0;
     }
     double inten = acc * inv_hpw;
@@ -321,7 +426,8 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
         for (int band = W.band0; band <= W.band1; ++band) {
             const int32_t* lb = d.lut + band * (d.nlut + 1);
             const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
-            inten += near_one_terms<PLANAR>(d, p0, p1, L, P3, q, width, x2phi);
+            if constexpr (PRECISE) inten += near_one_terms64<PLANAR>(d, p0, p1, L64, P3, q, width, x2phi);
+            else inten += near_one_terms<PLANAR>(d, p0, p1, L, P3, q, width, x2phi);
         }
     }
     if (stats && (threadIdx.x & 63) == 0) {
@@ -752,12 +858,14 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
             bool need_exact = false;
             double dropped = 0.0;
             Window W;
-            inten = s.centerbeam ? lattice_mixed<false>(d, P3, q, width, need_exact, dropped, W, stats)
-                                 : lattice_mixed<true>(d, P3, q, width, need_exact, dropped, W, stats);
+            constexpr bool kPrecise = precise_kind(KIND);
+            inten = s.centerbeam ? lattice_mixed<false, kPrecise>(d, P3, q, width, need_exact, dropped, W, stats)
+                                 : lattice_mixed<true, kPrecise>(d, P3, q, width, need_exact, dropped, W, stats);
             radint = (s.iso != 0) ? iso_mixed(d, q) : 0.0;
             // Conditioning guard.  szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
-            // The FP32 peak sum I carries up to ~1e-5 relative error plus this lane's dropped
-            // mass; szq amplifies an error of I by |B|/|szq|.  A pixel where szq could move by
+            // The peak sum I carries up to kRelErrFP32 (FP32 loop) or kRelErrPrecise (precise
+            // loop) relative error plus this lane's dropped mass; szq amplifies an error of I
+            // by |B|/|szq|.  A pixel where szq could move by
             // more than 3e-5 relative (and 2^-60 absolute, below the rounding of szq's own
             // FP64 terms) takes the same window in FP64, whose only error is the dropped mass;
             // if that is still too coarse, the reference's full FP64 sum, in its order.
@@ -768,7 +876,7 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 const double dsz = fabs(bs) * (rel * fabs(I) + dropped * inv_hpw);
                 return !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60));
             };
-            const bool want2 = !need_exact && loose(inten, 1.0e-5);
+            const bool want2 = !need_exact && loose(inten, kPrecise ? kRelErrPrecise : kRelErrFP32);
             if (__ballot(want2)) {
                 if (want2) need_q(s, P3);
                 coop_window_sum64(d, W, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want2, inten);
@@ -1079,6 +1187,7 @@ constexpr int kLutBins = 256;
 // |q_hkl|, the bin table, unpaired reflections and the isotropic list.
 struct FastTables {
     std::vector<PairRec> pairs;
+    std::vector<PairRec64> pairs64;  // built when the kind sums in the precise loop
     std::vector<int32_t> lut;
     std::vector<int32_t> singles;
     std::vector<float2> iso;
@@ -1089,9 +1198,11 @@ struct FastTables {
 
 void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L2, const float* L3, FastTables& ft) {
     ft = FastTables{};
+    const bool precise = precise_kind(kind_of(s.combo_key));
     const int n2 = s.peakmax2;
     std::vector<char> used(n2 + 1, 0);
     std::vector<PairRec> found;
+    std::vector<PairRec64> found64;
     std::vector<int> band_of;
     auto valid = [&](int ii) {
         const double qh = L3[ii * 17 + 5];
@@ -1126,6 +1237,16 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
         const double A = (double)(L2[ii * 6 + 4] * L2[ii * 6 + 5]) * (double)L3[ii * 17 + 10] / (double)L3[ii * 17 + 11];
         const double ux = (double)L3[ii * 17 + 2], uy = (double)L3[ii * 17 + 3], uz = (double)L3[ii * 17 + 4];
         const double un = std::sqrt(ux * ux + uy * uy + uz * uz);
+        PairRec64 r64;
+        r64.ux = ux / qh;
+        r64.uy = uy / qh;
+        r64.uz = uz / qh;
+        r64.A = A;
+        r64.qh = L3[ii * 17 + 5];
+        r64.ia = ii;
+        r64.ib = jj;
+        r64.pad = 0.0f;
+        found64.push_back(r64);
         PairRec r;
         r.ux = (float)(ux / un);
         r.uy = (float)(uy / un);
@@ -1152,16 +1273,23 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
     const double dqb = 1.0 / (double)ft.lut_inv_dq;
     ft.lut.assign((size_t)(kBands + 1) * (kLutBins + 1), 0);
     const PairRec inert{0.0f, 0.0f, 0.0f, 1.0e30f, 0.0f, 0.0f, 0, 0};
+    const PairRec64 inert64{0.0, 0.0, 0.0, 0.0, 1.0e30f, 0, 0, 0.0f};
     // Segments 0..kBands-1: one |u_z| band each; segment kBands: every pair.
     for (int b = 0; b <= kBands; ++b) {
-        std::vector<PairRec> band;
+        std::vector<int> band;
         for (size_t k = 0; k < found.size(); ++k)
-            if (b == kBands || band_of[k] == b) band.push_back(found[k]);
-        std::stable_sort(band.begin(), band.end(), [](const PairRec& x, const PairRec& y) { return x.qh < y.qh; });
+            if (b == kBands || band_of[k] == b) band.push_back((int)k);
+        std::stable_sort(band.begin(), band.end(), [&](int x, int y) { return found[x].qh < found[y].qh; });
         const int start = (int)ft.pairs.size();  // a multiple of 4
-        for (const PairRec& r : band) ft.pairs.push_back(r);
+        for (int k : band) {
+            ft.pairs.push_back(found[k]);
+            if (precise) ft.pairs64.push_back(found64[k]);
+        }
         const int end = (int)ft.pairs.size();
-        while (ft.pairs.size() % 4) ft.pairs.push_back(inert);
+        while (ft.pairs.size() % 4) {
+            ft.pairs.push_back(inert);
+            if (precise) ft.pairs64.push_back(inert64);
+        }
         int32_t* lb = ft.lut.data() + (size_t)b * (kLutBins + 1);
         int idx = start;
         for (int k = 0; k < kLutBins; ++k) {
@@ -1312,14 +1440,14 @@ struct csx_ctx {
     std::string err;
     bool uploaded = false;
     DevSet set{};
-    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, lut, singles, iso, tiles, fqs_tmp, stats, radial, cqdl, cqtt,
+    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, pairs64, lut, singles, iso, tiles, fqs_tmp, stats, radial, cqdl, cqtt,
         cqtab;
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
     std::vector<int2> btiles;
     std::vector<char> barena;  // host image of the device arena (tables of all sets)
-    struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_lut, off_singles, off_iso; };
+    struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso; };
     std::vector<Pending> bpend;
     DevBuf barena_d, bsets_d, btiles_d, bpart_d, bntiles_d, bsums_d, bcnts_d;
 };
@@ -1441,7 +1569,7 @@ void csx_destroy(csx_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->lut, &c->singles,
+    for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial, &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
                       &c->bsums_d, &c->bcnts_d})
@@ -1539,11 +1667,13 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
         c->set.latpar2 = (const int32_t*)c->lat2.p;
         c->set.latpar3 = (const float*)c->lat3.p;
         rc = put_table(c, c->pairs, ft.pairs);
+        if (!rc && !ft.pairs64.empty()) rc = put_table(c, c->pairs64, ft.pairs64);
         if (!rc) rc = put_table(c, c->lut, ft.lut);
         if (!rc) rc = put_table(c, c->singles, ft.singles);
         if (!rc) rc = put_table(c, c->iso, ft.iso);
         if (rc) return rc;
         c->set.pairs = (const PairRec*)c->pairs.p;
+        c->set.pairs64 = ft.pairs64.empty() ? nullptr : (const PairRec64*)c->pairs64.p;
         c->set.lut = (const int32_t*)c->lut.p;
         c->set.singles = (const int32_t*)c->singles.p;
         c->set.iso = (const float2*)c->iso.p;
@@ -1661,7 +1791,7 @@ static size_t arena_put(std::vector<char>& a, const void* p, size_t bytes) {
 }
 
 struct csx_staged {
-    struct Offsets { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_lut, off_singles, off_iso; };
+    struct Offsets { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso; };
     FitSet f;
     Offsets pd;
     std::vector<char> arena;
@@ -1718,6 +1848,8 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
         FastTables ft;
         build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
         pd.off_pairs = arena_put(st.arena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec));
+        pd.off_pairs64 = ft.pairs64.empty() ? SIZE_MAX
+                                            : arena_put(st.arena, ft.pairs64.data(), ft.pairs64.size() * sizeof(PairRec64));
         pd.off_lut = arena_put(st.arena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
         pd.off_singles = arena_put(st.arena, ft.singles.data(), ft.singles.size() * sizeof(int32_t));
         pd.off_iso = arena_put(st.arena, ft.iso.data(), ft.iso.size() * sizeof(float2));
@@ -1739,6 +1871,7 @@ int csx_stage_commit(csx_ctx* c, const csx_staged* stp) {
     pd.off_l2 = base + st.pd.off_l2;
     pd.off_l3 = base + st.pd.off_l3;
     pd.off_pairs = base + st.pd.off_pairs;
+    pd.off_pairs64 = st.pd.off_pairs64 == SIZE_MAX ? SIZE_MAX : base + st.pd.off_pairs64;
     pd.off_lut = base + st.pd.off_lut;
     pd.off_singles = base + st.pd.off_singles;
     pd.off_iso = base + st.pd.off_iso;
@@ -1788,6 +1921,7 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
             f.d.latpar2 = (const int32_t*)(base + pd.off_l2);
             f.d.latpar3 = (const float*)(base + pd.off_l3);
             f.d.pairs = (const PairRec*)(base + pd.off_pairs);
+            f.d.pairs64 = pd.off_pairs64 == SIZE_MAX ? nullptr : (const PairRec64*)(base + pd.off_pairs64);
             f.d.lut = (const int32_t*)(base + pd.off_lut);
             f.d.singles = (const int32_t*)(base + pd.off_singles);
             f.d.iso = (const float2*)(base + pd.off_iso);

@@ -1,0 +1,644 @@
+// Surrogate network of the fits: a small MLP that maps a parameter set and a pixel's (qx, qy)
+// to the normalised log-intensity of that pixel.  It has no counterpart in the reference; the
+// fit driver uses it to propose starting simplices and to pre-screen vertices, and the exact
+// kernels re-score every point it accepts.
+//
+//   input   6 normalised parameters + 64 Fourier features sin/cos(2*pi*B.q), B 32x2 fixed
+//           (70 values, zero-padded to 128 columns)
+//   hidden  4 x 256, SiLU
+//   output  1 (linear)
+//
+// Precision: BF16 weights and activations on the matrix cores (v_mfma_f32_16x16x32_bf16),
+// FP32 accumulation, FP32 master weights and Adam state.  Every matrix product is one
+// "NT" kernel, C[m][n] = sum_k A[m][k] * B[n][k] with both operands k-contiguous:
+//   forward   Z_l  = H_{l-1} . W_l^T      A = H_{l-1} [M][K], B = W_l   [N][K] (out x in)
+//   backward  dH   = dZ_l . W_l          A = dZ_l [M][N],    B = W_l^T [K][N]
+//   weights   dW_l = dZ_l^T . H_{l-1}    A = dZ_l^T [N][M],  B = H_{l-1}^T [K][M], split over M
+// so the forward pass also stores H^T and the backward pass dZ^T (bf16), and each weight is
+// kept in bf16 in both orientations.  Per-sample buffers are sized for max_batch rows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "csx.h"
+
+namespace csm {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = CSX_MLP_IN_PAD;    // padded input width
+constexpr int kUsed = CSX_MLP_IN;      // used inputs: 6 parameters + 64 Fourier features
+constexpr int kH = CSX_MLP_HIDDEN;     // hidden width
+constexpr int kL = CSX_MLP_LAYERS;     // hidden layers
+constexpr int kNF = CSX_MLP_NFREQ;     // Fourier frequencies
+constexpr int kNP = CSX_MLP_NPAR;      // parameters per set
+constexpr int kBM = 128, kBN = 128, kBK = 64, kLd = kBK + 8;  // GEMM tile, LDS row stride (bf16)
+constexpr int kHeadRows = 64;           // rows per block of the output-layer kernels
+
+// Offsets of the FP32 master parameters (one buffer): hidden layer l has W_l [kH][in_l]
+// (out x in) then b_l [kH]; then the output weights w [kH] and bias.
+__host__ __device__ constexpr int in_of(int l) { return l == 0 ? kIn : kH; }
+__host__ __device__ constexpr int64_t w_off(int l) {
+    return l == 0 ? 0 : (int64_t)kH * kIn + kH + (int64_t)(l - 1) * ((int64_t)kH * kH + kH);
+}
+__host__ __device__ constexpr int64_t b_off(int l) { return w_off(l) + (int64_t)kH * in_of(l); }
+constexpr int64_t kOutW = w_off(kL);
+constexpr int64_t kOutB = kOutW + kH;
+constexpr int64_t kNParams = kOutB + 1;
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+    const float s = sigm(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// ---- input features ----------------------------------------------------------------
+// Row i < n: parameters of set[i] (already normalised by the caller; an index outside
+// [0, nsets) is clamped, the host checks them), then sin and cos of
+// 2*pi*(B q_i); rows n..mpad-1 are zero.  Writes X [mpad][kIn] and X^T [kIn][mpad].
+__global__ void __launch_bounds__(256) k_embed(int n, int mpad, int nsets, const float* __restrict__ par,
+                                               const int32_t* __restrict__ set, const float* __restrict__ q,
+                                               const float* __restrict__ fb, bf16* __restrict__ X,
+                                               bf16* __restrict__ XT) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= mpad) return;
+    float v[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) v[k] = 0.0f;
+    if (i < n) {
+        const float* p = par + (size_t)min(max(set[i], 0), nsets - 1) * kNP;  // clamped: never out of bounds
+#pragma unroll
+        for (int k = 0; k < kNP; ++k) v[k] = p[k];
+        const float qx = q[2 * i], qy = q[2 * i + 1];
+#pragma unroll
+        for (int f = 0; f < kNF; ++f) {
+            float s, c;
+            __sincosf(6.283185307f * (fb[2 * f] * qx + fb[2 * f + 1] * qy), &s, &c);
+            v[kNP + f] = s;
+            v[kNP + kNF + f] = c;
+        }
+    }
+    bf16* row = X + (size_t)i * kIn;
+#pragma unroll
+    for (int k = 0; k < kIn; k += 8) {
+        bf16x8 pk;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pk[j] = (bf16)v[k + j];
+        *(bf16x8*)(row + k) = pk;
+    }
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) XT[(size_t)k * mpad + i] = (bf16)v[k];
+}
+
+// ---- the NT matrix product with its epilogues ---------------------------------------
+enum Epi { EPI_FWD = 0, EPI_BWD = 1, EPI_PART = 2 };
+struct EpiArgs {
+    const float* bias;   // FWD: b_l
+    bf16* z;             // FWD: Z_l [M][N] out
+    bf16* h;             // FWD: H_l [M][N] out
+    bf16* ht;            // FWD: H_l^T [N][M] out
+    const bf16* zprev;   // BWD: Z_{l-1} [M][N]
+    bf16* dz;            // BWD: dZ_{l-1} [M][N] out
+    bf16* dzt;           // BWD: dZ_{l-1}^T [N][M] out
+    float* part;         // PART: partials [split][M][N]
+    int ldo;             // leading dimension of [M][N] outputs (N)
+    int ldt;             // leading dimension of the transposed outputs (M)
+};
+
+// Grid (M/128, N/128, splits); 256 threads, 4 waves in 2x2, each wave a 64x64 block of
+// 4x4 MFMA tiles.  Requires M, N multiples of 128, K and kchunk multiples of 64, lda/ldb
+// multiples of 8 (checked by the host).
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nt(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
+                                                 int ldb, int K, int kchunk, EpiArgs ep) {
+    __shared__ bf16 sA[kBM * kLd];
+    __shared__ bf16 sB[kBN * kLd];
+    const int tm = blockIdx.x * kBM, tn = blockIdx.y * kBN;
+    const int k0 = blockIdx.z * kchunk, k1 = min(K, k0 + kchunk);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;  // fragment row, k offset
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kk = k0; kk < k1; kk += kBK) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int idx = t + r * 256;
+            const int row = idx >> 3, c8 = (idx & 7) * 8;
+            *(uint4*)&sA[row * kLd + c8] = *(const uint4*)&A[(size_t)(tm + row) * lda + kk + c8];
+            *(uint4*)&sB[row * kLd + c8] = *(const uint4*)&B[(size_t)(tn + row) * ldb + kk + c8];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kBK; ks += 32) {
+            bf16x8 af[4], bfg[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8*)&sA[(wm + i * 16 + fr) * kLd + ks + fk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bfg[j] = *(const bf16x8*)&sB[(wn + j * 16 + fr) * kLd + ks + fk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfg[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // accumulator (i, j)[r] is C[tm + wm + 16i + 4(lane/16) + r][tn + wn + 16j + lane%16]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m0 = tm + wm + i * 16 + (lane >> 4) * 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = tn + wn + j * 16 + fr;
+            if constexpr (EPI == EPI_PART) {
+                float* out = ep.part + ((size_t)blockIdx.z * gridDim.x * kBM) * ep.ldo;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) out[(size_t)(m0 + r) * ep.ldo + n] = acc[i][j][r];
+            } else {
+                bf16 tcol[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const size_t o = (size_t)(m0 + r) * ep.ldo + n;
+                    if constexpr (EPI == EPI_FWD) {
+                        const float z = acc[i][j][r] + ep.bias[n];
+                        const float hv = z * sigm(z);
+                        ep.z[o] = (bf16)z;
+                        ep.h[o] = (bf16)hv;
+                        tcol[r] = (bf16)hv;
+                    } else {
+                        const float dz = acc[i][j][r] * silu_grad((float)ep.zprev[o]);
+                        ep.dz[o] = (bf16)dz;
+                        tcol[r] = (bf16)dz;
+                    }
+                }
+                bf16* tp = (EPI == EPI_FWD ? ep.ht : ep.dzt) + (size_t)n * ep.ldt + m0;
+                *(uint2*)tp = *(const uint2*)tcol;  // 4 consecutive rows, 8 bytes
+            }
+        }
+    }
+}
+
+// ---- output layer --------------------------------------------------------------------
+// y[m] = H4[m] . w + b for rows m < mpad; one wave per row.
+__global__ void __launch_bounds__(256) k_head_fwd(int mpad, const bf16* __restrict__ H, const float* __restrict__ w,
+                                                  const float* __restrict__ b, float* __restrict__ y) {
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (m >= mpad) return;
+    const bf16* row = H + (size_t)m * kH + lane * 4;
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s = fmaf((float)row[c], w[lane * 4 + c], s);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[m] = s + b[0];
+}
+
+// Loss 1/n sum (y - t)^2 and the output layer's backward for a block of kHeadRows rows:
+// dy = 2 (y - t) / n (0 for rows >= n), dZ4[m][c] = dy * w[c] * silu'(Z4[m][c]) written in
+// both layouts, and per-block partials of dw (sum dy * H4), db and the loss.
+__global__ void __launch_bounds__(256) k_head_bwd(int n, int mpad, const float* __restrict__ y,
+                                                  const float* __restrict__ tgt, const bf16* __restrict__ Z,
+                                                  const bf16* __restrict__ H, const float* __restrict__ w,
+                                                  bf16* __restrict__ dZ, bf16* __restrict__ dZT,
+                                                  float* __restrict__ part_w, float* __restrict__ part_b) {
+    __shared__ bf16 tile[kHeadRows][kH + 8];
+    __shared__ float s_dy[kHeadRows];
+    const int m0 = blockIdx.x * kHeadRows, t = threadIdx.x;
+    if (t < kHeadRows) {
+        const int m = m0 + t;
+        s_dy[t] = (m < n) ? 2.0f * (y[m] - tgt[m]) / (float)n : 0.0f;
+    }
+    __syncthreads();
+    // thread t owns column c = t: its dw partial and the transposed store
+    const int c = t;
+    const float wc = w[c];
+    float dw = 0.0f;
+    for (int r = 0; r < kHeadRows; ++r) {
+        const size_t o = (size_t)(m0 + r) * kH + c;
+        const float dy = s_dy[r];
+        dw = fmaf(dy, (float)H[o], dw);
+        const float dz = dy * wc * silu_grad((float)Z[o]);
+        tile[r][c] = (bf16)dz;
+        dZ[o] = tile[r][c];
+    }
+    part_w[(size_t)blockIdx.x * kH + c] = dw;
+    // column c of the tile is 64 consecutive rows of dZ^T
+    bf16* dst = dZT + (size_t)c * mpad + m0;
+#pragma unroll
+    for (int r = 0; r < kHeadRows; r += 8) {
+        bf16x8 pk;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pk[j] = tile[r + j][c];
+        *(bf16x8*)(dst + r) = pk;
+    }
+    if (t == 0) {
+        float db = 0.0f, loss = 0.0f;
+        for (int r = 0; r < kHeadRows; ++r) {
+            db += s_dy[r];
+            const int m = m0 + r;
+            if (m < n) loss += (y[m] - tgt[m]) * (y[m] - tgt[m]);
+        }
+        part_b[2 * blockIdx.x] = db;
+        part_b[2 * blockIdx.x + 1] = loss;
+    }
+}
+
+// g[c] = sum over s < ns of part[s*ld + c]  (fixed order: deterministic)
+__global__ void k_sum_parts(int ncols, int ns, int ld, const float* __restrict__ part, float* __restrict__ g) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    float s = 0.0f;
+    for (int k = 0; k < ns; ++k) s += part[(size_t)k * ld + c];
+    g[c] = s;
+}
+
+// Bias gradient of a hidden layer: db[c] = sum_m dZ^T[c][m], one block per column.
+__global__ void __launch_bounds__(256) k_rowsum(int mpad, const bf16* __restrict__ dZT, float* __restrict__ db) {
+    __shared__ float red[256];
+    const bf16* row = dZT + (size_t)blockIdx.x * mpad;
+    float s = 0.0f;
+    for (int m = threadIdx.x; m < mpad; m += 256) s += (float)row[m];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[blockIdx.x] = red[0];
+}
+
+// Adam on the FP32 master parameters, then the bf16 copies of the hidden weights in both
+// orientations (W [out][in] and W^T [in][out]).
+__global__ void __launch_bounds__(256) k_adam(int64_t np, float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m1, float* __restrict__ m2, float lr, float b1,
+                                              float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    p[i] -= lr * (a * c1) / (sqrtf(v * c2) + eps);
+}
+__global__ void __launch_bounds__(256) k_cast_weights(const float* __restrict__ p, bf16* __restrict__ wb,
+                                                      bf16* __restrict__ wtb) {
+    // flat index over all hidden weights, layer by layer
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (int l = 0; l < kL; ++l) {
+        const int64_t cnt = (int64_t)kH * in_of(l);
+        if (i < cnt) {
+            const int o = (int)(i / in_of(l)), k = (int)(i % in_of(l));
+            const bf16 v = (bf16)p[w_off(l) + i];
+            const int64_t base = l == 0 ? 0 : (int64_t)kH * kIn + (int64_t)(l - 1) * kH * kH;
+            wb[base + i] = v;
+            wtb[base + (int64_t)k * kH + o] = v;
+            return;
+        }
+        i -= cnt;
+    }
+}
+
+}  // namespace csm
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace csm;
+
+struct csx_mlp {
+    int device = 0;
+    int max_batch = 0;  // rows, a multiple of 128
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    float fb[2 * kNF];
+    // parameters
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr, *fbd = nullptr;
+    bf16 *wb = nullptr, *wtb = nullptr;  // hidden weights, [out][in] and [in][out]
+    int64_t adam_t = 0;
+    // per-row buffers
+    bf16 *X = nullptr, *XT = nullptr;
+    bf16 *Z[kL] = {}, *H[kL] = {}, *HT[kL] = {};
+    bf16 *dZ[2] = {}, *dZT[2] = {};
+    float *y = nullptr, *part = nullptr, *hpart_w = nullptr, *hpart_b = nullptr;
+    int splits = 0;
+};
+
+namespace {
+
+int fail(csx_mlp* m, int code, const std::string& msg) {
+    if (m) m->err = msg;
+    return code;
+}
+#define MCHK(m, call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(m, CSX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+const bf16* wb_layer(const csx_mlp* m, int l) {
+    return m->wb + (l == 0 ? 0 : (int64_t)kH * kIn + (int64_t)(l - 1) * kH * kH);
+}
+const bf16* wtb_layer(const csx_mlp* m, int l) {
+    return m->wtb + (l == 0 ? 0 : (int64_t)kH * kIn + (int64_t)(l - 1) * kH * kH);
+}
+
+// splitmix64 -> uniform doubles, Box-Muller normals (host-side initialisation only)
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+int cast_weights(csx_mlp* m) {
+    const int64_t nw = (int64_t)kH * kIn + (int64_t)(kL - 1) * kH * kH;
+    hipLaunchKernelGGL(k_cast_weights, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, m->stream, m->p, m->wb, m->wtb);
+    MCHK(m, hipGetLastError());
+    return CSX_OK;
+}
+
+int check_rows(csx_mlp* m, int n, int nsets) {
+    if (n < 1 || n > m->max_batch) return fail(m, CSX_E_ARG, "row count must be in [1, max_batch]");
+    if (nsets < 1) return fail(m, CSX_E_ARG, "need at least one parameter set");
+    return CSX_OK;
+}
+
+// Forward pass over n rows (padded to a multiple of 128) into m->y.
+int forward(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q) {
+    const int mpad = (n + kBM - 1) / kBM * kBM;
+    hipLaunchKernelGGL(k_embed, dim3((mpad + 255) / 256), dim3(256), 0, m->stream, n, mpad, nsets, d_par, d_set, d_q, m->fbd,
+                       m->X, m->XT);
+    MCHK(m, hipGetLastError());
+    for (int l = 0; l < kL; ++l) {
+        EpiArgs ep{};
+        ep.bias = m->p + b_off(l);
+        ep.z = m->Z[l];
+        ep.h = m->H[l];
+        ep.ht = m->HT[l];
+        ep.ldo = kH;
+        ep.ldt = mpad;
+        const bf16* A = l == 0 ? m->X : m->H[l - 1];
+        const int K = in_of(l);
+        hipLaunchKernelGGL(k_gemm_nt<EPI_FWD>, dim3(mpad / kBM, kH / kBN, 1), dim3(256), 0, m->stream, A, K,
+                           wb_layer(m, l), K, K, K, ep);
+        MCHK(m, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_head_fwd, dim3(mpad / 4), dim3(256), 0, m->stream, mpad, m->H[kL - 1], m->p + kOutW,
+                       m->p + kOutB, m->y);
+    MCHK(m, hipGetLastError());
+    return CSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t csx_mlp_nparams(void) { return kNParams; }
+
+int csx_mlp_create(int device, int max_batch, uint64_t seed, csx_mlp** out) {
+    if (!out || max_batch < 1 || max_batch > (1 << 24)) return CSX_E_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return CSX_E_HIP;
+    csx_mlp* m = new csx_mlp();
+    m->device = device;
+    m->max_batch = (max_batch + kBM - 1) / kBM * kBM;
+    const size_t M = (size_t)m->max_batch;
+    // weight-gradient splits over the rows: enough blocks to fill the chip, >= 512 rows each
+    m->splits = (int)std::max<size_t>(1, std::min<size_t>(64, M / 512));
+    auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
+    bool ok = hipStreamCreate(&m->stream) == hipSuccess;
+    m->own_stream = ok;
+    const int64_t nw = (int64_t)kH * kIn + (int64_t)(kL - 1) * kH * kH;
+    ok = ok && alloc((void**)&m->p, kNParams * 4) && alloc((void**)&m->g, kNParams * 4) &&
+         alloc((void**)&m->m1, kNParams * 4) && alloc((void**)&m->m2, kNParams * 4) &&
+         alloc((void**)&m->fbd, 2 * kNF * 4) && alloc((void**)&m->wb, nw * 2) && alloc((void**)&m->wtb, nw * 2) &&
+         alloc((void**)&m->X, M * kIn * 2) && alloc((void**)&m->XT, M * kIn * 2) && alloc((void**)&m->y, M * 4) &&
+         alloc((void**)&m->part, (size_t)m->splits * kH * kH * 4) &&
+         alloc((void**)&m->hpart_w, (M / kHeadRows) * kH * 4) && alloc((void**)&m->hpart_b, (M / kHeadRows) * 2 * 4);
+    for (int l = 0; l < kL && ok; ++l)
+        ok = alloc((void**)&m->Z[l], M * kH * 2) && alloc((void**)&m->H[l], M * kH * 2) &&
+             alloc((void**)&m->HT[l], M * kH * 2);
+    for (int k = 0; k < 2 && ok; ++k) ok = alloc((void**)&m->dZ[k], M * kH * 2) && alloc((void**)&m->dZT[k], M * kH * 2);
+    if (!ok) {
+        csx_mlp_destroy(m);
+        return CSX_E_NOMEM;
+    }
+    // Initialisation: He-uniform hidden weights (fan-in = used inputs), zero biases, a
+    // small output layer; Fourier frequencies with log-uniform magnitude in [0.5, 16]
+    // cycles per nm^-1 and uniform directions.
+    std::vector<float> hp(kNParams, 0.0f);
+    Rng r{seed};
+    for (int l = 0; l < kL; ++l) {
+        const int fan = l == 0 ? kUsed : kH;
+        const double bound = std::sqrt(6.0 / fan);
+        for (int o = 0; o < kH; ++o)
+            for (int k = 0; k < in_of(l); ++k)
+                hp[w_off(l) + (int64_t)o * in_of(l) + k] = (l == 0 && k >= kUsed) ? 0.0f : (float)((2 * r.uni() - 1) * bound);
+    }
+    for (int c = 0; c < kH; ++c) hp[kOutW + c] = (float)((2 * r.uni() - 1) * std::sqrt(3.0 / kH));
+    for (int f = 0; f < kNF; ++f) {
+        const double mag = 0.5 * std::pow(32.0, r.uni()), ang = 2 * M_PI * r.uni();
+        m->fb[2 * f] = (float)(mag * std::cos(ang));
+        m->fb[2 * f + 1] = (float)(mag * std::sin(ang));
+    }
+    ok = hipMemcpyAsync(m->p, hp.data(), kNParams * 4, hipMemcpyHostToDevice, m->stream) == hipSuccess &&
+         hipMemcpyAsync(m->fbd, m->fb, sizeof m->fb, hipMemcpyHostToDevice, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->m1, 0, kNParams * 4, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->m2, 0, kNParams * 4, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->g, 0, kNParams * 4, m->stream) == hipSuccess && cast_weights(m) == CSX_OK &&
+         hipStreamSynchronize(m->stream) == hipSuccess;
+    if (!ok) {
+        csx_mlp_destroy(m);
+        return CSX_E_HIP;
+    }
+    *out = m;
+    return CSX_OK;
+}
+
+void csx_mlp_destroy(csx_mlp* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* q : {(void*)m->p, (void*)m->g, (void*)m->m1, (void*)m->m2, (void*)m->fbd, (void*)m->wb, (void*)m->wtb,
+                    (void*)m->X, (void*)m->XT, (void*)m->y, (void*)m->part, (void*)m->hpart_w, (void*)m->hpart_b})
+        if (q) (void)hipFree(q);
+    for (int l = 0; l < kL; ++l)
+        for (void* q : {(void*)m->Z[l], (void*)m->H[l], (void*)m->HT[l]})
+            if (q) (void)hipFree(q);
+    for (int k = 0; k < 2; ++k)
+        for (void* q : {(void*)m->dZ[k], (void*)m->dZT[k]})
+            if (q) (void)hipFree(q);
+    if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int csx_mlp_last_error(const csx_mlp* m, char* buf, int len) {
+    if (!m || !buf || len <= 0) return CSX_E_ARG;
+    std::snprintf(buf, (size_t)len, "%s", m->err.c_str());
+    return CSX_OK;
+}
+
+int csx_mlp_set_stream(csx_mlp* m, void* stream) {
+    if (!m) return CSX_E_ARG;
+    MCHK(m, hipStreamSynchronize(m->stream));
+    if (m->own_stream) (void)hipStreamDestroy(m->stream);
+    m->own_stream = false;
+    m->stream = (hipStream_t)stream;
+    return CSX_OK;
+}
+
+int csx_mlp_get_params(csx_mlp* m, float* h_params, float* h_fourier) {
+    if (!m) return CSX_E_ARG;
+    MCHK(m, hipSetDevice(m->device));
+    if (h_params) MCHK(m, hipMemcpyAsync(h_params, m->p, kNParams * 4, hipMemcpyDeviceToHost, m->stream));
+    if (h_fourier) std::memcpy(h_fourier, m->fb, sizeof m->fb);
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return CSX_OK;
+}
+
+int csx_mlp_set_params(csx_mlp* m, const float* h_params, const float* h_fourier) {
+    if (!m) return CSX_E_ARG;
+    MCHK(m, hipSetDevice(m->device));
+    if (h_params) {
+        // the padded input columns of the first layer stay zero
+        std::vector<float> hp(h_params, h_params + kNParams);
+        for (int o = 0; o < kH; ++o)
+            for (int k = kUsed; k < kIn; ++k) hp[w_off(0) + (int64_t)o * kIn + k] = 0.0f;
+        MCHK(m, hipMemcpyAsync(m->p, hp.data(), kNParams * 4, hipMemcpyHostToDevice, m->stream));
+        MCHK(m, hipStreamSynchronize(m->stream));
+        int rc = cast_weights(m);
+        if (rc) return rc;
+    }
+    if (h_fourier) {
+        std::memcpy(m->fb, h_fourier, sizeof m->fb);
+        MCHK(m, hipMemcpyAsync(m->fbd, m->fb, sizeof m->fb, hipMemcpyHostToDevice, m->stream));
+    }
+    m->adam_t = 0;
+    MCHK(m, hipMemsetAsync(m->m1, 0, kNParams * 4, m->stream));
+    MCHK(m, hipMemsetAsync(m->m2, 0, kNParams * 4, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return CSX_OK;
+}
+
+int csx_mlp_forward(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                    float* d_y) {
+    if (!m || !d_par || !d_set || !d_q || !d_y) return CSX_E_ARG;
+    int rc = check_rows(m, n, nsets);
+    if (rc) return rc;
+    MCHK(m, hipSetDevice(m->device));
+    rc = forward(m, n, nsets, d_par, d_set, d_q);
+    if (rc) return rc;
+    MCHK(m, hipMemcpyAsync(d_y, m->y, (size_t)n * 4, hipMemcpyDeviceToDevice, m->stream));
+    return CSX_OK;
+}
+
+int csx_mlp_gradient(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                     const float* d_t, float* h_loss) {
+    if (!m || !d_par || !d_set || !d_q || !d_t) return CSX_E_ARG;
+    int rc = check_rows(m, n, nsets);
+    if (rc) return rc;
+    MCHK(m, hipSetDevice(m->device));
+    rc = forward(m, n, nsets, d_par, d_set, d_q);
+    if (rc) return rc;
+    const int mpad = (n + kBM - 1) / kBM * kBM;
+    const int nhb = mpad / kHeadRows;
+    // output layer: dZ of the last hidden layer, dw, db, loss
+    hipLaunchKernelGGL(k_head_bwd, dim3(nhb), dim3(256), 0, m->stream, n, mpad, m->y, d_t, m->Z[kL - 1], m->H[kL - 1],
+                       m->p + kOutW, m->dZ[0], m->dZT[0], m->hpart_w, m->hpart_b);
+    MCHK(m, hipGetLastError());
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, m->stream, kH, nhb, kH, m->hpart_w, m->g + kOutW);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, m->stream, 1, nhb, 2, m->hpart_b, m->g + kOutB);
+    MCHK(m, hipGetLastError());
+    // hidden layers, top down; dZ of layer l is in buffer cur
+    int cur = 0;
+    for (int l = kL - 1; l >= 0; --l) {
+        const int K = in_of(l);
+        // weight gradient dW_l [kH][K] = dZ_l^T . H_{l-1}, split over the rows
+        const int kchunk = ((mpad / m->splits + kBK - 1) / kBK) * kBK;
+        const int ns = (mpad + kchunk - 1) / kchunk;
+        EpiArgs pe{};
+        pe.part = m->part;
+        pe.ldo = K;
+        const bf16* BT = l == 0 ? m->XT : m->HT[l - 1];
+        hipLaunchKernelGGL(k_gemm_nt<EPI_PART>, dim3(kH / kBM, K / kBN, ns), dim3(256), 0, m->stream, m->dZT[cur], mpad,
+                           BT, mpad, mpad, kchunk, pe);
+        MCHK(m, hipGetLastError());
+        hipLaunchKernelGGL(k_sum_parts, dim3((kH * K + 255) / 256), dim3(256), 0, m->stream, kH * K, ns, kH * K, m->part,
+                           m->g + w_off(l));
+        hipLaunchKernelGGL(k_rowsum, dim3(kH), dim3(256), 0, m->stream, mpad, m->dZT[cur], m->g + b_off(l));
+        MCHK(m, hipGetLastError());
+        if (l == 0) break;
+        // dH_{l-1} = dZ_l . W_l, then dZ_{l-1} = dH_{l-1} * silu'(Z_{l-1})
+        EpiArgs be{};
+        be.zprev = m->Z[l - 1];
+        be.dz = m->dZ[cur ^ 1];
+        be.dzt = m->dZT[cur ^ 1];
+        be.ldo = kH;
+        be.ldt = mpad;
+        hipLaunchKernelGGL(k_gemm_nt<EPI_BWD>, dim3(mpad / kBM, kH / kBN, 1), dim3(256), 0, m->stream, m->dZ[cur], kH,
+                           wtb_layer(m, l), kH, kH, kH, be);
+        MCHK(m, hipGetLastError());
+        cur ^= 1;
+    }
+    if (h_loss) {
+        std::vector<float> pb(2 * (size_t)nhb);
+        MCHK(m, hipMemcpyAsync(pb.data(), m->hpart_b, pb.size() * 4, hipMemcpyDeviceToHost, m->stream));
+        MCHK(m, hipStreamSynchronize(m->stream));
+        double s = 0.0;
+        for (int k = 0; k < nhb; ++k) s += pb[2 * k + 1];
+        *h_loss = (float)(s / n);
+    }
+    return CSX_OK;
+}
+
+int csx_mlp_get_grads(csx_mlp* m, float* h_grads) {
+    if (!m || !h_grads) return CSX_E_ARG;
+    MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipMemcpyAsync(h_grads, m->g, kNParams * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return CSX_OK;
+}
+
+int csx_mlp_adam(csx_mlp* m, float lr, float b1, float b2, float eps) {
+    if (!m || !(lr >= 0) || !(b1 >= 0 && b1 < 1) || !(b2 >= 0 && b2 < 1)) return CSX_E_ARG;
+    MCHK(m, hipSetDevice(m->device));
+    ++m->adam_t;
+    const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)m->adam_t)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)m->adam_t)));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((kNParams + 255) / 256)), dim3(256), 0, m->stream, kNParams, m->p, m->g,
+                       m->m1, m->m2, lr, b1, b2, eps, c1, c2);
+    MCHK(m, hipGetLastError());
+    return cast_weights(m);
+}
+
+int csx_mlp_train_step(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                       const float* d_t, float lr, float* h_loss) {
+    int rc = csx_mlp_gradient(m, n, nsets, d_par, d_set, d_q, d_t, h_loss);
+    if (rc) return rc;
+    return csx_mlp_adam(m, lr, 0.9f, 0.999f, 1e-8f);
+}
+
+int csx_mlp_synchronize(csx_mlp* m) {
+    if (!m) return CSX_E_ARG;
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return CSX_OK;
+}
+
+}  // extern "C"

@@ -30,6 +30,14 @@ struct alignas(32) PairRec {
     float uz2, A;
     int32_t ia, ib;
 };
+// The same pair in FP64 for the precise pair loop: u = q_hkl / |q_hkl| with the stored
+// |q_hkl| (the reference's cosine divides by it), and the weight A in FP64.
+struct alignas(16) PairRec64 {
+    double ux, uy, uz, A;
+    float qh;
+    int32_t ia, ib;
+    float pad;
+};
 constexpr int kBands = 8;
 constexpr int kCqDelta = 64;  // polar nodes
 constexpr int kCqPsi = 64;    // azimuth nodes on [0, pi] (symmetric pairs: kCqPsi/2 values of t)
@@ -53,6 +61,7 @@ struct DevSet {
     // padded to one with inert records.  lut[b*(nlut+1)+k] is the index of the first pair of
     // segment b with |q_hkl| >= k/lut_inv_dq (the segment end for k = nlut).
     const struct PairRec* pairs;
+    const struct PairRec64* pairs64;  // index-aligned with pairs; lattice kinds with a precise loop only
     const int32_t* lut;
     const int32_t* singles;  // latpar indices of unpaired reflections
     const float2* iso;       // (|q_hkl|, mhkl*fhkl)

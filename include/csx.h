@@ -218,6 +218,51 @@ const csx_scalars* csx_engine_scalars(csx_engine* e);
 csx_ctx* csx_engine_ctx(csx_engine* e);
 int csx_engine_last_error(csx_engine* e, char* buf, int len);
 
+/* ---- Surrogate network (no counterpart in the reference) --------------------------------
+ * The learned part of the fits: an MLP from a normalised parameter set (CSX_MLP_NPAR values)
+ * and a pixel's (qx, qy) to the normalised log-intensity of that pixel.  Inputs: the
+ * parameters and sin/cos(2*pi*B.q) for a fixed CSX_MLP_NFREQ x 2 matrix B (CSX_MLP_IN values,
+ * zero-padded to CSX_MLP_IN_PAD); CSX_MLP_LAYERS hidden layers of CSX_MLP_HIDDEN SiLU units;
+ * one linear output.  BF16 weights and activations on the matrix cores with FP32 accumulation;
+ * FP32 master weights and Adam state.  Samples are rows: row i uses parameter set set[i] of
+ * d_par (nsets x CSX_MLP_NPAR floats; an index outside [0, nsets) is clamped into it) and
+ * d_q[2i], d_q[2i+1].  All pointers named d_* are
+ * device memory; calls are ordered on the model's stream. */
+#define CSX_MLP_NPAR 6
+#define CSX_MLP_NFREQ 32
+#define CSX_MLP_IN (CSX_MLP_NPAR + 2 * CSX_MLP_NFREQ)
+#define CSX_MLP_IN_PAD 128
+#define CSX_MLP_HIDDEN 256
+#define CSX_MLP_LAYERS 4
+typedef struct csx_mlp csx_mlp;
+/* Parameter count of the packed FP32 vector: per hidden layer W [HIDDEN][in] (out x in; the
+ * first layer's in = IN_PAD, its padding columns zero) then b [HIDDEN]; then w_out [HIDDEN]
+ * and b_out. */
+int64_t csx_mlp_nparams(void);
+/* max_batch: rows per call (rounded up to a multiple of 128); seed: initialisation. */
+int csx_mlp_create(int device, int max_batch, uint64_t seed, csx_mlp** out);
+void csx_mlp_destroy(csx_mlp* m);
+int csx_mlp_set_stream(csx_mlp* m, void* stream);
+int csx_mlp_last_error(const csx_mlp* m, char* buf, int len);
+/* Host copies of the FP32 parameters (csx_mlp_nparams floats) and of B (2*NFREQ floats,
+ * row-major); either pointer may be NULL.  set_params also resets the Adam state. */
+int csx_mlp_get_params(csx_mlp* m, float* h_params, float* h_fourier);
+int csx_mlp_set_params(csx_mlp* m, const float* h_params, const float* h_fourier);
+/* d_y[i] = network(row i), n rows (asynchronous). */
+int csx_mlp_forward(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                    float* d_y);
+/* Gradient of the loss 1/n sum_i (y_i - d_t[i])^2 into the model's gradient buffer;
+ * *h_loss (optional, synchronous) = the loss. */
+int csx_mlp_gradient(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                     const float* d_t, float* h_loss);
+int csx_mlp_get_grads(csx_mlp* m, float* h_grads);
+/* One Adam step with the current gradient (bias-corrected). */
+int csx_mlp_adam(csx_mlp* m, float lr, float b1, float b2, float eps);
+/* csx_mlp_gradient + csx_mlp_adam(lr, 0.9, 0.999, 1e-8). */
+int csx_mlp_train_step(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                       const float* d_t, float lr, float* h_loss);
+int csx_mlp_synchronize(csx_mlp* m);
+
 #ifdef __cplusplus
 }
 #endif
