@@ -71,12 +71,9 @@ def measured_traffic():
     return None
 
 
-# Batched 2D fits: BCT lattice of cylinders, z-axis orientation (key 0x01020806).
-CFG4_BASE = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, ComboBoxInterior=0, theta=0, phi=0,
-                 EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1, Base=0, HKLmax=5)
-FIT_KEYS = ["EditRadius", "Length", "uca,ucb", "ucc", "EditDomainSize", "EditAzi"]
-FIT_LO = np.array([2.0, 10.0, 15.0, 15.0, 100.0, 2.0])
-FIT_HI = np.array([6.0, 60.0, 40.0, 80.0, 400.0, 10.0])
+# Batched 2D fits: BCT lattice of cylinders, z-axis orientation (key 0x01020806); the fit
+# parameters, their box and the fixed keys are the package's.
+from crystalscatter_amd.surrogate import BCT_BASE as CFG4_BASE, FIT_HI, FIT_KEYS, FIT_LO  # noqa: E402
 
 
 def fit_problems(nfits, seed=1234, noise_seed=5678):
@@ -112,8 +109,9 @@ def fit_problems(nfits, seed=1234, noise_seed=5678):
         start = np.clip(truth * (1 + rng.uniform(-0.1, 0.1, 6)), FIT_LO * 1.001, FIT_HI * 0.999)
         base = SasCalc(device=-1, params=dict(p, I0=scale))  # the count scale is known
         base.set("ignore_cbs", 1)
+        base.prepare()
         probs.append(dict(engine=base, data=torch.tensor(data.ravel(), dtype=torch.float64, device="cuda"),
-                          w=x1 - x0, h=y1 - y0, truth=truth, x0=start, n=n))
+                          w=x1 - x0, h=y1 - y0, truth=truth, x0=start, n=n, scale=scale))
     return probs
 
 
@@ -165,6 +163,103 @@ def bench_fits(args):
     print(json.dumps(line), flush=True)
 
 
+# Matrix-core work per training row of the surrogate, algorithmic (70 inputs, not the padded
+# 128): forward 2*(70*256 + 3*256^2 + 256), backward the same for the weight gradients plus
+# 2*3*256^2 + 2*256 for the activation gradients (none for the inputs).
+MLP_FWD_FLOP = 2 * (70 * 256 + 3 * 256 * 256 + 256)
+MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * 3 * 256 * 256 + 2 * 256
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def bench_surrogate(args):
+    """Surrogate workload: simulate training patterns with the exact kernel, train the network
+    (BF16 MFMA forward/backward/Adam), measure its held-out log10 error, then run the same
+    fits from an uninformed start (the centre of the box) twice: plain batched Nelder-Mead
+    with maxit iterations, and surrogate-proposed starts followed by maxit_surrogate exact
+    iterations.  Every point the fits accept is scored by the exact kernel."""
+    import torch
+    from crystalscatter_amd import Context, nm_batch
+    from crystalscatter_amd import surrogate as sg
+    t0 = time.perf_counter()
+    data = sg.generate(args.train_patterns, grid_points=128, px_per_pattern=args.train_px, seed=7)
+    held = sg.generate(64, grid_points=128, px_per_pattern=4096, seed=8, mu=data.mu, sigma=data.sigma)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = sg.MLP(device=0, max_batch=args.batch, seed=1)
+    p0, fb0 = net.get_params()
+    sg.train(net, data, 20, batch=args.batch)  # warm-up (kernels loaded, buffers touched)
+    net.set_params(p0, fb0)                    # back to the initial weights, Adam state reset
+    net.synchronize()
+    t0 = time.perf_counter()
+    hist = sg.train(net, data, args.train_steps, batch=args.batch, log_every=max(1, args.train_steps // 10))
+    net.synchronize()
+    t_train = time.perf_counter() - t0
+    rows_per_s = args.train_steps * args.batch / t_train
+    pred = sg.predict_log10(net, held.par, held.sets, held.q, data.mu, data.sigma)
+    err = pred - (held.t * data.sigma + data.mu)
+    rms_log10 = float(torch.sqrt((err * err).mean()))
+    med_abs_log10 = float(err.abs().median())
+
+    probs = fit_problems(args.nfits)
+    torch.cuda.synchronize()
+    centre = 0.5 * (FIT_LO + FIT_HI)
+    ctx = Context(0)
+    fits = [dict(engine=p["engine"], d_data=p["data"].data_ptr(), w=p["w"], h=p["h"], x0=centre, lo=FIT_LO,
+                 hi=FIT_HI) for p in probs]
+    t0 = time.perf_counter()
+    res_p, st_p = nm_batch(ctx, fits, FIT_KEYS, stp=0.1, maxit=args.maxit, ftol=1e-6, precision=args.precision,
+                           threads=args.cpu_threads)
+    t_plain = time.perf_counter() - t0
+    rng = np.random.default_rng(99)
+    t0 = time.perf_counter()
+    fpx = [sg.FitPixels(p["engine"], p["data"], p["scale"], 512, rng) for p in probs]
+    starts, sval = sg.propose_starts(net, fpx, data.mu, data.sigma, [centre] * len(probs), rng)
+    torch.cuda.synchronize()
+    t_prop = time.perf_counter() - t0
+    for f, x in zip(fits, starts):
+        f["x0"] = x
+    t0 = time.perf_counter()
+    res_s, st_s = nm_batch(ctx, fits, FIT_KEYS, stp=0.02, maxit=args.maxit_surrogate, ftol=1e-6,
+                           precision=args.precision, threads=args.cpu_threads)
+    t_sur = time.perf_counter() - t0 + t_prop
+    chi_p = np.array([r[1] for r in res_p])
+    chi_s = np.array([r[1] for r in res_s])
+    err_p = np.array([np.abs(r[0] / p["truth"] - 1) for p, r in zip(probs, res_p)])
+    err_s = np.array([np.abs(r[0] / p["truth"] - 1) for p, r in zip(probs, res_s)])
+    line = {
+        "metric": "2D fits/s, surrogate-started vs plain (batched Nelder-Mead, exact kernel scoring)",
+        "value": len(probs) / t_sur,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": int(st_s["rounds"]),
+        "warmup": 0,
+        "ms_per_step": 1e3 * t_sur / max(1, st_s["rounds"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network, f32 exact kernel",
+        "data": "synthetic",
+        "config": {"workload": "%d fits of BCT cylinders (key 0x01020806) from the centre of the box; surrogate "
+                               "MLP 70->4x256 SiLU->1 trained on %d patterns of 256x256 (%d px kept each)"
+                               % (len(probs), args.train_patterns, args.train_px),
+                   "maxit_plain": args.maxit, "maxit_surrogate": args.maxit_surrogate},
+        "plain_fits_per_s": len(probs) / t_plain,
+        "speedup_vs_plain": t_plain / t_sur,
+        "chi2_median_plain": float(np.median(chi_p)), "chi2_median_surrogate": float(np.median(chi_s)),
+        "frac_fits_surrogate_chi2_le_plain": float(np.mean(chi_s <= chi_p * (1 + 1e-3))),
+        "median_rel_param_err_plain": float(np.median(err_p)),
+        "median_rel_param_err_surrogate": float(np.median(err_s)),
+        "surrogate_rms_log10_err_heldout": rms_log10, "surrogate_median_abs_log10_err_heldout": med_abs_log10,
+        "train_loss_history": hist, "training_rows": len(data), "training_rows_per_s": rows_per_s,
+        "training_s": t_train, "data_generation_s": t_data, "proposal_s": t_prop,
+        "roofline": {"bound": "mfma", "achieved": rows_per_s * MLP_TRAIN_FLOP / 1e12, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": rows_per_s * MLP_TRAIN_FLOP / 1e12 / PEAK_BF16_TFLOPS,
+                     "traffic": None, "kernel": "training step (embed + 4 fwd + 3 bwd + 4 weight-gradient GEMMs)",
+                     "flop_per_row": MLP_TRAIN_FLOP},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(threads):
     """FP64 oracle (bit-exact restatement of the reference CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -187,10 +282,16 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "ref64"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("CSX_BENCH_CPU_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits"],
-                    help="fcc: the 1024^2 FCC pattern benchmark (default); fits: batched 2D fits")
+    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits", "surrogate"],
+                    help="fcc: the 1024^2 FCC pattern benchmark (default); fits: batched 2D fits; "
+                         "surrogate: train the surrogate network, then fits with and without it")
     ap.add_argument("--nfits", type=int, default=512)
     ap.add_argument("--maxit", type=int, default=100)
+    ap.add_argument("--maxit-surrogate", type=int, default=10, help="exact iterations after a surrogate start")
+    ap.add_argument("--train-patterns", type=int, default=4096)
+    ap.add_argument("--train-px", type=int, default=4096, help="pixels kept per training pattern")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=65536, help="training rows per step")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 256 MiB device to device once (a plain copy of "
                          "known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -202,6 +303,9 @@ def main():
 
     if args.workload == "fits":
         bench_fits(args)
+        return
+    if args.workload == "surrogate":
+        bench_surrogate(args)
         return
 
     import ctypes

@@ -121,3 +121,189 @@ class MLP:
 
     def synchronize(self):
         self._chk(self._L.csx_mlp_synchronize(self._h), "synchronize")
+
+
+# ---- the fits' parameter space ---------------------------------------------------------
+# Free parameters of the BCT-cylinder fits and their box (the fit driver's bounds): radius R,
+# length L, a (= uca = ucb), c (= ucc), domain size and angular peak width EditAzi.
+FIT_KEYS = ["EditRadius", "Length", "uca,ucb", "ucc", "EditDomainSize", "EditAzi"]
+FIT_LO = np.array([2.0, 10.0, 15.0, 15.0, 100.0, 2.0])
+FIT_HI = np.array([6.0, 60.0, 40.0, 80.0, 400.0, 10.0])
+# Fixed keys of the BCT-cylinder combination (key 0x01020806): z-oriented cylinders on a BCT
+# lattice with Gaussian peaks, Debye-Scherrer widths, q in [-1, 1] nm^-1.
+BCT_BASE = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, ComboBoxInterior=0, theta=0, phi=0,
+                EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1, Base=0, HKLmax=5)
+
+
+def normalize(x):
+    """Fit parameters -> [-1, 1]^6 (the network's parameter inputs)."""
+    return 2.0 * (np.asarray(x, dtype=np.float64) - FIT_LO) / (FIT_HI - FIT_LO) - 1.0
+
+
+def draw_parameters(rng, n):
+    """n parameter vectors drawn like the fit workload: R, L, a, c/a, domain, width uniform."""
+    R, L = rng.uniform(2, 6, n), rng.uniform(10, 60, n)
+    a, ca = rng.uniform(15, 40, n), rng.uniform(1, 2, n)
+    dom, azi = rng.uniform(100, 400, n), rng.uniform(2, 10, n)
+    return np.stack([R, L, a, a * ca, dom, azi], axis=1)
+
+
+def params_dict(x, grid_points):
+    R, L, a, c, dom, azi = (float(v) for v in x)
+    return dict(BCT_BASE, EditRadius=R, Length=L, uca=a, ucb=a, ucc=c, EditDomainSize=dom, EditAzi=azi,
+                GridPoints=grid_points)
+
+
+def pixel_q(engine, idx):
+    """(qx, qy) of pixels at IDX positions of an engine's image (midpoint grid):
+    ihex = xmin + idx % W (-> qy), i = ymin + idx // W (-> qx), q = qmax * index / zmax."""
+    s = engine.scalars()
+    W = s.xmax - s.xmin
+    ihex = s.xmin + idx % W
+    i = s.ymin + idx // W
+    f = s.qmax / s.zmax
+    return i * f, ihex * f
+
+
+class TrainingSet:
+    """Pixels of simulated patterns on the device: set index, (qx, qy), normalised log10 I."""
+
+    def __init__(self, par, sets, q, t, mu, sigma, npatterns, px_per_pattern):
+        self.par, self.sets, self.q, self.t = par, sets, q, t
+        self.mu, self.sigma = mu, sigma
+        self.npatterns, self.px_per_pattern = npatterns, px_per_pattern
+
+    def __len__(self):
+        return int(self.t.numel())
+
+
+def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, mu=None, sigma=None):
+    """Render npatterns BCT-cylinder patterns with the exact kernel (mixed precision, I0 = 1)
+    at parameters drawn like the fit workload, and keep px_per_pattern random pixels of each
+    with a positive, finite intensity.  Targets are (log10 I - mu) / sigma; mu and sigma come
+    from the first patterns unless given."""
+    import torch
+    from . import SasCalc
+    rng = np.random.default_rng(seed)
+    xs = draw_parameters(rng, npatterns)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    sets, qs, ts = [], [], []
+    buf = None
+    e = SasCalc(device=device, params=params_dict(xs[0], grid_points))
+    e.set("ignore_cbs", 1)
+    for k in range(npatterns):
+        e.set_params(params_dict(xs[k], grid_points))
+        e.prepare()
+        x0, x1, y0, y1 = e.bounds()
+        npx = (x1 - x0) * (y1 - y0)
+        if buf is None or buf.numel() < npx:
+            buf = torch.empty(npx, dtype=torch.float64, device="cuda")
+        e.doCalculationDevice(buf.data_ptr(), "mixed")
+        e._L.csx_synchronize(e._L.csx_engine_ctx(e._h))
+        img = buf[:npx]
+        idx = torch.randint(0, npx, (2 * px_per_pattern,), device="cuda", generator=gen)
+        v = img[idx]
+        ok = torch.isfinite(v) & (v > 0)
+        idx, v = idx[ok][:px_per_pattern], v[ok][:px_per_pattern]
+        qx, qy = pixel_q(e, idx.to(torch.float64))
+        sets.append(torch.full((idx.numel(),), k, dtype=torch.int32, device="cuda"))
+        qs.append(torch.stack([qx, qy], dim=1).to(torch.float32))
+        ts.append(torch.log10(v))
+    e.close()
+    t = torch.cat(ts)
+    if mu is None:
+        mu, sigma = float(t.mean()), float(t.std())
+    par = torch.from_numpy(normalize(xs).astype(np.float32)).cuda()
+    return TrainingSet(par, torch.cat(sets), torch.cat(qs).contiguous(), ((t - mu) / sigma).to(torch.float32),
+                       mu, sigma, npatterns, px_per_pattern)
+
+
+def train(net, data, steps, batch=65536, lr=2e-3, lr_end=1e-4, seed=0, log_every=0):
+    """Adam over random minibatches of the training set; cosine decay from lr to lr_end.
+    Returns the losses at the logged steps (synchronous only there)."""
+    import torch
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    n = len(data)
+    hist = []
+    for s in range(steps):
+        idx = torch.randint(0, n, (batch,), device="cuda", generator=gen)
+        rate = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * s / max(1, steps - 1)))
+        want = bool(log_every) and (s % log_every == 0 or s == steps - 1)
+        loss = net.train_step(data.par, data.sets[idx].contiguous(), data.q[idx].contiguous(), data.t[idx].contiguous(),
+                              float(rate), want_loss=want)
+        if want:
+            hist.append((s, loss))
+    return hist
+
+
+def predict_log10(net, data_par, sets, q, mu, sigma):
+    """log10 I predicted for rows (device tensors), in chunks of the network's batch."""
+    import torch
+    n = int(q.shape[0])
+    out = torch.empty(n, dtype=torch.float32, device="cuda")
+    for a in range(0, n, net.max_batch):
+        b = min(n, a + net.max_batch)
+        net.forward(data_par, sets[a:b].contiguous(), q[a:b].contiguous(), out=out[a:b])
+    return out * sigma + mu
+
+
+class FitPixels:
+    """A subsample of one fit's usable pixels (data > 0, the pixels the objective counts):
+    (qx, qy) and log10(data / scale), where scale is the fit's known count scale (I0)."""
+
+    def __init__(self, engine, data, scale, npx, rng):
+        import torch
+        d = data if data.dim() == 1 else data.reshape(-1)
+        good = torch.nonzero(d > 0).squeeze(1)
+        pick = torch.from_numpy(rng.choice(good.numel(), min(npx, good.numel()), replace=False)).cuda()
+        idx = good[pick]
+        qx, qy = pixel_q(engine, idx.to(torch.float64))
+        self.q = torch.stack([qx, qy], dim=1).to(torch.float32).contiguous()
+        self.logd = (torch.log10(d[idx]) - np.log10(scale)).to(torch.float32)
+        self.n = int(idx.numel())
+
+
+def surrogate_objectives(net, fits_px, cands, mu, sigma):
+    """Surrogate FQS (mean over each fit's pixel subsample of (log10 I_pred - log10 I_data)^2)
+    for cands[f] (k_f x 6 parameter vectors) of every fit f.  Returns a list of arrays."""
+    import torch
+    par_rows, set_rows, q_rows, tgt_rows, seg = [], [], [], [], []
+    base = 0
+    for f, (fp, C) in enumerate(zip(fits_px, cands)):
+        k = C.shape[0]
+        par_rows.append(normalize(C))
+        set_rows.append(torch.arange(base, base + k, dtype=torch.int32, device="cuda").repeat_interleave(fp.n))
+        q_rows.append(fp.q.repeat(k, 1))
+        tgt_rows.append(fp.logd.repeat(k))
+        seg.append((k, fp.n))
+        base += k
+    par = torch.from_numpy(np.concatenate(par_rows).astype(np.float32)).cuda()
+    sets, q, tgt = torch.cat(set_rows), torch.cat(q_rows).contiguous(), torch.cat(tgt_rows)
+    pred = predict_log10(net, par, sets, q, mu, sigma)
+    r2 = (pred - tgt) ** 2
+    out, a = [], 0
+    for k, n in seg:
+        out.append(r2[a:a + k * n].reshape(k, n).mean(dim=1).double().cpu().numpy())
+        a += k * n
+    return out
+
+
+def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=256, n_keep=8, n_local=32, shrink=0.1):
+    """Starting points for the exact fits: the best of n_global draws over the box (plus
+    x_init), then the best of n_local draws in a box of +-shrink of the range around each of
+    the n_keep best.  Returns (starts, surrogate objective of each start)."""
+    nf = len(fits_px)
+    cand = [np.vstack([x_init[f][None, :], draw_parameters(rng, n_global)]) for f in range(nf)]
+    cand = [np.clip(C, FIT_LO, FIT_HI) for C in cand]
+    vals = surrogate_objectives(net, fits_px, cand, mu, sigma)
+    loc = []
+    for f in range(nf):
+        best = cand[f][np.argsort(vals[f])[:n_keep]]
+        span = shrink * (FIT_HI - FIT_LO)
+        pts = best.repeat(n_local, axis=0) + rng.uniform(-1, 1, (n_keep * n_local, 6)) * span
+        loc.append(np.clip(np.vstack([best, pts]), FIT_LO, FIT_HI))
+    lv = surrogate_objectives(net, fits_px, loc, mu, sigma)
+    starts = np.array([loc[f][np.argmin(lv[f])] for f in range(nf)])
+    return starts, np.array([lv[f].min() for f in range(nf)])
