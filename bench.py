@@ -29,7 +29,7 @@ CFG3 = dict(LType=5, ComboBoxParticle=0, Ordis=0, ComboBoxPeak=1, ComboBoxInteri
             EditSigma=0.08, uca=30, ucb=30, ucc=30, EditDomainSize=200, RadButDebyeScherrer=1,
             EditDebyeWaller=1.5, EditAzi=5, HKLmax=5, GridPoints=512, EditQmax=1, I0=1, Base=0)
 
-# Roofline accounting of the dominant kernel, k_image<K_FCC, mixed> (with its radial
+# Roofline accounting of the dominant kernel, k_image_lat<K_FCC, mixed> (with its radial
 # pre-pass k_radial, bracketed by the same events).
 #
 # roofline.achieved counts the work the kernel's algorithm executes in its FP32 pair loop:
@@ -412,7 +412,7 @@ def main():
         "region_ms_per_launch": region_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "k_radial<K_FCC, CSX_PREC_MIXED> + k_image<K_FCC, CSX_PREC_MIXED>",
+                     "kernel": "k_radial<K_FCC, CSX_PREC_MIXED> + k_image_lat<K_FCC, CSX_PREC_MIXED>",
                      "pair_evals_per_launch": pair_evals, "flop_per_pair_eval": EXEC_FLOP_PER_PAIR_EVAL,
                      "pair_evals_uncut": px * ((nrefl + 1) // 2),
                      "transcendental_rate_tops": exec_tops, "transcendental_peak_tops": PEAK_TRANS_TOPS,
