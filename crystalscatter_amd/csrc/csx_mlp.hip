@@ -41,6 +41,7 @@ constexpr int kL = CSX_MLP_LAYERS;     // hidden layers
 constexpr int kNF = CSX_MLP_NFREQ;     // Fourier frequencies
 constexpr int kNP = CSX_MLP_NPAR;      // parameters per set
 constexpr int kBM = 128, kBN = 128, kBK = 64, kLd = kBK + 8;  // GEMM tile, LDS row stride (bf16)
+constexpr int kTp = kBN + 8;            // row stride (bf16) of the epilogue's output tile in LDS
 constexpr int kHeadRows = 64;           // rows per block of the output-layer kernels
 
 // Offsets of the FP32 master parameters (one buffer): hidden layer l has W_l [kH][in_l]
@@ -116,11 +117,39 @@ struct EpiArgs {
 // Grid (M/128, N/128, splits); 256 threads, 4 waves in 2x2, each wave a 64x64 block of
 // 4x4 MFMA tiles.  Requires M, N multiples of 128, K and kchunk multiples of 64, lda/ldb
 // multiples of 8 (checked by the host).
+// Row phase of the epilogue: the 128x128 bf16 tile T (row stride kTp) to out[m][n] rows with
+// 16-byte stores (out row stride ld).
+__device__ __forceinline__ void tile_store_rows(const bf16* T, bf16* __restrict__ out, int ld, int tm, int tn) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int id = threadIdx.x + 256 * it;
+        const int row = id >> 4, c8 = (id & 15) * 8;
+        *(uint4*)&out[(size_t)(tm + row) * ld + tn + c8] = *(const uint4*)&T[row * kTp + c8];
+    }
+}
+// Column phase: the transpose of T to outT[n][m] (row stride ldt), 8 rows per 16-byte store;
+// consecutive threads take consecutive columns (conflict-free LDS reads).
+__device__ __forceinline__ void tile_store_cols(const bf16* T, bf16* __restrict__ outT, int ldt, int tm, int tn) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int id = threadIdx.x + 256 * it;
+        const int col = id & 127, m8 = (id >> 7) * 8;
+        bf16x8 pk;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pk[k] = T[(m8 + k) * kTp + col];
+        *(bf16x8*)&outT[(size_t)(tn + col) * ldt + tm + m8] = pk;
+    }
+}
+
 template <int EPI>
 __global__ void __launch_bounds__(256) k_gemm_nt(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
                                                  int ldb, int K, int kchunk, EpiArgs ep) {
-    __shared__ bf16 sA[kBM * kLd];
-    __shared__ bf16 sB[kBN * kLd];
+    // operand tiles during the main loop, the output tile (128 x kTp bf16) in the epilogue
+    __shared__ __attribute__((aligned(16))) bf16 smem[2 * kBM * kLd];
+    static_assert(kBM * kTp <= 2 * kBM * kLd, "epilogue tile must fit in the operand tiles");
+    bf16* sA = smem;
+    bf16* sB = smem + kBM * kLd;
+    bf16* T = smem;
     const int tm = blockIdx.x * kBM, tn = blockIdx.y * kBN;
     const int k0 = blockIdx.z * kchunk, k1 = min(K, k0 + kchunk);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -156,37 +185,67 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const bf16* __restrict__ A, int
         __syncthreads();
     }
     // accumulator (i, j)[r] is C[tm + wm + 16i + 4(lane/16) + r][tn + wn + 16j + lane%16]
+    if constexpr (EPI == EPI_PART) {
+        float* out = ep.part + ((size_t)blockIdx.z * gridDim.x * kBM) * ep.ldo;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int m0 = tm + wm + i * 16 + (lane >> 4) * 4;
+        for (int i = 0; i < 4; ++i) {
+            const int m0 = tm + wm + i * 16 + (lane >> 4) * 4;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = tn + wn + j * 16 + fr;
-            if constexpr (EPI == EPI_PART) {
-                float* out = ep.part + ((size_t)blockIdx.z * gridDim.x * kBM) * ep.ldo;
+            for (int j = 0; j < 4; ++j) {
+                const int n = tn + wn + j * 16 + fr;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) out[(size_t)(m0 + r) * ep.ldo + n] = acc[i][j][r];
-            } else {
-                bf16 tcol[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const size_t o = (size_t)(m0 + r) * ep.ldo + n;
-                    if constexpr (EPI == EPI_FWD) {
-                        const float z = acc[i][j][r] + ep.bias[n];
-                        const float hv = z * sigm(z);
-                        ep.z[o] = (bf16)z;
-                        ep.h[o] = (bf16)hv;
-                        tcol[r] = (bf16)hv;
-                    } else {
-                        const float dz = acc[i][j][r] * silu_grad((float)ep.zprev[o]);
-                        ep.dz[o] = (bf16)dz;
-                        tcol[r] = (bf16)dz;
-                    }
-                }
-                bf16* tp = (EPI == EPI_FWD ? ep.ht : ep.dzt) + (size_t)n * ep.ldt + m0;
-                *(uint2*)tp = *(const uint2*)tcol;  // 4 consecutive rows, 8 bytes
             }
         }
+    } else if constexpr (EPI == EPI_FWD) {
+        // Z = acc + b, then H = silu(Z), each staged through LDS for row and column stores
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float b = ep.bias[tn + wn + j * 16 + fr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    T[(wm + i * 16 + (lane >> 4) * 4 + r) * kTp + wn + j * 16 + fr] = (bf16)(acc[i][j][r] + b);
+        }
+        __syncthreads();
+        tile_store_rows(T, ep.z, ep.ldo, tm, tn);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float b = ep.bias[tn + wn + j * 16 + fr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float z = acc[i][j][r] + b;
+                    T[(wm + i * 16 + (lane >> 4) * 4 + r) * kTp + wn + j * 16 + fr] = (bf16)(z * sigm(z));
+                }
+        }
+        __syncthreads();
+        tile_store_rows(T, ep.h, ep.ldo, tm, tn);
+        tile_store_cols(T, ep.ht, ep.ldt, tm, tn);
+    } else {
+        // dZ = dH * silu'(Z_prev): Z_prev tile into LDS (row loads), combined in place
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int id = threadIdx.x + 256 * it;
+            const int row = id >> 4, c8 = (id & 15) * 8;
+            *(uint4*)&T[row * kTp + c8] = *(const uint4*)&ep.zprev[(size_t)(tm + row) * ep.ldo + tn + c8];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    bf16& e = T[(wm + i * 16 + (lane >> 4) * 4 + r) * kTp + wn + j * 16 + fr];
+                    e = (bf16)(acc[i][j][r] * silu_grad((float)e));
+                }
+        __syncthreads();
+        tile_store_rows(T, ep.dz, ep.ldo, tm, tn);
+        tile_store_cols(T, ep.dzt, ep.ldt, tm, tn);
     }
 }
 
@@ -205,34 +264,32 @@ __global__ void __launch_bounds__(256) k_head_fwd(int mpad, const bf16* __restri
 }
 
 // Loss 1/n sum (y - t)^2 and the output layer's backward for a block of kHeadRows rows:
-// dy = 2 (y - t) / n (0 for rows >= n), dZ4[m][c] = dy * w[c] * silu'(Z4[m][c]) written in
-// both layouts, and per-block partials of dw (sum dy * H4), db and the loss.
+// dy = 2 (y - t) / n (0 for rows >= n) into dy_out, dZ4[m][c] = dy * w[c] * silu'(Z4[m][c])
+// written in both layouts, and per-block partials of db = sum dy and of the loss.
 __global__ void __launch_bounds__(256) k_head_bwd(int n, int mpad, const float* __restrict__ y,
                                                   const float* __restrict__ tgt, const bf16* __restrict__ Z,
-                                                  const bf16* __restrict__ H, const float* __restrict__ w,
+                                                  const float* __restrict__ w, float* __restrict__ dy_out,
                                                   bf16* __restrict__ dZ, bf16* __restrict__ dZT,
-                                                  float* __restrict__ part_w, float* __restrict__ part_b) {
+                                                  float* __restrict__ part_b) {
     __shared__ bf16 tile[kHeadRows][kH + 8];
     __shared__ float s_dy[kHeadRows];
     const int m0 = blockIdx.x * kHeadRows, t = threadIdx.x;
     if (t < kHeadRows) {
         const int m = m0 + t;
-        s_dy[t] = (m < n) ? 2.0f * (y[m] - tgt[m]) / (float)n : 0.0f;
+        const float dy = (m < n) ? 2.0f * (y[m] - tgt[m]) / (float)n : 0.0f;
+        s_dy[t] = dy;
+        dy_out[m] = dy;
     }
     __syncthreads();
-    // thread t owns column c = t: its dw partial and the transposed store
+    // thread t owns column c = t
     const int c = t;
     const float wc = w[c];
-    float dw = 0.0f;
     for (int r = 0; r < kHeadRows; ++r) {
         const size_t o = (size_t)(m0 + r) * kH + c;
-        const float dy = s_dy[r];
-        dw = fmaf(dy, (float)H[o], dw);
-        const float dz = dy * wc * silu_grad((float)Z[o]);
-        tile[r][c] = (bf16)dz;
-        dZ[o] = tile[r][c];
+        const bf16 dz = (bf16)(s_dy[r] * wc * silu_grad((float)Z[o]));
+        tile[r][c] = dz;
+        dZ[o] = dz;
     }
-    part_w[(size_t)blockIdx.x * kH + c] = dw;
     // column c of the tile is 64 consecutive rows of dZ^T
     bf16* dst = dZT + (size_t)c * mpad + m0;
 #pragma unroll
@@ -254,28 +311,81 @@ __global__ void __launch_bounds__(256) k_head_bwd(int n, int mpad, const float* 
     }
 }
 
-// g[c] = sum over s < ns of part[s*ld + c]  (fixed order: deterministic)
-__global__ void k_sum_parts(int ncols, int ns, int ld, const float* __restrict__ part, float* __restrict__ g) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    float s = 0.0f;
-    for (int k = 0; k < ns; ++k) s += part[(size_t)k * ld + c];
-    g[c] = s;
+__device__ __forceinline__ float block_sum(float v, float* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const float s = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return s;
 }
 
-// Bias gradient of a hidden layer: db[c] = sum_m dZ^T[c][m], one block per column.
-__global__ void __launch_bounds__(256) k_rowsum(int mpad, const bf16* __restrict__ dZT, float* __restrict__ db) {
-    __shared__ float red[256];
-    const bf16* row = dZT + (size_t)blockIdx.x * mpad;
-    float s = 0.0f;
-    for (int m = threadIdx.x; m < mpad; m += 256) s += (float)row[m];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+// Row reductions over the batch for rows of a [rows][mpad] bf16 matrix, in segments of
+// kSeg columns: part[r * S + s] = sum over the segment of A[r][m] * (w ? w[m] : 1).
+// Grid (rows, S = ceil(mpad / kSeg)); each thread takes one 8-column chunk.
+constexpr int kSeg = 256 * 8;
+__global__ void __launch_bounds__(256) k_rowdot(int mpad, const bf16* __restrict__ A, const float* __restrict__ w,
+                                                float* __restrict__ part) {
+    __shared__ float red[4];
+    const int r = blockIdx.x, sg = blockIdx.y;
+    const int m = sg * kSeg + threadIdx.x * 8;
+    float acc = 0.0f;
+    if (m < mpad) {  // mpad is a multiple of 128: chunks lie wholly inside or outside
+        const bf16x8 v = *(const bf16x8*)(A + (size_t)r * mpad + m);
+        if (w) {
+            const float4 w0 = *(const float4*)(w + m), w1 = *(const float4*)(w + m + 4);
+            acc = (float)v[0] * w0.x + (float)v[1] * w0.y + (float)v[2] * w0.z + (float)v[3] * w0.w +
+                  (float)v[4] * w1.x + (float)v[5] * w1.y + (float)v[6] * w1.z + (float)v[7] * w1.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc += (float)v[j];
+        }
     }
-    if (threadIdx.x == 0) db[blockIdx.x] = red[0];
+    const float s = block_sum(acc, red);
+    if (threadIdx.x == 0) part[(size_t)r * gridDim.y + sg] = s;
+}
+// out[r] = sum over s < S of part[r * S + s] (fixed order)
+__global__ void k_rowdot_finish(int rows, int S, const float* __restrict__ part, float* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    float s = 0.0f;
+    for (int k = 0; k < S; ++k) s += part[(size_t)r * S + k];
+    out[r] = s;
+}
+
+// Sums of the output layer's block partials: g_b = sum db, loss_sum = sum loss (one block).
+__global__ void __launch_bounds__(256) k_head_finish(int nb, const float* __restrict__ part_b, float* __restrict__ g_b,
+                                                     float* __restrict__ loss_sum) {
+    __shared__ float red[4];
+    float db = 0.0f, ls = 0.0f;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        db += part_b[2 * k];
+        ls += part_b[2 * k + 1];
+    }
+    db = block_sum(db, red);
+    ls = block_sum(ls, red);
+    if (threadIdx.x == 0) {
+        *g_b = db;
+        *loss_sum = ls;
+    }
+}
+
+// g[c] = sum over s < ns of part[s*ld + c] for the weight-gradient splits, 4 columns per
+// thread (fixed order: deterministic).
+__global__ void __launch_bounds__(256) k_sum_parts(int n4, int ns, int ld4, const float4* __restrict__ part,
+                                                   float4* __restrict__ g) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n4) return;
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 8
+    for (int k = 0; k < ns; ++k) {
+        const float4 v = part[(size_t)k * ld4 + c];
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    g[c] = s;
 }
 
 // Adam on the FP32 master parameters, then the bf16 copies of the hidden weights in both
@@ -332,7 +442,7 @@ struct csx_mlp {
     bf16 *X = nullptr, *XT = nullptr;
     bf16 *Z[kL] = {}, *H[kL] = {}, *HT[kL] = {};
     bf16 *dZ[2] = {}, *dZT[2] = {};
-    float *y = nullptr, *part = nullptr, *hpart_w = nullptr, *hpart_b = nullptr;
+    float *y = nullptr, *dy = nullptr, *part = nullptr, *rpart = nullptr, *hpart_b = nullptr, *loss = nullptr;
     int splits = 0;
 };
 
@@ -430,8 +540,9 @@ int csx_mlp_create(int device, int max_batch, uint64_t seed, csx_mlp** out) {
          alloc((void**)&m->m1, kNParams * 4) && alloc((void**)&m->m2, kNParams * 4) &&
          alloc((void**)&m->fbd, 2 * kNF * 4) && alloc((void**)&m->wb, nw * 2) && alloc((void**)&m->wtb, nw * 2) &&
          alloc((void**)&m->X, M * kIn * 2) && alloc((void**)&m->XT, M * kIn * 2) && alloc((void**)&m->y, M * 4) &&
-         alloc((void**)&m->part, (size_t)m->splits * kH * kH * 4) &&
-         alloc((void**)&m->hpart_w, (M / kHeadRows) * kH * 4) && alloc((void**)&m->hpart_b, (M / kHeadRows) * 2 * 4);
+         alloc((void**)&m->dy, M * 4) && alloc((void**)&m->part, (size_t)m->splits * kH * kH * 4) &&
+         alloc((void**)&m->rpart, (size_t)kH * ((M + kSeg - 1) / kSeg) * 4) &&
+         alloc((void**)&m->hpart_b, (M / kHeadRows) * 2 * 4) && alloc((void**)&m->loss, 4);
     for (int l = 0; l < kL && ok; ++l)
         ok = alloc((void**)&m->Z[l], M * kH * 2) && alloc((void**)&m->H[l], M * kH * 2) &&
              alloc((void**)&m->HT[l], M * kH * 2);
@@ -477,7 +588,8 @@ void csx_mlp_destroy(csx_mlp* m) {
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (void* q : {(void*)m->p, (void*)m->g, (void*)m->m1, (void*)m->m2, (void*)m->fbd, (void*)m->wb, (void*)m->wtb,
-                    (void*)m->X, (void*)m->XT, (void*)m->y, (void*)m->part, (void*)m->hpart_w, (void*)m->hpart_b})
+                    (void*)m->X, (void*)m->XT, (void*)m->y, (void*)m->dy, (void*)m->part, (void*)m->rpart,
+                    (void*)m->hpart_b, (void*)m->loss})
         if (q) (void)hipFree(q);
     for (int l = 0; l < kL; ++l)
         for (void* q : {(void*)m->Z[l], (void*)m->H[l], (void*)m->HT[l]})
@@ -559,12 +671,14 @@ int csx_mlp_gradient(csx_mlp* m, int n, int nsets, const float* d_par, const int
     if (rc) return rc;
     const int mpad = (n + kBM - 1) / kBM * kBM;
     const int nhb = mpad / kHeadRows;
-    // output layer: dZ of the last hidden layer, dw, db, loss
-    hipLaunchKernelGGL(k_head_bwd, dim3(nhb), dim3(256), 0, m->stream, n, mpad, m->y, d_t, m->Z[kL - 1], m->H[kL - 1],
-                       m->p + kOutW, m->dZ[0], m->dZT[0], m->hpart_w, m->hpart_b);
+    const int nseg = (mpad + kSeg - 1) / kSeg;
+    // output layer: dy, dZ of the last hidden layer, db and the loss; dw = H4^T . dy
+    hipLaunchKernelGGL(k_head_bwd, dim3(nhb), dim3(256), 0, m->stream, n, mpad, m->y, d_t, m->Z[kL - 1],
+                       m->p + kOutW, m->dy, m->dZ[0], m->dZT[0], m->hpart_b);
     MCHK(m, hipGetLastError());
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, m->stream, kH, nhb, kH, m->hpart_w, m->g + kOutW);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, m->stream, 1, nhb, 2, m->hpart_b, m->g + kOutB);
+    hipLaunchKernelGGL(k_head_finish, dim3(1), dim3(256), 0, m->stream, nhb, m->hpart_b, m->g + kOutB, m->loss);
+    hipLaunchKernelGGL(k_rowdot, dim3(kH, nseg), dim3(256), 0, m->stream, mpad, m->HT[kL - 1], m->dy, m->rpart);
+    hipLaunchKernelGGL(k_rowdot_finish, dim3(1), dim3(kH), 0, m->stream, kH, nseg, m->rpart, m->g + kOutW);
     MCHK(m, hipGetLastError());
     // hidden layers, top down; dZ of layer l is in buffer cur
     int cur = 0;
@@ -580,9 +694,12 @@ int csx_mlp_gradient(csx_mlp* m, int n, int nsets, const float* d_par, const int
         hipLaunchKernelGGL(k_gemm_nt<EPI_PART>, dim3(kH / kBM, K / kBN, ns), dim3(256), 0, m->stream, m->dZT[cur], mpad,
                            BT, mpad, mpad, kchunk, pe);
         MCHK(m, hipGetLastError());
-        hipLaunchKernelGGL(k_sum_parts, dim3((kH * K + 255) / 256), dim3(256), 0, m->stream, kH * K, ns, kH * K, m->part,
-                           m->g + w_off(l));
-        hipLaunchKernelGGL(k_rowsum, dim3(kH), dim3(256), 0, m->stream, mpad, m->dZT[cur], m->g + b_off(l));
+        const int n4 = kH * K / 4;
+        hipLaunchKernelGGL(k_sum_parts, dim3((n4 + 255) / 256), dim3(256), 0, m->stream, n4, ns, n4,
+                           (const float4*)m->part, (float4*)(m->g + w_off(l)));
+        // bias gradient: row sums of dZ^T
+        hipLaunchKernelGGL(k_rowdot, dim3(kH, nseg), dim3(256), 0, m->stream, mpad, m->dZT[cur], nullptr, m->rpart);
+        hipLaunchKernelGGL(k_rowdot_finish, dim3(1), dim3(kH), 0, m->stream, kH, nseg, m->rpart, m->g + b_off(l));
         MCHK(m, hipGetLastError());
         if (l == 0) break;
         // dH_{l-1} = dZ_l . W_l, then dZ_{l-1} = dH_{l-1} * silu'(Z_{l-1})
@@ -598,12 +715,10 @@ int csx_mlp_gradient(csx_mlp* m, int n, int nsets, const float* d_par, const int
         cur ^= 1;
     }
     if (h_loss) {
-        std::vector<float> pb(2 * (size_t)nhb);
-        MCHK(m, hipMemcpyAsync(pb.data(), m->hpart_b, pb.size() * 4, hipMemcpyDeviceToHost, m->stream));
+        float ls = 0.0f;
+        MCHK(m, hipMemcpyAsync(&ls, m->loss, 4, hipMemcpyDeviceToHost, m->stream));
         MCHK(m, hipStreamSynchronize(m->stream));
-        double s = 0.0;
-        for (int k = 0; k < nhb; ++k) s += pb[2 * k + 1];
-        *h_loss = (float)(s / n);
+        *h_loss = ls / (float)n;
     }
     return CSX_OK;
 }
