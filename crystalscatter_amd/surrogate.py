@@ -165,11 +165,20 @@ def pixel_q(engine, idx):
     return i * f, ihex * f
 
 
-class TrainingSet:
-    """Pixels of simulated patterns on the device: set index, (qx, qy), normalised log10 I."""
+# Decades below a pattern's maximum that the targets resolve.  The fits see counts up to
+# ~1e4 at the brightest pixel and drop pixels without counts, so intensities more than ~5
+# decades below the maximum never enter an objective; targets are floored at 6 decades.
+FLOOR_DECADES = 6.0
+EVAL_DECADES = 5.0
 
-    def __init__(self, par, sets, q, t, mu, sigma, npatterns, px_per_pattern):
-        self.par, self.sets, self.q, self.t = par, sets, q, t
+
+class TrainingSet:
+    """Pixels of simulated patterns on the device: set index, (qx, qy), normalised log10 I
+    (floored at FLOOR_DECADES below the pattern's maximum), and whether the pixel lies within
+    EVAL_DECADES of the maximum (the range the fits' objective sees)."""
+
+    def __init__(self, par, sets, q, t, inrange, mu, sigma, npatterns, px_per_pattern):
+        self.par, self.sets, self.q, self.t, self.inrange = par, sets, q, t, inrange
         self.mu, self.sigma = mu, sigma
         self.npatterns, self.px_per_pattern = npatterns, px_per_pattern
 
@@ -180,15 +189,15 @@ class TrainingSet:
 def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, mu=None, sigma=None):
     """Render npatterns BCT-cylinder patterns with the exact kernel (mixed precision, I0 = 1)
     at parameters drawn like the fit workload, and keep px_per_pattern random pixels of each
-    with a positive, finite intensity.  Targets are (log10 I - mu) / sigma; mu and sigma come
-    from the first patterns unless given."""
+    with a positive, finite intensity.  Targets are (max(log10 I, log10 I_max - FLOOR_DECADES)
+    - mu) / sigma; mu and sigma come from the data unless given."""
     import torch
     from . import SasCalc
     rng = np.random.default_rng(seed)
     xs = draw_parameters(rng, npatterns)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
-    sets, qs, ts = [], [], []
+    sets, qs, ts, rs = [], [], [], []
     buf = None
     e = SasCalc(device=device, params=params_dict(xs[0], grid_points))
     e.set("ignore_cbs", 1)
@@ -202,6 +211,7 @@ def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, 
         e.doCalculationDevice(buf.data_ptr(), "mixed")
         e._L.csx_synchronize(e._L.csx_engine_ctx(e._h))
         img = buf[:npx]
+        top = torch.log10(torch.nan_to_num(img, nan=0.0, posinf=0.0).max())
         idx = torch.randint(0, npx, (2 * px_per_pattern,), device="cuda", generator=gen)
         v = img[idx]
         ok = torch.isfinite(v) & (v > 0)
@@ -209,14 +219,16 @@ def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, 
         qx, qy = pixel_q(e, idx.to(torch.float64))
         sets.append(torch.full((idx.numel(),), k, dtype=torch.int32, device="cuda"))
         qs.append(torch.stack([qx, qy], dim=1).to(torch.float32))
-        ts.append(torch.log10(v))
+        lv = torch.log10(v)
+        ts.append(torch.maximum(lv, top - FLOOR_DECADES))
+        rs.append(lv >= top - EVAL_DECADES)
     e.close()
     t = torch.cat(ts)
     if mu is None:
         mu, sigma = float(t.mean()), float(t.std())
     par = torch.from_numpy(normalize(xs).astype(np.float32)).cuda()
     return TrainingSet(par, torch.cat(sets), torch.cat(qs).contiguous(), ((t - mu) / sigma).to(torch.float32),
-                       mu, sigma, npatterns, px_per_pattern)
+                       torch.cat(rs), mu, sigma, npatterns, px_per_pattern)
 
 
 def train(net, data, steps, batch=65536, lr=2e-3, lr_end=1e-4, seed=0, log_every=0):
@@ -290,7 +302,7 @@ def surrogate_objectives(net, fits_px, cands, mu, sigma):
     return out
 
 
-def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=256, n_keep=8, n_local=32, shrink=0.1):
+def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4, n_local=32, shrink=0.1):
     """Starting points for the exact fits: the best of n_global draws over the box (plus
     x_init), then the best of n_local draws in a box of +-shrink of the range around each of
     the n_keep best.  Returns (starts, surrogate objective of each start)."""
