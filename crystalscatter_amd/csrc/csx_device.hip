@@ -1440,8 +1440,15 @@ struct csx_ctx {
     std::string err;
     bool uploaded = false;
     DevSet set{};
-    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, pairs64, lut, singles, iso, tiles, fqs_tmp, stats, radial, cqdl, cqtt,
-        cqtab;
+    DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, pairs64, lut, singles, iso, tiles, fqs_tmp, stats, cqdl, cqtt, cqtab;
+    // Radial pre-pass: two tables used alternately, filled on a second stream so that a
+    // render's pre-pass overlaps the previous render's image kernel.  ev_rad[s]: table s is
+    // filled; ev_img[s]: the image kernel that read table s is done.
+    hipStream_t side = nullptr;
+    DevBuf radial[2];
+    hipEvent_t ev_rad[2] = {nullptr, nullptr}, ev_img[2] = {nullptr, nullptr};
+    bool img_recorded[2] = {false, false};
+    int rslot = 0;
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
@@ -1463,19 +1470,21 @@ static int fail(csx_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(ctx, CSX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Radial pre-pass for kinds whose form factors depend on |q| only, in the midpoint grid.
 template <int K, int P>
-static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
-                         Radial* rtab, hipStream_t st) {
-    // Radial pre-pass for kinds whose form factors depend on |q| only, in the midpoint grid.
-    if (rtab && radial_kind(K) && !d.s.centerbeam) {
+static void launch_radial(const DevSet& d, const csx_grid& g, Radial* rtab, hipStream_t st) {
+    if constexpr (radial_kind(K)) {
         const int m = std::max(std::max(std::abs(g.xmin), std::abs(g.xmax - 1)),
                                std::max(std::abs(g.ymin), std::abs(g.ymax - 1)));
         const int total = (m + 1) * (m + 2) / 2;
         const int half = (total + 255) / 256 * 256;
         hipLaunchKernelGGL((k_radial<K, P>), dim3(2 * half / 256), dim3(256), 0, st, d, m, half, rtab);
-    } else {
-        rtab = nullptr;
     }
+}
+
+template <int K, int P>
+static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
+                         const Radial* rtab, hipStream_t st) {
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
     if constexpr (K == K_FCC || K == K_BCT)
         hipLaunchKernelGGL((k_image_lat<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
@@ -1523,15 +1532,26 @@ static int render_kind(const csx_ctx* c, int precision) {
     return k;
 }
 
-// Radial table of a render (midpoint grid, radial kinds), sized for the grid; null otherwise.
-static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out) {
+// Radial table of a render (midpoint grid, radial kinds), sized for the grid, and its slot;
+// null otherwise.
+static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) {
     *out = nullptr;
+    *slot = -1;
     if (!radial_kind(kind_of(c->set.s.combo_key)) || c->set.s.centerbeam) return CSX_OK;
     const long long m = std::max(std::max(std::llabs(g.xmin), std::llabs((long long)g.xmax - 1)),
                                  std::max(std::llabs(g.ymin), std::llabs((long long)g.ymax - 1)));
     if (m > 46000) return CSX_OK;  // triangle index would overflow int: evaluate per pixel
-    HIPCHK(c, c->radial.ensure((size_t)((m + 1) * (m + 2) / 2) * sizeof(Radial)));
-    *out = (Radial*)c->radial.p;
+    const int sl = c->rslot;
+    const size_t bytes = (size_t)((m + 1) * (m + 2) / 2) * sizeof(Radial);
+    if (c->radial[sl].n < bytes) {
+        // the table is reallocated: its last reader and writer must be done
+        if (c->img_recorded[sl]) HIPCHK(c, hipEventSynchronize(c->ev_img[sl]));
+        HIPCHK(c, hipStreamSynchronize(c->side));
+    }
+    HIPCHK(c, c->radial[sl].ensure(bytes));
+    c->rslot ^= 1;
+    *slot = sl;
+    *out = (Radial*)c->radial[sl].p;
     return CSX_OK;
 }
 
@@ -1545,6 +1565,30 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out) {
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
+// One render: the radial pre-pass (if any) on the side stream, after the image kernel that
+// last read the same table; the image kernel on the context stream, after the pre-pass.
+// Back-to-back renders thus overlap a pre-pass with the previous image kernel.
+static int render_launch(csx_ctx* c, int kind, int precision, const csx_grid& g, double* out,
+                         unsigned long long* stats) {
+    Radial* rtab = nullptr;
+    int sl = -1;
+    if (int rc = radial_table(c, g, &rtab, &sl)) return rc;
+    if (rtab) {
+        if (c->img_recorded[sl]) HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_img[sl], 0));
+        DISPATCH_KIND(kind, precision, launch_radial, c->set, g, rtab, c->side);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev_rad[sl], c->side));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_rad[sl], 0));
+    }
+    DISPATCH_KIND(kind, precision, launch_image, c->set, g, out, stats, rtab, c->stream);
+    HIPCHK(c, hipGetLastError());
+    if (rtab) {
+        HIPCHK(c, hipEventRecord(c->ev_img[sl], c->stream));
+        c->img_recorded[sl] = true;
+    }
+    return CSX_OK;
+}
+
 extern "C" {
 
 int csx_create(int device, csx_ctx** out) {
@@ -1556,8 +1600,14 @@ int csx_create(int device, csx_ctx** out) {
     csx_ctx* c = new csx_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rad[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rad[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_img[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_img[1], hipEventDisableTiming) != hipSuccess) {
+        c->own_stream = c->stream != nullptr;
+        csx_destroy(c);
         return CSX_E_HIP;
     }
     c->own_stream = true;
@@ -1569,13 +1619,15 @@ void csx_destroy(csx_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
-                      &c->iso, &c->stats, &c->radial, &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
+                      &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
                       &c->bsums_d, &c->bcnts_d})
         b->release();
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : {c->ev0, c->ev1, c->ev_rad[0], c->ev_rad[1], c->ev_img[0], c->ev_img[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1639,6 +1691,8 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
     if (rc) return rc;
     if (!h_carr) return fail(c, CSX_E_ARG, "null coefficient table");
     HIPCHK(c, hipSetDevice(c->device));
+    // a radial pre-pass still running reads the tables about to be replaced
+    HIPCHK(c, hipStreamSynchronize(c->side));
     const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
     HIPCHK(c, c->carr.ensure(18 * CSX_NCOEF * sizeof(double)));
     HIPCHK(c, hipMemcpyAsync(c->carr.p, h_carr, 18 * CSX_NCOEF * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1708,10 +1762,7 @@ int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
     HIPCHK(c, hipSetDevice(c->device));
     const int kind = render_kind(c, precision);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    Radial* rtab = nullptr;
-    if (int rc = radial_table(c, *g, &rtab)) return rc;
-    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, nullptr, rtab, c->stream);
-    HIPCHK(c, hipGetLastError());
+    if (int rc = render_launch(c, kind, precision, *g, d_out, nullptr)) return rc;
     if (c->timing) {
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         c->timing_pending = true;  // asynchronous: csx_last_kernel_ms waits for the events
@@ -1729,10 +1780,7 @@ int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out
     const std::vector<uint64_t> zero(CSX_NSTATS, 0);
     HIPCHK(c, hipMemcpyAsync(c->stats.p, zero.data(), CSX_NSTATS * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
     const int kind = render_kind(c, precision);
-    Radial* rtab = nullptr;
-    if (int rc = radial_table(c, *g, &rtab)) return rc;
-    DISPATCH_KIND(kind, precision, launch_image, c->set, *g, d_out, (unsigned long long*)c->stats.p, rtab, c->stream);
-    HIPCHK(c, hipGetLastError());
+    if (int rc = render_launch(c, kind, precision, *g, d_out, (unsigned long long*)c->stats.p)) return rc;
     HIPCHK(c, hipMemcpyAsync(h_stats, c->stats.p, CSX_NSTATS * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CSX_OK;
