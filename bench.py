@@ -262,6 +262,59 @@ def bench_surrogate(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_bct(args, nsets=8):
+    """The BCT-cylinder kernel (key 0x01020806, mixed precision) at 512x512 on nsets random
+    fit parameter sets (seed 1234, drawn like the fit workload): per set the kernel time
+    (HIP events, best of 5), the guard counters and the maximum relative error against the
+    FP64 reference mode on every pixel."""
+    import ctypes
+    import torch
+    from crystalscatter_amd import SasCalc, _lib
+    from crystalscatter_amd import surrogate as sg
+    L = _lib.lib()
+    xs = sg.draw_parameters(np.random.default_rng(1234), nsets)
+    out = torch.empty(512 * 512, dtype=torch.float64, device="cuda")
+    per = []
+    for k in range(nsets):
+        e = SasCalc(device=0, params=sg.params_dict(xs[k], 256))
+        e.set("ignore_cbs", 1)
+        e.prepare()
+        ctx = L.csx_engine_ctx(e._h)
+        e.doCalculationDevice(out.data_ptr(), "ref64")
+        L.csx_synchronize(ctx)
+        ref = out.cpu().numpy().copy()
+        e.doCalculationDevice(out.data_ptr(), "mixed")
+        L.csx_set_timing(ctx, 1)
+        ms = []
+        for _ in range(5):
+            e.doCalculationDevice(out.data_ptr(), "mixed")
+            ms.append(e.last_kernel_ms())
+        L.csx_set_timing(ctx, 0)
+        L.csx_synchronize(ctx)
+        img = out.cpu().numpy()
+        ok = np.isfinite(ref) & (ref != 0)
+        rel = float((np.abs(img[ok] - ref[ok]) / np.abs(ref[ok])).max())
+        st = (ctypes.c_uint64 * 5)()
+        s = e.scalars()
+        g = _lib.Grid(s.xmin, s.xmax, s.ymin, s.ymax)
+        assert L.csx_render_stats(ctx, ctypes.byref(g), 1, ctypes.c_void_p(out.data_ptr()), st) == 0
+        per.append(dict(params=[round(float(v), 3) for v in xs[k]], kernel_us=1e3 * min(ms), max_rel_err=rel,
+                        pair_evals=int(st[0]), fp64_window_pixels=int(st[1]), fp64_full_pixels=int(st[4]),
+                        near_one_pixels=int(st[2])))
+        e.close()
+    us = np.array([p["kernel_us"] for p in per])
+    line = {
+        "metric": "BCT-cylinder 512^2 patterns/s (mixed, guarded), random fit parameter sets",
+        "value": 1e6 / us.mean(), "unit": "patterns/s", "n_gpus": 1, "steps": 5 * nsets, "warmup": 1,
+        "ms_per_step": us.mean() / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 exponents, f32 exp2", "data": "synthetic",
+        "config": {"workload": "%d BCT-cylinder parameter sets (key 0x01020806) at 512x512, q in [-1,1] nm^-1"
+                               % nsets},
+        "max_rel_err_vs_fp64": max(p["max_rel_err"] for p in per), "sets": per,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(threads):
     """FP64 oracle (bit-exact restatement of the reference CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -284,9 +337,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "ref64"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("CSX_BENCH_CPU_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits", "surrogate"],
+    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits", "surrogate", "bct"],
                     help="fcc: the 1024^2 FCC pattern benchmark (default); fits: batched 2D fits; "
-                         "surrogate: train the surrogate network, then fits with and without it")
+                         "surrogate: train the surrogate network, then fits with and without it; "
+                         "bct: the BCT-cylinder kernel on random fit parameter sets")
     ap.add_argument("--nfits", type=int, default=512)
     ap.add_argument("--maxit", type=int, default=100)
     ap.add_argument("--maxit-surrogate", type=int, default=10, help="exact iterations after a surrogate start")
@@ -308,6 +362,9 @@ def main():
         return
     if args.workload == "surrogate":
         bench_surrogate(args)
+        return
+    if args.workload == "bct":
+        bench_bct(args)
         return
 
     import ctypes

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -455,6 +456,7 @@ __device__ __forceinline__ PixArgs shfl_args(const PixArgs& a, int lane) {
 // Tier 2: both reflections of every pair in the wave's window, with the reference's
 // per-reflection formula (acos mapping included).  Differs from the reference's sum by the
 // order of the terms and by the window's dropped mass only.
+template <bool PRECISE>
 __device__ void coop_window_sum64(const DevSet& d, const Window& W, const PixArgs& me, bool want, double& inten) {
     unsigned long long m = __ballot(want);
     const int lane = threadIdx.x & 63;
@@ -469,10 +471,12 @@ __device__ void coop_window_sum64(const DevSet& d, const Window& W, const PixArg
             const int32_t* lb = d.lut + band * (d.nlut + 1);
             const int p0 = uniform_i(lb[W.b0] & ~3), p1 = uniform_i((lb[W.b1] + 3) & ~3);
             for (int p = p0 + lane; p < p1; p += 64) {
-                const PairRec P = d.pairs[p];
-                if (P.ia == 0) continue;  // padding record
-                acc += aniso_term_ref(d, P.ia, a.qx, a.qy, a.qz, a.q, a.width, x2phi) +
-                       aniso_term_ref(d, P.ib, a.qx, a.qy, a.qz, a.q, a.width, x2phi);
+                // the precise loop's kinds carry only the FP64 records
+                const int ia = PRECISE ? d.pairs64[p].ia : d.pairs[p].ia;
+                const int ib = PRECISE ? d.pairs64[p].ib : d.pairs[p].ib;
+                if (ia == 0) continue;  // padding record
+                acc += aniso_term_ref(d, ia, a.qx, a.qy, a.qz, a.q, a.width, x2phi) +
+                       aniso_term_ref(d, ib, a.qx, a.qy, a.qz, a.q, a.width, x2phi);
             }
         }
         acc = wave_sum(acc);
@@ -879,7 +883,7 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
             const bool want2 = !need_exact && loose(inten, kPrecise ? kRelErrPrecise : kRelErrFP32);
             if (__ballot(want2)) {
                 if (want2) need_q(s, P3);
-                coop_window_sum64(d, W, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want2, inten);
+                coop_window_sum64<kPrecise>(d, W, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want2, inten);
                 if (stats && want2) atomicAdd(&stats[1], 1ull);
             }
             const bool want3 = need_exact || (want2 && loose(inten, 1.0e-13));
@@ -1162,6 +1166,31 @@ __global__ void k_reduce_sets(const FitSet* __restrict__ sets, int nsets, const 
 using namespace csx;
 
 namespace {
+
+// Host staging memory of a batch, pinned (asynchronous, full-rate H2D copies); grows by
+// reallocation.  Only touched between batched runs, which end with a stream synchronize.
+struct PinnedArena {
+    char* p = nullptr;
+    size_t cap = 0, size = 0;
+    hipError_t reserve(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        const size_t nc = std::max(bytes, 2 * cap);
+        char* q = nullptr;
+        hipError_t e = hipHostMalloc((void**)&q, nc, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        if (size) std::memcpy(q, p, size);
+        if (p) (void)hipHostFree(p);
+        p = q;
+        cap = nc;
+        return hipSuccess;
+    }
+    void clear() { size = 0; }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = size = 0;
+    }
+};
 
 struct DevBuf {
     void* p = nullptr;
@@ -1453,7 +1482,7 @@ struct csx_ctx {
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
     std::vector<int2> btiles;
-    std::vector<char> barena;  // host image of the device arena (tables of all sets)
+    PinnedArena barena;  // host image of the device arena (tables of all sets), pinned
     struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso; };
     std::vector<Pending> bpend;
     DevBuf barena_d, bsets_d, btiles_d, bpart_d, bntiles_d, bsums_d, bcnts_d;
@@ -1620,6 +1649,7 @@ void csx_destroy(csx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    c->barena.release();
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
@@ -1895,7 +1925,9 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
         pd.off_l3 = arena_put(st.arena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
         FastTables ft;
         build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
-        pd.off_pairs = arena_put(st.arena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec));
+        // kinds with the precise loop read only the FP64 records
+        pd.off_pairs = ft.pairs64.empty() ? arena_put(st.arena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec))
+                                          : SIZE_MAX;
         pd.off_pairs64 = ft.pairs64.empty() ? SIZE_MAX
                                             : arena_put(st.arena, ft.pairs64.data(), ft.pairs64.size() * sizeof(PairRec64));
         pd.off_lut = arena_put(st.arena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
@@ -1905,32 +1937,62 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
     }
 }
 
-// Append a built set to the context's pending batch (serial; cheap copies only).
-int csx_stage_commit(csx_ctx* c, const csx_staged* stp) {
-    if (!c || !stp) return CSX_E_ARG;
-    const csx_staged& st = *stp;
-    if (st.rc != CSX_OK) return fail(c, st.rc, st.err);
-    if (!c->bsets.empty() && kind_of(c->bsets[0].d.s.combo_key) != kind_of(st.f.d.s.combo_key))
-        return fail(c, CSX_E_ARG, "all sets of a batch must share the combination");
-    const size_t base = arena_put(c->barena, st.arena.data(), st.arena.size());  // 256-aligned
-    csx_ctx::Pending pd{};
-    pd.off_carr = base + st.pd.off_carr;
-    pd.off_l1 = base + st.pd.off_l1;
-    pd.off_l2 = base + st.pd.off_l2;
-    pd.off_l3 = base + st.pd.off_l3;
-    pd.off_pairs = base + st.pd.off_pairs;
-    pd.off_pairs64 = st.pd.off_pairs64 == SIZE_MAX ? SIZE_MAX : base + st.pd.off_pairs64;
-    pd.off_lut = base + st.pd.off_lut;
-    pd.off_singles = base + st.pd.off_singles;
-    pd.off_iso = base + st.pd.off_iso;
-    FitSet f = st.f;
-    f.first_tile = (int)c->btiles.size();
-    for (int k = 0; k < st.ntiles; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
-    c->bsets.push_back(f);
-    c->bntiles.push_back(st.ntiles);
-    c->bpend.push_back(pd);
+// Append built sets to the context's pending batch: offsets in order, then the byte copies
+// into the pinned arena on up to `threads` threads.
+int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int threads) {
+    if (!c || n < 0 || (n > 0 && !sts)) return CSX_E_ARG;
+    std::vector<size_t> base(n);
+    size_t off = c->barena.size;
+    for (int i = 0; i < n; ++i) {
+        const csx_staged& st = *sts[i];
+        if (st.rc != CSX_OK) return fail(c, st.rc, st.err);
+        const int k0 = c->bsets.empty() ? kind_of(sts[0]->f.d.s.combo_key) : kind_of(c->bsets[0].d.s.combo_key);
+        if (kind_of(st.f.d.s.combo_key) != k0) return fail(c, CSX_E_ARG, "all sets of a batch must share the combination");
+        off = (off + 255) & ~size_t(255);
+        base[i] = off;
+        off += st.arena.size();
+    }
+    HIPCHK(c, c->barena.reserve(off));
+    auto copy = [&](int i) {
+        if (!sts[i]->arena.empty()) std::memcpy(c->barena.p + base[i], sts[i]->arena.data(), sts[i]->arena.size());
+    };
+    const int nt = std::max(1, std::min(threads, n / 16));
+    if (nt == 1) {
+        for (int i = 0; i < n; ++i) copy(i);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                for (int i = t; i < n; i += nt) copy(i);
+            });
+        for (auto& th : pool) th.join();
+    }
+    c->barena.size = off;
+    for (int i = 0; i < n; ++i) {
+        const csx_staged& st = *sts[i];
+        const size_t b = base[i];
+        auto at = [b](size_t o) { return o == SIZE_MAX ? SIZE_MAX : b + o; };
+        csx_ctx::Pending pd{};
+        pd.off_carr = at(st.pd.off_carr);
+        pd.off_l1 = at(st.pd.off_l1);
+        pd.off_l2 = at(st.pd.off_l2);
+        pd.off_l3 = at(st.pd.off_l3);
+        pd.off_pairs = at(st.pd.off_pairs);
+        pd.off_pairs64 = at(st.pd.off_pairs64);
+        pd.off_lut = at(st.pd.off_lut);
+        pd.off_singles = at(st.pd.off_singles);
+        pd.off_iso = at(st.pd.off_iso);
+        FitSet f = st.f;
+        f.first_tile = (int)c->btiles.size();
+        for (int k = 0; k < st.ntiles; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
+        c->bsets.push_back(f);
+        c->bntiles.push_back(st.ntiles);
+        c->bpend.push_back(pd);
+    }
     return CSX_OK;
 }
+
+int csx_stage_commit(csx_ctx* c, const csx_staged* st) { return csx_stage_commit_many(c, 1, &st, 1); }
 
 extern "C" {
 int csx_fit_batch_begin(csx_ctx* c) {
@@ -1958,7 +2020,7 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
     const int nsets = (int)c->bsets.size();
     if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, c->barena_d.ensure(c->barena.size()));
+    HIPCHK(c, c->barena_d.ensure(c->barena.size));
     char* base = (char*)c->barena_d.p;
     for (int i = 0; i < nsets; ++i) {
         FitSet& f = c->bsets[i];
@@ -1968,7 +2030,7 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
             f.d.latpar1 = (const int32_t*)(base + pd.off_l1);
             f.d.latpar2 = (const int32_t*)(base + pd.off_l2);
             f.d.latpar3 = (const float*)(base + pd.off_l3);
-            f.d.pairs = (const PairRec*)(base + pd.off_pairs);
+            f.d.pairs = pd.off_pairs == SIZE_MAX ? nullptr : (const PairRec*)(base + pd.off_pairs);
             f.d.pairs64 = pd.off_pairs64 == SIZE_MAX ? nullptr : (const PairRec64*)(base + pd.off_pairs64);
             f.d.lut = (const int32_t*)(base + pd.off_lut);
             f.d.singles = (const int32_t*)(base + pd.off_singles);
@@ -1982,7 +2044,7 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
     HIPCHK(c, c->bntiles_d.ensure(nsets * sizeof(int)));
     HIPCHK(c, c->bsums_d.ensure(nsets * sizeof(double)));
     HIPCHK(c, c->bcnts_d.ensure(nsets * sizeof(long long)));
-    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.data(), c->barena.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, c->barena.size, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->bsets_d.p, c->bsets.data(), nsets * sizeof(FitSet), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->btiles_d.p, c->btiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->bntiles_d.p, c->bntiles.data(), nsets * sizeof(int), hipMemcpyHostToDevice, c->stream));

@@ -355,11 +355,13 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, cons
         const auto t1 = clk::now();
         int rc = csx_fit_batch_begin(ctx);
         if (rc) return rc;
+        std::vector<const csx_staged*> built(reqs.size());
         for (size_t r = 0; r < reqs.size(); ++r) {
             if (!reqs[r].err.empty()) return CSX_E_PARAM;
-            rc = csx_stage_commit(ctx, slots[r]->p);
-            if (rc) return rc;
+            built[r] = slots[r]->p;
         }
+        rc = csx_stage_commit_many(ctx, (int)built.size(), built.data(), threads);
+        if (rc) return rc;
         sums.resize(reqs.size());
         cnts.resize(reqs.size());
         const auto t2 = clk::now();

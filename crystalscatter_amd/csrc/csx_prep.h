@@ -77,3 +77,5 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
                      const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
                      const double* d_data);
 int csx_stage_commit(csx_ctx* c, const csx_staged* st);
+// n built sets at once, their bytes copied on up to `threads` threads.
+int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int threads);
