@@ -116,8 +116,10 @@ SIGNATURES = [
     ("csx_mlp_destroy", None, [_VP]),
     ("csx_mlp_set_stream", _I, [_VP, _VP]),
     ("csx_mlp_last_error", _I, [_VP, _CP, _I]),
-    ("csx_mlp_get_params", _I, [_VP, _PF, _PF]),
-    ("csx_mlp_set_params", _I, [_VP, _PF, _PF]),
+    ("csx_mlp_get_params", _I, [_VP, _PF]),
+    ("csx_mlp_set_params", _I, [_VP, _PF]),
+    ("csx_mlp_get_features", _I, [_VP, _PF, _PI32, _PF]),
+    ("csx_mlp_set_features", _I, [_VP, _PF, _PI32, _PF]),
     ("csx_mlp_forward", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP]),
     ("csx_mlp_gradient", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _PF]),
     ("csx_mlp_get_grads", _I, [_VP, _PF]),

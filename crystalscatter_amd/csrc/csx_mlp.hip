@@ -3,7 +3,9 @@
 // fit driver uses it to propose starting simplices and to pre-screen vertices, and the exact
 // kernels re-score every point it accepts.
 //
-//   input   6 normalised parameters + 64 Fourier features sin/cos(2*pi*B.q), B 32x2 fixed
+//   input   6 normalised parameters + 64 Fourier features sin/cos(a_f), 32 phases
+//           a_f = 2*pi*(F_f0*qx + F_f1*qy) + F_f2*|q|*v_f, where v_f is the physical value of
+//           parameter pidx_f (offset + scale * normalised value) or 1 (pidx_f < 0)
 //           (70 values, zero-padded to 128 columns)
 //   hidden  4 x 256, SiLU
 //   output  1 (linear)
@@ -62,13 +64,20 @@ __device__ __forceinline__ float silu_grad(float z) {
 }
 
 // ---- input features ----------------------------------------------------------------
+// Feature phases (see the header comment): frequencies F [kNF][3], parameter index per
+// phase, and the affine map normalised -> physical value per parameter.
+struct Features {
+    float f[kNF][3];
+    int32_t pidx[kNF];
+    float off[kNP], sc[kNP];
+};
+
 // Row i < n: parameters of set[i] (already normalised by the caller; an index outside
-// [0, nsets) is clamped, the host checks them), then sin and cos of
-// 2*pi*(B q_i); rows n..mpad-1 are zero.  Writes X [mpad][kIn] and X^T [kIn][mpad].
+// [0, nsets) is clamped, the host checks them), then sin and cos of the kNF phases; rows
+// n..mpad-1 are zero.  Writes X [mpad][kIn] and X^T [kIn][mpad].
 __global__ void __launch_bounds__(256) k_embed(int n, int mpad, int nsets, const float* __restrict__ par,
                                                const int32_t* __restrict__ set, const float* __restrict__ q,
-                                               const float* __restrict__ fb, bf16* __restrict__ X,
-                                               bf16* __restrict__ XT) {
+                                               const Features ft, bf16* __restrict__ X, bf16* __restrict__ XT) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= mpad) return;
     float v[kIn];
@@ -79,10 +88,15 @@ __global__ void __launch_bounds__(256) k_embed(int n, int mpad, int nsets, const
 #pragma unroll
         for (int k = 0; k < kNP; ++k) v[k] = p[k];
         const float qx = q[2 * i], qy = q[2 * i + 1];
+        const float qa = sqrtf(qx * qx + qy * qy);
 #pragma unroll
         for (int f = 0; f < kNF; ++f) {
+            const int j = ft.pidx[f];
+            float pv = 1.0f;  // constant indices only: v stays in registers
+#pragma unroll
+            for (int k = 0; k < kNP; ++k) pv = (k == j) ? ft.off[k] + ft.sc[k] * v[k] : pv;
             float s, c;
-            __sincosf(6.283185307f * (fb[2 * f] * qx + fb[2 * f + 1] * qy), &s, &c);
+            sincosf(6.283185307f * (ft.f[f][0] * qx + ft.f[f][1] * qy) + ft.f[f][2] * qa * pv, &s, &c);
             v[kNP + f] = s;
             v[kNP + kNF + f] = c;
         }
@@ -433,9 +447,9 @@ struct csx_mlp {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
-    float fb[2 * kNF];
+    Features feat;
     // parameters
-    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr, *fbd = nullptr;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     bf16 *wb = nullptr, *wtb = nullptr;  // hidden weights, [out][in] and [in][out]
     int64_t adam_t = 0;
     // per-row buffers
@@ -493,8 +507,8 @@ int check_rows(csx_mlp* m, int n, int nsets) {
 // Forward pass over n rows (padded to a multiple of 128) into m->y.
 int forward(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q) {
     const int mpad = (n + kBM - 1) / kBM * kBM;
-    hipLaunchKernelGGL(k_embed, dim3((mpad + 255) / 256), dim3(256), 0, m->stream, n, mpad, nsets, d_par, d_set, d_q, m->fbd,
-                       m->X, m->XT);
+    hipLaunchKernelGGL(k_embed, dim3((mpad + 255) / 256), dim3(256), 0, m->stream, n, mpad, nsets, d_par, d_set, d_q,
+                       m->feat, m->X, m->XT);
     MCHK(m, hipGetLastError());
     for (int l = 0; l < kL; ++l) {
         EpiArgs ep{};
@@ -538,7 +552,7 @@ int csx_mlp_create(int device, int max_batch, uint64_t seed, csx_mlp** out) {
     const int64_t nw = (int64_t)kH * kIn + (int64_t)(kL - 1) * kH * kH;
     ok = ok && alloc((void**)&m->p, kNParams * 4) && alloc((void**)&m->g, kNParams * 4) &&
          alloc((void**)&m->m1, kNParams * 4) && alloc((void**)&m->m2, kNParams * 4) &&
-         alloc((void**)&m->fbd, 2 * kNF * 4) && alloc((void**)&m->wb, nw * 2) && alloc((void**)&m->wtb, nw * 2) &&
+         alloc((void**)&m->wb, nw * 2) && alloc((void**)&m->wtb, nw * 2) &&
          alloc((void**)&m->X, M * kIn * 2) && alloc((void**)&m->XT, M * kIn * 2) && alloc((void**)&m->y, M * 4) &&
          alloc((void**)&m->dy, M * 4) && alloc((void**)&m->part, (size_t)m->splits * kH * kH * 4) &&
          alloc((void**)&m->rpart, (size_t)kH * ((M + kSeg - 1) / kSeg) * 4) &&
@@ -564,13 +578,16 @@ int csx_mlp_create(int device, int max_batch, uint64_t seed, csx_mlp** out) {
                 hp[w_off(l) + (int64_t)o * in_of(l) + k] = (l == 0 && k >= kUsed) ? 0.0f : (float)((2 * r.uni() - 1) * bound);
     }
     for (int c = 0; c < kH; ++c) hp[kOutW + c] = (float)((2 * r.uni() - 1) * std::sqrt(3.0 / kH));
+    // default phases: 2*pi*F.q with log-uniform magnitudes in [0.5, 16] and uniform directions
+    std::memset(&m->feat, 0, sizeof m->feat);
     for (int f = 0; f < kNF; ++f) {
         const double mag = 0.5 * std::pow(32.0, r.uni()), ang = 2 * M_PI * r.uni();
-        m->fb[2 * f] = (float)(mag * std::cos(ang));
-        m->fb[2 * f + 1] = (float)(mag * std::sin(ang));
+        m->feat.f[f][0] = (float)(mag * std::cos(ang));
+        m->feat.f[f][1] = (float)(mag * std::sin(ang));
+        m->feat.pidx[f] = -1;
     }
+    for (int j = 0; j < kNP; ++j) m->feat.sc[j] = 1.0f;
     ok = hipMemcpyAsync(m->p, hp.data(), kNParams * 4, hipMemcpyHostToDevice, m->stream) == hipSuccess &&
-         hipMemcpyAsync(m->fbd, m->fb, sizeof m->fb, hipMemcpyHostToDevice, m->stream) == hipSuccess &&
          hipMemsetAsync(m->m1, 0, kNParams * 4, m->stream) == hipSuccess &&
          hipMemsetAsync(m->m2, 0, kNParams * 4, m->stream) == hipSuccess &&
          hipMemsetAsync(m->g, 0, kNParams * 4, m->stream) == hipSuccess && cast_weights(m) == CSX_OK &&
@@ -587,7 +604,7 @@ void csx_mlp_destroy(csx_mlp* m) {
     if (!m) return;
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    for (void* q : {(void*)m->p, (void*)m->g, (void*)m->m1, (void*)m->m2, (void*)m->fbd, (void*)m->wb, (void*)m->wtb,
+    for (void* q : {(void*)m->p, (void*)m->g, (void*)m->m1, (void*)m->m2, (void*)m->wb, (void*)m->wtb,
                     (void*)m->X, (void*)m->XT, (void*)m->y, (void*)m->dy, (void*)m->part, (void*)m->rpart,
                     (void*)m->hpart_b, (void*)m->loss})
         if (q) (void)hipFree(q);
@@ -616,16 +633,45 @@ int csx_mlp_set_stream(csx_mlp* m, void* stream) {
     return CSX_OK;
 }
 
-int csx_mlp_get_params(csx_mlp* m, float* h_params, float* h_fourier) {
-    if (!m) return CSX_E_ARG;
+int csx_mlp_get_params(csx_mlp* m, float* h_params) {
+    if (!m || !h_params) return CSX_E_ARG;
     MCHK(m, hipSetDevice(m->device));
-    if (h_params) MCHK(m, hipMemcpyAsync(h_params, m->p, kNParams * 4, hipMemcpyDeviceToHost, m->stream));
-    if (h_fourier) std::memcpy(h_fourier, m->fb, sizeof m->fb);
+    MCHK(m, hipMemcpyAsync(h_params, m->p, kNParams * 4, hipMemcpyDeviceToHost, m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
     return CSX_OK;
 }
 
-int csx_mlp_set_params(csx_mlp* m, const float* h_params, const float* h_fourier) {
+int csx_mlp_get_features(const csx_mlp* m, float* h_freq, int32_t* h_pidx, float* h_affine) {
+    if (!m) return CSX_E_ARG;
+    for (int f = 0; f < kNF; ++f) {
+        if (h_freq) for (int c = 0; c < 3; ++c) h_freq[3 * f + c] = m->feat.f[f][c];
+        if (h_pidx) h_pidx[f] = m->feat.pidx[f];
+    }
+    if (h_affine)
+        for (int j = 0; j < kNP; ++j) {
+            h_affine[2 * j] = m->feat.off[j];
+            h_affine[2 * j + 1] = m->feat.sc[j];
+        }
+    return CSX_OK;
+}
+
+int csx_mlp_set_features(csx_mlp* m, const float* h_freq, const int32_t* h_pidx, const float* h_affine) {
+    if (!m || !h_freq || !h_pidx || !h_affine) return CSX_E_ARG;
+    for (int f = 0; f < kNF; ++f)
+        if (h_pidx[f] < -1 || h_pidx[f] >= kNP) return fail(m, CSX_E_ARG, "parameter index of a phase out of range");
+    MCHK(m, hipStreamSynchronize(m->stream));  // launches already queued keep their copy
+    for (int f = 0; f < kNF; ++f) {
+        for (int c = 0; c < 3; ++c) m->feat.f[f][c] = h_freq[3 * f + c];
+        m->feat.pidx[f] = h_pidx[f];
+    }
+    for (int j = 0; j < kNP; ++j) {
+        m->feat.off[j] = h_affine[2 * j];
+        m->feat.sc[j] = h_affine[2 * j + 1];
+    }
+    return CSX_OK;
+}
+
+int csx_mlp_set_params(csx_mlp* m, const float* h_params) {
     if (!m) return CSX_E_ARG;
     MCHK(m, hipSetDevice(m->device));
     if (h_params) {
@@ -637,10 +683,6 @@ int csx_mlp_set_params(csx_mlp* m, const float* h_params, const float* h_fourier
         MCHK(m, hipStreamSynchronize(m->stream));
         int rc = cast_weights(m);
         if (rc) return rc;
-    }
-    if (h_fourier) {
-        std::memcpy(m->fb, h_fourier, sizeof m->fb);
-        MCHK(m, hipMemcpyAsync(m->fbd, m->fb, sizeof m->fb, hipMemcpyHostToDevice, m->stream));
     }
     m->adam_t = 0;
     MCHK(m, hipMemsetAsync(m->m1, 0, kNParams * 4, m->stream));

@@ -55,16 +55,28 @@ class MLP:
 
     def get_params(self):
         p = np.empty(self.nparams, dtype=np.float32)
-        fb = np.empty(2 * self.NFREQ, dtype=np.float32)
-        self._chk(self._L.csx_mlp_get_params(self._h, p.ctypes.data_as(_lib._PF), fb.ctypes.data_as(_lib._PF)),
-                  "get_params")
-        return p, fb
+        self._chk(self._L.csx_mlp_get_params(self._h, p.ctypes.data_as(_lib._PF)), "get_params")
+        return p
 
-    def set_params(self, p=None, fb=None):
-        pa = None if p is None else np.ascontiguousarray(p, dtype=np.float32)
-        fa = None if fb is None else np.ascontiguousarray(fb, dtype=np.float32)
-        self._chk(self._L.csx_mlp_set_params(self._h, None if pa is None else pa.ctypes.data_as(_lib._PF),
-                                             None if fa is None else fa.ctypes.data_as(_lib._PF)), "set_params")
+    def set_params(self, p):
+        pa = np.ascontiguousarray(p, dtype=np.float32)
+        self._chk(self._L.csx_mlp_set_params(self._h, pa.ctypes.data_as(_lib._PF)), "set_params")
+
+    def get_features(self):
+        """(F [NFREQ, 3], pidx [NFREQ], affine [NPAR, 2]) of the input phases."""
+        F = np.empty((self.NFREQ, 3), dtype=np.float32)
+        pi = np.empty(self.NFREQ, dtype=np.int32)
+        af = np.empty((self.NPAR, 2), dtype=np.float32)
+        self._chk(self._L.csx_mlp_get_features(self._h, F.ctypes.data_as(_lib._PF), pi.ctypes.data_as(_lib._PI32),
+                                               af.ctypes.data_as(_lib._PF)), "get_features")
+        return F, pi, af
+
+    def set_features(self, F, pidx, affine):
+        F = np.ascontiguousarray(F, dtype=np.float32).reshape(self.NFREQ, 3)
+        pi = np.ascontiguousarray(pidx, dtype=np.int32).reshape(self.NFREQ)
+        af = np.ascontiguousarray(affine, dtype=np.float32).reshape(self.NPAR, 2)
+        self._chk(self._L.csx_mlp_set_features(self._h, F.ctypes.data_as(_lib._PF), pi.ctypes.data_as(_lib._PI32),
+                                               af.ctypes.data_as(_lib._PF)), "set_features")
 
     def get_grads(self):
         g = np.empty(self.nparams, dtype=np.float32)
@@ -133,6 +145,26 @@ FIT_HI = np.array([6.0, 60.0, 40.0, 80.0, 400.0, 10.0])
 # lattice with Gaussian peaks, Debye-Scherrer widths, q in [-1, 1] nm^-1.
 BCT_BASE = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, ComboBoxInterior=0, theta=0, phi=0,
                 EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1, Base=0, HKLmax=5)
+
+
+def physics_features(seed=5):
+    """Input phases for the BCT-cylinder fits.  Bragg rings sit at |q| proportional to 1/a
+    and 1/c, the cylinders' form factor oscillates in |q| R: eight harmonics of |q| a, of
+    |q| c and of |q| R, plus eight plain Fourier features of (qx, qy) for the anisotropy."""
+    rng = np.random.default_rng(seed)
+    F = np.zeros((32, 3))
+    pidx = np.full(32, -1, dtype=np.int32)
+    lattice = [0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 2.5]
+    for k, m in enumerate(lattice):
+        F[k, 2], pidx[k] = m, 2           # |q| a
+        F[8 + k, 2], pidx[8 + k] = m, 3   # |q| c
+    for k, m in enumerate([0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 4.0, 5.0]):
+        F[16 + k, 2], pidx[16 + k] = m, 0  # |q| R
+    for k in range(8):
+        mag, ang = 0.5 * 32.0 ** rng.uniform(), 2 * np.pi * rng.uniform()
+        F[24 + k, :2] = mag * np.cos(ang), mag * np.sin(ang)
+    affine = np.stack([0.5 * (FIT_LO + FIT_HI), 0.5 * (FIT_HI - FIT_LO)], axis=1)
+    return F, pidx, affine
 
 
 def normalize(x):

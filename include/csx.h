@@ -221,12 +221,13 @@ int csx_engine_last_error(csx_engine* e, char* buf, int len);
 /* ---- Surrogate network (no counterpart in the reference) --------------------------------
  * The learned part of the fits: an MLP from a normalised parameter set (CSX_MLP_NPAR values)
  * and a pixel's (qx, qy) to the normalised log-intensity of that pixel.  Inputs: the
- * parameters and sin/cos(2*pi*B.q) for a fixed CSX_MLP_NFREQ x 2 matrix B (CSX_MLP_IN values,
- * zero-padded to CSX_MLP_IN_PAD); CSX_MLP_LAYERS hidden layers of CSX_MLP_HIDDEN SiLU units;
- * one linear output.  BF16 weights and activations on the matrix cores with FP32 accumulation;
- * FP32 master weights and Adam state.  Samples are rows: row i uses parameter set set[i] of
- * d_par (nsets x CSX_MLP_NPAR floats; an index outside [0, nsets) is clamped into it) and
- * d_q[2i], d_q[2i+1].  All pointers named d_* are
+ * parameters and sin/cos of CSX_MLP_NFREQ phases a_f = 2*pi*(F_f0*qx + F_f1*qy) +
+ * F_f2*|q|*v_f, where v_f = offset_j + scale_j * (normalised parameter j) for j = pidx_f, or 1
+ * for pidx_f = -1 (CSX_MLP_IN values, zero-padded to CSX_MLP_IN_PAD); CSX_MLP_LAYERS hidden
+ * layers of CSX_MLP_HIDDEN SiLU units; one linear output.  BF16 weights and activations on
+ * the matrix cores with FP32 accumulation; FP32 master weights and Adam state.  Samples are
+ * rows: row i uses parameter set set[i] of d_par (nsets x CSX_MLP_NPAR floats; an index
+ * outside [0, nsets) is clamped into it) and d_q[2i], d_q[2i+1].  All pointers named d_* are
  * device memory; calls are ordered on the model's stream. */
 #define CSX_MLP_NPAR 6
 #define CSX_MLP_NFREQ 32
@@ -244,10 +245,14 @@ int csx_mlp_create(int device, int max_batch, uint64_t seed, csx_mlp** out);
 void csx_mlp_destroy(csx_mlp* m);
 int csx_mlp_set_stream(csx_mlp* m, void* stream);
 int csx_mlp_last_error(const csx_mlp* m, char* buf, int len);
-/* Host copies of the FP32 parameters (csx_mlp_nparams floats) and of B (2*NFREQ floats,
- * row-major); either pointer may be NULL.  set_params also resets the Adam state. */
-int csx_mlp_get_params(csx_mlp* m, float* h_params, float* h_fourier);
-int csx_mlp_set_params(csx_mlp* m, const float* h_params, const float* h_fourier);
+/* Host copies of the FP32 parameters (csx_mlp_nparams floats); set_params also resets the
+ * Adam state. */
+int csx_mlp_get_params(csx_mlp* m, float* h_params);
+int csx_mlp_set_params(csx_mlp* m, const float* h_params);
+/* Input phases: F (NFREQ x 3, row-major), pidx (NFREQ), affine (NPAR x {offset, scale}).
+ * The default after create: F_f2 = 0, pidx = -1 (plain Fourier features of (qx, qy)). */
+int csx_mlp_get_features(const csx_mlp* m, float* h_freq, int32_t* h_pidx, float* h_affine);
+int csx_mlp_set_features(csx_mlp* m, const float* h_freq, const int32_t* h_pidx, const float* h_affine);
 /* d_y[i] = network(row i), n rows (asynchronous). */
 int csx_mlp_forward(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
                     float* d_y);

@@ -66,15 +66,25 @@ def unpack(p):
     return Ws, bs, p[wo:wo + HID], p[bo]
 
 
-def features(par, sets, q, fb):
-    """Rows of inputs: par[sets[i]] (6), sin(2 pi B q_i) (32), cos(2 pi B q_i) (32), zeros."""
+def features(par, sets, q, freq, pidx, affine):
+    """Rows of inputs: par[sets[i]] (6), sin(a_f) (32), cos(a_f) (32), zeros, with phases
+    a_f = 2 pi (F_f0 qx + F_f1 qy) + F_f2 |q| v_f, v_f = offset_j + scale_j par_j for
+    j = pidx_f >= 0, else 1."""
     par = np.asarray(par, dtype=np.float64).reshape(-1, NPAR)
     q = np.asarray(q, dtype=np.float64).reshape(-1, 2)
-    fb = np.asarray(fb, dtype=np.float64).reshape(NFREQ, 2)
+    F = np.asarray(freq, dtype=np.float64).reshape(NFREQ, 3)
+    pidx = np.asarray(pidx).reshape(NFREQ)
+    affine = np.asarray(affine, dtype=np.float64).reshape(NPAR, 2)
     n = q.shape[0]
+    P = par[np.asarray(sets)]
     X = np.zeros((n, IN_PAD))
-    X[:, :NPAR] = par[np.asarray(sets)]
-    a = 2 * np.pi * (q @ fb.T)
+    X[:, :NPAR] = P
+    qa = np.sqrt(q[:, 0] ** 2 + q[:, 1] ** 2)
+    a = 2 * np.pi * (q @ F[:, :2].T)
+    for f in range(NFREQ):
+        j = int(pidx[f])
+        v = affine[j, 0] + affine[j, 1] * P[:, j] if j >= 0 else 1.0
+        a[:, f] += F[f, 2] * qa * v
     X[:, NPAR:NPAR + NFREQ] = np.sin(a)
     X[:, NPAR + NFREQ:IN] = np.cos(a)
     return X

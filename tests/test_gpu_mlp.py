@@ -13,9 +13,12 @@ import mlp_oracle as mo  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-def _net(max_batch=4096, seed=11):
-    from crystalscatter_amd.surrogate import MLP
-    return MLP(device=0, max_batch=max_batch, seed=seed)
+def _net(max_batch=4096, seed=11, physics=False):
+    from crystalscatter_amd.surrogate import MLP, physics_features
+    net = MLP(device=0, max_batch=max_batch, seed=seed)
+    if physics:
+        net.set_features(*physics_features())
+    return net
 
 
 def _inputs(n, nsets=5, seed=0):
@@ -29,13 +32,13 @@ def _inputs(n, nsets=5, seed=0):
     return (par, sets, q, t), (dev(par), dev(sets), dev(q), dev(t))
 
 
-@pytest.mark.parametrize("n", [1000, 4096])
-def test_forward_matches_oracle(n):
-    net = _net()
+@pytest.mark.parametrize("n,physics", [(1000, False), (4096, False), (1000, True)])
+def test_forward_matches_oracle(n, physics):
+    net = _net(physics=physics)
     (par, sets, q, _), (dpar, dsets, dq, _) = _inputs(n)
     y = net.forward(dpar, dsets, dq).cpu().numpy().astype(np.float64)
-    p, fb = net.get_params()
-    X = mo.features(par, sets, q, fb)
+    p = net.get_params()
+    X = mo.features(par, sets, q, *net.get_features())
     yb, _ = mo.forward(p.astype(np.float64), X, bf16=True)
     y64, _ = mo.forward(p.astype(np.float64), X, bf16=False)
     scale = np.abs(y64).max()
@@ -48,12 +51,12 @@ def test_forward_matches_oracle(n):
 
 def test_gradient_matches_oracle():
     n = 1500
-    net = _net()
+    net = _net(physics=True)
     (par, sets, q, t), (dpar, dsets, dq, dt) = _inputs(n, seed=4)
     loss = net.gradient(dpar, dsets, dq, dt)
     g = net.get_grads().astype(np.float64)
-    p, fb = net.get_params()
-    X = mo.features(par, sets, q, fb)
+    p = net.get_params()
+    X = mo.features(par, sets, q, *net.get_features())
     lb, gb = mo.loss_and_grad(p.astype(np.float64), X, t, bf16=True)
     l64, g64 = mo.loss_and_grad(p.astype(np.float64), X, t, bf16=False)
     assert abs(loss - lb) <= 1e-3 * lb
@@ -76,11 +79,11 @@ def test_adam_step_follows_the_update_rule():
     n = 512
     net = _net(seed=5)
     (_, _, _, _), (dpar, dsets, dq, dt) = _inputs(n, seed=5)
-    p0, _ = net.get_params()
+    p0 = net.get_params()
     net.gradient(dpar, dsets, dq, dt)
     g = net.get_grads()
     net._chk(net._L.csx_mlp_adam(net._h, 1e-3, 0.9, 0.999, 1e-8), "adam")
-    p1, _ = net.get_params()
+    p1 = net.get_params()
     # first Adam step: m^ = g, v^ = g^2 -> p1 = p0 - lr * g / (|g| + eps)
     exp = p0 - 1e-3 * g / (np.abs(g) + 1e-8)
     assert np.allclose(p1, exp, rtol=1e-5, atol=1e-7)

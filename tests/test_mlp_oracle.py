@@ -25,8 +25,9 @@ def _problem(n=12, seed=3):
     par = rng.uniform(-1, 1, (3, mo.NPAR))
     sets = rng.integers(0, 3, n)
     q = rng.uniform(-1, 1, (n, 2))
-    fb = rng.normal(0, 2, (mo.NFREQ, 2))
-    X = mo.features(par, sets, q, fb)
+    F = np.zeros((mo.NFREQ, 3))
+    F[:, :2] = rng.normal(0, 2, (mo.NFREQ, 2))
+    X = mo.features(par, sets, q, F, np.full(mo.NFREQ, -1), np.tile([0.0, 1.0], (mo.NPAR, 1)))
     t = rng.normal(0, 1, n)
     return p, X, t
 
@@ -39,13 +40,21 @@ def test_parameter_count_and_layout():
 
 def test_features_are_parameters_then_sin_cos():
     par = np.array([[0.1, -0.2, 0.3, -0.4, 0.5, -0.6]])
-    fb = np.zeros((mo.NFREQ, 2))
-    fb[0] = (1.0, 0.0)
-    X = mo.features(par, [0], [[0.25, 0.7]], fb)
+    F = np.zeros((mo.NFREQ, 3))
+    F[0, :2] = (1.0, 0.0)
+    F[1, 2] = 2.0                              # 2 |q| (offset + scale * par_3)
+    pidx = np.full(mo.NFREQ, -1)
+    pidx[1] = 3
+    affine = np.tile([0.0, 1.0], (mo.NPAR, 1))
+    affine[3] = (10.0, 5.0)                    # physical value 10 + 5 * (-0.4) = 8
+    q = np.array([[0.25, 0.7]])
+    X = mo.features(par, [0], q, F, pidx, affine)
     assert np.allclose(X[0, :6], par[0])
     assert np.isclose(X[0, 6], 1.0)          # sin(2 pi * 0.25)
     assert np.isclose(X[0, 6 + mo.NFREQ], 0.0, atol=1e-15)
-    assert np.all(X[0, 7:6 + mo.NFREQ] == 0) and np.all(X[0, 7 + mo.NFREQ:mo.IN] == 1)
+    a1 = 2.0 * np.hypot(0.25, 0.7) * 8.0
+    assert np.isclose(X[0, 7], np.sin(a1)) and np.isclose(X[0, 7 + mo.NFREQ], np.cos(a1))
+    assert np.all(X[0, 8:6 + mo.NFREQ] == 0) and np.all(X[0, 8 + mo.NFREQ:mo.IN] == 1)
     assert np.all(X[0, mo.IN:] == 0)
 
 
