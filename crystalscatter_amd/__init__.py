@@ -17,7 +17,8 @@ import numpy as np
 from ._lib import (CSX_PREC_MIXED, CSX_PREC_REF64, CsxError, Grid, Mask, NCOEF, NIM2, Scalars,
                    lib)
 
-__all__ = ["SasCalc", "Context", "CsxError", "CSX_PREC_MIXED", "CSX_PREC_REF64", "PRECISIONS"]
+__all__ = ["SasCalc", "Context", "CsxError", "CSX_PREC_MIXED", "CSX_PREC_REF64", "PRECISIONS", "simplex",
+           "nm_batch", "nm_eval"]
 
 PRECISIONS = {"ref64": CSX_PREC_REF64, "mixed": CSX_PREC_MIXED, CSX_PREC_REF64: CSX_PREC_REF64,
               CSX_PREC_MIXED: CSX_PREC_MIXED}
@@ -220,13 +221,7 @@ def simplex(f, x0, lo, hi, stp=0.1, maxit=200, ftol=1e-6):
     return np.array(r.x[:n]), r.f, r.iters, r.evals
 
 
-def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", threads=0):
-    """Batched Nelder-Mead (csx_nm_batch).  fits: list of dicts with 'engine' (SasCalc, may be
-    host-only), 'd_data' (device pointer), 'w', 'h', 'x0', 'lo', 'hi', optional 'bstop',
-    'border', 'rects'.  Returns (list of (x, f, iters, evals), stats dict: rounds, evals,
-    prep_ms, stage_ms, run_ms)."""
-    L = _lib.lib()
-    n = len(keys)
+def _nm_fits(fits, n):
     arr = (_lib.NMFit * len(fits))()
     for i, fd in enumerate(fits):
         a = arr[i]
@@ -240,7 +235,37 @@ def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", 
                 a.mask.rects[4 * r + k] = rect[k]
         a.mask.bstop, a.mask.border = fd.get("bstop", 0), fd.get("border", 0)
         for j in range(n):
-            a.x0[j], a.lo[j], a.hi[j] = fd["x0"][j], fd["lo"][j], fd["hi"][j]
+            a.x0[j], a.lo[j], a.hi[j] = fd.get("x0", fd["lo"])[j], fd["lo"][j], fd["hi"][j]
+    return arr
+
+
+def nm_eval(ctx, fits, keys, X, precision="mixed", threads=0):
+    """The batched objective (csx_nm_eval) at points X[i] (nv x npar) of every fit i, in one
+    launch; returns an (nfits, nv) array (1e20 outside the bounds)."""
+    L = _lib.lib()
+    n = len(keys)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    nf, nv = X.shape[0], X.shape[1]
+    if X.shape != (len(fits), nv, n):
+        raise ValueError("X must be (nfits, nv, npar)")
+    arr = _nm_fits(fits, n)
+    ckeys = (ctypes.c_char_p * n)(*[k.encode() for k in keys])
+    out = np.empty((nf, nv))
+    rc = L.csx_nm_eval(ctx._h if isinstance(ctx, Context) else ctx, nf, arr, n, ckeys, nv, _dp(X), _prec(precision),
+                       threads, _dp(out))
+    if rc != 0:
+        raise CsxError("csx_nm_eval failed (%d)" % rc)
+    return out
+
+
+def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", threads=0):
+    """Batched Nelder-Mead (csx_nm_batch).  fits: list of dicts with 'engine' (SasCalc, may be
+    host-only), 'd_data' (device pointer), 'w', 'h', 'x0', 'lo', 'hi', optional 'bstop',
+    'border', 'rects'.  Returns (list of (x, f, iters, evals), stats dict: rounds, evals,
+    prep_ms, stage_ms, run_ms)."""
+    L = _lib.lib()
+    n = len(keys)
+    arr = _nm_fits(fits, n)
     ckeys = (ctypes.c_char_p * n)(*[k.encode() for k in keys])
     res = (_lib.NMResult * len(fits))()
     st = _lib.NMStats()

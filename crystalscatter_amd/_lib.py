@@ -94,6 +94,7 @@ SIGNATURES = [
     ("csx_nm_run_cb", _I, [_I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
     ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
                           ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),
+    ("csx_nm_eval", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _I, _PD, _I, _I, _PD]),
     ("csx_last_error", _I, [_VP, _CP, _I]),
     ("csx_last_kernel_ms", _D, [_VP]),
     ("csx_set_timing", _I, [_VP, _I]),

@@ -298,24 +298,67 @@ int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const
     return CSX_OK;
 }
 
-int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
-                 int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats) {
-    if (!ctx || nfits < 1 || !fits || npar < 1 || npar > CSX_NM_MAXPAR || !keys || !out) return CSX_E_ARG;
-    std::vector<NelderMead> nm(nfits);
-    std::vector<const csx::ParamSet*> base(nfits);
-    for (int i = 0; i < nfits; ++i) {
-        base[i] = csx_engine_paramset(fits[i].base);
-        if (!base[i]) return CSX_E_ARG;
-        nm[i].init(npar, fits[i].x0, fits[i].lo, fits[i].hi, stp, maxit, ftol);
-    }
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
-    struct Req {
-        int fit, k;
-        csx::PreparedSet P;
-        std::string err;
+}  // extern "C"
+
+namespace {
+
+// Evaluator of batched points: a point is (fit, parameter vector); one call prepares every
+// point on host threads (prepareCalculation + the host part of staging), commits them and
+// evaluates them in one batched launch.  Staging slots are kept across calls.
+class PointEval {
+public:
+    PointEval(csx_ctx* ctx, const csx_nm_fit* fits, const std::vector<const csx::ParamSet*>& base, int npar,
+              const char* const* keys, int precision, int threads)
+        : ctx_(ctx), fits_(fits), base_(base), npar_(npar), keys_(keys), prec_(precision), threads_(threads) {}
+
+    struct Point {
+        int fit;
+        const double* x;
     };
-    std::vector<Req> reqs;
-    // one staging slot per request, reused across rounds
+
+    int run(const std::vector<Point>& pts, double* sums) {
+        using clk = std::chrono::steady_clock;
+        auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        const int n = (int)pts.size();
+        while ((int)slots_.size() < n) slots_.push_back(std::make_unique<Slot>());
+        errs_.assign(n, std::string());
+        preps_.resize(std::max(preps_.size(), (size_t)n));
+        const auto t0 = clk::now();
+        parallel_for(n, threads_, [&](int r) {
+            csx::ParamSet ps = *base_[pts[r].fit];
+            for (int j = 0; j < npar_; ++j) set_tied(ps, keys_[j], pts[r].x[j]);
+            csx::PreparedSet& P = preps_[r];
+            errs_[r] = csx::prepare(ps, P);
+            if (!errs_[r].empty()) return;
+            const csx_nm_fit& F = fits_[pts[r].fit];
+            csx_stage_build(slots_[r]->p, &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
+                            P.latpar2.empty() ? nullptr : P.latpar2.data(),
+                            P.latpar3.empty() ? nullptr : P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
+        });
+        const auto t1 = clk::now();
+        int rc = csx_fit_batch_begin(ctx_);
+        if (rc) return rc;
+        std::vector<const csx_staged*> built(n);
+        for (int r = 0; r < n; ++r) {
+            if (!errs_[r].empty()) return CSX_E_PARAM;
+            built[r] = slots_[r]->p;
+        }
+        rc = csx_stage_commit_many(ctx_, n, built.data(), threads_);
+        if (rc) return rc;
+        cnts_.resize(n);
+        const auto t2 = clk::now();
+        rc = csx_fit_batch_run(ctx_, prec_, sums, cnts_.data());
+        if (rc) return rc;
+        const auto t3 = clk::now();
+        t_prep += ms(t0, t1);
+        t_stage += ms(t1, t2);
+        t_run += ms(t2, t3);
+        return CSX_OK;
+    }
+
+    double t_prep = 0, t_stage = 0, t_run = 0;
+
+private:
     struct Slot {
         csx_staged* p = csx_staged_new();
         ~Slot() { csx_staged_free(p); }
@@ -323,54 +366,52 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, cons
         Slot(const Slot&) = delete;
         Slot& operator=(const Slot&) = delete;
     };
-    std::vector<std::unique_ptr<Slot>> slots;
+    csx_ctx* ctx_;
+    const csx_nm_fit* fits_;
+    const std::vector<const csx::ParamSet*>& base_;
+    int npar_;
+    const char* const* keys_;
+    int prec_, threads_;
+    std::vector<std::unique_ptr<Slot>> slots_;
+    std::vector<csx::PreparedSet> preps_;
+    std::vector<std::string> errs_;
+    std::vector<int64_t> cnts_;
+};
+
+int check_fits(int nfits, const csx_nm_fit* fits, std::vector<const csx::ParamSet*>& base) {
+    base.resize(nfits);
+    for (int i = 0; i < nfits; ++i) {
+        base[i] = csx_engine_paramset(fits[i].base);
+        if (!base[i]) return CSX_E_ARG;
+    }
+    return CSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
+                 int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats) {
+    if (!ctx || nfits < 1 || !fits || npar < 1 || npar > CSX_NM_MAXPAR || !keys || !out) return CSX_E_ARG;
+    std::vector<const csx::ParamSet*> base;
+    if (check_fits(nfits, fits, base)) return CSX_E_ARG;
+    std::vector<NelderMead> nm(nfits);
+    for (int i = 0; i < nfits; ++i) nm[i].init(npar, fits[i].x0, fits[i].lo, fits[i].hi, stp, maxit, ftol);
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    PointEval ev(ctx, fits, base, npar, keys, precision, threads);
+    std::vector<PointEval::Point> pts;
     std::vector<double> sums;
-    std::vector<int64_t> cnts;
     int64_t rounds = 0, evals = 0;
-    double t_prep = 0, t_stage = 0, t_run = 0;
-    using clk = std::chrono::steady_clock;
-    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     for (;;) {
-        reqs.clear();
+        pts.clear();
         for (int i = 0; i < nfits; ++i)
-            for (int k = 0; k < nm[i].npending(); ++k) reqs.push_back(Req{i, k, {}, {}});
-        if (reqs.empty()) break;
+            for (int k = 0; k < nm[i].npending(); ++k) pts.push_back({i, nm[i].pending(k)});
+        if (pts.empty()) break;
         ++rounds;
-        evals += (int64_t)reqs.size();
-        while (slots.size() < reqs.size()) slots.push_back(std::make_unique<Slot>());
-        const auto t0 = clk::now();
-        // preparation and the host part of staging, one vertex per task
-        parallel_for((int)reqs.size(), threads, [&](int r) {
-            Req& q = reqs[r];
-            csx::ParamSet ps = *base[q.fit];
-            const double* x = nm[q.fit].pending(q.k);
-            for (int j = 0; j < npar; ++j) set_tied(ps, keys[j], x[j]);
-            q.err = csx::prepare(ps, q.P);
-            if (!q.err.empty()) return;
-            const csx_nm_fit& F = fits[q.fit];
-            csx_stage_build(slots[r]->p, &q.P.s, q.P.carr.data(), q.P.latpar1.empty() ? nullptr : q.P.latpar1.data(),
-                            q.P.latpar2.empty() ? nullptr : q.P.latpar2.data(),
-                            q.P.latpar3.empty() ? nullptr : q.P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
-        });
-        const auto t1 = clk::now();
-        int rc = csx_fit_batch_begin(ctx);
-        if (rc) return rc;
-        std::vector<const csx_staged*> built(reqs.size());
-        for (size_t r = 0; r < reqs.size(); ++r) {
-            if (!reqs[r].err.empty()) return CSX_E_PARAM;
-            built[r] = slots[r]->p;
-        }
-        rc = csx_stage_commit_many(ctx, (int)built.size(), built.data(), threads);
-        if (rc) return rc;
-        sums.resize(reqs.size());
-        cnts.resize(reqs.size());
-        const auto t2 = clk::now();
-        rc = csx_fit_batch_run(ctx, precision, sums.data(), cnts.data());
-        if (rc) return rc;
-        const auto t3 = clk::now();
-        t_prep += ms(t0, t1);
-        t_stage += ms(t1, t2);
-        t_run += ms(t2, t3);
+        evals += (int64_t)pts.size();
+        sums.resize(pts.size());
+        if (int rc = ev.run(pts, sums.data())) return rc;
         size_t r = 0;
         for (int i = 0; i < nfits; ++i) {
             const int np = nm[i].npending();
@@ -383,10 +424,40 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, cons
     if (stats) {
         stats->rounds = rounds;
         stats->evals = evals;
-        stats->prep_ms = t_prep;
-        stats->stage_ms = t_stage;
-        stats->run_ms = t_run;
+        stats->prep_ms = ev.t_prep;
+        stats->stage_ms = ev.t_stage;
+        stats->run_ms = ev.t_run;
     }
+    return CSX_OK;
+}
+
+int csx_nm_eval(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, int nv,
+                const double* x, int precision, int threads, double* out) {
+    if (!ctx || nfits < 1 || !fits || npar < 1 || npar > CSX_NM_MAXPAR || !keys || nv < 1 || !x || !out)
+        return CSX_E_ARG;
+    std::vector<const csx::ParamSet*> base;
+    if (check_fits(nfits, fits, base)) return CSX_E_ARG;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<PointEval::Point> pts;
+    std::vector<size_t> where;
+    for (int i = 0; i < nfits; ++i)
+        for (int v = 0; v < nv; ++v) {
+            const double* xv = x + ((size_t)i * nv + v) * npar;
+            bool in = true;
+            for (int j = 0; j < npar; ++j)  // the simplex's bounds rule, angles 0..360 exempt
+                in = in && ((fits[i].lo[j] == 0.0 && fits[i].hi[j] == 360.0) ||
+                            (xv[j] >= fits[i].lo[j] && xv[j] <= fits[i].hi[j]));
+            out[(size_t)i * nv + v] = kOutOfBounds;  // the objective's value outside the bounds
+            if (in) {
+                pts.push_back({i, xv});
+                where.push_back((size_t)i * nv + v);
+            }
+        }
+    if (pts.empty()) return CSX_OK;
+    PointEval ev(ctx, fits, base, npar, keys, precision, threads);
+    std::vector<double> sums(pts.size());
+    if (int rc = ev.run(pts, sums.data())) return rc;
+    for (size_t k = 0; k < pts.size(); ++k) out[where[k]] = sums[k];
     return CSX_OK;
 }
 

@@ -334,10 +334,11 @@ def surrogate_objectives(net, fits_px, cands, mu, sigma):
     return out
 
 
-def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4, n_local=32, shrink=0.1):
-    """Starting points for the exact fits: the best of n_global draws over the box (plus
-    x_init), then the best of n_local draws in a box of +-shrink of the range around each of
-    the n_keep best.  Returns (starts, surrogate objective of each start)."""
+def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4, n_local=32, shrink=0.1, top=8):
+    """Candidate starting points for the exact fits, ranked by the surrogate objective: the
+    best of n_global draws over the box (plus x_init), then n_local draws in a box of +-shrink
+    of the range around each of the n_keep best.  Returns (candidates [nfits, top, 6],
+    their surrogate objectives [nfits, top]); the exact objective picks among them."""
     nf = len(fits_px)
     cand = [np.vstack([x_init[f][None, :], draw_parameters(rng, n_global)]) for f in range(nf)]
     cand = [np.clip(C, FIT_LO, FIT_HI) for C in cand]
@@ -349,5 +350,6 @@ def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4,
         pts = best.repeat(n_local, axis=0) + rng.uniform(-1, 1, (n_keep * n_local, 6)) * span
         loc.append(np.clip(np.vstack([best, pts]), FIT_LO, FIT_HI))
     lv = surrogate_objectives(net, fits_px, loc, mu, sigma)
-    starts = np.array([loc[f][np.argmin(lv[f])] for f in range(nf)])
-    return starts, np.array([lv[f].min() for f in range(nf)])
+    order = [np.argsort(lv[f])[:top] for f in range(nf)]
+    return (np.stack([loc[f][order[f]] for f in range(nf)]),
+            np.stack([lv[f][order[f]] for f in range(nf)]))

@@ -183,6 +183,11 @@ typedef struct csx_nm_stats {
 } csx_nm_stats;
 int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
                  int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats);
+/* The batched objective at explicit points: out[i*nv + v] = fehlerquadratsumme of fit i at
+ * x[(i*nv + v)*npar ...] (1e20 outside the bounds, like the simplex), all points of all fits in
+ * one batched launch.  No reference counterpart: used to score proposed starting points. */
+int csx_nm_eval(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, int nv,
+                const double* x, int precision, int threads, double* out);
 
 int csx_last_error(csx_ctx* ctx, char* buf, int len);
 /* Kernel time of the last csx_fit_fqs / csx_fit_batch_run, and of the last csx_render when
