@@ -72,3 +72,32 @@ def test_batched_nm_lowers_the_objective():
     res, stats = nm_batch(ctx, fits, KEYS, stp=0.05, maxit=200, ftol=1e-12, precision="mixed")
     for (x0r, f0, _, _), (x, f, it, ev) in zip(start, res):
         assert np.isfinite(f) and f < 0.5 * f0, (f0, f)
+
+
+def test_nm_eval_equals_per_point_objective():
+    """csx_nm_eval at explicit points (one batched launch over fits x points) equals the
+    per-fit objective evaluated one point at a time; out-of-box points get 1e20."""
+    from crystalscatter_amd import nm_eval
+    params = manifest()["fixtures"]["cfg4_bct.npz"]["params"][:3]
+    probs = [_problem(k, p) for k, p in enumerate(params)]
+    fits = []
+    for e, p, d, w, x0, lo, hi in probs:
+        base = SasCalc(device=-1, params=p)
+        base.set("ignore_cbs", 1)
+        fits.append(dict(engine=base, d_data=d.data_ptr(), w=w, h=w, x0=x0, lo=lo, hi=hi, bstop=2))
+    rng = np.random.default_rng(3)
+    X = np.stack([np.stack([np.clip(x0 * (1 + rng.uniform(-0.1, 0.1, 6)), lo, hi) for _ in range(4)])
+                  for (_, _, _, _, x0, lo, hi) in probs])
+    X[1, 2, 0] = 100.0  # outside the box
+    got = nm_eval(Context(0), fits, KEYS, X, precision="mixed")
+    for i, (e, p, d, w, x0, lo, hi) in enumerate(probs):
+        e.setFitData(w, w, d.cpu().numpy())
+        for v in range(4):
+            if i == 1 and v == 2:
+                assert got[i, v] == 1e20
+                continue
+            for k, val in zip(KEYS, X[i, v]):
+                for name in k.split(","):
+                    e.set(name, val)
+            ref = e.doFitCalculation(bstop=2, border=0, precision="mixed")[0]
+            assert got[i, v] == ref, (i, v, got[i, v], ref)

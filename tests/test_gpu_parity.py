@@ -282,3 +282,35 @@ def test_error_paths():
         e.doCalculation("fp16")
     with pytest.raises(CsxError):
         SasCalc(device=0, params=dict(CFG1, uca=0.001)).doCalculation()
+
+
+def test_back_to_back_renders_keep_their_own_radial_tables():
+    """Renders queued without synchronisation alternate the radial tables (the pre-pass of one
+    render overlaps the previous image kernel): every image equals its own isolated render."""
+    import torch
+    CFG3_SMALL = dict(CFG3, GridPoints=64)
+    variants = [dict(CFG3_SMALL, EditRadius=r, uca=a, ucb=a, ucc=a) for r, a in ((8, 30), (6, 24), (9, 33), (7, 28))]
+    engines = [SasCalc(device=0, params=v) for v in variants]
+    n = 128 * 128
+    alone = []
+    for e in engines:
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        e.doCalculationDevice(out.data_ptr(), "mixed")
+        torch.cuda.synchronize()
+        alone.append(out.clone())
+    # one context, parameters switched between queued renders
+    e = SasCalc(device=0, params=variants[0])
+    outs = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(8)]
+    for k in range(8):
+        e.set_params(variants[k % 4])
+        e.doCalculationDevice(outs[k].data_ptr(), "mixed")
+    torch.cuda.synchronize()
+    for k in range(8):
+        assert torch.equal(outs[k], alone[k % 4]), k
+    # same parameters, no upload in between: the renders overlap pre-pass and image kernel
+    e.set_params(variants[1])
+    for k in range(8):
+        e.doCalculationDevice(outs[k].data_ptr(), "mixed")
+    torch.cuda.synchronize()
+    for k in range(8):
+        assert torch.equal(outs[k], alone[1]), k
