@@ -1483,9 +1483,10 @@ struct csx_ctx {
     std::vector<int> bntiles;
     std::vector<int2> btiles;
     PinnedArena barena;  // host image of the device arena (tables of all sets), pinned
+    PinnedArena bres;    // sums and counts of the batch in flight (size = its set count)
     struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso; };
     std::vector<Pending> bpend;
-    DevBuf barena_d, bsets_d, btiles_d, bpart_d, bntiles_d, bsums_d, bcnts_d;
+    DevBuf barena_d, bpart_d, bsums_d;
 };
 
 static int fail(csx_ctx* c, int code, const std::string& msg) {
@@ -1650,10 +1651,10 @@ void csx_destroy(csx_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
     c->barena.release();
+    c->bres.release();
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
-                      &c->fqs_tmp, &c->barena_d, &c->bsets_d, &c->btiles_d, &c->bpart_d, &c->bntiles_d,
-                      &c->bsums_d, &c->bcnts_d})
+                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d})
         b->release();
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_rad[0], c->ev_rad[1], c->ev_img[0], c->ev_img[1]})
         if (e) (void)hipEventDestroy(e);
@@ -1997,6 +1998,10 @@ int csx_stage_commit(csx_ctx* c, const csx_staged* st) { return csx_stage_commit
 extern "C" {
 int csx_fit_batch_begin(csx_ctx* c) {
     if (!c) return CSX_E_ARG;
+    if (c->bres.size) {  // a launched batch nobody waited for: let its copies finish, drop it
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->bres.size = 0;
+    }
     c->bsets.clear();
     c->bntiles.clear();
     c->btiles.clear();
@@ -2017,10 +2022,29 @@ int csx_fit_batch_stage(csx_ctx* c, const csx_scalars* s, const double* h_carr, 
 
 int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts) {
     if (!c || !h_sums) return CSX_E_ARG;
+    if (int rc = csx_fit_batch_launch(c, precision)) return rc;
+    return csx_fit_batch_wait(c, h_sums, h_cnts);
+}
+
+}  // extern "C"
+
+// Enqueue the staged batch on the context's stream without waiting: the set records, the
+// tile list and the per-set tile counts are appended to the pinned arena, which goes over in
+// one copy; then the batched kernel, the per-set reduction and the copy of the sums into
+// pinned host memory.  The arena must not be touched until csx_fit_batch_wait.
+int csx_fit_batch_launch(csx_ctx* c, int precision) {
+    if (!c) return CSX_E_ARG;
     const int nsets = (int)c->bsets.size();
     if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, c->barena_d.ensure(c->barena.size));
+    const int ntiles = (int)c->btiles.size();
+    auto al = [](size_t o) { return (o + 255) & ~size_t(255); };
+    const size_t o_sets = al(c->barena.size);
+    const size_t o_tiles = al(o_sets + nsets * sizeof(FitSet));
+    const size_t o_ntiles = al(o_tiles + ntiles * sizeof(int2));
+    const size_t total = o_ntiles + nsets * sizeof(int);
+    HIPCHK(c, c->barena_d.ensure(total));
+    HIPCHK(c, c->barena.reserve(total));
     char* base = (char*)c->barena_d.p;
     for (int i = 0; i < nsets; ++i) {
         FitSet& f = c->bsets[i];
@@ -2037,33 +2061,43 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
             f.d.iso = (const float2*)(base + pd.off_iso);
         }
     }
-    const int ntiles = (int)c->btiles.size();
-    HIPCHK(c, c->bsets_d.ensure(nsets * sizeof(FitSet)));
-    HIPCHK(c, c->btiles_d.ensure(ntiles * sizeof(int2)));
+    std::memcpy(c->barena.p + o_sets, c->bsets.data(), nsets * sizeof(FitSet));
+    std::memcpy(c->barena.p + o_tiles, c->btiles.data(), ntiles * sizeof(int2));
+    std::memcpy(c->barena.p + o_ntiles, c->bntiles.data(), nsets * sizeof(int));
+    c->barena.size = total;
     HIPCHK(c, c->bpart_d.ensure(ntiles * sizeof(TileSum)));
-    HIPCHK(c, c->bntiles_d.ensure(nsets * sizeof(int)));
-    HIPCHK(c, c->bsums_d.ensure(nsets * sizeof(double)));
-    HIPCHK(c, c->bcnts_d.ensure(nsets * sizeof(long long)));
-    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, c->barena.size, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->bsets_d.p, c->bsets.data(), nsets * sizeof(FitSet), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->btiles_d.p, c->btiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->bntiles_d.p, c->bntiles.data(), nsets * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    const int nblocks = ntiles;  // one block per tile
+    HIPCHK(c, c->bsums_d.ensure(nsets * (sizeof(double) + sizeof(long long))));
+    HIPCHK(c, c->bres.reserve(nsets * (sizeof(double) + sizeof(long long))));
+    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
+    const FitSet* d_sets = (const FitSet*)(base + o_sets);
     const int kind = kind_of(c->bsets[0].d.s.combo_key);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    DISPATCH_KIND(kind, precision, launch_fit_batch, (const FitSet*)c->bsets_d.p, (const int2*)c->btiles_d.p,
-                  ntiles, (TileSum*)c->bpart_d.p, nblocks, c->stream);
+    DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, (const int2*)(base + o_tiles), ntiles,
+                  (TileSum*)c->bpart_d.p, ntiles, c->stream);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_sets, dim3((nsets + 255) / 256), dim3(256), 0, c->stream, (const FitSet*)c->bsets_d.p,
-                       nsets, (const TileSum*)c->bpart_d.p, (const int*)c->bntiles_d.p, (double*)c->bsums_d.p,
-                       (long long*)c->bcnts_d.p);
+    double* d_sums = (double*)c->bsums_d.p;
+    long long* d_cnts = (long long*)(d_sums + nsets);
+    hipLaunchKernelGGL(k_reduce_sets, dim3((nsets + 255) / 256), dim3(256), 0, c->stream, d_sets, nsets,
+                       (const TileSum*)c->bpart_d.p, (const int*)(base + o_ntiles), d_sums, d_cnts);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(h_sums, c->bsums_d.p, nsets * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    std::vector<long long> cn(nsets);
-    HIPCHK(c, hipMemcpyAsync(cn.data(), c->bcnts_d.p, nsets * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bres.p, d_sums, nsets * (sizeof(double) + sizeof(long long)), hipMemcpyDeviceToHost,
+                             c->stream));
+    c->bres.size = (size_t)nsets;
+    return CSX_OK;
+}
+
+// Wait for the batch enqueued by csx_fit_batch_launch and return its sums and counts.
+int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts) {
+    if (!c || !h_sums) return CSX_E_ARG;
+    const int nsets = (int)c->bres.size;
+    if (nsets == 0) return fail(c, CSX_E_STATE, "no batch in flight");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = cn[i];
+    const double* s = (const double*)c->bres.p;
+    const long long* n = (const long long*)(s + nsets);
+    for (int i = 0; i < nsets; ++i) h_sums[i] = s[i];
+    if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = n[i];
+    c->bres.size = 0;
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
@@ -2071,4 +2105,11 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
     return CSX_OK;
 }
 
+// A second context on the same device (its own stream and staging memory).
+int csx_create_like(const csx_ctx* c, csx_ctx** out) {
+    if (!c || !out) return CSX_E_ARG;
+    return csx_create(c->device, out);
+}
+
+extern "C" {
 }  // extern "C"

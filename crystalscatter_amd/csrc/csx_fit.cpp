@@ -317,6 +317,12 @@ public:
     };
 
     int run(const std::vector<Point>& pts, double* sums) {
+        if (int rc = launch(pts)) return rc;
+        return finish(sums);
+    }
+
+    // Prepare and stage the points, then enqueue their evaluation (no wait).
+    int launch(const std::vector<Point>& pts) {
         using clk = std::chrono::steady_clock;
         auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
         const int n = (int)pts.size();
@@ -345,15 +351,23 @@ public:
         }
         rc = csx_stage_commit_many(ctx_, n, built.data(), threads_);
         if (rc) return rc;
-        cnts_.resize(n);
-        const auto t2 = clk::now();
-        rc = csx_fit_batch_run(ctx_, prec_, sums, cnts_.data());
+        rc = csx_fit_batch_launch(ctx_, prec_);
         if (rc) return rc;
-        const auto t3 = clk::now();
+        const auto t2 = clk::now();
         t_prep += ms(t0, t1);
         t_stage += ms(t1, t2);
-        t_run += ms(t2, t3);
+        n_ = n;
         return CSX_OK;
+    }
+
+    // Wait for the enqueued points; sums in point order.
+    int finish(double* sums) {
+        using clk = std::chrono::steady_clock;
+        const auto t0 = clk::now();
+        cnts_.resize(n_);
+        const int rc = csx_fit_batch_wait(ctx_, sums, cnts_.data());
+        t_run += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        return rc;
     }
 
     double t_prep = 0, t_stage = 0, t_run = 0;
@@ -376,6 +390,7 @@ private:
     std::vector<csx::PreparedSet> preps_;
     std::vector<std::string> errs_;
     std::vector<int64_t> cnts_;
+    int n_ = 0;
 };
 
 int check_fits(int nfits, const csx_nm_fit* fits, std::vector<const csx::ParamSet*>& base) {
@@ -399,34 +414,75 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, cons
     std::vector<NelderMead> nm(nfits);
     for (int i = 0; i < nfits; ++i) nm[i].init(npar, fits[i].x0, fits[i].lo, fits[i].hi, stp, maxit, ftol);
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
-    PointEval ev(ctx, fits, base, npar, keys, precision, threads);
-    std::vector<PointEval::Point> pts;
-    std::vector<double> sums;
+    // Two groups of fits on two contexts (two streams): while the device evaluates one group's
+    // round, the host prepares the other's.  Small batches use one group.
+    const int ngroups = nfits >= 32 ? 2 : 1;
+    csx_ctx* ctx2 = nullptr;
+    if (ngroups == 2 && csx_create_like(ctx, &ctx2) != CSX_OK) return CSX_E_HIP;
+    struct Group {
+        int f0, f1;
+        std::unique_ptr<PointEval> ev;
+        std::vector<PointEval::Point> pts;
+        std::vector<double> sums;
+        bool inflight = false, done = false;
+    };
+    std::vector<Group> gs(ngroups);
+    for (int g = 0; g < ngroups; ++g) {
+        gs[g].f0 = (int)((int64_t)nfits * g / ngroups);
+        gs[g].f1 = (int)((int64_t)nfits * (g + 1) / ngroups);
+        gs[g].ev = std::make_unique<PointEval>(g == 0 ? ctx : ctx2, fits, base, npar, keys, precision, threads);
+    }
     int64_t rounds = 0, evals = 0;
-    for (;;) {
-        pts.clear();
-        for (int i = 0; i < nfits; ++i)
-            for (int k = 0; k < nm[i].npending(); ++k) pts.push_back({i, nm[i].pending(k)});
-        if (pts.empty()) break;
-        ++rounds;
-        evals += (int64_t)pts.size();
-        sums.resize(pts.size());
-        if (int rc = ev.run(pts, sums.data())) return rc;
-        size_t r = 0;
-        for (int i = 0; i < nfits; ++i) {
-            const int np = nm[i].npending();
-            if (np == 0) continue;
-            nm[i].deliver(sums.data() + r);
-            r += np;
+    int rc = CSX_OK;
+    for (int live = ngroups; live > 0 && rc == CSX_OK;) {
+        for (auto& G : gs) {
+            if (G.done) continue;
+            if (G.inflight) {  // deliver the group's previous round
+                G.sums.resize(G.pts.size());
+                if ((rc = G.ev->finish(G.sums.data()))) break;
+                G.inflight = false;
+                size_t r = 0;
+                for (int i = G.f0; i < G.f1; ++i) {
+                    const int np = nm[i].npending();
+                    if (np == 0) continue;
+                    nm[i].deliver(G.sums.data() + r);
+                    r += np;
+                }
+            }
+            G.pts.clear();
+            for (int i = G.f0; i < G.f1; ++i)
+                for (int k = 0; k < nm[i].npending(); ++k) G.pts.push_back({i, nm[i].pending(k)});
+            if (G.pts.empty()) {
+                G.done = true;
+                --live;
+                continue;
+            }
+            ++rounds;
+            evals += (int64_t)G.pts.size();
+            if ((rc = G.ev->launch(G.pts))) break;
+            G.inflight = true;
         }
     }
+    double tp = 0, ts = 0, tr = 0;
+    for (auto& G : gs) {
+        if (G.inflight) {  // an error left a launch in flight: drain it
+            G.sums.resize(G.pts.size());
+            (void)G.ev->finish(G.sums.data());
+        }
+        tp += G.ev->t_prep;
+        ts += G.ev->t_stage;
+        tr += G.ev->t_run;
+    }
+    gs.clear();
+    if (ctx2) csx_destroy(ctx2);
+    if (rc) return rc;
     for (int i = 0; i < nfits; ++i) nm[i].result(out[i]);
     if (stats) {
         stats->rounds = rounds;
         stats->evals = evals;
-        stats->prep_ms = ev.t_prep;
-        stats->stage_ms = ev.t_stage;
-        stats->run_ms = ev.t_run;
+        stats->prep_ms = tp;
+        stats->stage_ms = ts;
+        stats->run_ms = tr;
     }
     return CSX_OK;
 }

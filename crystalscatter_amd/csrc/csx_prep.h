@@ -79,3 +79,9 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
 int csx_stage_commit(csx_ctx* c, const csx_staged* st);
 // n built sets at once, their bytes copied on up to `threads` threads.
 int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int threads);
+// csx_fit_batch_run in two halves: enqueue without waiting, then wait for the sums (the
+// batched fit driver prepares another group of fits in between).
+int csx_fit_batch_launch(csx_ctx* c, int precision);
+int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts);
+// A second context on the same device.
+int csx_create_like(const csx_ctx* c, csx_ctx** out);

@@ -175,7 +175,7 @@ int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const
 /* nfits fits over the npar parameters named in keys (a key "uca,ucb" sets both names to the
  * parameter's value); threads <= 0: all host cores. */
 typedef struct csx_nm_stats {
-    int64_t rounds;           /* lockstep rounds (one batched launch each) */
+    int64_t rounds;           /* batched launches (from 32 fits on, two groups of fits alternate) */
     int64_t evals;            /* objective evaluations over all fits */
     double prep_ms;           /* host preparation of the parameter sets and their peak tables (threads) */
     double stage_ms;          /* appending the built sets to the batch (serial) */
