@@ -1239,27 +1239,53 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
     };
     double asum = 0.0;
     bool negative = false;
-    // (h, k, l) -> reflections with those indices, in list order
+    // (h, k, l) -> reflections with those indices, in list order: a dense table over the
+    // index box (chained through `next`), falling back to a hash map for a very large box
+    int hm = 0;
+    for (int ii = 1; ii <= n2; ++ii)
+        for (int c = 1; c <= 3; ++c) hm = std::max(hm, std::abs(L2[ii * 6 + c]));
+    const int side = 2 * hm + 1;
+    const bool dense = (int64_t)side * side * side <= (1 << 22);
     auto key = [](int h, int k, int l) { return ((int64_t)(h + 1024) << 22) | ((int64_t)(k + 1024) << 11) | (l + 1024); };
+    auto cell = [&](int h, int k, int l) { return ((h + hm) * side + (k + hm)) * side + (l + hm); };
+    static thread_local std::vector<int> head;
+    std::vector<int> next(n2 + 1, 0), tail;
     std::unordered_map<int64_t, std::vector<int>> byhkl;
-    byhkl.reserve(2 * n2 + 1);
-    for (int ii = 1; ii <= n2; ++ii) byhkl[key(L2[ii * 6 + 1], L2[ii * 6 + 2], L2[ii * 6 + 3])].push_back(ii);
+    if (dense) {
+        head.assign((size_t)side * side * side, 0);
+        tail.assign((size_t)side * side * side, 0);
+        for (int ii = 1; ii <= n2; ++ii) {
+            const int c = cell(L2[ii * 6 + 1], L2[ii * 6 + 2], L2[ii * 6 + 3]);
+            if (!head[c]) head[c] = ii; else next[tail[c]] = ii;
+            tail[c] = ii;
+        }
+    } else {
+        byhkl.reserve(2 * n2 + 1);
+        for (int ii = 1; ii <= n2; ++ii) byhkl[key(L2[ii * 6 + 1], L2[ii * 6 + 2], L2[ii * 6 + 3])].push_back(ii);
+    }
     for (int ii = 1; ii <= n2; ++ii) {
         if (used[ii] || !valid(ii)) continue;
         used[ii] = 1;
         const int h = L2[ii * 6 + 1], k = L2[ii * 6 + 2], l = L2[ii * 6 + 3];
         int jj = 0;
-        const auto it = byhkl.find(key(-h, -k, -l));
-        if (it != byhkl.end())
-            for (int c : it->second) {
-                if (jj) break;
-                if (!used[c] && valid(c) && L3[c * 17 + 5] == L3[ii * 17 + 5] &&
-                    L3[c * 17 + 10] == L3[ii * 17 + 10] && L3[c * 17 + 11] == L3[ii * 17 + 11] &&
-                    L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5] &&
-                    L3[c * 17 + 2] == -L3[ii * 17 + 2] && L3[c * 17 + 3] == -L3[ii * 17 + 3] &&
-                    L3[c * 17 + 4] == -L3[ii * 17 + 4])
-                    jj = c;
-            }
+        auto partner = [&](int c) {
+            return !used[c] && valid(c) && L3[c * 17 + 5] == L3[ii * 17 + 5] &&
+                   L3[c * 17 + 10] == L3[ii * 17 + 10] && L3[c * 17 + 11] == L3[ii * 17 + 11] &&
+                   L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5] &&
+                   L3[c * 17 + 2] == -L3[ii * 17 + 2] && L3[c * 17 + 3] == -L3[ii * 17 + 3] &&
+                   L3[c * 17 + 4] == -L3[ii * 17 + 4];
+        };
+        if (dense) {
+            for (int c = head[cell(-h, -k, -l)]; c && !jj; c = next[c])
+                if (partner(c)) jj = c;
+        } else {
+            const auto it = byhkl.find(key(-h, -k, -l));
+            if (it != byhkl.end())
+                for (int c : it->second) {
+                    if (jj) break;
+                    if (partner(c)) jj = c;
+                }
+        }
         if (!jj) { ft.singles.push_back(ii); continue; }
         used[jj] = 1;
         const double qh = L3[ii * 17 + 5];
@@ -1303,12 +1329,19 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
     ft.lut.assign((size_t)(kBands + 1) * (kLutBins + 1), 0);
     const PairRec inert{0.0f, 0.0f, 0.0f, 1.0e30f, 0.0f, 0.0f, 0, 0};
     const PairRec64 inert64{0.0, 0.0, 0.0, 0.0, 1.0e30f, 0, 0, 0.0f};
-    // Segments 0..kBands-1: one |u_z| band each; segment kBands: every pair.
+    // Segments 0..kBands-1: one |u_z| band each; segment kBands: every pair.  One stable sort
+    // by |q_hkl|; each segment takes its pairs in that order.
+    std::vector<int> order(found.size());
+    for (size_t k = 0; k < found.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return found[x].qh < found[y].qh; });
+    ft.pairs.reserve(2 * found.size() + 4 * (kBands + 1));
+    if (precise) ft.pairs64.reserve(2 * found.size() + 4 * (kBands + 1));
+    std::vector<int> band;
+    band.reserve(found.size());
     for (int b = 0; b <= kBands; ++b) {
-        std::vector<int> band;
-        for (size_t k = 0; k < found.size(); ++k)
-            if (b == kBands || band_of[k] == b) band.push_back((int)k);
-        std::stable_sort(band.begin(), band.end(), [&](int x, int y) { return found[x].qh < found[y].qh; });
+        band.clear();
+        for (int k : order)
+            if (b == kBands || band_of[k] == b) band.push_back(k);
         const int start = (int)ft.pairs.size();  // a multiple of 4
         for (int k : band) {
             ft.pairs.push_back(found[k]);

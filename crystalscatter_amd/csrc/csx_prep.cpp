@@ -193,9 +193,12 @@ void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState
     fkv[0] = 1; fk2v[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12vl[0] = 1; z12v[0] = 1.0;
     long double xln = 1.0;
     double xrn = 1.0;
+    double p4 = 1.0;  // 4^n, exact
+    const double sqrt_pi = std::sqrt(kPi);
     bool find1 = true, find4 = true;
     double *c1p = cs.arr[0], *c4p = cs.arr[3], *c1f = cs.arr[9], *c4f = cs.arr[12];
     for (int n = 1; n <= kNN; ++n) {
+        p4 *= 4.0;
         z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
         z12vl[n] = z12vl[n - 1] * ((zl + 1) - 2 + 2 * n) * ((zl + 1) - 1 + 2 * n);
         fkv[n] = fkv[n - 1] * n;
@@ -203,7 +206,7 @@ void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState
         gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
         xln = -xln * xl2z;
         xrn = -xrn * xr2z;
-        c1p[n] = std::sqrt(kPi) * std::pow(4.0, n) * z12vl[n] * xln / (2.0 * (n + 1) * gam3[n] * fkv[n]);
+        c1p[n] = sqrt_pi * p4 * z12vl[n] * xln / (2.0 * (n + 1) * gam3[n] * fkv[n]);
         c4p[n] = 4 * (n + 1 / 2.0) * fk2v[n] * z12v[n] * xrn /
                  ((n + 2) * (n + 1) * fkv[n] * fkv[n] * (n + 1) * fkv[n]);
         if (!cached) {
