@@ -73,6 +73,18 @@ class SasCalc:
     def get(self, key):
         return self._L.csx_engine_get(self._h, key.encode())
 
+    def load_ini(self, path):
+        """Set the parameters of a reference .ini file (SC_CalcCons::loadParameter,
+        Source/sc_calcCons.cpp:107-399).  Returns (fitflags, unused keys)."""
+        from .formats import read_ini
+        params, fitflags, unused = read_ini(path)
+        for k, v in params.items():
+            try:
+                self.set(k, v)
+            except KeyError:
+                unused.append(k)
+        return fitflags, unused
+
     def prepare(self):
         rc = self._L.csx_engine_prepare(self._h)
         if rc != 0:
