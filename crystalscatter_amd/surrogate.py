@@ -94,6 +94,12 @@ class MLP:
             raise ValueError("par is (nsets, 6), sets (n,), q (n, 2)")
         return n, int(par.shape[0])
 
+    def _in_use(self, *ts):
+        """The model's stream reads/writes these torch tensors: keep the caching allocator from
+        handing their memory to the caller's stream before that work is done."""
+        for t in ts:
+            t.record_stream(self._stream)
+
     def forward(self, par, sets, q, out=None):
         """y[i] = network(par[sets[i]], q[i]); device tensors; returns a float32 tensor."""
         import torch
@@ -101,6 +107,7 @@ class MLP:
         if out is None:
             out = torch.empty(n, dtype=torch.float32, device=self.device)
         self._stream.wait_stream(torch.cuda.current_stream(self.device))
+        self._in_use(par, sets, q, out)
         self._chk(self._L.csx_mlp_forward(self._h, n, ns, _p(par), _p(sets), _p(q), _p(out)), "forward")
         torch.cuda.current_stream(self.device).wait_stream(self._stream)
         return out
@@ -117,6 +124,7 @@ class MLP:
         self._target(target, n)
         loss = ctypes.c_float()
         self._stream.wait_stream(torch.cuda.current_stream(self.device))
+        self._in_use(par, sets, q, target)
         self._chk(self._L.csx_mlp_gradient(self._h, n, ns, _p(par), _p(sets), _p(q), _p(target), ctypes.byref(loss)),
                   "gradient")
         return loss.value
@@ -127,6 +135,7 @@ class MLP:
         self._target(target, n)
         loss = ctypes.c_float()
         self._stream.wait_stream(torch.cuda.current_stream(self.device))
+        self._in_use(par, sets, q, target)
         self._chk(self._L.csx_mlp_train_step(self._h, n, ns, _p(par), _p(sets), _p(q), _p(target), ctypes.c_float(lr),
                                              ctypes.byref(loss) if want_loss else None), "train_step")
         return loss.value if want_loss else None

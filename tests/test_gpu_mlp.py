@@ -106,3 +106,24 @@ def test_training_lowers_the_loss():
         net.train_step(dpar, dsets, dq, dt, 1e-3)
     last = net.gradient(dpar, dsets, dq, dt)
     assert last < 0.05 * first, (first, last)
+
+
+def test_training_is_deterministic_with_fresh_batches():
+    """Two runs of the same steps on freshly gathered batches give identical weights (the
+    batches are torch tensors recycled by the caching allocator between steps)."""
+    import torch
+    from crystalscatter_amd import surrogate as sg
+    rng = np.random.default_rng(8)
+    n = 20000
+    par = torch.from_numpy(rng.uniform(-1, 1, (64, 6)).astype(np.float32)).cuda()
+    data = sg.TrainingSet(par, torch.from_numpy(rng.integers(0, 64, n).astype(np.int32)).cuda(),
+                          torch.from_numpy(rng.uniform(-1, 1, (n, 2)).astype(np.float32)).cuda(),
+                          torch.from_numpy(rng.normal(0, 1, n).astype(np.float32)).cuda(),
+                          None, 0.0, 1.0, 64, n // 64)
+    out = []
+    for _ in range(2):
+        net = _net(max_batch=8192, seed=4, physics=True)
+        sg.train(net, data, 40, batch=8192, lr=1e-3)
+        net.synchronize()
+        out.append(net.get_params())
+    assert np.array_equal(out[0], out[1])
