@@ -214,7 +214,7 @@ def bench_surrogate(args):
     t_plain = time.perf_counter() - t0
     rng = np.random.default_rng(99)
     t0 = time.perf_counter()
-    fpx = [sg.FitPixels(p["engine"], p["data"], p["scale"], 256, rng) for p in probs]
+    fpx = [sg.FitPixels(p["engine"], p["data"], p["scale"], args.surrogate_px, rng) for p in probs]
     cands, sval = sg.propose_starts(net, fpx, data.mu, data.sigma, [centre] * len(probs), rng, top=args.rescore)
     torch.cuda.synchronize()
     t_prop = time.perf_counter() - t0
@@ -226,8 +226,12 @@ def bench_surrogate(args):
     for f, C, k in zip(fits, cands, pick):
         f["x0"] = C[k]
     t0 = time.perf_counter()
-    res_s, st_s = nm_batch(ctx, fits, FIT_KEYS, stp=0.02, maxit=args.maxit_surrogate, ftol=1e-6,
-                           precision=args.precision, threads=args.cpu_threads)
+    if args.maxit_surrogate > 0:
+        res_s, st_s = nm_batch(ctx, fits, FIT_KEYS, stp=0.02, maxit=args.maxit_surrogate, ftol=1e-6,
+                               precision=args.precision, threads=args.cpu_threads)
+    else:  # no simplex: the exactly scored best candidate is the result
+        res_s = [(C[k], float(exact[i, k]), 0, args.rescore) for i, (C, k) in enumerate(zip(cands, pick))]
+        st_s = {"rounds": 1}
     t_sur = time.perf_counter() - t0 + t_prop + t_rescore
     chi_p = np.array([r[1] for r in res_p])
     chi_s = np.array([r[1] for r in res_s])
@@ -261,7 +265,7 @@ def bench_surrogate(args):
                    % sg.EVAL_DECADES,
         "train_loss_history": hist, "training_rows": len(data), "training_rows_per_s": rows_per_s,
         "training_s": t_train, "data_generation_s": t_data, "proposal_s": t_prop, "exact_rescore_s": t_rescore,
-        "rescored_candidates_per_fit": args.rescore,
+        "rescored_candidates_per_fit": args.rescore, "surrogate_px_per_fit": args.surrogate_px,
         "start_chi2_median": float(np.median(exact.min(axis=1))),
         "roofline": {"bound": "mfma", "achieved": rows_per_s * MLP_TRAIN_FLOP / 1e12, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": rows_per_s * MLP_TRAIN_FLOP / 1e12 / PEAK_BF16_TFLOPS,
@@ -352,12 +356,14 @@ def main():
                          "bct: the BCT-cylinder kernel on random fit parameter sets")
     ap.add_argument("--nfits", type=int, default=512)
     ap.add_argument("--maxit", type=int, default=100)
-    ap.add_argument("--maxit-surrogate", type=int, default=10, help="exact iterations after a surrogate start")
+    ap.add_argument("--maxit-surrogate", type=int, default=0,
+                    help="exact simplex iterations after a surrogate start (0: the best re-scored candidate)")
     ap.add_argument("--train-patterns", type=int, default=4096)
     ap.add_argument("--train-px", type=int, default=4096, help="pixels kept per training pattern")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=65536, help="training rows per step")
     ap.add_argument("--rescore", type=int, default=8, help="surrogate candidates per fit scored exactly")
+    ap.add_argument("--surrogate-px", type=int, default=256, help="pixels per fit in the surrogate objective")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 256 MiB device to device once (a plain copy of "
                          "known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")

@@ -510,9 +510,42 @@ int csx_nm_eval(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const
             }
         }
     if (pts.empty()) return CSX_OK;
-    PointEval ev(ctx, fits, base, npar, keys, precision, threads);
+    // Chunks of points alternate between two contexts, as the rounds of csx_nm_batch do:
+    // the host prepares one chunk while the device evaluates the other, and the staging
+    // memory stays at the size of a chunk.
+    constexpr size_t kChunk = 256;
+    const size_t nchunks = (pts.size() + kChunk - 1) / kChunk;
+    csx_ctx* ctx2 = nullptr;
+    if (nchunks > 1 && csx_create_like(ctx, &ctx2) != CSX_OK) return CSX_E_HIP;
+    std::unique_ptr<PointEval> ev[2];
+    ev[0] = std::make_unique<PointEval>(ctx, fits, base, npar, keys, precision, threads);
+    if (ctx2) ev[1] = std::make_unique<PointEval>(ctx2, fits, base, npar, keys, precision, threads);
     std::vector<double> sums(pts.size());
-    if (int rc = ev.run(pts, sums.data())) return rc;
+    std::vector<PointEval::Point> part[2];
+    size_t at[2] = {0, 0};
+    bool busy[2] = {false, false};
+    int rc = CSX_OK;
+    for (size_t c = 0; c < nchunks && rc == CSX_OK; ++c) {
+        const int e = ctx2 ? (int)(c & 1) : 0;
+        if (busy[e]) {
+            rc = ev[e]->finish(sums.data() + at[e]);
+            busy[e] = false;
+            if (rc) break;
+        }
+        at[e] = c * kChunk;
+        part[e].assign(pts.begin() + at[e], pts.begin() + std::min(pts.size(), at[e] + kChunk));
+        rc = ev[e]->launch(part[e]);
+        busy[e] = rc == CSX_OK;
+    }
+    for (int e = 0; e < 2; ++e)
+        if (busy[e]) {
+            const int r2 = ev[e]->finish(sums.data() + at[e]);
+            if (rc == CSX_OK) rc = r2;
+        }
+    ev[0].reset();
+    ev[1].reset();
+    if (ctx2) csx_destroy(ctx2);
+    if (rc) return rc;
     for (size_t k = 0; k < pts.size(); ++k) out[where[k]] = sums[k];
     return CSX_OK;
 }

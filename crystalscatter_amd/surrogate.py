@@ -320,27 +320,22 @@ class FitPixels:
 
 def surrogate_objectives(net, fits_px, cands, mu, sigma):
     """Surrogate FQS (mean over each fit's pixel subsample of (log10 I_pred - log10 I_data)^2)
-    for cands[f] (k_f x 6 parameter vectors) of every fit f.  Returns a list of arrays."""
+    for cands[f] (k x 6 parameter vectors, the same k for every fit) of every fit f.  Rows are
+    built with whole-tensor operations: fit-major, then candidate, then pixel.  Returns an
+    (nfits, k) array."""
     import torch
-    par_rows, set_rows, q_rows, tgt_rows, seg = [], [], [], [], []
-    base = 0
-    for f, (fp, C) in enumerate(zip(fits_px, cands)):
-        k = C.shape[0]
-        par_rows.append(normalize(C))
-        set_rows.append(torch.arange(base, base + k, dtype=torch.int32, device="cuda").repeat_interleave(fp.n))
-        q_rows.append(fp.q.repeat(k, 1))
-        tgt_rows.append(fp.logd.repeat(k))
-        seg.append((k, fp.n))
-        base += k
-    par = torch.from_numpy(np.concatenate(par_rows).astype(np.float32)).cuda()
-    sets, q, tgt = torch.cat(set_rows), torch.cat(q_rows).contiguous(), torch.cat(tgt_rows)
-    pred = predict_log10(net, par, sets, q, mu, sigma)
-    r2 = (pred - tgt) ** 2
-    out, a = [], 0
-    for k, n in seg:
-        out.append(r2[a:a + k * n].reshape(k, n).mean(dim=1).double().cpu().numpy())
-        a += k * n
-    return out
+    nf = len(fits_px)
+    C = np.asarray(cands, dtype=np.float64)
+    k = C.shape[1]
+    npx = min(fp.n for fp in fits_px)
+    Q = torch.stack([fp.q[:npx] for fp in fits_px])                    # nf x npx x 2
+    T = torch.stack([fp.logd[:npx] for fp in fits_px])                 # nf x npx
+    par = torch.from_numpy(normalize(C.reshape(-1, 6)).astype(np.float32)).cuda()
+    sets = torch.arange(nf * k, dtype=torch.int32, device="cuda").repeat_interleave(npx)
+    q = Q[:, None].expand(nf, k, npx, 2).reshape(-1, 2).contiguous()
+    pred = predict_log10(net, par, sets, q, mu, sigma).reshape(nf, k, npx)
+    r2 = (pred - T[:, None, :]) ** 2
+    return r2.mean(dim=2).double().cpu().numpy()
 
 
 def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4, n_local=32, shrink=0.1, top=8):
@@ -349,16 +344,13 @@ def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4,
     of the range around each of the n_keep best.  Returns (candidates [nfits, top, 6],
     their surrogate objectives [nfits, top]); the exact objective picks among them."""
     nf = len(fits_px)
-    cand = [np.vstack([x_init[f][None, :], draw_parameters(rng, n_global)]) for f in range(nf)]
-    cand = [np.clip(C, FIT_LO, FIT_HI) for C in cand]
+    cand = np.stack([np.vstack([x_init[f][None, :], draw_parameters(rng, n_global)]) for f in range(nf)])
+    cand = np.clip(cand, FIT_LO, FIT_HI)
     vals = surrogate_objectives(net, fits_px, cand, mu, sigma)
-    loc = []
-    for f in range(nf):
-        best = cand[f][np.argsort(vals[f])[:n_keep]]
-        span = shrink * (FIT_HI - FIT_LO)
-        pts = best.repeat(n_local, axis=0) + rng.uniform(-1, 1, (n_keep * n_local, 6)) * span
-        loc.append(np.clip(np.vstack([best, pts]), FIT_LO, FIT_HI))
+    best = np.take_along_axis(cand, np.argsort(vals, axis=1)[:, :n_keep, None], axis=1)   # nf x keep x 6
+    span = shrink * (FIT_HI - FIT_LO)
+    pts = best.repeat(n_local, axis=1) + rng.uniform(-1, 1, (nf, n_keep * n_local, 6)) * span
+    loc = np.clip(np.concatenate([best, pts], axis=1), FIT_LO, FIT_HI)
     lv = surrogate_objectives(net, fits_px, loc, mu, sigma)
-    order = [np.argsort(lv[f])[:top] for f in range(nf)]
-    return (np.stack([loc[f][order[f]] for f in range(nf)]),
-            np.stack([lv[f][order[f]] for f in range(nf)]))
+    order = np.argsort(lv, axis=1)[:, :top]
+    return np.take_along_axis(loc, order[:, :, None], axis=1), np.take_along_axis(lv, order, axis=1)
