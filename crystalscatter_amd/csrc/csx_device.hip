@@ -172,18 +172,52 @@ __device__ __forceinline__ void need_q(const csx_scalars& s, PixQ& p) {
     }
 }
 
+// Pair records are read-only during a kernel and indexed wave-uniformly: through the constant
+// address space the compiler loads them with scalar loads (into SGPRs, no VGPRs, no vector
+// memory latency per pair) also when the table pointer itself was loaded from memory, as in
+// the batched fit kernels.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* uniform_table(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+__device__ __forceinline__ PairRec load_rec(const __attribute__((address_space(4))) PairRec* t, int i) {
+    PairRec r;
+    r.ux = t[i].ux;
+    r.uy = t[i].uy;
+    r.uz = t[i].uz;
+    r.qh = t[i].qh;
+    r.uz2 = t[i].uz2;
+    r.A = t[i].A;
+    r.ia = t[i].ia;
+    r.ib = t[i].ib;
+    return r;
+}
+__device__ __forceinline__ PairRec64 load_rec(const __attribute__((address_space(4))) PairRec64* t, int i) {
+    PairRec64 r;
+    r.ux = t[i].ux;
+    r.uy = t[i].uy;
+    r.uz = t[i].uz;
+    r.A = t[i].A;
+    r.qh = t[i].qh;
+    r.ia = t[i].ia;
+    r.ib = t[i].ib;
+    r.pad = 0.0f;
+    return r;
+}
+
 // Sum over pairs [p0, p1) (p0, p1 multiples of 4, wave-uniform) of A*(2^e1 [+ 2^e2]).
 // y^2 = sin^2(y) * acos^2(c)/(1-c^2), with sin^2 y = |q^ x u|^2 taken from the cross
 // product: accurate where the Gaussian is narrow (y small), where 1-|q^.u| is not.
 // Pairs with |q^.u| > kNearOne contribute 0 here and set `flag`.
 template <bool PLANAR, bool TWO>
-__device__ __forceinline__ void pair_sum(const PairRec* __restrict__ pr, int p0, int p1, const PixelConsts& L,
+__device__ __forceinline__ void pair_sum(const PairRec* __restrict__ prg, int p0, int p1, const PixelConsts& L,
                                          double& acc, bool& flag) {
+    const auto pr = uniform_table(prg);
     for (int p = p0; p < p1; p += 4) {
         float part = 0.0f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const PairRec P = pr[p + j];
+            const PairRec P = load_rec(pr, p + j);
             float dt = __builtin_fmaf(L.hy, P.uy, L.hx * P.ux);
             float ss;
             if constexpr (PLANAR) {
@@ -267,11 +301,12 @@ struct PixelConsts64 {
 // cosine c = q.q_hkl / (|q| |q_hkl|) is formed in FP64 and y^2 = acos(c)^2 = x*g(x), x = 1-c,
 // needs no cancellation-free rewrite in FP64.  Same window, same near-one rule.
 template <bool PLANAR, bool TWO>
-__device__ __forceinline__ void pair_sum64(const PairRec64* __restrict__ pr, int p0, int p1, const PixelConsts64& L,
+__device__ __forceinline__ void pair_sum64(const PairRec64* __restrict__ prg, int p0, int p1, const PixelConsts64& L,
                                            double& acc, bool& flag) {
     constexpr double kPi = 3.14159265358979323846;
+    const auto pr = uniform_table(prg);
     for (int p = p0; p < p1; ++p) {
-        const PairRec64 P = pr[p];
+        const PairRec64 P = load_rec(pr, p);
         double dt = fma(L.hy, P.uy, L.hx * P.ux);
         if constexpr (!PLANAR) dt = fma(L.hz, P.uz, dt);
         const double c = fabs(dt);
