@@ -33,9 +33,9 @@ class Scalars(ctypes.Structure):
         ("limq", ctypes.c_double * 18), ("norm", ctypes.c_double),
     ] + [(n, ctypes.c_double) for n in (
         "base", "i0", "iso", "ifluc", "rfluc", "dwfactor", "uca", "ucb", "domainsize",
-        "width_zuf", "reldis", "dist", "phiwidth", "ceff", "sphno_cubevol")] + [
+        "width_zuf", "reldis", "dist", "phiwidth", "ceff", "por", "sphno_cubevol")] + [
         ("xmin", ctypes.c_int32), ("xmax", ctypes.c_int32), ("ymin", ctypes.c_int32),
-        ("ymax", ctypes.c_int32), ("iimin", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("ymax", ctypes.c_int32), ("iimin", ctypes.c_int32), ("cs", ctypes.c_int32),
     ]
 
 

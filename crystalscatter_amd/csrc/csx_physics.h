@@ -156,6 +156,17 @@ __device__ inline double pq_sphere(const DevSet& d, double q) {
     return 9.0 * (pq1 - 2.0 * pq2 + pq3);
 }
 
+// Isotropic cube (Source/sc_lib_formpq_partCube.h:27-61, Source/sc_gpu_pCub_lNon_oIso.h:37-45):
+// the series below 1.2 limq4 (it is summed, with the 1e-4 exit rule, also where it no longer
+// converges, between limq4 and 1.2 limq4), the Porod law por/q^4 above; P(q) = 1 for any
+// interior other than homogeneous.
+__device__ inline double pq_cube(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    if (s.cs != 0) return 1.0;
+    if (q < 1.2 * s.limq[3]) return series_x2(carr_of(d, 3), q, 120);
+    return s.por / (q * q * q * q);
+}
+
 __device__ inline double fq_sphere(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);

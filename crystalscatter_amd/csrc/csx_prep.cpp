@@ -143,7 +143,7 @@ double delta_weight_integral(double dbeta_deg, bool order_param) {
 }
 
 // ---------------------------------------------------------------------------
-// Series coefficients (coefficients(), the branches the five kernels use)
+// Series coefficients (coefficients(), the branches the six kernels use)
 // ---------------------------------------------------------------------------
 struct CoefState {
     double* arr[18];      // carr1p..carr9p, carr1f..carr9f
@@ -173,6 +173,79 @@ void coef_sphere(PreparedSet& P, double z, double xr2z, CoefState& cs) {
         if (std::fabs(c4f[n]) < 1E-30 && n < cs.nidx[12]) cs.nidx[12] = n;
     }
     (void)P;
+}
+
+// Cube, homogeneous, isotropic: Source/sc_calc_generic_cpu.cpp:2757-2829.
+// P(q): carr4p[n] = sqrt(pi) 4^n z12v[n] (-xr2z)^n (u*u*u)[n] / (16 gam3[n]) with
+// u[n] = 1/((n+1/2)(n+1) n!) (u[0] = 2), a triple self-convolution that depends on n only.
+// <F>^2: carr4f[n] = pi^2 (-xr2z)^n T[n] / (128 gam3[n]), T a triple convolution of
+// fsum[n] gam3[n]/(n+1/2) that depends on z only (O(n^3), cached per thread for the last z).
+// Storage types follow the reference: fsum, u, v and the convolution accumulators are
+// doubles, fkv, gam3 and z12v long doubles, so the tables equal the reference's bit for bit.
+struct CubeSums {
+    double z = -1.0;
+    double tf[kNN + 1];
+    bool have_p = false;
+    double tp[kNN + 1];  // (u*u*u)[n]
+};
+
+void coef_cube(PreparedSet& P, double z, double xr2z, double radius, CoefState& cs) {
+    static thread_local CubeSums cache;
+    const double area = 6.0 * 4.0 * sq(radius);
+    const double vol = 8.0 * radius * sq(radius);
+    P.s.por = 2 * kPi * std::pow(z + 1, 4) * area / (z * (z - 1) * (z - 2) * (z - 3) * vol * vol);
+    long double fkv[kNN + 2], gam3[kNN + 2], z12v[kNN + 1];
+    fkv[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12v[0] = 1.0;
+    for (int n = 1; n <= kNN; ++n) {
+        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+        fkv[n] = fkv[n - 1] * n;
+        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+    }
+    if (!cache.have_p) {
+        double u[kNN + 1], v[kNN + 1];
+        u[0] = 2;
+        for (int n = 1; n <= kNN; ++n) u[n] = 1 / ((n + 1 / 2.0) * (n + 1) * fkv[n]);
+        for (int n = 0; n <= kNN; ++n) {
+            double acc = 0.0;
+            for (int m = 0; m <= n; ++m) acc += u[n - m] * u[m];
+            v[n] = acc;
+        }
+        for (int n = 1; n <= kNN; ++n) {
+            double acc = 0.0;
+            for (int m = 0; m <= n; ++m) acc += u[n - m] * v[m];
+            cache.tp[n] = acc;
+        }
+        cache.have_p = true;
+    }
+    if (cache.z != z) {
+        double fsum[kNN + 1];
+        fsum[0] = 1;
+        for (int n = 1; n <= kNN; ++n) {
+            long double acc = 0.0;
+            for (int m = 0; m <= n; ++m) acc += z12v[n - m] * z12v[m] / (gam3[n - m] * gam3[m] * fkv[n - m] * fkv[m]);
+            fsum[n] = static_cast<double>(acc);
+            // T[n] = sum_m w[n-m] (sum_k fsum[m-k] fsum[k] gam3[m-k] gam3[k] / ((k+1/2)(m-k+1/2)))
+            double t = 0.0;
+            for (int m = 0; m <= n; ++m) {
+                double inner = 0.0;
+                for (int k = 0; k <= m; ++k)
+                    inner += fsum[m - k] * fsum[k] * gam3[m - k] * gam3[k] / ((k + 1 / 2.0) * (m - k + 1 / 2.0));
+                t += inner * fsum[n - m] * gam3[n - m] / (n - m + 1 / 2.0);
+            }
+            cache.tf[n] = t;
+        }
+        cache.z = z;
+    }
+    double* c4p = cs.arr[3];
+    double* c4f = cs.arr[12];
+    long double xrn = 1.0;
+    for (int n = 1; n <= kNN; ++n) {
+        xrn = -xrn * xr2z;
+        c4p[n] = std::sqrt(kPi) * std::pow(4.0, n) * z12v[n] * xrn * cache.tp[n] / (16.0 * gam3[n]);
+        c4f[n] = kPi * kPi * xrn * cache.tf[n] / (128.0 * gam3[n]);
+        if (std::fabs(c4p[n]) < 1E-30 && n < cs.nidx[3]) cs.nidx[3] = n;
+        if (std::fabs(c4f[n]) < 1E-30 && n < cs.nidx[12]) cs.nidx[12] = n;
+    }
 }
 
 // The O(n^2) inner sums of the <F>^2 coefficients (carr1f, carr4f) depend only on the
@@ -632,11 +705,18 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     const double xr2z = sq(p.radius / (2.0 * (z + 1)));
     s.norm = 1;
     bool done = false;
-    if (p.interior == 0) {
+    s.cs = p.interior;
+    if (p.ordis == 7 && P.cdim == 4) {
+        // Isotropic cubes: only the homogeneous interior has coefficients; the pixel of any
+        // other interior uses P(q) = 1 (Source/sc_gpu_pCub_lNon_oIso.h:37-45).
+        P.order = 0;
+        if (p.interior == 0) coef_cube(P, z, xr2z, p.radius, cs);
+        done = true;
+    } else if (p.interior == 0) {
         if (P.cdim == 3) {
             coef_sphere(P, z, xr2z, cs);
             done = true;
-        } else if (p.ordis == 6) {
+        } else if (p.ordis == 6 && P.cdim == 1) {
             P.order = 1;
             coef_cylinder_zdir(z, zl, xl2z, xr2z, cs);
             done = true;

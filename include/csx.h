@@ -54,13 +54,14 @@ enum {
 #define CSX_KEY_CYL_NONE_GAUSS    0x01000C00u
 #define CSX_KEY_CYL_NONE_ZDIR     0x01000C06u
 #define CSX_KEY_CYL_BCT_ZDIR      0x01020806u
+#define CSX_KEY_CUBE_NONE_ISO     0x04000C07u
 
 #define CSX_NCOEF 151   /* coeffarray_len */
 #define CSX_NIM2 131    /* imax2d_len */
 #define CSX_LATLEN 2500 /* latparlen */
 
 /* Per-parameter-set scalars the device path reads (the by-value engine copy of
- * the reference, reduced to what the five kernels use).  Filled by the host
+ * the reference, reduced to what the six kernels use).  Filled by the host
  * preparation (csx_engine_* or the caller's own prepareCalculation). */
 typedef struct csx_scalars {
     uint32_t combo_key;
@@ -80,10 +81,11 @@ typedef struct csx_scalars {
     double norm;
     double base, i0, iso, ifluc, rfluc;
     double dwfactor, uca, ucb, domainsize, width_zuf, reldis, dist, phiwidth, ceff;
+    double por;                         /* Porod constant of the isotropic cube */
     double sphno_cubevol;               /* latpar[1]*latpar[2] */
     int32_t xmin, xmax, ymin, ymax;     /* image bounds _xmin.._ymax (ihex, i) */
     int32_t iimin;                      /* offset of fit coordinates (checkFitRanges) */
-    int32_t pad0;
+    int32_t cs;                         /* particle interior (0 homogeneous) */
 } csx_scalars;
 
 /* Block of pixels to evaluate: ihex in [xmin,xmax), i in [ymin,ymax); output in the

@@ -434,6 +434,53 @@ static void coef_sphere(csxo_engine* e, double z, double xr2z, int* n4, int* n4f
     }
 }
 
+/* Cube, homogeneous, isotropic (Source/sc_calc_generic_cpu.cpp:2757-2829): Porod constant,
+ * P(q) coefficients from the double self-convolution of u_n = 1/((n+1/2)(n+1) n!), and the
+ * <F>^2 coefficients from the triple convolution of the sphere-like fsum.  fsum, u and v are
+ * doubles and fkv, gam3, z12v long doubles, as the reference declares them (:2127-2130). */
+static void coef_cube(csxo_engine* e, double z, double xr2z, int* n4, int* n4f)
+{
+    long double fkv[NNMAX + 2], gam3[NNMAX + 2], z12v[NNMAX + 1];
+    double fsum[NNMAX + 1], u1[NNMAX + 1], v1[NNMAX + 1];
+    const double tiny = 1E-30;
+    e->norm = 1;
+    e->order = 0;
+    const double area = 6.0 * 4.0 * sq(e->radius);
+    const double vol = 8.0 * e->radius * sq(e->radius);
+    e->por = 2 * M_PI * pow(z + 1, 4) * area / (z * (z - 1) * (z - 2) * (z - 3) * vol * vol);
+    fkv[0] = 1; gam3[0] = sqrt(M_PI) / 2.0; z12v[0] = 1.0; fsum[0] = 1; u1[0] = 2;
+    for (int n = 1; n <= NNMAX; ++n) {
+        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+        fkv[n] = fkv[n - 1] * n;
+        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+        u1[n] = 1 / ((n + 1 / 2.0) * (n + 1) * fkv[n]);
+    }
+    for (int n = 0; n <= NNMAX; ++n) {
+        double s = 0.0;
+        for (int m = 0; m <= n; ++m) s += u1[n - m] * u1[m];
+        v1[n] = s;
+    }
+    long double xrn = 1.0;
+    for (int n = 1; n <= NNMAX; ++n) {
+        xrn = -xrn * xr2z;
+        long double sf = 0.0;
+        for (int m = 0; m <= n; ++m) sf += z12v[n - m] * z12v[m] / (gam3[n - m] * gam3[m] * fkv[n - m] * fkv[m]);
+        fsum[n] = sf;
+        double sp = 0.0;
+        for (int m = 0; m <= n; ++m) sp += u1[n - m] * v1[m];
+        e->C.c4p[n] = sqrt(M_PI) * pow(4.0, n) * z12v[n] * xrn * sp / (16.0 * gam3[n]);
+        sp = 0.0;
+        for (int m = 0; m <= n; ++m) {
+            double s1 = 0.0;
+            for (int k = 0; k <= m; ++k) s1 += fsum[m - k] * fsum[k] * gam3[m - k] * gam3[k] / ((k + 1 / 2.0) * (m - k + 1 / 2.0));
+            sp += s1 * fsum[n - m] * gam3[n - m] / (n - m + 1 / 2.0);
+        }
+        e->C.c4f[n] = M_PI * M_PI * xrn * sp / (128.0 * gam3[n]);
+        if (fabs(e->C.c4p[n]) < tiny && n < *n4) *n4 = n;
+        if (fabs(e->C.c4f[n]) < tiny && n < *n4f) *n4f = n;
+    }
+}
+
 /* Cylinder, perfect (z-dir) orientation, homogeneous
  * (Source/sc_calc_generic_cpu.cpp:4124-4212 incl. the signed first-sign-change searches). */
 static void coef_cyl_zdir(csxo_engine* e, double z, double zl, double xl2z, double xr2z, int* n1,
@@ -588,6 +635,13 @@ static int coefficients(csxo_engine* e)
     int done = 0;
     if (e->cdim == 3) {
         if (e->cs == 0) { coef_sphere(e, z, xr2z, &nidx[3], &nidx[12]); done = 1; }
+    } else if (e->ordis == 7 && e->cdim == 4) {
+        /* Only the homogeneous cube has a form factor; the pixel of any other interior uses
+         * P(q) = 1 (Source/sc_gpu_pCub_lNon_oIso.h:37-45), so its tables stay unused. */
+        if (e->cs == 0) coef_cube(e, z, xr2z, &nidx[3], &nidx[12]);
+        done = 1;
+    } else if (e->ordis == 6 && e->cdim == 4) {
+        /* perfectly oriented cubes (:2880-2945) are not restated */
     } else if (e->ordis == 6 && e->cdim != 3) {
         if (e->cs == 0) { coef_cyl_zdir(e, z, zl, xl2z, xr2z, &nidx[0], &nidx[3], &nidx[9], &nidx[12]); done = 1; }
     } else if (e->ordis == 0 && e->cdim != 3) {
@@ -978,6 +1032,24 @@ static double pq_sphere(const csxo_engine* e, double q)
     return 9.0 * (pq1 - 2.0 * pq2 + pq3);
 }
 
+/* formpq_partCube, isotropic and homogeneous (Source/sc_lib_formpq_partCube.h:27-61): the
+ * series up to 120 terms below 1.2 limq4, the Porod law por/q^4 above. */
+static double pq_cube(const csxo_engine* e, double q)
+{
+    if (q < 1.2 * e->limq[3]) {
+        double s = 1.0, old = 0.0, qq = 1.0;
+        for (int n = 1; n <= 120; ++n) {
+            qq = qq * q * q;
+            s += e->C.c4p[n] * qq;
+            const double d = fabs((s - old) / s);
+            if (d < 0.0001) break;
+            old = s;
+        }
+        return s;
+    }
+    return e->por / (q * q * q * q);
+}
+
 /* formfq_partSphere, cs=0 (Source/sc_lib_formfq_partSphere.h:8-52). */
 static double fq_sphere(const csxo_engine* e, double q)
 {
@@ -1180,6 +1252,11 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
     case 0x00000C07u: { /* Source/sc_gpu_pSph_lNon_oIso.h:15-51 */
         const double pq = pq_sphere(e, q);
         *out = e->base + e->i0 * (pq + e->iso * pq) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+        return 0;
+    }
+    case 0x04000C07u: { /* Source/sc_gpu_pCub_lNon_oIso.h:14-57 */
+        const double pq = (e->cs == 0) ? pq_cube(e, q) : 1.0;
+        *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }
     case 0x00020500u: { /* Source/sc_gpu_pSph_pGau_lFCC_oGau.h:15-171 */

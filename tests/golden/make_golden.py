@@ -7,7 +7,8 @@ reference's only golden image (Documentation/Vitess-Sphere.dat) decoded.
 Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
+    python tests/golden/make_golden.py cube                              # only the cube ones
 """
 import json
 import os
@@ -38,6 +39,13 @@ CFG3 = dict(LType=5, ComboBoxParticle=0, Ordis=0, ComboBoxPeak=1, ComboBoxInteri
 CFG4_BASE = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, ComboBoxInterior=0, theta=0,
                  phi=0, EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1, Base=0,
                  HKLmax=5)
+# Isotropic homogeneous cubes (key 0x04000C07): two edge/width combinations, the first with
+# the series/Porod switch inside the image, the second with iso and ifluc terms.
+CUBE_A = dict(LType=12, ComboBoxParticle=4, Ordis=7, ComboBoxInterior=0, EditRadius=10,
+              EditSigma=0.1, GridPoints=32, EditQmax=2, I0=1, Base=0, iso=0)
+CUBE_B = dict(LType=12, ComboBoxParticle=4, Ordis=7, ComboBoxInterior=0, EditRadius=4,
+              EditSigma=0.25, GridPoints=24, EditQmax=3, I0=2, Base=0.5, iso=0.25, ifluc=3,
+              rfluc=2)
 VITESS = dict(LType=12, ComboBoxParticle=0, Ordis=7, EditRadius=16.7, EditSigma=0.0639, EditQmax=2,
               GridPoints=128, I0=1, Base=0, CenterBeam=1, EditPixelX=2.5, EditPixelY=2.5,
               EditPixelNoX=256, EditPixelNoY=256, EditDet=10, EditWavelength=0.154)
@@ -83,9 +91,27 @@ def table_dict(r, with_lattice):
     return d
 
 
+def write_cube(manifest):
+    """Cube fixtures: images, the carr4p/carr4f tables, limq and the Porod constant."""
+    for name, p in (("cube_a.npz", CUBE_A), ("cube_b.npz", CUBE_B)):
+        r = run_ref(p)
+        np.savez_compressed(os.path.join(HERE, name), image=r["image"], bounds=r["bounds"],
+                            por=r["por"], **table_dict(r, False))
+        manifest["fixtures"][name] = {"params": p}
+
+
 def main():
     if not os.path.exists(REF_DRIVER):
         sys.exit("build the reference first: make -C oracle ref")
+    only = sys.argv[1:]  # e.g. "cube": write only those fixtures, keep the manifest's others
+    if only:
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
+        if "cube" in only:
+            write_cube(manifest)
+        with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
+        return
     manifest = {"generator": "tests/golden/make_golden.py", "driver": "oracle/ref_driver.cpp",
                 "compiler": subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
                 "libc": platform.libc_ver(), "fixtures": {}}
@@ -155,6 +181,8 @@ def main():
     np.savez_compressed(os.path.join(HERE, "cfg4_bct.npz"), **arrays)
     manifest["fixtures"]["cfg4_bct.npz"] = {"params": sets, "fit_params": p, "fits": fits,
                                             "fitdata": "uniform(0.01,10) seed 5678, 32x32"}
+
+    write_cube(manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))

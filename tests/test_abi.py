@@ -35,6 +35,21 @@ def test_python_signatures_cover_the_header():
     assert {n for n, _, _ in _lib.SIGNATURES} == _declared()
 
 
-def test_scalars_struct_layout_matches_header():
-    # the C struct has 11 int32, 22 + 18 + 1 + 15 doubles, then 6 int32 (with natural alignment)
-    assert ctypes.sizeof(_lib.Scalars) == 11 * 4 + 4 + (22 + 18 + 1 + 15) * 8 + 6 * 4
+def test_scalars_struct_layout_matches_header(tmp_path):
+    """Every field of the ctypes mirrors at the offset the C compiler gives it in include/csx.h."""
+    structs = {"csx_scalars": _lib.Scalars, "csx_grid": _lib.Grid, "csx_mask": _lib.Mask}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "csx.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got[cname + "." + f[0]]) == getattr(py, f[0]).offset, (cname, f[0])

@@ -36,6 +36,37 @@ def test_tables_match_reference(name):
         np.testing.assert_array_equal(e.table("carr11pm").reshape(131, 131)[:121, :121], f["carr11pm"])
 
 
+@pytest.mark.parametrize("name", ["cube_a.npz", "cube_b.npz"])
+def test_cube_tables_match_reference(name):
+    """Isotropic cubes: P(q) and <F>^2 coefficients, limq and the Porod constant."""
+    f = fixture(name)
+    e = _host(params(name))
+    for t in TABLES:
+        np.testing.assert_array_equal(e.table(t), f[t], err_msg=t)
+    s = e.scalars()
+    assert s.por == f["por"][0] and s.norm == f["norm"][0] and s.cs == 0
+    # the z-only sums are cached per thread: a second preparation at another size and the
+    # same width, and one after a different width, give the same tables as a fresh one
+    p = params(name)
+    for other in (dict(p, EditRadius=p["EditRadius"] * 1.5), dict(p, EditSigma=0.3), p):
+        e2 = _host(other)
+        np.testing.assert_array_equal(e2.table("carr4f"), _host_fresh_carr4f(other))
+    np.testing.assert_array_equal(e2.table("carr4f"), f["carr4f"])
+
+
+def _host_fresh_carr4f(p):
+    """carr4f computed in a fresh process (empty per-thread cache)."""
+    import json
+    import subprocess
+    import sys
+    code = ("import json,sys,numpy as np;from crystalscatter_amd import SasCalc;"
+            "e=SasCalc(device=-1,params=json.loads(sys.argv[1]));e.set('ignore_cbs',1);e.prepare();"
+            "sys.stdout.write(json.dumps(e.table('carr4f').tolist()))")
+    out = subprocess.run([sys.executable, "-c", code, json.dumps(p)], check=True, capture_output=True, text=True,
+                         cwd=__import__("os").path.dirname(__import__("os").path.dirname(__file__)))
+    return np.array(json.loads(out.stdout.replace("Infinity", "1e999").replace("NaN", "null")), dtype=float)
+
+
 def test_bct_sets_match_reference():
     f = fixture("cfg4_bct.npz")
     for k, p in enumerate(manifest()["fixtures"]["cfg4_bct.npz"]["params"]):
@@ -70,6 +101,9 @@ def test_parameter_checks_like_the_reference():
 
 
 def test_unsupported_combination_is_an_error():
-    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=4, Ordis=7))  # cube
+    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=5, Ordis=7))  # ellipsoid
+    with pytest.raises(CsxError):
+        e.prepare()
+    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=4, Ordis=6))  # oriented cube
     with pytest.raises(CsxError):
         e.prepare()

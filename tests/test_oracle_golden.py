@@ -46,13 +46,16 @@ def test_vitess_golden_image(oracle_lib):
     assert (r > 1e-13).sum() == 11499
 
 
-@pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz"])
+@pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "cube_a.npz",
+                                  "cube_b.npz"])
 def test_tables_and_image_bitexact(oracle_lib, name):
     f = fixture(name)
     e = _engine(oracle_lib, params(name))
     for t in TABLES:
         np.testing.assert_array_equal(e.table(t), f[t], err_msg=t)
     assert e.scalar("norm") == f["norm"][0]
+    if "por" in f:
+        assert e.scalar("por") == f["por"][0]
     assert e.scalar("orcase") == f["orcase"][0]
     if "latpar3" in f:
         assert (e.scalar("peakmax1"), e.scalar("peakmax2")) == tuple(f["peakmax"])

@@ -54,6 +54,74 @@ def test_sphere_monodisperse_limit_is_rayleigh(oracle_lib):
         assert abs(v / _sphere_amp(x) ** 2 - 1) < 1e-4  # series exit rule: relative change < 1e-4
 
 
+CUBE = dict(LType=12, ComboBoxParticle=4, Ordis=7, ComboBoxInterior=0, GridPoints=16, EditQmax=1,
+            I0=1, Base=0, iso=0)
+
+
+def _cube_orient_avg(x):
+    """Orientation average of [sinc(x nx) sinc(x ny) sinc(x nz)]^2: the form factor of a cube
+    of half-edge 1 at q = x, by a 96x96 Gauss-Legendre rule over one octant of directions."""
+    t, wt = np.polynomial.legendre.leggauss(96)
+    th = (t + 1) * np.pi / 4
+    w = wt * np.pi / 4
+    TH, PH = np.meshgrid(th, th, indexing="ij")
+    W = np.outer(w, w) * np.sin(TH)
+    nx, ny, nz = np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)
+    f = (np.sinc(x * nx / np.pi) * np.sinc(x * ny / np.pi) * np.sinc(x * nz / np.pi)) ** 2
+    return float((W * f).sum() / (np.pi / 2))
+
+
+def test_cube_series_is_the_schulz_average_of_the_orientation_average(oracle_lib):
+    """Below the series radius, P(q) of the isotropic cube is the number-weighted Schulz
+    average of the orientation-averaged monodisperse cube form factor."""
+    R, sig = 10.0, 0.1
+    z = (1 - sig * sig) / (sig * sig)
+    e = oracle_lib.OracleEngine(dict(CUBE, EditRadius=R, EditSigma=sig))
+    e.prepare()
+    limq4 = e.table("limq")[3]
+    for q in (0.02, 0.1, 0.25, 0.5 * limq4):
+        v = e.pixel(q - 1e-9, 0, 0)
+        assert abs(v / _schulz_avg(lambda r: _cube_orient_avg(q * r), R, z) - 1) < 2e-5, q
+
+
+def test_cube_monodisperse_series_coefficients(oracle_lib):
+    """sigma -> 0: P(q) = 1 - (qR)^2/3 + 11 (qR)^4/225 - ... (Guinier with Rg^2 = R^2 for the
+    half-edge R, and the fourth-order term of exp(-x^2/3)(1 - x^4 <n^4 sum>/90))."""
+    R = 3.0
+    e = oracle_lib.OracleEngine(dict(CUBE, EditRadius=R, EditSigma=0.0005))
+    e.prepare()
+    c = e.table("carr4p")
+    assert abs(c[1] / (-R ** 2 / 3) - 1) < 1e-5
+    assert abs(c[2] / (11 * R ** 4 / 225) - 1) < 1e-5
+    for x in (1.0, 2.5, 4.0):
+        assert x / R < e.table("limq")[3]
+        assert abs(e.pixel(x / R - 1e-9, 0, 0) / _cube_orient_avg(x) - 1) < 1e-4
+
+
+def test_cube_porod_constant_is_the_schulz_average(oracle_lib):
+    """por/q^4 is the Porod law 2 pi S/(V^2 q^4) = (3 pi/4)/(r q)^4 averaged over the Schulz
+    distribution: por = (3 pi/4) <r^-4>."""
+    for R, sig in ((10.0, 0.1), (4.0, 0.25)):
+        z = (1 - sig * sig) / (sig * sig)
+        e = oracle_lib.OracleEngine(dict(CUBE, EditRadius=R, EditSigma=sig))
+        e.prepare()
+        # full-range Schulz average (the r^-4 weight reaches below the helper's 0.3 R cut)
+        w = lambda r: np.exp(z * np.log(r / R) - (z + 1) * (r / R - 1))
+        num = integrate.quad(lambda r: w(r) * 0.75 * np.pi / r ** 4, 0, 12 * R, limit=500, epsrel=1e-12)[0]
+        den = integrate.quad(w, 0, 12 * R, limit=500, epsrel=1e-12)[0]
+        assert abs(e.scalar("por") / (num / den) - 1) < 1e-9
+        q = 3.0 * e.table("limq")[3]
+        assert abs(e.pixel(q - 1e-9, 0, 0) / (e.scalar("por") / q ** 4) - 1) < 1e-14
+
+
+def test_cube_other_interiors_have_unit_form_factor(oracle_lib):
+    e = oracle_lib.OracleEngine(dict(CUBE, EditRadius=5, EditSigma=0.1, ComboBoxInterior=1, I0=2, iso=0.5,
+                                     Base=0.25, ifluc=3, rfluc=2))
+    e.prepare()
+    for q in (0.1, 0.7):
+        assert e.pixel(q - 1e-9, 0, 0) == 0.25 + 2 * (1 + 0.5) + 3 / (1 + q * q * 4)
+
+
 def test_center_pixel_is_i0_times_one_plus_iso(oracle_lib):
     e = oracle_lib.OracleEngine(dict(CFG1, I0=7.5, iso=0.25, Base=0.5))
     e.prepare()
