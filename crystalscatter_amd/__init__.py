@@ -85,6 +85,19 @@ class SasCalc:
                 unused.append(k)
         return fitflags, unused
 
+    def set_keyvalues(self, text, center_midpoint=False):
+        """Set the parameters of a key=value list ("radius=5;cbparticle=cube;...", the
+        console's --mcpval, Source/sc_mainCons.cpp:858-893).  Returns the pairs not applied:
+        names the list does not define and console-only settings."""
+        from .formats import read_keyvalues
+        params, ignored = read_keyvalues(text, center_midpoint)
+        for k, v in params.items():
+            try:
+                self.set(k, v)
+            except KeyError:
+                ignored.append("%s=%r" % (k, v))
+        return ignored
+
     def prepare(self):
         rc = self._L.csx_engine_prepare(self._h)
         if rc != 0:

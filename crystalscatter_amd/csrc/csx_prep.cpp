@@ -63,6 +63,10 @@ const KeyDef kKeys[] = {
     {"CheckBoxWAXS", &ParamSet::waxs, nullptr}, {"P1", nullptr, &ParamSet::p1},
     {"iso", nullptr, &ParamSet::iso}, {"I0", nullptr, &ParamSet::i0}, {"Base", nullptr, &ParamSet::base},
     {"width_zuf", nullptr, &ParamSet::width_zuf}, {"fix_orcase", &ParamSet::fix_orcase, nullptr},
+    {"VAx1", nullptr, &ParamSet::vax1}, {"VAx2", nullptr, &ParamSet::vax2}, {"VAx3", nullptr, &ParamSet::vax3},
+    {"Ay1", nullptr, &ParamSet::ay1}, {"Ay2", nullptr, &ParamSet::ay2}, {"Ay3", nullptr, &ParamSet::ay3},
+    {"Az1", nullptr, &ParamSet::az1}, {"Az2", nullptr, &ParamSet::az2}, {"Az3", nullptr, &ParamSet::az3},
+    {"SigX", nullptr, &ParamSet::sigx}, {"SigY", nullptr, &ParamSet::sigy}, {"SigZ", nullptr, &ParamSet::sigz},
 };
 
 // ---------------------------------------------------------------------------

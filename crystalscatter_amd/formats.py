@@ -12,6 +12,9 @@
   (``Source/sc_calcCons.cpp:107-399``): QSettings INI groups, the calculation's keys in the
   method group (``FCC Spheres``, historically, or ``CalcData``), grid and beam inputs in
   ``[Inputs]``, then the per-lattice-type overrides.
+- key=value lists (``sas_scatter2Cons --mcpval "radius=5;cbparticle=cube;..."``,
+  ``Source/sc_mainCons.cpp:76-118, 858-912``): ';'-separated pairs whose keys are the
+  parameter names with their GUI prefixes shortened and lower-cased.
 
 The image arrays are in the engine's IDX order (x = ihex fastest), which is the order of the
 reference's ``data()``, so ``SasCalc.doCalculation()`` output goes to these writers as is.
@@ -22,7 +25,8 @@ from urllib.parse import unquote
 
 import numpy as np
 
-__all__ = ["write_dat", "read_dat", "write_csv", "read_csv", "read_nofit", "write_nofit", "read_ini"]
+__all__ = ["write_dat", "read_dat", "write_csv", "read_csv", "read_nofit", "write_nofit", "read_ini",
+           "PARAMETERS", "keyvalue_names", "read_keyvalues"]
 
 
 def _bounds(b):
@@ -204,3 +208,131 @@ def read_ini(path, known_keys=None):
     if lt in (13, 19):
         params["ComboBoxPeak"] = 7.0
     return params, fitflags, unused
+
+
+# The calculation's parameters as the console builds them from the key table
+# (Source/sc_calc_generic.cpp:36-138 via Source/sc_calcCons.cpp:24-101): name, kind
+# ('C' selection, 'N' number, 'I' integer, 'T' toggle) and, for selections, the item texts
+# in index order.  Output-only entries (EditRelDis, EditDist) are not parameters.
+_LTYPES = ("Lamellae|Hexagonally packed cylinders (P6/mm)|Square packed cylinders (P4/mm)|"
+           "Rectangular centered cylinders (cmm)|BCC (lm3m)|FCC (Fm3m)|HCP (P6/mmc)|SC (Pm3m)|"
+           "BCT (l4/mm)|Gyroid (Ia3d)|OBDD (Pn3m)|Plumbers Nightmare (Im3m)|None|CP-Layers|"
+           "2DHex (GISAXS)|2DSquare (GISAXS)|1DLam (GISAXS)|Fd3m|Orthorombic|LDQ12").split("|")
+PARAMETERS = [
+    ("LType", "C", _LTYPES),
+    ("uca", "N", None), ("ucb", "N", None), ("ucc", "N", None), ("ucalpha", "N", None),
+    ("ucbeta", "N", None), ("ucgamma", "N", None), ("EditCeff", "N", None), ("CheckBoxTwinned", "T", None),
+    ("EditCeffcyl", "N", None), ("reff", "N", None), ("acpl", "N", None), ("bcpl", "N", None),
+    ("ifluc", "N", None), ("rfluc", "N", None),
+    ("ComboBoxParticle", "C", ("sphere|cylinder|disk|vesicle|cube|ellipsoid|triaxial ellipsoid|"
+                               "super ellipsoid, barrel|superball|excluded volume chain|Kratky Porod chain").split("|")),
+    ("Ordis", "C", ("Gaussian|Exponential|Onsager|Maier-Saupe|Cut-off|Laguerre|z-dir|isotropic|"
+                    "mirrored Gaussian|mirrored Exponential|mirrored Onsager|mirrored Maier-Saupe|"
+                    "mirrored Cut-off|fiber pattern").split("|")),
+    ("ComboBoxInterior", "C", "homogeneous|core + homogeneous sh|core + inhomogeneous sh|multi-shell|myelin".split("|")),
+    ("EditRadius", "N", None), ("EditRadiusi", "N", None), ("EditSigma", "N", None), ("EditDbeta", "N", None),
+    ("Length", "N", None), ("SigmaL", "N", None), ("Alpha", "N", None), ("EditRho", "N", None),
+    ("RotAlpha", "N", None),
+    ("ComboBoxPeak", "C", ("Lorentzian|Gaussian|mod. 1 Lorentzian|mod. 2 Lorentzian|Pseudo-Voigt|"
+                           "Pearson VII|Gamma|Anisotropic Gaussian").split("|")),
+    ("EditPeakPar", "N", None), ("EditDebyeWaller", "N", None), ("EditAzi", "N", None),
+    ("EditDomainSize", "N", None), ("RadButDebyeScherrer", "T", None), ("RadioButtonPara", "T", None),
+    ("VAx1", "N", None), ("VAx2", "N", None), ("VAx3", "N", None), ("Ay1", "N", None), ("Ay2", "N", None),
+    ("Ay3", "N", None), ("Az1", "N", None), ("Az2", "N", None), ("Az3", "N", None),
+    ("SigX", "N", None), ("SigY", "N", None), ("SigZ", "N", None),
+    ("ucpsi", "N", None), ("ucn1", "N", None), ("ucn2", "N", None), ("ucn3", "N", None),
+    ("theta", "N", None), ("phi", "N", None), ("rotTheta", "N", None), ("rotPhi", "N", None),
+    ("GridPoints", "I", None), ("HKLmax", "I", None), ("EditQmax", "N", None), ("CalcQmax", "N", None),
+    ("EditQmaxData", "T", None), ("EditQmaxPreset", "T", None), ("EditQmin", "N", None),
+    ("EditQsteps", "I", None),
+    ("EditPixelNoX", "N", None), ("EditPixelNoY", "N", None), ("EditPixelX", "N", None),
+    ("EditPixelY", "N", None), ("EditDet", "N", None), ("EditWavelength", "N", None),
+    ("BeamPosX", "N", None), ("BeamPosY", "N", None), ("CenterBeam", "T", None), ("CenterMidpoint", "T", None),
+    ("EditBFactor", "N", None), ("CheckBoxWAXS", "T", None), ("P1", "N", None),
+    ("iso", "N", None), ("I0", "N", None), ("Base", "N", None),
+]
+_KIND = {k: (t, items) for k, t, items in PARAMETERS}
+
+# Keys left out of the key=value list (Source/sc_mainCons.cpp:87-97): prefix and substring tests.
+_KV_PREFIX = ("EditAxis", "Editdom", "ExpandImage", "RadioButtonQ", "P1", "EditQmaxData", "EditQmaxPreset")
+_KV_SUBSTR = ("EditQsteps", "EditQmin", "EditBFactor", "CheckBoxWAXS", "CalcQmax", "CenterBeam",
+              "CenterMidpoint", "EditRelDis", "EditDist")
+
+
+def _short_name(key):
+    """The printed name of a key (Source/sc_mainCons.cpp:104-112): GUI prefixes shortened, lower case."""
+    if key.startswith("VAx"):
+        k = key[1:]
+    elif key.startswith("Edit"):
+        k = key[4:]
+    elif key.startswith("CheckBox"):
+        k = key[8:]
+    elif key.startswith("ComboBox"):
+        k = "CB" + key[8:]
+    elif key.startswith("RadBut"):
+        k = key[6:]
+    elif key.startswith("RadioButton"):
+        k = "RB" + key[11:]
+    else:
+        k = key
+    return k.lower()
+
+
+def keyvalue_names(center_midpoint=False):
+    """{printed name: parameter key} of the key=value interface.  With the beam centre at the
+    image midpoint the reference also drops BeamPosX/BeamPosY (Source/sc_mainCons.cpp:81-86)."""
+    keys = sorted(k for k, _, _ in PARAMETERS)
+    if center_midpoint:
+        keys = [k for k in keys if k not in ("BeamPosX", "BeamPosY")]
+    out = {}
+    for k in keys:
+        if k.startswith(_KV_PREFIX) or any(s in k for s in _KV_SUBSTR):
+            continue
+        out.setdefault(_short_name(k), k)
+    return out
+
+
+def _leading_number(v):
+    """QString::toDouble of the whole text: the number, or 0 when the text is not one."""
+    try:
+        return float(v)
+    except ValueError:
+        return 0.0
+
+
+def read_keyvalues(text, center_midpoint=False):
+    """Parameters of a key=value list as engine keys (calculateFromChatbotKeyValList,
+    Source/sc_mainCons.cpp:858-893).  Returns (params, ignored).
+
+    Pairs are separated by ';' (empty pairs skipped); the name is taken as written (not
+    trimmed), the value trimmed.  A value starting with a digit, '-' or '+' is a number; one
+    starting with 'T' or 'F' a toggle (so is an item text such as "FCC (Fm3m)", which sets 0,
+    as in the reference); "NULL" is skipped; anything else is a selection's item text, whose
+    index is stored (-1 when no item has that text, as the reference does).
+    Names the list does not define, and item texts given for non-selections, are returned in
+    `ignored`."""
+    names = keyvalue_names(center_midpoint)
+    params, ignored = {}, []
+    for kv in text.split(";"):
+        if not kv:
+            continue
+        p = kv.find("=")
+        if p < 0:
+            continue
+        val = kv[p + 1:].strip()
+        key = names.get(kv[:p])
+        if key is None or not val:
+            ignored.append(kv)
+            continue
+        kind, items = _KIND[key]
+        if val[0].isdigit() or val[0] in "-+":
+            params[key] = _leading_number(val)
+        elif val[0] in "FT":
+            params[key] = 1.0 if val[0] == "T" else 0.0
+        elif val == "NULL":
+            continue
+        elif kind == "C":
+            params[key] = float(items.index(val)) if val in items else -1.0
+        else:
+            ignored.append(kv)
+    return params, ignored

@@ -5,7 +5,7 @@ import os
 
 import numpy as np
 
-from helpers import fixture, manifest
+from helpers import fixture, manifest, params
 
 from crystalscatter_amd import SasCalc
 from crystalscatter_amd import formats as fm
@@ -71,4 +71,52 @@ def test_engine_takes_the_ini():
     e = SasCalc(device=-1)
     flags, unused = e.load_ini(os.path.join(GOLD, "vitess_sphere.ini"))
     assert e.get("EditRadius") == 16.7 and e.get("GridPoints") == 128
-    assert "SigX" in unused                                              # not an engine key
+    assert "CalcQmax" in unused                                          # not an engine key
+
+
+def test_keyvalue_names_follow_the_console_prefix_rules():
+    from crystalscatter_amd.formats import keyvalue_names
+    n = keyvalue_names()
+    expect = {"radius": "EditRadius", "cbparticle": "ComboBoxParticle", "cbinterior": "ComboBoxInterior",
+              "cbpeak": "ComboBoxPeak", "rbpara": "RadioButtonPara", "debyescherrer": "RadButDebyeScherrer",
+              "ax1": "VAx1", "ay2": "Ay2", "twinned": "CheckBoxTwinned", "ltype": "LType", "qmax": "EditQmax",
+              "gridpoints": "GridPoints", "hklmax": "HKLmax", "wavelength": "EditWavelength",
+              "pixelnox": "EditPixelNoX", "i0": "I0", "beamposx": "BeamPosX", "sigmal": "SigmaL"}
+    for k, v in expect.items():
+        assert n[k] == v, k
+    # left out by the console (Source/sc_mainCons.cpp:87-97)
+    for k in ("qmin", "qsteps", "bfactor", "waxs", "p1", "centerbeam", "centermidpoint", "calcqmax",
+              "qmaxdata", "qmaxpreset", "reldis", "dist"):
+        assert k not in n, k
+    assert len(set(n.values())) == len(n)
+    assert "beamposx" not in keyvalue_names(center_midpoint=True)
+
+
+def test_keyvalue_value_rules():
+    from crystalscatter_amd.formats import read_keyvalues
+    p, ign = read_keyvalues("radius=5;;sigma=.1;twinned=True;debyescherrer=F;cbparticle=cube;ordis=isotropic;"
+                            "cbpeak=gaussian;ltype=BCT (l4/mm);uca=+3e1;beamposx=NULL;nosuch=1; i0=2;length=abc")
+    assert p["EditRadius"] == 5 and "EditSigma" not in p       # ".1" is not read as a number
+    assert p["CheckBoxTwinned"] == 1 and p["RadButDebyeScherrer"] == 0
+    assert p["ComboBoxParticle"] == 4 and p["Ordis"] == 7 and p["LType"] == 8
+    assert p["ComboBoxPeak"] == -1                                # item texts are case-sensitive
+    # an item text starting with 'F' or 'T' is taken as a toggle first (the console tests the
+    # first character before the selection lookup): "FCC (Fm3m)" sets LType 0
+    assert read_keyvalues("ltype=FCC (Fm3m)")[0] == {"LType": 0.0}
+    assert p["uca"] == 30 and "BeamPosX" not in p and "I0" not in p
+    assert set(ign) == {"sigma=.1", "nosuch=1", " i0=2", "length=abc"}
+
+
+def test_keyvalue_list_drives_the_engine():
+    from crystalscatter_amd import SasCalc
+    kv = "ltype=None;cbparticle=cube;ordis=isotropic;cbinterior=homogeneous;radius=10;sigma=0.1;gridpoints=32;qmax=2;i0=1;base=0;iso=0"
+    a = SasCalc(device=-1)
+    assert a.set_keyvalues(kv) == []
+    a.set("ignore_cbs", 1)
+    a.prepare()
+    b = SasCalc(device=-1, params=params("cube_a.npz"))
+    b.set("ignore_cbs", 1)
+    b.prepare()
+    assert a.scalars().combo_key == 0x04000C07
+    assert bytes(a.scalars()) == bytes(b.scalars())
+    np.testing.assert_array_equal(a.table("carr4p"), fixture("cube_a.npz")["carr4p"])
