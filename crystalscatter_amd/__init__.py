@@ -18,7 +18,7 @@ from ._lib import (CSX_PREC_MIXED, CSX_PREC_REF64, CsxError, Grid, Mask, NCOEF, 
                    lib)
 
 __all__ = ["SasCalc", "Context", "CsxError", "CSX_PREC_MIXED", "CSX_PREC_REF64", "PRECISIONS", "simplex",
-           "nm_batch", "nm_eval"]
+           "nm_batch", "nm_eval", "Random", "render_batch", "tpv_images"]
 
 PRECISIONS = {"ref64": CSX_PREC_REF64, "mixed": CSX_PREC_MIXED, CSX_PREC_REF64: CSX_PREC_REF64,
               CSX_PREC_MIXED: CSX_PREC_MIXED}
@@ -166,6 +166,58 @@ class SasCalc:
     def last_kernel_ms(self):
         return self._L.csx_last_kernel_ms(self._L.csx_engine_ctx(self._h))
 
+    def tpvPerformRandom(self, ids, rng=None):
+        """Random variation of the listed variables (tpvPerformRandom,
+        Source/sc_calc_generic_cpu.cpp:1028-1143): ids is a list of "key=factor" strings; an
+        empty list clears.  Returns the reference's report lines.  The varied values apply to
+        the next calculation; get() still returns the parameters as set."""
+        text = "\n".join(ids)
+        if text and rng is None:
+            raise ValueError("a Random generator is needed")
+        rep = ctypes.create_string_buffer(4096)
+        rc = self._L.csx_engine_tpv(self._h, text.encode(), rng._h if rng is not None else None, rep, 4096)
+        if rc != 0:
+            raise CsxError("tpvPerformRandom: " + self._err())
+        return rep.value.decode()
+
+    def tpv_value(self, key):
+        """A parameter as the last tpvPerformRandom varied it."""
+        return self._L.csx_engine_tpv_value(self._h, key.encode())
+
+    def stage_image(self, ctx, d_out_ptr, cap):
+        """Stage this engine's image into ctx's pending batch (see render_batch); d_out_ptr
+        has room for cap doubles."""
+        if self._L.csx_engine_stage_image(self._h, ctx._h, ctypes.c_void_p(d_out_ptr), int(cap)) != 0:
+            raise CsxError("stage_image: " + self._err())
+
+
+class Random:
+    """The C library's rand() sequence after srand(seed), in a private state (csx_rand)."""
+
+    def __init__(self, seed=1):
+        self._L = lib()
+        self._h = self._L.csx_rand_new(int(seed) & 0xFFFFFFFF)
+        if not self._h:
+            raise CsxError("csx_rand_new failed")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.csx_rand_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def next(self):
+        return int(self._L.csx_rand_next(self._h))
+
+    def zuf(self, n):
+        """n values of the reference's zuf() = (rand()/RAND_MAX*200 - 100)/100."""
+        out = np.empty(int(n), dtype=np.float64)
+        if self._L.csx_rand_zuf(self._h, _dp(out), out.size) != 0:
+            raise CsxError("csx_rand_zuf failed")
+        return out
+
 
 class Context:
     """Device context of the C ABI (csx_create): upload / render / batched objective."""
@@ -227,6 +279,58 @@ class Context:
 
     def synchronize(self):
         self._chk(self._L.csx_synchronize(self._h), "csx_synchronize")
+
+
+def _check_out(out, n):
+    if out.dtype != torch_float64() or not out.is_cuda or out.dim() != 2 or out.shape[0] < n or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 CUDA tensor of shape (>= %d, npx)" % n)
+
+
+def torch_float64():
+    import torch
+    return torch.float64
+
+
+def render_batch(ctx, engines, precision="mixed", out=None):
+    """Images of many engines (same grid size and combination) in one launch
+    (csx_render_batch_run).  Returns a float64 CUDA tensor (len(engines), npx), IDX order."""
+    import torch
+    if not engines:
+        raise ValueError("no engines")
+    engines[0].prepare()
+    x0, x1, y0, y1 = engines[0].bounds()
+    npx = (x1 - x0) * (y1 - y0)
+    if out is None:
+        out = torch.empty((len(engines), npx), dtype=torch.float64, device="cuda")
+    _check_out(out, len(engines))
+    L = ctx._L
+    ctx._chk(L.csx_fit_batch_begin(ctx._h), "csx_fit_batch_begin")
+    for k, e in enumerate(engines):
+        e.stage_image(ctx, out[k].data_ptr(), out.shape[1])
+    ctx._chk(L.csx_render_batch_run(ctx._h, _prec(precision)), "csx_render_batch_run")
+    return out
+
+
+def tpv_images(ctx, engine, ids, n, rng, precision="mixed", out=None):
+    """n images of engine's parameters, each after one tpvPerformRandom(ids, rng) call (the
+    AI-data loop's random variation, Source/sc_mainCons.cpp:1236-1246), in one launch.
+    Returns (images as a (n, npx) CUDA tensor, the report of every call)."""
+    import torch
+    engine.tpvPerformRandom([], None)
+    engine.prepare()
+    x0, x1, y0, y1 = engine.bounds()
+    npx = (x1 - x0) * (y1 - y0)
+    if out is None:
+        out = torch.empty((n, npx), dtype=torch.float64, device="cuda")
+    _check_out(out, n)
+    L = ctx._L
+    ctx._chk(L.csx_fit_batch_begin(ctx._h), "csx_fit_batch_begin")
+    reports = []
+    for k in range(n):
+        reports.append(engine.tpvPerformRandom(ids, rng))
+        engine.stage_image(ctx, out[k].data_ptr(), out.shape[1])
+    ctx._chk(L.csx_render_batch_run(ctx._h, _prec(precision)), "csx_render_batch_run")
+    return out, reports
 
 
 def simplex(f, x0, lo, hi, stp=0.1, maxit=200, ftol=1e-6):

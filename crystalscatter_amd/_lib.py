@@ -91,6 +91,7 @@ SIGNATURES = [
     ("csx_fit_batch_stage", _I, [_VP, ctypes.POINTER(Scalars), _PD, _PI32, _PI32, _PF, _I, _I,
                                  ctypes.POINTER(Mask), _VP]),
     ("csx_fit_batch_run", _I, [_VP, _I, _PD, _PI64]),
+    ("csx_render_batch_run", _I, [_VP, _I]),
     ("csx_nm_run_cb", _I, [_I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
     ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
                           ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),
@@ -112,6 +113,13 @@ SIGNATURES = [
     ("csx_engine_scalars", ctypes.POINTER(Scalars), [_VP]),
     ("csx_engine_ctx", _VP, [_VP]),
     ("csx_engine_last_error", _I, [_VP, _CP, _I]),
+    ("csx_engine_stage_image", _I, [_VP, _VP, _VP, ctypes.c_int64]),
+    ("csx_rand_new", _VP, [ctypes.c_uint32]),
+    ("csx_rand_free", None, [_VP]),
+    ("csx_rand_next", ctypes.c_int32, [_VP]),
+    ("csx_rand_zuf", _I, [_VP, _PD, ctypes.c_int64]),
+    ("csx_engine_tpv", _I, [_VP, _CP, _VP, _CP, _I]),
+    ("csx_engine_tpv_value", _D, [_VP, _CP]),
     ("csx_mlp_nparams", ctypes.c_int64, []),
     ("csx_mlp_create", _I, [_I, _I, ctypes.c_uint64, ctypes.POINTER(_VP)]),
     ("csx_mlp_destroy", None, [_VP]),

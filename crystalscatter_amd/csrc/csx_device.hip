@@ -1186,6 +1186,41 @@ k_fit_batch_lat(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
     fit_batch_tiles<KIND, PREC>(sets, tiles, ntiles, out);
 }
 
+// Batched images: one launch over the flat tile list of many parameter sets, each set's
+// image (its whole grid, IDX order) written to the buffer staged as its data pointer.  The
+// tile goes through LDS so that every wave stores whole 128-byte rows.
+template <int KIND, int PREC>
+__device__ __forceinline__ void image_batch_tiles(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
+                                                  int ntiles) {
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int2 st = tiles[t];
+    const FitSet& f = sets[st.x];
+    const int tx = st.y % f.tiles_x, ty = st.y / f.tiles_x;
+    int px, py;
+    tile_pixel(px, py);
+    const csx_scalars& s = f.d.s;
+    const int x = min(tx * kTile + px, f.w - 1), y = min(ty * kTile + py, f.h - 1);
+    PixQ P3{s.xmin + x, s.ymin + y, 0.0, 0.0, 0.0, false};
+    const double v = pixel_value<KIND, PREC>(f.d, P3);
+    __shared__ double tile[kTile][kTile + 1];
+    tile[py][px] = v;
+    __syncthreads();
+    const int r = threadIdx.x / kTile, cx = threadIdx.x % kTile;
+    const int ox = tx * kTile + cx, oy = ty * kTile + r;
+    if (ox < f.w && oy < f.h) const_cast<double*>(f.data)[(size_t)ox + (size_t)f.w * oy] = tile[r][cx];
+}
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) k_image_batch(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
+                                                        int ntiles) {
+    image_batch_tiles<KIND, PREC>(sets, tiles, ntiles);
+}
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_image_batch_lat(const FitSet* __restrict__ sets, const int2* __restrict__ tiles, int ntiles) {
+    image_batch_tiles<KIND, PREC>(sets, tiles, ntiles);
+}
+
 // Per-set sum of the tile partials (deterministic order).
 __global__ void k_reduce_sets(const FitSet* __restrict__ sets, int nsets, const TileSum* __restrict__ part,
                               const int* __restrict__ ntiles_of, double* __restrict__ sums, long long* __restrict__ cnts) {
@@ -1619,6 +1654,14 @@ static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, 
         hipLaunchKernelGGL((k_fit_batch_lat<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
     else
         hipLaunchKernelGGL((k_fit_batch<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
+}
+
+template <int K, int P>
+static void launch_image_batch(const FitSet* sets, const int2* tiles, int ntiles, hipStream_t st) {
+    if constexpr (K == K_FCC || K == K_BCT)
+        hipLaunchKernelGGL((k_image_batch_lat<K, P>), dim3(ntiles), dim3(kBlock), 0, st, sets, tiles, ntiles);
+    else
+        hipLaunchKernelGGL((k_image_batch<K, P>), dim3(ntiles), dim3(kBlock), 0, st, sets, tiles, ntiles);
 }
 
 template <class T>
@@ -2107,11 +2150,11 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
 // tile list and the per-set tile counts are appended to the pinned arena, which goes over in
 // one copy; then the batched kernel, the per-set reduction and the copy of the sums into
 // pinned host memory.  The arena must not be touched until csx_fit_batch_wait.
-int csx_fit_batch_launch(csx_ctx* c, int precision) {
-    if (!c) return CSX_E_ARG;
+// The staged batch's set records, tile list and per-set tile counts appended to the pinned
+// arena, with the device addresses of every set's tables filled in; one H2D copy on the
+// context's stream.  Returns the device addresses of the records and the tile list.
+static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles, const int** d_ntiles) {
     const int nsets = (int)c->bsets.size();
-    if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
-    HIPCHK(c, hipSetDevice(c->device));
     const int ntiles = (int)c->btiles.size();
     auto al = [](size_t o) { return (o + 255) & ~size_t(255); };
     const size_t o_sets = al(c->barena.size);
@@ -2140,20 +2183,35 @@ int csx_fit_batch_launch(csx_ctx* c, int precision) {
     std::memcpy(c->barena.p + o_tiles, c->btiles.data(), ntiles * sizeof(int2));
     std::memcpy(c->barena.p + o_ntiles, c->bntiles.data(), nsets * sizeof(int));
     c->barena.size = total;
+    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
+    *d_sets = (const FitSet*)(base + o_sets);
+    *d_tiles = (const int2*)(base + o_tiles);
+    *d_ntiles = (const int*)(base + o_ntiles);
+    return CSX_OK;
+}
+
+int csx_fit_batch_launch(csx_ctx* c, int precision) {
+    if (!c) return CSX_E_ARG;
+    const int nsets = (int)c->bsets.size();
+    if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int ntiles = (int)c->btiles.size();
+    const FitSet* d_sets;
+    const int2* d_tiles;
+    const int* d_ntiles;
+    if (int rc = batch_upload(c, &d_sets, &d_tiles, &d_ntiles)) return rc;
     HIPCHK(c, c->bpart_d.ensure(ntiles * sizeof(TileSum)));
     HIPCHK(c, c->bsums_d.ensure(nsets * (sizeof(double) + sizeof(long long))));
     HIPCHK(c, c->bres.reserve(nsets * (sizeof(double) + sizeof(long long))));
-    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
-    const FitSet* d_sets = (const FitSet*)(base + o_sets);
     const int kind = kind_of(c->bsets[0].d.s.combo_key);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, (const int2*)(base + o_tiles), ntiles,
-                  (TileSum*)c->bpart_d.p, ntiles, c->stream);
+    DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, d_tiles, ntiles, (TileSum*)c->bpart_d.p, ntiles,
+                  c->stream);
     HIPCHK(c, hipGetLastError());
     double* d_sums = (double*)c->bsums_d.p;
     long long* d_cnts = (long long*)(d_sums + nsets);
     hipLaunchKernelGGL(k_reduce_sets, dim3((nsets + 255) / 256), dim3(256), 0, c->stream, d_sets, nsets,
-                       (const TileSum*)c->bpart_d.p, (const int*)(base + o_ntiles), d_sums, d_cnts);
+                       (const TileSum*)c->bpart_d.p, d_ntiles, d_sums, d_cnts);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->bres.p, d_sums, nsets * (sizeof(double) + sizeof(long long)), hipMemcpyDeviceToHost,
@@ -2173,6 +2231,32 @@ int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts) {
     for (int i = 0; i < nsets; ++i) h_sums[i] = s[i];
     if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = n[i];
     c->bres.size = 0;
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->timing_pending = false;
+    return CSX_OK;
+}
+
+extern "C" int csx_render_batch_run(csx_ctx* c, int precision) {
+    if (!c) return CSX_E_ARG;
+    const int nsets = (int)c->bsets.size();
+    if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (const FitSet& f : c->bsets) {
+        if (f.h != f.d.s.ymax - f.d.s.ymin) return fail(c, CSX_E_ARG, "a batched image must cover its whole grid");
+    }
+    const int ntiles = (int)c->btiles.size();
+    const FitSet* d_sets;
+    const int2* d_tiles;
+    const int* d_ntiles;
+    if (int rc = batch_upload(c, &d_sets, &d_tiles, &d_ntiles)) return rc;
+    const int kind = kind_of(c->bsets[0].d.s.combo_key);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    DISPATCH_KIND(kind, precision, launch_image_batch, d_sets, d_tiles, ntiles, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;

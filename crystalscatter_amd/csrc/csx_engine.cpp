@@ -5,7 +5,9 @@
 // doCalculation, setFitData / setNoFitRect / doFitCalculation, prepared tables.
 // Host preparation runs here in C++; every pixel runs on the GPU.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -14,8 +16,54 @@
 #include "csx.h"
 #include "csx_prep.h"
 
+// The C library's rand() sequence (glibc: the additive feedback generator of random(), 31
+// words of state) with the state held here, so that a seeded generator reproduces what
+// srand(seed) and rand() give a reference run without touching the process-wide state.
+struct csx_rand {
+    random_data rd;
+    char state[128];
+};
+
+namespace {
+
+// zuf() of the reference (Source/sc_calc_generic_cpu.cpp:1085, Source/sc_mainCons.cpp:850-853):
+// uniform on [-1, 1] from one rand() value.
+double zuf(csx_rand* r) {
+    int32_t v = 0;
+    random_r(&r->rd, &v);
+    return ((static_cast<double>(v) / RAND_MAX * 200.0) - 100.0) / 100.0;
+}
+
+// The variables tpvPerformRandom can vary (Source/sc_calc_generic_cpu.cpp:1057-1081, in its
+// order; the table lists EditDbeta twice and the map keeps the first).  norm: old * (1 + f zuf);
+// uca: the current uca; phi: 2 + 43 (zuf + 1) degrees; azi: phiwidth is varied, but
+// prepareCalculation recomputes it from the unchanged azimuthal width (cpu:413), so only the
+// random number is used up; dw: the Debye-Waller factor d^2/3 times (1 + f zuf).
+enum TpvFlag { TPV_NORM, TPV_UCA, TPV_PHI, TPV_AZI, TPV_DW };
+struct TpvVar {
+    const char* key;
+    double csx::ParamSet::*field;
+    TpvFlag flag;
+};
+const TpvVar kTpv[] = {
+    {"I0", &csx::ParamSet::i0, TPV_NORM},           {"EditRadius", &csx::ParamSet::radius, TPV_NORM},
+    {"EditSigma", &csx::ParamSet::sigma, TPV_NORM},  {"Length", &csx::ParamSet::length, TPV_NORM},
+    {"SigmaL", &csx::ParamSet::sigmal, TPV_NORM},    {"uca", &csx::ParamSet::uca, TPV_NORM},
+    {"ucb", &csx::ParamSet::ucb, TPV_UCA},           {"ucc", &csx::ParamSet::ucc, TPV_UCA},
+    {"ucpsi", &csx::ParamSet::ucpsi, TPV_NORM},      {"EditDbeta", &csx::ParamSet::dbeta, TPV_NORM},
+    {"EditDomainSize", &csx::ParamSet::width_zuf, TPV_NORM}, {"EditAzi", &csx::ParamSet::azi, TPV_AZI},
+    {"EditDebyeWaller", &csx::ParamSet::displacement, TPV_DW}, {"phi", &csx::ParamSet::phi, TPV_PHI},
+    {"Base", &csx::ParamSet::base, TPV_NORM},        {"EditRho", &csx::ParamSet::rho, TPV_NORM},
+};
+constexpr int kNTpv = sizeof(kTpv) / sizeof(kTpv[0]);
+
+}  // namespace
+
 struct csx_engine {
-    csx::ParamSet p;
+    csx::ParamSet p;        // parameters as set (the reference's parameter store)
+    csx::ParamSet pt;       // the values a TPV call derived from them (used while tpv_on)
+    bool tpv_on = false;
+    double tpv_old[kNTpv];  // values at the first TPV call after a clear
     csx::PreparedSet P;
     csx_ctx* ctx = nullptr;
     int device = 0;
@@ -58,6 +106,7 @@ int csx_engine_set(csx_engine* e, const char* key, double v) {
     if (!e || !key) return CSX_E_ARG;
     if (std::strcmp(key, "ignore_cbs") == 0) { e->ignore_cbs = v != 0; e->prepared = false; return CSX_OK; }
     if (!e->p.set(key, v)) { e->err = std::string("unknown key ") + key; return CSX_E_ARG; }
+    if (e->tpv_on) e->pt.set(key, v);  // a setter overwrites the randomized value until the next call
     e->prepared = false;
     return CSX_OK;
 }
@@ -70,10 +119,10 @@ double csx_engine_get(csx_engine* e, const char* key) {
 
 static int engine_prepare(csx_engine* e, bool check) {
     if (check) {
-        const std::string m = csx::check_parameters(e->p, e->ignore_cbs);
+        const std::string m = csx::check_parameters(e->tpv_on ? e->pt : e->p, e->ignore_cbs);
         if (!m.empty()) { e->err = m; return CSX_E_PARAM; }
     }
-    const std::string m = csx::prepare(e->p, e->P);
+    const std::string m = csx::prepare(e->tpv_on ? e->pt : e->p, e->P);
     if (!m.empty()) { e->err = m; return CSX_E_ARG; }
     const csx::PreparedSet& P = e->P;
     if (!e->ctx) { e->prepared = true; return CSX_OK; }  // host-only engine
@@ -90,6 +139,16 @@ static int engine_prepare(csx_engine* e, bool check) {
 }
 
 int csx_engine_prepare(csx_engine* e) { return e ? engine_prepare(e, true) : CSX_E_ARG; }
+
+// Checks and host preparation only (the tables go to a batch, not to the engine's context).
+static int engine_prepare_host(csx_engine* e) {
+    csx_ctx* keep = e->ctx;
+    e->ctx = nullptr;
+    const int rc = engine_prepare(e, true);
+    e->ctx = keep;
+    e->prepared = false;  // the engine's own context has not received these tables
+    return rc;
+}
 
 int csx_engine_calc_device(csx_engine* e, int precision, double* d_out, int32_t* b4) {
     if (!e || !d_out) return CSX_E_ARG;
@@ -190,6 +249,126 @@ const void* csx_engine_table(csx_engine* e, const char* name, int64_t* n, int* k
     if (std::strcmp(name, "latpar3") == 0) { *n = (int64_t)(e->P.s.peakmax2 + 1) * 17; *kind = 2; return e->P.latpar3.data(); }
     *n = 0;
     return nullptr;
+}
+
+csx_rand* csx_rand_new(uint32_t seed) {
+    csx_rand* r = new csx_rand();
+    std::memset(&r->rd, 0, sizeof r->rd);
+    if (initstate_r(seed, r->state, sizeof r->state, &r->rd) != 0) { delete r; return nullptr; }
+    return r;
+}
+
+void csx_rand_free(csx_rand* r) { delete r; }
+
+int32_t csx_rand_next(csx_rand* r) {
+    int32_t v = 0;
+    if (r) random_r(&r->rd, &v);
+    return v;
+}
+
+int csx_rand_zuf(csx_rand* r, double* out, int64_t n) {
+    if (!r || (!out && n > 0) || n < 0) return CSX_E_ARG;
+    for (int64_t i = 0; i < n; ++i) out[i] = zuf(r);
+    return CSX_OK;
+}
+
+int csx_engine_tpv(csx_engine* e, const char* ids, csx_rand* rng, char* report, int cap) {
+    if (!e) return CSX_E_ARG;
+    std::string rep;
+    if (!ids || !*ids) {  // clear (cpu:1033-1049)
+        e->tpv_on = false;
+        e->p.width_zuf = 1.0;
+        e->prepared = false;
+        rep = "TPV: clear";
+    } else {
+        if (!rng) return CSX_E_ARG;
+        csx::ParamSet& t = e->pt;
+        const double width_zuf = e->tpv_on ? t.width_zuf : e->p.width_zuf;
+        t = e->p;  // the console sets every parameter from its store before each call
+        t.width_zuf = width_zuf;  // not a parameter: keeps the last random factor
+        t.dw_zuf = 1.0;
+        if (!e->tpv_on) {
+            for (int k = 0; k < kNTpv; ++k) e->tpv_old[k] = e->p.*kTpv[k].field;
+            e->tpv_old[10] = width_zuf;
+        }
+        const char* q = ids;
+        while (*q) {
+            const char* nl = std::strchr(q, '\n');
+            std::string line = nl ? std::string(q, nl) : std::string(q);
+            q = nl ? nl + 1 : q + line.size();
+            if (line.empty()) continue;
+            const size_t pos = line.find('=');
+            const std::string key = line.substr(0, pos);
+            std::string val = pos == std::string::npos ? line : line.substr(pos + 1);
+            double dval;
+            try {
+                dval = std::stod(val);
+                if (dval < 0.00001 && val.length() > 2) {  // decimal comma written by another locale
+                    size_t pk = val.find('.');
+                    if (pk != std::string::npos) val[pk] = ',';
+                    else if ((pk = val.find(',')) != std::string::npos) val[pk] = '.';
+                    dval = std::stod(val);
+                }
+            } catch (const std::exception&) {
+                e->err = "TPV: not a number in '" + line + "'";
+                return CSX_E_ARG;
+            }
+            int k = 0;
+            while (k < kNTpv && key != kTpv[k].key) ++k;
+            if (k == kNTpv) continue;  // a name the list does not have
+            double shown = 0.0;
+            switch (kTpv[k].flag) {
+            case TPV_NORM: shown = t.*kTpv[k].field = e->tpv_old[k] * (1 + dval * zuf(rng)); break;
+            case TPV_UCA: shown = t.*kTpv[k].field = t.uca; break;
+            case TPV_PHI: shown = t.phi = 2.0 + (zuf(rng) + 1) * 43.0; break;
+            case TPV_AZI: shown = 4.0 / e->tpv_old[k] * (1 + dval * zuf(rng)); break;
+            case TPV_DW: {
+                const double z = zuf(rng);
+                t.dw_zuf = 1 + dval * z;
+                shown = e->tpv_old[k] * e->tpv_old[k] / 3.0 * t.dw_zuf;
+                break;
+            }
+            }
+            char num[64];
+            std::snprintf(num, sizeof num, "%f", shown);
+            // EOL of Source/sc_math.h:15-19: "\r\n" unless Q_OS_LINUX is defined before it is
+            // included, which the engine's include order does not do
+            rep += key + " = " + num + "\r\n";
+        }
+        // consistency of the particle's dimensions (cpu:1136-1140)
+        if ((t.interior == 1 || t.particle == 3) && t.length <= t.radius) t.length = t.radius + t.length;
+        if (t.particle == 2 && t.length <= 2. * t.radius) t.length = 2. * t.radius + t.length;
+        e->tpv_on = true;
+        e->prepared = false;
+    }
+    if (report && cap > 0) std::snprintf(report, (size_t)cap, "%s", rep.c_str());
+    return CSX_OK;
+}
+
+double csx_engine_tpv_value(csx_engine* e, const char* key) {
+    double v = 0;
+    if (!e || !key || !(e->tpv_on ? e->pt : e->p).get(key, &v)) return 0.0 / 0.0;
+    return v;
+}
+
+int csx_engine_stage_image(csx_engine* e, csx_ctx* ctx, double* d_out, int64_t cap) {
+    if (!e || !ctx || !d_out) return CSX_E_ARG;
+    int rc = e->prepared ? CSX_OK : engine_prepare_host(e);
+    if (rc != CSX_OK) return rc;
+    const csx::PreparedSet& P = e->P;
+    const int w = P.s.xmax - P.s.xmin, h = P.s.ymax - P.s.ymin;
+    if ((int64_t)w * h > cap) { e->err = "image larger than its output buffer"; return CSX_E_ARG; }
+    csx_mask m{};
+    for (int i = 0; i < 16; ++i) m.rects[i] = -1;
+    rc = csx_fit_batch_stage(ctx, &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
+                             P.latpar2.empty() ? nullptr : P.latpar2.data(), P.latpar3.empty() ? nullptr : P.latpar3.data(),
+                             w, h, &m, d_out);
+    if (rc != CSX_OK) {
+        char b[256];
+        csx_last_error(ctx, b, sizeof b);
+        e->err = b;
+    }
+    return rc;
 }
 
 const csx_scalars* csx_engine_scalars(csx_engine* e) { return e ? &e->P.s : nullptr; }

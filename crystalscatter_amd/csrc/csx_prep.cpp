@@ -62,7 +62,8 @@ const KeyDef kKeys[] = {
     {"CenterBeam", &ParamSet::centerbeam, nullptr}, {"EditBFactor", nullptr, &ParamSet::bfactor},
     {"CheckBoxWAXS", &ParamSet::waxs, nullptr}, {"P1", nullptr, &ParamSet::p1},
     {"iso", nullptr, &ParamSet::iso}, {"I0", nullptr, &ParamSet::i0}, {"Base", nullptr, &ParamSet::base},
-    {"width_zuf", nullptr, &ParamSet::width_zuf}, {"fix_orcase", &ParamSet::fix_orcase, nullptr},
+    {"width_zuf", nullptr, &ParamSet::width_zuf}, {"dw_zuf", nullptr, &ParamSet::dw_zuf},
+    {"fix_orcase", &ParamSet::fix_orcase, nullptr},
     {"VAx1", nullptr, &ParamSet::vax1}, {"VAx2", nullptr, &ParamSet::vax2}, {"VAx3", nullptr, &ParamSet::vax3},
     {"Ay1", nullptr, &ParamSet::ay1}, {"Ay2", nullptr, &ParamSet::ay2}, {"Ay3", nullptr, &ParamSet::ay3},
     {"Az1", nullptr, &ParamSet::az1}, {"Az2", nullptr, &ParamSet::az2}, {"Az3", nullptr, &ParamSet::az3},
@@ -663,7 +664,7 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     s.det = p.det; s.wave = p.wave; s.beamx = p.beamx; s.beamy = p.beamy;
     s.radius = p.radius; s.sigma = p.sigma; s.length = p.length; s.sigmal = p.sigmal; s.dbeta = p.dbeta;
     s.base = p.base; s.i0 = p.i0; s.iso = p.iso; s.ifluc = p.ifluc; s.rfluc = p.rfluc;
-    s.dwfactor = p.displacement * p.displacement / 3.0;
+    s.dwfactor = p.displacement * p.displacement / 3.0 * p.dw_zuf;
     s.uca = p.uca; s.ucb = p.ucb; s.domainsize = p.domainsize; s.width_zuf = p.width_zuf;
     s.phiwidth = 4. / p.azi;
     s.ceff = p.ceff;

@@ -31,6 +31,7 @@ struct ParamSet {
     double qmax = 1, qmin = 0, pixnox = 2048, pixnoy = 2048, pixx_mm = 1, pixy_mm = 1, det = 1;
     double wave = 0.09499, beamx = 0, beamy = 0, bfactor = 1, p1 = 0, iso = 0, i0 = 10000, base = 0;
     double width_zuf = 1;
+    double dw_zuf = 1;  // random factor on the Debye-Waller term (TPV), 1 = none
     // Peak-shape axes and domain sizes (setPeakShapeMatrix): accepted and kept for a complete
     // key table; none of the device combinations reads them.
     double vax1 = 1, vax2 = 0, vax3 = 0, ay1 = 0, ay2 = 1, ay3 = 0, az1 = 0, az2 = 0, az3 = 1;

@@ -148,6 +148,12 @@ int csx_fit_batch_stage(csx_ctx* ctx, const csx_scalars* s, const double* h_carr
                         const int32_t* h_latpar1, const int32_t* h_latpar2, const float* h_latpar3,
                         int w, int h, const csx_mask* mask, const double* d_data);
 int csx_fit_batch_run(csx_ctx* ctx, int precision, double* h_sums, int64_t* h_cnts);
+/* Batched images (no reference counterpart; replaces one doCalculation per parameter set in
+ * the AI-data loop, Source/sc_mainCons.cpp:916-1440): renders every set staged since
+ * csx_fit_batch_begin in one launch.  Each set is staged with w, h = its whole grid
+ * (xmax-xmin, ymax-ymin; Q4 or Q1 layout), no mask, and d_data = its device output of w*h
+ * doubles, written in IDX order.  All sets share one combination key.  Synchronous. */
+int csx_render_batch_run(csx_ctx* ctx, int precision);
 
 /* ---- Batched Nelder-Mead fits ------------------------------------------------------
  * The simplex of SasCalc_SimplexFit2D::doSimplexFit2D (Source/sc_simplexfit2d.cpp:254-611,
@@ -224,6 +230,30 @@ const void* csx_engine_table(csx_engine* e, const char* name, int64_t* n, int* k
 const csx_scalars* csx_engine_scalars(csx_engine* e);
 csx_ctx* csx_engine_ctx(csx_engine* e);
 int csx_engine_last_error(csx_engine* e, char* buf, int len);
+/* Stage the engine's image (checks and host preparation included) into ctx's pending batch
+ * for csx_render_batch_run, output d_out (room for cap doubles); the engine can be changed
+ * right after. */
+int csx_engine_stage_image(csx_engine* e, csx_ctx* ctx, double* d_out, int64_t cap);
+
+/* ---- Random variation for training data (tpvPerformRandom) ------------------------------
+ * csx_rand: the C library's rand() sequence (srand(seed), then rand() values) in a private
+ * state; the reference draws from the process-wide one, seeded 1 unless srand is called.
+ * csx_rand_zuf fills n values of zuf() = (rand()/RAND_MAX*200 - 100)/100 in [-1, 1]
+ * (Source/sc_mainCons.cpp:850-853), e.g. the noise of the AI-data loop. */
+typedef struct csx_rand csx_rand;
+csx_rand* csx_rand_new(uint32_t seed);
+void csx_rand_free(csx_rand* r);
+int32_t csx_rand_next(csx_rand* r);
+int csx_rand_zuf(csx_rand* r, double* h_out, int64_t n);
+/* tpvPerformRandom (Source/sc_calc_generic_cpu.cpp:1028-1143): ids holds "key=factor" lines
+ * ('\n'-separated); each listed variable gets its value at the first call after a clear times
+ * (1 + factor*zuf()) (ucb/ucc: the current uca; phi: 2 + 43*(zuf()+1); EditAzi: draws but has
+ * no effect; EditDebyeWaller: the Debye-Waller factor), then the cylinder/disk length checks.
+ * ids NULL or empty clears.  The values apply to the next preparation; csx_engine_get keeps
+ * returning the parameters as set, csx_engine_tpv_value the varied ones.  report (optional)
+ * receives the reference's "key = value" lines. */
+int csx_engine_tpv(csx_engine* e, const char* ids, csx_rand* rng, char* report, int cap);
+double csx_engine_tpv_value(csx_engine* e, const char* key);
 
 /* ---- Surrogate network (no counterpart in the reference) --------------------------------
  * The learned part of the fits: an MLP from a normalised parameter set (CSX_MLP_NPAR values)

@@ -61,7 +61,7 @@ struct csxo_engine {
     double vax[3], vay[3], vaz[3], sigxyz[3];
     double ucpsi, ucn1, ucn2, ucn3, theta, phi, rottheta, rotphi;
     double qmax, qmin, pixnox, pixnoy, pixx_mm, pixy_mm, det, wave, beamx, beamy;
-    double bfactor, p1in, iso, i0, base, width_zuf;
+    double bfactor, p1in, iso, i0, base, width_zuf, dw_zuf;
     int fix_orcase;
     /* ---- prepared state ---- */
     int prepared, shp, cdim, cs, lattice, twin, zmax, use1d;
@@ -95,7 +95,7 @@ csxo_engine* csxo_new(void)
     e->ucn1 = e->ucn2 = e->ucn3 = 1; e->theta = 90; e->phi = 0; e->rottheta = 90; e->rotphi = 90;
     e->gridpoints = 64; e->hklmax = 5; e->qmax = 1; e->radq4 = 1;
     e->pixnox = e->pixnoy = 2048; e->pixx_mm = e->pixy_mm = 1; e->det = 1; e->wave = 0.09499;
-    e->bfactor = 1; e->i0 = 10000; e->width_zuf = 1;
+    e->bfactor = 1; e->i0 = 10000; e->width_zuf = 1; e->dw_zuf = 1;
     return e;
 }
 
@@ -128,7 +128,7 @@ static const keydef kKeys[] = {
     KD("EditPixelY", pixy_mm), KD("EditDet", det), KD("EditWavelength", wave),
     KD("BeamPosX", beamx), KD("BeamPosY", beamy), KI("CenterBeam", centerbeam),
     KD("EditBFactor", bfactor), KI("CheckBoxWAXS", waxs), KD("P1", p1in), KD("iso", iso),
-    KD("I0", i0), KD("Base", base), KD("width_zuf", width_zuf), KI("fix_orcase", fix_orcase),
+    KD("I0", i0), KD("Base", base), KD("width_zuf", width_zuf), KD("dw_zuf", dw_zuf), KI("fix_orcase", fix_orcase),
 };
 
 int csxo_set(csxo_engine* e, const char* key, double v)
@@ -887,7 +887,8 @@ int csxo_prepare(csxo_engine* e)
 {
     /* Source/sc_calc_generic_cpu.cpp:319-988, the parts that feed the hot path. */
     e->shp = e->peakidx + 1; /* setComboBoxPeak: shp = index+1 (.h:254) */
-    e->dwfactor = e->displacement * e->displacement / 3.0;   /* setDisplacement (.h:257) */
+    /* setDisplacement (.h:257); dw_zuf: the TPV factor applied afterwards (cpu:1126-1130) */
+    e->dwfactor = e->displacement * e->displacement / 3.0 * e->dw_zuf;
     e->pixx_m = e->pixx_mm / 1000.; e->pixy_m = e->pixy_mm / 1000.;
     e->zmax = e->gridpoints;
     e->use1d = 0;

@@ -20,6 +20,8 @@
 //     nofit<i>=x0,y0,x1,y1 no-fit rectangles (fit mode)
 //     ignorecbs=0|1        bIgnoreCbs for doCalculation (default 1)
 //     repeat=<n>           repeat the calculation n times (timing)
+//     tpv=<k=f|k=f|...>    tpvPerformRandom(list) before each of tpvn=<n> images
+//                          (srand=<s> seeds rand() first; default: the process's seed 1)
 
 #include "sc_calc_generic_gpu.h"
 
@@ -27,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <list>
 #include <map>
 #include <sstream>
 #include <string>
@@ -75,37 +78,42 @@ int main(int argc, char** argv) {
 
     SasCalc_GENERIC_calculation* c = new SasCalc_GENERIC_calculation();
 
-    // Defaults are the key-table defaults (Source/sc_calc_generic.cpp:36-138).
-    c->setLatticeParams((int)num("LType", 12), num("uca", 30), num("ucb", 30), num("ucc", 30),
-                        num("ucalpha", 90), num("ucbeta", 90), num("ucgamma", 90),
-                        num("EditCeff", 1), num("CheckBoxTwinned", 0) != 0, num("EditCeffcyl", 0),
-                        num("reff", 0), num("acpl", 0), num("bcpl", 0),
-                        num("ifluc", 0), num("rfluc", 0));
-    c->setParticleParams((int)num("ComboBoxParticle", 0), (int)num("Ordis", 7),
-                         (int)num("ComboBoxInterior", 0), num("EditRadius", 1.0),
-                         num("EditRadiusi", 0.5), num("EditSigma", 0.07), num("EditDbeta", 0.4),
-                         num("Length", 1), num("SigmaL", 0.06), num("Alpha", 0), num("EditRho", 0),
-                         num("RotAlpha", 0));
-    c->setPeakShapeParams((int)num("ComboBoxPeak", 1), num("EditPeakPar", 0),
-                          num("EditDebyeWaller", 2), num("EditAzi", 180),
-                          num("EditDomainSize", 400), num("RadButDebyeScherrer", 0) != 0,
-                          num("RadioButtonPara", 0) != 0);
-    c->setPeakShapeMatrix(num("VAx1", 1), num("VAx2", 0), num("VAx3", 0),
-                          num("Ay1", 0), num("Ay2", 1), num("Ay3", 0),
-                          num("Az1", 0), num("Az2", 0), num("Az3", 1),
-                          num("SigX", 40), num("SigY", 40), num("SigZ", 40));
-    c->setOrientationParams(num("ucpsi", 0), num("ucn1", 1), num("ucn2", 1), num("ucn3", 1),
-                            num("theta", 90), num("phi", 0), num("rotTheta", 90), num("rotPhi", 90));
-    int q1 = (int)num("RadioButtonQ1", 0), q2 = (int)num("RadioButtonQ2", 0);
-    int q4 = (int)num("RadioButtonQ4", (q1 || q2) ? 0 : 1);
-    c->setCalculationParams((int)num("GridPoints", 64), (int)num("HKLmax", 5), num("EditQmax", 1),
-                            q1 != 0, q2 != 0, q4 != 0, num("ExpandImage", 0) != 0, false);
-    c->setExperimentParams(num("EditPixelNoX", 2048), num("EditPixelNoY", 2048),
-                           num("EditPixelX", 1), num("EditPixelY", 1), num("EditDet", 1),
-                           num("EditWavelength", 0.09499), num("BeamPosX", 0), num("BeamPosY", 0),
-                           num("CenterBeam", 0) != 0);
-    c->setControlParams(num("EditBFactor", 1), num("CheckBoxWAXS", 0) != 0, num("P1", 0));
-    c->setPixelManipulationParams(num("iso", 0), num("I0", 10000), num("Base", 0));
+    // Every setter, as the console's prepareData calls them before each image
+    // (Source/sc_calc_generic.cpp:142-250).
+    auto apply_params = [&]() {
+        // Defaults are the key-table defaults (Source/sc_calc_generic.cpp:36-138).
+        c->setLatticeParams((int)num("LType", 12), num("uca", 30), num("ucb", 30), num("ucc", 30),
+                            num("ucalpha", 90), num("ucbeta", 90), num("ucgamma", 90),
+                            num("EditCeff", 1), num("CheckBoxTwinned", 0) != 0, num("EditCeffcyl", 0),
+                            num("reff", 0), num("acpl", 0), num("bcpl", 0),
+                            num("ifluc", 0), num("rfluc", 0));
+        c->setParticleParams((int)num("ComboBoxParticle", 0), (int)num("Ordis", 7),
+                             (int)num("ComboBoxInterior", 0), num("EditRadius", 1.0),
+                             num("EditRadiusi", 0.5), num("EditSigma", 0.07), num("EditDbeta", 0.4),
+                             num("Length", 1), num("SigmaL", 0.06), num("Alpha", 0), num("EditRho", 0),
+                             num("RotAlpha", 0));
+        c->setPeakShapeParams((int)num("ComboBoxPeak", 1), num("EditPeakPar", 0),
+                              num("EditDebyeWaller", 2), num("EditAzi", 180),
+                              num("EditDomainSize", 400), num("RadButDebyeScherrer", 0) != 0,
+                              num("RadioButtonPara", 0) != 0);
+        c->setPeakShapeMatrix(num("VAx1", 1), num("VAx2", 0), num("VAx3", 0),
+                              num("Ay1", 0), num("Ay2", 1), num("Ay3", 0),
+                              num("Az1", 0), num("Az2", 0), num("Az3", 1),
+                              num("SigX", 40), num("SigY", 40), num("SigZ", 40));
+        c->setOrientationParams(num("ucpsi", 0), num("ucn1", 1), num("ucn2", 1), num("ucn3", 1),
+                                num("theta", 90), num("phi", 0), num("rotTheta", 90), num("rotPhi", 90));
+        int q1 = (int)num("RadioButtonQ1", 0), q2 = (int)num("RadioButtonQ2", 0);
+        int q4 = (int)num("RadioButtonQ4", (q1 || q2) ? 0 : 1);
+        c->setCalculationParams((int)num("GridPoints", 64), (int)num("HKLmax", 5), num("EditQmax", 1),
+                                q1 != 0, q2 != 0, q4 != 0, num("ExpandImage", 0) != 0, false);
+        c->setExperimentParams(num("EditPixelNoX", 2048), num("EditPixelNoY", 2048),
+                               num("EditPixelX", 1), num("EditPixelY", 1), num("EditDet", 1),
+                               num("EditWavelength", 0.09499), num("BeamPosX", 0), num("BeamPosY", 0),
+                               num("CenterBeam", 0) != 0);
+        c->setControlParams(num("EditBFactor", 1), num("CheckBoxWAXS", 0) != 0, num("P1", 0));
+        c->setPixelManipulationParams(num("iso", 0), num("I0", 10000), num("Base", 0));
+    };
+    apply_params();
 
     const int threads = (int)num("threads", 1);
     const int repeat = (int)num("repeat", 1);
@@ -139,6 +147,28 @@ int main(int argc, char** argv) {
         w.scalar("cnt", (double)cnt);
         w.scalar("nancnt", (double)nancnt);
         w.d("fitfqs", c->arrFitFqs, (long long)fw * fh);
+    } else if (g_kv.count("tpv")) {
+        // tpvPerformRandom before each of tpvn images (Source/sc_mainCons.cpp:1236-1246), with
+        // the process's rand() state as the reference run has it (seed 1 unless srand=<n>).
+        if (g_kv.count("srand")) std::srand((unsigned)num("srand", 1));
+        std::list<std::string> ids;
+        std::stringstream ss(g_kv["tpv"]);
+        std::string item;
+        while (std::getline(ss, item, '|')) ids.push_back(item);
+        const int n = (int)num("tpvn", 1);
+        std::vector<double> imgs;
+        std::string reports;
+        for (int r = 0; r < n; ++r) {
+            apply_params();
+            reports += c->tpvPerformRandom(ids) + "#";
+            c->doCalculation(threads, false, num("ignorecbs", 1) != 0);
+            const long long np = (long long)(c->maxX() - c->minX()) * (c->maxY() - c->minY());
+            imgs.insert(imgs.end(), c->data(), c->data() + np);
+        }
+        w.d("tpv_images", imgs.data(), (long long)imgs.size());
+        w.put("tpv_report", 's', reports.c_str(), (long long)reports.size(), 1);
+        double b[4] = {(double)c->minX(), (double)c->maxX(), (double)c->minY(), (double)c->maxY()};
+        w.d("bounds", b, 4);
     } else {
         for (int r = 0; r < repeat; ++r) {
             c->doCalculation(threads, false, num("ignorecbs", 1) != 0);

@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube                              # only the cube ones
+    python tests/golden/make_golden.py cube tpv                          # only these
 """
 import json
 import os
@@ -100,6 +100,22 @@ def write_cube(manifest):
         manifest["fixtures"][name] = {"params": p}
 
 
+# Random variation (tpvPerformRandom) before each of four FCC images, rand() from seed 1.
+TPV_IDS = ["EditRadius=0.1", "EditSigma=0.2", "uca=0.05", "ucb=1", "ucc=1", "I0=0.3", "Base=0.1",
+           "EditDomainSize=0.4", "EditDebyeWaller=0.5", "EditAzi=0.3", "phi=1"]
+TPV_N = 4
+
+
+def write_tpv(manifest):
+    p = dict(CFG3, GridPoints=16)
+    r = run_ref(p, {"tpv": "|".join(TPV_IDS), "tpvn": TPV_N})
+    rep = r["tpv_report"]
+    rep = rep if isinstance(rep, str) else bytes(rep).decode()
+    np.savez_compressed(os.path.join(HERE, "tpv_fcc.npz"), images=r["tpv_images"].reshape(TPV_N, -1),
+                        bounds=r["bounds"], report=np.frombuffer(rep.encode(), np.uint8))
+    manifest["fixtures"]["tpv_fcc.npz"] = {"params": p, "tpv": TPV_IDS, "n": TPV_N, "seed": 1}
+
+
 def main():
     if not os.path.exists(REF_DRIVER):
         sys.exit("build the reference first: make -C oracle ref")
@@ -109,6 +125,8 @@ def main():
             manifest = json.load(fh)
         if "cube" in only:
             write_cube(manifest)
+        if "tpv" in only:
+            write_tpv(manifest)
         with open(os.path.join(HERE, "manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
         return
@@ -183,6 +201,7 @@ def main():
                                             "fitdata": "uniform(0.01,10) seed 5678, 32x32"}
 
     write_cube(manifest)
+    write_tpv(manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
