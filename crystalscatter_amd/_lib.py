@@ -120,6 +120,12 @@ SIGNATURES = [
     ("csx_rand_zuf", _I, [_VP, _PD, ctypes.c_int64]),
     ("csx_engine_tpv", _I, [_VP, _CP, _VP, _CP, _I]),
     ("csx_engine_tpv_value", _D, [_VP, _CP]),
+    ("csx_post_noise", _I, [_VP, _VP, _VP, ctypes.c_int64, _I]),
+    ("csx_post_zero", _I, [_VP, _VP, _VP, ctypes.c_int64]),
+    ("csx_post_scale_intensity", _I, [_VP, _VP, ctypes.c_int64, _I, _I]),
+    ("csx_post_log_scale", _I, [_VP, _VP, ctypes.c_int64, _I, _I, _I, _I, _I]),
+    ("csx_post_scale_clip", _I, [_VP, _VP, ctypes.c_int64, _I, _I, _I, _I, _I]),
+    ("csx_post_rphi", _I, [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _VP, _I, _I, _I]),
     ("csx_mlp_nparams", ctypes.c_int64, []),
     ("csx_mlp_create", _I, [_I, _I, ctypes.c_uint64, ctypes.POINTER(_VP)]),
     ("csx_mlp_destroy", None, [_VP]),

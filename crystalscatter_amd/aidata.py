@@ -1,0 +1,205 @@
+"""Training images on the GPU: the console's AI-data loop (generateAIfiles,
+Source/sc_mainCons.cpp:916-1440) for one parameter line, batched.
+
+Per image the reference runs: tpvPerformRandom, the calculation, then optionally
+scaleIntensity (linear), noise d + z sqrt(d), a randomly placed beam stop and random lines of
+zeroed pixels, and the r,phi resampling of the (log-scaled) image with its log rescaling.  All
+random numbers come from one rand() sequence in that order, image after image.  Here the host
+walks the images in the same order -- random variation, staging, noise values, beam stop and
+line positions -- and the device then renders all images in one launch and post-processes them
+batch-wide (csx_render_batch_run, csx_post_*), so the images equal a reference run that starts
+from the same rand() state.
+
+Options (names after the reference's command-line settings, Source/sc_mainCons.cpp:1000-1120):
+scale_scat (tpvScaleScat), noise (tpvAddNoise), bs = (x0, y0, xs, ys) factors (tpvBSx0..ys),
+lines_x / lines_x_width and lines_y / lines_y_width (tpvLinesX, ...), rphi_size (fftRphiSize)
+with rphi_scale / rphi_clip / rphi_clip40, and fft_input_lin (fftInputLin).  The FFT output
+and the PNG/file writing are not part of this path.
+"""
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import CsxError, _lib, _prec
+
+
+@dataclass
+class AIOptions:
+    scale_scat: bool = False
+    noise: bool = False
+    bs: tuple = (0.0, 0.0, 0.0, 0.0)
+    lines_x: int = 0
+    lines_x_width: int = 1
+    lines_y: int = 0
+    lines_y_width: int = 1
+    rphi_size: int = 0
+    rphi_scale: bool = False
+    rphi_clip: bool = False
+    rphi_clip40: bool = False
+    fft_input_lin: bool = False
+
+
+def _zuf(rng):
+    return float(rng.zuf(1)[0])
+
+
+def _zero_indices(rng, opts, bounds, beam):
+    """Pixels the beam stop and the lines zero, and the (random) beam-stop centre, drawing
+    from rng in the reference's order (Source/sc_mainCons.cpp:1298-1348).  Indices follow the
+    reference's ii = (x - minX) + sx (y - minY), kept when 0 <= ii < len (a line past the
+    right edge therefore continues on the next row, as there)."""
+    x0, x1, y0, y1 = bounds
+    sx = x1 - x0
+    n = sx * (y1 - y0)
+    bsx0, bsy0 = int(beam[0]), int(beam[1])
+    bsdx = bsdy = 2
+    idx = []
+    if opts.bs[0] > 0:
+        bsx0 = int(bsx0 * (1.0 + opts.bs[0] * _zuf(rng)))
+        bsy0 = int(bsy0 * (1.0 + opts.bs[1] * _zuf(rng)))
+        bsdx = int(bsdx * (1.0 + opts.bs[2] * _zuf(rng)))
+        bsdy = int(bsdy * (1.0 + opts.bs[3] * _zuf(rng)))
+        for x in range(x0, x1):
+            if bsx0 - bsdx <= x <= bsx0 + bsdx:
+                for y in range(y0, y1):
+                    if bsy0 - bsdy <= y <= bsy0 + bsdy:
+                        idx.append((x - x0) + sx * (y - y0))
+    if opts.lines_x > 0:
+        xc = (x1 - x0 + 1) // opts.lines_x
+        for i in range(opts.lines_x):
+            xp = int(x0 + i * xc + math.fabs(_zuf(rng)) * xc)
+            for x in range(xp, xp + opts.lines_x_width):
+                for y in range(y0, y1):
+                    idx.append((x - x0) + sx * (y - y0))
+    if opts.lines_y > 0:
+        yc = (y1 - y0 + 1) // opts.lines_y
+        for i in range(opts.lines_y):
+            yp = int(y0 + i * yc + math.fabs(_zuf(rng)) * yc)
+            for y in range(yp, yp + opts.lines_y_width):
+                for x in range(x0, x1):
+                    idx.append((x - x0) + sx * (y - y0))
+    ii = np.array([v for v in idx if 0 <= v < n], dtype=np.int64)
+    return ii, (bsx0, bsy0)
+
+
+def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed"):
+    """n training images of engine's parameters with random variation `ids` (see
+    SasCalc.tpvPerformRandom), post-processed per `opts`.  Returns a dict with "scat"
+    ((n, npx) float64 CUDA tensor, IDX order), "rphi" ((n, s*s), when rphi_size > 0),
+    "bounds", "centres" (the beam-stop centre per image) and "reports"."""
+    import torch
+    opts = opts or AIOptions()
+    L = _lib.lib()
+    engine.tpvPerformRandom([], None)
+    engine.prepare()
+    bounds = tuple(engine.bounds())
+    x0, x1, y0, y1 = bounds
+    npx = (x1 - x0) * (y1 - y0)
+    beam = (engine.get("BeamPosX"), engine.get("BeamPosY"))
+    scat = torch.empty((n, npx), dtype=torch.float64, device="cuda")
+    z = np.empty((n, npx)) if opts.noise else None
+    zero_idx, centres, reports = [], [], []
+    ctx._chk(L.csx_fit_batch_begin(ctx._h), "csx_fit_batch_begin")
+    for k in range(n):
+        reports.append(engine.tpvPerformRandom(ids, rng))
+        engine.stage_image(ctx, scat[k].data_ptr(), npx)
+        if opts.noise:
+            z[k] = rng.zuf(npx)
+        ii, c = _zero_indices(rng, opts, bounds, beam)
+        zero_idx.append(ii + k * npx)
+        centres.append(c)
+    ctx._chk(L.csx_render_batch_run(ctx._h, _prec(precision)), "csx_render_batch_run")
+    d = ctypes.c_void_p(scat.data_ptr())
+    if opts.scale_scat:
+        ctx._chk(L.csx_post_scale_intensity(ctx._h, d, npx, n, 1), "csx_post_scale_intensity")
+    keep = []
+    if opts.noise:
+        zt = torch.from_numpy(z.reshape(-1)).cuda()
+        keep.append(zt)
+        _torch_done()
+        ctx._chk(L.csx_post_noise(ctx._h, d, ctypes.c_void_p(zt.data_ptr()), n * npx, int(opts.scale_scat)),
+                 "csx_post_noise")
+    allz = np.concatenate(zero_idx) if zero_idx else np.zeros(0, np.int64)
+    if allz.size:
+        it = torch.from_numpy(allz).cuda()
+        keep.append(it)
+        _torch_done()
+        ctx._chk(L.csx_post_zero(ctx._h, d, ctypes.c_void_p(it.data_ptr()), it.numel()), "csx_post_zero")
+    out = {"scat": scat, "bounds": bounds, "centres": centres, "reports": reports}
+    if opts.rphi_size > 0:
+        s = int(opts.rphi_size)
+        src = scat
+        if not opts.fft_input_lin:
+            ctx._chk(L.csx_synchronize(ctx._h), "csx_synchronize")
+            src = scat.clone()
+            _torch_done()
+            ctx._chk(L.csx_post_log_scale(ctx._h, ctypes.c_void_p(src.data_ptr()), npx, n, x1 - x0, x0, y0, 1),
+                     "csx_post_log_scale")
+        rphi = torch.empty((n, s * s), dtype=torch.float64, device="cuda")
+        groups = {}
+        for k, c in enumerate(centres):
+            groups.setdefault(c, []).append(k)
+        for c, ks in groups.items():
+            if len(ks) == n:
+                args = [(0, n)]
+            else:
+                args = [(k, 1) for k in ks]
+            for k0, cnt in args:
+                ctx._chk(L.csx_post_rphi(ctx._h, ctypes.c_void_p(src[k0].data_ptr()), x0, x1, y0, y1, c[0], c[1], s, cnt,
+                                         ctypes.c_void_p(rphi[k0].data_ptr()), int(opts.rphi_scale), int(opts.rphi_clip),
+                                         int(opts.rphi_clip40)), "csx_post_rphi")
+        ctx._chk(L.csx_post_scale_clip(ctx._h, ctypes.c_void_p(rphi.data_ptr()), s * s, n, 0, 0, 0, 1),
+                 "csx_post_scale_clip")
+        out["rphi"] = rphi
+        keep.append(src)
+    ctx._chk(L.csx_synchronize(ctx._h), "csx_synchronize")
+    return out
+
+
+def _torch_done():
+    """Work queued on torch's stream (copies, clones) is finished before the library's stream
+    reads it."""
+    import torch
+    torch.cuda.current_stream().synchronize()
+
+
+def post_scale_intensity(ctx, imgs, linlog):
+    """scaleIntensity of every row of imgs (float64 CUDA tensor (nimg, npx)), in place."""
+    _chk_rows(imgs)
+    _torch_done()
+    ctx._chk(_lib.lib().csx_post_scale_intensity(ctx._h, ctypes.c_void_p(imgs.data_ptr()), imgs.shape[1],
+                                                 imgs.shape[0], int(linlog)), "csx_post_scale_intensity")
+    ctx.synchronize()
+
+
+def post_scale_clip(ctx, data, scale=False, clip=False, clip40=False, genlog=False):
+    """scaleAndClipData of every row of data, in place."""
+    _chk_rows(data)
+    _torch_done()
+    ctx._chk(_lib.lib().csx_post_scale_clip(ctx._h, ctypes.c_void_p(data.data_ptr()), data.shape[1], data.shape[0],
+                                            int(scale), int(clip), int(clip40), int(genlog)), "csx_post_scale_clip")
+    ctx.synchronize()
+
+
+def post_rphi(ctx, imgs, bounds, centre, s, scale=False, clip=False, clip40=False):
+    """generateRphi of every row of imgs around one centre; returns (nimg, s*s)."""
+    import torch
+    _chk_rows(imgs)
+    x0, x1, y0, y1 = (int(b) for b in bounds)
+    if (x1 - x0) * (y1 - y0) != imgs.shape[1]:
+        raise ValueError("bounds do not match the image size")
+    out = torch.empty((imgs.shape[0], s * s), dtype=torch.float64, device="cuda")
+    _torch_done()
+    ctx._chk(_lib.lib().csx_post_rphi(ctx._h, ctypes.c_void_p(imgs.data_ptr()), x0, x1, y0, y1, int(centre[0]),
+                                      int(centre[1]), int(s), imgs.shape[0], ctypes.c_void_p(out.data_ptr()),
+                                      int(scale), int(clip), int(clip40)), "csx_post_rphi")
+    ctx.synchronize()
+    return out
+
+
+def _chk_rows(t):
+    import torch
+    if t.dtype != torch.float64 or not t.is_cuda or t.dim() != 2 or not t.is_contiguous():
+        raise CsxError("expected a contiguous float64 CUDA tensor (rows, values)")

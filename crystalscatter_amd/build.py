@@ -44,7 +44,8 @@ def build(verbose=True):
     _run(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, "csx_device.hip"), "-o", dev_o])
     eng_o = os.path.join(BUILD, "csx_engine.o")
     _run(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, "csx_engine.cpp"), "-o", eng_o])
-    extra = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.startswith("csx_mlp") and f.endswith(".hip")]
+    extra = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC))
+             if f.startswith(("csx_mlp", "csx_post")) and f.endswith(".hip")]
     objs = [prep_o, fit_o, dev_o, eng_o]
     for src in extra:
         o = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))

@@ -1596,6 +1596,7 @@ struct csx_ctx {
     struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso; };
     std::vector<Pending> bpend;
     DevBuf barena_d, bpart_d, bsums_d;
+    DevBuf post[2];  // scratch of the post-processing calls (csx_post.hip)
 };
 
 static int fail(csx_ctx* c, int code, const std::string& msg) {
@@ -1772,7 +1773,7 @@ void csx_destroy(csx_ctx* c) {
     c->bres.release();
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
-                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d})
+                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1]})
         b->release();
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_rad[0], c->ev_rad[1], c->ev_img[0], c->ev_img[1]})
         if (e) (void)hipEventDestroy(e);
@@ -2262,6 +2263,14 @@ extern "C" int csx_render_batch_run(csx_ctx* c, int precision) {
     c->last_ms = ms;
     c->timing_pending = false;
     return CSX_OK;
+}
+
+int csx_ctx_device(const csx_ctx* c) { return c ? c->device : -1; }
+void* csx_ctx_stream(const csx_ctx* c) { return c ? (void*)c->stream : nullptr; }
+int csx_ctx_fail(csx_ctx* c, int code, const char* msg) { return fail(c, code, msg ? msg : ""); }
+void* csx_ctx_scratch(csx_ctx* c, int which, size_t bytes) {
+    if (!c || which < 0 || which > 1) return nullptr;
+    return c->post[which].ensure(bytes) == hipSuccess ? c->post[which].p : nullptr;
 }
 
 // A second context on the same device (its own stream and staging memory).

@@ -1,0 +1,336 @@
+// Post-processing of batches of images already in HBM, for training data: the steps of the
+// console's AI-data loop after each calculation (Source/sc_mainCons.cpp:1254-1400) -- noise,
+// zeroed pixels (beam stop, lines), intensity scaling (scaleIntensity,
+// Source/sc_calc_generic_cpu.cpp:11195-11236; scaleAndClipData, Source/sc_postproc.cpp:764-830)
+// and the r,phi resampling (generateRphi, Source/sc_postproc.cpp:53-360).
+//
+// Images are n consecutive arrays of npx doubles.  The reference's scans keep a running
+// minimum and test the maximum only when a value is not a new minimum ("if (v < lo) lo = v;
+// else if (v > hi) hi = v"), so hi is the maximum over the values that are not strict prefix
+// minima.  One workgroup per image reproduces that exactly: per-lane chunk minima, a
+// workgroup prefix-min scan, then each lane walks its chunk in order from its prefix minimum.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "csx.h"
+#include "csx_prep.h"
+
+namespace csp {
+
+constexpr int kStat = 1024;  // lanes of the per-image statistics workgroup
+
+// Which values a statistics pass reads and its start values.
+enum StatMode {
+    ST_LIN = 0,      // every value; lo 1e20, hi 1e-20 (scaleIntensity, linear)
+    ST_LOG = 1,      // log10 of positive values; lo 1e20, hi 1e-20 (scaleIntensity, log)
+    ST_GENLOG = 2,   // log10 of positive values; lo 1e26, hi 0 (scaleAndClipData, genlog)
+    ST_GENLOG_AXES = 3  // as ST_GENLOG, skipping the pixels on x = 0 or y = 0 (AI loop, :1366-1378)
+};
+
+struct StatArgs {
+    int64_t npx;
+    int mode;
+    int extra_zero;   // the scan covers one more value, 0 (scaleIntensity reads one past the image)
+    int w, xmin, ymin;  // image geometry for ST_GENLOG_AXES
+};
+
+__device__ __forceinline__ bool stat_value(const StatArgs& a, const double* img, int64_t i, double& v) {
+    const double p = (i < a.npx) ? img[i] : 0.0;
+    if (a.mode == ST_LIN) { v = p; return true; }
+    if (!(p > 0)) return false;
+    if (a.mode == ST_GENLOG_AXES && a.xmin < 0) {
+        const int x = (int)(i % a.w) + a.xmin, y = (int)(i / a.w) + a.ymin;
+        if (x == 0 || y == 0) return false;
+    }
+    v = log10(p);
+    return true;
+}
+
+// lohi[2*image] = lo, [2*image+1] = hi of each image (blockIdx.x = image).
+__global__ void __launch_bounds__(kStat) k_stats(const double* __restrict__ imgs, StatArgs a, double* __restrict__ lohi) {
+    __shared__ double s_min[kStat];
+    __shared__ double s_max[kStat];
+    const double* img = imgs + (int64_t)blockIdx.x * a.npx;
+    const int64_t n = a.npx + (a.extra_zero ? 1 : 0);
+    const int64_t chunk = (n + kStat - 1) / kStat;
+    const int64_t b = (int64_t)threadIdx.x * chunk, e = min(n, b + chunk);
+    const bool sil = a.mode == ST_LIN || a.mode == ST_LOG;
+    const double lo0 = sil ? 1e20 : 1e26, hi0 = sil ? 1e-20 : 0.0;
+    double m = INFINITY;
+    for (int64_t i = b; i < e; ++i) {
+        double v;
+        if (stat_value(a, img, i, v) && v < m) m = v;
+    }
+    s_min[threadIdx.x] = m;
+    __syncthreads();
+    // inclusive prefix minimum over the lanes (Hillis-Steele)
+    for (int off = 1; off < kStat; off <<= 1) {
+        const double o = threadIdx.x >= off ? s_min[threadIdx.x - off] : INFINITY;
+        __syncthreads();
+        if (o < s_min[threadIdx.x]) s_min[threadIdx.x] = o;
+        __syncthreads();
+    }
+    double run = threadIdx.x ? s_min[threadIdx.x - 1] : INFINITY;
+    if (lo0 < run) run = lo0;
+    double hi = -INFINITY;
+    for (int64_t i = b; i < e; ++i) {
+        double v;
+        if (!stat_value(a, img, i, v)) continue;
+        if (v < run) run = v;
+        else if (v > hi) hi = v;
+    }
+    s_max[threadIdx.x] = hi;
+    __syncthreads();
+    for (int off = kStat / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off && s_max[threadIdx.x + off] > s_max[threadIdx.x]) s_max[threadIdx.x] = s_max[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double lo = s_min[kStat - 1] < lo0 ? s_min[kStat - 1] : lo0;
+        double h = s_max[0] > hi0 ? s_max[0] : hi0;
+        if (sil && fabs(lo - h) < 1e-6) {  // scaleIntensity: widen an empty range
+            lo *= 0.998;
+            h *= 1.002;
+            if (h == 0) h = 0.2;
+        }
+        lohi[2 * blockIdx.x] = lo;
+        lohi[2 * blockIdx.x + 1] = h;
+    }
+}
+
+// Map every value with its image's (lo, hi): linear (v - lo)/(hi - lo), or for the log modes
+// (log10 v - lo)/(hi - lo) for v > 0, else 0.  Images along blockIdx.y.
+__global__ void k_map(double* __restrict__ imgs, int64_t npx, int lin, const double* __restrict__ lohi) {
+    double* img = imgs + (int64_t)blockIdx.y * npx;
+    const double lo = lohi[2 * blockIdx.y], hi = lohi[2 * blockIdx.y + 1];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npx; i += (int64_t)gridDim.x * blockDim.x) {
+        const double p = img[i];
+        img[i] = lin ? (p - lo) / (hi - lo) : ((p > 0) ? (log10(p) - lo) / (hi - lo) : 0);
+    }
+}
+
+// Per-image maximum with start value 0 (scaleAndClipData, scale).
+__global__ void __launch_bounds__(kStat) k_max0(const double* __restrict__ imgs, int64_t npx, double* __restrict__ mx) {
+    __shared__ double s[kStat];
+    const double* img = imgs + (int64_t)blockIdx.x * npx;
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < npx; i += kStat) if (m < img[i]) m = img[i];
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = kStat / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off && s[threadIdx.x + off] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mx[blockIdx.x] = s[0];
+}
+
+// scaleAndClipData's elementwise steps after its statistics: divide by the maximum (scale),
+// the [1e-6, 1] log clip, the 40 % clip.
+__global__ void k_scale_clip(double* __restrict__ imgs, int64_t npx, const double* __restrict__ mx, int clip,
+                             int clip40) {
+    double* img = imgs + (int64_t)blockIdx.y * npx;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npx; i += (int64_t)gridDim.x * blockDim.x) {
+        double p = img[i];
+        if (mx) p = p / mx[blockIdx.y];
+        if (clip) {
+            p = ((log(fabs(p) / 1e-20) / log(10.0)) + 6.0) / 6.0;
+            if (p < 0) p = 0;
+        }
+        if (clip40) {
+            p = 1.0 - (1.0 - p) / (1.0 - 0.4);
+            if (p < 0) p = 0;
+        }
+        img[i] = p;
+    }
+}
+
+// Noise d + z sqrt(d) (z/100 for images already scaled to [0, 1]); z has one value per pixel of
+// every image, in the order the reference draws them.
+__global__ void k_noise(double* __restrict__ img, const double* __restrict__ z, int64_t n, int scaled) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double d = img[i];
+        img[i] = d + (scaled ? z[i] / 100.0 : z[i]) * sqrt(d);
+    }
+}
+
+__global__ void k_zero(double* __restrict__ img, const int64_t* __restrict__ idx, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) img[idx[i]] = 0.0;
+}
+
+// r,phi cell: its writer (step k of the float radius loop, angle row i) or -1.
+struct RphiArgs {
+    int sx, sy, s;
+    double xc, yc, phiber;
+    int64_t npx;
+};
+
+__global__ void k_rphi(const double* __restrict__ imgs, RphiArgs a, const int* __restrict__ writer,
+                       const float* __restrict__ irs, double* __restrict__ outs) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.s * a.s) return;
+    const double* d = imgs + (int64_t)blockIdx.y * a.npx;
+    double* out = outs + (int64_t)blockIdx.y * a.s * a.s;
+    const int wr = writer[cell];
+    if (wr < 0) { out[cell] = 0.0; return; }
+    const int k = wr / (a.s + 1), i = wr % (a.s + 1);
+    const double rad = irs[k];
+    double twt = 0.0, avg = 0.0;
+    int umf = 1024 - (int)(1024. - a.phiber * rad);
+    if (umf < 1) umf = 1;
+    for (int j = 1; j <= umf; ++j) {
+        const double phi = (i - 1.5) * a.phiber + (j - 0.5) * a.phiber / umf;
+        const double xs = a.xc + rad * cos(phi), ys = a.yc + rad * sin(phi);
+        const int ix1 = (int)xs, iy1 = (int)ys, ix2 = ix1 + 1, iy2 = iy1 + 1;
+        const double rex = xs - ix1, rey = ys - iy1;
+        const bool x1in = ix1 >= 0 && ix1 < a.sx, x2in = ix2 >= 0 && ix2 < a.sx;
+        const bool y1in = iy1 >= 0 && iy1 < a.sy, y2in = iy2 >= 0 && iy2 < a.sy;
+        if (x1in && y1in) { const double wt = (1.0 - rex) * (1.0 - rey); twt = twt + wt; avg = avg + wt * d[ix1 + (int64_t)a.sx * iy1]; }
+        if (x2in && y1in) { const double wt = (rex) * (1.0 - rey); twt = twt + wt; avg = avg + wt * d[ix2 + (int64_t)a.sx * iy1]; }
+        if (x1in && y2in) { const double wt = (1.0 - rex) * (rey); twt = twt + wt; avg = avg + wt * d[ix1 + (int64_t)a.sx * iy2]; }
+        if (x2in && y2in) { const double wt = (rex) * (rey); twt = twt + wt; avg = avg + wt * d[ix2 + (int64_t)a.sx * iy2]; }
+    }
+    if (twt > 0.0) avg = avg / twt;
+    out[cell] = (avg < 0) ? 0 : avg;
+}
+
+int blocks_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
+
+}  // namespace csp
+
+using namespace csp;
+
+#define PCHK(c, call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) return csx_ctx_fail(c, CSX_E_HIP, (std::string(#call) + ": " + hipGetErrorString(e_)).c_str()); \
+    } while (0)
+
+namespace {
+
+int stats_and_map(csx_ctx* c, double* d, int64_t npx, int nimg, const StatArgs& a, int lin) {
+    hipStream_t st = (hipStream_t)csx_ctx_stream(c);
+    double* lohi = (double*)csx_ctx_scratch(c, 0, (size_t)nimg * 2 * sizeof(double));
+    if (!lohi) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
+    hipLaunchKernelGGL(k_stats, dim3(nimg), dim3(kStat), 0, st, d, a, lohi);
+    PCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_map, dim3(blocks_for(npx), nimg), dim3(256), 0, st, d, npx, lin, lohi);
+    PCHK(c, hipGetLastError());
+    return CSX_OK;
+}
+
+int check(csx_ctx* c, const void* d, int64_t npx, int nimg) {
+    if (!c) return CSX_E_ARG;
+    if (!d || npx <= 0 || nimg <= 0 || nimg > 65535) return csx_ctx_fail(c, CSX_E_ARG, "bad image batch");
+    if (hipSetDevice(csx_ctx_device(c)) != hipSuccess) return csx_ctx_fail(c, CSX_E_HIP, "hipSetDevice");
+    return CSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csx_post_noise(csx_ctx* c, double* d_img, const double* d_z, int64_t n, int scaled) {
+    if (int rc = check(c, d_img, n, 1)) return rc;
+    if (!d_z) return csx_ctx_fail(c, CSX_E_ARG, "no noise values");
+    hipLaunchKernelGGL(k_noise, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)csx_ctx_stream(c), d_img, d_z, n, scaled);
+    PCHK(c, hipGetLastError());
+    return CSX_OK;
+}
+
+int csx_post_zero(csx_ctx* c, double* d_img, const int64_t* d_idx, int64_t n) {
+    if (!c) return CSX_E_ARG;
+    if (n == 0) return CSX_OK;
+    if (int rc = check(c, d_img, n, 1)) return rc;
+    if (!d_idx) return csx_ctx_fail(c, CSX_E_ARG, "no indices");
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)csx_ctx_stream(c), d_img,
+                       d_idx, n);
+    PCHK(c, hipGetLastError());
+    return CSX_OK;
+}
+
+int csx_post_scale_intensity(csx_ctx* c, double* d_img, int64_t npx, int nimg, int linlog) {
+    if (int rc = check(c, d_img, npx, nimg)) return rc;
+    StatArgs a{npx, linlog ? ST_LIN : ST_LOG, 1, 1, 0, 0};
+    return stats_and_map(c, d_img, npx, nimg, a, linlog);
+}
+
+int csx_post_log_scale(csx_ctx* c, double* d_img, int64_t npx, int nimg, int w, int xmin, int ymin, int skip_axes) {
+    if (int rc = check(c, d_img, npx, nimg)) return rc;
+    if (skip_axes && (w <= 0 || npx % w)) return csx_ctx_fail(c, CSX_E_ARG, "image width does not divide the pixel count");
+    StatArgs a{npx, skip_axes ? ST_GENLOG_AXES : ST_GENLOG, 0, w > 0 ? w : 1, xmin, ymin};
+    return stats_and_map(c, d_img, npx, nimg, a, 0);
+}
+
+int csx_post_scale_clip(csx_ctx* c, double* d_img, int64_t npx, int nimg, int scale, int clip, int clip40, int genlog) {
+    if (int rc = check(c, d_img, npx, nimg)) return rc;
+    hipStream_t st = (hipStream_t)csx_ctx_stream(c);
+    if (genlog) {
+        StatArgs a{npx, ST_GENLOG, 0, 1, 0, 0};
+        if (int rc = stats_and_map(c, d_img, npx, nimg, a, 0)) return rc;
+    }
+    if (!scale && !clip && !clip40) return CSX_OK;
+    double* mx = nullptr;
+    if (scale) {
+        mx = (double*)csx_ctx_scratch(c, 1, (size_t)nimg * sizeof(double));
+        if (!mx) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
+        hipLaunchKernelGGL(k_max0, dim3(nimg), dim3(kStat), 0, st, d_img, npx, mx);
+        PCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_scale_clip, dim3(blocks_for(npx), nimg), dim3(256), 0, st, d_img, npx, mx, clip, clip40);
+    PCHK(c, hipGetLastError());
+    return CSX_OK;
+}
+
+int csx_post_rphi(csx_ctx* c, const double* d_img, int x0, int x1, int y0, int y1, int xc, int yc, int s, int nimg,
+                  double* d_out, int scale, int clip, int clip40) {
+    const int64_t npx = (int64_t)(x1 - x0) * (y1 - y0);
+    if (int rc = check(c, d_img, npx, nimg)) return rc;
+    if (!d_out || s < 2 || s > 8192) return csx_ctx_fail(c, CSX_E_ARG, "r,phi size must be 2..8192");
+    // the float radius loop and the cell each (step, row) writes, as the reference computes
+    // them; a later write of a cell replaces an earlier one
+    if (x0 < 0) { x1 -= x0; xc -= x0; x0 = 0; }
+    if (y0 < 0) { y1 -= y0; yc -= y0; y0 = 0; }
+    const int sx = x1 - x0, sy = y1 - y0;
+    const double phiber = 2. * M_PI / s;
+    auto sq = [](double v) { return v * v; };
+    double rmax = std::sqrt(sq(1. - xc) + sq(1. - yc));
+    rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq(1. - yc)));
+    rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq((sy - 1.) - yc)));
+    rmax = std::max(rmax, std::sqrt(sq(1. - xc) + sq((sy - 1.) - yc)));
+    const int irmax = (int)(rmax + 0.5);
+    const float irstp = (float)irmax / (float)(s - 1);
+    std::vector<float> irs;
+    const int64_t cells = (int64_t)s * s;
+    std::vector<int> writer((size_t)cells, -1);
+    for (float ir = 0; ir <= irmax; ir += irstp) {
+        const int k = (int)irs.size();
+        irs.push_back(ir);
+        for (int i = 0; i <= s; ++i) {
+            const int idx = (int)((float)(i * s) + ir / irstp);
+            if (idx < cells && idx >= 0) writer[(size_t)idx] = k * (s + 1) + i;
+        }
+        if (irs.size() > (size_t)(4 * s + 16)) return csx_ctx_fail(c, CSX_E_ARG, "r,phi radius loop does not end");
+    }
+    const size_t wbytes = (size_t)cells * sizeof(int), ibytes = irs.size() * sizeof(float);
+    char* scr = (char*)csx_ctx_scratch(c, 0, wbytes + ibytes + 256);
+    if (!scr) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
+    int* d_writer = (int*)scr;
+    float* d_irs = (float*)(scr + ((wbytes + 255) & ~size_t(255)));
+    hipStream_t st = (hipStream_t)csx_ctx_stream(c);
+    PCHK(c, hipMemcpyAsync(d_writer, writer.data(), wbytes, hipMemcpyHostToDevice, st));
+    PCHK(c, hipMemcpyAsync(d_irs, irs.data(), ibytes, hipMemcpyHostToDevice, st));
+    RphiArgs a{sx, sy, s, (double)xc, (double)yc, phiber, npx};
+    hipLaunchKernelGGL(k_rphi, dim3((unsigned)((cells + 255) / 256), nimg), dim3(256), 0, st, d_img, a, d_writer, d_irs,
+                       d_out);
+    PCHK(c, hipGetLastError());
+    // the host vectors must outlive the copies
+    PCHK(c, hipStreamSynchronize(st));
+    return csx_post_scale_clip(c, d_out, cells, nimg, scale, clip, clip40, 0);
+}
+
+}  // extern "C"

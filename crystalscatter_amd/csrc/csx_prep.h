@@ -90,3 +90,11 @@ int csx_fit_batch_launch(csx_ctx* c, int precision);
 int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts);
 // A second context on the same device.
 int csx_create_like(const csx_ctx* c, csx_ctx** out);
+// The context's device and stream, and its error message (for the post-processing kernels
+// of csx_post.hip, which live outside csx_device.hip).
+int csx_ctx_device(const csx_ctx* c);
+void* csx_ctx_stream(const csx_ctx* c);
+int csx_ctx_fail(csx_ctx* c, int code, const char* msg);
+// Device scratch `which` (0 or 1) of the context, at least `bytes` long; work on the
+// context's stream is ordered, so its calls can reuse it.  Null on failure.
+void* csx_ctx_scratch(csx_ctx* c, int which, size_t bytes);

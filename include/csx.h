@@ -255,6 +255,31 @@ int csx_rand_zuf(csx_rand* r, double* h_out, int64_t n);
 int csx_engine_tpv(csx_engine* e, const char* ids, csx_rand* rng, char* report, int cap);
 double csx_engine_tpv_value(csx_engine* e, const char* key);
 
+/* ---- Post-processing of image batches in HBM (the AI-data loop after each calculation,
+ * Source/sc_mainCons.cpp:1254-1400) -------------------------------------------------------
+ * d_img holds nimg consecutive images of npx doubles; work is ordered on ctx's stream.
+ * The scans keep the reference's running rule (a value that is a new minimum is not tested
+ * against the maximum), exactly.
+ * noise: d += z*sqrt(d) (z/100 when scaled) for n values, d_z one zuf() value per pixel in the
+ *   reference's draw order (:1275-1290);
+ * zero: d[idx[i]] = 0 (beam stop and lines, :1298-1348, indices from the host);
+ * scale_intensity: scaleIntensity (Source/sc_calc_generic_cpu.cpp:11195-11236), linlog 1
+ *   linear, 0 log10; the engine's scan covers one zero past each image, so does this one;
+ * log_scale: the AI loop's log scaling of an image (:1366-1384; skip_axes: ignore the pixels
+ *   on x = 0 or y = 0 of an image whose x range xmin.. (width w) is below 0);
+ * scale_clip: scaleAndClipData (Source/sc_postproc.cpp:764-830);
+ * rphi: generateRphi (Source/sc_postproc.cpp:53-360) of every image (bounds x0..x1, y0..y1,
+ *   centre xc, yc) into d_out (nimg x s*s; cells the reference never writes are 0). */
+int csx_post_noise(csx_ctx* ctx, double* d_img, const double* d_z, int64_t n, int scaled);
+int csx_post_zero(csx_ctx* ctx, double* d_img, const int64_t* d_idx, int64_t n);
+int csx_post_scale_intensity(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int linlog);
+int csx_post_log_scale(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int w, int xmin, int ymin,
+                       int skip_axes);
+int csx_post_scale_clip(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int scale, int clip,
+                        int clip40, int genlog);
+int csx_post_rphi(csx_ctx* ctx, const double* d_img, int x0, int x1, int y0, int y1, int xc, int yc,
+                  int s, int nimg, double* d_out, int scale, int clip, int clip40);
+
 /* ---- Surrogate network (no counterpart in the reference) --------------------------------
  * The learned part of the fits: an MLP from a normalised parameter set (CSX_MLP_NPAR values)
  * and a pixel's (qx, qy) to the normalised log-intensity of that pixel.  Inputs: the

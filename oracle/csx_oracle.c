@@ -1634,3 +1634,118 @@ int csxo_simplex(csxo_objective f, void* user, int numfit, int mmax, double* x, 
     if (nevals_out) *nevals_out = c.nev;
     return 0;
 }
+
+
+/* ========================================================================== */
+/* AI-data post-processing                                                     */
+/* ========================================================================== */
+
+void csxo_scale_intensity(double* p, long n, int linlog)
+{
+    double vmin = 1e20, vmax = 1e-20;
+    if (linlog) {
+        for (long i = 0; i < n; ++i)
+            if (p[i] < vmin) vmin = p[i];
+            else if (p[i] > vmax) vmax = p[i];
+    } else {
+        for (long i = 0; i < n; ++i)
+            if (p[i] > 0) {
+                if (log10(p[i]) < vmin) vmin = log10(p[i]);
+                else if (log10(p[i]) > vmax) vmax = log10(p[i]);
+            }
+    }
+    if (fabs(vmin - vmax) < 1e-6) {
+        vmin *= 0.998;
+        vmax *= 1.002;
+        if (vmax == 0) vmax = 0.2;
+    }
+    for (long i = 0; i < n; ++i) {
+        if (linlog) p[i] = (p[i] - vmin) / (vmax - vmin);
+        else p[i] = (p[i] > 0) ? (log10(p[i]) - vmin) / (vmax - vmin) : 0;
+    }
+}
+
+void csxo_scale_clip(double* data, long len, int scale, int clip, int clip40, int genlog)
+{
+    if (genlog) {
+        double lo = 1e26, hi = 0;
+        for (long i = 0; i < len; ++i)
+            if (data[i] > 0) {
+                if (log10(data[i]) < lo) lo = log10(data[i]);
+                else if (log10(data[i]) > hi) hi = log10(data[i]);
+            }
+        for (long i = 0; i < len; ++i)
+            data[i] = (data[i] > 0) ? (log10(data[i]) - lo) / (hi - lo) : 0;
+    }
+    if (scale) {
+        double mx = 0;
+        for (long i = 0; i < len; ++i)
+            if (mx < data[i]) mx = data[i];
+        for (long i = 0; i < len; ++i) data[i] = data[i] / mx;
+    }
+    if (clip)
+        for (long i = 0; i < len; ++i) {
+            data[i] = ((log(fabs(data[i]) / 1e-20) / log(10)) + 6.0) / 6.0;
+            if (data[i] < 0) data[i] = 0;
+        }
+    if (clip40)
+        for (long i = 0; i < len; ++i) {
+            data[i] = 1.0 - (1.0 - data[i]) / (1.0 - 0.4);
+            if (data[i] < 0) data[i] = 0;
+        }
+}
+
+static double dmax(double a, double b) { return (a < b) ? b : a; } /* qMax */
+
+void csxo_rphi(const double* d, int x0, int x1, int y0, int y1, int xc, int yc, int s, double* out,
+               int scale, int clip, int clip40)
+{
+    if (x0 < 0) { x1 -= x0; xc -= x0; x0 = 0; }
+    if (y0 < 0) { y1 -= y0; yc -= y0; y0 = 0; }
+    const int sx = x1 - x0, sy = y1 - y0;
+    const long len = (long)sx * sy, arrlen = (long)s * s;
+    const int phisteps = s;
+    const double phiber = 2. * M_PI / phisteps;
+    double rmax = sqrt(sq(1. - xc) + sq(1. - yc));
+    rmax = dmax(rmax, sqrt(sq((sx - 1.) - xc) + sq(1. - yc)));
+    rmax = dmax(rmax, sqrt(sq((sx - 1.) - xc) + sq((sy - 1.) - yc)));
+    rmax = dmax(rmax, sqrt(sq(1. - xc) + sq((sy - 1.) - yc)));
+    const int irmax = (int)(rmax + 0.5);
+    const float irstp = (float)irmax / (float)(s - 1);
+    for (float ir = 0; ir <= irmax; ir += irstp) {
+        const double rad = ir;
+        for (int i = 0; i <= phisteps; ++i) {
+            double twt = 0.0, avg = 0.0;
+            int umf = 1024 - (int)(1024. - phiber * rad);
+            if (umf < 1) umf = 1;
+            for (int j = 1; j <= umf; ++j) {
+                const double phi = (i - 1.5) * phiber + (j - 0.5) * phiber / umf;
+                const double xs = xc + rad * cos(phi), ys = yc + rad * sin(phi);
+                const int ix1 = (int)xs, iy1 = (int)ys, ix2 = ix1 + 1, iy2 = iy1 + 1;
+                const double rex = xs - ix1, rey = ys - iy1;
+                /* the four corners in the reference's order; data(x,y) = d[x + sx*y] */
+                if (ix1 >= 0 && ix1 < sx && iy1 >= 0 && iy1 < sy) {
+                    const double wt = 1 * (1.0 - rex) * (1.0 - rey);
+                    twt = twt + wt; avg = avg + wt * d[ix1 + (long)sx * iy1];
+                }
+                if (ix2 >= 0 && ix2 < sx && iy1 >= 0 && iy1 < sy) {
+                    const double wt = 1 * (rex) * (1.0 - rey);
+                    twt = twt + wt; avg = avg + wt * d[ix2 + (long)sx * iy1];
+                }
+                if (ix1 >= 0 && ix1 < sx && iy2 >= 0 && iy2 < sy) {
+                    const double wt = 1 * (1.0 - rex) * (rey);
+                    twt = twt + wt; avg = avg + wt * d[ix1 + (long)sx * iy2];
+                }
+                if (ix2 >= 0 && ix2 < sx && iy2 >= 0 && iy2 < sy) {
+                    const double wt = 1 * (rex) * (rey);
+                    twt = twt + wt; avg = avg + wt * d[ix2 + (long)sx * iy2];
+                }
+            }
+            if (twt > 0.0) avg = avg / twt;
+            const int idx = (int)((float)(i * s) + ir / irstp);
+            if (idx < arrlen) out[idx] = (avg < 0) ? 0 : avg;
+        }
+    }
+    (void)len;
+    csxo_scale_clip(out, arrlen, scale, clip, clip40, 0);
+}

@@ -72,6 +72,19 @@ int csxo_simplex(csxo_objective f, void* user, int numfit, int mmax, double* x,
                  const double* minv, const double* maxv, double stp, int maxit, double ftol,
                  int* iters_out, double* fbest_out, long* nevals_out);
 
+/* Post-processing of the AI-data loop.
+ * scaleIntensity (Source/sc_calc_generic_cpu.cpp:11195-11236): p holds n values (the engine
+ * scans one element past the image); linlog 1 linear, 0 log10, to [0,1] with its running
+ * min/else-if-max rule.
+ * scaleAndClipData (Source/sc_postproc.cpp:764-830) with its four flags.
+ * generateRphi (Source/sc_postproc.cpp:53-360): r,phi resampling of the image with bounds
+ * x0..x1, y0..y1 around (xc, yc) into out (s*s values, cells nothing writes are left as they
+ * are), then scaleAndClipData(scale, clip, clip40, no log). */
+void csxo_scale_intensity(double* p, long n, int linlog);
+void csxo_scale_clip(double* data, long len, int scale, int clip, int clip40, int genlog);
+void csxo_rphi(const double* d, int x0, int x1, int y0, int y1, int xc, int yc, int s, double* out,
+               int scale, int clip, int clip40);
+
 #ifdef __cplusplus
 }
 #endif

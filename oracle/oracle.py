@@ -21,6 +21,10 @@ def lib():
         L = ctypes.CDLL(path)
         vp, cp, d, i = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int
         L.csxo_new.restype = vp
+        pd = ctypes.POINTER(d)
+        L.csxo_scale_intensity.argtypes = [pd, ctypes.c_long, i]
+        L.csxo_scale_clip.argtypes = [pd, ctypes.c_long, i, i, i, i]
+        L.csxo_rphi.argtypes = [pd, i, i, i, i, i, i, i, pd, i, i, i]
         L.csxo_free.argtypes = [vp]
         L.csxo_set.argtypes = [vp, cp, d]
         L.csxo_set.restype = i
@@ -176,3 +180,29 @@ def simplex(f, x0, numfit, minv, maxv, stp=0.1, maxit=200, ftol=1e-6):
     if rc != 0:
         raise RuntimeError("simplex failed")
     return x, fb.value, it.value, nev.value
+
+
+def _pd(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def scale_intensity(img, linlog, extra=True):
+    """scaleIntensity of an image (a copy); extra: the engine also scans one zero past it."""
+    a = np.ascontiguousarray(np.append(img, 0.0) if extra else img, dtype=np.float64).copy()
+    lib().csxo_scale_intensity(_pd(a), a.size, int(linlog))
+    return a[:len(img)]
+
+
+def scale_clip(data, scale=False, clip=False, clip40=False, genlog=False):
+    a = np.ascontiguousarray(data, dtype=np.float64).copy()
+    lib().csxo_scale_clip(_pd(a), a.size, int(scale), int(clip), int(clip40), int(genlog))
+    return a
+
+
+def rphi(img, bounds, xc, yc, s, scale=False, clip=False, clip40=False):
+    """generateRphi of img (IDX order over bounds x0,x1,y0,y1); cells never written stay 0."""
+    a = np.ascontiguousarray(img, dtype=np.float64)
+    out = np.zeros(s * s)
+    x0, x1, y0, y1 = (int(b) for b in bounds)
+    lib().csxo_rphi(_pd(a), x0, x1, y0, y1, int(xc), int(yc), int(s), _pd(out), int(scale), int(clip), int(clip40))
+    return out

@@ -20,10 +20,14 @@
 //     nofit<i>=x0,y0,x1,y1 no-fit rectangles (fit mode)
 //     ignorecbs=0|1        bIgnoreCbs for doCalculation (default 1)
 //     repeat=<n>           repeat the calculation n times (timing)
+//     rphi=<s> bsx= bsy=   r,phi transform of the image (s x s, centre bsx,bsy) and its log
+//                          scaling; rphiscale/rphiclip/rphiclip40 = flags
+//     scaleint=0|1         scaleIntensity of the image (1 linear, 0 log)
 //     tpv=<k=f|k=f|...>    tpvPerformRandom(list) before each of tpvn=<n> images
 //                          (srand=<s> seeds rand() first; default: the process's seed 1)
 
 #include "sc_calc_generic_gpu.h"
+#include "sc_postproc.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -177,6 +181,23 @@ int main(int argc, char** argv) {
         }
         const long long n = (long long)(c->maxX() - c->minX()) * (c->maxY() - c->minY());
         w.d("image", c->data(), n);
+        if (g_kv.count("rphi")) {
+            // generateRphi (Source/sc_postproc.cpp:53-360) of the image, then the log scaling
+            // the AI-data loop applies to it (Source/sc_mainCons.cpp:1385-1395)
+            const int rs = (int)num("rphi", 64);
+            double* rp = SasCalc_PostProc::inst()->generateRphi(c->minX(), c->maxX(), c->minY(), c->maxY(),
+                                                               (int)num("bsx", 0), (int)num("bsy", 0), rs, c->data(),
+                                                               num("rphiscale", 0) != 0, num("rphiclip", 0) != 0,
+                                                               num("rphiclip40", 0) != 0);
+            w.d("rphi", rp, (long long)rs * rs);
+            SasCalc_PostProc::inst()->scaleAndClipData(rp, rs * rs, false, false, false, true);
+            w.d("rphi_log", rp, (long long)rs * rs);
+        }
+        if (g_kv.count("scaleint")) {
+            // scaleIntensity (Source/sc_calc_generic_cpu.cpp:11195-11236): 1 linear, 0 log
+            c->scaleIntensity(num("scaleint", 0) != 0);
+            w.d("scaled", c->data(), n);
+        }
         double b[4] = {(double)c->minX(), (double)c->maxX(), (double)c->minY(), (double)c->maxY()};
         w.d("bounds", b, 4);
         std::string err = c->getLastErrorMessage();

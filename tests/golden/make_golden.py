@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv                          # only these
+    python tests/golden/make_golden.py cube tpv post                     # only these
 """
 import json
 import os
@@ -116,6 +116,29 @@ def write_tpv(manifest):
     manifest["fixtures"]["tpv_fcc.npz"] = {"params": p, "tpv": TPV_IDS, "n": TPV_N, "seed": 1}
 
 
+# Post-processing: r,phi resampling (with its log scaling) and scaleIntensity of FCC and
+# sphere images, from the reference's own routines.
+POST_CASES = [("fcc_rphi", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 0, "bsy": 0, "scaleint": 0}),
+              ("fcc_rphi_off", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 10, "bsy": -5, "rphiscale": 1}),
+              ("fcc_rphi_clip", dict(CFG3, GridPoints=24), {"rphi": 128, "bsx": 3, "bsy": 4, "rphiclip": 1,
+                                                             "rphiclip40": 1}),
+              ("sphere_lin", dict(CFG1, GridPoints=32), {"scaleint": 1})]
+
+
+def write_post(manifest):
+    arrays, cases = {}, {}
+    for name, p, extra in POST_CASES:
+        r = run_ref(p, extra)
+        arrays[name + "_image"] = r["image"]
+        arrays[name + "_bounds"] = r["bounds"]
+        for k in ("rphi", "rphi_log", "scaled"):
+            if k in r:
+                arrays["%s_%s" % (name, k)] = r[k]
+        cases[name] = {"params": p, "post": extra}
+    np.savez_compressed(os.path.join(HERE, "post.npz"), **arrays)
+    manifest["fixtures"]["post.npz"] = {"cases": cases}
+
+
 def main():
     if not os.path.exists(REF_DRIVER):
         sys.exit("build the reference first: make -C oracle ref")
@@ -127,6 +150,8 @@ def main():
             write_cube(manifest)
         if "tpv" in only:
             write_tpv(manifest)
+        if "post" in only:
+            write_post(manifest)
         with open(os.path.join(HERE, "manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
         return
@@ -202,6 +227,7 @@ def main():
 
     write_cube(manifest)
     write_tpv(manifest)
+    write_post(manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))

@@ -55,3 +55,80 @@ def test_tpv_batch_matches_the_reference_run(prec, tol):
         r = rel_err(imgs[k], f["images"][k])
         print("tpv %s image %d: max rel %.3e" % (prec, k, np.nanmax(r)), flush=True)
         assert np.nanmax(r) < tol
+
+
+def _post_case(name):
+    f = fixture("post.npz")
+    c = manifest()["fixtures"]["post.npz"]["cases"][name]
+    return f, c["post"], f[name + "_image"], f[name + "_bounds"]
+
+
+@pytest.mark.parametrize("name", ["fcc_rphi", "fcc_rphi_off", "fcc_rphi_clip"])
+def test_rphi_matches_the_reference(oracle_lib, name):
+    import torch
+    from crystalscatter_amd import aidata
+    f, post, img, b = _post_case(name)
+    ctx = Context(0)
+    imgs = torch.from_numpy(np.stack([img, img * 2.0])).cuda()
+    r = aidata.post_rphi(ctx, imgs, b, (post["bsx"], post["bsy"]), post["rphi"], post.get("rphiscale", 0),
+                         post.get("rphiclip", 0), post.get("rphiclip40", 0))
+    got = r[0].cpu().numpy()
+    err = np.abs(got - f[name + "_rphi"]).max() / np.abs(f[name + "_rphi"]).max()
+    print("%s rphi: max err %.3e (of the maximum)" % (name, err), flush=True)
+    # device cos/sin within an ulp of glibc move the bilinear sample points by ~1e-14 pixel
+    assert err < 1e-12
+    aidata.post_scale_clip(ctx, r, genlog=True)
+    lg = r[0].cpu().numpy()
+    assert np.abs(lg - f[name + "_rphi_log"]).max() < 1e-12
+    # the second image of the batch against the oracle (the log scaling's start value 0 for
+    # the maximum makes it depend on the overall scale)
+    o = oracle_lib.rphi(img * 2.0, b, post["bsx"], post["bsy"], post["rphi"], post.get("rphiscale", 0),
+                        post.get("rphiclip", 0), post.get("rphiclip40", 0))
+    assert np.abs(r[1].cpu().numpy() - oracle_lib.scale_clip(o, genlog=True)).max() < 1e-12
+
+
+@pytest.mark.parametrize("name,lin", [("fcc_rphi", 0), ("sphere_lin", 1)])
+def test_scale_intensity_matches_the_reference(name, lin):
+    import torch
+    from crystalscatter_amd import aidata
+    f, post, img, b = _post_case(name)
+    t = torch.from_numpy(np.stack([img, img])).cuda()
+    aidata.post_scale_intensity(Context(0), t, lin)
+    for row in t.cpu().numpy():
+        assert np.abs(row - f[name + "_scaled"]).max() < 1e-14
+
+
+@pytest.mark.parametrize("opts_kw", [dict(noise=True, bs=(0.2, 0.2, 0.5, 0.5), lines_x=2, lines_y=1, rphi_size=128),
+                                     dict(scale_scat=True, noise=True, rphi_size=128, fft_input_lin=True,
+                                          rphi_scale=True)])
+def test_ai_pipeline_matches_the_oracle(oracle_lib, opts_kw):
+    """The batched pipeline against the per-image CPU restatement, from the same rand() state."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import aidata_oracle as AO
+    from crystalscatter_amd import aidata
+    p = dict(CFG3, GridPoints=32, BeamPosX=3, BeamPosY=-4)
+    ids = ["EditRadius=0.1", "uca=0.05", "ucb=1", "ucc=1", "I0=0.3", "EditDomainSize=0.4"]
+    opts = aidata.AIOptions(**opts_kw)
+    e = SasCalc(device=-1, params=p)
+    e.set("ignore_cbs", 1)
+    out = aidata.generate(Context(0), e, ids, 5, Random(11), opts, "ref64")
+    scat = out["scat"].cpu().numpy()
+    rphi = out["rphi"].cpu().numpy()
+    # CPU: the same sequence image by image
+    e2 = SasCalc(device=-1, params=p)
+    e2.set("ignore_cbs", 1)
+    rng = Random(11)
+    for k in range(5):
+        e2.tpvPerformRandom(ids, rng)
+        q = dict(p)
+        q.update({key: e2.tpv_value(key) for key in ["EditRadius", "uca", "ucb", "ucc", "I0", "width_zuf"]})
+        o = oracle_lib.OracleEngine(q)
+        o.prepare()
+        img, centre, rp = AO.post_image(o.image(nthreads=8), o.bounds(), (3, -4), rng, opts)
+        assert tuple(out["centres"][k]) == centre
+        ok = np.isfinite(img)
+        assert np.array_equal(np.isfinite(scat[k]), ok) and np.array_equal(scat[k] == 0, img == 0)
+        assert rel_err(scat[k][ok], img[ok]).max() < 1e-12, k
+        assert np.abs(rphi[k] - rp).max() < 1e-10, k
