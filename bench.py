@@ -328,6 +328,90 @@ def bench_bct(args, nsets=8):
     print(json.dumps(line), flush=True)
 
 
+AI_IDS = ["EditRadius=0.1", "EditSigma=0.2", "uca=0.05", "ucb=1", "ucc=1", "I0=0.3", "EditDomainSize=0.4",
+          "EditDebyeWaller=0.5"]
+
+
+def bench_aidata(args):
+    """Training images of the console's AI-data loop for one parameter line: FCC spheres at
+    128x128 with random variation of eight parameters (tpvPerformRandom), noise, a random beam
+    stop and the 128x128 r,phi image of each, rand() from seed 1.  Timed end to end (host
+    random variation and staging, one batched render, the post-processing on the device), and
+    against one render launch per image of the same parameter sets."""
+    import ctypes
+    import torch
+    from crystalscatter_amd import Context, Random, SasCalc, _lib, aidata
+    L = _lib.lib()
+    n = args.nimages
+    p = dict(CFG3, GridPoints=64, BeamPosX=0, BeamPosY=0)
+    opts = aidata.AIOptions(noise=True, bs=(0.1, 0.1, 0.5, 0.5), rphi_size=128)
+    ctx = Context(0)
+    e = SasCalc(device=-1, params=p)
+    e.set("ignore_cbs", 1)
+    aidata.generate(ctx, e, AI_IDS, 16, Random(1), opts, args.precision)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = aidata.generate(ctx, e, AI_IDS, n, Random(1), opts, args.precision)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # the batched render alone, and one render launch per image, on the same parameter sets
+    imgs = torch.empty((n, 128 * 128), dtype=torch.float64, device="cuda")
+    from crystalscatter_amd import tpv_images
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tpv_images(ctx, e, AI_IDS, n, Random(1), args.precision, out=imgs)
+    t_batch = time.perf_counter() - t1
+    k_batch_ms = L.csx_last_kernel_ms(ctx._h)
+    one = SasCalc(device=0, params=p)
+    one.set("ignore_cbs", 1)
+    rng = Random(1)
+    buf = torch.empty(128 * 128, dtype=torch.float64, device="cuda")
+    one.doCalculationDevice(buf.data_ptr(), args.precision)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for k in range(n):
+        one.tpvPerformRandom(AI_IDS, rng)
+        one.doCalculationDevice(buf.data_ptr(), args.precision)
+    L.csx_synchronize(L.csx_engine_ctx(one._h))
+    t_seq = time.perf_counter() - t2
+    ref_last = imgs[n - 1]
+    single_vs_batch = float(((buf - ref_last).abs() / ref_last.abs().clamp_min(1e-300)).max())
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # checker only: the CPU baseline leg
+        import aidata_oracle as AO
+        m = 32
+        e2 = SasCalc(device=-1, params=p)
+        e2.set("ignore_cbs", 1)
+        rng2 = Random(1)
+        keys = ["EditRadius", "EditSigma", "uca", "ucb", "ucc", "I0", "width_zuf", "dw_zuf"]
+        tc = time.perf_counter()
+        for k in range(m):
+            e2.tpvPerformRandom(AI_IDS, rng2)
+            q = dict(p)
+            q.update({key: e2.tpv_value(key) for key in keys})
+            o = oracle.OracleEngine(q)
+            o.prepare()
+            AO.post_image(o.image(nthreads=args.cpu_threads), o.bounds(), (0, 0), rng2, opts)
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": m / tcpu, "unit": "images/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": "%d images of the same workload: FP64 oracle image (%d threads) and the CPU "
+                         "post-processing restatement, %.2f s" % (m, args.cpu_threads, tcpu)}
+    line = {
+        "metric": "AI training images/s (random variation + 128^2 FCC render + noise + beam stop + r,phi)",
+        "value": n / dt, "unit": "images/s", "n_gpus": 1, "steps": n, "warmup": 16,
+        "ms_per_step": 1e3 * dt / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "%d images, FCC spheres (key 0x00020500) 128x128, tpv %s, noise, beam stop, "
+                               "r,phi 128x128" % (n, ";".join(AI_IDS))},
+        "render_batch_images_per_s": n / t_batch, "render_batch_kernel_ms": k_batch_ms,
+        "one_launch_per_image_images_per_s": n / t_seq, "single_vs_batched_max_rel": single_vs_batch,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(threads):
     """FP64 oracle (bit-exact restatement of the reference CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -350,11 +434,12 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "ref64"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("CSX_BENCH_CPU_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits", "surrogate", "bct"],
+    ap.add_argument("--workload", default="fcc", choices=["fcc", "fits", "surrogate", "bct", "aidata"],
                     help="fcc: the 1024^2 FCC pattern benchmark (default); fits: batched 2D fits; "
                          "surrogate: train the surrogate network, then fits with and without it; "
                          "bct: the BCT-cylinder kernel on random fit parameter sets")
     ap.add_argument("--nfits", type=int, default=512)
+    ap.add_argument("--nimages", type=int, default=1024, help="images of the aidata workload")
     ap.add_argument("--maxit", type=int, default=100)
     ap.add_argument("--maxit-surrogate", type=int, default=0,
                     help="exact simplex iterations after a surrogate start (0: the best re-scored candidate)")
@@ -379,6 +464,8 @@ def main():
     if args.workload == "surrogate":
         bench_surrogate(args)
         return
+    if args.workload == "aidata":
+        return bench_aidata(args)
     if args.workload == "bct":
         bench_bct(args)
         return
