@@ -212,6 +212,11 @@ def bench_surrogate(args):
     res_p, st_p = nm_batch(ctx, fits, FIT_KEYS, stp=0.1, maxit=args.maxit, ftol=1e-6, precision=args.precision,
                            threads=args.cpu_threads)
     t_plain = time.perf_counter() - t0
+    # warm-up of the proposal path (its torch kernels load on first use), on two fits
+    wrng = np.random.default_rng(5)
+    wfpx = [sg.FitPixels(p["engine"], p["data"], p["scale"], args.surrogate_px, wrng) for p in probs[:2]]
+    sg.propose_starts(net, wfpx, data.mu, data.sigma, [centre] * 2, wrng, top=args.rescore)
+    torch.cuda.synchronize()
     rng = np.random.default_rng(99)
     t0 = time.perf_counter()
     fpx = [sg.FitPixels(p["engine"], p["data"], p["scale"], args.surrogate_px, rng) for p in probs]

@@ -233,37 +233,47 @@ def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, 
     with a positive, finite intensity.  Targets are (max(log10 I, log10 I_max - FLOOR_DECADES)
     - mu) / sigma; mu and sigma come from the data unless given."""
     import torch
-    from . import SasCalc
+    from . import Context, SasCalc, _lib, _prec
     rng = np.random.default_rng(seed)
     xs = draw_parameters(rng, npatterns)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     sets, qs, ts, rs = [], [], [], []
-    buf = None
-    e = SasCalc(device=device, params=params_dict(xs[0], grid_points))
+    # one host-only engine: each pattern's tables are staged into a batch, and every chunk of
+    # patterns is rendered in one launch (the images equal one render per pattern)
+    e = SasCalc(device=-1, params=params_dict(xs[0], grid_points))
     e.set("ignore_cbs", 1)
-    for k in range(npatterns):
-        e.set_params(params_dict(xs[k], grid_points))
-        e.prepare()
-        x0, x1, y0, y1 = e.bounds()
-        npx = (x1 - x0) * (y1 - y0)
-        if buf is None or buf.numel() < npx:
-            buf = torch.empty(npx, dtype=torch.float64, device="cuda")
-        e.doCalculationDevice(buf.data_ptr(), "mixed")
-        e._L.csx_synchronize(e._L.csx_engine_ctx(e._h))
-        img = buf[:npx]
-        top = torch.log10(torch.nan_to_num(img, nan=0.0, posinf=0.0).max())
-        idx = torch.randint(0, npx, (2 * px_per_pattern,), device="cuda", generator=gen)
-        v = img[idx]
-        ok = torch.isfinite(v) & (v > 0)
-        idx, v = idx[ok][:px_per_pattern], v[ok][:px_per_pattern]
-        qx, qy = pixel_q(e, idx.to(torch.float64))
-        sets.append(torch.full((idx.numel(),), k, dtype=torch.int32, device="cuda"))
-        qs.append(torch.stack([qx, qy], dim=1).to(torch.float32))
-        lv = torch.log10(v)
-        ts.append(torch.maximum(lv, top - FLOOR_DECADES))
-        rs.append(lv >= top - EVAL_DECADES)
+    e.prepare()
+    x0, x1, y0, y1 = e.bounds()
+    npx = (x1 - x0) * (y1 - y0)
+    chunk = max(1, min(npatterns, (1 << 27) // (8 * npx)))
+    buf = torch.empty((chunk, npx), dtype=torch.float64, device="cuda")
+    ctx = Context(device)
+    L = _lib.lib()
+    for k0 in range(0, npatterns, chunk):
+        kn = min(chunk, npatterns - k0)
+        ctx._chk(L.csx_fit_batch_begin(ctx._h), "csx_fit_batch_begin")
+        for j in range(kn):
+            e.set_params(params_dict(xs[k0 + j], grid_points))
+            e.stage_image(ctx, buf[j].data_ptr(), npx)
+        ctx._chk(L.csx_render_batch_run(ctx._h, _prec("mixed")), "csx_render_batch_run")
+        for j in range(kn):
+            k = k0 + j
+            img = buf[j]
+            top = torch.log10(torch.nan_to_num(img, nan=0.0, posinf=0.0).max())
+            idx = torch.randint(0, npx, (2 * px_per_pattern,), device="cuda", generator=gen)
+            v = img[idx]
+            ok = torch.isfinite(v) & (v > 0)
+            idx, v = idx[ok][:px_per_pattern], v[ok][:px_per_pattern]
+            qx, qy = pixel_q(e, idx.to(torch.float64))
+            sets.append(torch.full((idx.numel(),), k, dtype=torch.int32, device="cuda"))
+            qs.append(torch.stack([qx, qy], dim=1).to(torch.float32))
+            lv = torch.log10(v)
+            ts.append(torch.maximum(lv, top - FLOOR_DECADES))
+            rs.append(lv >= top - EVAL_DECADES)
+        torch.cuda.current_stream().synchronize()  # the chunk buffer is reused by the next batch
     e.close()
+    ctx.close()
     t = torch.cat(ts)
     if mu is None:
         mu, sigma = float(t.mean()), float(t.std())
