@@ -340,7 +340,7 @@ AI_IDS = ["EditRadius=0.1", "EditSigma=0.2", "uca=0.05", "ucb=1", "ucc=1", "I0=0
 def bench_aidata(args):
     """Training images of the console's AI-data loop for one parameter line: FCC spheres at
     128x128 with random variation of eight parameters (tpvPerformRandom), noise, a random beam
-    stop and the 128x128 r,phi image of each, rand() from seed 1.  Timed end to end (host
+    stop, the 128x128 r,phi image of each and its FFT modulus image, rand() from seed 1.  Timed end to end (host
     random variation and staging, one batched render, the post-processing on the device), and
     against one render launch per image of the same parameter sets."""
     import ctypes
@@ -349,7 +349,8 @@ def bench_aidata(args):
     L = _lib.lib()
     n = args.nimages
     p = dict(CFG3, GridPoints=64, BeamPosX=0, BeamPosY=0)
-    opts = aidata.AIOptions(noise=True, bs=(0.1, 0.1, 0.5, 0.5), rphi_size=128)
+    opts = aidata.AIOptions(noise=True, bs=(0.1, 0.1, 0.5, 0.5), rphi_size=128, fft_size=128, fft_type=2,
+                            fft_swap=True, fft_scale=True)
     ctx = Context(0)
     e = SasCalc(device=-1, params=p)
     e.set("ignore_cbs", 1)
@@ -404,12 +405,12 @@ def bench_aidata(args):
                "sample": "%d images of the same workload: FP64 oracle image (%d threads) and the CPU "
                          "post-processing restatement, %.2f s" % (m, args.cpu_threads, tcpu)}
     line = {
-        "metric": "AI training images/s (random variation + 128^2 FCC render + noise + beam stop + r,phi)",
+        "metric": "AI training images/s (random variation + 128^2 FCC render + noise + beam stop + r,phi + FFT)",
         "value": n / dt, "unit": "images/s", "n_gpus": 1, "steps": n, "warmup": 16,
         "ms_per_step": 1e3 * dt / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.precision, "data": "synthetic",
         "config": {"workload": "%d images, FCC spheres (key 0x00020500) 128x128, tpv %s, noise, beam stop, "
-                               "r,phi 128x128" % (n, ";".join(AI_IDS))},
+                               "r,phi 128x128, FFT modulus 128x128" % (n, ";".join(AI_IDS))},
         "render_batch_images_per_s": n / t_batch, "render_batch_kernel_ms": k_batch_ms,
         "one_launch_per_image_images_per_s": n / t_seq, "single_vs_batched_max_rel": single_vs_batch,
         "cpu_baseline": cpu,

@@ -13,8 +13,11 @@ from the same rand() state.
 Options (names after the reference's command-line settings, Source/sc_mainCons.cpp:1000-1120):
 scale_scat (tpvScaleScat), noise (tpvAddNoise), bs = (x0, y0, xs, ys) factors (tpvBSx0..ys),
 lines_x / lines_x_width and lines_y / lines_y_width (tpvLinesX, ...), rphi_size (fftRphiSize)
-with rphi_scale / rphi_clip / rphi_clip40, and fft_input_lin (fftInputLin).  The FFT output
-and the PNG/file writing are not part of this path.
+with rphi_scale / rphi_clip / rphi_clip40, fft_input_lin (fftInputLin), and the FFT image of
+the r,phi image: fft_size (fftOutSize, equal to rphi_size: the reference's FFT without FFTW only
+transforms inputs of the output's size), fft_type (fftOutFormat), fft_scale / fft_clip /
+fft_clip40, fft_swap (fftOutputSwap), fft_cut (fftOutCutX/Y).  File writing is not part of
+this path.
 """
 import ctypes
 import math
@@ -39,6 +42,13 @@ class AIOptions:
     rphi_clip: bool = False
     rphi_clip40: bool = False
     fft_input_lin: bool = False
+    fft_size: int = 0          # fftOutSize: the FFT image of the r,phi image (same size)
+    fft_type: int = 0          # fftOutFormat: 0 real, 1 imaginary, 2 modulus, 3 re^2 - im^2
+    fft_scale: bool = False
+    fft_clip: bool = False
+    fft_clip40: bool = False
+    fft_swap: bool = False
+    fft_cut: tuple = (0, 0)    # fftOutCutX, fftOutCutY
 
 
 def _zuf(rng):
@@ -91,6 +101,8 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed"):
     "bounds", "centres" (the beam-stop centre per image) and "reports"."""
     import torch
     opts = opts or AIOptions()
+    if opts.fft_size and opts.fft_size != opts.rphi_size:
+        raise ValueError("the FFT image needs fft_size == rphi_size (the r,phi image is its input)")
     L = _lib.lib()
     engine.tpvPerformRandom([], None)
     engine.prepare()
@@ -154,6 +166,14 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed"):
                  "csx_post_scale_clip")
         out["rphi"] = rphi
         keep.append(src)
+        if opts.fft_size:
+            cx, cy = (int(v) for v in opts.fft_cut)
+            cut = 0 < cx < s and 0 < cy < s
+            fft = torch.empty((n, cx * cy if cut else s * s), dtype=torch.float64, device="cuda")
+            ctx._chk(L.csx_post_ifft(ctx._h, ctypes.c_void_p(rphi.data_ptr()), s, n, 0, int(opts.fft_type),
+                                     int(opts.fft_scale), int(opts.fft_clip), int(opts.fft_clip40),
+                                     int(opts.fft_swap), cx, cy, ctypes.c_void_p(fft.data_ptr())), "csx_post_ifft")
+            out["fft"] = fft
     ctx._chk(L.csx_synchronize(ctx._h), "csx_synchronize")
     return out
 
@@ -195,6 +215,23 @@ def post_rphi(ctx, imgs, bounds, centre, s, scale=False, clip=False, clip40=Fals
     ctx._chk(_lib.lib().csx_post_rphi(ctx._h, ctypes.c_void_p(imgs.data_ptr()), x0, x1, y0, y1, int(centre[0]),
                                       int(centre[1]), int(s), imgs.shape[0], ctypes.c_void_p(out.data_ptr()),
                                       int(scale), int(clip), int(clip40)), "csx_post_rphi")
+    ctx.synchronize()
+    return out
+
+
+def post_ifft(ctx, data, s, forward=False, out_type=0, scale=False, clip=False, clip40=False, swap=False, cut=(0, 0)):
+    """calculateIFFT of every row of data (s x s each); returns (nimg, s*s) or the cut-out."""
+    import torch
+    _chk_rows(data)
+    if data.shape[1] != s * s:
+        raise ValueError("rows must hold s*s values")
+    cx, cy = (int(v) for v in cut)
+    n = cx * cy if (0 < cx < s and 0 < cy < s) else s * s
+    out = torch.empty((data.shape[0], n), dtype=torch.float64, device="cuda")
+    _torch_done()
+    ctx._chk(_lib.lib().csx_post_ifft(ctx._h, ctypes.c_void_p(data.data_ptr()), int(s), data.shape[0], int(forward),
+                                      int(out_type), int(scale), int(clip), int(clip40), int(swap), cx, cy,
+                                      ctypes.c_void_p(out.data_ptr())), "csx_post_ifft")
     ctx.synchronize()
     return out
 

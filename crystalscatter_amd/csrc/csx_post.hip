@@ -198,6 +198,107 @@ __global__ void k_rphi(const double* __restrict__ imgs, RphiArgs a, const int* _
     out[cell] = (avg < 0) ? 0 : avg;
 }
 
+// ---- FFT image (calculateIFFT without FFTW, Source/sc_postproc.cpp:364-560, 833-1000) ----
+// One workgroup per line of the 2D transform (rows of one pass, columns of the other), the
+// line in LDS: bit-reversal permutation on load, then the butterfly levels.  The twiddles are
+// the reference's recurrence, evaluated on the host in the same order and passed in a table
+// (level l, l1 = 2^l entries from index l1 - 1), so every butterfly rounds as there.
+__global__ void k_fft_lines(double2* __restrict__ buf, int64_t img_stride, int64_t line_stride, int64_t elem_stride,
+                            int m, const double2* __restrict__ tw, int forward) {
+    extern __shared__ double lds[];
+    const int nn = 1 << m;
+    double* xr = lds;
+    double* xi = lds + nn;
+    double2* line = buf + (int64_t)blockIdx.y * img_stride + (int64_t)blockIdx.x * line_stride;
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+        const int r = (int)(__brev((unsigned)i) >> (32 - m));
+        const double2 v = line[(int64_t)r * elem_stride];
+        xr[i] = v.x;
+        xi[i] = v.y;
+    }
+    __syncthreads();
+    for (int l = 0; l < m; ++l) {
+        const int l1 = 1 << l, l2 = l1 << 1;
+        for (int b = threadIdx.x; b < nn / 2; b += blockDim.x) {
+            const int jj = b & (l1 - 1);
+            const int i = (b >> l) * l2 + jj, i1 = i + l1;
+            const double2 u = tw[l1 - 1 + jj];
+            const double t1 = u.x * xr[i1] - u.y * xi[i1];
+            const double t2 = u.x * xi[i1] + u.y * xr[i1];
+            xr[i1] = xr[i] - t1;
+            xi[i1] = xi[i] - t2;
+            xr[i] += t1;
+            xi[i] += t2;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+        double a = xr[i], b = xi[i];
+        if (forward) {
+            a /= (double)nn;
+            b /= (double)nn;
+        }
+        line[(int64_t)i * elem_stride] = make_double2(a, b);
+    }
+}
+
+// Input of the transform: c[y][x] = d[x + s y] / max, imaginary part 0.
+__global__ void k_fft_load(const double* __restrict__ d, int64_t len, const double* __restrict__ mx,
+                           double2* __restrict__ buf) {
+    const double* src = d + (int64_t)blockIdx.y * len;
+    double2* dst = buf + (int64_t)blockIdx.y * len;
+    const double m = mx[blockIdx.y];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = make_double2(src[i] / m, 0.0);
+}
+
+// Output cell (x, y) takes c[x][y] (the reference reads the array transposed) at
+// calculateIndex(swap, x, y); with the swap, the sources x = 0 and y = 0 are overwritten by
+// x = s-1 and y = s-1 (same target), and row and column 0 are never written: 0 here.
+__device__ __forceinline__ int64_t fft_index(int swap, int x, int y, int dim) {
+    if (swap) {
+        const int h = dim / 2;
+        const int64_t cx = x < h ? (h - x) : (dim - 1 - x + h);
+        const int64_t cy = y < h ? (h - y) : (dim - 1 - y + h);
+        return cx + (int64_t)dim * cy;
+    }
+    return x + (int64_t)dim * y;
+}
+
+__global__ void k_fft_out(const double2* __restrict__ buf, int s, int out_type, int swap, double* __restrict__ out) {
+    const int64_t len = (int64_t)s * s;
+    const double2* c = buf + (int64_t)blockIdx.y * len;
+    double* o = out + (int64_t)blockIdx.y * len;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(k / s), x = (int)(k % s);
+        if (swap && (x == 0 || y == 0)) {
+            o[k] = 0.0;  // cell (0, k) or (k, 0) of the output: never written
+            continue;
+        }
+        const double2 v = c[(int64_t)x * s + y];
+        double r;
+        switch (out_type) {
+        case 0: r = v.x; break;
+        case 1: r = v.y; break;
+        case 2: r = sqrt(v.x * v.x + v.y * v.y); break;
+        default: r = (v.x + v.y) * (v.x - v.y); break;
+        }
+        o[fft_index(swap, x, y, s)] = r;
+    }
+}
+
+// Centred cut-out of each s x s image into cx x cy.
+__global__ void k_cut(const double* __restrict__ full, int s, int cx, int cy, double* __restrict__ out) {
+    const int64_t n = (int64_t)cx * cy;
+    const double* f = full + (int64_t)blockIdx.y * s * s;
+    double* o = out + (int64_t)blockIdx.y * n;
+    const int x0 = s / 2 - cx / 2, y0 = s / 2 - cy / 2;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int yo = (int)(k / cx), xo = (int)(k % cx);
+        o[k] = f[(x0 + xo) + (int64_t)s * (y0 + yo)];
+    }
+}
+
 int blocks_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
 
 }  // namespace csp
@@ -331,6 +432,67 @@ int csx_post_rphi(csx_ctx* c, const double* d_img, int x0, int x1, int y0, int y
     // the host vectors must outlive the copies
     PCHK(c, hipStreamSynchronize(st));
     return csx_post_scale_clip(c, d_out, cells, nimg, scale, clip, clip40, 0);
+}
+
+int csx_post_ifft(csx_ctx* c, const double* d_in, int s, int nimg, int forward, int out_type, int scaled, int clip,
+                  int clip40, int swap, int cutx, int cuty, double* d_out) {
+    const int64_t len = (int64_t)s * s;
+    if (int rc = check(c, d_in, len, nimg)) return rc;
+    int m = 0;
+    while ((1 << m) < s) ++m;
+    if (!d_out || s < 4 || s > 2048 || (1 << m) != s) return csx_ctx_fail(c, CSX_E_ARG, "FFT size must be a power of two, 4..2048");
+    if (out_type < 0 || out_type > 3) return csx_ctx_fail(c, CSX_E_ARG, "FFT output type 0..3");
+    const bool cut = cutx > 0 && cuty > 0 && cutx < s && cuty < s;
+    // twiddles of every level, by the reference's recurrence
+    std::vector<double2> tw((size_t)s - 1);
+    double c1 = -1.0, c2 = 0.0;
+    for (int l = 0, l1 = 1; l < m; ++l, l1 <<= 1) {
+        double u1 = 1.0, u2 = 0.0;
+        for (int jj = 0; jj < l1; ++jj) {
+            tw[(size_t)(l1 - 1 + jj)] = make_double2(u1, u2);
+            const double z = u1 * c1 - u2 * c2;
+            u2 = u1 * c2 + u2 * c1;
+            u1 = z;
+        }
+        c2 = std::sqrt((1.0 - c1) / 2.0);
+        if (forward) c2 = -c2;
+        c1 = std::sqrt((1.0 + c1) / 2.0);
+    }
+    // scratch 0: complex lines; scratch 1: maxima, twiddles, and the full image before a cut-out
+    const size_t cbytes = (size_t)len * nimg * sizeof(double2);
+    const size_t o_tw = ((size_t)nimg * sizeof(double) + 255) & ~size_t(255);
+    const size_t o_full = (o_tw + tw.size() * sizeof(double2) + 255) & ~size_t(255);
+    const size_t sbytes = o_full + (cut ? (size_t)len * nimg * sizeof(double) : 0);
+    double2* buf = (double2*)csx_ctx_scratch(c, 0, cbytes);
+    char* s1 = (char*)csx_ctx_scratch(c, 1, sbytes);
+    if (!buf || !s1) return csx_ctx_fail(c, CSX_E_NOMEM, "FFT scratch");
+    double* mx = (double*)s1;
+    double2* d_tw = (double2*)(s1 + o_tw);
+    double* full = cut ? (double*)(s1 + o_full) : d_out;
+    hipStream_t st = (hipStream_t)csx_ctx_stream(c);
+    PCHK(c, hipMemcpyAsync(d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_max0, dim3(nimg), dim3(kStat), 0, st, d_in, len, mx);
+    hipLaunchKernelGGL(k_fft_load, dim3(blocks_for(len), nimg), dim3(256), 0, st, d_in, len, mx, buf);
+    const int threads = std::min(1024, std::max(64, s / 2));
+    const size_t shm = 2 * (size_t)s * sizeof(double);
+    // along the first index (stride s) for every second index, then along the second
+    hipLaunchKernelGGL(k_fft_lines, dim3(s, nimg), dim3(threads), shm, st, buf, len, (int64_t)1, (int64_t)s, m, d_tw,
+                       forward);
+    hipLaunchKernelGGL(k_fft_lines, dim3(s, nimg), dim3(threads), shm, st, buf, len, (int64_t)s, (int64_t)1, m, d_tw,
+                       forward);
+    hipLaunchKernelGGL(k_fft_out, dim3(blocks_for(len), nimg), dim3(256), 0, st, buf, s, out_type, swap, full);
+    PCHK(c, hipGetLastError());
+    // (its maxima go to the start of scratch 1, which mx no longer needs; the twiddles and
+    // the full image lie past them)
+    if (int rc = csx_post_scale_clip(c, full, len, nimg, scaled, clip, clip40, 0)) return rc;
+    if (cut) {
+        hipLaunchKernelGGL(k_cut, dim3(blocks_for((int64_t)cutx * cuty), nimg), dim3(256), 0, st, full, s, cutx, cuty,
+                           d_out);
+        PCHK(c, hipGetLastError());
+    }
+    // the twiddle table on the host must outlive its copy
+    PCHK(c, hipStreamSynchronize(st));
+    return CSX_OK;
 }
 
 }  // extern "C"

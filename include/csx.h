@@ -279,6 +279,15 @@ int csx_post_scale_clip(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int 
                         int clip40, int genlog);
 int csx_post_rphi(csx_ctx* ctx, const double* d_img, int x0, int x1, int y0, int y1, int xc, int yc,
                   int s, int nimg, double* d_out, int scale, int clip, int clip40);
+/* FFT image: calculateIFFT of builds without FFTW (Source/sc_postproc.cpp:364-560, its radix-2
+ * FFT2D/FFT :833-1000) for inputs of the output's size (s x s, s a power of two 4..2048, as the
+ * r,phi image the AI loop passes): each input scaled by its maximum, 2D transform (forward 1 /
+ * backward 0), output type 0 real, 1 imaginary, 2 modulus, 3 re^2-im^2, optional swap (0 at the
+ * centre; the row and column it never writes are 0), scaleAndClipData(scaled, clip, clip40),
+ * then the centred cutx x cuty cut-out when 0 < cutx, cuty < s.  d_out: nimg outputs.  An
+ * input whose maximum is 0 has no FFT in the reference (here its output is not finite). */
+int csx_post_ifft(csx_ctx* ctx, const double* d_in, int s, int nimg, int forward, int out_type, int scaled,
+                  int clip, int clip40, int swap, int cutx, int cuty, double* d_out);
 
 /* ---- Surrogate network (no counterpart in the reference) --------------------------------
  * The learned part of the fits: an MLP from a normalised parameter set (CSX_MLP_NPAR values)

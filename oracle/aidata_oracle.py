@@ -4,8 +4,8 @@ only: the checker for the batched GPU pipeline.  Only tests/ import it.
 Restates, per image and in the reference's order, the steps of the console's AI-data loop
 after each calculation (Source/sc_mainCons.cpp:1254-1400): optional scaleIntensity (linear),
 noise d + z sqrt(d) with one zuf() per pixel, the beam stop (four zuf() draws), the lines
-(one zuf() per line), the log scaling of the image for the r,phi input (axes skipped) and the
-r,phi resampling with its log rescaling.  scaleIntensity, generateRphi and scaleAndClipData
+(one zuf() per line), the log scaling of the image for the r,phi input (axes skipped), the
+r,phi resampling with its log rescaling and the FFT image of the r,phi image (calculateIFFT).  scaleIntensity, generateRphi and scaleAndClipData
 are the C restatements of liboracle.so (pinned bit for bit against the reference's own
 routines, tests/golden/post.npz); the random variation comes from the product engine's
 tpvPerformRandom, pinned against the reference's run (tests/golden/tpv_fcc.npz).
@@ -79,4 +79,8 @@ def post_image(img, bounds, beam, rng, opts):
         rphi = O.rphi(data, (x0, x1, y0, y1), bsx0, bsy0, opts.rphi_size, opts.rphi_scale, opts.rphi_clip,
                       opts.rphi_clip40)
         rphi = O.scale_clip(rphi, genlog=True)
+        if opts.fft_size:
+            fft = O.ifft(rphi, opts.rphi_size, False, opts.fft_type, opts.fft_scale, opts.fft_clip, opts.fft_clip40,
+                         opts.fft_swap, opts.fft_cut[0], opts.fft_cut[1])
+            return src, (bsx0, bsy0), rphi, fft
     return src, (bsx0, bsy0), rphi

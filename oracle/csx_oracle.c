@@ -1749,3 +1749,117 @@ void csxo_rphi(const double* d, int x0, int x1, int y0, int y1, int xc, int yc, 
     (void)len;
     csxo_scale_clip(out, arrlen, scale, clip, clip40, 0);
 }
+
+
+/* The radix-2 FFT the reference carries for builds without FFTW (Source/sc_postproc.cpp:833-1000,
+ * from P. Bourke's published DFT/FFT notes): bit reversal, then m butterfly levels whose
+ * twiddles follow the recurrence u <- u * (c1 + i c2) with c1, c2 halved-angle square roots. */
+static void fft1d(int dir, int m, double* x, double* y)
+{
+    long nn = 1;
+    for (int i = 0; i < m; ++i) nn *= 2;
+    long i2 = nn >> 1, j = 0;
+    for (long i = 0; i < nn - 1; ++i) {
+        if (i < j) {
+            double tx = x[i], ty = y[i];
+            x[i] = x[j]; y[i] = y[j];
+            x[j] = tx; y[j] = ty;
+        }
+        long k = i2;
+        while (k <= j) { j -= k; k >>= 1; }
+        j += k;
+    }
+    double c1 = -1.0, c2 = 0.0;
+    long l2 = 1;
+    for (int l = 0; l < m; ++l) {
+        const long l1 = l2;
+        l2 <<= 1;
+        double u1 = 1.0, u2 = 0.0;
+        for (long jj = 0; jj < l1; ++jj) {
+            for (long i = jj; i < nn; i += l2) {
+                const long i1 = i + l1;
+                const double t1 = u1 * x[i1] - u2 * y[i1];
+                const double t2 = u1 * y[i1] + u2 * x[i1];
+                x[i1] = x[i] - t1;
+                y[i1] = y[i] - t2;
+                x[i] += t1;
+                y[i] += t2;
+            }
+            const double z = u1 * c1 - u2 * c2;
+            u2 = u1 * c2 + u2 * c1;
+            u1 = z;
+        }
+        c2 = sqrt((1.0 - c1) / 2.0);
+        if (dir == 1) c2 = -c2;
+        c1 = sqrt((1.0 + c1) / 2.0);
+    }
+    if (dir == 1)
+        for (long i = 0; i < nn; ++i) { x[i] /= (double)nn; y[i] /= (double)nn; }
+}
+
+/* calculateIndex (:729-752): 0 at the centre when swap is set. */
+static long fft_index(int swap, int x, int y, int dim)
+{
+    if (swap) {
+        if (y < dim / 2 && x < dim / 2) return (dim / 2 - x) + (long)dim * (dim / 2 - y);
+        if (y >= dim / 2 && x < dim / 2) return (dim / 2 - x) + (long)dim * (dim - 1 - y + dim / 2);
+        if (y < dim / 2 && x >= dim / 2) return (dim - 1 - x + dim / 2) + (long)dim * (dim / 2 - y);
+        return (dim - 1 - x + dim / 2) + (long)dim * (dim - 1 - y + dim / 2);
+    }
+    return x + (long)dim * y;
+}
+
+long csxo_ifft(const double* d, int s, int forward, int out_type, int scaled, int clip, int clip40, int swap,
+               int cutx, int cuty, double* out)
+{
+    const long len = (long)s * s;
+    double mx = 0;
+    for (long i = 0; i < len; ++i) if (d[i] > mx) mx = d[i];
+    if (mx == 0) return 0;
+    int m = 0;
+    while ((1L << m) < s) ++m;
+    if ((1L << m) != s || s < 4) return 0;
+    /* c[first][second]: first = y, second = x of the input (_complex[yc][xc]) */
+    double* re = (double*)malloc(sizeof(double) * len);
+    double* im = (double*)malloc(sizeof(double) * len);
+    double* lr = (double*)malloc(sizeof(double) * s);
+    double* li = (double*)malloc(sizeof(double) * s);
+    for (int y = 0; y < s; ++y)
+        for (int x = 0; x < s; ++x) { re[(long)y * s + x] = d[x + (long)s * y] / mx; im[(long)y * s + x] = 0.0; }
+    const int dir = forward ? 1 : -1;
+    for (int j = 0; j < s; ++j) { /* FFT2D's "rows": along the first index for each second index */
+        for (int i = 0; i < s; ++i) { lr[i] = re[(long)i * s + j]; li[i] = im[(long)i * s + j]; }
+        fft1d(dir, m, lr, li);
+        for (int i = 0; i < s; ++i) { re[(long)i * s + j] = lr[i]; im[(long)i * s + j] = li[i]; }
+    }
+    for (int i = 0; i < s; ++i) { /* then along the second index for each first index */
+        for (int j = 0; j < s; ++j) { lr[j] = re[(long)i * s + j]; li[j] = im[(long)i * s + j]; }
+        fft1d(dir, m, lr, li);
+        for (int j = 0; j < s; ++j) { re[(long)i * s + j] = lr[j]; im[(long)i * s + j] = li[j]; }
+    }
+    double* full = (double*)calloc((size_t)len, sizeof(double)); /* the swap leaves row and column 0 unwritten */
+    for (int y = 0; y < s; ++y)
+        for (int x = 0; x < s; ++x) {
+            const double a = re[(long)x * s + y], b = im[(long)x * s + y]; /* _complex[x][y] */
+            double v;
+            switch (out_type) {
+            case 0: v = a; break;
+            case 1: v = b; break;
+            case 2: v = sqrt(sq(a) + sq(b)); break;
+            default: v = (a + b) * (a - b); break;
+            }
+            full[fft_index(swap, x, y, s)] = v;
+        }
+    csxo_scale_clip(full, len, scaled, clip, clip40, 0);
+    long n = len;
+    if (cutx > 0 && cuty > 0 && cutx < s && cuty < s) {
+        const int x0 = s / 2 - cutx / 2, y0 = s / 2 - cuty / 2;
+        for (int yo = 0; yo < cuty; ++yo)
+            for (int xo = 0; xo < cutx; ++xo) out[xo + (long)cutx * yo] = full[(x0 + xo) + (long)s * (y0 + yo)];
+        n = (long)cutx * cuty;
+    } else {
+        memcpy(out, full, sizeof(double) * len);
+    }
+    free(re); free(im); free(lr); free(li); free(full);
+    return n;
+}

@@ -84,6 +84,14 @@ void csxo_scale_intensity(double* p, long n, int linlog);
 void csxo_scale_clip(double* data, long len, int scale, int clip, int clip40, int genlog);
 void csxo_rphi(const double* d, int x0, int x1, int y0, int y1, int xc, int yc, int s, double* out,
                int scale, int clip, int clip40);
+/* calculateIFFT (Source/sc_postproc.cpp:364-560, the build without FFTW: the radix-2 FFT2D/FFT
+ * of :833-1000) for an s x s input the size of the output: input scaled by its maximum, 2D
+ * FFT (forward 1 / backward 0), output type 0 real, 1 imag, 2 abs, 3 re^2-im^2, the swap to
+ * centre 0, scaleAndClipData, and the centred cut-out when 0 < cutx, cuty < s.  out: s*s
+ * values (cutx*cuty with the cut-out).  Returns the output length, 0 when the input maximum
+ * is 0 or s is not a power of two >= 4. */
+long csxo_ifft(const double* d, int s, int forward, int out_type, int scaled, int clip, int clip40, int swap,
+               int cutx, int cuty, double* out);
 
 #ifdef __cplusplus
 }

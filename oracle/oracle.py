@@ -25,6 +25,8 @@ def lib():
         L.csxo_scale_intensity.argtypes = [pd, ctypes.c_long, i]
         L.csxo_scale_clip.argtypes = [pd, ctypes.c_long, i, i, i, i]
         L.csxo_rphi.argtypes = [pd, i, i, i, i, i, i, i, pd, i, i, i]
+        L.csxo_ifft.argtypes = [pd, i, i, i, i, i, i, i, i, i, pd]
+        L.csxo_ifft.restype = ctypes.c_long
         L.csxo_free.argtypes = [vp]
         L.csxo_set.argtypes = [vp, cp, d]
         L.csxo_set.restype = i
@@ -206,3 +208,12 @@ def rphi(img, bounds, xc, yc, s, scale=False, clip=False, clip40=False):
     x0, x1, y0, y1 = (int(b) for b in bounds)
     lib().csxo_rphi(_pd(a), x0, x1, y0, y1, int(xc), int(yc), int(s), _pd(out), int(scale), int(clip), int(clip40))
     return out
+
+
+def ifft(data, s, forward=False, out_type=0, scaled=False, clip=False, clip40=False, swap=False, cutx=0, cuty=0):
+    """calculateIFFT of an s x s input (see csx_oracle.h); None where the reference returns none."""
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    out = np.zeros(s * s)
+    n = lib().csxo_ifft(_pd(a), int(s), int(forward), int(out_type), int(scaled), int(clip), int(clip40), int(swap),
+                        int(cutx), int(cuty), _pd(out))
+    return out[:n] if n else None

@@ -22,6 +22,9 @@
 //     repeat=<n>           repeat the calculation n times (timing)
 //     rphi=<s> bsx= bsy=   r,phi transform of the image (s x s, centre bsx,bsy) and its log
 //                          scaling; rphiscale/rphiclip/rphiclip40 = flags
+//     fft=<type>           calculateIFFT of the log-scaled r,phi image (0 re, 1 im, 2 abs,
+//                          3 spec); fftforward, fftscale, fftclip, fftclip40, fftswap flags,
+//                          fftcutx/fftcuty cut-out
 //     scaleint=0|1         scaleIntensity of the image (1 linear, 0 log)
 //     tpv=<k=f|k=f|...>    tpvPerformRandom(list) before each of tpvn=<n> images
 //                          (srand=<s> seeds rand() first; default: the process's seed 1)
@@ -192,6 +195,17 @@ int main(int argc, char** argv) {
             w.d("rphi", rp, (long long)rs * rs);
             SasCalc_PostProc::inst()->scaleAndClipData(rp, rs * rs, false, false, false, true);
             w.d("rphi_log", rp, (long long)rs * rs);
+            if (g_kv.count("fft")) {
+                // calculateIFFT of the r,phi image (Source/sc_mainCons.cpp:1400-1420): backward,
+                // output size = r,phi size
+                const int cx = (int)num("fftcutx", 0), cy = (int)num("fftcuty", 0);
+                double* f = SasCalc_PostProc::inst()->calculateIFFT(
+                    num("fftforward", 0) != 0, 0, rs, 0, rs, rp, rs,
+                    static_cast<SasCalc_PostProc::_outType>((int)num("fft", 0)), num("fftscale", 0) != 0,
+                    num("fftclip", 0) != 0, num("fftclip40", 0) != 0, num("fftswap", 0) != 0, cx, cy);
+                const bool cut = cx > 0 && cy > 0 && cx < rs && cy < rs;
+                if (f) w.d("fft", f, cut ? (long long)cx * cy : (long long)rs * rs);
+            }
         }
         if (g_kv.count("scaleint")) {
             // scaleIntensity (Source/sc_calc_generic_cpu.cpp:11195-11236): 1 linear, 0 log

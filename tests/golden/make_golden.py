@@ -118,10 +118,13 @@ def write_tpv(manifest):
 
 # Post-processing: r,phi resampling (with its log scaling) and scaleIntensity of FCC and
 # sphere images, from the reference's own routines.
-POST_CASES = [("fcc_rphi", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 0, "bsy": 0, "scaleint": 0}),
-              ("fcc_rphi_off", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 10, "bsy": -5, "rphiscale": 1}),
+POST_CASES = [("fcc_rphi", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 0, "bsy": 0, "scaleint": 0,
+                                                        "fft": 2, "fftswap": 1, "fftscale": 1}),
+              ("fcc_rphi_off", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 10, "bsy": -5, "rphiscale": 1,
+                                                            "fft": 3, "fftclip": 1, "fftswap": 1, "fftcutx": 64,
+                                                            "fftcuty": 32}),
               ("fcc_rphi_clip", dict(CFG3, GridPoints=24), {"rphi": 128, "bsx": 3, "bsy": 4, "rphiclip": 1,
-                                                             "rphiclip40": 1}),
+                                                             "rphiclip40": 1, "fft": 1, "fftforward": 1}),
               ("sphere_lin", dict(CFG1, GridPoints=32), {"scaleint": 1})]
 
 
@@ -131,7 +134,7 @@ def write_post(manifest):
         r = run_ref(p, extra)
         arrays[name + "_image"] = r["image"]
         arrays[name + "_bounds"] = r["bounds"]
-        for k in ("rphi", "rphi_log", "scaled"):
+        for k in ("rphi", "rphi_log", "scaled", "fft"):
             if k in r:
                 arrays["%s_%s" % (name, k)] = r[k]
         cases[name] = {"params": p, "post": extra}

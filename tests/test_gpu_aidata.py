@@ -87,6 +87,25 @@ def test_rphi_matches_the_reference(oracle_lib, name):
     assert np.abs(r[1].cpu().numpy() - oracle_lib.scale_clip(o, genlog=True)).max() < 1e-12
 
 
+@pytest.mark.parametrize("name", ["fcc_rphi", "fcc_rphi_off", "fcc_rphi_clip"])
+def test_fft_matches_the_reference(name):
+    """The FFT image of the reference's r,phi image (calculateIFFT without FFTW)."""
+    import torch
+    from crystalscatter_amd import aidata
+    f, post, img, b = _post_case(name)
+    x = torch.from_numpy(np.stack([f[name + "_rphi_log"]] * 3)).cuda()
+    out = aidata.post_ifft(Context(0), x, post["rphi"], post.get("fftforward", 0), post["fft"],
+                           post.get("fftscale", 0), post.get("fftclip", 0), post.get("fftclip40", 0),
+                           post.get("fftswap", 0), (post.get("fftcutx", 0), post.get("fftcuty", 0))).cpu().numpy()
+    ref = f[name + "_fft"]
+    for row in out:
+        err = np.abs(row - ref).max() / max(np.abs(ref).max(), 1e-300)
+        print("%s fft: max err %.3e (of the maximum)" % (name, err), flush=True)
+        # same butterflies and twiddles; only sqrt of the modulus and the log clip go
+        # through the device libm
+        assert err < 1e-13
+
+
 @pytest.mark.parametrize("name,lin", [("fcc_rphi", 0), ("sphere_lin", 1)])
 def test_scale_intensity_matches_the_reference(name, lin):
     import torch
@@ -100,7 +119,8 @@ def test_scale_intensity_matches_the_reference(name, lin):
 
 @pytest.mark.parametrize("opts_kw", [dict(noise=True, bs=(0.2, 0.2, 0.5, 0.5), lines_x=2, lines_y=1, rphi_size=128),
                                      dict(scale_scat=True, noise=True, rphi_size=128, fft_input_lin=True,
-                                          rphi_scale=True)])
+                                          rphi_scale=True, fft_size=128, fft_type=2, fft_swap=True,
+                                          fft_scale=True, fft_cut=(64, 32))])
 def test_ai_pipeline_matches_the_oracle(oracle_lib, opts_kw):
     """The batched pipeline against the per-image CPU restatement, from the same rand() state."""
     import sys
@@ -126,7 +146,11 @@ def test_ai_pipeline_matches_the_oracle(oracle_lib, opts_kw):
         q.update({key: e2.tpv_value(key) for key in ["EditRadius", "uca", "ucb", "ucc", "I0", "width_zuf"]})
         o = oracle_lib.OracleEngine(q)
         o.prepare()
-        img, centre, rp = AO.post_image(o.image(nthreads=8), o.bounds(), (3, -4), rng, opts)
+        res = AO.post_image(o.image(nthreads=8), o.bounds(), (3, -4), rng, opts)
+        img, centre, rp = res[:3]
+        if opts.fft_size:
+            ft = out["fft"][k].cpu().numpy()
+            assert np.abs(ft - res[3]).max() < 1e-10 * max(1.0, np.abs(res[3]).max()), k
         assert tuple(out["centres"][k]) == centre
         ok = np.isfinite(img)
         assert np.array_equal(np.isfinite(scat[k]), ok) and np.array_equal(scat[k] == 0, img == 0)

@@ -102,3 +102,48 @@ def test_cfg4_bct_sets_and_fit_masks(oracle_lib):
         assert s == fit["fqs"] and c == fit["cnt"] and n == fit["nancnt"]
         key = "fitfqs_b%d_r%d_%s" % (fit["bstop"], fit["border"], "rect" if fit["rect"] else "none")
         np.testing.assert_array_equal(arr, f[key])
+
+
+def test_post_processing_bitexact(oracle_lib):
+    """r,phi, its log rescaling, scaleIntensity and the FFT image against the reference's own
+    post-processing routines (tests/golden/post.npz)."""
+    O = oracle_lib
+    f = fixture("post.npz")
+    for name, c in manifest()["fixtures"]["post.npz"]["cases"].items():
+        q, img, b = c["post"], f[name + "_image"], f[name + "_bounds"]
+        if "rphi" in q:
+            r = O.rphi(img, b, q["bsx"], q["bsy"], q["rphi"], q.get("rphiscale", 0), q.get("rphiclip", 0),
+                       q.get("rphiclip40", 0))
+            np.testing.assert_array_equal(r, f[name + "_rphi"], err_msg=name)
+            np.testing.assert_array_equal(O.scale_clip(r, genlog=True), f[name + "_rphi_log"], err_msg=name)
+        if "fft" in q:
+            o = O.ifft(f[name + "_rphi_log"], q["rphi"], q.get("fftforward", 0), q["fft"], q.get("fftscale", 0),
+                       q.get("fftclip", 0), q.get("fftclip40", 0), q.get("fftswap", 0), q.get("fftcutx", 0),
+                       q.get("fftcuty", 0))
+            np.testing.assert_array_equal(o, f[name + "_fft"], err_msg=name)
+        if "scaleint" in q:
+            np.testing.assert_array_equal(O.scale_intensity(img, q["scaleint"]), f[name + "_scaled"], err_msg=name)
+
+
+@pytest.mark.parametrize("s", [8, 64])
+def test_fft_is_the_discrete_fourier_transform(oracle_lib, s):
+    """The radix-2 FFT against numpy's: backward = unscaled inverse DFT, forward = DFT / N per
+    axis; the output is read transposed (cell (x, y) holds c[x][y])."""
+    rng = np.random.default_rng(s)
+    d = rng.uniform(0, 1, (s, s))
+    A = d / d.max()
+    back = np.fft.ifft2(A) * s * s
+    fwd = np.fft.fft2(A) / (s * s)
+    for forward, C in ((0, back), (1, fwd)):
+        re = oracle_lib.ifft(d.ravel(), s, forward, 0).reshape(s, s)
+        im = oracle_lib.ifft(d.ravel(), s, forward, 1).reshape(s, s)
+        assert np.abs(re - C.real.T).max() < 1e-12 * s * s
+        assert np.abs(im - C.imag.T).max() < 1e-12 * s * s
+    # modulus and swap: cell (x, y) goes to ((s/2 - x) or (3s/2 - 1 - x), same for y); sources
+    # x = 0 and y = 0 land where x = s-1, y = s-1 land later, so the centre ends with
+    # |C[s-1][s-1]|, and row and column 0 are never written
+    ab = oracle_lib.ifft(d.ravel(), s, 0, 2, swap=True).reshape(s, s)
+    h = s // 2
+    assert abs(ab[h, h] - abs(back[s - 1, s - 1])) < 1e-9 * s * s
+    assert abs(ab[h - 1, h + 1] - abs(back[s - 2, 1])) < 1e-9 * s * s  # x = s-2 -> h+1, y = 1 -> h-1
+    assert np.all(ab[0] == 0) and np.all(ab[:, 0] == 0)
