@@ -417,9 +417,12 @@ const int kMultCubic[15] = {0, 24, 24, 24, 6, 6, 6, 24, 24, 24, 12, 12, 12, 8, 4
 const int kMultTetra[15] = {0, 8, 8, 8, 4, 4, 2, 8, 8, 8, 4, 8, 8, 8, 16};
 
 // Reflection conditions: FCC = cubic F--- (hkl: h+k,h+l,k+l; 0kl: k,l; hhl: h+l; 00l: l);
-// BCT = tetragonal I--- (hkl: h+k+l; 0kl: k+l; hk0: h+k; hhl: l; 00l: l; 0k0: k).
+// BCC = cubic I--- (hkl: h+k+l, which implies its zonal and serial conditions); SC = cubic
+// P--- (none); BCT = tetragonal I--- (hkl: h+k+l; 0kl: k+l; hk0: h+k; hhl: l; 00l: l; 0k0: k).
 bool allowed(int ltype, int h, int k, int l) {
     if (h == 0 && k == 0 && l == 0) return false;
+    if (ltype == 7) return true;
+    if (ltype == 4) return ev(h + k + l);
     if (ltype == 5) {
         if (!(ev(h + k) && ev(h + l) && ev(k + l))) return false;
         if ((h == 0 && !(ev(k) && ev(l))) || (k == 0 && !(ev(h) && ev(l))) || (l == 0 && !(ev(h) && ev(k))))
@@ -439,9 +442,13 @@ bool allowed(int ltype, int h, int k, int l) {
     return true;
 }
 
-// |F(hkl)|^2 of the FCC (4 atoms) / BCT (2 atoms) basis, float basis coordinates
-// (fhkl_c, Source/sc_calc_generic_cpu.cpp:10397-10478).
+// |F(hkl)|^2 of the FCC (4 atoms) / BCC and BCT (2 atoms) basis, float basis coordinates;
+// SC: 1 (fhkl_c, Source/sc_calc_generic_cpu.cpp:10374-10478).
 double structure_factor(int ltype, int h, int k, int l, double& sphno) {
+    if (ltype == 7) {
+        sphno = 1;
+        return 1;
+    }
     static const float fcc[4][3] = {{0, 0, 0}, {0.5f, 0.5f, 0}, {0, 0.5f, 0.5f}, {0.5f, 0, 0.5f}};
     static const float bct[2][3] = {{0, 0, 0}, {0.5f, 0.5f, 0.5f}};
     const float(*r)[3] = ltype == 5 ? fcc : bct;
@@ -749,7 +756,9 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     P.latpar2.assign(2 * 6, 0);
     P.latpar3.assign(2 * 17, 0.f);
     if (p.ltype != 12) {
-        if (p.ltype != 5 && p.ltype != 8) return "lattice type not supported by the device path";
+        if (p.ltype != 4 && p.ltype != 5 && p.ltype != 7 && p.ltype != 8)
+            return "lattice type not supported by the device path";
+        const bool cubic = p.ltype != 8;
         const int H = p.hklmax;
         std::vector<int> iso, ani;
         for (int pass = 0; pass < 2; ++pass) {
@@ -761,7 +770,7 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
                         if (allowed(p.ltype, h, k, l)) {
                             const int pat = pattern(h, k, l);
                             v.push_back(h); v.push_back(k); v.push_back(l);
-                            v.push_back(p.ltype == 5 ? kMultCubic[pat] : kMultTetra[pat]);
+                            v.push_back(cubic ? kMultCubic[pat] : kMultTetra[pat]);
                         }
         }
         const int n1 = static_cast<int>(iso.size() / 4), n2 = static_cast<int>(ani.size() / 4);

@@ -55,6 +55,13 @@ enum {
 #define CSX_KEY_CYL_NONE_ZDIR     0x01000C06u
 #define CSX_KEY_CYL_BCT_ZDIR      0x01020806u
 #define CSX_KEY_CUBE_NONE_ISO     0x04000C07u
+/* Spheres with Gaussian peaks on other cubic lattices and with isotropic orientation (the
+ * reference's generic kernel, Source/sc_gpu_generic.h; same sums as the FCC kernel). */
+#define CSX_KEY_SPHERE_FCC_ISO    0x00020507u
+#define CSX_KEY_SPHERE_BCC_GAUSS  0x00020400u
+#define CSX_KEY_SPHERE_BCC_ISO    0x00020407u
+#define CSX_KEY_SPHERE_SC_GAUSS   0x00020700u
+#define CSX_KEY_SPHERE_SC_ISO     0x00020707u
 
 #define CSX_NCOEF 151   /* coeffarray_len */
 #define CSX_NIM2 131    /* imax2d_len */

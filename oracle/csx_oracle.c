@@ -746,12 +746,30 @@ static int allowed_bct(int h, int k, int l)
     return c1 * c2 * c4 * c6 * c7 * c10;
 }
 
+/* BCC, cubic group I--- (ComboBoxCubic 7: c1=0, c2=0, c7=0, c10=0; :8564-8567) and SC, P---
+ * (ComboBoxCubic 0: no condition; :8534-8537), with the cubic checks of :9941-9998. */
+static int allowed_bcc(int h, int k, int l)
+{
+    int c1 = 0, c2 = 1, c7 = 1, c10 = 1;
+    if (even(h + k + l)) c1 = 1;
+    if (h == 0 && !even(k + l)) c2 = 0;
+    if (k == 0 && !even(h + l)) c2 = 0;
+    if (l == 0 && !even(h + k)) c2 = 0;
+    if (h == k && l != 0 && !even(l)) c10 = 0;
+    if (h == l && k != 0 && !even(k)) c10 = 0;
+    if (k == l && h != 0 && !even(h)) c10 = 0;
+    if (h == 0 && k == 0 && !even(l)) c7 = 0;
+    if (h == 0 && l == 0 && !even(k)) c7 = 0;
+    if (k == 0 && l == 0 && !even(h)) c7 = 0;
+    return c1 * c2 * c7 * c10;
+}
+
 /* hkl loops (Source/sc_calc_generic_cpu.cpp:9712-10291): pass 1 (isotropic) over
  * [0,H]^3 with multiplicity, pass 2 (anisotropic) over [-H,H]^3, h-major. */
 static void build_reflections(csxo_engine* e)
 {
     const int H = e->hklmax;
-    const int fcc = (e->ltype == 5);
+    const int cubic = (e->ltype == 4 || e->ltype == 5 || e->ltype == 7);
     for (int pass = 1; pass <= 2; ++pass) {
         const int lo = (pass == 1) ? 0 : -H;
         int ct = 1;
@@ -761,8 +779,14 @@ static void build_reflections(csxo_engine* e)
             for (int k = lo; k <= H; ++k)
                 for (int l = lo; l <= H; ++l) {
                     const int pat = hkl_pattern(h, k, l);
-                    const int mult = fcc ? mult_cubic(pat) : mult_tetragonal(pat);
-                    int ok = fcc ? allowed_fcc(h, k, l) : allowed_bct(h, k, l);
+                    const int mult = cubic ? mult_cubic(pat) : mult_tetragonal(pat);
+                    int ok;
+                    switch (e->ltype) {
+                    case 4: ok = allowed_bcc(h, k, l); break;
+                    case 5: ok = allowed_fcc(h, k, l); break;
+                    case 7: ok = 1; break;
+                    default: ok = allowed_bct(h, k, l); break;
+                    }
                     if (h == 0 && k == 0 && l == 0) ok = 0;
                     if (ok) {
                         lp[1][0] = ct;
@@ -773,13 +797,15 @@ static void build_reflections(csxo_engine* e)
     }
 }
 
-/* Structure factor |sum exp(2 pi i h.r)|^2 over the basis (fhkl_c, FCC :10397-10420,
- * BCT :10457-10478; basis coordinates in float as in the reference). */
+/* Structure factor |sum exp(2 pi i h.r)|^2 over the basis (fhkl_c, BCC :10374-10393, FCC
+ * :10397-10420, SC :10449-10455, BCT :10457-10478; basis coordinates in float as in the
+ * reference). */
 static double fhkl_value(int ltype, int h, int k, int l, double* sphno)
 {
     float r[5][4];
     int nn;
     memset(r, 0, sizeof(r));
+    if (ltype == 7) { *sphno = 1; return 1; }
     if (ltype == 5) {
         *sphno = 4; nn = 4;
         r[2][1] = 1 / 2.; r[2][2] = 1 / 2.;
@@ -927,7 +953,7 @@ int csxo_prepare(csxo_engine* e)
     e->peakmax1 = e->peakmax2 = 0;
     e->lat_ucvol = e->ucvol; e->lat_sphno = 1;
     if (e->lattice) {
-        if (e->ltype != 5 && e->ltype != 8) return -3; /* only FCC / BCT restated */
+        if (e->ltype != 4 && e->ltype != 5 && e->ltype != 7 && e->ltype != 8) return -3; /* BCC, FCC, SC, BCT */
         build_reflections(e);
         e->peakmax1 = e->latpar1[1][0];
         e->peakmax2 = e->latpar2[1][0];
@@ -1260,7 +1286,12 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }
-    case 0x00020500u: { /* Source/sc_gpu_pSph_pGau_lFCC_oGau.h:15-171 */
+    case 0x00020500u: /* Source/sc_gpu_pSph_pGau_lFCC_oGau.h:15-171 */
+    case 0x00020507u: /* spheres with Gaussian peaks on FCC, BCC, SC, Gaussian or isotropic orientation: */
+    case 0x00020400u: /* calc_GENERIC's sphere and Gaussian-peak branches (Source/sc_gpu_generic.h:40-52, */
+    case 0x00020407u: /* 691-880), the same sums; the orientation distribution does not enter them */
+    case 0x00020700u:
+    case 0x00020707u: {
         const double pq = pq_sphere(e, q);
         const double fq = fq_sphere(e, q);
         double I, szqiso;

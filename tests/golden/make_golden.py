@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv post                     # only these
+    python tests/golden/make_golden.py cube tpv post lattices            # only these
 """
 import json
 import os
@@ -100,6 +100,21 @@ def write_cube(manifest):
         manifest["fixtures"][name] = {"params": p}
 
 
+# Spheres with Gaussian peaks on BCC and SC lattices (the reference's generic kernel), and FCC
+# with isotropic orientation: 64x64 images with their tables.
+LATTICES = [("bcc_sphere", dict(CFG3, LType=4, uca=30, ucb=30, ucc=30, GridPoints=32)),
+            ("sc_sphere_iso", dict(CFG3, LType=7, uca=25, ucb=25, ucc=25, Ordis=7, GridPoints=32)),
+            ("fcc_sphere_iso", dict(CFG3, Ordis=7, GridPoints=32))]
+
+
+def write_lattices(manifest):
+    for name, p in LATTICES:
+        r = run_ref(p)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), image=r["image"], bounds=r["bounds"],
+                            **table_dict(r, True))
+        manifest["fixtures"][name + ".npz"] = {"params": p}
+
+
 # Random variation (tpvPerformRandom) before each of four FCC images, rand() from seed 1.
 TPV_IDS = ["EditRadius=0.1", "EditSigma=0.2", "uca=0.05", "ucb=1", "ucc=1", "I0=0.3", "Base=0.1",
            "EditDomainSize=0.4", "EditDebyeWaller=0.5", "EditAzi=0.3", "phi=1"]
@@ -155,6 +170,8 @@ def main():
             write_tpv(manifest)
         if "post" in only:
             write_post(manifest)
+        if "lattices" in only:
+            write_lattices(manifest)
         with open(os.path.join(HERE, "manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
         return
@@ -231,6 +248,7 @@ def main():
     write_cube(manifest)
     write_tpv(manifest)
     write_post(manifest)
+    write_lattices(manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))

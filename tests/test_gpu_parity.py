@@ -314,3 +314,30 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
     torch.cuda.synchronize()
     for k in range(8):
         assert torch.equal(outs[k], alone[1]), k
+
+
+@pytest.mark.parametrize("prec,tol", [("ref64", 1e-13), ("mixed", 1e-4)])
+@pytest.mark.parametrize("name", ["bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz"])
+def test_sphere_lattices_generic_keys(name, prec, tol):
+    """Spheres on BCC and SC lattices and with isotropic orientation (the reference's generic
+    kernel) against its own output."""
+    f = fixture(name)
+    e, img = _calc(params(name), prec)
+    r = rel_err(img, f["image"])
+    _report("%s %s" % (name, prec), r)
+    assert r.max() < tol
+
+
+def test_bcc_512_against_oracle_random_pixels(oracle_lib):
+    p = dict(params("bcc_sphere.npz"), GridPoints=256)
+    e, img = _calc(p, "mixed")
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    rng = np.random.default_rng(3)
+    idx = rng.choice(img.size, 3000, replace=False)
+    ref = o.image_points(idx % 512 - 256, idx // 512 - 256, nthreads=16)
+    r = rel_err(img[idx], ref)
+    _report("bcc 512^2 vs oracle", r)
+    assert r.max() < 1e-4
+    a = img.reshape(512, 512)[1:, 1:]
+    assert rel_err(a, a[::-1, ::-1]).max() < 1e-6

@@ -17,7 +17,8 @@ def _host(p, ignore_cbs=True):
     return e
 
 
-@pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz"])
+@pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "bcc_sphere.npz",
+                                  "sc_sphere_iso.npz", "fcc_sphere_iso.npz"])
 def test_tables_match_reference(name):
     f = fixture(name)
     e = _host(params(name))
@@ -31,7 +32,8 @@ def test_tables_match_reference(name):
         np.testing.assert_array_equal(e.table("latpar1")[6:], f["latpar1"][6:])
         np.testing.assert_array_equal(e.table("latpar2")[6:], f["latpar2"][6:])
         np.testing.assert_array_equal(e.table("latpar3").reshape(-1, 17)[1:, 1:14], f["latpar3"][1:, 1:14])
-        assert s.sphno_cubevol == f["ucvol"][0] * 4
+        sphno = {4: 2, 5: 4, 7: 1}[int(params(name)["LType"])]
+        assert s.sphno_cubevol == f["ucvol"][0] * sphno
     if name == "cfg2_cylgauss.npz":
         np.testing.assert_array_equal(e.table("carr11pm").reshape(131, 131)[:121, :121], f["carr11pm"])
 
