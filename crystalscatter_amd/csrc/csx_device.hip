@@ -38,13 +38,17 @@ enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4,
 
 inline int kind_of(uint32_t key) {
     switch (key) {
-    case CSX_KEY_SPHERE_NONE_ISO: return K_SPHERE;
+    case CSX_KEY_SPHERE_NONE_ISO:
+    case CSX_KEY_SPHERE_NONE_GAUSS:
+    case CSX_KEY_SPHERE_NONE_ZDIR: return K_SPHERE;
     case CSX_KEY_SPHERE_FCC_GAUSS:
     case CSX_KEY_SPHERE_FCC_ISO:
     case CSX_KEY_SPHERE_BCC_GAUSS:
     case CSX_KEY_SPHERE_BCC_ISO:
     case CSX_KEY_SPHERE_SC_GAUSS:
-    case CSX_KEY_SPHERE_SC_ISO: return K_FCC;
+    case CSX_KEY_SPHERE_SC_ISO:
+    case CSX_KEY_SPHERE_BCT_GAUSS:
+    case CSX_KEY_SPHERE_BCT_ISO: return K_FCC;
     case CSX_KEY_CYL_NONE_GAUSS: return K_CYL_GAUSS;
     case CSX_KEY_CYL_NONE_ZDIR: return K_CYL_ZDIR;
     case CSX_KEY_CYL_BCT_ZDIR: return K_BCT;

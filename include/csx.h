@@ -62,6 +62,10 @@ enum {
 #define CSX_KEY_SPHERE_BCC_ISO    0x00020407u
 #define CSX_KEY_SPHERE_SC_GAUSS   0x00020700u
 #define CSX_KEY_SPHERE_SC_ISO     0x00020707u
+#define CSX_KEY_SPHERE_BCT_GAUSS  0x00020800u
+#define CSX_KEY_SPHERE_BCT_ISO    0x00020807u
+#define CSX_KEY_SPHERE_NONE_GAUSS 0x00000C00u  /* no lattice: the isotropic sphere's value */
+#define CSX_KEY_SPHERE_NONE_ZDIR  0x00000C06u
 
 #define CSX_NCOEF 151   /* coeffarray_len */
 #define CSX_NIM2 131    /* imax2d_len */

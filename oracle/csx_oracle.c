@@ -1276,6 +1276,8 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
     const double q = sqrt(qx * qx + qy * qy + qz * qz) + kEps9;
     const unsigned key = csxo_combo_key(e);
     switch (key) {
+    case 0x00000C00u: /* spheres without lattice, Gaussian or z-oriented distribution: the generic */
+    case 0x00000C06u: /* kernel's sphere branch (Source/sc_gpu_generic.h:40-52, 868-880), the same value */
     case 0x00000C07u: { /* Source/sc_gpu_pSph_lNon_oIso.h:15-51 */
         const double pq = pq_sphere(e, q);
         *out = e->base + e->i0 * (pq + e->iso * pq) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
@@ -1291,7 +1293,9 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
     case 0x00020400u: /* calc_GENERIC's sphere and Gaussian-peak branches (Source/sc_gpu_generic.h:40-52, */
     case 0x00020407u: /* 691-880), the same sums; the orientation distribution does not enter them */
     case 0x00020700u:
-    case 0x00020707u: {
+    case 0x00020707u:
+    case 0x00020800u: /* spheres on the BCT lattice (generic kernel) */
+    case 0x00020807u: {
         const double pq = pq_sphere(e, q);
         const double fq = fq_sphere(e, q);
         double I, szqiso;

@@ -104,7 +104,9 @@ def write_cube(manifest):
 # with isotropic orientation: 64x64 images with their tables.
 LATTICES = [("bcc_sphere", dict(CFG3, LType=4, uca=30, ucb=30, ucc=30, GridPoints=32)),
             ("sc_sphere_iso", dict(CFG3, LType=7, uca=25, ucb=25, ucc=25, Ordis=7, GridPoints=32)),
-            ("fcc_sphere_iso", dict(CFG3, Ordis=7, GridPoints=32))]
+            ("fcc_sphere_iso", dict(CFG3, Ordis=7, GridPoints=32)),
+            ("bct_sphere", dict(CFG3, LType=8, uca=30, ucb=30, ucc=45, GridPoints=32)),
+            ("sphere_zdir", dict(CFG1, Ordis=6, GridPoints=32))]
 
 
 def write_lattices(manifest):

@@ -47,7 +47,8 @@ def test_vitess_golden_image(oracle_lib):
 
 
 @pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "cube_a.npz",
-                                  "cube_b.npz", "bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz"])
+                                  "cube_b.npz", "bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz",
+                                  "bct_sphere.npz", "sphere_zdir.npz"])
 def test_tables_and_image_bitexact(oracle_lib, name):
     f = fixture(name)
     e = _engine(oracle_lib, params(name))
