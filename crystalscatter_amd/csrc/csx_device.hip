@@ -34,7 +34,8 @@ constexpr double kLog2e = 1.4426950408889634074;
 
 // K_CYL_GAUSS_Q: Gaussian cylinders in physics mode with the FP32 orientation quadrature
 // (mixed precision only; chosen by csx_render, never by the key alone).
-enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4, K_CYL_GAUSS_Q = 5, K_CUBE = 6 };
+enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4, K_CYL_GAUSS_Q = 5, K_CUBE = 6,
+            K_CYL_ISO = 7 };
 
 inline int kind_of(uint32_t key) {
     switch (key) {
@@ -53,6 +54,7 @@ inline int kind_of(uint32_t key) {
     case CSX_KEY_CYL_NONE_ZDIR: return K_CYL_ZDIR;
     case CSX_KEY_CYL_BCT_ZDIR: return K_BCT;
     case CSX_KEY_CUBE_NONE_ISO: return K_CUBE;
+    case CSX_KEY_CYL_NONE_ISO: return K_CYL_ISO;
     default: return -1;
     }
 }
@@ -839,6 +841,10 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
         // Source/sc_gpu_pCub_lNon_oIso.h:14-57: no lattice, iso added to P(q) unweighted
         const double pq = rv ? rv->pq : pq_cube(d, q);
         return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+    } else if constexpr (KIND == K_CYL_ISO) {
+        // calc_GENERIC, cylinders, isotropic (Source/sc_gpu_generic.h:58-66, 868-880)
+        const double pq = rv ? rv->pq : pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+        return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
     } else if constexpr (KIND == K_CYL_ZDIR) {
         double pq;
         switch (s.orcase) {
@@ -970,7 +976,7 @@ __device__ __forceinline__ void tile_pixel(int& px, int& py) {
 }
 
 // Kinds whose form factors depend on |q| only.
-constexpr bool radial_kind(int kind) { return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE; }
+constexpr bool radial_kind(int kind) { return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE || kind == K_CYL_ISO; }
 
 // Index of (a, b), 0 <= a <= b, in the triangle table of radial values.
 __device__ __forceinline__ int tri_index(int a, int b) { return b * (b + 1) / 2 + a; }
@@ -999,6 +1005,7 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, int half,
     if (!second) {
         double pq;
         if constexpr (KIND == K_CUBE) pq = pq_cube(d, q);
+        else if constexpr (KIND == K_CYL_ISO) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
         else if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
         else pq = pq_sphere_fast(d, q);
         e[0] = q;
@@ -1720,6 +1727,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     case K_CYL_ZDIR: if (prec) FN<K_CYL_ZDIR, 1>(__VA_ARGS__); else FN<K_CYL_ZDIR, 0>(__VA_ARGS__); break; \
     case K_CYL_GAUSS_Q: FN<K_CYL_GAUSS_Q, 1>(__VA_ARGS__); break; \
     case K_CUBE: if (prec) FN<K_CUBE, 1>(__VA_ARGS__); else FN<K_CUBE, 0>(__VA_ARGS__); break; \
+    case K_CYL_ISO: if (prec) FN<K_CYL_ISO, 1>(__VA_ARGS__); else FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 

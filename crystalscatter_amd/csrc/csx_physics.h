@@ -1,4 +1,5 @@
-// Per-pixel physics of the five specialised kernels, FP64, reference formulas.
+// Per-pixel physics of the specialised kernels and of the generic kernel's combinations served
+// here, FP64, reference formulas.
 //
 // Device functions (HIP, gfx950).  Compiled with -ffp-contract=off so the FP64
 // arithmetic rounds operation by operation like the reference's CPU build;
@@ -7,6 +8,7 @@
 //
 // Reference functions restated (paths relative to the reference checkout):
 //   pq_sphere     formpq_partSphere cs=0     Source/sc_lib_formpq_partSphere.h:11-75
+//   pq_cube       formpq_partCube ordis 7    Source/sc_lib_formpq_partCube.h:27-61
 //   fq_sphere     formfq_partSphere cs=0     Source/sc_lib_formfq_partSphere.h:8-52
 //   pq_cylinder   formpq_partCylinder        Source/sc_lib_formpq_partCylinder.h:8-257
 //   fq_cylinder   formfq_partCylinder ord 6  Source/sc_lib_formfq_partCylinder.h:8-206
@@ -252,6 +254,20 @@ __device__ inline double pq_cyl_transverse(const DevSet& d, double q) {
     const double pqr2 = (1 / (zr * (zr - 1) * (zr - 2))) * pow(argpq, -3) * sin((zr - 2) * at) / pow(b, (zr - 2) / 2.0);
     const double pqr3 = (1 / (zr * (zr - 1) * (zr - 2) * (zr - 3))) * pow(argpq, -4) * cos((zr - 3) * at) / pow(b, (zr - 3) / 2.0);
     return (4. / kPiD) * (pqr1 - pqr2 - (9. / 8.0) * pqr3);
+}
+
+// Longitudinal part for isotropic orientation, factorisation branch (L/R >= 2,
+// Source/sc_lib_formpq_partCylinder.h:30-90): the P|| series below 0.6 limq1, else the
+// rod average's closed form.
+__device__ inline double pq_cyl_long_iso(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    const double zl = s.sigmal == 1.0 ? 0.000000001 : (1 - sqd(s.sigmal)) / sqd(s.sigmal);
+    if (q < (0.6 * s.limq[0])) return series_x2(carr_of(d, 0), q, 120);
+    const double arglq = q * s.length / (zl + 1);
+    double pql = (kPiD / (2.0 * zl)) * (1 / arglq);
+    pql = pql - (1 / (2.0 * zl * (zl - 1) * arglq * arglq)) * cos((zl - 1) * atan(2.0 * arglq)) /
+                    pow(1.0 + 4 * arglq * arglq, (zl - 1) / 2.0);
+    return pql;
 }
 
 // Longitudinal part for perfect (z-dir) orientation, ordis 6 (:16143-16168).

@@ -262,6 +262,39 @@ struct FSums {
     double acc1[kNN + 1], acc4[kNN + 1];
 };
 
+// Cylinder, isotropic, homogeneous, L/R >= 2 (the factorisation branch,
+// Source/sc_calc_generic_cpu.cpp:3686-3746): P|| and P- series and the <F>^2 series.
+void coef_cylinder_iso(double z, double zl, double xl2z, double xr2z, CoefState& cs) {
+    long double fkv[kNN + 2], fk2v[kNN + 1], gam3[kNN + 2], z12v[kNN + 1], z12vl[kNN + 1];
+    fkv[0] = 1; fk2v[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12v[0] = 1.0; z12vl[0] = 1;
+    long double xln = 1.0;
+    double xrn = 1.0;
+    double *c1p = cs.arr[0], *c4p = cs.arr[3], *c1f = cs.arr[9], *c4f = cs.arr[12];
+    for (int n = 1; n <= kNN; ++n) {
+        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+        z12vl[n] = z12vl[n - 1] * ((zl + 1) - 2 + 2 * n) * ((zl + 1) - 1 + 2 * n);
+        fkv[n] = fkv[n - 1] * n;
+        fk2v[n] = fk2v[n - 1] * (2 * n - 1) * (2 * n);
+        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+        xln = -xln * xl2z;
+        xrn = -xrn * xr2z;
+        c1p[n] = std::sqrt(kPi) * std::pow(4.0, n) * z12vl[n] * xln / (2.0 * (2 * n + 1) * (n + 1) * gam3[n] * fkv[n]);
+        c4p[n] = 4 * (n + 1 / 2.0) * fk2v[n] * z12v[n] * xrn /
+                 ((n + 2) * (n + 1) * fkv[n] * fkv[n] * (n + 1) * fkv[n] * fkv[n]);
+        double acc = 0.0;
+        for (int m = 0; m <= n; ++m) acc += z12vl[m] * z12vl[n - m] / (gam3[m] * gam3[n - m] * fkv[m] * fkv[n - m]);
+        c1f[n] = kPi * xln * acc / (4.0 * (2 * n + 1));
+        acc = 0.0;
+        for (int m = 0; m <= n; ++m)
+            acc += z12v[m] * z12v[n - m] / ((m + 1) * fkv[m] * (n - m + 1) * fkv[n - m] * fkv[m] * fkv[n - m]);
+        c4f[n] = xrn * acc;
+        if (std::fabs(c1p[n]) < 1E-30 && n < cs.nidx[0]) cs.nidx[0] = n;
+        if (std::fabs(c1f[n]) < 1E-30 && n < cs.nidx[9]) cs.nidx[9] = n;
+        if (std::fabs(c4p[n]) < 1E-30 && n < cs.nidx[3]) cs.nidx[3] = n;
+        if (std::fabs(c4f[n]) < 1E-30 && n < cs.nidx[12]) cs.nidx[12] = n;
+    }
+}
+
 // Cylinder, perfect orientation, homogeneous: Source/sc_calc_generic_cpu.cpp:4124-4212.
 void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState& cs) {
     static thread_local FSums fs;
@@ -727,6 +760,13 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     } else if (p.interior == 0) {
         if (P.cdim == 3) {
             coef_sphere(P, z, xr2z, cs);
+            done = true;
+        } else if (p.ordis == 7 && P.cdim == 1) {
+            // isotropic cylinders: the factorisation branch (L/R >= 2); the exact-average branch
+            // for short cylinders (:3592-3685) is not on the device path
+            if (p.length / p.radius < 2.0) return "isotropic cylinders need L/R >= 2 on the device path";
+            P.order = 0;
+            coef_cylinder_iso(z, zl, xl2z, xr2z, cs);
             done = true;
         } else if (p.ordis == 6 && P.cdim == 1) {
             P.order = 1;

@@ -481,6 +481,44 @@ static void coef_cube(csxo_engine* e, double z, double xr2z, int* n4, int* n4f)
     }
 }
 
+/* Cylinder, isotropic, homogeneous, axial ratio L/R >= 2 (Source/sc_calc_generic_cpu.cpp:3578-3583,
+ * 3686-3746, the factorisation branch): P|| and P- series, <F>^2 series, with the 1e-30 switch
+ * points.  z12v, z12vl, fkv, fk2v, gam3 and xln are long double, xrn double, as declared there. */
+static void coef_cyl_iso(csxo_engine* e, double z, double zl, double xl2z, double xr2z, int* n1, int* n4,
+                         int* n1f, int* n4f)
+{
+    long double fkv[NNMAX + 2], fk2v[NNMAX + 1], gam3[NNMAX + 2], z12v[NNMAX + 1], z12vl[NNMAX + 1];
+    fkv[0] = 1; fk2v[0] = 1; gam3[0] = sqrt(M_PI) / 2.0; z12v[0] = 1.0; z12vl[0] = 1;
+    long double xln = 1.0;
+    double xrn = 1.0;
+    const double tiny = 1E-30;
+    e->norm = 1;
+    e->order = 0;
+    for (int n = 1; n <= NNMAX; ++n) {
+        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+        z12vl[n] = z12vl[n - 1] * ((zl + 1) - 2 + 2 * n) * ((zl + 1) - 1 + 2 * n);
+        fkv[n] = fkv[n - 1] * n;
+        fk2v[n] = fk2v[n - 1] * (2 * n - 1) * (2 * n);
+        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+        xln = -xln * xl2z;
+        xrn = -xrn * xr2z;
+        e->C.c1p[n] = sqrt(M_PI) * pow(4.0, n) * z12vl[n] * xln / (2.0 * (2 * n + 1) * (n + 1) * gam3[n] * fkv[n]);
+        e->C.c4p[n] = 4 * (n + 1 / 2.0) * fk2v[n] * z12v[n] * xrn /
+                      ((n + 2) * (n + 1) * fkv[n] * fkv[n] * (n + 1) * fkv[n] * fkv[n]);
+        double bs = 0.0;
+        for (int m = 0; m <= n; ++m) bs += z12vl[m] * z12vl[n - m] / (gam3[m] * gam3[n - m] * fkv[m] * fkv[n - m]);
+        e->C.c1f[n] = M_PI * xln * bs / (4.0 * (2 * n + 1));
+        bs = 0.0;
+        for (int m = 0; m <= n; ++m)
+            bs += z12v[m] * z12v[n - m] / ((m + 1) * fkv[m] * (n - m + 1) * fkv[n - m] * fkv[m] * fkv[n - m]);
+        e->C.c4f[n] = xrn * bs;
+        if (fabs(e->C.c1p[n]) < tiny && n < *n1) *n1 = n;
+        if (fabs(e->C.c1f[n]) < tiny && n < *n1f) *n1f = n;
+        if (fabs(e->C.c4p[n]) < tiny && n < *n4) *n4 = n;
+        if (fabs(e->C.c4f[n]) < tiny && n < *n4f) *n4f = n;
+    }
+}
+
 /* Cylinder, perfect (z-dir) orientation, homogeneous
  * (Source/sc_calc_generic_cpu.cpp:4124-4212 incl. the signed first-sign-change searches). */
 static void coef_cyl_zdir(csxo_engine* e, double z, double zl, double xl2z, double xr2z, int* n1,
@@ -642,6 +680,12 @@ static int coefficients(csxo_engine* e)
         done = 1;
     } else if (e->ordis == 6 && e->cdim == 4) {
         /* perfectly oriented cubes (:2880-2945) are not restated */
+    } else if (e->ordis == 7 && e->cdim == 1) {
+        /* short cylinders (L/R < 2, the exact-average branch :3592-3685) are not restated */
+        if (e->cs == 0 && e->length / e->radius >= 2.0) {
+            coef_cyl_iso(e, z, zl, xl2z, xr2z, &nidx[0], &nidx[3], &nidx[9], &nidx[12]);
+            done = 1;
+        }
     } else if (e->ordis == 6 && e->cdim != 3) {
         if (e->cs == 0) { coef_cyl_zdir(e, z, zl, xl2z, xr2z, &nidx[0], &nidx[3], &nidx[9], &nidx[12]); done = 1; }
     } else if (e->ordis == 0 && e->cdim != 3) {
@@ -1113,7 +1157,24 @@ static double pq_cylinder(const csxo_engine* e, double qxs, double qys, double q
     const double zl = e->sigmal == 1.0 ? kEps9 : (1 - sq(e->sigmal)) / sq(e->sigmal);
     const double zr = e->sigma == 1.0 ? kEps9 : (1 - sq(e->sigma)) / (sq(e->sigma));
     double pql = 0, pqr;
-    if (e->ordis == 6) {
+    if (e->ordis == 7) { /* isotropic, factorisation (L/R >= 2; Source/sc_lib_formpq_partCylinder.h:30-90) */
+        if (q < (0.6 * e->limq[0])) {
+            pql = 1.0;
+            double old = 0.0, qq = 1.0;
+            for (int n = 1; n <= 120; ++n) {
+                qq = qq * q * q;
+                pql += e->C.c1p[n] * qq;
+                const double d = fabs((pql - old) / pql);
+                if (d < 0.0001) break;
+                old = pql;
+            }
+        } else {
+            const double arglq = q * e->length / (zl + 1);
+            pql = (M_PI / (2.0 * zl)) * (1 / arglq);
+            pql = pql - (1 / (2.0 * zl * (zl - 1) * arglq * arglq)) * cos((zl - 1) * atan(2.0 * arglq)) /
+                            pow(1.0 + 4 * arglq * arglq, (zl - 1) / 2.0);
+        }
+    } else if (e->ordis == 6) {
         if (orc_is(e, 4)) pql = 1.0;
         else if (q < (0.6 * e->limq[0])) {
             pql = 1.0;
@@ -1310,6 +1371,11 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         case ORC_GENERAL: pq = pq_cylinder(e, qx * e->cosphic - qy * e->sinphic, qx * e->sinphic + qy * e->cosphic, q); break;
         default: pq = pq_cylinder(e, qx, qy, q); break;
         }
+        *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+        return 0;
+    }
+    case 0x01000C07u: { /* calc_GENERIC, cylinder, isotropic (Source/sc_gpu_generic.h:58-66, 868-880) */
+        const double pq = pq_cylinder(e, qx, qy, q);
         *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }

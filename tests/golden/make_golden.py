@@ -106,7 +106,10 @@ LATTICES = [("bcc_sphere", dict(CFG3, LType=4, uca=30, ucb=30, ucc=30, GridPoint
             ("sc_sphere_iso", dict(CFG3, LType=7, uca=25, ucb=25, ucc=25, Ordis=7, GridPoints=32)),
             ("fcc_sphere_iso", dict(CFG3, Ordis=7, GridPoints=32)),
             ("bct_sphere", dict(CFG3, LType=8, uca=30, ucb=30, ucc=45, GridPoints=32)),
-            ("sphere_zdir", dict(CFG1, Ordis=6, GridPoints=32))]
+            ("sphere_zdir", dict(CFG1, Ordis=6, GridPoints=32)),
+            ("cyl_iso", dict(CFG2, Ordis=7, EditRadius=3, Length=40, GridPoints=32, EditQmax=2)),
+            ("cyl_iso_b", dict(CFG2, Ordis=7, EditRadius=5, Length=12, EditSigma=0.1, SigmaL=0.1, GridPoints=32,
+                               EditQmax=3, iso=0.2, I0=3, Base=0.1))]
 
 
 def write_lattices(manifest):

@@ -318,10 +318,10 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
 
 @pytest.mark.parametrize("prec,tol", [("ref64", 1e-13), ("mixed", 1e-4)])
 @pytest.mark.parametrize("name", ["bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz",
-                                  "sphere_zdir.npz"])
-def test_sphere_lattices_generic_keys(name, prec, tol):
-    """Spheres on BCC and SC lattices and with isotropic orientation (the reference's generic
-    kernel) against its own output."""
+                                  "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz"])
+def test_generic_kernel_keys(name, prec, tol):
+    """Combinations of the reference's generic kernel served here (spheres on BCC, SC and BCT
+    lattices, other orientation distributions, isotropic cylinders) against its own output."""
     f = fixture(name)
     e, img = _calc(params(name), prec)
     r = rel_err(img, f["image"])

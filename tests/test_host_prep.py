@@ -18,7 +18,8 @@ def _host(p, ignore_cbs=True):
 
 
 @pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "bcc_sphere.npz",
-                                  "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz", "sphere_zdir.npz"])
+                                  "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz", "sphere_zdir.npz",
+                                  "cyl_iso.npz", "cyl_iso_b.npz"])
 def test_tables_match_reference(name):
     f = fixture(name)
     e = _host(params(name))
@@ -108,4 +109,7 @@ def test_unsupported_combination_is_an_error():
         e.prepare()
     e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=4, Ordis=6))  # oriented cube
     with pytest.raises(CsxError):
+        e.prepare()
+    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=1, Ordis=7, EditRadius=5, Length=8))  # L/R < 2
+    with pytest.raises(CsxError, match="L/R"):
         e.prepare()
