@@ -35,7 +35,7 @@ constexpr double kLog2e = 1.4426950408889634074;
 // K_CYL_GAUSS_Q: Gaussian cylinders in physics mode with the FP32 orientation quadrature
 // (mixed precision only; chosen by csx_render, never by the key alone).
 enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4, K_CYL_GAUSS_Q = 5, K_CUBE = 6,
-            K_CYL_ISO = 7 };
+            K_CYL_ISO = 7, K_CUBE_ZDIR = 8 };
 
 inline int kind_of(uint32_t key) {
     switch (key) {
@@ -55,6 +55,7 @@ inline int kind_of(uint32_t key) {
     case CSX_KEY_CYL_BCT_ZDIR: return K_BCT;
     case CSX_KEY_CUBE_NONE_ISO: return K_CUBE;
     case CSX_KEY_CYL_NONE_ISO: return K_CYL_ISO;
+    case CSX_KEY_CUBE_NONE_ZDIR: return K_CUBE_ZDIR;
     default: return -1;
     }
 }
@@ -840,6 +841,10 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
     } else if constexpr (KIND == K_CUBE) {
         // Source/sc_gpu_pCub_lNon_oIso.h:14-57: no lattice, iso added to P(q) unweighted
         const double pq = rv ? rv->pq : pq_cube(d, q);
+        return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
+    } else if constexpr (KIND == K_CUBE_ZDIR) {
+        // calc_GENERIC, cubes, perfect orientation (Source/sc_gpu_generic.h:307-314, 868-880)
+        const double pq = pq_cube_zdir(d, qx, qy, q);
         return s.base + s.i0 * (pq + s.iso) + s.ifluc / (1 + q * q * s.rfluc * s.rfluc);
     } else if constexpr (KIND == K_CYL_ISO) {
         // calc_GENERIC, cylinders, isotropic (Source/sc_gpu_generic.h:58-66, 868-880)
@@ -1728,6 +1733,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     case K_CYL_GAUSS_Q: FN<K_CYL_GAUSS_Q, 1>(__VA_ARGS__); break; \
     case K_CUBE: if (prec) FN<K_CUBE, 1>(__VA_ARGS__); else FN<K_CUBE, 0>(__VA_ARGS__); break; \
     case K_CYL_ISO: if (prec) FN<K_CYL_ISO, 1>(__VA_ARGS__); else FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
+    case K_CUBE_ZDIR: if (prec) FN<K_CUBE_ZDIR, 1>(__VA_ARGS__); else FN<K_CUBE_ZDIR, 0>(__VA_ARGS__); break; \
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
@@ -1857,6 +1863,9 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
     int rc = check_scalars(c, s);
     if (rc) return rc;
     if (!h_carr) return fail(c, CSX_E_ARG, "null coefficient table");
+    const int k = kind_of(s->combo_key);
+    if (!h_carr11pm && (k == K_CYL_GAUSS || k == K_CUBE_ZDIR))
+        return fail(c, CSX_E_ARG, "this combination reads the carr11pm table: pass it");
     HIPCHK(c, hipSetDevice(c->device));
     // a radial pre-pass still running reads the tables about to be replaced
     HIPCHK(c, hipStreamSynchronize(c->side));
@@ -2032,6 +2041,9 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
     };
     if (!s || !h_carr || !mask || !d_data || w <= 0 || h <= 0) return bad(CSX_E_ARG, "null argument or empty fit array");
     if (kind_of(s->combo_key) < 0) return bad(CSX_E_ARG, "combination key not supported by the device path");
+    // the batch carries no carr11pm table: kinds that read it render and fit one set at a time
+    if (kind_of(s->combo_key) == K_CYL_GAUSS || kind_of(s->combo_key) == K_CUBE_ZDIR)
+        return bad(CSX_E_ARG, "this combination needs carr11pm, which batches do not carry");
     if (s->peakmax1 < 0 || s->peakmax2 < 0 || s->peakmax1 >= CSX_LATLEN || s->peakmax2 >= CSX_LATLEN)
         return bad(CSX_E_ARG, "peakmax out of range");
     if (s->peakmax1 > s->peakmax2 && s->peakmax1 > 0)

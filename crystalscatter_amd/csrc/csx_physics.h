@@ -169,6 +169,39 @@ __device__ inline double pq_cube(const DevSet& d, double q) {
     return s.por / (q * q * q * q);
 }
 
+// Perfectly oriented cube (Source/sc_lib_formpq_partCube.h:70-106): below q = 1/R the double
+// series sum_n sum_m carr11pm[m][n-m] qx^2(n-m) qy^2m with the powers built as running
+// products (kept in private arrays, as the reference indexes them), else the product of the
+// two edge factors.
+__device__ inline double pq_cube_zdir(const DevSet& d, double qxs, double qys, double q) {
+    const csx_scalars& s = d.s;
+    const double zr = (1 - sqd(s.sigma)) / (sqd(s.sigma));
+    if (q < (1.0 / s.radius)) {
+        double qxn[81], qyn[81];
+        double pql = 1.0, old = 0.0;
+        qxn[0] = 1.0;
+        qyn[0] = 1.0;
+        for (int n = 1; n <= 80; ++n) {
+            qxn[n] = qxn[n - 1] * qxs * qxs;
+            qyn[n] = qyn[n - 1] * qys * qys;
+            double bs = 0.0;
+            for (int m = 0; m <= n; ++m) bs += d.carr11pm[m * CSX_NIM2 + (n - m)] * qxn[n - m] * qyn[m];
+            pql += bs;
+            const double del = fabs((pql - old) / pql);
+            if (del < 0.0001) break;
+            old = pql;
+        }
+        return pql;
+    }
+    const double ax = qxs * s.radius / (zr + 1) + 0.000000001;
+    const double ay = qys * s.radius / (zr + 1) + 0.000000001;
+    const double px = (1 / (2.0 * zr * (zr - 1))) * (1 / (ax * ax)) *
+                      (1 - cos((zr - 1) * atan(2.0 * ax)) / pow(1.0 + 4 * ax * ax, (zr - 1) / 2.0));
+    const double py = (1 / (2.0 * zr * (zr - 1))) * (1 / (ay * ay)) *
+                      (1 - cos((zr - 1) * atan(2.0 * ay)) / pow(1.0 + 4 * ay * ay, (zr - 1) / 2.0));
+    return px * py;
+}
+
 __device__ inline double fq_sphere(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);

@@ -262,6 +262,31 @@ struct FSums {
     double acc1[kNN + 1], acc4[kNN + 1];
 };
 
+// Cube, perfect orientation, homogeneous (Source/sc_calc_generic_cpu.cpp:2880-2910): the
+// double-series table carr11pm[n][m], m <= n (the other entries keep their initial 1, which the
+// pixel reads as well), and carr4p/carr4f.
+void coef_cube_zdir(PreparedSet& P, double z, double xr2z, CoefState& cs) {
+    long double fkv[kNN + 2], gam3[kNN + 2], z12v[kNN + 1];
+    fkv[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12v[0] = 1.0;
+    long double xrn = 1.0;
+    double* pm = P.carr11pm.data();
+    double* c4p = cs.arr[3];
+    double* c4f = cs.arr[12];
+    for (int n = 1; n <= kNN; ++n) {
+        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+        fkv[n] = fkv[n - 1] * n;
+        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+        xrn = -xrn * xr2z;
+        for (int m = 0; m <= n; ++m)
+            pm[n * CSX_NIM2 + m] = (kPi / 4.0) * std::pow(4.0, n) * z12v[n - m] * z12v[m] * xrn /
+                                   (gam3[n - m] * gam3[m] * (n - m + 1) * fkv[n - m] * (m + 1) * fkv[m]);
+        c4p[n] = std::sqrt(kPi) * std::pow(4.0, n) * xrn / (16.0 * gam3[n]);
+        c4f[n] = kPi * kPi * xrn / (128.0 * gam3[n]);
+        if (std::fabs(c4p[n]) < 1E-30 && n < cs.nidx[3]) cs.nidx[3] = n;
+        if (std::fabs(c4f[n]) < 1E-30 && n < cs.nidx[12]) cs.nidx[12] = n;
+    }
+}
+
 // Cylinder, isotropic, homogeneous, L/R >= 2 (the factorisation branch,
 // Source/sc_calc_generic_cpu.cpp:3686-3746): P|| and P- series and the <F>^2 series.
 void coef_cylinder_iso(double z, double zl, double xl2z, double xr2z, CoefState& cs) {
@@ -751,7 +776,15 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     s.norm = 1;
     bool done = false;
     s.cs = p.interior;
-    if (p.ordis == 7 && P.cdim == 4) {
+    if (p.ordis == 6 && P.cdim == 4) {
+        // perfectly oriented cubes: homogeneous only (the reference leaves the form factor of
+        // the other interiors undefined)
+        if (p.interior != 0) return "oriented cubes need a homogeneous interior";
+        P.order = 1;
+        P.carr11pm.assign(CSX_NIM2 * CSX_NIM2, 1.0);
+        coef_cube_zdir(P, z, xr2z, cs);
+        done = true;
+    } else if (p.ordis == 7 && P.cdim == 4) {
         // Isotropic cubes: only the homogeneous interior has coefficients; the pixel of any
         // other interior uses P(q) = 1 (Source/sc_gpu_pCub_lNon_oIso.h:37-45).
         P.order = 0;

@@ -56,6 +56,7 @@ enum {
 #define CSX_KEY_CYL_BCT_ZDIR      0x01020806u
 #define CSX_KEY_CUBE_NONE_ISO     0x04000C07u
 #define CSX_KEY_CYL_NONE_ISO      0x01000C07u  /* generic kernel; L/R >= 2 */
+#define CSX_KEY_CUBE_NONE_ZDIR    0x04000C06u  /* generic kernel; homogeneous */
 /* Spheres with Gaussian peaks on other cubic lattices and with isotropic orientation (the
  * reference's generic kernel, Source/sc_gpu_generic.h; same sums as the FCC kernel). */
 #define CSX_KEY_SPHERE_FCC_ISO    0x00020507u

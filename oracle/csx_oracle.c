@@ -481,6 +481,31 @@ static void coef_cube(csxo_engine* e, double z, double xr2z, int* n4, int* n4f)
     }
 }
 
+/* Cube, perfectly oriented (z-dir), homogeneous (Source/sc_calc_generic_cpu.cpp:2880-2910):
+ * carr11pm[n][m] for m <= n (the rest keeps its initial 1), carr4p and carr4f. */
+static void coef_cube_zdir(csxo_engine* e, double z, double xr2z, int* n4, int* n4f)
+{
+    long double fkv[NNMAX + 2], gam3[NNMAX + 2], z12v[NNMAX + 1];
+    fkv[0] = 1; gam3[0] = sqrt(M_PI) / 2.0; z12v[0] = 1.0;
+    long double xrn = 1.0;
+    const double tiny = 1E-30;
+    e->norm = 1;
+    e->order = 1;
+    for (int n = 1; n <= NNMAX; ++n) {
+        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+        fkv[n] = fkv[n - 1] * n;
+        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+        xrn = -xrn * xr2z;
+        for (int m = 0; m <= n; ++m)
+            e->C.c11pm[n][m] = (M_PI / 4.0) * pow(4.0, n) * z12v[n - m] * z12v[m] * xrn /
+                               (gam3[n - m] * gam3[m] * (n - m + 1) * fkv[n - m] * (m + 1) * fkv[m]);
+        e->C.c4p[n] = sqrt(M_PI) * pow(4.0, n) * xrn / (16.0 * gam3[n]);
+        e->C.c4f[n] = M_PI * M_PI * xrn / (128.0 * gam3[n]);
+        if (fabs(e->C.c4p[n]) < tiny && n < *n4) *n4 = n;
+        if (fabs(e->C.c4f[n]) < tiny && n < *n4f) *n4f = n;
+    }
+}
+
 /* Cylinder, isotropic, homogeneous, axial ratio L/R >= 2 (Source/sc_calc_generic_cpu.cpp:3578-3583,
  * 3686-3746, the factorisation branch): P|| and P- series, <F>^2 series, with the 1e-30 switch
  * points.  z12v, z12vl, fkv, fk2v, gam3 and xln are long double, xrn double, as declared there. */
@@ -679,7 +704,9 @@ static int coefficients(csxo_engine* e)
         if (e->cs == 0) coef_cube(e, z, xr2z, &nidx[3], &nidx[12]);
         done = 1;
     } else if (e->ordis == 6 && e->cdim == 4) {
-        /* perfectly oriented cubes (:2880-2945) are not restated */
+        /* perfectly oriented cubes: the homogeneous interior only (the pixel of the others is
+         * undefined in the reference) */
+        if (e->cs == 0) { coef_cube_zdir(e, z, xr2z, &nidx[3], &nidx[12]); done = 1; }
     } else if (e->ordis == 7 && e->cdim == 1) {
         /* short cylinders (L/R < 2, the exact-average branch :3592-3685) are not restated */
         if (e->cs == 0 && e->length / e->radius >= 2.0) {
@@ -1121,6 +1148,39 @@ static double pq_cube(const csxo_engine* e, double q)
     return e->por / (q * q * q * q);
 }
 
+/* formpq_partCube, perfect orientation (Source/sc_lib_formpq_partCube.h:70-106): the double
+ * series over carr11pm below q = 1/R (read as carr11pm[m][n-m]), else the product of the two
+ * edge factors. */
+static double pq_cube_zdir(const csxo_engine* e, double qxs, double qys, double q)
+{
+    const double zr = (1 - sq(e->sigma)) / (sq(e->sigma));
+    double pql;
+    if (q < (1.0 / e->radius)) {
+        double qxn[121], qyn[121], old = 0.0;
+        pql = 1.0;
+        qxn[0] = 1.0; qyn[0] = 1.0;
+        for (int n = 1; n <= 80; ++n) {
+            qxn[n] = qxn[n - 1] * qxs * qxs;
+            qyn[n] = qyn[n - 1] * qys * qys;
+            double bs = 0.0;
+            for (int m = 0; m <= n; ++m) bs += e->C.c11pm[m][n - m] * qxn[n - m] * qyn[m];
+            pql += bs;
+            const double d = fabs((pql - old) / pql);
+            if (d < 0.0001) break;
+            old = pql;
+        }
+    } else {
+        const double ax = qxs * e->radius / (zr + 1) + kEps9;
+        const double ay = qys * e->radius / (zr + 1) + kEps9;
+        const double px = (1 / (2.0 * zr * (zr - 1))) * (1 / (ax * ax)) *
+                          (1 - cos((zr - 1) * atan(2.0 * ax)) / pow(1.0 + 4 * ax * ax, (zr - 1) / 2.0));
+        const double py = (1 / (2.0 * zr * (zr - 1))) * (1 / (ay * ay)) *
+                          (1 - cos((zr - 1) * atan(2.0 * ay)) / pow(1.0 + 4 * ay * ay, (zr - 1) / 2.0));
+        pql = px * py;
+    }
+    return pql;
+}
+
 /* formfq_partSphere, cs=0 (Source/sc_lib_formfq_partSphere.h:8-52). */
 static double fq_sphere(const csxo_engine* e, double q)
 {
@@ -1342,6 +1402,11 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
     case 0x00000C07u: { /* Source/sc_gpu_pSph_lNon_oIso.h:15-51 */
         const double pq = pq_sphere(e, q);
         *out = e->base + e->i0 * (pq + e->iso * pq) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+        return 0;
+    }
+    case 0x04000C06u: { /* calc_GENERIC, cube, perfect orientation (Source/sc_gpu_generic.h:307-314, 868-880) */
+        const double pq = pq_cube_zdir(e, qx, qy, q);
+        *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }
     case 0x04000C07u: { /* Source/sc_gpu_pCub_lNon_oIso.h:14-57 */

@@ -109,14 +109,19 @@ LATTICES = [("bcc_sphere", dict(CFG3, LType=4, uca=30, ucb=30, ucc=30, GridPoint
             ("sphere_zdir", dict(CFG1, Ordis=6, GridPoints=32)),
             ("cyl_iso", dict(CFG2, Ordis=7, EditRadius=3, Length=40, GridPoints=32, EditQmax=2)),
             ("cyl_iso_b", dict(CFG2, Ordis=7, EditRadius=5, Length=12, EditSigma=0.1, SigmaL=0.1, GridPoints=32,
-                               EditQmax=3, iso=0.2, I0=3, Base=0.1))]
+                               EditQmax=3, iso=0.2, I0=3, Base=0.1)),
+            ("cube_zdir", dict(CUBE_A, Ordis=6)),
+            ("cube_zdir_b", dict(CUBE_B, Ordis=6))]
 
 
 def write_lattices(manifest):
     for name, p in LATTICES:
         r = run_ref(p)
+        extra = {}
+        if int(p.get("ComboBoxParticle", 0)) == 4 and int(p.get("Ordis", 7)) == 6:
+            extra["carr11pm"] = r["carr11pm"].reshape(131, 131)[:121, :121].copy()
         np.savez_compressed(os.path.join(HERE, name + ".npz"), image=r["image"], bounds=r["bounds"],
-                            **table_dict(r, True))
+                            **table_dict(r, True), **extra)
         manifest["fixtures"][name + ".npz"] = {"params": p}
 
 

@@ -4,7 +4,7 @@ feeding the batch against the reference's own run (tests/golden/tpv_fcc.npz)."""
 import numpy as np
 import pytest
 
-from helpers import CFG3, fixture, manifest, rel_err
+from helpers import CFG3, fixture, manifest, params, rel_err
 
 from crystalscatter_amd import Context, Random, SasCalc, render_batch, tpv_images
 
@@ -156,3 +156,18 @@ def test_ai_pipeline_matches_the_oracle(oracle_lib, opts_kw):
         assert np.array_equal(np.isfinite(scat[k]), ok) and np.array_equal(scat[k] == 0, img == 0)
         assert rel_err(scat[k][ok], img[ok]).max() < 1e-12, k
         assert np.abs(rphi[k] - rp).max() < 1e-10, k
+
+
+def test_batch_rejects_sets_that_need_carr11pm():
+    """Gaussian-oriented cylinders and oriented cubes read the carr11pm table, which a batch
+    does not carry: staging them is an error on the host (nothing is launched)."""
+    import torch
+    from crystalscatter_amd import CsxError
+    ctx = Context(0)
+    out = torch.empty(64 * 64, dtype=torch.float64, device="cuda")
+    for p in (dict(params("cfg2_cylgauss.npz"), GridPoints=32), dict(CUBE, Ordis=6, GridPoints=32)):
+        e = SasCalc(device=-1, params=p)
+        e.set("ignore_cbs", 1)
+        e.prepare()
+        with pytest.raises(CsxError, match="carr11pm"):
+            e.stage_image(ctx, out.data_ptr(), out.numel())

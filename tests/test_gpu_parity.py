@@ -318,7 +318,8 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
 
 @pytest.mark.parametrize("prec,tol", [("ref64", 1e-13), ("mixed", 1e-4)])
 @pytest.mark.parametrize("name", ["bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz",
-                                  "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz"])
+                                  "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz",
+                                  "cube_zdir_b.npz"])
 def test_generic_kernel_keys(name, prec, tol):
     """Combinations of the reference's generic kernel served here (spheres on BCC, SC and BCT
     lattices, other orientation distributions, isotropic cylinders) against its own output."""

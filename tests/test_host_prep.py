@@ -19,7 +19,7 @@ def _host(p, ignore_cbs=True):
 
 @pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "bcc_sphere.npz",
                                   "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz", "sphere_zdir.npz",
-                                  "cyl_iso.npz", "cyl_iso_b.npz"])
+                                  "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz", "cube_zdir_b.npz"])
 def test_tables_match_reference(name):
     f = fixture(name)
     e = _host(params(name))
@@ -35,7 +35,7 @@ def test_tables_match_reference(name):
         np.testing.assert_array_equal(e.table("latpar3").reshape(-1, 17)[1:, 1:14], f["latpar3"][1:, 1:14])
         sphno = {4: 2, 5: 4, 7: 1, 8: 2}[int(params(name)["LType"])]
         assert s.sphno_cubevol == f["ucvol"][0] * sphno
-    if name == "cfg2_cylgauss.npz":
+    if "carr11pm" in f:
         np.testing.assert_array_equal(e.table("carr11pm").reshape(131, 131)[:121, :121], f["carr11pm"])
 
 
@@ -107,8 +107,8 @@ def test_unsupported_combination_is_an_error():
     e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=5, Ordis=7))  # ellipsoid
     with pytest.raises(CsxError):
         e.prepare()
-    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=4, Ordis=6))  # oriented cube
-    with pytest.raises(CsxError):
+    e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=4, Ordis=6, ComboBoxInterior=1))  # core/shell
+    with pytest.raises(CsxError, match="homogeneous"):                                           # oriented cube
         e.prepare()
     e = SasCalc(device=-1, params=dict(LType=12, ComboBoxParticle=1, Ordis=7, EditRadius=5, Length=8))  # L/R < 2
     with pytest.raises(CsxError, match="L/R"):
