@@ -218,6 +218,23 @@ class Random:
             raise CsxError("csx_rand_zuf failed")
         return out
 
+    def skip(self, n):
+        """Advance by n rand() values without drawing them."""
+        if self._L.csx_rand_skip(self._h, int(n)) != 0:
+            raise CsxError("csx_rand_skip failed")
+
+    def zuf_device(self, ctx, out):
+        """The next out.numel() zuf() values into a contiguous float64 CUDA tensor, drawn on
+        the device (csx_rand_zuf_device); equal to zuf(out.numel())."""
+        if out.dtype != torch_float64() or not out.is_cuda or not out.is_contiguous():
+            raise CsxError("zuf_device needs a contiguous float64 CUDA tensor")
+        import torch
+        torch.cuda.current_stream().synchronize()
+        ctx._chk(self._L.csx_rand_zuf_device(ctx._h, self._h, out.numel(), ctypes.c_void_p(out.data_ptr())),
+                 "csx_rand_zuf_device")
+        ctx.synchronize()
+        return out
+
 
 class Context:
     """Device context of the C ABI (csx_create): upload / render / batched objective."""

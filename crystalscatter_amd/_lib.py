@@ -118,8 +118,11 @@ SIGNATURES = [
     ("csx_rand_free", None, [_VP]),
     ("csx_rand_next", ctypes.c_int32, [_VP]),
     ("csx_rand_zuf", _I, [_VP, _PD, ctypes.c_int64]),
+    ("csx_rand_skip", _I, [_VP, ctypes.c_int64]),
+    ("csx_rand_zuf_device", _I, [_VP, _VP, ctypes.c_int64, _VP]),
     ("csx_engine_tpv", _I, [_VP, _CP, _VP, _CP, _I]),
     ("csx_engine_tpv_value", _D, [_VP, _CP]),
+    ("csx_engine_tpv_batch", _I, [_VP, _VP, _CP, _VP, _I, _VP, ctypes.c_int64, _VP, _I, _PD, _I]),
     ("csx_post_noise", _I, [_VP, _VP, _VP, ctypes.c_int64, _I]),
     ("csx_post_zero", _I, [_VP, _VP, _VP, ctypes.c_int64]),
     ("csx_post_scale_intensity", _I, [_VP, _VP, ctypes.c_int64, _I, _I]),

@@ -4,11 +4,12 @@ Source/sc_mainCons.cpp:916-1440) for one parameter line, batched.
 Per image the reference runs: tpvPerformRandom, the calculation, then optionally
 scaleIntensity (linear), noise d + z sqrt(d), a randomly placed beam stop and random lines of
 zeroed pixels, and the r,phi resampling of the (log-scaled) image with its log rescaling.  All
-random numbers come from one rand() sequence in that order, image after image.  Here the host
-walks the images in the same order -- random variation, staging, noise values, beam stop and
-line positions -- and the device then renders all images in one launch and post-processes them
-batch-wide (csx_render_batch_run, csx_post_*), so the images equal a reference run that starts
-from the same rand() state.
+random numbers come from one rand() sequence in that order, image after image.  Here one
+native call (csx_engine_tpv_batch) draws them in the same order -- random variation, noise
+values, beam-stop and line values -- and prepares and stages the images on host threads; the
+device then renders all images in one launch and post-processes them batch-wide
+(csx_render_batch_run, csx_post_*), so the images equal a reference run that starts from the
+same rand() state.
 
 Options (names after the reference's command-line settings, Source/sc_mainCons.cpp:1000-1120):
 scale_scat (tpvScaleScat), noise (tpvAddNoise), bs = (x0, y0, xs, ys) factors (tpvBSx0..ys),
@@ -51,13 +52,14 @@ class AIOptions:
     fft_cut: tuple = (0, 0)    # fftOutCutX, fftOutCutY
 
 
-def _zuf(rng):
-    return float(rng.zuf(1)[0])
+def _n_extra(opts):
+    """zuf() values the beam stop and the lines of one image draw."""
+    return (4 if opts.bs[0] > 0 else 0) + max(0, opts.lines_x) + max(0, opts.lines_y)
 
 
-def _zero_indices(rng, opts, bounds, beam):
-    """Pixels the beam stop and the lines zero, and the (random) beam-stop centre, drawing
-    from rng in the reference's order (Source/sc_mainCons.cpp:1298-1348).  Indices follow the
+def _zero_indices(draw, opts, bounds, beam):
+    """Pixels the beam stop and the lines zero, and the (random) beam-stop centre; draw()
+    gives the next zuf() value in the reference's order (Source/sc_mainCons.cpp:1298-1348).  Indices follow the
     reference's ii = (x - minX) + sx (y - minY), kept when 0 <= ii < len (a line past the
     right edge therefore continues on the next row, as there)."""
     x0, x1, y0, y1 = bounds
@@ -67,10 +69,10 @@ def _zero_indices(rng, opts, bounds, beam):
     bsdx = bsdy = 2
     idx = []
     if opts.bs[0] > 0:
-        bsx0 = int(bsx0 * (1.0 + opts.bs[0] * _zuf(rng)))
-        bsy0 = int(bsy0 * (1.0 + opts.bs[1] * _zuf(rng)))
-        bsdx = int(bsdx * (1.0 + opts.bs[2] * _zuf(rng)))
-        bsdy = int(bsdy * (1.0 + opts.bs[3] * _zuf(rng)))
+        bsx0 = int(bsx0 * (1.0 + opts.bs[0] * draw()))
+        bsy0 = int(bsy0 * (1.0 + opts.bs[1] * draw()))
+        bsdx = int(bsdx * (1.0 + opts.bs[2] * draw()))
+        bsdy = int(bsdy * (1.0 + opts.bs[3] * draw()))
         for x in range(x0, x1):
             if bsx0 - bsdx <= x <= bsx0 + bsdx:
                 for y in range(y0, y1):
@@ -79,14 +81,14 @@ def _zero_indices(rng, opts, bounds, beam):
     if opts.lines_x > 0:
         xc = (x1 - x0 + 1) // opts.lines_x
         for i in range(opts.lines_x):
-            xp = int(x0 + i * xc + math.fabs(_zuf(rng)) * xc)
+            xp = int(x0 + i * xc + math.fabs(draw()) * xc)
             for x in range(xp, xp + opts.lines_x_width):
                 for y in range(y0, y1):
                     idx.append((x - x0) + sx * (y - y0))
     if opts.lines_y > 0:
         yc = (y1 - y0 + 1) // opts.lines_y
         for i in range(opts.lines_y):
-            yp = int(y0 + i * yc + math.fabs(_zuf(rng)) * yc)
+            yp = int(y0 + i * yc + math.fabs(draw()) * yc)
             for y in range(yp, yp + opts.lines_y_width):
                 for x in range(x0, x1):
                     idx.append((x - x0) + sx * (y - y0))
@@ -94,11 +96,13 @@ def _zero_indices(rng, opts, bounds, beam):
     return ii, (bsx0, bsy0)
 
 
-def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed"):
+def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16):
     """n training images of engine's parameters with random variation `ids` (see
     SasCalc.tpvPerformRandom), post-processed per `opts`.  Returns a dict with "scat"
-    ((n, npx) float64 CUDA tensor, IDX order), "rphi" ((n, s*s), when rphi_size > 0),
-    "bounds", "centres" (the beam-stop centre per image) and "reports"."""
+    ((n, npx) float64 CUDA tensor, IDX order), "rphi" ((n, s*s), when rphi_size > 0), "fft"
+    (when fft_size > 0), "bounds" and "centres" (the beam-stop centre per image).  The
+    parameter values of each image are those tpvPerformRandom gives (see SasCalc.tpv_value
+    for one call at a time)."""
     import torch
     opts = opts or AIOptions()
     if opts.fft_size and opts.fft_size != opts.rphi_size:
@@ -111,26 +115,32 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed"):
     npx = (x1 - x0) * (y1 - y0)
     beam = (engine.get("BeamPosX"), engine.get("BeamPosY"))
     scat = torch.empty((n, npx), dtype=torch.float64, device="cuda")
-    z = np.empty((n, npx)) if opts.noise else None
-    zero_idx, centres, reports = [], [], []
+    zt = torch.empty((n, npx), dtype=torch.float64, device="cuda") if opts.noise else None
+    _torch_done()
+    nx = _n_extra(opts)
+    extra = np.empty((n, max(nx, 1)))
     ctx._chk(L.csx_fit_batch_begin(ctx._h), "csx_fit_batch_begin")
+    # random variation, noise (drawn on the device) and beam-stop/line values in the loop's
+    # order, preparation and staging on host threads (csx_engine_tpv_batch)
+    pd = ctypes.POINTER(ctypes.c_double)
+    rc = L.csx_engine_tpv_batch(engine._h, ctx._h, "\n".join(ids).encode(), rng._h, n,
+                                ctypes.c_void_p(scat.data_ptr()), npx,
+                                ctypes.c_void_p(zt.data_ptr()) if zt is not None else None, nx,
+                                extra.ctypes.data_as(pd) if nx else None, int(threads))
+    if rc != 0:
+        raise CsxError("csx_engine_tpv_batch: " + engine._err())
+    zero_idx, centres = [], []
     for k in range(n):
-        reports.append(engine.tpvPerformRandom(ids, rng))
-        engine.stage_image(ctx, scat[k].data_ptr(), npx)
-        if opts.noise:
-            z[k] = rng.zuf(npx)
-        ii, c = _zero_indices(rng, opts, bounds, beam)
+        vals = iter(extra[k, :nx])
+        ii, c = _zero_indices(lambda: float(next(vals)), opts, bounds, beam)
         zero_idx.append(ii + k * npx)
         centres.append(c)
     ctx._chk(L.csx_render_batch_run(ctx._h, _prec(precision)), "csx_render_batch_run")
     d = ctypes.c_void_p(scat.data_ptr())
     if opts.scale_scat:
         ctx._chk(L.csx_post_scale_intensity(ctx._h, d, npx, n, 1), "csx_post_scale_intensity")
-    keep = []
+    keep = [zt]
     if opts.noise:
-        zt = torch.from_numpy(z.reshape(-1)).cuda()
-        keep.append(zt)
-        _torch_done()
         ctx._chk(L.csx_post_noise(ctx._h, d, ctypes.c_void_p(zt.data_ptr()), n * npx, int(opts.scale_scat)),
                  "csx_post_noise")
     allz = np.concatenate(zero_idx) if zero_idx else np.zeros(0, np.int64)
@@ -139,7 +149,7 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed"):
         keep.append(it)
         _torch_done()
         ctx._chk(L.csx_post_zero(ctx._h, d, ctypes.c_void_p(it.data_ptr()), it.numel()), "csx_post_zero")
-    out = {"scat": scat, "bounds": bounds, "centres": centres, "reports": reports}
+    out = {"scat": scat, "bounds": bounds, "centres": centres}
     if opts.rphi_size > 0:
         s = int(opts.rphi_size)
         src = scat

@@ -4,34 +4,29 @@
 // sc_simplexfit2d (Source/sc_calc_generic_gpu.h:108-598): parameters by key,
 // doCalculation, setFitData / setNoFitRect / doFitCalculation, prepared tables.
 // Host preparation runs here in C++; every pixel runs on the GPU.
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <hip/hip_runtime.h>
 
 #include "csx.h"
 #include "csx_prep.h"
-
-// The C library's rand() sequence (glibc: the additive feedback generator of random(), 31
-// words of state) with the state held here, so that a seeded generator reproduces what
-// srand(seed) and rand() give a reference run without touching the process-wide state.
-struct csx_rand {
-    random_data rd;
-    char state[128];
-};
+#include "csx_rand.h"
 
 namespace {
 
 // zuf() of the reference (Source/sc_calc_generic_cpu.cpp:1085, Source/sc_mainCons.cpp:850-853):
 // uniform on [-1, 1] from one rand() value.
 double zuf(csx_rand* r) {
-    int32_t v = 0;
-    random_r(&r->rd, &v);
-    return ((static_cast<double>(v) / RAND_MAX * 200.0) - 100.0) / 100.0;
+    const int32_t v = csr::value(csr::step(r->s));
+    return ((static_cast<double>(v) / 2147483647.0 * 200.0) - 100.0) / 100.0;
 }
 
 // The variables tpvPerformRandom can vary (Source/sc_calc_generic_cpu.cpp:1057-1081, in its
@@ -253,17 +248,24 @@ const void* csx_engine_table(csx_engine* e, const char* name, int64_t* n, int* k
 
 csx_rand* csx_rand_new(uint32_t seed) {
     csx_rand* r = new csx_rand();
-    std::memset(&r->rd, 0, sizeof r->rd);
-    if (initstate_r(seed, r->state, sizeof r->state, &r->rd) != 0) { delete r; return nullptr; }
+    csr::seed(r->s, seed);
     return r;
 }
 
 void csx_rand_free(csx_rand* r) { delete r; }
 
-int32_t csx_rand_next(csx_rand* r) {
-    int32_t v = 0;
-    if (r) random_r(&r->rd, &v);
-    return v;
+int32_t csx_rand_next(csx_rand* r) { return r ? csr::value(csr::step(r->s)) : 0; }
+
+int csx_rand_skip(csx_rand* r, int64_t n) {
+    if (!r || n < 0) return CSX_E_ARG;
+    if (n < 4 * csr::kDeg) {
+        for (int64_t i = 0; i < n; ++i) csr::step(r->s);
+    } else {
+        std::vector<uint32_t> rows(csr::kRows);
+        csr::jump_rows((uint64_t)n, rows.data());
+        csr::advance(r->s, rows.data());
+    }
+    return CSX_OK;
 }
 
 int csx_rand_zuf(csx_rand* r, double* out, int64_t n) {
@@ -363,6 +365,75 @@ int csx_engine_stage_image(csx_engine* e, csx_ctx* ctx, double* d_out, int64_t c
     rc = csx_fit_batch_stage(ctx, &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
                              P.latpar2.empty() ? nullptr : P.latpar2.data(), P.latpar3.empty() ? nullptr : P.latpar3.data(),
                              w, h, &m, d_out);
+    if (rc != CSX_OK) {
+        char b[256];
+        csx_last_error(ctx, b, sizeof b);
+        e->err = b;
+    }
+    return rc;
+}
+
+int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand* rng, int n, double* d_out,
+                         int64_t stride, double* d_noise, int nextra, double* h_extra, int threads) {
+    if (!e || !ctx || !ids || !*ids || !rng || n <= 0 || !d_out || nextra < 0 || (nextra && !h_extra))
+        return CSX_E_ARG;
+    // the image size of the parameters as set (the random variation keeps the grid)
+    int rc = e->prepared ? CSX_OK : engine_prepare_host(e);
+    if (rc != CSX_OK) return rc;
+    const int64_t npx = (int64_t)(e->P.s.xmax - e->P.s.xmin) * (e->P.s.ymax - e->P.s.ymin);
+    if (npx > stride) { e->err = "image larger than its output stride"; return CSX_E_ARG; }
+    // draws in the AI-data loop's order, image after image: variation, noise, then the extra
+    // values (beam stop, lines) of the caller's post-processing.  The noise is jumped over
+    // here, its start state kept, and drawn on the device.
+    std::vector<csx::ParamSet> sets((size_t)n);
+    std::vector<uint32_t> wins, rows;
+    if (d_noise) {
+        wins.resize((size_t)n * csr::kDeg);
+        rows.resize(csr::kRows);
+        csr::jump_rows((uint64_t)npx, rows.data());
+    }
+    for (int k = 0; k < n; ++k) {
+        if ((rc = csx_engine_tpv(e, ids, rng, nullptr, 0)) != CSX_OK) return rc;
+        sets[(size_t)k] = e->pt;
+        if (d_noise) {
+            csr::window(rng->s, &wins[(size_t)k * csr::kDeg]);
+            csr::advance(rng->s, rows.data());
+        }
+        if (nextra) csx_rand_zuf(rng, h_extra + (size_t)k * nextra, nextra);
+    }
+    // preparation and the host part of staging on threads
+    std::vector<csx::PreparedSet> preps((size_t)n);
+    std::vector<std::unique_ptr<csx_staged, void (*)(csx_staged*)>> st;
+    st.reserve((size_t)n);
+    for (int k = 0; k < n; ++k) st.emplace_back(csx_staged_new(), csx_staged_free);
+    std::vector<std::string> errs((size_t)n);
+    csx_mask m{};
+    for (int i = 0; i < 16; ++i) m.rects[i] = -1;
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int k = next++; k < n; k = next++) {
+            const csx::ParamSet& ps = sets[(size_t)k];
+            std::string msg = csx::check_parameters(ps, e->ignore_cbs);
+            if (msg.empty()) msg = csx::prepare(ps, preps[(size_t)k]);
+            if (!msg.empty()) { errs[(size_t)k] = msg; continue; }
+            const csx::PreparedSet& P = preps[(size_t)k];
+            const int w = P.s.xmax - P.s.xmin, h = P.s.ymax - P.s.ymin;
+            csx_stage_build(st[(size_t)k].get(), &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
+                            P.latpar2.empty() ? nullptr : P.latpar2.data(),
+                            P.latpar3.empty() ? nullptr : P.latpar3.data(), w, h, &m, d_out + (size_t)k * stride);
+        }
+    };
+    const int nt = std::max(1, std::min(threads > 0 ? threads : 1, n));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    for (int k = 0; k < n; ++k)
+        if (!errs[(size_t)k].empty()) { e->err = errs[(size_t)k]; return CSX_E_PARAM; }
+    std::vector<const csx_staged*> built((size_t)n);
+    for (int k = 0; k < n; ++k) built[(size_t)k] = st[(size_t)k].get();
+    rc = csx_stage_commit_many(ctx, n, built.data(), nt);
+    if (rc == CSX_OK && d_noise) rc = csx_post_rand_zuf(ctx, wins.data(), n, npx, d_noise);
     if (rc != CSX_OK) {
         char b[256];
         csx_last_error(ctx, b, sizeof b);

@@ -18,6 +18,7 @@
 
 #include "csx.h"
 #include "csx_prep.h"
+#include "csx_rand.h"
 
 namespace csp {
 
@@ -301,6 +302,58 @@ __global__ void k_cut(const double* __restrict__ full, int s, int cx, int cy, do
 
 int blocks_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
 
+// ---- zuf() values of the rand() sequence on the device ------------------------------------
+// One lane per (run, chunk of kRandChunk values).  The run's start state (its last 31 terms,
+// oldest first) is read, moved to the chunk's start with the chunk's jump rows (the same for
+// the whole workgroup, so they are read through the scalar path), and the chunk's terms follow
+// from r[i] = r[i-31] + r[i-3] with the 31 terms in registers: in a block of 31 steps the term
+// three back is the one written three steps before, or for the first three steps one of the
+// block's last three old terms.
+constexpr int kRandChunk = 31 * 16;
+
+__device__ inline double zuf_of(int32_t v) {
+#pragma clang fp contract(off)
+    return ((double)v / 2147483647.0 * 200.0 - 100.0) / 100.0;
+}
+
+__global__ void __launch_bounds__(64) k_rand_zuf(const uint32_t* __restrict__ win, int64_t nseq,
+                                                 const uint32_t* __restrict__ jump, int64_t len,
+                                                 double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= nseq) return;
+    uint32_t w[31];
+    {
+        uint32_t u[31];
+#pragma unroll
+        for (int j = 0; j < 31; ++j) u[j] = win[k * 31 + j];
+        if (c == 0) {
+#pragma unroll
+            for (int j = 0; j < 31; ++j) w[j] = u[j];
+        } else {
+            const uint32_t* A = jump + (size_t)(c - 1) * 961;
+#pragma unroll
+            for (int m = 0; m < 31; ++m) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int j = 0; j < 31; ++j) v += A[m * 31 + j] * u[j];
+                w[m] = v;
+            }
+        }
+    }
+    const int64_t t0 = (int64_t)c * kRandChunk;
+    const int n = (int)min((int64_t)kRandChunk, len - t0);
+    double* o = out + k * len + t0;
+    for (int b = 0; b < n; b += 31) {
+#pragma unroll
+        for (int j = 0; j < 31; ++j) {
+            const uint32_t v = w[j] + w[(j + 28) % 31];
+            w[j] = v;
+            if (b + j < n) o[b + j] = zuf_of((int32_t)(v >> 1));
+        }
+    }
+}
+
 }  // namespace csp
 
 using namespace csp;
@@ -310,6 +363,39 @@ using namespace csp;
         hipError_t e_ = (call);                                                          \
         if (e_ != hipSuccess) return csx_ctx_fail(c, CSX_E_HIP, (std::string(#call) + ": " + hipGetErrorString(e_)).c_str()); \
     } while (0)
+
+int csx_post_rand_zuf(csx_ctx* c, const uint32_t* h_win, int64_t nseq, int64_t len, double* d_out) {
+    if (!c) return CSX_E_ARG;
+    if (nseq == 0 || len == 0) return CSX_OK;
+    const int64_t nch = (len + kRandChunk - 1) / kRandChunk;
+    if (!h_win || !d_out || nseq < 0 || len < 0 || nch > 65535 || (nseq + 63) / 64 > INT32_MAX)
+        return csx_ctx_fail(c, CSX_E_ARG, "bad random-value runs");
+    if (hipSetDevice(csx_ctx_device(c)) != hipSuccess) return csx_ctx_fail(c, CSX_E_HIP, "hipSetDevice");
+    // jump rows of c * kRandChunk terms, c = 1 .. nch-1
+    std::vector<uint32_t> rows((size_t)std::max<int64_t>(nch - 1, 1) * csr::kRows);
+    uint32_t step[csr::kDeg], a[csr::kDeg], t[csr::kDeg];
+    csr::xpow(kRandChunk, step);
+    for (int j = 0; j < csr::kDeg; ++j) a[j] = step[j];
+    for (int64_t ci = 1; ci < nch; ++ci) {
+        csr::poly_rows(a, &rows[(size_t)(ci - 1) * csr::kRows]);
+        csr::polymulmod(a, step, t);
+        for (int j = 0; j < csr::kDeg; ++j) a[j] = t[j];
+    }
+    const size_t wbytes = (size_t)nseq * csr::kDeg * sizeof(uint32_t), rbytes = rows.size() * sizeof(uint32_t);
+    char* scr = (char*)csx_ctx_scratch(c, 0, ((wbytes + 255) & ~size_t(255)) + rbytes);
+    if (!scr) return csx_ctx_fail(c, CSX_E_NOMEM, "random-value scratch");
+    uint32_t* d_win = (uint32_t*)scr;
+    uint32_t* d_rows = (uint32_t*)(scr + ((wbytes + 255) & ~size_t(255)));
+    hipStream_t st = (hipStream_t)csx_ctx_stream(c);
+    PCHK(c, hipMemcpyAsync(d_win, h_win, wbytes, hipMemcpyHostToDevice, st));
+    PCHK(c, hipMemcpyAsync(d_rows, rows.data(), rbytes, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_rand_zuf, dim3((unsigned)((nseq + 63) / 64), (unsigned)nch), dim3(64), 0, st, d_win, nseq,
+                       d_rows, len, d_out);
+    PCHK(c, hipGetLastError());
+    // the host arrays must outlive the copies
+    PCHK(c, hipStreamSynchronize(st));
+    return CSX_OK;
+}
 
 namespace {
 
@@ -334,6 +420,28 @@ int check(csx_ctx* c, const void* d, int64_t npx, int nimg) {
 }  // namespace
 
 extern "C" {
+
+int csx_rand_zuf_device(csx_ctx* c, csx_rand* r, int64_t n, double* d_out) {
+    if (!c) return CSX_E_ARG;
+    if (!r || n < 0 || (!d_out && n > 0)) return csx_ctx_fail(c, CSX_E_ARG, "bad random-value request");
+    // runs of kRun values from states kRun apart (one jump each), then the rest as one run
+    constexpr int64_t kRun = (int64_t)kRandChunk * 64;
+    const int64_t nfull = n / kRun, rest = n - nfull * kRun;
+    std::vector<uint32_t> wins((size_t)(nfull + 1) * csr::kDeg), rows(csr::kRows);
+    if (nfull) csr::jump_rows((uint64_t)kRun, rows.data());
+    for (int64_t k = 0; k < nfull; ++k) {
+        csr::window(r->s, &wins[(size_t)k * csr::kDeg]);
+        csr::advance(r->s, rows.data());
+    }
+    int rc = csx_post_rand_zuf(c, wins.data(), nfull, kRun, d_out);
+    if (rc == CSX_OK && rest) {
+        csr::window(r->s, &wins[0]);
+        csr::jump_rows((uint64_t)rest, rows.data());
+        csr::advance(r->s, rows.data());
+        rc = csx_post_rand_zuf(c, wins.data(), 1, rest, d_out + nfull * kRun);
+    }
+    return rc;
+}
 
 int csx_post_noise(csx_ctx* c, double* d_img, const double* d_z, int64_t n, int scaled) {
     if (int rc = check(c, d_img, n, 1)) return rc;

@@ -98,3 +98,7 @@ int csx_ctx_fail(csx_ctx* c, int code, const char* msg);
 // Device scratch `which` (0 or 1) of the context, at least `bytes` long; work on the
 // context's stream is ordered, so its calls can reuse it.  Null on failure.
 void* csx_ctx_scratch(csx_ctx* c, int which, size_t bytes);
+// nseq runs of len zuf() values on the device, run k from the generator state whose 31 terms
+// (oldest first) are h_win[31k .. 31k+30], into d_out + k*len; ordered on the context's
+// stream, the host arrays free to reuse on return.
+int csx_post_rand_zuf(csx_ctx* c, const uint32_t* h_win, int64_t nseq, int64_t len, double* d_out);

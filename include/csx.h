@@ -252,12 +252,18 @@ int csx_engine_stage_image(csx_engine* e, csx_ctx* ctx, double* d_out, int64_t c
  * csx_rand: the C library's rand() sequence (srand(seed), then rand() values) in a private
  * state; the reference draws from the process-wide one, seeded 1 unless srand is called.
  * csx_rand_zuf fills n values of zuf() = (rand()/RAND_MAX*200 - 100)/100 in [-1, 1]
- * (Source/sc_mainCons.cpp:850-853), e.g. the noise of the AI-data loop. */
+ * (Source/sc_mainCons.cpp:850-853), e.g. the noise of the AI-data loop.
+ * csx_rand_skip advances the state by n values without drawing them (a jump of O(log n));
+ * csx_rand_zuf_device writes the next n zuf() values to device memory (d_out, on ctx's
+ * stream; equal to csx_rand_zuf's) and advances the state by n (no reference counterpart:
+ * the device continuation of the sequence). */
 typedef struct csx_rand csx_rand;
 csx_rand* csx_rand_new(uint32_t seed);
 void csx_rand_free(csx_rand* r);
 int32_t csx_rand_next(csx_rand* r);
 int csx_rand_zuf(csx_rand* r, double* h_out, int64_t n);
+int csx_rand_skip(csx_rand* r, int64_t n);
+int csx_rand_zuf_device(csx_ctx* ctx, csx_rand* r, int64_t n, double* d_out);
 /* tpvPerformRandom (Source/sc_calc_generic_cpu.cpp:1028-1143): ids holds "key=factor" lines
  * ('\n'-separated); each listed variable gets its value at the first call after a clear times
  * (1 + factor*zuf()) (ucb/ucc: the current uca; phi: 2 + 43*(zuf()+1); EditAzi: draws but has
@@ -267,6 +273,15 @@ int csx_rand_zuf(csx_rand* r, double* h_out, int64_t n);
  * receives the reference's "key = value" lines. */
 int csx_engine_tpv(csx_engine* e, const char* ids, csx_rand* rng, char* report, int cap);
 double csx_engine_tpv_value(csx_engine* e, const char* key);
+/* n images of one AI-data parameter line staged at once (no reference counterpart; the loop
+ * of Source/sc_mainCons.cpp:1236-1300 for one line): per image, in the loop's rand() order,
+ * csx_engine_tpv(ids), npx noise values (if d_noise: written by the device to
+ * d_noise + k*npx, on ctx's stream), nextra values into h_extra + k*nextra (the caller's beam
+ * stop and lines); then preparation and staging of all images on `threads` host threads into
+ * ctx's pending batch, image k's output at d_out + k*stride.  Follow with
+ * csx_render_batch_run. */
+int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand* rng, int n, double* d_out,
+                         int64_t stride, double* d_noise, int nextra, double* h_extra, int threads);
 
 /* ---- Post-processing of image batches in HBM (the AI-data loop after each calculation,
  * Source/sc_mainCons.cpp:1254-1400) -------------------------------------------------------

@@ -158,6 +158,22 @@ def test_ai_pipeline_matches_the_oracle(oracle_lib, opts_kw):
         assert np.abs(rphi[k] - rp).max() < 1e-10, k
 
 
+@pytest.mark.parametrize("n", [0, 1, 30, 31, 495, 496, 497, 31743, 31744, 31745, 100003])
+def test_device_zuf_equals_the_host_sequence(n):
+    """rand() continued on the device (jumped start states, 31 terms in registers) gives the
+    host's values bit for bit and leaves the generator where n draws leave it."""
+    import torch
+    ctx = Context(0)
+    a, b = Random(31337), Random(31337)
+    a.zuf(17)
+    b.zuf(17)
+    out = torch.full((max(n, 1),), 7.0, dtype=torch.float64, device="cuda")[:n]
+    a.zuf_device(ctx, out)
+    ref = b.zuf(n)
+    assert np.array_equal(out.cpu().numpy(), ref)
+    assert [a.next() for _ in range(50)] == [b.next() for _ in range(50)]
+
+
 def test_batch_rejects_sets_that_need_carr11pm():
     """Gaussian-oriented cylinders and oriented cubes read the carr11pm table, which a batch
     does not carry: staging them is an error on the host (nothing is launched)."""
