@@ -125,6 +125,7 @@ SIGNATURES = [
     ("csx_engine_tpv_batch", _I, [_VP, _VP, _CP, _VP, _I, _VP, ctypes.c_int64, _VP, _I, _PD, _I]),
     ("csx_post_noise", _I, [_VP, _VP, _VP, ctypes.c_int64, _I]),
     ("csx_post_zero", _I, [_VP, _VP, _VP, ctypes.c_int64]),
+    ("csx_post_zero_rects", _I, [_VP, _VP, ctypes.c_int64, _I, _VP, _I, _I, _I, _I, _I]),
     ("csx_post_scale_intensity", _I, [_VP, _VP, ctypes.c_int64, _I, _I]),
     ("csx_post_log_scale", _I, [_VP, _VP, ctypes.c_int64, _I, _I, _I, _I, _I]),
     ("csx_post_scale_clip", _I, [_VP, _VP, ctypes.c_int64, _I, _I, _I, _I, _I]),

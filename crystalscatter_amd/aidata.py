@@ -21,7 +21,6 @@ fft_clip40, fft_swap (fftOutputSwap), fft_cut (fftOutCutX/Y).  File writing is n
 this path.
 """
 import ctypes
-import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -57,43 +56,40 @@ def _n_extra(opts):
     return (4 if opts.bs[0] > 0 else 0) + max(0, opts.lines_x) + max(0, opts.lines_y)
 
 
-def _zero_indices(draw, opts, bounds, beam):
-    """Pixels the beam stop and the lines zero, and the (random) beam-stop centre; draw()
-    gives the next zuf() value in the reference's order (Source/sc_mainCons.cpp:1298-1348).  Indices follow the
-    reference's ii = (x - minX) + sx (y - minY), kept when 0 <= ii < len (a line past the
-    right edge therefore continues on the next row, as there)."""
+def _zero_rects(extra, opts, bounds, beam):
+    """The beam stop and the lines of every image as rectangles (xa, xb, ya, yb; inclusive) for
+    csx_post_zero_rects, and the beam-stop centres, from the zuf() values each image drew
+    (extra: one row per image, in the reference's order; Source/sc_mainCons.cpp:1298-1348).
+    The arithmetic is the reference's, elementwise: int() of the same double expressions."""
     x0, x1, y0, y1 = bounds
-    sx = x1 - x0
-    n = sx * (y1 - y0)
-    bsx0, bsy0 = int(beam[0]), int(beam[1])
-    bsdx = bsdy = 2
-    idx = []
+    n = extra.shape[0]
+    full = lambda v: np.full(n, v, dtype=np.int64)
+    bsx0, bsy0 = full(int(beam[0])), full(int(beam[1]))
+    rects, col = [], 0
     if opts.bs[0] > 0:
-        bsx0 = int(bsx0 * (1.0 + opts.bs[0] * draw()))
-        bsy0 = int(bsy0 * (1.0 + opts.bs[1] * draw()))
-        bsdx = int(bsdx * (1.0 + opts.bs[2] * draw()))
-        bsdy = int(bsdy * (1.0 + opts.bs[3] * draw()))
-        for x in range(x0, x1):
-            if bsx0 - bsdx <= x <= bsx0 + bsdx:
-                for y in range(y0, y1):
-                    if bsy0 - bsdy <= y <= bsy0 + bsdy:
-                        idx.append((x - x0) + sx * (y - y0))
+        bsx0 = np.trunc(bsx0 * (1.0 + opts.bs[0] * extra[:, 0])).astype(np.int64)
+        bsy0 = np.trunc(bsy0 * (1.0 + opts.bs[1] * extra[:, 1])).astype(np.int64)
+        bsdx = np.trunc(2 * (1.0 + opts.bs[2] * extra[:, 2])).astype(np.int64)
+        bsdy = np.trunc(2 * (1.0 + opts.bs[3] * extra[:, 3])).astype(np.int64)
+        # the reference walks the image's x and y and tests the box
+        rects.append(np.stack([np.maximum(bsx0 - bsdx, x0), np.minimum(bsx0 + bsdx, x1 - 1),
+                               np.maximum(bsy0 - bsdy, y0), np.minimum(bsy0 + bsdy, y1 - 1)], 1))
+        col = 4
     if opts.lines_x > 0:
         xc = (x1 - x0 + 1) // opts.lines_x
         for i in range(opts.lines_x):
-            xp = int(x0 + i * xc + math.fabs(draw()) * xc)
-            for x in range(xp, xp + opts.lines_x_width):
-                for y in range(y0, y1):
-                    idx.append((x - x0) + sx * (y - y0))
+            xp = np.trunc((x0 + i * xc) + np.abs(extra[:, col]) * xc).astype(np.int64)
+            col += 1
+            rects.append(np.stack([xp, xp + opts.lines_x_width - 1, full(y0), full(y1 - 1)], 1))
     if opts.lines_y > 0:
         yc = (y1 - y0 + 1) // opts.lines_y
         for i in range(opts.lines_y):
-            yp = int(y0 + i * yc + math.fabs(draw()) * yc)
-            for y in range(yp, yp + opts.lines_y_width):
-                for x in range(x0, x1):
-                    idx.append((x - x0) + sx * (y - y0))
-    ii = np.array([v for v in idx if 0 <= v < n], dtype=np.int64)
-    return ii, (bsx0, bsy0)
+            yp = np.trunc((y0 + i * yc) + np.abs(extra[:, col]) * yc).astype(np.int64)
+            col += 1
+            rects.append(np.stack([full(x0), full(x1 - 1), yp, yp + opts.lines_y_width - 1], 1))
+    big = np.iinfo(np.int32)
+    r = np.ascontiguousarray(np.clip(np.stack(rects, 1), big.min, big.max).astype(np.int32)) if rects else None
+    return r, list(zip(bsx0.tolist(), bsy0.tolist()))
 
 
 def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16):
@@ -129,12 +125,7 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16)
                                 extra.ctypes.data_as(pd) if nx else None, int(threads))
     if rc != 0:
         raise CsxError("csx_engine_tpv_batch: " + engine._err())
-    zero_idx, centres = [], []
-    for k in range(n):
-        vals = iter(extra[k, :nx])
-        ii, c = _zero_indices(lambda: float(next(vals)), opts, bounds, beam)
-        zero_idx.append(ii + k * npx)
-        centres.append(c)
+    rects, centres = _zero_rects(extra[:, :nx], opts, bounds, beam)
     ctx._chk(L.csx_render_batch_run(ctx._h, _prec(precision)), "csx_render_batch_run")
     d = ctypes.c_void_p(scat.data_ptr())
     if opts.scale_scat:
@@ -143,12 +134,9 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16)
     if opts.noise:
         ctx._chk(L.csx_post_noise(ctx._h, d, ctypes.c_void_p(zt.data_ptr()), n * npx, int(opts.scale_scat)),
                  "csx_post_noise")
-    allz = np.concatenate(zero_idx) if zero_idx else np.zeros(0, np.int64)
-    if allz.size:
-        it = torch.from_numpy(allz).cuda()
-        keep.append(it)
-        _torch_done()
-        ctx._chk(L.csx_post_zero(ctx._h, d, ctypes.c_void_p(it.data_ptr()), it.numel()), "csx_post_zero")
+    if rects is not None:
+        ctx._chk(L.csx_post_zero_rects(ctx._h, d, npx, n, rects.ctypes.data_as(ctypes.c_void_p), rects.shape[1],
+                                       x0, x1, y0, y1), "csx_post_zero_rects")
     out = {"scat": scat, "bounds": bounds, "centres": centres}
     if opts.rphi_size > 0:
         s = int(opts.rphi_size)

@@ -300,6 +300,22 @@ __global__ void k_cut(const double* __restrict__ full, int s, int cx, int cy, do
     }
 }
 
+// Zeroed rectangles (beam stop, lines): one workgroup per (rectangle, image); a cell (x, y) of
+// rectangle xa..xb, ya..yb (inclusive) is pixel ii = (x - x0) + sx (y - y0), zeroed when
+// 0 <= ii < npx (a line past the right edge continues on the next row).
+__global__ void k_zero_rects(double* __restrict__ img, int64_t npx, const int4* __restrict__ rects, int nrect, int x0,
+                             int sx, int y0) {
+    const int4 r = rects[(int64_t)blockIdx.y * nrect + blockIdx.x];
+    const int64_t wx = (int64_t)r.y - r.x + 1, wy = (int64_t)r.w - r.z + 1;
+    if (wx <= 0 || wy <= 0) return;
+    double* im = img + (int64_t)blockIdx.y * npx;
+    for (int64_t c = threadIdx.x; c < wx * wy; c += blockDim.x) {
+        const int64_t x = r.x + c % wx, y = r.z + c / wx;
+        const int64_t ii = (x - x0) + (int64_t)sx * (y - y0);
+        if (ii >= 0 && ii < npx) im[ii] = 0.0;
+    }
+}
+
 int blocks_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
 
 // ---- zuf() values of the rand() sequence on the device ------------------------------------
@@ -459,6 +475,30 @@ int csx_post_zero(csx_ctx* c, double* d_img, const int64_t* d_idx, int64_t n) {
     hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)csx_ctx_stream(c), d_img,
                        d_idx, n);
     PCHK(c, hipGetLastError());
+    return CSX_OK;
+}
+
+int csx_post_zero_rects(csx_ctx* c, double* d_img, int64_t npx, int nimg, const int32_t* h_rects, int nrect, int x0,
+                        int x1, int y0, int y1) {
+    if (int rc = check(c, d_img, npx, nimg)) return rc;
+    if (nrect == 0) return CSX_OK;
+    if (!h_rects || nrect < 0 || nrect > 65535 || x1 <= x0 || y1 <= y0 || (int64_t)(x1 - x0) * (y1 - y0) != npx)
+        return csx_ctx_fail(c, CSX_E_ARG, "bad rectangles or image bounds");
+    const size_t count = (size_t)nimg * nrect;
+    for (size_t i = 0; i < count; ++i) {
+        const int32_t* r = h_rects + 4 * i;
+        const int64_t wx = (int64_t)r[1] - r[0] + 1, wy = (int64_t)r[3] - r[2] + 1;
+        if (wx > 0 && wy > 0 && wx * wy > 64 * npx) return csx_ctx_fail(c, CSX_E_ARG, "rectangle far larger than the image");
+    }
+    const size_t bytes = count * 4 * sizeof(int32_t);
+    int4* d_r = (int4*)csx_ctx_scratch(c, 0, bytes);
+    if (!d_r) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
+    hipStream_t st = (hipStream_t)csx_ctx_stream(c);
+    PCHK(c, hipMemcpyAsync(d_r, h_rects, bytes, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_zero_rects, dim3(nrect, nimg), dim3(256), 0, st, d_img, npx, d_r, nrect, x0, x1 - x0, y0);
+    PCHK(c, hipGetLastError());
+    // the host array must outlive the copy
+    PCHK(c, hipStreamSynchronize(st));
     return CSX_OK;
 }
 

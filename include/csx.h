@@ -300,6 +300,13 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
  *   centre xc, yc) into d_out (nimg x s*s; cells the reference never writes are 0). */
 int csx_post_noise(csx_ctx* ctx, double* d_img, const double* d_z, int64_t n, int scaled);
 int csx_post_zero(csx_ctx* ctx, double* d_img, const int64_t* d_idx, int64_t n);
+/* zero_rects: the same zeroing given as nrect rectangles per image (h_rects: nimg x nrect x
+ *   {xa, xb, ya, yb}, inclusive, host memory) over an image with bounds x0..x1, y0..y1: cell
+ *   (x, y) zeroes pixel (x - x0) + (x1 - x0)(y - y0) when that lies in the image, so the beam
+ *   stop is its box cut to the image and a line is x = xp..xp+width-1 (or y = ...) over the
+ *   full other range (:1298-1348). */
+int csx_post_zero_rects(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, const int32_t* h_rects, int nrect,
+                        int x0, int x1, int y0, int y1);
 int csx_post_scale_intensity(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int linlog);
 int csx_post_log_scale(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int w, int xmin, int ymin,
                        int skip_axes);
