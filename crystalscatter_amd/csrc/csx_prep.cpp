@@ -260,6 +260,8 @@ void coef_cube(PreparedSet& P, double z, double xr2z, double radius, CoefState& 
 struct FSums {
     double z = -1.0, zl = -1.0;
     double acc1[kNN + 1], acc4[kNN + 1];
+    // the factors of carr1p / carr4p without the powers of xl2z, xr2z (numerator, denominator)
+    long double a1[kNN + 1], d1[kNN + 1], b4[kNN + 1], e4[kNN + 1];
 };
 
 // Cube, perfect orientation, homogeneous (Source/sc_calc_generic_cpu.cpp:2880-2910): the
@@ -322,30 +324,28 @@ void coef_cylinder_iso(double z, double zl, double xl2z, double xr2z, CoefState&
 
 // Cylinder, perfect orientation, homogeneous: Source/sc_calc_generic_cpu.cpp:4124-4212.
 void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState& cs) {
+    // Everything but the powers of xl2z and xr2z depends on z and zl only, and is kept per
+    // thread while they stay the same (fit vertices and training images change R and L far
+    // more often than the widths).  The products are the same operations in the same order as
+    // when formed inline, so the coefficients do not change by a bit.
     static thread_local FSums fs;
-    const bool cached = fs.z == z && fs.zl == zl;
-    long double fkv[kNN + 2], fk2v[kNN + 1], gam3[kNN + 2], z12vl[kNN + 1];
-    double z12v[kNN + 1];
-    fkv[0] = 1; fk2v[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12vl[0] = 1; z12v[0] = 1.0;
-    long double xln = 1.0;
-    double xrn = 1.0;
-    double p4 = 1.0;  // 4^n, exact
-    const double sqrt_pi = std::sqrt(kPi);
-    bool find1 = true, find4 = true;
-    double *c1p = cs.arr[0], *c4p = cs.arr[3], *c1f = cs.arr[9], *c4f = cs.arr[12];
-    for (int n = 1; n <= kNN; ++n) {
-        p4 *= 4.0;
-        z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
-        z12vl[n] = z12vl[n - 1] * ((zl + 1) - 2 + 2 * n) * ((zl + 1) - 1 + 2 * n);
-        fkv[n] = fkv[n - 1] * n;
-        fk2v[n] = fk2v[n - 1] * (2 * n - 1) * (2 * n);
-        gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
-        xln = -xln * xl2z;
-        xrn = -xrn * xr2z;
-        c1p[n] = sqrt_pi * p4 * z12vl[n] * xln / (2.0 * (n + 1) * gam3[n] * fkv[n]);
-        c4p[n] = 4 * (n + 1 / 2.0) * fk2v[n] * z12v[n] * xrn /
-                 ((n + 2) * (n + 1) * fkv[n] * fkv[n] * (n + 1) * fkv[n]);
-        if (!cached) {
+    if (!(fs.z == z && fs.zl == zl)) {
+        long double fkv[kNN + 2], fk2v[kNN + 1], gam3[kNN + 2], z12vl[kNN + 1];
+        double z12v[kNN + 1];
+        fkv[0] = 1; fk2v[0] = 1; gam3[0] = std::sqrt(kPi) / 2.0; z12vl[0] = 1; z12v[0] = 1.0;
+        double p4 = 1.0;  // 4^n, exact
+        const double sqrt_pi = std::sqrt(kPi);
+        for (int n = 1; n <= kNN; ++n) {
+            p4 *= 4.0;
+            z12v[n] = z12v[n - 1] * ((z + 1) - 2 + 2 * n) * ((z + 1) - 1 + 2 * n);
+            z12vl[n] = z12vl[n - 1] * ((zl + 1) - 2 + 2 * n) * ((zl + 1) - 1 + 2 * n);
+            fkv[n] = fkv[n - 1] * n;
+            fk2v[n] = fk2v[n - 1] * (2 * n - 1) * (2 * n);
+            gam3[n] = gam3[n - 1] * (2 * n + 1) / 2.0;
+            fs.a1[n] = sqrt_pi * p4 * z12vl[n];
+            fs.d1[n] = 2.0 * (n + 1) * gam3[n] * fkv[n];
+            fs.b4[n] = 4 * (n + 1 / 2.0) * fk2v[n] * z12v[n];
+            fs.e4[n] = (n + 2) * (n + 1) * fkv[n] * fkv[n] * (n + 1) * fkv[n];
             double acc = 0.0;
             for (int m = 0; m <= n; ++m)
                 acc += z12vl[m] * z12vl[n - m] / (gam3[m] * gam3[n - m] * fkv[m] * fkv[n - m] * fkv[m] * fkv[n - m]);
@@ -355,6 +355,18 @@ void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState
                 acc += z12v[m] * z12v[n - m] / ((m + 1) * fkv[m] * (n - m + 1) * fkv[n - m] * fkv[m] * fkv[n - m]);
             fs.acc4[n] = acc;
         }
+        fs.z = z;
+        fs.zl = zl;
+    }
+    long double xln = 1.0;
+    double xrn = 1.0;
+    bool find1 = true, find4 = true;
+    double *c1p = cs.arr[0], *c4p = cs.arr[3], *c1f = cs.arr[9], *c4f = cs.arr[12];
+    for (int n = 1; n <= kNN; ++n) {
+        xln = -xln * xl2z;
+        xrn = -xrn * xr2z;
+        c1p[n] = fs.a1[n] * xln / fs.d1[n];
+        c4p[n] = fs.b4[n] * xrn / fs.e4[n];
         c1f[n] = kPi * xln * fs.acc1[n] / 4.0;
         c4f[n] = xrn * fs.acc4[n];
         // The reference's switch-point search stops at the first coefficient below 1e-50
@@ -364,8 +376,6 @@ void coef_cylinder_zdir(double z, double zl, double xl2z, double xr2z, CoefState
         if (std::fabs(c1f[n]) < 1E-30 && n < cs.nidx[9]) cs.nidx[9] = n;
         if (std::fabs(c4f[n]) < 1E-30 && n < cs.nidx[12]) cs.nidx[12] = n;
     }
-    fs.z = z;
-    fs.zl = zl;
 }
 
 // Cylinder, Gaussian orientation, general case: Source/sc_calc_generic_cpu.cpp:4648-4720 and
@@ -831,21 +841,42 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     if (p.ltype != 12) {
         if (p.ltype != 4 && p.ltype != 5 && p.ltype != 7 && p.ltype != 8)
             return "lattice type not supported by the device path";
-        const bool cubic = p.ltype != 8;
+        // The reflection lists, multiplicities and structure factors depend only on the
+        // lattice type and HKLmax: built once per thread and kept while those stay the same
+        // (a fit's vertices and a training set's images change only the lattice constants).
+        struct HklLists {
+            int ltype = -1, H = -1;
+            std::vector<int> iso, ani;      // h, k, l, multiplicity
+            std::vector<double> f1, f2;     // structure factor per entry
+            std::vector<double> s1, s2;     // its sphno
+        };
+        static thread_local HklLists hl;
         const int H = p.hklmax;
-        std::vector<int> iso, ani;
-        for (int pass = 0; pass < 2; ++pass) {
-            const int lo = pass == 0 ? 0 : -H;
-            auto& v = pass == 0 ? iso : ani;
-            for (int h = lo; h <= H; ++h)
-                for (int k = lo; k <= H; ++k)
-                    for (int l = lo; l <= H; ++l)
-                        if (allowed(p.ltype, h, k, l)) {
-                            const int pat = pattern(h, k, l);
-                            v.push_back(h); v.push_back(k); v.push_back(l);
-                            v.push_back(cubic ? kMultCubic[pat] : kMultTetra[pat]);
-                        }
+        if (hl.ltype != p.ltype || hl.H != H) {
+            const bool cubic = p.ltype != 8;
+            hl = HklLists{};
+            for (int pass = 0; pass < 2; ++pass) {
+                const int lo = pass == 0 ? 0 : -H;
+                auto& v = pass == 0 ? hl.iso : hl.ani;
+                auto& f = pass == 0 ? hl.f1 : hl.f2;
+                auto& sp = pass == 0 ? hl.s1 : hl.s2;
+                for (int h = lo; h <= H; ++h)
+                    for (int k = lo; k <= H; ++k)
+                        for (int l = lo; l <= H; ++l)
+                            if (allowed(p.ltype, h, k, l)) {
+                                const int pat = pattern(h, k, l);
+                                v.push_back(h); v.push_back(k); v.push_back(l);
+                                v.push_back(cubic ? kMultCubic[pat] : kMultTetra[pat]);
+                                double sph;
+                                f.push_back(structure_factor(p.ltype, h, k, l, sph));
+                                sp.push_back(sph);
+                            }
+            }
+            hl.ltype = p.ltype;
+            hl.H = H;
         }
+        const std::vector<int>& iso = hl.iso;
+        const std::vector<int>& ani = hl.ani;
         const int n1 = static_cast<int>(iso.size() / 4), n2 = static_cast<int>(ani.size() / 4);
         if (n1 >= CSX_LATLEN || n2 >= CSX_LATLEN) return "reflection table exceeds latparlen";
         s.peakmax1 = n1;
@@ -862,20 +893,18 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
         for (int i = 1; i <= n1; ++i) {
             int* L = &P.latpar1[static_cast<size_t>(i) * 6];
             L[1] = iso[4 * (i - 1)]; L[2] = iso[4 * (i - 1) + 1]; L[3] = iso[4 * (i - 1) + 2]; L[4] = iso[4 * (i - 1) + 3];
-            double sph;
-            L[5] = static_cast<int>(std::round(structure_factor(p.ltype, L[1], L[2], L[3], sph)));
+            L[5] = static_cast<int>(std::round(hl.f1[static_cast<size_t>(i - 1)]));
             lat_vol = P.ucvol;
-            lat_sph = sph;
+            lat_sph = hl.s1[static_cast<size_t>(i - 1)];
         }
         if (n1 == 0) { lat_vol = P.ucvol; lat_sph = 1; }
         for (int i = 1; i <= n2; ++i) {
             int* L = &P.latpar2[static_cast<size_t>(i) * 6];
             const int h = ani[4 * (i - 1)], k = ani[4 * (i - 1) + 1], l = ani[4 * (i - 1) + 2];
             L[1] = h; L[2] = k; L[3] = l;
-            double sph;
-            L[5] = static_cast<int>(std::round(structure_factor(p.ltype, h, k, l, sph)));
+            L[5] = static_cast<int>(std::round(hl.f2[static_cast<size_t>(i - 1)]));
             L[4] = 1;
-            P.sphno = sph;
+            P.sphno = hl.s2[static_cast<size_t>(i - 1)];
             float* T = &P.latpar3[static_cast<size_t>(i) * 17];
             T[1] = 2 * kPi / p.uca;
             const double qx = (2 * kPi) * (R[0] * h + R[3] * k + R[6] * l);
