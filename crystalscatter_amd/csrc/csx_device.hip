@@ -2307,6 +2307,3 @@ int csx_create_like(const csx_ctx* c, csx_ctx** out) {
     if (!c || !out) return CSX_E_ARG;
     return csx_create(c->device, out);
 }
-
-extern "C" {
-}  // extern "C"
