@@ -130,6 +130,7 @@ SIGNATURES = [
     ("csx_post_log_scale", _I, [_VP, _VP, ctypes.c_int64, _I, _I, _I, _I, _I]),
     ("csx_post_scale_clip", _I, [_VP, _VP, ctypes.c_int64, _I, _I, _I, _I, _I]),
     ("csx_post_rphi", _I, [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _VP, _I, _I, _I]),
+    ("csx_post_rphi_many", _I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _I, _VP, _I, _I, _I]),
     ("csx_post_ifft", _I, [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _VP]),
     ("csx_mlp_nparams", ctypes.c_int64, []),
     ("csx_mlp_create", _I, [_I, _I, ctypes.c_uint64, ctypes.POINTER(_VP)]),

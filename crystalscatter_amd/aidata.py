@@ -148,18 +148,11 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16)
             ctx._chk(L.csx_post_log_scale(ctx._h, ctypes.c_void_p(src.data_ptr()), npx, n, x1 - x0, x0, y0, 1),
                      "csx_post_log_scale")
         rphi = torch.empty((n, s * s), dtype=torch.float64, device="cuda")
-        groups = {}
-        for k, c in enumerate(centres):
-            groups.setdefault(c, []).append(k)
-        for c, ks in groups.items():
-            if len(ks) == n:
-                args = [(0, n)]
-            else:
-                args = [(k, 1) for k in ks]
-            for k0, cnt in args:
-                ctx._chk(L.csx_post_rphi(ctx._h, ctypes.c_void_p(src[k0].data_ptr()), x0, x1, y0, y1, c[0], c[1], s, cnt,
-                                         ctypes.c_void_p(rphi[k0].data_ptr()), int(opts.rphi_scale), int(opts.rphi_clip),
-                                         int(opts.rphi_clip40)), "csx_post_rphi")
+        cen = np.ascontiguousarray(np.array(centres, dtype=np.int64).reshape(n, 2).astype(np.int32))
+        ctx._chk(L.csx_post_rphi_many(ctx._h, ctypes.c_void_p(src.data_ptr()), x0, x1, y0, y1,
+                                      cen.ctypes.data_as(ctypes.c_void_p), s, n, ctypes.c_void_p(rphi.data_ptr()),
+                                      int(opts.rphi_scale), int(opts.rphi_clip), int(opts.rphi_clip40)),
+                 "csx_post_rphi_many")
         ctx._chk(L.csx_post_scale_clip(ctx._h, ctypes.c_void_p(rphi.data_ptr()), s * s, n, 0, 0, 0, 1),
                  "csx_post_scale_clip")
         out["rphi"] = rphi

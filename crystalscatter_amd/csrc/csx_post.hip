@@ -166,26 +166,34 @@ __global__ void k_zero(double* __restrict__ img, const int64_t* __restrict__ idx
 // r,phi cell: its writer (step k of the float radius loop, angle row i) or -1.
 struct RphiArgs {
     int sx, sy, s;
-    double xc, yc, phiber;
+    double phiber;
     int64_t npx;
 };
 
-__global__ void k_rphi(const double* __restrict__ imgs, RphiArgs a, const int* __restrict__ writer,
-                       const float* __restrict__ irs, double* __restrict__ outs) {
+// Per image: the centre and its cell map (writer table and radius steps of its irmax).
+struct RphiImg {
+    double xc, yc;
+    int64_t writer_off;
+    int32_t irs_off, pad;
+};
+
+__global__ void k_rphi(const double* __restrict__ imgs, RphiArgs a, const RphiImg* __restrict__ per,
+                       const int* __restrict__ writers, const float* __restrict__ irss, double* __restrict__ outs) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.s * a.s) return;
+    const RphiImg im = per[blockIdx.y];
     const double* d = imgs + (int64_t)blockIdx.y * a.npx;
     double* out = outs + (int64_t)blockIdx.y * a.s * a.s;
-    const int wr = writer[cell];
+    const int wr = writers[im.writer_off + cell];
     if (wr < 0) { out[cell] = 0.0; return; }
     const int k = wr / (a.s + 1), i = wr % (a.s + 1);
-    const double rad = irs[k];
+    const double rad = irss[im.irs_off + k];
     double twt = 0.0, avg = 0.0;
     int umf = 1024 - (int)(1024. - a.phiber * rad);
     if (umf < 1) umf = 1;
     for (int j = 1; j <= umf; ++j) {
         const double phi = (i - 1.5) * a.phiber + (j - 0.5) * a.phiber / umf;
-        const double xs = a.xc + rad * cos(phi), ys = a.yc + rad * sin(phi);
+        const double xs = im.xc + rad * cos(phi), ys = im.yc + rad * sin(phi);
         const int ix1 = (int)xs, iy1 = (int)ys, ix2 = ix1 + 1, iy2 = iy1 + 1;
         const double rex = xs - ix1, rey = ys - iy1;
         const bool x1in = ix1 >= 0 && ix1 < a.sx, x2in = ix2 >= 0 && ix2 < a.sx;
@@ -535,51 +543,86 @@ int csx_post_scale_clip(csx_ctx* c, double* d_img, int64_t npx, int nimg, int sc
     return CSX_OK;
 }
 
-int csx_post_rphi(csx_ctx* c, const double* d_img, int x0, int x1, int y0, int y1, int xc, int yc, int s, int nimg,
-                  double* d_out, int scale, int clip, int clip40) {
+int csx_post_rphi_many(csx_ctx* c, const double* d_img, int x0, int x1, int y0, int y1, const int32_t* h_centres,
+                       int s, int nimg, double* d_out, int scale, int clip, int clip40) {
     const int64_t npx = (int64_t)(x1 - x0) * (y1 - y0);
     if (int rc = check(c, d_img, npx, nimg)) return rc;
-    if (!d_out || s < 2 || s > 8192) return csx_ctx_fail(c, CSX_E_ARG, "r,phi size must be 2..8192");
-    // the float radius loop and the cell each (step, row) writes, as the reference computes
-    // them; a later write of a cell replaces an earlier one
-    if (x0 < 0) { x1 -= x0; xc -= x0; x0 = 0; }
-    if (y0 < 0) { y1 -= y0; yc -= y0; y0 = 0; }
+    if (!d_out || !h_centres || s < 2 || s > 8192) return csx_ctx_fail(c, CSX_E_ARG, "r,phi size must be 2..8192");
+    // image coordinates from 0 (the reference shifts the range and the centre alike)
+    const int dx = x0 < 0 ? -x0 : 0, dy = y0 < 0 ? -y0 : 0;
+    x0 += dx; x1 += dx; y0 += dy; y1 += dy;
     const int sx = x1 - x0, sy = y1 - y0;
     const double phiber = 2. * M_PI / s;
-    auto sq = [](double v) { return v * v; };
-    double rmax = std::sqrt(sq(1. - xc) + sq(1. - yc));
-    rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq(1. - yc)));
-    rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq((sy - 1.) - yc)));
-    rmax = std::max(rmax, std::sqrt(sq(1. - xc) + sq((sy - 1.) - yc)));
-    const int irmax = (int)(rmax + 0.5);
-    const float irstp = (float)irmax / (float)(s - 1);
-    std::vector<float> irs;
     const int64_t cells = (int64_t)s * s;
-    std::vector<int> writer((size_t)cells, -1);
-    for (float ir = 0; ir <= irmax; ir += irstp) {
-        const int k = (int)irs.size();
-        irs.push_back(ir);
-        for (int i = 0; i <= s; ++i) {
-            const int idx = (int)((float)(i * s) + ir / irstp);
-            if (idx < cells && idx >= 0) writer[(size_t)idx] = k * (s + 1) + i;
+    auto sq = [](double v) { return v * v; };
+    // the float radius loop and the cell each (step, row) writes, as the reference computes
+    // them (a later write of a cell replaces an earlier one); it depends on irmax only, so one
+    // map serves every image with the same irmax
+    std::vector<int> writers, irmax_of;
+    std::vector<float> irss;
+    std::vector<int64_t> woff;
+    std::vector<int32_t> ioff;
+    std::vector<RphiImg> per((size_t)nimg);
+    for (int k = 0; k < nimg; ++k) {
+        const int xc = h_centres[2 * k] + dx, yc = h_centres[2 * k + 1] + dy;
+        double rmax = std::sqrt(sq(1. - xc) + sq(1. - yc));
+        rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq(1. - yc)));
+        rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq((sy - 1.) - yc)));
+        rmax = std::max(rmax, std::sqrt(sq(1. - xc) + sq((sy - 1.) - yc)));
+        const int irmax = (int)(rmax + 0.5);
+        size_t t = 0;
+        while (t < irmax_of.size() && irmax_of[t] != irmax) ++t;
+        if (t == irmax_of.size()) {
+            const float irstp = (float)irmax / (float)(s - 1);
+            const int64_t w0 = (int64_t)writers.size();
+            const int32_t i0 = (int32_t)irss.size();
+            writers.resize(writers.size() + (size_t)cells, -1);
+            int steps = 0;
+            for (float ir = 0; ir <= irmax; ir += irstp) {
+                irss.push_back(ir);
+                for (int i = 0; i <= s; ++i) {
+                    const int idx = (int)((float)(i * s) + ir / irstp);
+                    if (idx < cells && idx >= 0) writers[(size_t)(w0 + idx)] = steps * (s + 1) + i;
+                }
+                if (++steps > 4 * s + 16) return csx_ctx_fail(c, CSX_E_ARG, "r,phi radius loop does not end");
+            }
+            irmax_of.push_back(irmax);
+            woff.push_back(w0);
+            ioff.push_back(i0);
         }
-        if (irs.size() > (size_t)(4 * s + 16)) return csx_ctx_fail(c, CSX_E_ARG, "r,phi radius loop does not end");
+        per[(size_t)k] = RphiImg{(double)xc, (double)yc, woff[t], ioff[t], 0};
     }
-    const size_t wbytes = (size_t)cells * sizeof(int), ibytes = irs.size() * sizeof(float);
-    char* scr = (char*)csx_ctx_scratch(c, 0, wbytes + ibytes + 256);
+    const size_t wbytes = writers.size() * sizeof(int), ibytes = std::max<size_t>(irss.size(), 1) * sizeof(float),
+                 pbytes = per.size() * sizeof(RphiImg);
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    char* scr = (char*)csx_ctx_scratch(c, 0, up(wbytes) + up(ibytes) + pbytes);
     if (!scr) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
-    int* d_writer = (int*)scr;
-    float* d_irs = (float*)(scr + ((wbytes + 255) & ~size_t(255)));
+    int* d_writers = (int*)scr;
+    float* d_irs = (float*)(scr + up(wbytes));
+    RphiImg* d_per = (RphiImg*)(scr + up(wbytes) + up(ibytes));
     hipStream_t st = (hipStream_t)csx_ctx_stream(c);
-    PCHK(c, hipMemcpyAsync(d_writer, writer.data(), wbytes, hipMemcpyHostToDevice, st));
-    PCHK(c, hipMemcpyAsync(d_irs, irs.data(), ibytes, hipMemcpyHostToDevice, st));
-    RphiArgs a{sx, sy, s, (double)xc, (double)yc, phiber, npx};
-    hipLaunchKernelGGL(k_rphi, dim3((unsigned)((cells + 255) / 256), nimg), dim3(256), 0, st, d_img, a, d_writer, d_irs,
-                       d_out);
+    PCHK(c, hipMemcpyAsync(d_writers, writers.data(), wbytes, hipMemcpyHostToDevice, st));
+    if (!irss.empty()) PCHK(c, hipMemcpyAsync(d_irs, irss.data(), irss.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    PCHK(c, hipMemcpyAsync(d_per, per.data(), pbytes, hipMemcpyHostToDevice, st));
+    RphiArgs a{sx, sy, s, phiber, npx};
+    hipLaunchKernelGGL(k_rphi, dim3((unsigned)((cells + 255) / 256), nimg), dim3(256), 0, st, d_img, a, d_per, d_writers,
+                       d_irs, d_out);
     PCHK(c, hipGetLastError());
     // the host vectors must outlive the copies
     PCHK(c, hipStreamSynchronize(st));
     return csx_post_scale_clip(c, d_out, cells, nimg, scale, clip, clip40, 0);
+}
+
+int csx_post_rphi(csx_ctx* c, const double* d_img, int x0, int x1, int y0, int y1, int xc, int yc, int s, int nimg,
+                  double* d_out, int scale, int clip, int clip40) {
+    if (!c) return CSX_E_ARG;
+    if (nimg <= 0 || nimg > 65535) return csx_ctx_fail(c, CSX_E_ARG, "bad image batch");
+    std::vector<int32_t> centres((size_t)nimg * 2);
+    for (int k = 0; k < nimg; ++k) {
+        centres[2 * (size_t)k] = xc;
+        centres[2 * (size_t)k + 1] = yc;
+    }
+    return csx_post_rphi_many(c, d_img, x0, x1, y0, y1, centres.data(), s, nimg, d_out, scale, clip, clip40);
 }
 
 int csx_post_ifft(csx_ctx* c, const double* d_in, int s, int nimg, int forward, int out_type, int scaled, int clip,

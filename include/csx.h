@@ -314,6 +314,10 @@ int csx_post_scale_clip(csx_ctx* ctx, double* d_img, int64_t npx, int nimg, int 
                         int clip40, int genlog);
 int csx_post_rphi(csx_ctx* ctx, const double* d_img, int x0, int x1, int y0, int y1, int xc, int yc,
                   int s, int nimg, double* d_out, int scale, int clip, int clip40);
+/* rphi_many: the same with one centre per image (h_centres: nimg x {xc, yc}, host memory), in
+ *   one launch; images with the same rounded maximum radius share one cell map. */
+int csx_post_rphi_many(csx_ctx* ctx, const double* d_img, int x0, int x1, int y0, int y1, const int32_t* h_centres,
+                       int s, int nimg, double* d_out, int scale, int clip, int clip40);
 /* FFT image: calculateIFFT of builds without FFTW (Source/sc_postproc.cpp:364-560, its radix-2
  * FFT2D/FFT :833-1000) for inputs of the output's size (s x s, s a power of two 4..2048, as the
  * r,phi image the AI loop passes): each input scaled by its maximum, 2D transform (forward 1 /

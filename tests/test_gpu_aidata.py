@@ -87,6 +87,31 @@ def test_rphi_matches_the_reference(oracle_lib, name):
     assert np.abs(r[1].cpu().numpy() - oracle_lib.scale_clip(o, genlog=True)).max() < 1e-12
 
 
+def test_rphi_many_centres_against_the_oracle(oracle_lib):
+    """One launch with a centre per image (cell maps shared by rounded maximum radius) gives
+    each image's r,phi as the oracle does for that centre alone."""
+    import ctypes
+    import torch
+    from crystalscatter_amd import _lib
+    f, post, img, b = _post_case("fcc_rphi")
+    centres = [(0, 0), (3, -4), (-7, 5), (3, -4), (12, 9), (-1, 0)]
+    n, s = len(centres), 32
+    imgs = torch.from_numpy(np.stack([img * (1.0 + 0.5 * k) for k in range(n)])).cuda()
+    out = torch.empty((n, s * s), dtype=torch.float64, device="cuda")
+    cen = np.array(centres, dtype=np.int32)
+    ctx = Context(0)
+    torch.cuda.synchronize()
+    x0, x1, y0, y1 = (int(v) for v in b)
+    ctx._chk(_lib.lib().csx_post_rphi_many(ctx._h, ctypes.c_void_p(imgs.data_ptr()), x0, x1, y0, y1,
+                                           cen.ctypes.data_as(ctypes.c_void_p), s, n,
+                                           ctypes.c_void_p(out.data_ptr()), 0, 0, 0), "csx_post_rphi_many")
+    ctx.synchronize()
+    got = out.cpu().numpy()
+    for k, (xc, yc) in enumerate(centres):
+        ref = oracle_lib.rphi(img * (1.0 + 0.5 * k), b, xc, yc, s, 0, 0, 0)
+        assert np.abs(got[k] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), k
+
+
 @pytest.mark.parametrize("name", ["fcc_rphi", "fcc_rphi_off", "fcc_rphi_clip"])
 def test_fft_matches_the_reference(name):
     """The FFT image of the reference's r,phi image (calculateIFFT without FFTW)."""
