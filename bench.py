@@ -126,7 +126,7 @@ def bench_fits(args):
                  hi=FIT_HI) for p in probs]
     ctx = Context(0)
     t0 = time.perf_counter()
-    res, st = nm_batch(ctx, fits, FIT_KEYS, stp=0.05, maxit=args.maxit, ftol=1e-6, precision=args.precision,
+    res, st = nm_batch(ctx, fits, FIT_KEYS, stp=0.05, maxit=args.maxit, ftol=1e-6, precision=args.precision, mode="corrected",
                        threads=args.cpu_threads)
     rounds = st["rounds"]
     wall = time.perf_counter() - t0
@@ -209,7 +209,7 @@ def bench_surrogate(args):
     fits = [dict(engine=p["engine"], d_data=p["data"].data_ptr(), w=p["w"], h=p["h"], x0=centre, lo=FIT_LO,
                  hi=FIT_HI) for p in probs]
     t0 = time.perf_counter()
-    res_p, st_p = nm_batch(ctx, fits, FIT_KEYS, stp=0.1, maxit=args.maxit, ftol=1e-6, precision=args.precision,
+    res_p, st_p = nm_batch(ctx, fits, FIT_KEYS, stp=0.1, maxit=args.maxit, ftol=1e-6, precision=args.precision, mode="corrected",
                            threads=args.cpu_threads)
     t_plain = time.perf_counter() - t0
     # warm-up of the proposal path (its torch kernels load on first use), on two fits
@@ -232,7 +232,7 @@ def bench_surrogate(args):
         f["x0"] = C[k]
     t0 = time.perf_counter()
     if args.maxit_surrogate > 0:
-        res_s, st_s = nm_batch(ctx, fits, FIT_KEYS, stp=0.02, maxit=args.maxit_surrogate, ftol=1e-6,
+        res_s, st_s = nm_batch(ctx, fits, FIT_KEYS, stp=0.02, maxit=args.maxit_surrogate, ftol=1e-6, mode="corrected",
                                precision=args.precision, threads=args.cpu_threads)
     else:  # no simplex: the exactly scored best candidate is the result
         res_s = [(C[k], float(exact[i, k]), 0, args.rescore) for i, (C, k) in enumerate(zip(cands, pick))]

@@ -350,18 +350,30 @@ def tpv_images(ctx, engine, ids, n, rng, precision="mixed", out=None):
     return out, reports
 
 
-def simplex(f, x0, lo, hi, stp=0.1, maxit=200, ftol=1e-6):
+def _nm_mode(mode):
+    modes = {"reference": _lib.NM_REFERENCE, "corrected": _lib.NM_CORRECTED}
+    if mode not in modes:
+        raise ValueError("mode must be 'reference' or 'corrected'")
+    return modes[mode]
+
+
+def simplex(f, x0, lo, hi, stp=0.1, maxit=200, ftol=1e-6, nfree=None, mode="reference"):
     """Nelder-Mead on a Python objective f(x) -> float with the simplex of doSimplexFit2D
-    (Source/sc_simplexfit2d.cpp:254-611).  Returns (x, f, iters, evals)."""
+    (Source/sc_simplexfit2d.cpp:254-611).  x0, lo, hi have npar entries (the reference's
+    parameter hash); the first nfree (default all) are fitted.  mode "reference" follows the
+    reference exactly (see csx_nm_run_cb), "corrected" is the textbook simplex over the fitted
+    entries.  Returns (x, f, iters, evals), x with all npar entries."""
     L = _lib.lib()
     n = len(x0)
     if n > _lib.NM_MAXPAR:
         raise ValueError("at most %d parameters" % _lib.NM_MAXPAR)
+    nfree = n if nfree is None else nfree
     cb = _lib.OBJECTIVE(lambda user, p: float(f(np.ctypeslib.as_array(p, shape=(n,)).copy())))
     arr = lambda v: np.ascontiguousarray(v, dtype=np.float64)
     x0, lo, hi = arr(x0), arr(lo), arr(hi)
     r = _lib.NMResult()
-    rc = L.csx_nm_run_cb(n, cb, None, _dp(x0), _dp(lo), _dp(hi), stp, maxit, ftol, ctypes.byref(r))
+    rc = L.csx_nm_run_cb(n, nfree, _nm_mode(mode), cb, None, _dp(x0), _dp(lo), _dp(hi), stp, maxit, ftol,
+                         ctypes.byref(r))
     if rc != 0:
         raise CsxError("csx_nm_run_cb failed (%d)" % rc)
     return np.array(r.x[:n]), r.f, r.iters, r.evals
@@ -404,19 +416,22 @@ def nm_eval(ctx, fits, keys, X, precision="mixed", threads=0):
     return out
 
 
-def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", threads=0):
+def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", threads=0, nfree=None,
+             mode="reference"):
     """Batched Nelder-Mead (csx_nm_batch).  fits: list of dicts with 'engine' (SasCalc, may be
     host-only), 'd_data' (device pointer), 'w', 'h', 'x0', 'lo', 'hi', optional 'bstop',
-    'border', 'rects'.  Returns (list of (x, f, iters, evals), stats dict: rounds, evals,
-    prep_ms, stage_ms, run_ms)."""
+    'border', 'rects'.  keys: the npar parameter names, the first nfree (default all) fitted.
+    mode: "reference" (doSimplexFit2D exactly) or "corrected" (see simplex).  Returns (list
+    of (x, f, iters, evals), stats dict: rounds, evals, prep_ms, stage_ms, run_ms)."""
     L = _lib.lib()
     n = len(keys)
+    nfree = n if nfree is None else nfree
     arr = _nm_fits(fits, n)
     ckeys = (ctypes.c_char_p * n)(*[k.encode() for k in keys])
     res = (_lib.NMResult * len(fits))()
     st = _lib.NMStats()
-    rc = L.csx_nm_batch(ctx._h if isinstance(ctx, Context) else ctx, len(fits), arr, n, ckeys, stp, maxit, ftol,
-                        _prec(precision), threads, res, ctypes.byref(st))
+    rc = L.csx_nm_batch(ctx._h if isinstance(ctx, Context) else ctx, len(fits), arr, n, nfree, _nm_mode(mode), ckeys,
+                        stp, maxit, ftol, _prec(precision), threads, res, ctypes.byref(st))
     if rc != 0:
         raise CsxError("csx_nm_batch failed (%d)" % rc)
     stats = {k: getattr(st, k) for k, _ in _lib.NMStats._fields_}

@@ -48,7 +48,8 @@ class Mask(ctypes.Structure):
     _fields_ = [("rects", ctypes.c_int32 * 16), ("bstop", ctypes.c_int32), ("border", ctypes.c_int32)]
 
 
-NM_MAXPAR = 16
+NM_MAXPAR = 30
+NM_REFERENCE, NM_CORRECTED = 0, 1
 OBJECTIVE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
 
 
@@ -92,8 +93,8 @@ SIGNATURES = [
                                  ctypes.POINTER(Mask), _VP]),
     ("csx_fit_batch_run", _I, [_VP, _I, _PD, _PI64]),
     ("csx_render_batch_run", _I, [_VP, _I]),
-    ("csx_nm_run_cb", _I, [_I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
-    ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
+    ("csx_nm_run_cb", _I, [_I, _I, _I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
+    ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, _I, _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
                           ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),
     ("csx_nm_eval", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _I, _PD, _I, _I, _PD]),
     ("csx_last_error", _I, [_VP, _CP, _I]),

@@ -27,33 +27,55 @@ constexpr double kOutOfBounds = 1e20;
 
 class NelderMead {
 public:
-    void init(int n, const double* x0, const double* lo, const double* hi, double stp, int maxit, double ftol) {
-        n_ = n;
+    // m: length of the parameter vector (the reference's mmax: every entry of the fit's
+    // parameter hash); nf: the first nf entries are fitted (numFit), the rest are fixed.
+    void init(int m, int nf, int mode, const double* x0, const double* lo, const double* hi, double stp, int maxit,
+              double ftol) {
+        m_ = m;
+        nf_ = nf;
+        ref_ = mode == CSX_NM_REFERENCE;
         maxit_ = maxit;
         ftol_ = ftol;
-        std::memcpy(lo_, lo, n * sizeof(double));
-        std::memcpy(hi_, hi, n * sizeof(double));
+        std::memcpy(lo_, lo, m * sizeof(double));
+        std::memcpy(hi_, hi, m * sizeof(double));
+        std::memcpy(x0_, x0, m * sizeof(double));
         std::memset(p_, 0, sizeof p_);
         std::memset(y_, 0, sizeof y_);
-        const double rootn1 = std::sqrt(n + 1.0) - 1.0, mroot2 = 1.0 / (n * std::sqrt(2.0));
+        // The reference sizes the steps and the trial factor by mmax (:343-346, :231); the
+        // corrected simplex by the number of fitted parameters.
+        d_ = ref_ ? m : nf;
+        const double rootn1 = std::sqrt(d_ + 1.0) - 1.0, mroot2 = 1.0 / (d_ * std::sqrt(2.0));
         double pp[CSX_NM_MAXPAR], qv[CSX_NM_MAXPAR];
-        for (int i = 0; i < n; ++i) {
-            double step;
-            if (x0[i] == 0) step = (lo[i] == 0) ? stp : lo[i] * stp;
-            else step = x0[i] * stp;
-            pp[i] = step * (rootn1 + n) * mroot2;
+        for (int i = 0; i < m; ++i) {
+            double step = 0.0;
+            if (i < nf) {
+                if (x0[i] == 0) step = (lo[i] == 0) ? stp : lo[i] * stp;
+                else step = x0[i] * stp;
+            }
+            pp[i] = step * (rootn1 + d_) * mroot2;
             qv[i] = step * rootn1 * mroot2;
             p_[0][i] = x0[i];
         }
-        for (int i = 1; i <= n; ++i) {
-            for (int j = 0; j < n; ++j) p_[i][j] = p_[0][j] + qv[j];
+        // Reference: the clearing loop of :384-389 runs its inner loop over i, so it zeroes
+        // the first entry of rows 0..29, vertex 0's included, after the vertex was set.
+        // y[0] is still evaluated at the start values (:392).
+        if (ref_)
+            for (int i = 0; i < CSX_NM_MAXPAR; ++i) p_[i][0] = 0.0;
+        for (int i = 1; i <= d_; ++i) {
+            for (int j = 0; j < m; ++j) p_[i][j] = p_[0][j] + qv[j];
             p_[i][i - 1] = p_[i][i - 1] + pp[i - 1];
         }
-        phase_ = kInit0;
+        phase_ = kInit;
         rep_ = 0;
         evals_ = 0;
         done_ = false;
-        post({0});
+        // y[0] at x0, then vertices 1..d; the reference evaluates up to vertex nf+1 and copies
+        // y[nf+1] into the rest (:411-424).  The values do not depend on each other, so they
+        // are posted together.
+        std::vector<const double*> pts{x0_};
+        nev0_ = std::min(d_, nf_ + 1);
+        for (int i = 1; i <= nev0_; ++i) pts.push_back(p_[i]);
+        post(pts);
         drain();
     }
 
@@ -64,7 +86,7 @@ public:
     // Values of the pending points (in posting order); advances to the next request.
     void deliver(const double* f) {
         evals_ += npend_;
-        double v[CSX_NM_MAXPAR + 1];
+        double v[CSX_NM_MAXPAR + 2];
         for (int k = 0; k < nall_; ++k) v[k] = kOutOfBounds;
         for (int w = 0; w < npend_; ++w) v[slot_[w]] = f[w];
         step(v);
@@ -72,25 +94,23 @@ public:
     }
 
     void result(csx_nm_result& r) const {
-        for (int j = 0; j < n_; ++j) r.x[j] = p_[ilo_][j];
+        for (int j = 0; j < m_; ++j) r.x[j] = p_[ilo_][j];
         r.f = y_[ilo_];
         r.iters = rep_;
         r.evals = evals_;
     }
 
 private:
-    enum Phase { kInit0, kInitV, kReflect, kExpand, kContract, kShrink };
+    enum Phase { kInit, kReflect, kExpand, kContract, kShrink };
 
-    // Post points: rows of p_ (ids >= 0) or the trial point (id -1).  Points outside the
-    // bounds are not evaluated: they get 1e20, as the reference's objective returns.
-    void post(std::initializer_list<int> ids) { post(std::vector<int>(ids)); }
-    void post(const std::vector<int>& ids) {
-        nall_ = (int)ids.size();
+    // Post points.  Points outside the bounds are not evaluated: they get 1e20, as the
+    // reference's objective returns (:25-44).
+    void post(const std::vector<const double*>& xs) {
+        nall_ = (int)xs.size();
         npend_ = 0;
         for (int k = 0; k < nall_; ++k) {
-            const double* x = ids[k] < 0 ? pt_ : p_[ids[k]];
-            if (out_of_bounds(x)) continue;
-            std::memcpy(pts_[npend_], x, n_ * sizeof(double));
+            if (out_of_bounds(xs[k])) continue;
+            std::memcpy(pts_[npend_], xs[k], m_ * sizeof(double));
             slot_[npend_++] = k;
         }
         resolve_now_ = npend_ == 0;
@@ -100,47 +120,51 @@ private:
     void drain() {
         while (resolve_now_ && !done_) {
             resolve_now_ = false;
-            double v[CSX_NM_MAXPAR + 1];
+            double v[CSX_NM_MAXPAR + 2];
             for (int k = 0; k < nall_; ++k) v[k] = kOutOfBounds;
             npend_ = 0;
             step(v);
         }
     }
 
+    // Only the fitted entries are checked (enabled[j], :27-42); bounds 0..360 are exempt.
     bool out_of_bounds(const double* x) const {
-        for (int j = 0; j < n_; ++j) {
+        for (int j = 0; j < nf_; ++j) {
             if (lo_[j] == 0.0 && hi_[j] == 360.0) continue;
             if (x[j] < lo_[j] || x[j] > hi_[j]) return true;
         }
         return false;
     }
 
-    // amotry (Source/sc_simplexfit2d.cpp:221-251): trial point of factor fac.
+    // amotry (Source/sc_simplexfit2d.cpp:221-251): trial point of factor fac.  The reference
+    // copies every entry of psum first, so its trial points carry the fixed entries at their
+    // column sums (:226); the corrected simplex keeps them at their start values.
     void trial(double fac) {
-        const double f1 = (1.0 - fac) / n_, f2 = f1 - fac;
-        for (int j = 0; j < n_; ++j) pt_[j] = psum_[j] * f1 - p_[ihi_][j] * f2;
+        const double f1 = (1.0 - fac) / d_, f2 = f1 - fac;
+        for (int j = 0; j < m_; ++j) pt_[j] = ref_ ? psum_[j] : x0_[j];
+        for (int j = 0; j < nf_; ++j) pt_[j] = psum_[j] * f1 - p_[ihi_][j] * f2;
     }
     void accept_if_better(double yt) {
         if (yt < y_[ihi_]) {
             y_[ihi_] = yt;
-            for (int j = 0; j < n_; ++j) {
+            for (int j = 0; j < nf_; ++j) {
                 psum_[j] += pt_[j] - p_[ihi_][j];
                 p_[ihi_][j] = pt_[j];
             }
         }
     }
     void sum_rows() {
-        for (int j = 0; j < n_; ++j) {
+        for (int j = 0; j < m_; ++j) {
             double s = 0.0;
-            for (int i = 0; i <= n_; ++i) s += p_[i][j];
+            for (int i = 0; i <= nf_; ++i) s += p_[i][j];
             psum_[j] = s;
         }
     }
 
-    // Start of one simplex iteration (the body of the reference's `rep` loop).
+    // Start of one simplex iteration (the body of the reference's `rep` loop, :440-600).
     void iterate() {
         if (rep_ >= maxit_) { finish(); return; }
-        const int mpts = n_ + 1;
+        const int mpts = nf_ + 1;
         ilo_ = 0;
         if (y_[0] > y_[1]) { ihi_ = 0; inhi_ = 1; }
         else { ihi_ = 1; inhi_ = 0; }
@@ -157,7 +181,7 @@ private:
         if (rtol <= ftol_) { finish(); return; }
         trial(-1.0);  // reflection, alpha = 1
         phase_ = kReflect;
-        post({-1});
+        post({pt_});
     }
 
     void next_iteration() {
@@ -167,16 +191,10 @@ private:
 
     void step(const double* v) {
         switch (phase_) {
-        case kInit0: {
+        case kInit:
             y_[0] = v[0];
-            std::vector<int> ids;
-            for (int i = 1; i <= n_; ++i) ids.push_back(i);
-            phase_ = kInitV;
-            post(ids);
-            return;
-        }
-        case kInitV:
-            for (int i = 1; i <= n_; ++i) y_[i] = v[i - 1];
+            for (int i = 1; i <= nev0_; ++i) y_[i] = v[i];
+            for (int i = nev0_ + 1; i <= d_; ++i) y_[i] = y_[nf_ + 1];
             sum_rows();
             rep_ = 0;
             iterate();
@@ -187,12 +205,12 @@ private:
             if (yt <= y_[ilo_]) {
                 trial(2.0);  // expansion, gamma = 2
                 phase_ = kExpand;
-                post({-1});
+                post({pt_});
             } else if (yt >= y_[inhi_]) {
                 ysave_ = y_[ihi_];
                 trial(0.5);  // contraction, beta = 0.5
                 phase_ = kContract;
-                post({-1});
+                post({pt_});
             } else {
                 next_iteration();
             }
@@ -206,14 +224,15 @@ private:
             const double yt = v[0];
             accept_if_better(yt);
             if (yt >= ysave_) {
-                std::vector<int> ids;
-                for (int i = 0; i <= n_; ++i) {
+                // Shrink towards the best vertex over every entry (:566-582).
+                std::vector<const double*> xs;
+                for (int i = 0; i <= nf_; ++i) {
                     if (i == ilo_) continue;
-                    for (int j = 0; j < n_; ++j) p_[i][j] = 0.5 * (p_[i][j] + p_[ilo_][j]);
-                    ids.push_back(i);
+                    for (int j = 0; j < m_; ++j) p_[i][j] = 0.5 * (p_[i][j] + p_[ilo_][j]);
+                    xs.push_back(p_[i]);
                 }
                 phase_ = kShrink;
-                post(ids);
+                post(xs);
             } else {
                 next_iteration();
             }
@@ -221,7 +240,7 @@ private:
         }
         case kShrink: {
             int k = 0;
-            for (int i = 0; i <= n_; ++i)
+            for (int i = 0; i <= nf_; ++i)
                 if (i != ilo_) y_[i] = v[k++];
             sum_rows();
             next_iteration();
@@ -236,15 +255,16 @@ private:
         nall_ = 0;
     }
 
-    int n_ = 0, maxit_ = 0;
-    double ftol_ = 0.0, lo_[CSX_NM_MAXPAR], hi_[CSX_NM_MAXPAR];
+    int m_ = 0, nf_ = 0, d_ = 0, nev0_ = 0, maxit_ = 0;
+    bool ref_ = true;
+    double ftol_ = 0.0, lo_[CSX_NM_MAXPAR], hi_[CSX_NM_MAXPAR], x0_[CSX_NM_MAXPAR];
     double p_[CSX_NM_MAXPAR + 1][CSX_NM_MAXPAR], y_[CSX_NM_MAXPAR + 1], psum_[CSX_NM_MAXPAR], pt_[CSX_NM_MAXPAR];
     double ysave_ = 0.0;
     int ilo_ = 0, ihi_ = 0, inhi_ = 0, rep_ = 0, evals_ = 0;
-    Phase phase_ = kInit0;
+    Phase phase_ = kInit;
     bool done_ = false, resolve_now_ = false;
-    int nall_ = 0, npend_ = 0, slot_[CSX_NM_MAXPAR + 1];
-    double pts_[CSX_NM_MAXPAR + 1][CSX_NM_MAXPAR];
+    int nall_ = 0, npend_ = 0, slot_[CSX_NM_MAXPAR + 2];
+    double pts_[CSX_NM_MAXPAR + 2][CSX_NM_MAXPAR];
 };
 
 // Set one free parameter: a key may name several parameters separated by commas, which
@@ -260,6 +280,13 @@ void set_tied(csx::ParamSet& ps, const char* key, double v) {
             k += *c;
         }
     }
+}
+
+// Vector length, fitted count and mode: at least one fitted entry, and the reference's arrays
+// hold at most map = 30 entries (mmax < nap = 31, Source/sc_simplexfit2d.h:113-114, :318).
+bool nm_args_ok(int npar, int nfree, int mode) {
+    return npar >= 1 && npar <= CSX_NM_MAXPAR && nfree >= 1 && nfree <= npar &&
+           (mode == CSX_NM_REFERENCE || mode == CSX_NM_CORRECTED);
 }
 
 // Run f over [0, n) on up to `threads` host threads.
@@ -283,11 +310,11 @@ void parallel_for(int n, int threads, F&& f) {
 
 extern "C" {
 
-int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const double* lo, const double* hi,
-                  double stp, int maxit, double ftol, csx_nm_result* out) {
-    if (npar < 1 || npar > CSX_NM_MAXPAR || !f || !x0 || !lo || !hi || !out) return CSX_E_ARG;
+int csx_nm_run_cb(int npar, int nfree, int mode, csx_objective f, void* user, const double* x0, const double* lo,
+                  const double* hi, double stp, int maxit, double ftol, csx_nm_result* out) {
+    if (!nm_args_ok(npar, nfree, mode) || !f || !x0 || !lo || !hi || !out) return CSX_E_ARG;
     NelderMead nm;
-    nm.init(npar, x0, lo, hi, stp, maxit, ftol);
+    nm.init(npar, nfree, mode, x0, lo, hi, stp, maxit, ftol);
     std::vector<double> vals;
     while (!nm.done()) {
         vals.resize(nm.npending());
@@ -406,13 +433,14 @@ int check_fits(int nfits, const csx_nm_fit* fits, std::vector<const csx::ParamSe
 
 extern "C" {
 
-int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
-                 int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats) {
-    if (!ctx || nfits < 1 || !fits || npar < 1 || npar > CSX_NM_MAXPAR || !keys || !out) return CSX_E_ARG;
+int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
+                 const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
+                 csx_nm_result* out, csx_nm_stats* stats) {
+    if (!ctx || nfits < 1 || !fits || !nm_args_ok(npar, nfree, mode) || !keys || !out) return CSX_E_ARG;
     std::vector<const csx::ParamSet*> base;
     if (check_fits(nfits, fits, base)) return CSX_E_ARG;
     std::vector<NelderMead> nm(nfits);
-    for (int i = 0; i < nfits; ++i) nm[i].init(npar, fits[i].x0, fits[i].lo, fits[i].hi, stp, maxit, ftol);
+    for (int i = 0; i < nfits; ++i) nm[i].init(npar, nfree, mode, fits[i].x0, fits[i].lo, fits[i].hi, stp, maxit, ftol);
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     // Two groups of fits on two contexts (two streams): while the device evaluates one group's
     // round, the host prepares the other's.  Small batches use one group.

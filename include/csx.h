@@ -171,16 +171,34 @@ int csx_render_batch_run(csx_ctx* ctx, int precision);
 /* ---- Batched Nelder-Mead fits ------------------------------------------------------
  * The simplex of SasCalc_SimplexFit2D::doSimplexFit2D (Source/sc_simplexfit2d.cpp:254-611,
  * amotry :221-251) with the bounds rule of fehlerquadratsumme (:25-44: a point outside
- * [lo, hi] scores 1e20 without evaluation, except parameters with bounds exactly 0..360),
- * run as a resumable state machine.  csx_nm_batch advances many independent fits in
- * lockstep: every point they need in a round is prepared on host threads and evaluated with
- * the batched objective in one launch.  It replaces the sequential fehlerquadratsumme calls. */
-#define CSX_NM_MAXPAR 16
+ * [lo, hi] in a fitted entry scores 1e20 without evaluation, except entries with bounds
+ * exactly 0..360), run as a resumable state machine.  csx_nm_batch advances many independent
+ * fits in lockstep: every point they need in a round is prepared on host threads and
+ * evaluated with the batched objective in one launch.  It replaces the sequential
+ * fehlerquadratsumme calls.
+ *
+ * A parameter vector has npar entries (the reference's mmax: every entry of the fit's
+ * parameter hash); the first nfree are fitted (numFit), the others are the hash's fixed
+ * entries.  The reference orders the hash by name, fitted names first (:280-313); callers that
+ * want its path pass the entries in that order.
+ * CSX_NM_REFERENCE follows doSimplexFit2D exactly, including what it does beyond the textbook
+ * simplex:
+ *   - the clearing loop of :384-389 zeroes entry 0 of vertex 0 after the vertex was set, while
+ *     y[0] is evaluated at the start values; the other vertices are built from that vertex;
+ *   - the steps (:343-346) and the trial factor (:231) use npar, not nfree;
+ *   - trial points carry the fixed entries at their simplex column sums (:226), and the
+ *     objective sets them (:45-52);
+ *   - vertex nfree+1 is evaluated when npar > nfree, its value copied to the later ones (:420-428).
+ * CSX_NM_CORRECTED is the textbook simplex over the nfree fitted entries with the reference's
+ * step rule: vertex 0 kept, steps and trial factor over nfree, fixed entries at their start
+ * values, nfree+1 initial evaluations. */
+#define CSX_NM_MAXPAR 30   /* map (Source/sc_simplexfit2d.h:114) */
+enum { CSX_NM_REFERENCE = 0, CSX_NM_CORRECTED = 1 };
 typedef double (*csx_objective)(void* user, const double* x);
 typedef struct csx_nm_result {
-    double x[CSX_NM_MAXPAR];  /* best vertex, p[ilo] as the reference returns it */
+    double x[CSX_NM_MAXPAR];  /* best vertex, p[ilo] as the reference returns it (all npar entries) */
     double f;                 /* its objective */
-    int32_t iters, evals;     /* simplex iterations, objective evaluations */
+    int32_t iters, evals;     /* simplex iterations (repetitions), objective evaluations (numImgCalc) */
 } csx_nm_result;
 struct csx_engine;
 typedef struct csx_nm_fit {
@@ -190,9 +208,10 @@ typedef struct csx_nm_fit {
     csx_mask mask;
     double x0[CSX_NM_MAXPAR], lo[CSX_NM_MAXPAR], hi[CSX_NM_MAXPAR];
 } csx_nm_fit;
-/* One fit on a host objective (the same state machine; test and scripting use). */
-int csx_nm_run_cb(int npar, csx_objective f, void* user, const double* x0, const double* lo, const double* hi,
-                  double stp, int maxit, double ftol, csx_nm_result* out);
+/* One fit on a host objective (the same state machine; test and scripting use).  The
+ * objective receives all npar entries. */
+int csx_nm_run_cb(int npar, int nfree, int mode, csx_objective f, void* user, const double* x0, const double* lo,
+                  const double* hi, double stp, int maxit, double ftol, csx_nm_result* out);
 /* nfits fits over the npar parameters named in keys (a key "uca,ucb" sets both names to the
  * parameter's value); threads <= 0: all host cores. */
 typedef struct csx_nm_stats {
@@ -202,10 +221,11 @@ typedef struct csx_nm_stats {
     double stage_ms;          /* appending the built sets to the batch (serial) */
     double run_ms;            /* batched launch, reductions and the copy of the sums */
 } csx_nm_stats;
-int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, double stp,
-                 int maxit, double ftol, int precision, int threads, csx_nm_result* out, csx_nm_stats* stats);
+int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
+                 const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
+                 csx_nm_result* out, csx_nm_stats* stats);
 /* The batched objective at explicit points: out[i*nv + v] = fehlerquadratsumme of fit i at
- * x[(i*nv + v)*npar ...] (1e20 outside the bounds, like the simplex), all points of all fits in
+ * x[(i*nv + v)*npar ...] (1e20 outside the bounds of any entry), all points of all fits in
  * one batched launch.  No reference counterpart: used to score proposed starting points. */
 int csx_nm_eval(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, int nv,
                 const double* x, int precision, int threads, double* out);

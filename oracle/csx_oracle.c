@@ -1747,6 +1747,11 @@ int csxo_simplex(csxo_objective f, void* user, int numfit, int mmax, double* x, 
         pp[i] = step[i] * (rootn1 + mmax) * mroot2;
         qv[i] = step[i] * rootn1 * mroot2;
     }
+    /* The clearing loop of :384-389 advances i in its inner loop: it zeroes p[i][0] for
+     * i = 0..map-1, vertex 0's first entry included, after vertex 0 was set.  y[0] is
+     * evaluated at the start values (:392) and the other vertices are built from the zeroed
+     * vertex 0 (:411-417). */
+    for (int i = 0; i < MAP; ++i) p[i][0] = 0.0;
     y[0] = nm_eval(&c, vals);
     for (int i = 1; i < nmax; ++i) {
         for (int j = 0; j < mmax; ++j) p[i][j] = p[0][j] + qv[j];

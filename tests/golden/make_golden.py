@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv post lattices            # only these
+    python tests/golden/make_golden.py cube tpv post lattices fits       # only these
 """
 import json
 import os
@@ -25,6 +25,7 @@ sys.path.insert(0, HERE)
 from refbin import read_refbin  # noqa: E402
 
 REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+REF_FIT = os.path.join(ROOT, "oracle", "_ref", "ref_fit")
 REF_DOC = "/root/reference/Documentation"
 
 # Parameter sets (reference key names).  Config numbers follow the benchmark brief.
@@ -64,14 +65,14 @@ def cfg4_params(rng):
     return p
 
 
-def run_ref(params, extra=None):
+def run_ref(params, extra=None, driver=REF_DRIVER):
     with tempfile.TemporaryDirectory() as td:
         pf = os.path.join(td, "p.txt")
         with open(pf, "w") as fh:
             for k, v in list(params.items()) + list((extra or {}).items()):
                 fh.write("%s=%r\n" % (k, v) if not isinstance(v, str) else "%s=%s\n" % (k, v))
         out = os.path.join(td, "o.bin")
-        subprocess.run([REF_DRIVER, pf, out], check=True, capture_output=True)
+        subprocess.run([driver, pf, out], check=True, capture_output=True)
         return read_refbin(out)
 
 
@@ -153,6 +154,69 @@ POST_CASES = [("fcc_rphi", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 0, "b
               ("sphere_lin", dict(CFG1, GridPoints=32), {"scaleint": 1})]
 
 
+# Reference 2D simplex fits (oracle/_ref/ref_fit: doSimplexFit2D over the reference engine).
+# Each case: the parameters at the start, the hash entries (name, used, min, max), the simplex
+# arguments and the true parameters that made the target image.  Bounds are the key table's
+# (Source/sc_calc_generic.cpp:36-138; -10000..10000 where it gives none, as
+# SC_CalcCons::limitsOfParamValue reads them) unless a case narrows them on purpose.
+def _fit_cases():
+    rng = np.random.default_rng(1234)
+    bct_true = cfg4_params(rng)
+    bct_true["GridPoints"] = 16
+    bct_start = dict(bct_true, EditRadius=bct_true["EditRadius"] * 1.04, Length=bct_true["Length"] * 0.95,
+                     uca=bct_true["uca"] * 1.01, ucb=bct_true["uca"] * 1.01, ucc=bct_true["ucc"] * 0.99,
+                     EditDomainSize=bct_true["EditDomainSize"] * 1.1, EditAzi=bct_true["EditAzi"] * 0.9)
+    table = dict(EditAzi=(-10000, 10000), EditDomainSize=(0, 5000), EditRadius=(0, 1000), Length=(0, 1000),
+                 uca=(0.01, 1000), ucb=(0.01, 1000), ucc=(0.01, 1000), I0=(0, 100000), Base=(-10000, 10000),
+                 EditSigma=(-10000, 10000))
+    used6 = ["EditAzi", "EditDomainSize", "EditRadius", "Length", "uca", "ucc"]
+    cube_true = dict(CUBE_A, GridPoints=16)
+    cube_start = dict(cube_true, EditRadius=11.0, EditSigma=0.12)
+    return {
+        # six fitted entries and two fixed ones in the hash (mmax 8 > numFit 6)
+        "bct_fixed": dict(true=bct_true, start=bct_start, used=used6, fixed=["I0", "ucb"], bounds=table,
+                          stp=0.05, maxit=40, ftol=1e-10, bstop=0, border=0),
+        # three fitted entries only (mmax = numFit), with the beam-stop mask
+        "bct_three": dict(true=bct_true, start=bct_start, used=["EditDomainSize", "EditRadius", "Length"],
+                          fixed=[], bounds=table, stp=0.1, maxit=40, ftol=1e-10, bstop=4, border=0),
+        # first fitted entry bounded away from 0: the zeroed vertex 0 puts every other
+        # vertex out of bounds
+        "bct_outside": dict(true=bct_true, start=bct_start, used=["EditRadius", "Length"], fixed=[],
+                            bounds=dict(table, EditRadius=(2, 8), Length=(10, 60)), stp=0.05, maxit=10,
+                            ftol=1e-10, bstop=0, border=0),
+        # isotropic cube, two fitted entries, two fixed ones
+        "cube": dict(true=cube_true, start=cube_start, used=["EditRadius", "EditSigma"], fixed=["Base", "I0"],
+                     bounds=table, stp=0.05, maxit=60, ftol=1e-12, bstop=0, border=2),
+    }
+
+
+def write_fits(manifest):
+    arrays, meta = {}, {}
+    for name, c in _fit_cases().items():
+        target = run_ref(c["true"])["image"]
+        gp = c["true"]["GridPoints"]
+        w = 2 * gp
+        with tempfile.TemporaryDirectory() as td:
+            fd = os.path.join(td, "fit.bin")
+            target.astype("<f8").tofile(fd)
+            extra = dict(fitdata=fd, fitw=w, fith=w, bstop=c["bstop"], border=c["border"], stp=c["stp"],
+                         maxit=c["maxit"], ftol=c["ftol"], threads=1)
+            for i, k in enumerate(c["used"] + c["fixed"]):
+                lo, hi = c["bounds"][k]
+                extra["fitpar%d" % i] = "%s,%d,%r,%r" % (k, int(k in c["used"]), float(lo), float(hi))
+            r = run_ref(c["start"], extra, driver=REF_FIT)
+        arrays[name + "_target"] = target
+        arrays[name + "_fitres"] = r["fitres"]
+        arrays[name + "_fitstart"] = r["fitstart"]
+        meta[name] = dict(start=c["start"], true=c["true"], used=c["used"], fixed=c["fixed"],
+                          bounds={k: c["bounds"][k] for k in c["used"] + c["fixed"]}, stp=c["stp"],
+                          maxit=c["maxit"], ftol=c["ftol"], bstop=c["bstop"], border=c["border"], w=w,
+                          repetitions=int(r["repetitions"][0]), numImgCalc=int(r["numImgCalc"][0]),
+                          aborted=int(r["aborted"][0]), retinfo=r["retinfo"], log=r["log"])
+    np.savez_compressed(os.path.join(HERE, "simplex_fits.npz"), **arrays)
+    manifest["fixtures"]["simplex_fits.npz"] = {"driver": "oracle/ref_fit.cpp", "cases": meta}
+
+
 def write_post(manifest):
     arrays, cases = {}, {}
     for name, p, extra in POST_CASES:
@@ -182,6 +246,8 @@ def main():
             write_post(manifest)
         if "lattices" in only:
             write_lattices(manifest)
+        if "fits" in only:
+            write_fits(manifest)
         with open(os.path.join(HERE, "manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
         return
@@ -259,6 +325,7 @@ def main():
     write_tpv(manifest)
     write_post(manifest)
     write_lattices(manifest)
+    write_fits(manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))

@@ -31,7 +31,8 @@ def _problem(k, p, n=16):
     return e, p, d, w, x0, lo, hi
 
 
-def test_batched_nm_equals_one_fit_at_a_time():
+@pytest.mark.parametrize("mode", ["reference", "corrected"])
+def test_batched_nm_equals_one_fit_at_a_time(mode):
     import torch
     params = manifest()["fixtures"]["cfg4_bct.npz"]["params"][:4]
     probs = [_problem(k, p) for k, p in enumerate(params)]
@@ -41,7 +42,7 @@ def test_batched_nm_equals_one_fit_at_a_time():
         base.set("ignore_cbs", 1)
         fits.append(dict(engine=base, d_data=d.data_ptr(), w=w, h=w, x0=x0, lo=lo, hi=hi, bstop=2))
     ctx = Context(0)
-    res, stats = nm_batch(ctx, fits, KEYS, stp=0.1, maxit=25, ftol=1e-8, precision="mixed")
+    res, stats = nm_batch(ctx, fits, KEYS, stp=0.1, maxit=25, ftol=1e-8, precision="mixed", mode=mode)
     assert stats["rounds"] > 25
     for (e, p, d, w, x0, lo, hi), (xb, fb, itb, evb) in zip(probs, res):
         e.setFitData(w, w, d.cpu().numpy())
@@ -52,14 +53,15 @@ def test_batched_nm_equals_one_fit_at_a_time():
                     e.set(name, v)
             return e.doFitCalculation(bstop=2, border=0, precision="mixed")[0]
 
-        xs, fs, its, evs = simplex(f, x0, lo, hi, stp=0.1, maxit=25, ftol=1e-8)
+        xs, fs, its, evs = simplex(f, x0, lo, hi, stp=0.1, maxit=25, ftol=1e-8, mode=mode)
         np.testing.assert_array_equal(xb, xs)
         assert fb == fs and itb == its and evb == evs
         torch.cuda.synchronize()
 
 
 def test_batched_nm_lowers_the_objective():
-    """Noise-free targets: the fits move downhill from the initial simplex."""
+    """Noise-free targets: the fits move downhill from the initial simplex (corrected mode;
+    the reference's zeroed first entry leaves its vertices outside these bounds)."""
     params = manifest()["fixtures"]["cfg4_bct.npz"]["params"][:3]
     probs = [_problem(k, p, n=24) for k, p in enumerate(params)]
     fits = []
@@ -68,8 +70,8 @@ def test_batched_nm_lowers_the_objective():
         base.set("ignore_cbs", 1)
         fits.append(dict(engine=base, d_data=d.data_ptr(), w=w, h=w, x0=x0, lo=lo, hi=hi))
     ctx = Context(0)
-    start, _ = nm_batch(ctx, fits, KEYS, stp=0.05, maxit=0, ftol=1e-12, precision="mixed")
-    res, stats = nm_batch(ctx, fits, KEYS, stp=0.05, maxit=200, ftol=1e-12, precision="mixed")
+    start, _ = nm_batch(ctx, fits, KEYS, stp=0.05, maxit=0, ftol=1e-12, precision="mixed", mode="corrected")
+    res, stats = nm_batch(ctx, fits, KEYS, stp=0.05, maxit=200, ftol=1e-12, precision="mixed", mode="corrected")
     for (x0r, f0, _, _), (x, f, it, ev) in zip(start, res):
         assert np.isfinite(f) and f < 0.5 * f0, (f0, f)
 

@@ -77,30 +77,6 @@ def test_cfg2_reference_mode_exact(prec):
     np.testing.assert_array_equal(img, f["image"])
 
 
-def _tight_orientation_average(qx, qy, p):
-    """Longitudinal orientation average of the Gaussian-cylinder rod factor by adaptive
-    scipy quadrature (epsrel 1e-11), independent of the kernels and of the oracle."""
-    import math
-    from scipy import integrate
-    sl, L, db, th = p["SigmaL"], p["Length"], p["EditDbeta"] * np.pi / 180, p["theta"] * np.pi / 180
-    zl = (1 - sl * sl) / (sl * sl)
-    dmax = float(np.float32(min(db * math.sqrt(math.log(1e5)), np.pi / 2)))
-    p11, p13, p21 = float(np.float32(-math.cos(th))), float(np.float32(math.sin(th))), -1.0
-
-    def rod(arg):
-        a1 = 1 / (2 * zl * (zl - 1))
-        return (a1 / arg ** 2) * (1 - np.cos((zl - 1) * np.arctan(2 * arg)) / (1 + 4 * arg * arg) ** ((zl - 1) / 2))
-
-    def inner(d):
-        f = lambda c: rod((qx * (p11 * np.cos(c) * np.sin(d) + p13 * np.cos(d)) + qy * (p21 * np.sin(c) * np.sin(d))
-                           + 1e-10) * L / (zl + 1))
-        return integrate.quad(f, 0, 2 * np.pi, limit=800, epsrel=1e-11)[0] / (2 * np.pi)
-
-    w = lambda d: np.sin(d) * np.exp(-d * d / db ** 2)
-    norm = integrate.quad(w, 0, dmax, epsrel=1e-13)[0]
-    return integrate.quad(lambda d: w(d) * inner(d), 0, dmax, limit=400, epsrel=1e-10)[0] / norm
-
-
 def test_cfg2_physics_mode_ref64_matches_oracle(oracle_lib):
     """Physics mode (fix_orcase=1): the GPU's FP64 nested Romberg against the oracle's."""
     from helpers import CFG2
@@ -113,26 +89,22 @@ def test_cfg2_physics_mode_ref64_matches_oracle(oracle_lib):
     assert r.max() < 1e-12
 
 
-def test_cfg2_physics_mode_mixed_vs_tight_quadrature(oracle_lib):
-    """Physics mode, FP32 quadrature kernel at full size: against an independent adaptive
-    quadrature of the same integral at pixels across the detector (the reference's own
-    Romberg, eps 1e-4, is off by up to a few percent at large q, so it is not the bar here)."""
+def test_cfg2_physics_mode_mixed_vs_tight_quadrature():
+    """Physics mode, FP32 quadrature kernel at full size (512x512): against an independent
+    nested adaptive quadrature of the same integral (scipy, epsrel 1e-10/1e-11) at 96 pixels
+    across the detector (tests/golden/cfg2_tight.npz, tests/golden/make_cfg2_tight.py).  The
+    reference's own Romberg (eps 1e-4) is off by up to a few percent at large q, so it is not
+    the bar here."""
     from helpers import CFG2
     p = dict(CFG2, fix_orcase=1)
     e, img = _calc(p, "mixed")
     x0, x1, y0, y1 = e.bounds()
-    W = x1 - x0
-    o = oracle_lib.OracleEngine(p)
-    o.prepare()
-    ot = oracle_lib.OracleEngine(dict(p, Ordis=6, theta=0, phi=0))  # transverse factor (pql = 1)
-    ot.prepare()
-    worst = 0.0
-    for ih, ii in ((255, 255), (-256, 100), (40, -200), (0, 255), (120, 7), (-30, -40)):
-        qx, qy, qz = o.idx2q(ih, ii)
-        ref = _tight_orientation_average(qx, qy, p) * ot.pixel(qx, qy, qz)
-        worst = max(worst, abs(img[(ih - x0) + W * (ii - y0)] / ref - 1))
-    print("cfg2 physics mixed vs tight quadrature: max rel %.3e" % worst)
-    assert worst < 2e-6
+    assert (x1 - x0, y1 - y0) == (512, 512)
+    t = fixture("cfg2_tight.npz")
+    got = img[(t["ihex"] - x0) + (x1 - x0) * (t["i"] - y0)]
+    r = rel_err(got, t["value"])
+    _report("cfg2 physics mixed vs tight quadrature (96 px)", r)
+    assert r.size == 96 and r.max() < 2e-6
 
 
 def test_cfg2_physics_small_q_series_same_in_both_precisions():
@@ -184,6 +156,36 @@ def test_cfg3_fcc_full_size(prec, tol):
     # at most the dropped mass (about 1e-8 relative here), far below the 1e-4 bound.
     a = img.reshape(1024, 1024)[1:, 1:]
     assert rel_err(a, a[::-1, ::-1]).max() < (1e-13 if prec == "ref64" else 1e-6)
+
+
+def test_cfg3_mixed_every_pixel_against_ref64():
+    """All 1,048,576 pixels of the headline pattern: the mixed kernel against the GPU's FP64
+    reference mode, which is itself pinned to the reference's own pixels (above, <= 1e-12)."""
+    e, mixed = _calc(CFG3, "mixed")
+    _, ref = _calc(CFG3, "ref64")
+    assert mixed.size == ref.size == 1024 * 1024
+    assert np.isfinite(ref).all()
+    r = rel_err(mixed, ref)
+    _report("cfg3 1024^2 every pixel mixed vs ref64", r)
+    assert r.max() < 1e-4 and np.sqrt(np.mean(r * r)) < 1e-6
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_bct_mixed_fit_sizes_against_ref64(n):
+    """The BCT mixed kernel (precise pair loop, conditioning guard) at fit detector sizes
+    256x256 and 512x512 against the GPU's FP64 reference mode, on the six fixture draws."""
+    worst, rms = 0.0, 0.0
+    for p in manifest()["fixtures"]["cfg4_bct.npz"]["params"]:
+        q = dict(p, GridPoints=n)
+        _, mixed = _calc(q, "mixed")
+        _, ref = _calc(q, "ref64")
+        assert mixed.size == (2 * n) ** 2
+        ok = np.abs(ref) > 0
+        r = rel_err(mixed[ok], ref[ok])
+        worst = max(worst, float(r.max()))
+        rms = max(rms, float(np.sqrt(np.mean(r * r))))
+    print("bct %dx%d mixed vs ref64: max rel %.3e  worst rms %.3e" % (2 * n, 2 * n, worst, rms), flush=True)
+    assert worst < 1e-4
 
 
 def test_cfg3_full_size_against_oracle_random_pixels(oracle_lib):
