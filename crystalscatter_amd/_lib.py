@@ -88,6 +88,7 @@ SIGNATURES = [
     ("csx_render_stats", _I, [_VP, ctypes.POINTER(Grid), _I, _VP, ctypes.POINTER(ctypes.c_uint64)]),
     ("csx_qgrid", _I, [_VP, ctypes.POINTER(Grid), _VP]),
     ("csx_fit_fqs", _I, [_VP, _I, _I, ctypes.POINTER(Mask), _I, _VP, _PD, _PI64, _PI64, _VP]),
+    ("csx_fit_fqs_noalloc", _I, [_VP, _I, _I, ctypes.POINTER(Mask), _I, _VP, _PD, _PI64, _PI64, _VP]),
     ("csx_fit_batch_begin", _I, [_VP]),
     ("csx_fit_batch_stage", _I, [_VP, ctypes.POINTER(Scalars), _PD, _PI32, _PI32, _PF, _I, _I,
                                  ctypes.POINTER(Mask), _VP]),

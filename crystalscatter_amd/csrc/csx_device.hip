@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -1255,7 +1256,30 @@ __global__ void k_reduce_sets(const FitSet* __restrict__ sets, int nsets, const 
     cnts[s] = c;
 }
 
+// Sum of n tile partials in tile order, by one lane: the same additions in the same order as
+// the host sum of csx_fit_fqs, so the total has the same bits.
+__global__ void k_sum_tiles(const TileSum* __restrict__ part, int n, TileSum* __restrict__ total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double t = 0.0;
+    int32_t c = 0, nn = 0;
+    for (int k = 0; k < n; ++k) {
+        t += part[k].sum;
+        c += part[k].cnt;
+        nn += part[k].nan;
+    }
+    total->sum = t;
+    total->cnt = c;
+    total->nan = nn;
+}
+
 }  // namespace csx
+
+// Scratch of csx_fit_fqs_noalloc: the tile partials of one fit array (up to kScratchTiles
+// 16x16 tiles, 2048 x 2048 pixels) and their total, 256 KiB per GPU for the life of the
+// process.  Host code never uses this name as an address: the device address of each GPU's
+// copy comes from hipGetSymbolAddress (scratch_address below).
+constexpr int kScratchTiles = 16384;
+static __device__ __align__(256) csx::TileSum scratch[kScratchTiles + 1];
 
 // ===========================================================================
 // C ABI
@@ -1631,6 +1655,30 @@ static int fail(csx_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(ctx, CSX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device address of `scratch` on each GPU, looked up once per device with that device
+// current (csx_create).
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_lookup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+    return e;
+}
+
+TileSum* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return static_cast<TileSum*>(g_scratch_ptr[device]);
+}
+}  // namespace
+
 // Radial pre-pass for kinds whose form factors depend on |q| only, in the midpoint grid.
 template <int K, int P>
 static void launch_radial(const DevSet& d, const csx_grid& g, Radial* rtab, hipStream_t st) {
@@ -1777,7 +1825,8 @@ int csx_create(int device, csx_ctx** out) {
         hipEventCreateWithFlags(&c->ev_rad[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_rad[1], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_img[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_img[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_img[1], hipEventDisableTiming) != hipSuccess ||
+        scratch_lookup(device) != hipSuccess) {
         c->own_stream = c->stream != nullptr;
         csx_destroy(c);
         return CSX_E_HIP;
@@ -2000,6 +2049,53 @@ int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, c
     *h_sum = sum;
     if (h_cnt) *h_cnt = cnt;
     if (h_nan) *h_nan = nn;
+    return CSX_OK;
+}
+
+// The objective of csx_fit_fqs with its tile partials in the static scratch array instead of
+// context memory, and their sum taken on the device: nothing is allocated, on the device or
+// on the host.
+int csx_fit_fqs_noalloc(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, const double* d_data,
+                        double* h_sum, int64_t* h_cnt, int64_t* h_nan, double* d_fqs) {
+    if (!c || !mask || !d_data || !h_sum || w <= 0 || h <= 0) return CSX_E_ARG;
+    if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_fit_fqs_noalloc before csx_upload");
+    const csx_scalars& s = c->set.s;
+    if ((long long)(s.xmax - s.xmin) * (s.ymax - s.ymin) < (long long)w * h || (s.xmax - s.xmin) != w)
+        return fail(c, CSX_E_ARG, "fit array must match the image width and fit inside the image");
+    if (s.iimin != s.xmin || s.iimin != s.ymin)
+        return fail(c, CSX_E_ARG, "fit needs the full (Q4) or first-quadrant (Q1) image layout");
+    const int tiles_x = (w + kTile - 1) / kTile;
+    const long long ntiles = (long long)tiles_x * ((h + kTile - 1) / kTile);
+    if (ntiles > kScratchTiles)
+        return fail(c, CSX_E_ARG, "csx_fit_fqs_noalloc: " + std::to_string(ntiles) + " tiles of 16x16 exceed its " +
+                                      std::to_string(kScratchTiles) + " (at most 2048 x 2048 fit pixels)");
+    TileSum* part = scratch_address(c->device);
+    if (!part) return fail(c, CSX_E_STATE, "csx_fit_fqs_noalloc: no scratch address for this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    FitSet f{};
+    f.d = c->set;
+    f.m = *mask;
+    f.data = d_data;
+    f.w = w;
+    f.h = h;
+    f.tiles_x = tiles_x;
+    const int kind = kind_of(s.combo_key);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    DISPATCH_KIND(kind, precision, launch_fit_one, f, (int)ntiles, d_fqs, part, c->stream);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(64), 0, c->stream, part, (int)ntiles, part + kScratchTiles);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    TileSum total{};
+    HIPCHK(c, hipMemcpyAsync(&total, part + kScratchTiles, sizeof(TileSum), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->timing_pending = false;
+    *h_sum = total.sum;
+    if (h_cnt) *h_cnt = total.cnt;
+    if (h_nan) *h_nan = total.nan;
     return CSX_OK;
 }
 

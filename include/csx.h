@@ -153,6 +153,18 @@ int csx_qgrid(csx_ctx* ctx, const csx_grid* grid, double* d_q3);
 int csx_fit_fqs(csx_ctx* ctx, int w, int h, const csx_mask* mask, int precision, const double* d_data,
                 double* h_sum, int64_t* h_cnt, int64_t* h_nancnt, double* d_fqs);
 
+/* csx_fit_fqs without allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving or fitting loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as csx_fit_fqs (bit for bit), but the
+ * per-tile partial sums live in a static device array of the library (one per GPU, 256 KiB,
+ * held until the process ends) and are summed on the device; nothing is allocated on the
+ * device or the host.  Limit: (w/16 rounded up) x (h/16 rounded up) <= 16384 tiles, i.e. at
+ * most 2048 x 2048 fit pixels; larger arrays return CSX_E_ARG.  Every call on one GPU uses
+ * the same array, so calls that overlap on one GPU (other threads, other contexts) race;
+ * the caller serialises them.  Synchronous, like csx_fit_fqs. */
+int csx_fit_fqs_noalloc(csx_ctx* ctx, int w, int h, const csx_mask* mask, int precision, const double* d_data,
+                        double* h_sum, int64_t* h_cnt, int64_t* h_nancnt, double* d_fqs);
+
 /* Batched objective: many parameter sets, each with its own tables, detector size, data and
  * mask, evaluated in one persistent launch.  Sets are staged with csx_fit_batch_stage after
  * csx_fit_batch_begin; csx_fit_batch_run evaluates all staged sets and writes h_sums[nsets]. */
