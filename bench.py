@@ -115,11 +115,13 @@ def fit_problems(nfits, seed=1234, noise_seed=5678):
     return probs
 
 
-def bench_fits(args):
+def run_fits(args, probs=None):
+    """Config 4: the batched simplex (corrected mode) over the fit workload; returns its line."""
     import torch
     from crystalscatter_amd import Context, nm_batch
     t0 = time.perf_counter()
-    probs = fit_problems(args.nfits)
+    if probs is None:
+        probs = fit_problems(args.nfits)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     fits = [dict(engine=p["engine"], d_data=p["data"].data_ptr(), w=p["w"], h=p["h"], x0=p["x0"], lo=FIT_LO,
@@ -134,7 +136,8 @@ def bench_fits(args):
     evals = np.array([r[3] for r in res])
     iters = np.array([r[2] for r in res])
     rel0 = np.array([np.abs(p["x0"] / p["truth"] - 1) for p in probs])
-    rel1 = np.array([np.abs(r[0] / p["truth"] - 1) for p, r in zip(probs, res)])
+    rel1 = np.array([np.abs(r[0][:6] / p["truth"] - 1) for p, r in zip(probs, res)])
+    chi2 = np.array([r[1] for r in res])
     line = {
         "metric": "2D fits/s (batched Nelder-Mead, exact kernel)",
         "value": len(probs) / wall,
@@ -150,9 +153,11 @@ def bench_fits(args):
         "data": "synthetic",
         "config": {"workload": "%d independent fits, BCT cylinders pCyl_pGau_lBCT_oZDi (key 0x01020806), "
                                "6 free parameters, detectors 128..512, Poisson targets, 8 px beam stop, "
-                               "maxit %d" % (len(probs), args.maxit),
-                   "precision": args.precision, "pixels_per_round_mean": px},
+                               "maxit %d, corrected simplex" % (len(probs), args.maxit),
+                   "precision": args.precision, "pixels_per_round_mean": px,
+                   "detectors": {str(n): int(sum(1 for p in probs if p["n"] == n)) for n in (128, 256, 384, 512)}},
         "rounds": int(rounds),
+        "chi2_median": float(np.median(chi2)),
         "evals_mean": float(evals.mean()),
         "iters_mean": float(iters.mean()),
         "median_rel_param_err_start": float(np.median(rel0)),
@@ -160,7 +165,11 @@ def bench_fits(args):
         "target_generation_s": t_gen,
         "host_prep_ms": st["prep_ms"], "staging_ms": st["stage_ms"], "batched_run_ms": st["run_ms"],
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_fits(args):
+    print(json.dumps(run_fits(args)), flush=True)
 
 
 # Matrix-core work per training row of the surrogate, algorithmic (70 inputs, not the padded
@@ -171,7 +180,7 @@ MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * 3 * 256 * 256 + 2 * 256
 PEAK_BF16_TFLOPS = 2500.0
 
 
-def bench_surrogate(args):
+def run_surrogate(args, probs=None):
     """Surrogate workload: simulate training patterns with the exact kernel, train the network
     (BF16 MFMA forward/backward/Adam), measure its held-out log10 error, then run the same
     fits from an uninformed start (the centre of the box) twice: plain batched Nelder-Mead
@@ -202,7 +211,8 @@ def bench_surrogate(args):
     rms_log10 = float(torch.sqrt((err * err).mean()))
     med_abs_log10 = float(err.abs().median())
 
-    probs = fit_problems(args.nfits)
+    if probs is None:
+        probs = fit_problems(args.nfits)
     torch.cuda.synchronize()
     centre = 0.5 * (FIT_LO + FIT_HI)
     ctx = Context(0)
@@ -277,7 +287,11 @@ def bench_surrogate(args):
                      "traffic": None, "kernel": "training step (embed + 4 fwd + 3 bwd + 4 weight-gradient GEMMs)",
                      "flop_per_row": MLP_TRAIN_FLOP},
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_surrogate(args):
+    print(json.dumps(run_surrogate(args)), flush=True)
 
 
 def bench_bct(args, nsets=8):
@@ -420,8 +434,86 @@ def bench_aidata(args):
     print(json.dumps(line), flush=True)
 
 
+CFG1 = dict(LType=12, ComboBoxParticle=0, Ordis=7, ComboBoxInterior=0, EditRadius=5, EditSigma=0.1,
+            GridPoints=64, EditQmax=2, I0=1, Base=0, iso=0)
+CFG2 = dict(LType=12, ComboBoxParticle=1, Ordis=0, ComboBoxInterior=0, EditRadius=3, Length=40,
+            EditSigma=0.06, SigmaL=0.06, EditDbeta=10, theta=30, phi=45, GridPoints=256, EditQmax=1.5,
+            I0=1, Base=0)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _rate(eng, out, precision, steps):
+    """Back-to-back device renders of one uploaded parameter set: (patterns/s, image)."""
+    import torch
+    eng.doCalculationDevice(out.data_ptr(), precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.doCalculationDevice(out.data_ptr(), precision)
+    torch.cuda.synchronize()
+    return steps / (time.perf_counter() - t0), out.cpu().numpy()
+
+
+def _err(a, b):
+    r = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    return float(r.max()), float(np.sqrt(np.mean(r * r)))
+
+
+def run_config1(args):
+    """Config 1: isotropic Schulz spheres, no lattice, 128x128, FP64 (reference mode): rate and
+    error against the reference's own image (tests/golden/cfg1_sphere.npz)."""
+    import torch
+    from crystalscatter_amd import SasCalc
+    eng = SasCalc(device=0, params=CFG1)
+    eng.prepare()
+    out = torch.empty(128 * 128, dtype=torch.float64, device="cuda")
+    rate, img = _rate(eng, out, "ref64", 500)
+    ref = np.load(os.path.join(GOLDEN, "cfg1_sphere.npz"))["image"]
+    mx, rms = _err(img, ref)
+    return {"workload": "pSph_lNon_oIso (key 0x00000C07) 128x128, q in [-2,2], R 5 nm, sigma 0.1, FP64",
+            "precision": "ref64", "patterns_per_s": rate, "max_rel_err_vs_reference": mx,
+            "rms_rel_err_vs_reference": rms, "checked_pixels": int(ref.size),
+            "reference": "the reference's own FP64 image (oracle/_ref, tests/golden/cfg1_sphere.npz)"}
+
+
+def run_config2(args):
+    """Config 2: Gaussian-oriented cylinders, 512x512, physics mode (converged orientation
+    average) with the FP32 quadrature kernel: rate and error against an independent nested
+    adaptive quadrature at 96 pixels (tests/golden/cfg2_tight.npz); the FP64 Romberg kernel's
+    rate beside it."""
+    import torch
+    from crystalscatter_amd import SasCalc
+    eng = SasCalc(device=0, params=dict(CFG2, fix_orcase=1))
+    eng.prepare()
+    out = torch.empty(512 * 512, dtype=torch.float64, device="cuda")
+    rate, img = _rate(eng, out, "mixed", 20)
+    t = np.load(os.path.join(GOLDEN, "cfg2_tight.npz"))
+    x0, x1, y0, y1 = eng.bounds()
+    got = img[(t["ihex"] - x0) + (x1 - x0) * (t["i"] - y0)]
+    mx, rms = _err(got, t["value"])
+    rate64, _ = _rate(eng, out, "ref64", 2)
+    return {"workload": "pCyl_lNon_oGau (key 0x01000C00) 512x512, q in [-1.5,1.5], R 3, L 40, sigma 0.06, "
+                        "Gaussian orientation 10 deg about theta 30, phi 45; physics mode",
+            "precision": "mixed (FP32 quadrature 64x64 nodes, FP64 series)", "patterns_per_s": rate,
+            "max_rel_err_vs_tight_quadrature": mx, "rms_rel_err_vs_tight_quadrature": rms, "checked_pixels": 96,
+            "ref64_romberg_patterns_per_s": rate64,
+            "reference": "scipy nested adaptive quadrature, epsrel 1e-10 (tests/golden/make_cfg2_tight.py)"}
+
+
+def _host_cpu():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(threads):
-    """FP64 oracle (bit-exact restatement of the reference CPU path) on the host cores."""
+    """FP64 oracle (bit-exact restatement of the reference CPU path) on the host cores, with
+    the oracle/reference time ratio measured on the build machine (oracle/cpu_calibration.json)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker only: the CPU baseline leg
     o = oracle.OracleEngine(CFG3)
@@ -429,9 +521,23 @@ def cpu_baseline(threads):
     t0 = time.perf_counter()
     o.image(nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "patterns/s", "cores": threads, "kind": "port",
+    cal = None
+    try:
+        with open(os.path.join(ROOT, "oracle", "cpu_calibration.json")) as fh:
+            cal = json.load(fh)
+    except OSError:
+        pass
+    line = {"value": 1.0 / dt, "unit": "patterns/s", "cores": threads, "kind": "port",
             "sample": "one full 1024x1024 pattern of the benchmark workload, FP64 oracle "
-                      "(oracle/csx_oracle.c, bit-exact with the reference CPU path), %d threads, %.2f s" % (threads, dt)}
+                      "(oracle/csx_oracle.c, bit-exact with the reference CPU path), %d threads, %.2f s" % (threads, dt),
+            "host_cpus": os.cpu_count(), "host_cpu_model": _host_cpu()}
+    if cal:
+        ratio = cal["oracle_over_reference"]
+        line["oracle_over_reference_time"] = ratio
+        line["reference_estimate_value"] = ratio / dt
+        line["calibration"] = "oracle time / compiled reference calc time, median over %d cases on %s (%d CPUs)" % (
+            len(cal["rows"]), cal["machine"], cal["cpus"])
+    return line
 
 
 def main():
@@ -457,6 +563,8 @@ def main():
     ap.add_argument("--batch", type=int, default=65536, help="training rows per step")
     ap.add_argument("--rescore", type=int, default=8, help="surrogate candidates per fit scored exactly")
     ap.add_argument("--surrogate-px", type=int, default=256, help="pixels per fit in the surrogate objective")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="default workload only: skip the other configurations (configs 1, 2, 4, 5)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 256 MiB device to device once (a plain copy of "
                          "known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -538,6 +646,21 @@ def main():
     pair_evals, guard_px, near_one_px, two_waves, full_px = (int(v) for v in stats)
 
     img = out.cpu().numpy()
+    # Every pixel against the GPU's FP64 reference mode (itself within 1e-12 of the
+    # reference's own pixels), its rate, and the end-to-end rate of doCalculation:
+    # preparation + upload + render + copy of the image to host memory.
+    full_max = full_rms = rate64 = e2e = None
+    if args.precision == "mixed":
+        out64 = torch.empty_like(out)
+        rate64, ref64 = _rate(eng, out64, "ref64", 5)
+        full_max, full_rms = _err(img, ref64)
+        del out64
+    t0 = time.perf_counter()
+    n_e2e = 10
+    for _ in range(n_e2e):
+        eng.prepare()
+        eng.doCalculation(args.precision)
+    e2e = n_e2e / (time.perf_counter() - t0)
     max_rel = rms_rel = None
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -578,6 +701,13 @@ def main():
                    "precision": args.precision, "global_batch": 1, "pixels": px},
         "max_rel_err_vs_fp64": max_rel,
         "rms_rel_err_vs_fp64": rms_rel,
+        "error_check": "max/rms: 4096 random pixels against the CPU oracle (bit-exact restatement of the "
+                       "reference); full_pattern_*: all 1048576 pixels against the GPU FP64 reference mode",
+        "full_pattern_max_rel_err_vs_fp64": full_max,
+        "full_pattern_rms_rel_err_vs_fp64": full_rms,
+        "ref64_patterns_per_s": rate64,
+        "end_to_end_patterns_per_s": e2e,
+        "end_to_end": "prepareCalculation + table upload + render + 8 MiB image copied to host memory, per pattern",
         "kernel_ms": kernel_ms,
         "region_ms_per_launch": region_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -593,7 +723,41 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+    if not args.no_configs:
+        line["configs"] = run_other_configs(args)
     print(json.dumps(line), flush=True)
+
+
+def run_other_configs(args):
+    """Configs 1, 2, 4 and 5 beside the headline (config 3); a failing one reports its error."""
+    configs = {}
+    for name, fn in (("config1", run_config1), ("config2", run_config2)):
+        t0 = time.perf_counter()
+        try:
+            configs[name] = fn(args)
+        except Exception as exc:  # reported, never silently dropped
+            configs[name] = {"error": repr(exc)}
+        configs[name]["wall_s"] = time.perf_counter() - t0
+        print("[bench] %s done in %.1f s" % (name, configs[name]["wall_s"]), file=sys.stderr, flush=True)
+    probs = None
+    try:
+        t0 = time.perf_counter()
+        probs = fit_problems(args.nfits)
+        t_gen = time.perf_counter() - t0
+        configs["config4"] = run_fits(args, probs)
+        configs["config4"]["target_generation_s"] = t_gen
+    except Exception as exc:
+        configs["config4"] = {"error": repr(exc)}
+    print("[bench] config4 done", file=sys.stderr, flush=True)
+    try:
+        configs["config5"] = run_surrogate(args, probs)
+        configs["config5"]["training_size_note"] = (
+            "%d training patterns (the configuration names 200k): the default benchmark run is kept to minutes; "
+            "--workload surrogate --train-patterns N runs larger sets" % args.train_patterns)
+    except Exception as exc:
+        configs["config5"] = {"error": repr(exc)}
+    print("[bench] config5 done", file=sys.stderr, flush=True)
+    return configs
 
 
 if __name__ == "__main__":

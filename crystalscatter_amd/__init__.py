@@ -107,6 +107,13 @@ class SasCalc:
         return self._L.csx_engine_scalars(self._h).contents
 
     def bounds(self):
+        """Bounds of the image array (minX, maxX, minY, maxY of the reference): the computed
+        block, or the whole image when ExpandImage fills it from a quadrant or half."""
+        s = self.scalars()
+        return s.axmin, s.axmax, s.aymin, s.aymax
+
+    def block(self):
+        """Bounds of the computed block (the fit array's and the batched images' layout)."""
         s = self.scalars()
         return s.xmin, s.xmax, s.ymin, s.ymax
 
@@ -315,7 +322,7 @@ def render_batch(ctx, engines, precision="mixed", out=None):
     if not engines:
         raise ValueError("no engines")
     engines[0].prepare()
-    x0, x1, y0, y1 = engines[0].bounds()
+    x0, x1, y0, y1 = engines[0].block()
     npx = (x1 - x0) * (y1 - y0)
     if out is None:
         out = torch.empty((len(engines), npx), dtype=torch.float64, device="cuda")
@@ -335,7 +342,7 @@ def tpv_images(ctx, engine, ids, n, rng, precision="mixed", out=None):
     import torch
     engine.tpvPerformRandom([], None)
     engine.prepare()
-    x0, x1, y0, y1 = engine.bounds()
+    x0, x1, y0, y1 = engine.block()
     npx = (x1 - x0) * (y1 - y0)
     if out is None:
         out = torch.empty((n, npx), dtype=torch.float64, device="cuda")
@@ -417,12 +424,15 @@ def nm_eval(ctx, fits, keys, X, precision="mixed", threads=0):
 
 
 def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", threads=0, nfree=None,
-             mode="reference"):
+             mode="reference", screen=None, margin=0.5):
     """Batched Nelder-Mead (csx_nm_batch).  fits: list of dicts with 'engine' (SasCalc, may be
     host-only), 'd_data' (device pointer), 'w', 'h', 'x0', 'lo', 'hi', optional 'bstop',
     'border', 'rects'.  keys: the npar parameter names, the first nfree (default all) fitted.
-    mode: "reference" (doSimplexFit2D exactly) or "corrected" (see simplex).  Returns (list
-    of (x, f, iters, evals), stats dict: rounds, evals, prep_ms, stage_ms, run_ms)."""
+    mode: "reference" (doSimplexFit2D exactly) or "corrected" (see simplex).  screen: optional
+    callable screen(fit_ids [npts], X [npts, npar]) -> scores [npts] (lower is better): trial
+    points scored above (1 + margin) times their simplex's worst vertex score are not evaluated
+    (csx_nm_batch_screened).  Returns (list of (x, f, iters, evals), stats dict: rounds, evals,
+    prep_ms, stage_ms, run_ms, pruned, screen_ms)."""
     L = _lib.lib()
     n = len(keys)
     nfree = n if nfree is None else nfree
@@ -430,8 +440,28 @@ def nm_batch(ctx, fits, keys, stp=0.1, maxit=200, ftol=1e-6, precision="mixed", 
     ckeys = (ctypes.c_char_p * n)(*[k.encode() for k in keys])
     res = (_lib.NMResult * len(fits))()
     st = _lib.NMStats()
-    rc = L.csx_nm_batch(ctx._h if isinstance(ctx, Context) else ctx, len(fits), arr, n, nfree, _nm_mode(mode), ckeys,
-                        stp, maxit, ftol, _prec(precision), threads, res, ctypes.byref(st))
+    opts = None
+    if screen is not None:
+        errors = []
+
+        def _cb(user, npts, fit_p, x_p, out_p):
+            try:
+                fit = np.ctypeslib.as_array(fit_p, shape=(npts,)).copy()
+                X = np.ctypeslib.as_array(x_p, shape=(npts, n)).copy()
+                sc = np.asarray(screen(fit, X), dtype=np.float64).reshape(npts)
+                np.ctypeslib.as_array(out_p, shape=(npts,))[:] = sc
+                return 0
+            except Exception as exc:  # reported after the call; the driver stops on a non-zero return
+                errors.append(exc)
+                return 1
+
+        cb = _lib.SCREEN(_cb)
+        opts = _lib.NMScreenOpts(cb, None, float(margin))
+    rc = L.csx_nm_batch_screened(ctx._h if isinstance(ctx, Context) else ctx, len(fits), arr, n, nfree,
+                                 _nm_mode(mode), ckeys, stp, maxit, ftol, _prec(precision), threads,
+                                 ctypes.byref(opts) if opts is not None else None, res, ctypes.byref(st))
+    if screen is not None and errors:
+        raise errors[0]
     if rc != 0:
         raise CsxError("csx_nm_batch failed (%d)" % rc)
     stats = {k: getattr(st, k) for k, _ in _lib.NMStats._fields_}

@@ -36,6 +36,8 @@ class Scalars(ctypes.Structure):
         "width_zuf", "reldis", "dist", "phiwidth", "ceff", "por", "sphno_cubevol")] + [
         ("xmin", ctypes.c_int32), ("xmax", ctypes.c_int32), ("ymin", ctypes.c_int32),
         ("ymax", ctypes.c_int32), ("iimin", ctypes.c_int32), ("cs", ctypes.c_int32),
+        ("use1d", ctypes.c_int32), ("qmin", ctypes.c_double), ("axmin", ctypes.c_int32),
+        ("axmax", ctypes.c_int32), ("aymin", ctypes.c_int32), ("aymax", ctypes.c_int32),
     ]
 
 
@@ -62,7 +64,17 @@ class NMResult(ctypes.Structure):
 class NMStats(ctypes.Structure):
     """Mirror of csx_nm_stats."""
     _fields_ = [("rounds", ctypes.c_int64), ("evals", ctypes.c_int64), ("prep_ms", ctypes.c_double),
-                ("stage_ms", ctypes.c_double), ("run_ms", ctypes.c_double)]
+                ("stage_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("pruned", ctypes.c_int64),
+                ("screen_ms", ctypes.c_double)]
+
+
+SCREEN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
+class NMScreenOpts(ctypes.Structure):
+    """Mirror of csx_nm_screen_opts."""
+    _fields_ = [("fn", SCREEN), ("user", ctypes.c_void_p), ("margin", ctypes.c_double)]
 
 
 class NMFit(ctypes.Structure):
@@ -97,6 +109,9 @@ SIGNATURES = [
     ("csx_nm_run_cb", _I, [_I, _I, _I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
     ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, _I, _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
                           ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),
+    ("csx_nm_batch_screened", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, _I, _I, ctypes.POINTER(_CP), _D, _I, _D, _I,
+                                   _I, ctypes.POINTER(NMScreenOpts), ctypes.POINTER(NMResult),
+                                   ctypes.POINTER(NMStats)]),
     ("csx_nm_eval", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, ctypes.POINTER(_CP), _I, _PD, _I, _I, _PD]),
     ("csx_last_error", _I, [_VP, _CP, _I]),
     ("csx_last_kernel_ms", _D, [_VP]),

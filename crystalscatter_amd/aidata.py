@@ -106,7 +106,7 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16)
     L = _lib.lib()
     engine.tpvPerformRandom([], None)
     engine.prepare()
-    bounds = tuple(engine.bounds())
+    bounds = tuple(engine.block())
     x0, x1, y0, y1 = bounds
     npx = (x1 - x0) * (y1 - y0)
     beam = (engine.get("BeamPosX"), engine.get("BeamPosY"))

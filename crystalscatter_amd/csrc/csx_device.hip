@@ -1256,6 +1256,31 @@ __global__ void k_reduce_sets(const FitSet* __restrict__ sets, int nsets, const 
     cnts[s] = c;
 }
 
+// ExpandImage (Source/sc_calc_generic_cpu.cpp:1246-1269): the whole image (-z..z)^2 from the
+// computed first quadrant (quad 1) or right half (quad 2), as a gather per output pixel of
+// what the reference's in-place copy loops leave there.  Quadrant 1: (-x, y), (-x, -y-1) and
+// (x, -y-1) take the value of (x, y).  Half: (-x, -y) takes the value of (x, y) for
+// y > -z; on the column x = 0 the copy runs in place in increasing y, so there the upper half
+// ends up with the mirrored lower half and the lower half keeps its own values.  Pixels the
+// loops never write (the column x = -z, and the row y = -z left of the axis for the half)
+// keep what the array held before; here they are 0.
+__global__ void k_expand(const double* __restrict__ region, int rw, int rxmin, int rymin, int z, int quad,
+                         double* __restrict__ full) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = 2 * z;
+    if (idx >= (long long)W * W) return;
+    const int x = (int)(idx % W) - z, y = (int)(idx / W) - z;
+    auto at = [&](int sx, int sy) { return region[(size_t)(sx - rxmin) + (size_t)rw * (size_t)(sy - rymin)]; };
+    double v = 0.0;
+    if (x != -z) {
+        if (quad == 1) v = at(x < 0 ? -x : x, y >= 0 ? y : -y - 1);
+        else if (x > 0) v = at(x, y);
+        else if (x == 0) v = y > 0 ? at(0, -y) : at(0, y);
+        else if (y != -z) v = at(-x, -y);
+    }
+    full[idx] = v;
+}
+
 // Sum of n tile partials in tile order, by one lane: the same additions in the same order as
 // the host sum of csx_fit_fqs, so the total has the same bits.
 __global__ void k_sum_tiles(const TileSum* __restrict__ part, int n, TileSum* __restrict__ total) {
@@ -1597,7 +1622,7 @@ void build_cyl_quad(const csx_scalars& s, CylQuad& cq) {
     // |arg| <= (|q| + 1e-10) L/(z+1): bound |q| over the grid (midpoint: sqrt(2) qmax;
     // detector: 4 pi / lambda)
     const double lz = s.length / (zl + 1);
-    const double qb = s.centerbeam ? 4 * kPiD / s.wave : std::sqrt(2.0) * s.qmax;
+    const double qb = s.use1d ? std::fabs(s.qmax - s.qmin) : (s.centerbeam ? 4 * kPiD / s.wave : std::sqrt(2.0) * s.qmax);
     const double amax = (qb * 1.001 + 1e-9) * lz;
     const double h = amax / kCqTab;
     cq.tab.resize(kCqTab + 1);
@@ -1754,7 +1779,7 @@ static int render_kind(const csx_ctx* c, int precision) {
 static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) {
     *out = nullptr;
     *slot = -1;
-    if (!radial_kind(kind_of(c->set.s.combo_key)) || c->set.s.centerbeam) return CSX_OK;
+    if (!radial_kind(kind_of(c->set.s.combo_key)) || c->set.s.centerbeam || c->set.s.use1d) return CSX_OK;
     const long long m = std::max(std::max(std::llabs(g.xmin), std::llabs((long long)g.xmax - 1)),
                                  std::max(std::llabs(g.ymin), std::llabs((long long)g.ymax - 1)));
     if (m > 46000) return CSX_OK;  // triangle index would overflow int: evaluate per pixel
@@ -2101,6 +2126,27 @@ int csx_fit_fqs_noalloc(csx_ctx* c, int w, int h, const csx_mask* mask, int prec
 
 // ---- batched objective ------------------------------------------------------
 }  // extern "C"
+
+// The whole image of an ExpandImage parameter set from its computed block (ordered on the
+// context's stream): d_region holds the block in IDX order, d_full receives the image array.
+int csx_expand_image(csx_ctx* c, const double* d_region, double* d_full) {
+    if (!c || !d_region || !d_full) return CSX_E_ARG;
+    if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_expand_image before csx_upload");
+    const csx_scalars& s = c->set.s;
+    const int z = s.zmax;
+    int quad = 0;
+    if (s.xmin == 0 && s.xmax == z && s.ymin == 0 && s.ymax == z) quad = 1;
+    if (s.xmin == 0 && s.xmax == z && s.ymin == -z && s.ymax == z) quad = 2;
+    if (!quad || s.axmin != -z || s.axmax != z || s.aymin != -z || s.aymax != z)
+        return fail(c, CSX_E_ARG, "csx_expand_image: not an ExpandImage parameter set");
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long n = 4LL * z * z;
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_region,
+                       s.xmax - s.xmin, s.xmin, s.ymin, z, quad, d_full);
+    HIPCHK(c, hipGetLastError());
+    return CSX_OK;
+}
+
 
 // Append bytes at the next 256-byte boundary; returns their offset.
 static size_t arena_put(std::vector<char>& a, const void* p, size_t bytes) {

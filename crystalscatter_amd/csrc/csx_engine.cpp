@@ -71,6 +71,8 @@ struct csx_engine {
     int rects[16];
     double* d_img = nullptr;
     size_t img_n = 0;
+    double* d_block = nullptr;  // computed block of an ExpandImage set
+    size_t block_n = 0;
 };
 
 const csx::ParamSet* csx_engine_paramset(const csx_engine* e) { return e ? &e->p : nullptr; }
@@ -92,6 +94,7 @@ void csx_engine_free(csx_engine* e) {
         (void)hipSetDevice(e->device);
         if (e->d_fit) (void)hipFree(e->d_fit);
         if (e->d_img) (void)hipFree(e->d_img);
+        if (e->d_block) (void)hipFree(e->d_block);
         csx_destroy(e->ctx);
     }
     delete e;
@@ -152,8 +155,25 @@ int csx_engine_calc_device(csx_engine* e, int precision, double* d_out, int32_t*
     if (rc != CSX_OK) return rc;
     const csx_scalars& s = e->P.s;
     csx_grid g{s.xmin, s.xmax, s.ymin, s.ymax};
-    if (b4) { b4[0] = s.xmin; b4[1] = s.xmax; b4[2] = s.ymin; b4[3] = s.ymax; }
-    rc = csx_render(e->ctx, &g, precision, d_out);
+    if (b4) { b4[0] = s.axmin; b4[1] = s.axmax; b4[2] = s.aymin; b4[3] = s.aymax; }
+    const bool expand = s.axmin != s.xmin || s.axmax != s.xmax || s.aymin != s.ymin || s.aymax != s.ymax;
+    if (expand) {
+        // ExpandImage: the computed block, then the whole image from it
+        const size_t nb = (size_t)(s.xmax - s.xmin) * (size_t)(s.ymax - s.ymin);
+        (void)hipSetDevice(e->device);
+        if (nb > e->block_n) {
+            if (csx_synchronize(e->ctx) != CSX_OK) { e->err = "synchronize"; return CSX_E_HIP; }
+            if (e->d_block) (void)hipFree(e->d_block);
+            e->d_block = nullptr;
+            e->block_n = 0;
+            if (hipMalloc(&e->d_block, nb * sizeof(double)) != hipSuccess) { e->err = "hipMalloc block"; return CSX_E_NOMEM; }
+            e->block_n = nb;
+        }
+        rc = csx_render(e->ctx, &g, precision, e->d_block);
+        if (rc == CSX_OK) rc = csx_expand_image(e->ctx, e->d_block, d_out);
+    } else {
+        rc = csx_render(e->ctx, &g, precision, d_out);
+    }
     if (rc != CSX_OK) {
         char b[256];
         csx_last_error(e->ctx, b, sizeof b);
@@ -168,7 +188,7 @@ int csx_engine_calc(csx_engine* e, int precision, double* h_out, int32_t* b4) {
     int rc = e->prepared ? CSX_OK : engine_prepare(e, true);
     if (rc != CSX_OK) return rc;
     const csx_scalars& s = e->P.s;
-    const size_t n = (size_t)(s.xmax - s.xmin) * (size_t)(s.ymax - s.ymin);
+    const size_t n = (size_t)(s.axmax - s.axmin) * (size_t)(s.aymax - s.aymin);
     (void)hipSetDevice(e->device);
     if (n > e->img_n) {
         if (e->d_img) (void)hipFree(e->d_img);

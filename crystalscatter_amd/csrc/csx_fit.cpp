@@ -84,13 +84,39 @@ public:
     const double* pending(int k) const { return pts_[k]; }
 
     // Values of the pending points (in posting order); advances to the next request.
-    void deliver(const double* f) {
-        evals_ += npend_;
+    // nexact: how many of them the exact objective evaluated (the others were screened out).
+    void deliver(const double* f, int nexact = -1) {
+        evals_ += nexact < 0 ? npend_ : nexact;
         double v[CSX_NM_MAXPAR + 2];
-        for (int k = 0; k < nall_; ++k) v[k] = kOutOfBounds;
-        for (int w = 0; w < npend_; ++w) v[slot_[w]] = f[w];
+        for (int k = 0; k < nall_; ++k) {
+            v[k] = kOutOfBounds;
+            spv_[k] = kNoScore;
+        }
+        for (int w = 0; w < npend_; ++w) {
+            v[slot_[w]] = f[w];
+            spv_[slot_[w]] = sp_[w];
+        }
         step(v);
         drain();
+    }
+
+    // Surrogate screening (no reference counterpart).  Scores of the pending points (lower is
+    // better), in posting order, before their exact evaluation.
+    void set_scores(const double* sc) {
+        for (int w = 0; w < npend_; ++w) sp_[w] = sc[w];
+    }
+    // Pending point w is a trial point whose score exceeds the worst vertex score of the
+    // simplex by more than the factor (1 + margin): the simplex would most likely not accept
+    // it, so it takes the value of a rejected point (1e20) without an exact evaluation.
+    // Initial and shrink vertices are always evaluated: the simplex keeps them.
+    bool prunable(int w, double margin) const {
+        if (phase_ != kReflect && phase_ != kExpand && phase_ != kContract) return false;
+        double worst = -1.0;
+        for (int i = 0; i <= nf_; ++i) {
+            if (!(sv_[i] < kNoScore)) return false;  // a vertex without a score: no basis
+            worst = std::max(worst, sv_[i]);
+        }
+        return sp_[w] > worst * (1.0 + margin);
     }
 
     void result(csx_nm_result& r) const {
@@ -121,7 +147,10 @@ private:
         while (resolve_now_ && !done_) {
             resolve_now_ = false;
             double v[CSX_NM_MAXPAR + 2];
-            for (int k = 0; k < nall_; ++k) v[k] = kOutOfBounds;
+            for (int k = 0; k < nall_; ++k) {
+                v[k] = kOutOfBounds;
+                spv_[k] = kNoScore;
+            }
             npend_ = 0;
             step(v);
         }
@@ -147,6 +176,7 @@ private:
     void accept_if_better(double yt) {
         if (yt < y_[ihi_]) {
             y_[ihi_] = yt;
+            sv_[ihi_] = spv_[0];
             for (int j = 0; j < nf_; ++j) {
                 psum_[j] += pt_[j] - p_[ihi_][j];
                 p_[ihi_][j] = pt_[j];
@@ -193,8 +223,15 @@ private:
         switch (phase_) {
         case kInit:
             y_[0] = v[0];
-            for (int i = 1; i <= nev0_; ++i) y_[i] = v[i];
-            for (int i = nev0_ + 1; i <= d_; ++i) y_[i] = y_[nf_ + 1];
+            sv_[0] = spv_[0];
+            for (int i = 1; i <= nev0_; ++i) {
+                y_[i] = v[i];
+                sv_[i] = spv_[i];
+            }
+            for (int i = nev0_ + 1; i <= d_; ++i) {
+                y_[i] = y_[nf_ + 1];
+                sv_[i] = sv_[nf_ + 1];
+            }
             sum_rows();
             rep_ = 0;
             iterate();
@@ -241,7 +278,10 @@ private:
         case kShrink: {
             int k = 0;
             for (int i = 0; i <= nf_; ++i)
-                if (i != ilo_) y_[i] = v[k++];
+                if (i != ilo_) {
+                    sv_[i] = spv_[k];
+                    y_[i] = v[k++];
+                }
             sum_rows();
             next_iteration();
             return;
@@ -265,6 +305,9 @@ private:
     bool done_ = false, resolve_now_ = false;
     int nall_ = 0, npend_ = 0, slot_[CSX_NM_MAXPAR + 2];
     double pts_[CSX_NM_MAXPAR + 2][CSX_NM_MAXPAR];
+    // surrogate scores: of the pending points (posting order and by slot) and of the vertices
+    static constexpr double kNoScore = HUGE_VAL;
+    double sp_[CSX_NM_MAXPAR + 2] = {}, spv_[CSX_NM_MAXPAR + 2] = {}, sv_[CSX_NM_MAXPAR + 1] = {};
 };
 
 // Set one free parameter: a key may name several parameters separated by commas, which
@@ -369,21 +412,27 @@ public:
                             P.latpar3.empty() ? nullptr : P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
         });
         const auto t1 = clk::now();
-        int rc = csx_fit_batch_begin(ctx_);
-        if (rc) return rc;
-        std::vector<const csx_staged*> built(n);
-        for (int r = 0; r < n; ++r) {
-            if (!errs_[r].empty()) return CSX_E_PARAM;
-            built[r] = slots_[r]->p;
+        // A point whose preparation fails is not evaluated: it scores 1e20, the value
+        // doFitCalculation returns on failure (Source/sc_calc_generic_cpu.cpp:1341).
+        std::vector<const csx_staged*> built;
+        ok_.clear();
+        for (int r = 0; r < n; ++r)
+            if (errs_[r].empty()) {
+                built.push_back(slots_[r]->p);
+                ok_.push_back(r);
+            }
+        n_ = n;
+        if (!built.empty()) {
+            int rc = csx_fit_batch_begin(ctx_);
+            if (rc) return rc;
+            rc = csx_stage_commit_many(ctx_, (int)built.size(), built.data(), threads_);
+            if (rc) return rc;
+            rc = csx_fit_batch_launch(ctx_, prec_);
+            if (rc) return rc;
         }
-        rc = csx_stage_commit_many(ctx_, n, built.data(), threads_);
-        if (rc) return rc;
-        rc = csx_fit_batch_launch(ctx_, prec_);
-        if (rc) return rc;
         const auto t2 = clk::now();
         t_prep += ms(t0, t1);
         t_stage += ms(t1, t2);
-        n_ = n;
         return CSX_OK;
     }
 
@@ -391,8 +440,12 @@ public:
     int finish(double* sums) {
         using clk = std::chrono::steady_clock;
         const auto t0 = clk::now();
-        cnts_.resize(n_);
-        const int rc = csx_fit_batch_wait(ctx_, sums, cnts_.data());
+        for (int r = 0; r < n_; ++r) sums[r] = kOutOfBounds;
+        if (ok_.empty()) return CSX_OK;
+        cnts_.resize(ok_.size());
+        got_.resize(ok_.size());
+        const int rc = csx_fit_batch_wait(ctx_, got_.data(), cnts_.data());
+        for (size_t k = 0; k < ok_.size(); ++k) sums[ok_[k]] = got_[k];
         t_run += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
         return rc;
     }
@@ -417,6 +470,8 @@ private:
     std::vector<csx::PreparedSet> preps_;
     std::vector<std::string> errs_;
     std::vector<int64_t> cnts_;
+    std::vector<int> ok_;
+    std::vector<double> got_;
     int n_ = 0;
 };
 
@@ -433,10 +488,12 @@ int check_fits(int nfits, const csx_nm_fit* fits, std::vector<const csx::ParamSe
 
 extern "C" {
 
-int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
-                 const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
-                 csx_nm_result* out, csx_nm_stats* stats) {
+int csx_nm_batch_screened(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
+                          const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
+                          const csx_nm_screen_opts* scr, csx_nm_result* out, csx_nm_stats* stats) {
     if (!ctx || nfits < 1 || !fits || !nm_args_ok(npar, nfree, mode) || !keys || !out) return CSX_E_ARG;
+    if (scr && scr->fn && !(scr->margin >= 0.0)) return CSX_E_ARG;
+    const bool screening = scr && scr->fn;
     std::vector<const csx::ParamSet*> base;
     if (check_fits(nfits, fits, base)) return CSX_E_ARG;
     std::vector<NelderMead> nm(nfits);
@@ -450,7 +507,9 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int 
     struct Group {
         int f0, f1;
         std::unique_ptr<PointEval> ev;
-        std::vector<PointEval::Point> pts;
+        std::vector<PointEval::Point> pts;  // every posted point of the round
+        std::vector<char> keep;             // evaluated exactly (not screened out)
+        std::vector<PointEval::Point> exact;
         std::vector<double> sums;
         bool inflight = false, done = false;
     };
@@ -460,20 +519,28 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int 
         gs[g].f1 = (int)((int64_t)nfits * (g + 1) / ngroups);
         gs[g].ev = std::make_unique<PointEval>(g == 0 ? ctx : ctx2, fits, base, npar, keys, precision, threads);
     }
-    int64_t rounds = 0, evals = 0;
+    int64_t rounds = 0, evals = 0, pruned = 0;
+    double screen_ms = 0.0;
+    std::vector<int32_t> sfit;
+    std::vector<double> sx, sscore;
     int rc = CSX_OK;
     for (int live = ngroups; live > 0 && rc == CSX_OK;) {
         for (auto& G : gs) {
             if (G.done) continue;
             if (G.inflight) {  // deliver the group's previous round
-                G.sums.resize(G.pts.size());
-                if ((rc = G.ev->finish(G.sums.data()))) break;
+                std::vector<double> got(G.exact.size());
+                if ((rc = G.ev->finish(got.data()))) break;
                 G.inflight = false;
+                G.sums.assign(G.pts.size(), kOutOfBounds);  // screened-out points: rejected
+                for (size_t p = 0, e = 0; p < G.pts.size(); ++p)
+                    if (G.keep[p]) G.sums[p] = got[e++];
                 size_t r = 0;
                 for (int i = G.f0; i < G.f1; ++i) {
                     const int np = nm[i].npending();
                     if (np == 0) continue;
-                    nm[i].deliver(G.sums.data() + r);
+                    int nexact = 0;
+                    for (int k = 0; k < np; ++k) nexact += G.keep[r + k];
+                    nm[i].deliver(G.sums.data() + r, nexact);
                     r += np;
                 }
             }
@@ -485,17 +552,51 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int 
                 --live;
                 continue;
             }
+            G.keep.assign(G.pts.size(), 1);
+            if (screening) {
+                // surrogate scores of every posted point, then the trial points the simplex
+                // would most likely reject are screened out of the exact evaluation
+                const auto t0 = std::chrono::steady_clock::now();
+                const size_t n = G.pts.size();
+                sfit.resize(n);
+                sx.resize(n * npar);
+                sscore.assign(n, 0.0);
+                for (size_t p = 0; p < n; ++p) {
+                    sfit[p] = G.pts[p].fit;
+                    std::memcpy(&sx[p * npar], G.pts[p].x, npar * sizeof(double));
+                }
+                if (scr->fn(scr->user, (int)n, sfit.data(), sx.data(), sscore.data()) != 0) {
+                    rc = CSX_E_STATE;
+                    break;
+                }
+                size_t r = 0;
+                for (int i = G.f0; i < G.f1; ++i) {
+                    const int np = nm[i].npending();
+                    if (np == 0) continue;
+                    nm[i].set_scores(sscore.data() + r);
+                    for (int k = 0; k < np; ++k)
+                        if (nm[i].prunable(k, scr->margin)) {
+                            G.keep[r + k] = 0;
+                            ++pruned;
+                        }
+                    r += np;
+                }
+                screen_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            }
+            G.exact.clear();
+            for (size_t p = 0; p < G.pts.size(); ++p)
+                if (G.keep[p]) G.exact.push_back(G.pts[p]);
             ++rounds;
-            evals += (int64_t)G.pts.size();
-            if ((rc = G.ev->launch(G.pts))) break;
+            evals += (int64_t)G.exact.size();
+            if ((rc = G.ev->launch(G.exact))) break;
             G.inflight = true;
         }
     }
     double tp = 0, ts = 0, tr = 0;
     for (auto& G : gs) {
         if (G.inflight) {  // an error left a launch in flight: drain it
-            G.sums.resize(G.pts.size());
-            (void)G.ev->finish(G.sums.data());
+            std::vector<double> got(G.exact.size());
+            (void)G.ev->finish(got.data());
         }
         tp += G.ev->t_prep;
         ts += G.ev->t_stage;
@@ -511,8 +612,17 @@ int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int 
         stats->prep_ms = tp;
         stats->stage_ms = ts;
         stats->run_ms = tr;
+        stats->pruned = pruned;
+        stats->screen_ms = screen_ms;
     }
     return CSX_OK;
+}
+
+int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
+                 const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
+                 csx_nm_result* out, csx_nm_stats* stats) {
+    return csx_nm_batch_screened(ctx, nfits, fits, npar, nfree, mode, keys, stp, maxit, ftol, precision, threads,
+                                 nullptr, out, stats);
 }
 
 int csx_nm_eval(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, const char* const* keys, int nv,

@@ -98,7 +98,11 @@ __device__ __forceinline__ double sqd(double x) { return x * x; }
 
 // q of pixel (ihex, i), convert_idx2q (.cu:249-287).
 __device__ __forceinline__ void pixel_q(const csx_scalars& s, int ihex, int i, double& qx, double& qy, double& qz) {
-    if (s.centerbeam) {
+    if (s.use1d) {
+        qx = (s.qmax - s.qmin) / (double)(s.zmax) * (double)ihex;
+        qy = 1e-20;
+        qz = 1e-20;
+    } else if (s.centerbeam) {
         const double mdet = (ihex) * s.pixnoy / (2.0 * s.zmax);
         const double ndet = (i) * s.pixnox / (2.0 * s.zmax);
         const double xdet = s.pixx_m * (ndet - s.beamx);

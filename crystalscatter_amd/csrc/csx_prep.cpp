@@ -63,7 +63,8 @@ const KeyDef kKeys[] = {
     {"CheckBoxWAXS", &ParamSet::waxs, nullptr}, {"P1", nullptr, &ParamSet::p1},
     {"iso", nullptr, &ParamSet::iso}, {"I0", nullptr, &ParamSet::i0}, {"Base", nullptr, &ParamSet::base},
     {"width_zuf", nullptr, &ParamSet::width_zuf}, {"dw_zuf", nullptr, &ParamSet::dw_zuf},
-    {"fix_orcase", &ParamSet::fix_orcase, nullptr},
+    {"fix_orcase", &ParamSet::fix_orcase, nullptr}, {"use1d", &ParamSet::use1d, nullptr},
+    {"EditQsteps", &ParamSet::qsteps, nullptr},
     {"VAx1", nullptr, &ParamSet::vax1}, {"VAx2", nullptr, &ParamSet::vax2}, {"VAx3", nullptr, &ParamSet::vax3},
     {"Ay1", nullptr, &ParamSet::ay1}, {"Ay2", nullptr, &ParamSet::ay2}, {"Ay3", nullptr, &ParamSet::ay3},
     {"Az1", nullptr, &ParamSet::az1}, {"Az2", nullptr, &ParamSet::az2}, {"Az3", nullptr, &ParamSet::az3},
@@ -729,7 +730,9 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     csx_scalars& s = P.s;
     std::memset(&s, 0, sizeof(s));
     s.combo_key = p.combo_key();
-    s.zmax = p.gridpoints;
+    s.zmax = p.use1d ? p.qsteps : p.gridpoints;  // setGridPoints(EditQsteps, true) in 1D
+    s.use1d = p.use1d;
+    s.qmin = p.qmin;
     s.centerbeam = p.centerbeam;
     s.fix_orcase = p.fix_orcase;
     s.ordis = p.ordis;
@@ -744,12 +747,26 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     s.phiwidth = 4. / p.azi;
     s.ceff = p.ceff;
 
-    // quadrant ranges (:373-394)
+    // quadrant ranges (:373-394); 1D: one row of zmax pixels, never expanded
     const int quad = p.radq1 ? 1 : (p.radq2 ? 2 : 4);
-    P.xmin = (quad == 4) ? -p.gridpoints : 0;
-    P.xmax = p.gridpoints;
-    P.ymin = (quad == 1) ? 0 : -p.gridpoints;
-    P.ymax = p.gridpoints;
+    if (p.use1d) {
+        P.xmin = 0;
+        P.xmax = s.zmax;
+        P.ymin = 0;
+        P.ymax = 1;
+    } else {
+        P.xmin = (quad == 4) ? -p.gridpoints : 0;
+        P.xmax = p.gridpoints;
+        P.ymin = (quad == 1) ? 0 : -p.gridpoints;
+        P.ymax = p.gridpoints;
+    }
+    // The image array: the whole image when ExpandImage fills it from the computed quadrant
+    // or half (checkArrays(-zmax, zmax, -zmax, zmax), :979-986), else the computed block.
+    const bool expand = p.expand && quad != 4 && !p.use1d;
+    P.axmin = expand ? -p.gridpoints : P.xmin;
+    P.axmax = expand ? p.gridpoints : P.xmax;
+    P.aymin = expand ? -p.gridpoints : P.ymin;
+    P.aymax = expand ? p.gridpoints : P.ymax;
 
     double polPhi = p.phi;
     const double polTheta = p.theta;
@@ -935,6 +952,7 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
     s.cosphic = std::cos(-polPhi * kPi / 180.);
     s.xmin = P.xmin; s.xmax = P.xmax; s.ymin = P.ymin; s.ymax = P.ymax;
     s.iimin = P.ymin;
+    s.axmin = P.axmin; s.axmax = P.axmax; s.aymin = P.aymin; s.aymax = P.aymax;
     return "";
 }
 

@@ -22,6 +22,7 @@ struct ParamSet {
     int ltype = 12, particle = 0, ordis = 7, interior = 0, peakidx = 1, hklmax = 5, gridpoints = 64;
     int radq1 = 0, radq2 = 0, radq4 = 1, expand = 0, centerbeam = 0, twinned = 0, dsch = 0, para = 0;
     int waxs = 0, fix_orcase = 0;
+    int use1d = 0, qsteps = 200;  // 1D mode (the console's prepareData argument) and EditQsteps
     double uca = 30, ucb = 30, ucc = 30, ucalpha = 90, ucbeta = 90, ucgamma = 90, ceff = 1;
     double ceffcyl = 0, reff = 0, acpl = 0, bcpl = 0, ifluc = 0, rfluc = 0;
     double radius = 1.0, radiusi = 0.5, sigma = 0.07, dbeta = 0.4, length = 1, sigmal = 0.06;
@@ -50,7 +51,8 @@ struct PreparedSet {
     std::vector<int32_t> latpar1, latpar2;      // (peakmax+1) x 6
     std::vector<float> latpar3;                 // (peakmax2+1) x 17
     double order = 0, ucvol = 0, sphno = 1;
-    int xmin = 0, xmax = 0, ymin = 0, ymax = 0; // image bounds (ihex, i)
+    int xmin = 0, xmax = 0, ymin = 0, ymax = 0; // computed block (ihex, i)
+    int axmin = 0, axmax = 0, aymin = 0, aymax = 0;  // image array (whole image when expanded)
     int cdim = 3;
 };
 
@@ -88,6 +90,9 @@ int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int t
 // batched fit driver prepares another group of fits in between).
 int csx_fit_batch_launch(csx_ctx* c, int precision);
 int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts);
+// ExpandImage: the image array of the uploaded parameter set from its computed block
+// (csx_device.hip), ordered on the context's stream.
+int csx_expand_image(csx_ctx* c, const double* d_region, double* d_full);
 // A second context on the same device.
 int csx_create_like(const csx_ctx* c, csx_ctx** out);
 // The context's device and stream, and its error message (for the post-processing kernels

@@ -244,7 +244,7 @@ def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, 
     e = SasCalc(device=-1, params=params_dict(xs[0], grid_points))
     e.set("ignore_cbs", 1)
     e.prepare()
-    x0, x1, y0, y1 = e.bounds()
+    x0, x1, y0, y1 = e.block()
     npx = (x1 - x0) * (y1 - y0)
     chunk = max(1, min(npatterns, (1 << 27) // (8 * npx)))
     buf = torch.empty((chunk, npx), dtype=torch.float64, device="cuda")
@@ -364,3 +364,31 @@ def propose_starts(net, fits_px, mu, sigma, x_init, rng, n_global=128, n_keep=4,
     lv = surrogate_objectives(net, fits_px, loc, mu, sigma)
     order = np.argsort(lv, axis=1)[:, :top]
     return np.take_along_axis(loc, order[:, :, None], axis=1), np.take_along_axis(lv, order, axis=1)
+
+
+class SurrogateScreen:
+    """Screen of the batched simplex's posted points (nm_batch(screen=...)): the network's
+    objective at each point, the mean over its fit's pixel subsample (FitPixels) of the squared
+    log10 residual.  The fits' first six parameter entries are FIT_KEYS; every point of a
+    round goes through the network in one pass."""
+
+    def __init__(self, net, fits_px, mu, sigma):
+        import torch
+        self.net, self.mu, self.sigma = net, mu, sigma
+        self.npx = min(fp.n for fp in fits_px)
+        self.Q = torch.stack([fp.q[:self.npx] for fp in fits_px])      # nf x npx x 2
+        self.T = torch.stack([fp.logd[:self.npx] for fp in fits_px])   # nf x npx
+        self.calls = 0
+        self.points = 0
+
+    def __call__(self, fit_ids, X):
+        import torch
+        P = len(fit_ids)
+        f = torch.from_numpy(np.asarray(fit_ids, dtype=np.int64)).cuda()
+        par = torch.from_numpy(normalize(np.asarray(X)[:, :6]).astype(np.float32)).cuda()
+        sets = torch.arange(P, dtype=torch.int32, device="cuda").repeat_interleave(self.npx)
+        q = self.Q[f].reshape(-1, 2).contiguous()
+        pred = predict_log10(self.net, par, sets, q, self.mu, self.sigma).reshape(P, self.npx)
+        self.calls += 1
+        self.points += P
+        return ((pred - self.T[f]) ** 2).mean(dim=1).double().cpu().numpy()

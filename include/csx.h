@@ -99,6 +99,11 @@ typedef struct csx_scalars {
     int32_t xmin, xmax, ymin, ymax;     /* image bounds _xmin.._ymax (ihex, i) */
     int32_t iimin;                      /* offset of fit coordinates (checkFitRanges) */
     int32_t cs;                         /* particle interior (0 homogeneous) */
+    int32_t use1d;                      /* 1D q mode: qx = (qmax-qmin)/zmax*ihex, one row */
+    double qmin;                        /* EditQmin (1D mode) */
+    int32_t axmin, axmax, aymin, aymax; /* bounds of the image array (data()): the computed
+                                           block, or the whole image when ExpandImage fills it
+                                           from a computed quadrant or half */
 } csx_scalars;
 
 /* Block of pixels to evaluate: ihex in [xmin,xmax), i in [ymin,ymax); output in the
@@ -232,10 +237,28 @@ typedef struct csx_nm_stats {
     double prep_ms;           /* host preparation of the parameter sets and their peak tables (threads) */
     double stage_ms;          /* appending the built sets to the batch (serial) */
     double run_ms;            /* batched launch, reductions and the copy of the sums */
+    int64_t pruned;           /* trial points screened out of the exact evaluation (csx_nm_batch_screened) */
+    double screen_ms;         /* time in the screen callback */
 } csx_nm_stats;
 int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
                  const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
                  csx_nm_result* out, csx_nm_stats* stats);
+/* csx_nm_batch with a screen of the simplex's trial points (no reference counterpart; the
+ * learned part of the fits).  Each round, fn scores every point the fits posted (npts points
+ * of fits fit[k], parameter vectors x[k*npar ...]; score[k]: lower is better, e.g. a surrogate
+ * network's objective) and returns 0.  A trial point (reflection, expansion, contraction)
+ * scored above (1 + margin) times the worst score of its simplex's vertices is not evaluated:
+ * it takes the value of a rejected point (1e20), so the simplex never accepts it.  Initial and
+ * shrink vertices, and every point a simplex accepts, have exact values.  fn NULL: csx_nm_batch. */
+typedef int (*csx_nm_screen)(void* user, int npts, const int32_t* fit, const double* x, double* score);
+typedef struct csx_nm_screen_opts {
+    csx_nm_screen fn;
+    void* user;
+    double margin;
+} csx_nm_screen_opts;
+int csx_nm_batch_screened(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
+                          const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
+                          const csx_nm_screen_opts* screen, csx_nm_result* out, csx_nm_stats* stats);
 /* The batched objective at explicit points: out[i*nv + v] = fehlerquadratsumme of fit i at
  * x[(i*nv + v)*npar ...] (1e20 outside the bounds of any entry), all points of all fits in
  * one batched launch.  No reference counterpart: used to score proposed starting points. */

@@ -20,6 +20,7 @@
 //     nofit<i>=x0,y0,x1,y1 no-fit rectangles (fit mode)
 //     ignorecbs=0|1        bIgnoreCbs for doCalculation (default 1)
 //     repeat=<n>           repeat the calculation n times (timing)
+//     use1d=1              1D mode: EditQsteps pixels, q from EditQmin..EditQmax
 //     rphi=<s> bsx= bsy=   r,phi transform of the image (s x s, centre bsx,bsy) and its log
 //                          scaling; rphiscale/rphiclip/rphiclip40 = flags
 //     fft=<type>           calculateIFFT of the log-scaled r,phi image (0 re, 1 im, 2 abs,
@@ -111,8 +112,12 @@ int main(int argc, char** argv) {
                                 num("theta", 90), num("phi", 0), num("rotTheta", 90), num("rotPhi", 90));
         int q1 = (int)num("RadioButtonQ1", 0), q2 = (int)num("RadioButtonQ2", 0);
         int q4 = (int)num("RadioButtonQ4", (q1 || q2) ? 0 : 1);
-        c->setCalculationParams((int)num("GridPoints", 64), (int)num("HKLmax", 5), num("EditQmax", 1),
-                                q1 != 0, q2 != 0, q4 != 0, num("ExpandImage", 0) != 0, false);
+        // 1D mode (use1d=1): the grid is EditQsteps and EditQmin is set, as the console's
+        // prepareData does for use1d (Source/sc_calc_generic.cpp:202-206)
+        const bool d1 = num("use1d", 0) != 0;
+        c->setCalculationParams(d1 ? (int)num("EditQsteps", 200) : (int)num("GridPoints", 64), (int)num("HKLmax", 5),
+                                num("EditQmax", 1), q1 != 0, q2 != 0, q4 != 0, num("ExpandImage", 0) != 0, d1);
+        if (d1) c->setQMin(num("EditQmin", 0));
         c->setExperimentParams(num("EditPixelNoX", 2048), num("EditPixelNoY", 2048),
                                num("EditPixelX", 1), num("EditPixelY", 1), num("EditDet", 1),
                                num("EditWavelength", 0.09499), num("BeamPosX", 0), num("BeamPosY", 0),

@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv post lattices fits       # only these
+    python tests/golden/make_golden.py cube tpv post lattices fits qmodes  # only these
 """
 import json
 import os
@@ -154,6 +154,31 @@ POST_CASES = [("fcc_rphi", dict(CFG3, GridPoints=32), {"rphi": 128, "bsx": 0, "b
               ("sphere_lin", dict(CFG1, GridPoints=32), {"scaleint": 1})]
 
 
+# q modes and image layouts: the 1D mode (use1d: EditQsteps pixels, qx = (qmax - qmin)/zmax * ihex)
+# and ExpandImage from the first quadrant and from the right half, on an anisotropic pattern
+# (FCC spheres, so that the mirror fill is not hidden by a symmetric image).
+QMODE_FCC = dict(CFG3, GridPoints=16, EditQmax=1.2)
+
+
+def write_qmodes(manifest):
+    arrays, meta = {}, {}
+    cases = {
+        "q1_expand": dict(QMODE_FCC, RadioButtonQ1=1, RadioButtonQ4=0, ExpandImage=1),
+        "q1_block": dict(QMODE_FCC, RadioButtonQ1=1, RadioButtonQ4=0, ExpandImage=0),
+        "q2_expand": dict(QMODE_FCC, RadioButtonQ2=1, RadioButtonQ4=0, ExpandImage=1),
+        "q2_block": dict(QMODE_FCC, RadioButtonQ2=1, RadioButtonQ4=0, ExpandImage=0),
+        "d1_sphere": dict(CFG1, use1d=1, EditQsteps=100, EditQmin=0.1, EditQmax=2),
+        "d1_fcc": dict(QMODE_FCC, use1d=1, EditQsteps=160, EditQmin=0.0, EditQmax=1.2),
+    }
+    for name, p in cases.items():
+        r = run_ref(p)
+        arrays[name] = r["image"]
+        arrays[name + "_bounds"] = r["bounds"]
+        meta[name] = p
+    np.savez_compressed(os.path.join(HERE, "qmodes.npz"), **arrays)
+    manifest["fixtures"]["qmodes.npz"] = {"cases": meta}
+
+
 # Reference 2D simplex fits (oracle/_ref/ref_fit: doSimplexFit2D over the reference engine).
 # Each case: the parameters at the start, the hash entries (name, used, min, max), the simplex
 # arguments and the true parameters that made the target image.  Bounds are the key table's
@@ -248,6 +273,8 @@ def main():
             write_lattices(manifest)
         if "fits" in only:
             write_fits(manifest)
+        if "qmodes" in only:
+            write_qmodes(manifest)
         with open(os.path.join(HERE, "manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))
         return
@@ -326,6 +353,7 @@ def main():
     write_post(manifest)
     write_lattices(manifest)
     write_fits(manifest)
+    write_qmodes(manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o))

@@ -40,11 +40,15 @@ def test_batched_nm_equals_one_fit_at_a_time(mode):
     for e, p, d, w, x0, lo, hi in probs:
         base = SasCalc(device=-1, params=p)
         base.set("ignore_cbs", 1)
+        if mode == "reference":  # the zeroed first entry (the radius) must stay inside the box
+            lo = lo.copy()
+            lo[0] = 0.0
         fits.append(dict(engine=base, d_data=d.data_ptr(), w=w, h=w, x0=x0, lo=lo, hi=hi, bstop=2))
     ctx = Context(0)
     res, stats = nm_batch(ctx, fits, KEYS, stp=0.1, maxit=25, ftol=1e-8, precision="mixed", mode=mode)
     assert stats["rounds"] > 25
-    for (e, p, d, w, x0, lo, hi), (xb, fb, itb, evb) in zip(probs, res):
+    for (e, p, d, w, x0, lo, hi), fd, (xb, fb, itb, evb) in zip(probs, fits, res):
+        lo = fd["lo"]
         e.setFitData(w, w, d.cpu().numpy())
 
         def f(x):
