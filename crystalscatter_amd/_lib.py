@@ -51,6 +51,7 @@ class Mask(ctypes.Structure):
 
 
 NM_MAXPAR = 30
+TUNE_VARIANTS, TUNE_SIZES = 4, 4
 NM_REFERENCE, NM_CORRECTED = 0, 1
 OBJECTIVE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
 
@@ -106,6 +107,12 @@ SIGNATURES = [
                                  ctypes.POINTER(Mask), _VP]),
     ("csx_fit_batch_run", _I, [_VP, _I, _PD, _PI64]),
     ("csx_render_batch_run", _I, [_VP, _I]),
+    ("csx_tune_set", _I, [ctypes.c_uint32, _I, _I]),
+    ("csx_tune_get", _I, [ctypes.c_uint32, _I]),
+    ("csx_tune_load", _I, [_CP]),
+    ("csx_tune_force", _I, [_VP, _I]),
+    ("csx_set_batch_graph", _I, [_VP, _I]),
+    ("csx_batch_graph_launches", ctypes.c_longlong, [_VP]),
     ("csx_nm_run_cb", _I, [_I, _I, _I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
     ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, _I, _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
                           ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),

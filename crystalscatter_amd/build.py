@@ -52,7 +52,7 @@ def build(verbose=True):
         _run(["hipcc"] + HIP_FLAGS + ["-c", src, "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + ["-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

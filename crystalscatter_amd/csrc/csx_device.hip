@@ -12,6 +12,7 @@
 // swapped angles, which decides those pixels).  The pair table is read with
 // wave-uniform addresses, i.e. through the scalar cache.
 #include <hip/hip_runtime.h>
+#include <dlfcn.h>
 
 #include <cmath>
 #include <cstdio>
@@ -1131,7 +1132,7 @@ struct TileSum {
 
 // One fit tile: per-pixel FQSVERGL terms, NaN counted, block-reduced in FP64.
 template <int KIND, int PREC>
-__device__ void fit_tile(const FitSet& f, int tile, double* __restrict__ fqs_out, TileSum* __restrict__ out) {
+__device__ __forceinline__ void fit_tile(const FitSet& f, int tile, double* __restrict__ fqs_out, TileSum* __restrict__ out) {
     __shared__ double s_sum[kBlock / 64];
     __shared__ int s_cnt[kBlock / 64], s_nan[kBlock / 64];
     const int tx = tile % f.tiles_x, ty = tile / f.tiles_x;
@@ -1208,6 +1209,26 @@ k_fit_batch_lat(const FitSet* __restrict__ sets, const int2* __restrict__ tiles,
     fit_batch_tiles<KIND, PREC>(sets, tiles, ntiles, out);
 }
 
+// Variants of the batched objective for the lattice kinds in mixed precision: the same code
+// with different register budgets, so the sums do not depend on the variant, only the time
+// does; a tuning table picks one per combination and detector size.  Variant 0 is
+// k_fit_batch_lat (budget cut for 4 waves per SIMD: cold paths spill), 1 is uncapped (no
+// spills, 3 waves), 2 and 3 cut for 5 and 6 waves (more spills, more waves).  Looping over
+// several tiles per workgroup was measured too: the compiler keeps a set's fields live across
+// the loop (spills of ~400 bytes per lane, or 240 VGPRs), so every variant takes one tile.
+constexpr int kTuneVariants = 4;
+constexpr int kTuneSizes = 4;  // mean tiles per set: <= 64 (128^2), <= 256, <= 576, more
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock)
+k_fit_batch_free(const FitSet* __restrict__ sets, const int2* __restrict__ tiles, int ntiles, TileSum* __restrict__ out) {
+    fit_batch_tiles<KIND, PREC>(sets, tiles, ntiles, out);
+}
+template <int KIND, int PREC, int WPE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_fit_batch_cap(const FitSet* __restrict__ sets, const int2* __restrict__ tiles, int ntiles, TileSum* __restrict__ out) {
+    fit_batch_tiles<KIND, PREC>(sets, tiles, ntiles, out);
+}
+
 // Batched images: one launch over the flat tile list of many parameter sets, each set's
 // image (its whole grid, IDX order) written to the buffer staged as its data pointer.  The
 // tile goes through LDS so that every wave stores whole 128-byte rows.
@@ -1243,17 +1264,51 @@ k_image_batch_lat(const FitSet* __restrict__ sets, const int2* __restrict__ tile
     image_batch_tiles<KIND, PREC>(sets, tiles, ntiles);
 }
 
-// Per-set sum of the tile partials (deterministic order).
-__global__ void k_reduce_sets(const FitSet* __restrict__ sets, int nsets, const TileSum* __restrict__ part,
-                              const int* __restrict__ ntiles_of, double* __restrict__ sums, long long* __restrict__ cnts) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// Sums of a set's tile partials: one workgroup per set.  Lane t adds tiles t, t + kBlock, ...
+// in order, then the lanes' partials meet in a fixed tree in LDS.  The order depends only on
+// the tile count, and every path that sums tiles (csx_fit_fqs, csx_fit_fqs_noalloc, the
+// batch) uses this one, so their sums agree bit for bit.
+struct SetSum {
+    double sum;
+    long long cnt, nan;
+};
+__device__ __forceinline__ void block_sum_tiles(const TileSum* __restrict__ part, int n, SetSum* __restrict__ out) {
+    __shared__ double s_s[kBlock];
+    __shared__ long long s_c[kBlock], s_n[kBlock];
+    const int t = threadIdx.x;
+    double v = 0.0;
+    long long c = 0, nn = 0;
+    for (int k = t; k < n; k += kBlock) {
+        v += part[k].sum;
+        c += part[k].cnt;
+        nn += part[k].nan;
+    }
+    s_s[t] = v;
+    s_c[t] = c;
+    s_n[t] = nn;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            s_s[t] += s_s[t + h];
+            s_c[t] += s_c[t + h];
+            s_n[t] += s_n[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out->sum = s_s[0];
+        out->cnt = s_c[0];
+        out->nan = s_n[0];
+    }
+}
+
+// Batch: set blockIdx.x, tiles [first_tile, first_tile + ntiles_of).
+__global__ void __launch_bounds__(kBlock) k_reduce_sets(const FitSet* __restrict__ sets, int nsets,
+                                                        const TileSum* __restrict__ part,
+                                                        const int* __restrict__ ntiles_of, SetSum* __restrict__ out) {
+    const int s = blockIdx.x;
     if (s >= nsets) return;
-    const int a = sets[s].first_tile, n = ntiles_of[s];
-    double t = 0.0;
-    long long c = 0;
-    for (int k = 0; k < n; ++k) { t += part[a + k].sum; c += part[a + k].cnt; }
-    sums[s] = t;
-    cnts[s] = c;
+    block_sum_tiles(part + sets[s].first_tile, ntiles_of[s], out + s);
 }
 
 // ExpandImage (Source/sc_calc_generic_cpu.cpp:1246-1269): the whole image (-z..z)^2 from the
@@ -1281,30 +1336,21 @@ __global__ void k_expand(const double* __restrict__ region, int rw, int rxmin, i
     full[idx] = v;
 }
 
-// Sum of n tile partials in tile order, by one lane: the same additions in the same order as
-// the host sum of csx_fit_fqs, so the total has the same bits.
-__global__ void k_sum_tiles(const TileSum* __restrict__ part, int n, TileSum* __restrict__ total) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double t = 0.0;
-    int32_t c = 0, nn = 0;
-    for (int k = 0; k < n; ++k) {
-        t += part[k].sum;
-        c += part[k].cnt;
-        nn += part[k].nan;
-    }
-    total->sum = t;
-    total->cnt = c;
-    total->nan = nn;
+// One set's sum: tiles [0, n) (csx_fit_fqs, csx_fit_fqs_noalloc).
+__global__ void __launch_bounds__(kBlock) k_sum_tiles(const TileSum* __restrict__ part, int n, SetSum* __restrict__ out) {
+    block_sum_tiles(part, n, out);
 }
 
 }  // namespace csx
 
 // Scratch of csx_fit_fqs_noalloc: the tile partials of one fit array (up to kScratchTiles
-// 16x16 tiles, 2048 x 2048 pixels) and their total, 256 KiB per GPU for the life of the
-// process.  Host code never uses this name as an address: the device address of each GPU's
-// copy comes from hipGetSymbolAddress (scratch_address below).
+// 16x16 tiles, 2048 x 2048 pixels) and their total (a SetSum in the two slots after them),
+// 256 KiB per GPU for the life of the process.  Host code never uses this name as an
+// address: the device address of each GPU's copy comes from hipGetSymbolAddress
+// (scratch_address below).
 constexpr int kScratchTiles = 16384;
-static __device__ __align__(256) csx::TileSum scratch[kScratchTiles + 1];
+static_assert(sizeof(csx::SetSum) <= 2 * sizeof(csx::TileSum), "the total fits in two tile slots");
+static __device__ __align__(256) csx::TileSum scratch[kScratchTiles + 2];
 
 // ===========================================================================
 // C ABI
@@ -1336,6 +1382,15 @@ struct PinnedArena {
         p = nullptr;
         cap = size = 0;
     }
+};
+
+// The graph of a batched-objective round and its nodes (rebuilt when the kernel changes).
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t h2d = nullptr, fit = nullptr, red = nullptr, d2h = nullptr;
+    const void* fit_fn = nullptr;
+    long long launches = 0;
 };
 
 struct DevBuf {
@@ -1667,6 +1722,9 @@ struct csx_ctx {
     std::vector<Pending> bpend;
     DevBuf barena_d, bpart_d, bsums_d;
     DevBuf post[2];  // scratch of the post-processing calls (csx_post.hip)
+    int tune_force = -1;  // batched-objective variant for every batch of this context (-1: table)
+    bool use_graph = true;  // batched rounds as one HIP graph launch
+    BatchGraph bgraph;
 };
 
 static int fail(csx_ctx* c, int code, const std::string& msg) {
@@ -1679,6 +1737,62 @@ static int fail(csx_ctx* c, int code, const std::string& msg) {
         hipError_t e_ = (call);                                                              \
         if (e_ != hipSuccess) return fail(ctx, CSX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Tuning table of the batched objective: the kernel variant per kind and detector size class.
+// Defaults to variant 0; csx_create loads tuning.txt from the library's directory once (the
+// autotuner's measured choices), csx_tune_set / csx_tune_load change it.
+namespace {
+constexpr int kMaxKinds = 16;
+std::mutex g_tune_mu;
+int g_tune[kMaxKinds][kTuneSizes] = {};
+std::once_flag g_tune_once;
+
+int tune_size_class(int ntiles, int nsets) {
+    const double m = nsets > 0 ? (double)ntiles / nsets : 0.0;
+    return m <= 64 ? 0 : (m <= 256 ? 1 : (m <= 576 ? 2 : 3));
+}
+
+int tune_variant(int kind, int size_class) {
+    if (kind < 0 || kind >= kMaxKinds || size_class < 0 || size_class >= kTuneSizes) return 0;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    return g_tune[kind][size_class];
+}
+
+// Lines "<combination key> <size class> <variant>" (key in hex or decimal); '#' starts a
+// comment.  Returns the number of entries taken, or -1 if the file cannot be read.
+int tune_load_file(const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return -1;
+    char line[512];
+    int n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (char* h = std::strchr(line, '#')) *h = 0;
+        unsigned long key = 0;
+        int sc = 0, v = 0;
+        char* end = nullptr;
+        key = std::strtoul(line, &end, 0);
+        if (end == line) continue;
+        if (std::sscanf(end, "%d %d", &sc, &v) != 2) continue;
+        const int k = kind_of((uint32_t)key);
+        if (k < 0 || k >= kMaxKinds || sc < 0 || sc >= kTuneSizes || v < 0 || v >= kTuneVariants) continue;
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune[k][sc] = v;
+        ++n;
+    }
+    std::fclose(f);
+    return n;
+}
+
+void tune_load_default() {
+    Dl_info info;
+    if (dladdr((const void*)&tune_load_default, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        const size_t slash = p.find_last_of('/');
+        p = (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/tuning.txt";
+        (void)tune_load_file(p.c_str());
+    }
+}
+}  // namespace
 
 // Device address of `scratch` on each GPU, looked up once per device with that device
 // current (csx_create).
@@ -1743,12 +1857,25 @@ static void launch_fit_one(const FitSet& f, int ntiles, double* fqs, TileSum* pa
     hipLaunchKernelGGL((k_fit_one<K, P>), dim3(ntiles), dim3(kBlock), 0, st, f, fqs, part);
 }
 
+// Kernel of the batched objective for kind K, precision P, variant v (one workgroup per tile).
 template <int K, int P>
-static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, TileSum* part, int nblocks, hipStream_t st) {
-    if constexpr (K == K_FCC || K == K_BCT)
-        hipLaunchKernelGGL((k_fit_batch_lat<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
-    else
-        hipLaunchKernelGGL((k_fit_batch<K, P>), dim3(nblocks), dim3(kBlock), 0, st, sets, tiles, ntiles, part);
+static const void* fit_batch_kernel(int variant) {
+    if constexpr ((K == K_FCC || K == K_BCT) && P == CSX_PREC_MIXED) {
+        static const void* const tab[kTuneVariants] = {
+            (const void*)k_fit_batch_lat<K, P>, (const void*)k_fit_batch_free<K, P>,
+            (const void*)k_fit_batch_cap<K, P, 5>, (const void*)k_fit_batch_cap<K, P, 6>};
+        return tab[(variant >= 0 && variant < kTuneVariants) ? variant : 0];
+    } else if constexpr (K == K_FCC || K == K_BCT) {
+        return (const void*)k_fit_batch_lat<K, P>;
+    } else {
+        return (const void*)k_fit_batch<K, P>;
+    }
+}
+template <int K, int P>
+static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, TileSum* part, int variant,
+                             hipStream_t st) {
+    void* args[] = {(void*)&sets, (void*)&tiles, (void*)&ntiles, (void*)&part};
+    (void)hipLaunchKernel(fit_batch_kernel<K, P>(variant), dim3(ntiles), dim3(kBlock), args, 0, st);
 }
 
 template <int K, int P>
@@ -1810,6 +1937,20 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
+// The same dispatch for a function returning a value (out = FN<K, P>(args)).
+#define DISPATCH_KIND_FN(kind, prec, FN, out, ...)                                  \
+    switch (kind) {                                                                 \
+    case K_SPHERE: out = (prec) ? FN<K_SPHERE, 1>(__VA_ARGS__) : FN<K_SPHERE, 0>(__VA_ARGS__); break; \
+    case K_FCC: out = (prec) ? FN<K_FCC, 1>(__VA_ARGS__) : FN<K_FCC, 0>(__VA_ARGS__); break; \
+    case K_CYL_GAUSS: out = (prec) ? FN<K_CYL_GAUSS, 1>(__VA_ARGS__) : FN<K_CYL_GAUSS, 0>(__VA_ARGS__); break; \
+    case K_CYL_ZDIR: out = (prec) ? FN<K_CYL_ZDIR, 1>(__VA_ARGS__) : FN<K_CYL_ZDIR, 0>(__VA_ARGS__); break; \
+    case K_CYL_GAUSS_Q: out = FN<K_CYL_GAUSS_Q, 1>(__VA_ARGS__); break; \
+    case K_CUBE: out = (prec) ? FN<K_CUBE, 1>(__VA_ARGS__) : FN<K_CUBE, 0>(__VA_ARGS__); break; \
+    case K_CYL_ISO: out = (prec) ? FN<K_CYL_ISO, 1>(__VA_ARGS__) : FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
+    case K_CUBE_ZDIR: out = (prec) ? FN<K_CUBE_ZDIR, 1>(__VA_ARGS__) : FN<K_CUBE_ZDIR, 0>(__VA_ARGS__); break; \
+    default: out = (prec) ? FN<K_BCT, 1>(__VA_ARGS__) : FN<K_BCT, 0>(__VA_ARGS__); break; \
+    }
+
 // One render: the radial pre-pass (if any) on the side stream, after the image kernel that
 // last read the same table; the image kernel on the context stream, after the pre-pass.
 // Back-to-back renders thus overlap a pre-pass with the previous image kernel.
@@ -1842,6 +1983,7 @@ int csx_create(int device, csx_ctx** out) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CSX_E_HIP;
     if (device < 0 || device >= n) return CSX_E_ARG;
+    std::call_once(g_tune_once, tune_load_default);
     csx_ctx* c = new csx_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1872,6 +2014,8 @@ void csx_destroy(csx_ctx* c) {
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1]})
         b->release();
+    if (c->bgraph.exec) (void)hipGraphExecDestroy(c->bgraph.exec);
+    if (c->bgraph.graph) (void)hipGraphDestroy(c->bgraph.graph);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_rad[0], c->ev_rad[1], c->ev_img[0], c->ev_img[1]})
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -1900,6 +2044,44 @@ int csx_synchronize(csx_ctx* c) {
 int csx_last_error(csx_ctx* c, char* buf, int len) {
     if (!c || !buf || len <= 0) return CSX_E_ARG;
     std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
+    return CSX_OK;
+}
+
+int csx_tune_set(uint32_t combo_key, int size_class, int variant) {
+    const int k = kind_of(combo_key);
+    if (k < 0 || k >= kMaxKinds || size_class < -1 || size_class >= kTuneSizes || variant < 0 ||
+        variant >= kTuneVariants)
+        return CSX_E_ARG;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (int sc = 0; sc < kTuneSizes; ++sc)
+        if (size_class < 0 || sc == size_class) g_tune[k][sc] = variant;
+    return CSX_OK;
+}
+
+int csx_tune_get(uint32_t combo_key, int size_class) {
+    const int k = kind_of(combo_key);
+    if (k < 0 || k >= kMaxKinds || size_class < 0 || size_class >= kTuneSizes) return CSX_E_ARG;
+    return tune_variant(k, size_class);
+}
+
+int csx_tune_load(const char* path) {
+    if (!path) return CSX_E_ARG;
+    const int n = tune_load_file(path);
+    return n < 0 ? CSX_E_ARG : n;
+}
+
+int csx_set_batch_graph(csx_ctx* c, int enable) {
+    if (!c) return CSX_E_ARG;
+    if (c->bres.size) return fail(c, CSX_E_STATE, "a batch is in flight");
+    c->use_graph = enable != 0;
+    return CSX_OK;
+}
+
+long long csx_batch_graph_launches(const csx_ctx* c) { return c ? c->bgraph.launches : -1; }
+
+int csx_tune_force(csx_ctx* c, int variant) {
+    if (!c || variant < -1 || variant >= kTuneVariants) return CSX_E_ARG;
+    c->tune_force = variant;
     return CSX_OK;
 }
 
@@ -2055,25 +2237,26 @@ int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, c
     f.h = h;
     f.tiles_x = (w + kTile - 1) / kTile;
     const int ntiles = f.tiles_x * ((h + kTile - 1) / kTile);
-    HIPCHK(c, c->tiles.ensure((size_t)ntiles * sizeof(TileSum)));
+    HIPCHK(c, c->tiles.ensure((size_t)ntiles * sizeof(TileSum) + sizeof(SetSum)));
+    TileSum* part = (TileSum*)c->tiles.p;
+    SetSum* total = (SetSum*)(part + ntiles);
     const int kind = kind_of(s.combo_key);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    DISPATCH_KIND(kind, precision, launch_fit_one, f, ntiles, d_fqs, (TileSum*)c->tiles.p, c->stream);
+    DISPATCH_KIND(kind, precision, launch_fit_one, f, ntiles, d_fqs, part, c->stream);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(kBlock), 0, c->stream, part, ntiles, total);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    std::vector<TileSum> part(ntiles);
-    HIPCHK(c, hipMemcpyAsync(part.data(), c->tiles.p, ntiles * sizeof(TileSum), hipMemcpyDeviceToHost, c->stream));
+    SetSum tot{};
+    HIPCHK(c, hipMemcpyAsync(&tot, total, sizeof(SetSum), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     c->timing_pending = false;
-    double sum = 0.0;
-    int64_t cnt = 0, nn = 0;
-    for (const auto& p : part) { sum += p.sum; cnt += p.cnt; nn += p.nan; }
-    *h_sum = sum;
-    if (h_cnt) *h_cnt = cnt;
-    if (h_nan) *h_nan = nn;
+    *h_sum = tot.sum;
+    if (h_cnt) *h_cnt = tot.cnt;
+    if (h_nan) *h_nan = tot.nan;
     return CSX_OK;
 }
 
@@ -2108,11 +2291,12 @@ int csx_fit_fqs_noalloc(csx_ctx* c, int w, int h, const csx_mask* mask, int prec
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     DISPATCH_KIND(kind, precision, launch_fit_one, f, (int)ntiles, d_fqs, part, c->stream);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(64), 0, c->stream, part, (int)ntiles, part + kScratchTiles);
+    SetSum* d_total = (SetSum*)(part + kScratchTiles);
+    hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(kBlock), 0, c->stream, part, (int)ntiles, d_total);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    TileSum total{};
-    HIPCHK(c, hipMemcpyAsync(&total, part + kScratchTiles, sizeof(TileSum), hipMemcpyDeviceToHost, c->stream));
+    SetSum total{};
+    HIPCHK(c, hipMemcpyAsync(&total, d_total, sizeof(SetSum), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -2325,7 +2509,8 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
 // The staged batch's set records, tile list and per-set tile counts appended to the pinned
 // arena, with the device addresses of every set's tables filled in; one H2D copy on the
 // context's stream.  Returns the device addresses of the records and the tile list.
-static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles, const int** d_ntiles) {
+static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles, const int** d_ntiles,
+                        size_t* bytes = nullptr) {
     const int nsets = (int)c->bsets.size();
     const int ntiles = (int)c->btiles.size();
     auto al = [](size_t o) { return (o + 255) & ~size_t(255); };
@@ -2355,10 +2540,58 @@ static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles,
     std::memcpy(c->barena.p + o_tiles, c->btiles.data(), ntiles * sizeof(int2));
     std::memcpy(c->barena.p + o_ntiles, c->bntiles.data(), nsets * sizeof(int));
     c->barena.size = total;
-    HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
+    if (bytes) *bytes = total;  // the caller copies (a node of the round's graph)
+    else HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
     *d_sets = (const FitSet*)(base + o_sets);
     *d_tiles = (const int2*)(base + o_tiles);
     *d_ntiles = (const int*)(base + o_ntiles);
+    return CSX_OK;
+}
+
+// One round of the batched objective as a HIP graph: the arena copy to HBM, the objective
+// kernel over the flat tile list, the per-set reduction and the copy of the sums to pinned
+// memory.  The graph is built once per context and kernel; every later round updates its
+// four nodes in place (sizes, pointers, grid) and launches it with one call.
+static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, const FitSet* d_sets,
+                              const int2* d_tiles, const int* d_ntiles, int nsets, int ntiles) {
+    BatchGraph& G = c->bgraph;
+    TileSum* part = (TileSum*)c->bpart_d.p;
+    SetSum* sums = (SetSum*)c->bsums_d.p;
+    void* fit_args[] = {(void*)&d_sets, (void*)&d_tiles, (void*)&ntiles, (void*)&part};
+    void* red_args[] = {(void*)&d_sets, (void*)&nsets, (void*)&part, (void*)&d_ntiles, (void*)&sums};
+    hipKernelNodeParams kf{};
+    kf.func = const_cast<void*>(fit_fn);
+    kf.gridDim = dim3(ntiles);
+    kf.blockDim = dim3(kBlock);
+    kf.kernelParams = fit_args;
+    hipKernelNodeParams kr{};
+    kr.func = (void*)k_reduce_sets;
+    kr.gridDim = dim3(nsets);
+    kr.blockDim = dim3(kBlock);
+    kr.kernelParams = red_args;
+    const size_t sbytes = (size_t)nsets * sizeof(SetSum);
+    if (!G.exec || G.fit_fn != fit_fn) {
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        if (G.graph) (void)hipGraphDestroy(G.graph);
+        G.exec = nullptr;
+        G.graph = nullptr;
+        HIPCHK(c, hipGraphCreate(&G.graph, 0));
+        HIPCHK(c, hipGraphAddMemcpyNode1D(&G.h2d, G.graph, nullptr, 0, c->barena_d.p, c->barena.p, bytes,
+                                          hipMemcpyHostToDevice));
+        HIPCHK(c, hipGraphAddKernelNode(&G.fit, G.graph, &G.h2d, 1, &kf));
+        HIPCHK(c, hipGraphAddKernelNode(&G.red, G.graph, &G.fit, 1, &kr));
+        HIPCHK(c, hipGraphAddMemcpyNode1D(&G.d2h, G.graph, &G.red, 1, c->bres.p, sums, sbytes, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+        G.fit_fn = fit_fn;
+    } else {
+        HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G.exec, G.h2d, c->barena_d.p, c->barena.p, bytes,
+                                                    hipMemcpyHostToDevice));
+        HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.fit, &kf));
+        HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.red, &kr));
+        HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G.exec, G.d2h, c->bres.p, sums, sbytes, hipMemcpyDeviceToHost));
+    }
+    HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
+    ++G.launches;
     return CSX_OK;
 }
 
@@ -2371,23 +2604,30 @@ int csx_fit_batch_launch(csx_ctx* c, int precision) {
     const FitSet* d_sets;
     const int2* d_tiles;
     const int* d_ntiles;
-    if (int rc = batch_upload(c, &d_sets, &d_tiles, &d_ntiles)) return rc;
+    size_t bytes = 0;
+    if (int rc = batch_upload(c, &d_sets, &d_tiles, &d_ntiles, c->use_graph ? &bytes : nullptr)) return rc;
     HIPCHK(c, c->bpart_d.ensure(ntiles * sizeof(TileSum)));
-    HIPCHK(c, c->bsums_d.ensure(nsets * (sizeof(double) + sizeof(long long))));
-    HIPCHK(c, c->bres.reserve(nsets * (sizeof(double) + sizeof(long long))));
+    HIPCHK(c, c->bsums_d.ensure(nsets * sizeof(SetSum)));
+    HIPCHK(c, c->bres.reserve(nsets * sizeof(SetSum)));
     const int kind = kind_of(c->bsets[0].d.s.combo_key);
+    const int variant = c->tune_force >= 0 ? c->tune_force : tune_variant(kind, tune_size_class(ntiles, nsets));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, d_tiles, ntiles, (TileSum*)c->bpart_d.p, ntiles,
-                  c->stream);
-    HIPCHK(c, hipGetLastError());
-    double* d_sums = (double*)c->bsums_d.p;
-    long long* d_cnts = (long long*)(d_sums + nsets);
-    hipLaunchKernelGGL(k_reduce_sets, dim3((nsets + 255) / 256), dim3(256), 0, c->stream, d_sets, nsets,
-                       (const TileSum*)c->bpart_d.p, d_ntiles, d_sums, d_cnts);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->bres.p, d_sums, nsets * (sizeof(double) + sizeof(long long)), hipMemcpyDeviceToHost,
-                             c->stream));
+    if (c->use_graph) {
+        const void* fn = nullptr;
+        DISPATCH_KIND_FN(kind, precision, fit_batch_kernel, fn, variant);
+        if (int rc = batch_graph_launch(c, fn, bytes, d_sets, d_tiles, d_ntiles, nsets, ntiles)) return rc;
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    } else {
+        DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, d_tiles, ntiles, (TileSum*)c->bpart_d.p, variant,
+                      c->stream);
+        HIPCHK(c, hipGetLastError());
+        SetSum* d_sums = (SetSum*)c->bsums_d.p;
+        hipLaunchKernelGGL(k_reduce_sets, dim3(nsets), dim3(kBlock), 0, c->stream, d_sets, nsets,
+                           (const TileSum*)c->bpart_d.p, d_ntiles, d_sums);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->bres.p, d_sums, nsets * sizeof(SetSum), hipMemcpyDeviceToHost, c->stream));
+    }
     c->bres.size = (size_t)nsets;
     return CSX_OK;
 }
@@ -2398,10 +2638,9 @@ int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts) {
     const int nsets = (int)c->bres.size;
     if (nsets == 0) return fail(c, CSX_E_STATE, "no batch in flight");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const double* s = (const double*)c->bres.p;
-    const long long* n = (const long long*)(s + nsets);
-    for (int i = 0; i < nsets; ++i) h_sums[i] = s[i];
-    if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = n[i];
+    const SetSum* r = (const SetSum*)c->bres.p;
+    for (int i = 0; i < nsets; ++i) h_sums[i] = r[i].sum;
+    if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = r[i].cnt;
     c->bres.size = 0;
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));

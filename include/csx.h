@@ -185,6 +185,29 @@ int csx_fit_batch_run(csx_ctx* ctx, int precision, double* h_sums, int64_t* h_cn
  * doubles, written in IDX order.  All sets share one combination key.  Synchronous. */
 int csx_render_batch_run(csx_ctx* ctx, int precision);
 
+/* Kernel variants of the batched objective (no reference counterpart).  For the lattice
+ * kinds in mixed precision, csx_fit_batch_run has CSX_TUNE_VARIANTS kernels, the same code
+ * with the register budget cut for 4 waves per SIMD (variant 0), uncapped (1), cut for 5 (2)
+ * or 6 waves (3).  They compute the same sums; a table picks the fastest per
+ * combination and detector size class (mean tiles per set: <= 64, <= 256, <= 576, more), filled
+ * from tuning.txt beside the library at the first csx_create (lines "key class variant", as
+ * crystalscatter_amd/autotune.py writes them).  size_class -1 sets every class.
+ * csx_tune_force makes one context use a variant for every batch (-1: the table), for
+ * measurement.  csx_tune_load returns the number of entries read. */
+#define CSX_TUNE_VARIANTS 4
+#define CSX_TUNE_SIZES 4
+int csx_tune_set(uint32_t combo_key, int size_class, int variant);
+int csx_tune_get(uint32_t combo_key, int size_class);
+int csx_tune_load(const char* path);
+int csx_tune_force(csx_ctx* ctx, int variant);
+/* Batched rounds as HIP graphs (default on): csx_fit_batch_run and the batched fits launch
+ * each round (arena copy, objective kernel, per-set reduction, copy of the sums) as one graph,
+ * built once per context and kernel and updated in place between rounds.  Off: the same four
+ * operations launched one by one (same results).  csx_batch_graph_launches counts the graph
+ * launches of a context. */
+int csx_set_batch_graph(csx_ctx* ctx, int enable);
+long long csx_batch_graph_launches(const csx_ctx* ctx);
+
 /* ---- Batched Nelder-Mead fits ------------------------------------------------------
  * The simplex of SasCalc_SimplexFit2D::doSimplexFit2D (Source/sc_simplexfit2d.cpp:254-611,
  * amotry :221-251) with the bounds rule of fehlerquadratsumme (:25-44: a point outside

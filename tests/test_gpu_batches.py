@@ -132,3 +132,85 @@ def test_reference_fits_on_the_gpu():
         print("%s: iters %d/%d evals %d/%d" % (name, it, c["repetitions"], ev, c["numImgCalc"]), flush=True)
         assert it == c["repetitions"] and ev == c["numImgCalc"], name
         np.testing.assert_array_equal(x, fx[name + "_fitres"])
+
+
+def _stage_sizes(ctx, sizes, seed, keep):
+    """Stage BCT sets of the given GridPoints (uneven detectors) with random data."""
+    import torch
+    L = _lib.lib()
+    m = manifest()["fixtures"]["cfg4_bct.npz"]
+    rng = np.random.default_rng(seed)
+    mask = _lib.Mask()
+    for i in range(16):
+        mask.rects[i] = -1
+    mask.bstop, mask.border = 3, 0
+    assert L.csx_fit_batch_begin(ctx._h) == 0
+    for k, n in enumerate(sizes):
+        h = SasCalc(device=-1, params=dict(m["params"][k % 6], GridPoints=n))
+        h.set("ignore_cbs", 1)
+        h.prepare()
+        carr, l1, l2, l3 = _tables(h)
+        d = torch.tensor(rng.uniform(0.01, 10.0, 4 * n * n), dtype=torch.float64, device="cuda")
+        keep += [carr, l1, l2, l3, h, d]
+        assert L.csx_fit_batch_stage(ctx._h, ctypes.byref(h.scalars()), _ptr(carr, ctypes.c_double),
+                                     _ptr(l1, ctypes.c_int32), _ptr(l2, ctypes.c_int32), _ptr(l3, ctypes.c_float),
+                                     2 * n, 2 * n, ctypes.byref(mask), ctypes.c_void_p(d.data_ptr())) == 0
+
+
+def _run(ctx, nsets, prec=1):
+    L = _lib.lib()
+    sums = np.zeros(nsets)
+    cnts = np.zeros(nsets, dtype=np.int64)
+    assert L.csx_fit_batch_run(ctx._h, prec, _ptr(sums, ctypes.c_double), _ptr(cnts, ctypes.c_int64)) == 0
+    return sums, cnts
+
+
+def test_batch_graph_replays_equal_eager_rounds():
+    """Rounds of different shapes launched as graph replays (the graph built once, updated in
+    place) against the same rounds launched operation by operation, with eager work and
+    synchronisations between the replays: every replay equals its eager round bit for bit."""
+    import torch
+    L = _lib.lib()
+    ctx = Context(0)
+    rounds = [[16, 24], [32, 16, 40, 8], [24], [40, 40, 16], [16, 24]]
+    n0 = L.csx_batch_graph_launches(ctx._h)
+    for r, sizes in enumerate(rounds):
+        for prec in (1, 0):
+            keep = []
+            assert L.csx_set_batch_graph(ctx._h, 0) == 0
+            _stage_sizes(ctx, sizes, r, keep)
+            eager = _run(ctx, len(sizes), prec)
+            x = torch.randn(4096, device="cuda").sum()  # eager work between replays
+            torch.cuda.synchronize()
+            assert L.csx_set_batch_graph(ctx._h, 1) == 0
+            _stage_sizes(ctx, sizes, r, keep)
+            graph = _run(ctx, len(sizes), prec)
+            np.testing.assert_array_equal(graph[0], eager[0])
+            np.testing.assert_array_equal(graph[1], eager[1])
+            del x
+    assert L.csx_batch_graph_launches(ctx._h) - n0 == 2 * len(rounds)
+
+
+@pytest.mark.parametrize("key_params", ["bct", "fcc"])
+def test_tuning_variants_give_the_same_sums(key_params):
+    """Every kernel variant of the batched objective (different register budgets) gives the
+    same sums and counts bit for bit."""
+    from crystalscatter_amd import autotune
+    import torch
+    L = _lib.lib()
+    key = 0x01020806 if key_params == "bct" else 0x00020500
+    rng = np.random.default_rng(3)
+    params = autotune._sets(key, 64, 12, rng)
+    data = torch.tensor(rng.uniform(0.01, 10.0, 64 * 64), dtype=torch.float64, device="cuda")
+    ctx = Context(0)
+    ref = None
+    for v in range(_lib.TUNE_VARIANTS):
+        assert L.csx_tune_force(ctx._h, v) == 0
+        keep = autotune._stage(ctx, params, 64, data)
+        got = _run(ctx, len(params))
+        del keep
+        if ref is None:
+            ref = got
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+    assert L.csx_tune_force(ctx._h, -1) == 0
