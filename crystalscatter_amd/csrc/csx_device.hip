@@ -90,6 +90,43 @@ __device__ double aniso_term_ref(const DevSet& d, int ii, double qx, double qy, 
     return sq * mf * psi;
 }
 
+// A pair with |q^.u| near 1 in the hot image pass.  The reference takes the angle of each
+// reflection as acos of its own FP64 cosine c = q.q_hkl / (|q| |q_hkl|) (|q_hkl| stored in
+// FP32), which near 1 is off the true angle by up to ~3e-4 rad, and maps c > 1 to pi and
+// c < -1 to 0.  The partner's cosine is exactly -c, so the pair's two terms are the pair
+// formula at y = acos(|c|), or at y = 0 when |c| > 1 (the two clamps swap the terms).  The
+// FP64 cosine decides; x = 1 - |c| is exact in FP64 and acos(1-x)^2 = 2x + x^2/3 + O(x^3)
+// (x < 3e-6 here).
+struct NearCtx {
+    const DevSet* d;
+    int ihex, i;
+    double q;
+};
+template <bool PLANAR>
+__device__ __forceinline__ double near_pair_x(const NearCtx& nc, int ia) {
+    const DevSet& d = *nc.d;
+    double qx, qy, qz;
+    if constexpr (PLANAR) {
+        // pixel_q without its beam-geometry branch (the detector plane has no centre beam)
+        const csx_scalars& s = d.s;
+        if (s.use1d) {
+            qx = (s.qmax - s.qmin) / (double)(s.zmax) * (double)nc.ihex;
+            qy = 1e-20;
+            qz = 1e-20;
+        } else {
+            qx = s.qmax * nc.i / (double)(s.zmax);
+            qy = s.qmax * nc.ihex / (double)(s.zmax);
+            qz = 1e-20;
+        }
+    } else {
+        pixel_q(d.s, nc.ihex, nc.i, qx, qy, qz);
+    }
+    const float* L3 = d.latpar3 + ia * 17;
+    const double qhkl = L3[5];
+    const double c = fabs((qx * (double)L3[2] + qy * (double)L3[3] + qz * (double)L3[4]) / (nc.q * qhkl));
+    return c > 1.0 ? 0.0 : 1.0 - c;
+}
+
 // Reference-order FP64 sums: intensity over the anisotropic list (stopping at the first
 // inf, :131-140) and radintensity over the isotropic list (reading the anisotropic q_hkl
 // column, :69-83).
@@ -133,14 +170,48 @@ __device__ __forceinline__ float acos2_over_sin2(float c) {
     p = __builtin_fmaf(p, x, 9.999999404e-01f);
     return p;
 }
+// The same polynomial for four arguments, Horner step by step across them (four independent
+// chains in flight); each result equals acos2_over_sin2 of its argument.
+__device__ __forceinline__ void acos2_over_sin2_x4(const float* c, float* out) {
+    constexpr float kC[10] = {9.084462374e-02f, -2.716867626e-01f, 4.152069092e-01f, -2.917656898e-01f,
+                              2.068143934e-01f, 6.417080760e-02f,  2.098862827e-01f, 3.772477508e-01f,
+                              6.666826606e-01f, 9.999999404e-01f};
+    float x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        x[j] = 1.0f - c[j];
+        out[j] = kC[0];
+    }
+#pragma unroll
+    for (int k = 1; k < 10; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[j] = __builtin_fmaf(out[j], x[j], kC[k]);
+}
 
+// Wave minimum / maximum through DPP moves (all 64 lanes active): prefix steps within each
+// 16-lane row (row_shr 1, 2, 4, 8), then the row_bcast steps carry rows 0..3 into lane 63,
+// whose value every lane reads.  A lane whose DPP source is out of its row keeps its value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
 __device__ __forceinline__ float wave_min_f(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fminf(v, dpp_move<0x111, 0xf>(v));  // row_shr:1
+    v = fminf(v, dpp_move<0x112, 0xf>(v));  // row_shr:2
+    v = fminf(v, dpp_move<0x114, 0xf>(v));  // row_shr:4
+    v = fminf(v, dpp_move<0x118, 0xf>(v));  // row_shr:8
+    v = fminf(v, dpp_move<0x142, 0xa>(v));  // row_bcast:15 into rows 1 and 3
+    v = fminf(v, dpp_move<0x143, 0xc>(v));  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ float wave_max_f(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmaxf(v, dpp_move<0x111, 0xf>(v));
+    v = fmaxf(v, dpp_move<0x112, 0xf>(v));
+    v = fmaxf(v, dpp_move<0x114, 0xf>(v));
+    v = fmaxf(v, dpp_move<0x118, 0xf>(v));
+    v = fmaxf(v, dpp_move<0x142, 0xa>(v));
+    v = fmaxf(v, dpp_move<0x143, 0xc>(v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // Butterfly sums: every lane gets the total, in the same order on every lane.
@@ -223,41 +294,80 @@ __device__ __forceinline__ PairRec64 load_rec(const __attribute__((address_space
 // Sum over pairs [p0, p1) (p0, p1 multiples of 4, wave-uniform) of A*(2^e1 [+ 2^e2]).
 // y^2 = sin^2(y) * acos^2(c)/(1-c^2), with sin^2 y = |q^ x u|^2 taken from the cross
 // product: accurate where the Gaussian is narrow (y small), where 1-|q^.u| is not.
-// Pairs with |q^.u| > kNearOne contribute 0 here and set `flag`.
-template <bool PLANAR, bool TWO>
+// With CHECK, pairs with |q^.u| > kNearOne set `flag` and contribute 0 here (the caller adds
+// their reference terms), or with NEAR as well, the pair formula at the reference's angle.
+template <bool PLANAR, bool TWO, bool CHECK, bool NEAR = false>
 __device__ __forceinline__ void pair_sum(const PairRec* __restrict__ prg, int p0, int p1, const PixelConsts& L,
-                                         double& acc, bool& flag) {
+                                         double& acc, bool& flag, const NearCtx* nc = nullptr) {
+    // The four pairs of a group are independent chains, written stage by stage so that the
+    // schedule interleaves them; the terms are added to `part` in pair order.
     const auto pr = uniform_table(prg);
     for (int p = p0; p < p1; p += 4) {
-        float part = 0.0f;
+        PairRec P[4];
+        float c[4], ss[4], y2[4], e1[4], w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) P[j] = load_rec(pr, p + j);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const PairRec P = load_rec(pr, p + j);
-            float dt = __builtin_fmaf(L.hy, P.uy, L.hx * P.ux);
-            float ss;
+            float dt = __builtin_fmaf(L.hy, P[j].uy, L.hx * P[j].ux);
             if constexpr (PLANAR) {
-                const float k = __builtin_fmaf(L.hx, P.uy, -(L.hy * P.ux));
-                ss = __builtin_fmaf(k, k, P.uz2);
+                const float k = __builtin_fmaf(L.hx, P[j].uy, -(L.hy * P[j].ux));
+                ss[j] = __builtin_fmaf(k, k, P[j].uz2);
             } else {
-                dt = __builtin_fmaf(L.hz, P.uz, dt);
-                const float k1 = __builtin_fmaf(L.hy, P.uz, -(L.hz * P.uy));
-                const float k2 = __builtin_fmaf(L.hz, P.ux, -(L.hx * P.uz));
-                const float k3 = __builtin_fmaf(L.hx, P.uy, -(L.hy * P.ux));
-                ss = __builtin_fmaf(k1, k1, __builtin_fmaf(k2, k2, k3 * k3));
+                dt = __builtin_fmaf(L.hz, P[j].uz, dt);
+                const float k1 = __builtin_fmaf(L.hy, P[j].uz, -(L.hz * P[j].uy));
+                const float k2 = __builtin_fmaf(L.hz, P[j].ux, -(L.hx * P[j].uz));
+                const float k3 = __builtin_fmaf(L.hx, P[j].uy, -(L.hy * P[j].ux));
+                ss[j] = __builtin_fmaf(k1, k1, __builtin_fmaf(k2, k2, k3 * k3));
             }
-            const float c = fabsf(dt);
-            const float y2 = ss * acos2_over_sin2(c);
-            const float t = (L.qf - P.qh) + L.qlo;
+            c[j] = fabsf(dt);
+        }
+        acos2_over_sin2_x4(c, y2);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            y2[j] = ss[j] * y2[j];
+            const float t = (L.qf - P[j].qh) + L.qlo;
             const float r = (t * L.kr) * t;
-            const float e1 = __builtin_fmaf(L.kp, y2, r);
-            float w = __builtin_amdgcn_exp2f(e1);
-            if constexpr (TWO) {
-                const float y = __builtin_amdgcn_sqrtf(y2);
-                w += __builtin_amdgcn_exp2f(__builtin_fmaf(L.c2, y, e1 + L.c1));
+            e1[j] = __builtin_fmaf(L.kp, y2[j], r);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] = __builtin_amdgcn_exp2f(e1[j]);
+        if constexpr (TWO) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float y = __builtin_amdgcn_sqrtf(y2[j]);
+                w[j] += __builtin_amdgcn_exp2f(__builtin_fmaf(L.c2, y, e1[j] + L.c1));
             }
-            const bool fl = c > kNearOne;
-            flag = flag || fl;
-            part = __builtin_fmaf(P.A, fl ? 0.0f : w, part);
+        }
+        float part = 0.0f;
+        bool fl[4] = {false, false, false, false};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if constexpr (CHECK) {
+                fl[j] = c[j] > kNearOne;
+                flag = flag || fl[j];
+                w[j] = fl[j] ? 0.0f : w[j];
+            }
+            part = __builtin_fmaf(P[j].A, w[j], part);
+        }
+        if constexpr (CHECK && NEAR) {
+            // the pair formula at the reference's angle (see near_pair_x), after the group's
+            // other pairs
+            if (__ballot(fl[0] || fl[1] || fl[2] || fl[3])) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (fl[j]) {
+                        const float x = (float)near_pair_x<PLANAR>(*nc, P[j].ia);
+                        const float yy = x * __builtin_fmaf(x, 1.0f / 3.0f, 2.0f);
+                        const float t = (L.qf - P[j].qh) + L.qlo;
+                        const float ee = __builtin_fmaf(L.kp, yy, (t * L.kr) * t);
+                        float wn = __builtin_amdgcn_exp2f(ee);
+                        if constexpr (TWO)
+                            wn += __builtin_amdgcn_exp2f(__builtin_fmaf(L.c2, __builtin_amdgcn_sqrtf(yy), ee + L.c1));
+                        part = __builtin_fmaf(P[j].A, wn, part);
+                    }
+                }
+            }
         }
         acc += (double)part;
     }
@@ -269,8 +379,9 @@ __device__ double near_one_terms(const DevSet& d, int p0, int p1, const PixelCon
                                  double width, double x2phi) {
     const double qx = P3.qx, qy = P3.qy, qz = P3.qz;
     double add = 0.0;
+    const auto pr = uniform_table(d.pairs);
     for (int p = p0; p < p1; ++p) {
-        const PairRec P = d.pairs[p];
+        const PairRec P = load_rec(pr, p);
         float dt = __builtin_fmaf(L.hy, P.uy, L.hx * P.ux);
         if constexpr (!PLANAR) dt = __builtin_fmaf(L.hz, P.uz, dt);
         if (fabsf(dt) > kNearOne)
@@ -315,9 +426,9 @@ struct PixelConsts64 {
 // The precise pair loop: the FP32 loop's sum with FP64 exponents.  The reference's own
 // cosine c = q.q_hkl / (|q| |q_hkl|) is formed in FP64 and y^2 = acos(c)^2 = x*g(x), x = 1-c,
 // needs no cancellation-free rewrite in FP64.  Same window, same near-one rule.
-template <bool PLANAR, bool TWO>
+template <bool PLANAR, bool TWO, bool CHECK, bool NEAR = false>
 __device__ __forceinline__ void pair_sum64(const PairRec64* __restrict__ prg, int p0, int p1, const PixelConsts64& L,
-                                           double& acc, bool& flag) {
+                                           double& acc, bool& flag, const NearCtx* nc = nullptr) {
     constexpr double kPi = 3.14159265358979323846;
     const auto pr = uniform_table(prg);
     for (int p = p0; p < p1; ++p) {
@@ -336,9 +447,25 @@ __device__ __forceinline__ void pair_sum64(const PairRec64* __restrict__ prg, in
             const double y = (double)__builtin_amdgcn_sqrtf((float)y2);
             w += exp2_split(fma(L.kp * kPi, kPi - 2.0 * y, e1));
         }
-        const bool fl = c > (double)kNearOne;
-        flag = flag || fl;
-        acc = fma(P.A, fl ? 0.0 : w, acc);
+        if constexpr (CHECK) {
+            const bool fl = c > (double)kNearOne;
+            flag = flag || fl;
+            w = fl ? 0.0 : w;
+            if constexpr (NEAR) {
+                if (fl) {
+                    const double xx = near_pair_x<PLANAR>(*nc, P.ia);
+                    const double yy = xx * acos2_over_x(xx);
+                    const double tt = L.q - (double)P.qh;
+                    const double ee = fma(L.kp, yy, (tt * L.kr) * tt);
+                    w = exp2_split(ee);
+                    if constexpr (TWO) {
+                        const double y = (double)__builtin_amdgcn_sqrtf((float)yy);
+                        w += exp2_split(fma(L.kp * kPi, kPi - 2.0 * y, ee));
+                    }
+                }
+            }
+        }
+        acc = fma(P.A, w, acc);
     }
 }
 
@@ -347,8 +474,9 @@ template <bool PLANAR>
 __device__ double near_one_terms64(const DevSet& d, int p0, int p1, const PixelConsts64& L, const PixQ& P3, double q,
                                    double width, double x2phi) {
     double add = 0.0;
+    const auto pr = uniform_table(d.pairs64);
     for (int p = p0; p < p1; ++p) {
-        const PairRec64 P = d.pairs64[p];
+        const PairRec64 P = load_rec(pr, p);
         double dt = fma(L.hy, P.uy, L.hx * P.ux);
         if constexpr (!PLANAR) dt = fma(L.hz, P.uz, dt);
         if (fabs(dt) > (double)kNearOne)
@@ -361,29 +489,85 @@ __device__ double near_one_terms64(const DevSet& d, int p0, int p1, const PixelC
 // Pair-window bookkeeping of one wave: |q| bins [b0, b1] and |u_z| bands [band0, band1].
 struct Window {
     int b0, b1, band0, band1;
+    bool near;  // some pair of this lane's window has |q^.u| > kNearOne
 };
 
-// Mixed-precision anisotropic peak sum (sum over reflections of sq*mhkl*fhkl*psi).  Must be
-// called by all 64 lanes of a wave: the wave's union of per-lane windows selects the pairs.
-// Pairs outside the window, and second Friedel terms where they cannot reach 2^cut, are
-// dropped: every dropped term is below 2^cut * A.  need_exact: unpaired reflections exist.
-template <bool PLANAR, bool PRECISE>
-__device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, double q, double width, bool& need_exact,
-                                double& dropped, Window& W, unsigned long long* stats) {
-    const csx_scalars& s = d.s;
-    PixelConsts L;
+// q^ of the FP32 pair loop: from the exact q vector when the caller has it, else (midpoint
+// grid) q^ = (i, ihex) * (qmax/zmax) / |q| in FP32 (q_z = 1e-20 drops out).
+__device__ __forceinline__ void pixel_dirs(const csx_scalars& s, const PixQ& P3, double q, PixelConsts& L) {
     if (P3.have) {
         const double rq = 1.0 / q;
         L.hx = (float)(P3.qx * rq);
         L.hy = (float)(P3.qy * rq);
         L.hz = (float)(P3.qz * rq);
     } else {
-        // midpoint grid: q^ = (i, ihex) * (qmax/zmax) / |q| in FP32 (q_z = 1e-20 drops out)
         const float c = (float)(s.qmax / s.zmax) * __builtin_amdgcn_rcpf((float)q);
         L.hx = (float)P3.i * c;
         L.hy = (float)P3.ihex * c;
         L.hz = 0.0f;
     }
+}
+__device__ __forceinline__ void pixel_dirs64(const csx_scalars& s, PixQ& P3, double q, PixelConsts64& L64) {
+    need_q(s, P3);
+    const double rq = 1.0 / q;
+    L64.hx = P3.qx * rq;
+    L64.hy = P3.qy * rq;
+    L64.hz = P3.qz * rq;
+}
+
+// A pixel that the hot image pass leaves to the fix pass: its FP32-loop peak sum without the
+// near-one terms, the bound of its dropped mass and its wave's window.
+struct ColdPix {
+    int ihex, i;
+    double inten, dropped;
+    int b0, b1, band0, band1;
+    int flags;  // 1: near-one pairs in the window, 2: unpaired reflections, 4: placeholder lane
+    int pad;
+};
+
+// The fix pass's queue: one slot per hot wave with cold pixels, slot k holding the wave's
+// lane mask and its cold pixels at ent[64 k + lane].  A render uses count[epoch] and clears
+// the other count for the next render (kernels of one stream run in order).
+struct ColdQueue {
+    int* count;
+    unsigned long long* mask;
+    ColdPix* ent;
+    int* next_count;
+};
+
+// How a pixel is evaluated.  kFull: every path in one pass.  kHot: the common paths only;
+// a pixel that needs an FP64 path (near-one pairs, the conditioning guard, unpaired
+// reflections) is reported `cold` and its whole wave is evaluated again by kFix, which runs
+// every path and does not repeat the pair-evaluation counts.
+enum PixMode { kFull = 0, kHot = 1, kFix = 2 };
+
+// Adds to inten, band by band, the FP64 terms (the reference's formula) of the window's pairs
+// with |q^.u| > kNearOne, which the pair loop left out: L (FP32 loop) or L64 (precise loop)
+// holds this pixel's q^.
+template <bool PLANAR, bool PRECISE>
+__device__ __forceinline__ void near_one_add(const DevSet& d, const Window& W, const PixelConsts& L,
+                                             const PixelConsts64& L64, PixQ& P3, double q, double width,
+                                             double& inten) {
+    need_q(d.s, P3);
+    const double x2phi = 4 * q * q / (kPiD * sqd(d.s.phiwidth));  // the reference's expression
+    for (int band = W.band0; band <= W.band1; ++band) {
+        const int32_t* lb = d.lut + band * (d.nlut + 1);
+        const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
+        if constexpr (PRECISE) inten += near_one_terms64<PLANAR>(d, p0, p1, L64, P3, q, width, x2phi);
+        else inten += near_one_terms<PLANAR>(d, p0, p1, L, P3, q, width, x2phi);
+    }
+}
+
+// Mixed-precision anisotropic peak sum (sum over reflections of sq*mhkl*fhkl*psi).  Must be
+// called by all 64 lanes of a wave: the wave's union of per-lane windows selects the pairs.
+// Pairs outside the window, and second Friedel terms where they cannot reach 2^cut, are
+// dropped: every dropped term is below 2^cut * A.  need_exact: unpaired reflections exist.
+template <bool PLANAR, bool PRECISE, int MODE = kFull>
+__device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, double q, double width, bool& need_exact,
+                                double& dropped, Window& W, unsigned long long* stats) {
+    const csx_scalars& s = d.s;
+    PixelConsts L;
+    pixel_dirs(s, P3, q, L);
     L.qf = (float)q;
     L.qlo = (float)(q - (double)L.qf);
     const double kpd = d.kp_coef * q * q;
@@ -445,11 +629,7 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
         dropped += 0.5 * d.asum2 * (double)__builtin_amdgcn_exp2f(fmaxf(L.kp * (hpi * hpi), -126.0f));
     PixelConsts64 L64;
     if constexpr (PRECISE) {
-        need_q(s, P3);
-        const double rq = 1.0 / q;
-        L64.hx = P3.qx * rq;
-        L64.hy = P3.qy * rq;
-        L64.hz = P3.qz * rq;
+        pixel_dirs64(s, P3, q, L64);
         L64.q = q;
         L64.kr = krd;
         L64.kp = kpd;
@@ -457,35 +637,72 @@ __device__ __forceinline__ double lattice_mixed(const DevSet& d, PixQ& P3, doubl
     double acc = 0.0;
     bool flag = false;
     long long evals = 0;
+    // The near-one test is needed only where some pair of the window can have |q^.u| >
+    // kNearOne.  In the detector plane those pairs lie in the in-plane segment: test them first
+    // (once per wave, on the records and in the arithmetic of the pair loop that follows: the
+    // FP64 records for the precise loop, which are the only ones its tables hold), and drop
+    // the test from the pair loop when none is near one.  The loop's 4-aligned range also
+    // takes pairs just outside the window; their terms are below 2^cut * A for every lane,
+    // like the dropped ones.
+    bool check = true;
+    if constexpr (PLANAR) {
+        check = false;
+        if (W.band0 == 0 || W.band0 == kBands) {
+            const int32_t* lp = d.lut + (kBands + 1) * (d.nlut + 1);
+            const int q0 = uniform_i(lp[W.b0]), q1 = uniform_i(lp[W.b1]);
+            bool hit = false;
+            if constexpr (PRECISE) {
+                const auto pr = uniform_table(d.pairs64);
+                for (int p = q0; p < q1; ++p) {
+                    const double dt = fma(L64.hy, pr[p].uy, L64.hx * pr[p].ux);
+                    hit = hit || fabs(dt) > (double)kNearOne;
+                }
+            } else {
+                const auto pr = uniform_table(d.pairs);
+                for (int p = q0; p < q1; ++p) {
+                    const float dt = __builtin_fmaf(L.hy, pr[p].uy, L.hx * pr[p].ux);
+                    hit = hit || fabsf(dt) > kNearOne;
+                }
+            }
+            check = __ballot(hit) != 0ull;
+        }
+    }
+    // The hot image pass evaluates the near-one pairs inside the loop in the detector plane;
+    // with a centre beam it leaves them to the fix pass (whose pixel q needs the FP64 beam
+    // geometry).
+    constexpr bool kNear = MODE == kHot && PLANAR;
+    const NearCtx nc{&d, P3.ihex, P3.i, q};
     for (int band = W.band0; band <= W.band1; ++band) {
         const int32_t* lb = d.lut + band * (d.nlut + 1);
         const int p0 = uniform_i(lb[W.b0] & ~3);
         const int p1 = uniform_i((lb[W.b1] + 3) & ~3);
         if constexpr (PRECISE) {
-            if (any_two) pair_sum64<PLANAR, true>(d.pairs64, p0, p1, L64, acc, flag);
-            else pair_sum64<PLANAR, false>(d.pairs64, p0, p1, L64, acc, flag);
+            if (check) {
+                if (any_two) pair_sum64<PLANAR, true, true, kNear>(d.pairs64, p0, p1, L64, acc, flag, &nc);
+                else pair_sum64<PLANAR, false, true, kNear>(d.pairs64, p0, p1, L64, acc, flag, &nc);
+            } else {
+                if (any_two) pair_sum64<PLANAR, true, false>(d.pairs64, p0, p1, L64, acc, flag);
+                else pair_sum64<PLANAR, false, false>(d.pairs64, p0, p1, L64, acc, flag);
+            }
         } else {
-            if (any_two) pair_sum<PLANAR, true>(d.pairs, p0, p1, L, acc, flag);
-            else pair_sum<PLANAR, false>(d.pairs, p0, p1, L, acc, flag);
+            if (check) {
+                if (any_two) pair_sum<PLANAR, true, true, kNear>(d.pairs, p0, p1, L, acc, flag, &nc);
+                else pair_sum<PLANAR, false, true, kNear>(d.pairs, p0, p1, L, acc, flag, &nc);
+            } else {
+                if (any_two) pair_sum<PLANAR, true, false>(d.pairs, p0, p1, L, acc, flag);
+                else pair_sum<PLANAR, false, false>(d.pairs, p0, p1, L, acc, flag);
+            }
         }
         evals += p1 - p0;
     }
     double inten = acc * inv_hpw;
-    if (flag) {
-        need_q(s, P3);
-        const double x2phi = 4 * q * q / (kPiD * sqd(s.phiwidth));  // the reference's expression
-        for (int band = W.band0; band <= W.band1; ++band) {
-            const int32_t* lb = d.lut + band * (d.nlut + 1);
-            const int p0 = lb[W.b0] & ~3, p1 = (lb[W.b1] + 3) & ~3;
-            if constexpr (PRECISE) inten += near_one_terms64<PLANAR>(d, p0, p1, L64, P3, q, width, x2phi);
-            else inten += near_one_terms<PLANAR>(d, p0, p1, L, P3, q, width, x2phi);
-        }
-    }
-    if (stats && (threadIdx.x & 63) == 0) {
+    W.near = !kNear && flag;
+    if (MODE != kHot && flag) near_one_add<PLANAR, PRECISE>(d, W, L, L64, P3, q, width, inten);
+    if (MODE != kFix && stats && (threadIdx.x & 63) == 0) {
         atomicAdd(&stats[0], (unsigned long long)(64 * evals));
         atomicAdd(&stats[3], (unsigned long long)(any_two ? 1 : 0));
     }
-    if (stats && flag) atomicAdd(&stats[2], 1ull);
+    if (MODE != kFix && stats && flag) atomicAdd(&stats[2], 1ull);
     need_exact = d.nsingle > 0;
     return inten;
 }
@@ -516,10 +733,13 @@ __device__ void coop_window_sum64(const DevSet& d, const Window& W, const PixArg
         m &= m - 1;
         const PixArgs a = shfl_args(me, l);
         const double x2phi = x2c * a.q * a.q;
+        // the requesting lane's window (the same for every lane of an image wave)
+        const int b0 = uniform_i(__shfl(W.b0, l, 64)), b1 = uniform_i(__shfl(W.b1, l, 64));
+        const int band0 = uniform_i(__shfl(W.band0, l, 64)), band1 = uniform_i(__shfl(W.band1, l, 64));
         double acc = 0.0;
-        for (int band = W.band0; band <= W.band1; ++band) {
+        for (int band = band0; band <= band1; ++band) {
             const int32_t* lb = d.lut + band * (d.nlut + 1);
-            const int p0 = uniform_i(lb[W.b0] & ~3), p1 = uniform_i((lb[W.b1] + 3) & ~3);
+            const int p0 = uniform_i(lb[b0] & ~3), p1 = uniform_i((lb[b1] + 3) & ~3);
             for (int p = p0 + lane; p < p1; p += 64) {
                 // the precise loop's kinds carry only the FP64 records
                 const int ia = PRECISE ? d.pairs64[p].ia : d.pairs[p].ia;
@@ -829,9 +1049,12 @@ __device__ double cyl_gauss_quad(const DevSet& d, double qa, double qb, int oc) 
 // ---------------------------------------------------------------------------
 // P3: the pixel (its exact q vector is computed here unless the caller has it); rv: its
 // radial-table entry (midpoint grid, sphere kinds) or null.
-template <int KIND, int PREC>
+// MODE kHot: *cold reports a pixel that needs an FP64 path, with its state in *cp; kFix:
+// finishes such a pixel from *cp.
+template <int KIND, int PREC, int MODE = kFull>
 __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigned long long* stats = nullptr,
-                                              const Radial* rv = nullptr) {
+                                              const Radial* rv = nullptr, bool* cold = nullptr,
+                                              ColdPix* cp = nullptr) {
     const csx_scalars& s = d.s;
     // The mixed FCC path with a radial entry needs only |q| and the pixel indices.
     if (!(rv && KIND == K_FCC && PREC == CSX_PREC_MIXED)) need_q(s, P3);
@@ -925,8 +1148,27 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
             double dropped = 0.0;
             Window W;
             constexpr bool kPrecise = precise_kind(KIND);
-            inten = s.centerbeam ? lattice_mixed<false, kPrecise>(d, P3, q, width, need_exact, dropped, W, stats)
-                                 : lattice_mixed<true, kPrecise>(d, P3, q, width, need_exact, dropped, W, stats);
+            if constexpr (MODE == kFix) {
+                inten = cp->inten;
+                dropped = cp->dropped;
+                // one queue slot holds the cold pixels of one hot wave: the same window
+                W = Window{uniform_i(cp->b0), uniform_i(cp->b1), uniform_i(cp->band0), uniform_i(cp->band1),
+                           (cp->flags & 1) != 0};
+                need_exact = (cp->flags & 2) != 0;
+                if (W.near && !(cp->flags & 4)) {
+                    PixelConsts L;
+                    PixelConsts64 L64;
+                    if constexpr (kPrecise) pixel_dirs64(s, P3, q, L64);
+                    else pixel_dirs(s, P3, q, L);
+                    if (s.centerbeam) near_one_add<false, kPrecise>(d, W, L, L64, P3, q, width, inten);
+                    else near_one_add<true, kPrecise>(d, W, L, L64, P3, q, width, inten);
+                    if (stats) atomicAdd(&stats[2], 1ull);
+                }
+            } else {
+                inten = s.centerbeam
+                            ? lattice_mixed<false, kPrecise, MODE>(d, P3, q, width, need_exact, dropped, W, stats)
+                            : lattice_mixed<true, kPrecise, MODE>(d, P3, q, width, need_exact, dropped, W, stats);
+            }
             radint = (s.iso != 0) ? iso_mixed(d, q) : 0.0;
             // Conditioning guard.  szq = 1 + B - (fq/pq)*dwe with B = (fq/pq)*(2*pi*I/V)*dwe.
             // The peak sum I carries up to kRelErrFP32 (FP32 loop) or kRelErrPrecise (precise
@@ -942,7 +1184,14 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 const double dsz = fabs(bs) * (rel * fabs(I) + dropped * inv_hpw);
                 return !(dsz <= fmax(3.0e-5 * fabs(szq_est), 0x1p-60));
             };
-            const bool want2 = !need_exact && loose(inten, kPrecise ? kRelErrPrecise : kRelErrFP32);
+            // placeholder lanes of the fix pass take no FP64 path
+            const bool live = MODE != kFix || !(cp->flags & 4);
+            const bool want2 = live && !need_exact && loose(inten, kPrecise ? kRelErrPrecise : kRelErrFP32);
+            if constexpr (MODE == kHot) {
+                *cold = W.near || need_exact || want2;
+                *cp = ColdPix{P3.ihex, P3.i, inten, dropped, W.b0, W.b1, W.band0, W.band1,
+                              (W.near ? 1 : 0) | (need_exact ? 2 : 0), 0};
+            } else {
             if (__ballot(want2)) {
                 if (want2) need_q(s, P3);
                 coop_window_sum64<kPrecise>(d, W, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want2, inten);
@@ -953,6 +1202,7 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 if (want3) need_q(s, P3);
                 coop_full_sum64(d, PixArgs{P3.qx, P3.qy, P3.qz, q, width}, want3, inten, radint);
                 if (stats && want3) atomicAdd(&stats[4], 1ull);
+            }
             }
         }
         const double szq = (1 + fqpq * (2 * kPiD * inten / s.sphno_cubevol - 1) * dwe);
@@ -1030,9 +1280,10 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, int half,
 
 // Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.  rtab: radial
 // table of k_radial (midpoint grid, radial kinds) or null.
-template <int KIND, int PREC>
+template <int KIND, int PREC, int MODE = kFull>
 __device__ __forceinline__ void image_tile(const DevSet& d, const csx_grid& g, double* __restrict__ out,
-                                           unsigned long long* __restrict__ stats, const Radial* __restrict__ rtab) {
+                                           unsigned long long* __restrict__ stats, const Radial* __restrict__ rtab,
+                                           ColdQueue cq = ColdQueue{}) {
     // Every lane computes (the mixed path needs all 64 lanes of a wave); lanes past the grid
     // edge evaluate the last valid pixel, and only in-grid pixels are stored.
     int px, py;
@@ -1052,7 +1303,24 @@ __device__ __forceinline__ void image_tile(const DevSet& d, const csx_grid& g, d
         const int a = abs(i), b = abs(ihex);
         rv = rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
     }
-    const double v = pixel_value<KIND, PREC>(d, P3, stats, rv);
+    bool cold = false;
+    ColdPix cpx;
+    const double v = pixel_value<KIND, PREC, MODE>(d, P3, stats, rv, &cold, &cpx);
+    if constexpr (MODE == kHot) {
+        // queue the in-grid cold pixels (lanes past the edge repeat an in-grid pixel)
+        const bool mine = cold && ihex0 < g.xmax && i0 < g.ymax;
+        const unsigned long long m = __ballot(mine);
+        if (m) {
+            const int lane = threadIdx.x & 63;
+            int slot = 0;
+            if (lane == 0) {
+                slot = atomicAdd(cq.count, 1);
+                cq.mask[slot] = m;
+            }
+            slot = __shfl(slot, 0, 64);
+            if (mine) cq.ent[(size_t)slot * 64 + lane] = cpx;
+        }
+    }
     // Stage the tile in LDS and store it row by row: each wave then writes four whole rows of
     // 16 doubles (full 128-byte lines) instead of half lines from its 8x8 quadrant.
     __shared__ double tile[kTile][kTile + 1];
@@ -1073,14 +1341,51 @@ __global__ void __launch_bounds__(kBlock) k_image(const DevSet d, const csx_grid
     image_tile<KIND, PREC>(d, g, out, stats, rtab);
 }
 
-// Lattice kinds: the rarely taken wave-cooperative FP64 re-sums would otherwise set the
-// register budget of the whole kernel (about 140 VGPRs, 3 waves per SIMD); capped at 128
-// (4 waves per SIMD), only those cold paths spill.
+// Lattice kinds in one pass (sets with unpaired reflections, where every pixel takes an FP64
+// path): the rarely taken wave-cooperative FP64 re-sums would otherwise set the register
+// budget of the whole kernel (about 140 VGPRs, 3 waves per SIMD); capped at 128 (4 waves per
+// SIMD), only those cold paths spill.
 template <int KIND, int PREC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_image_lat(const DevSet d, const csx_grid g, double* __restrict__ out, unsigned long long* __restrict__ stats,
             const Radial* __restrict__ rtab) {
     image_tile<KIND, PREC>(d, g, out, stats, rtab);
+}
+
+// Lattice kinds, two passes.  The hot pass carries no FP64 re-sum code (no spills, more waves
+// per SIMD) and queues the pixels that need one; the fix pass finishes them one lane each,
+// with the same arithmetic as the one-pass kernel, so every pixel is bitwise the same.
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(kBlock)
+k_image_lat_hot(const DevSet d, const csx_grid g, double* __restrict__ out, unsigned long long* __restrict__ stats,
+                const Radial* __restrict__ rtab, ColdQueue cq) {
+    image_tile<KIND, PREC, kHot>(d, g, out, stats, rtab, cq);
+}
+
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(64)
+k_image_lat_fix(const DevSet d, const csx_grid g, double* __restrict__ out, unsigned long long* __restrict__ stats,
+                const Radial* __restrict__ rtab, ColdQueue cq) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cq.next_count = 0;
+    const int n = *cq.count;
+    const size_t W = (size_t)(g.xmax - g.xmin);
+    // one wave per slot (the FP64 re-sums are wave-cooperative); lanes without a cold pixel
+    // repeat the slot's first one as placeholders
+    for (int slot = blockIdx.x; slot < n; slot += gridDim.x) {
+        const unsigned long long m = cq.mask[slot];
+        const int lane = (int)threadIdx.x;
+        const bool have = (m >> lane) & 1ull;
+        ColdPix cp = cq.ent[(size_t)slot * 64 + (have ? lane : __builtin_ctzll(m))];
+        if (!have) cp.flags = 4;
+        PixQ P3{cp.ihex, cp.i, 0.0, 0.0, 0.0, false};
+        const Radial* rv = nullptr;
+        if (rtab) {
+            const int a = abs(cp.i), b = abs(cp.ihex);
+            rv = rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
+        }
+        const double v = pixel_value<KIND, PREC, kFix>(d, P3, stats, rv, nullptr, &cp);
+        if (have) out[(size_t)(cp.ihex - g.xmin) + W * (size_t)(cp.i - g.ymin)] = v;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_qgrid(const csx_scalars s, const csx_grid g, double* __restrict__ out) {
@@ -1527,22 +1832,24 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
     const double span = (double)qtop * 1.001 + 1e-3;
     ft.lut_inv_dq = (float)(kLutBins / span);
     const double dqb = 1.0 / (double)ft.lut_inv_dq;
-    ft.lut.assign((size_t)(kBands + 1) * (kLutBins + 1), 0);
+    ft.lut.assign((size_t)kSegments * (kLutBins + 1), 0);
     const PairRec inert{0.0f, 0.0f, 0.0f, 1.0e30f, 0.0f, 0.0f, 0, 0};
     const PairRec64 inert64{0.0, 0.0, 0.0, 0.0, 1.0e30f, 0, 0, 0.0f};
-    // Segments 0..kBands-1: one |u_z| band each; segment kBands: every pair.  One stable sort
-    // by |q_hkl|; each segment takes its pairs in that order.
+    // Segments 0..kBands-1: one |u_z| band each; segment kBands: every pair; segment
+    // kBands + 1: the pairs with |u_z| <= kPlaneUz.  One stable sort by |q_hkl|; each segment
+    // takes its pairs in that order.
     std::vector<int> order(found.size());
     for (size_t k = 0; k < found.size(); ++k) order[k] = (int)k;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return found[x].qh < found[y].qh; });
-    ft.pairs.reserve(2 * found.size() + 4 * (kBands + 1));
-    if (precise) ft.pairs64.reserve(2 * found.size() + 4 * (kBands + 1));
+    ft.pairs.reserve(3 * found.size() + 4 * kSegments);
+    if (precise) ft.pairs64.reserve(3 * found.size() + 4 * kSegments);
     std::vector<int> band;
     band.reserve(found.size());
-    for (int b = 0; b <= kBands; ++b) {
+    for (int b = 0; b < kSegments; ++b) {
         band.clear();
         for (int k : order)
-            if (b == kBands || band_of[k] == b) band.push_back(k);
+            if (b == kBands || (b < kBands && band_of[k] == b) || (b > kBands && std::fabs(found[k].uz) <= kPlaneUz))
+                band.push_back(k);
         const int start = (int)ft.pairs.size();  // a multiple of 4
         for (int k : band) {
             ft.pairs.push_back(found[k]);
@@ -1712,6 +2019,8 @@ struct csx_ctx {
     hipEvent_t ev_rad[2] = {nullptr, nullptr}, ev_img[2] = {nullptr, nullptr};
     bool img_recorded[2] = {false, false};
     int rslot = 0;
+    DevBuf coldq;        // fix-pass queue of the two-pass lattice render: two counts, then entries
+    int cold_epoch = 0;  // which count the next render uses
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
@@ -1830,10 +2139,20 @@ static void launch_radial(const DevSet& d, const csx_grid& g, Radial* rtab, hipS
     }
 }
 
+// Blocks of the fix pass (one wave each; the pass steps through its queue).
+constexpr int kFixBlocks = 256;
+
 template <int K, int P>
 static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
-                         const Radial* rtab, hipStream_t st) {
+                         const Radial* rtab, const ColdQueue* cq, hipStream_t st) {
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
+    if constexpr ((K == K_FCC || K == K_BCT) && P == CSX_PREC_MIXED) {
+        if (cq) {
+            hipLaunchKernelGGL((k_image_lat_hot<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab, *cq);
+            hipLaunchKernelGGL((k_image_lat_fix<K, P>), dim3(kFixBlocks), dim3(64), 0, st, d, g, out, stats, rtab, *cq);
+            return;
+        }
+    }
     if constexpr (K == K_FCC || K == K_BCT)
         hipLaunchKernelGGL((k_image_lat<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
     else
@@ -1966,7 +2285,27 @@ static int render_launch(csx_ctx* c, int kind, int precision, const csx_grid& g,
         HIPCHK(c, hipEventRecord(c->ev_rad[sl], c->side));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_rad[sl], 0));
     }
-    DISPATCH_KIND(kind, precision, launch_image, c->set, g, out, stats, rtab, c->stream);
+    // Lattice kinds in mixed precision render in two passes (hot, then fix over a queue of at
+    // most one slot per wave), unless unpaired reflections send every pixel to FP64.
+    ColdQueue cq{};
+    const ColdQueue* cqp = nullptr;
+    if ((kind == K_FCC || kind == K_BCT) && precision == CSX_PREC_MIXED && c->set.nsingle == 0) {
+        const size_t nwaves = 4 * (size_t)((g.xmax - g.xmin + kTile - 1) / kTile) *
+                              (size_t)((g.ymax - g.ymin + kTile - 1) / kTile);
+        const size_t need = 256 + nwaves * (sizeof(unsigned long long) + 64 * sizeof(ColdPix));
+        if (need > c->coldq.n) {
+            HIPCHK(c, c->coldq.ensure(need));
+            HIPCHK(c, hipMemsetAsync(c->coldq.p, 0, 256, c->stream));
+        }
+        int* counts = static_cast<int*>(c->coldq.p);
+        cq.count = counts + c->cold_epoch;
+        cq.next_count = counts + (c->cold_epoch ^ 1);
+        cq.mask = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->coldq.p) + 256);
+        cq.ent = reinterpret_cast<ColdPix*>(static_cast<char*>(c->coldq.p) + 256 + nwaves * sizeof(unsigned long long));
+        c->cold_epoch ^= 1;
+        cqp = &cq;
+    }
+    DISPATCH_KIND(kind, precision, launch_image, c->set, g, out, stats, rtab, cqp, c->stream);
     HIPCHK(c, hipGetLastError());
     if (rtab) {
         HIPCHK(c, hipEventRecord(c->ev_img[sl], c->stream));
@@ -2012,7 +2351,7 @@ void csx_destroy(csx_ctx* c) {
     c->bres.release();
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
-                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1]})
+                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1], &c->coldq})
         b->release();
     if (c->bgraph.exec) (void)hipGraphExecDestroy(c->bgraph.exec);
     if (c->bgraph.graph) (void)hipGraphDestroy(c->bgraph.graph);

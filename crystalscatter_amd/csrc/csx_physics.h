@@ -41,6 +41,11 @@ struct alignas(16) PairRec64 {
     float pad;
 };
 constexpr int kBands = 8;
+// One more segment follows the merged one: the pairs with |u_z| <= kPlaneUz.  For q in the
+// detector plane (q_z = 0), |q^.u| <= sqrt(1 - u_z^2) plus rounding, so only those pairs can
+// come near |q^.u| = 1.
+constexpr float kPlaneUz = 0.003f;
+constexpr int kSegments = kBands + 2;
 constexpr int kCqDelta = 64;  // polar nodes
 constexpr int kCqPsi = 64;    // azimuth nodes on [0, pi] (symmetric pairs: kCqPsi/2 values of t)
 constexpr int kCqTab = 4096;  // intervals of the rod-factor table
@@ -57,9 +62,9 @@ struct DevSet {
     const int32_t* latpar1;  // (peakmax1+1) x 6
     const int32_t* latpar2;  // (peakmax2+1) x 6
     const float* latpar3;    // (peakmax2+1) x 17
-    // Mixed-precision tables (built at upload, see csx_device.hip).  Friedel pairs in kBands+1
+    // Mixed-precision tables (built at upload, see csx_device.hip).  Friedel pairs in kSegments
     // segments: segment b < kBands holds the pairs with b/kBands <= |u_z| < (b+1)/kBands,
-    // segment kBands all pairs; each is sorted by |q_hkl|, starts at a multiple of 4 and is
+    // segment kBands all pairs, segment kBands + 1 those with |u_z| <= kPlaneUz; each is sorted by |q_hkl|, starts at a multiple of 4 and is
     // padded to one with inert records.  lut[b*(nlut+1)+k] is the index of the first pair of
     // segment b with |q_hkl| >= k/lut_inv_dq (the segment end for k = nlut).
     const struct PairRec* pairs;
