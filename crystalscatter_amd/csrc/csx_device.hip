@@ -1899,6 +1899,22 @@ void set_fast(DevSet& d, const FastTables& ft) {
     d.kr_const = -4.0 / (kPiD * width * width) * kLog2e;
     d.inv_hpw = 1.0 / (kPiD * width / 2.0);
     d.two_pi_over_v = 2 * kPiD / s.sphno_cubevol;
+    // sphere closed-form constants (pq_sphere_fast: zr nudged as the reference nudges it,
+    // after R/(zr+1) is formed; fq_sphere_fast: no nudge)
+    {
+        const double z0 = (1 - s.sigma * s.sigma) / (s.sigma * s.sigma);
+        double zr = z0;
+        if (zr * (zr - 1) * (zr - 2) * (zr - 3) * (zr - 4) * (zr - 5) == 0) zr += 0.001;
+        d.sph_zr = zr;
+        d.sph_arg = s.radius / (z0 + 1);
+        d.sph_c0 = 1 / (2.0 * zr * (zr - 1) * (zr - 2) * (zr - 3));
+        d.sph_c4 = 1 / (zr - 4);
+        d.sph_c45 = 1 / ((zr - 4) * (zr - 5));
+        d.sfq_zr = z0;
+        d.sfq_arg = s.radius / (z0 + 1);
+        d.sfq_c0 = 1 / (z0 * (z0 - 1));
+        d.sfq_c2 = 1 / (z0 - 2);
+    }
     d.npairs = (int)ft.pairs.size();
     d.nsingle = (int)ft.singles.size();
     d.niso = (int)ft.iso.size();

@@ -81,6 +81,10 @@ struct DevSet {
     double kr_const;         // -4/(pi*width^2)*log2(e) when the width does not depend on q
     double inv_hpw;          // 1/(pi*width/2) when the width does not depend on q
     double two_pi_over_v;    // 2*pi/sphno_cubevol
+    // Per-set constants of the mixed path's sphere closed forms (pq_sphere_fast,
+    // fq_sphere_fast): zr, R/(zr+1) and the reciprocal coefficient products.
+    double sph_zr, sph_arg, sph_c0, sph_c4, sph_c45;
+    double sfq_zr, sfq_arg, sfq_c0, sfq_c2;
     // Gamma(zr+1), Gamma(zr-1/2), Gamma(zr-3/2) of the transverse cylinder <F>^2 (zr from
     // sigma): per-set constants, evaluated once on the host with the reference's Gamma.
     double cyl_g0, cyl_g1, cyl_g2;
@@ -224,55 +228,54 @@ __device__ inline double fq_sphere(const DevSet& d, double q) {
     return pq3 * pq3;
 }
 
-// Same closed forms with fewer FP64 library calls, for the mixed-precision path: the three
-// powers of one base b come from one log and one exp (b^((zr-3)/2) = P*b, b^((zr-4)/2) =
-// P*sqrt(b), b^(zr-5) = P^2 with P = b^((zr-5)/2)), and the cosines at (zr-3)*at and
-// (zr-5)*at from the sine and cosine at (zr-4)*at by angle addition, with cos(at) =
-// 1/sqrt(b) and sin(at) = 2*argq/sqrt(b) for at = atan(2*argq).  Differences from the
-// reference's evaluation are rounding-level (phases agree to ~1e-14 rad).
+// Same closed forms with fewer FP64 library calls and one division, for the mixed-precision
+// path: the per-set coefficient products are reciprocals taken once (sph_*, sfq_*); the
+// powers of one base b come from one log (b^(-(zr-5)/2) and b^(-1/2) by exp), the cosines at
+// (zr-3)*at and (zr-5)*at from the sine and cosine at (zr-4)*at by angle addition, with
+// cos(at) = 1/sqrt(b) and sin(at) = 2*argq/sqrt(b) for at = atan(2*argq).  Differences from
+// the reference's evaluation are rounding-level (phases agree to ~1e-14 rad).
 __device__ inline double pq_sphere_fast(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
-    double zr = (1 - sqd(s.sigma)) / (sqd(s.sigma));
     if (q < 0.4 * s.limq[3]) return series_x2(carr_of(d, 3), q, 100);
-    const double argq = q * s.radius / (zr + 1);
-    if (zr * (zr - 1) * (zr - 2) * (zr - 3) * (zr - 4) * (zr - 5) == 0) zr += 0.001;
+    const double zr = d.sph_zr;
+    const double argq = q * d.sph_arg;
+    const double ia = 1.0 / argq, ia2 = ia * ia;
     const double a2 = argq * argq;
-    const double pqr = (1 / (2.0 * zr * (zr - 1) * (zr - 2) * (zr - 3))) / (a2 * a2);
+    const double pqr = d.sph_c0 * (ia2 * ia2);
     const double u = 2.0 * argq;
-    const double at = atan(u), b = 1.0 + 4 * a2;
-    const double sb = sqrt(b);
-    const double P = exp(0.5 * (zr - 5) * log(b));
+    const double at = atan(u), lb = log(1.0 + 4 * a2);
+    const double isb = exp(-0.5 * lb);               // b^(-1/2)
+    const double iP = exp(-0.5 * (zr - 5) * lb);     // b^(-(zr-5)/2)
     double s4, c4;
     sincos((zr - 4) * at, &s4, &c4);
-    const double ca = 1.0 / sb, sa = u / sb;
-    const double cos3 = c4 * ca - s4 * sa, cos5 = c4 * ca + s4 * sa;
-    const double pq1 = pqr * (1 + cos3 / (P * b));
-    const double pq2 = (pqr / ((zr - 4) * argq)) * s4 / (P * sb);
-    const double pq3 = (pqr / ((zr - 4) * (zr - 5) * a2)) * (1 - cos5 / (P * P) / 2.0);
+    const double sa = u * isb;
+    const double cos3 = c4 * isb - s4 * sa, cos5 = c4 * isb + s4 * sa;
+    const double pq1 = pqr * (1 + cos3 * (iP * (isb * isb)));
+    const double pq2 = pqr * (d.sph_c4 * ia) * s4 * (iP * isb);
+    const double pq3 = pqr * (d.sph_c45 * ia2) * (1 - cos5 * (iP * iP) / 2.0);
     return 9.0 * (pq1 - 2.0 * pq2 + pq3);
 }
 
-// fq_sphere with one log/exp for both powers and the cosine at (zr-1)*at from the sine
-// and cosine at (zr-2)*at (at = atan(argq), cos(at) = 1/sqrt(b), sin(at) = argq/sqrt(b)).
+// fq_sphere with one division, one log and the cosine at (zr-1)*at from the sine and cosine
+// at (zr-2)*at (at = atan(argq), cos(at) = 1/sqrt(b), sin(at) = argq/sqrt(b)).
 __device__ inline double fq_sphere_fast(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
-    const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
     if (q < (0.4 * s.limq[12])) return series_x2(carr_of(d, 12), q, 100);
-    const double argq = q * s.radius / (zr + 1);
-    const double a2 = argq * argq;
-    const double pqr = (1 / (zr * (zr - 1))) / a2;
-    const double at = atan(argq), b = 1.0 + a2;
-    const double sb = sqrt(b);
-    const double P = exp(0.5 * (zr - 2) * log(b));  // b^((zr-2)/2)
+    const double zr = d.sfq_zr;
+    const double argq = q * d.sfq_arg;
+    const double ia = 1.0 / argq;
+    const double pqr = d.sfq_c0 * (ia * ia);
+    const double at = atan(argq), lb = log(1.0 + argq * argq);
+    const double isb = exp(-0.5 * lb);            // b^(-1/2)
+    const double iP = exp(-0.5 * (zr - 2) * lb);  // b^(-(zr-2)/2)
     double s2, c2;
     sincos((zr - 2) * at, &s2, &c2);
-    const double cos1 = c2 / sb - s2 * (argq / sb);
-    const double pq1 = pqr * cos1 / (P * sb);
-    const double pq2 = (pqr / ((zr - 2) * argq)) * s2 / P;
+    const double cos1 = c2 * isb - s2 * (argq * isb);
+    const double pq1 = pqr * cos1 * (iP * isb);
+    const double pq2 = pqr * (d.sfq_c2 * ia) * s2 * iP;
     const double pq3 = 3 * (pq2 - pq1);
     return pq3 * pq3;
 }
-
 
 // Schulz-averaged longitudinal P of a rod, argument (qxs+qys+eps9)*L/(zl+1).
 __device__ __forceinline__ double rod_p(double zl, double arg) {
