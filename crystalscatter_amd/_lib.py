@@ -52,6 +52,7 @@ class Mask(ctypes.Structure):
 
 NM_MAXPAR = 30
 TUNE_VARIANTS, TUNE_SIZES = 4, 4
+PASS_RING = 64  # CSX_PASS_RING
 NM_REFERENCE, NM_CORRECTED = 0, 1
 OBJECTIVE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
 
@@ -123,6 +124,7 @@ SIGNATURES = [
     ("csx_last_error", _I, [_VP, _CP, _I]),
     ("csx_last_kernel_ms", _D, [_VP]),
     ("csx_set_timing", _I, [_VP, _I]),
+    ("csx_pass_times", _I, [_VP, _I, _PD]),
     ("csx_engine_new", _VP, [_I]),
     ("csx_engine_free", None, [_VP]),
     ("csx_engine_set", _I, [_VP, _CP, _D]),

@@ -2037,6 +2037,8 @@ struct csx_ctx {
     int rslot = 0;
     DevBuf coldq;        // fix-pass queue of the two-pass lattice render: two counts, then entries
     int cold_epoch = 0;  // which count the next render uses
+    hipEvent_t pass_ev[CSX_PASS_RING][2] = {};  // image-pass timing ring (timing on)
+    long long pass_n = 0;                        // renders recorded into the ring
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
@@ -2158,13 +2160,16 @@ static void launch_radial(const DevSet& d, const csx_grid& g, Radial* rtab, hipS
 // Blocks of the fix pass (one wave each; the pass steps through its queue).
 constexpr int kFixBlocks = 256;
 
+// ev: null, or two events recorded around the per-pixel kernel (the hot pass when two).
 template <int K, int P>
 static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsigned long long* stats,
-                         const Radial* rtab, const ColdQueue* cq, hipStream_t st) {
+                         const Radial* rtab, const ColdQueue* cq, hipEvent_t* ev, hipStream_t st) {
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
+    if (ev) (void)hipEventRecord(ev[0], st);
     if constexpr ((K == K_FCC || K == K_BCT) && P == CSX_PREC_MIXED) {
         if (cq) {
             hipLaunchKernelGGL((k_image_lat_hot<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab, *cq);
+            if (ev) (void)hipEventRecord(ev[1], st);
             hipLaunchKernelGGL((k_image_lat_fix<K, P>), dim3(kFixBlocks), dim3(64), 0, st, d, g, out, stats, rtab, *cq);
             return;
         }
@@ -2173,6 +2178,7 @@ static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsign
         hipLaunchKernelGGL((k_image_lat<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
     else
         hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
+    if (ev) (void)hipEventRecord(ev[1], st);
 }
 
 int csx_qgrid(csx_ctx* c, const csx_grid* g, double* d_q3) {
@@ -2321,7 +2327,15 @@ static int render_launch(csx_ctx* c, int kind, int precision, const csx_grid& g,
         c->cold_epoch ^= 1;
         cqp = &cq;
     }
-    DISPATCH_KIND(kind, precision, launch_image, c->set, g, out, stats, rtab, cqp, c->stream);
+    hipEvent_t* ev = nullptr;
+    if (c->timing) {
+        hipEvent_t* e = c->pass_ev[c->pass_n % CSX_PASS_RING];
+        for (int k = 0; k < 2; ++k)
+            if (!e[k]) HIPCHK(c, hipEventCreate(&e[k]));
+        ev = e;
+        ++c->pass_n;
+    }
+    DISPATCH_KIND(kind, precision, launch_image, c->set, g, out, stats, rtab, cqp, ev, c->stream);
     HIPCHK(c, hipGetLastError());
     if (rtab) {
         HIPCHK(c, hipEventRecord(c->ev_img[sl], c->stream));
@@ -2373,6 +2387,9 @@ void csx_destroy(csx_ctx* c) {
     if (c->bgraph.graph) (void)hipGraphDestroy(c->bgraph.graph);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_rad[0], c->ev_rad[1], c->ev_img[0], c->ev_img[1]})
         if (e) (void)hipEventDestroy(e);
+    for (auto& pe : c->pass_ev)
+        for (hipEvent_t e : pe)
+            if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2443,7 +2460,23 @@ int csx_tune_force(csx_ctx* c, int variant) {
 int csx_set_timing(csx_ctx* c, int enable) {
     if (!c) return CSX_E_ARG;
     c->timing = enable != 0;
+    c->pass_n = 0;
     return CSX_OK;
+}
+
+int csx_pass_times(csx_ctx* c, int max, double* ms) {
+    if (!c || max < 0 || (max > 0 && !ms)) return CSX_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long have = std::min<long long>(c->pass_n, CSX_PASS_RING);
+    const int n = (int)std::min<long long>(have, max);
+    for (int k = 0; k < n; ++k) {
+        hipEvent_t* e = c->pass_ev[(c->pass_n - n + k) % CSX_PASS_RING];
+        HIPCHK(c, hipEventSynchronize(e[1]));
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, e[0], e[1]));
+        ms[k] = t;
+    }
+    return n;
 }
 
 double csx_last_kernel_ms(csx_ctx* c) {

@@ -296,6 +296,14 @@ double csx_last_kernel_ms(csx_ctx* ctx);
 /* Bracket every csx_render launch with HIP events (default off: the event pair keeps
  * back-to-back launches from overlapping, about 10 us per 1024^2 render). */
 int csx_set_timing(csx_ctx* ctx, int enable);
+/* Durations (ms) of the image pass of the last renders made with timing on, oldest first:
+ * the per-pixel kernel alone (for the lattice kinds in mixed precision, the hot pass; its
+ * radial pre-pass and fix pass are not included), from HIP events on the context's stream
+ * recorded around that launch, up to the last CSX_PASS_RING renders; waits for them.
+ * Returns the number written (at most max).  No reference counterpart: the per-kernel time
+ * that the roofline accounting divides by. */
+#define CSX_PASS_RING 64
+int csx_pass_times(csx_ctx* ctx, int max, double* ms);
 
 /* ---- SasCalc_GENERIC_calculation facade ------------------------------------------- */
 typedef struct csx_engine csx_engine;
