@@ -168,6 +168,7 @@ SIGNATURES = [
     ("csx_mlp_get_features", _I, [_VP, _PF, _PI32, _PF]),
     ("csx_mlp_set_features", _I, [_VP, _PF, _PI32, _PF]),
     ("csx_mlp_forward", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP]),
+    ("csx_mlp_predict", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP]),
     ("csx_mlp_gradient", _I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _PF]),
     ("csx_mlp_get_grads", _I, [_VP, _PF]),
     ("csx_mlp_adam", _I, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),

@@ -277,6 +277,109 @@ __global__ void __launch_bounds__(256) k_head_fwd(int mpad, const bf16* __restri
     if (lane == 0) y[m] = s + b[0];
 }
 
+// ---- inference in one pass -------------------------------------------------------------
+// Each block takes kPM rows through the embedding, the kL hidden layers and the output layer
+// with the rows' activations in one LDS tile (kPM x kH bf16, row stride kPLd): a layer's
+// products are accumulated in registers, the block synchronises, and the epilogue writes the
+// layer's output over its input.  The weights are MFMA operands read straight from the bf16
+// [out][in] copies (L2-resident).  Fragment mapping, k order (32 at a time), epilogue rounding
+// and the output layer's reduction are those of k_embed, k_gemm_nt<EPI_FWD> and k_head_fwd,
+// so y is bitwise the forward pass's; no per-row buffers, so any row count.
+constexpr int kPM = 64;        // rows per block
+constexpr int kPLd = kH + 8;   // LDS row stride (bf16): 16-byte aligned rows
+__global__ void __launch_bounds__(256) k_mlp_infer(int n, int nsets, const float* __restrict__ par,
+                                                   const int32_t* __restrict__ set, const float* __restrict__ q,
+                                                   const Features ft, const bf16* __restrict__ wb,
+                                                   const float* __restrict__ p, float* __restrict__ y) {
+    __shared__ __attribute__((aligned(16))) bf16 act[kPM * kPLd];
+    const int r0 = blockIdx.x * kPM;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    {
+        // embedding, four threads per row (k_embed's values, written to LDS one by one)
+        const int row = t >> 2, h = t & 3;
+        const int i = r0 + row;
+        bf16* dst = act + row * kPLd;
+        float v[kNP];
+#pragma unroll
+        for (int k = 0; k < kNP; ++k) v[k] = 0.0f;
+        const bool live = i < n;
+        float qx = 0.0f, qy = 0.0f;
+        if (live) {
+            const float* pp = par + (size_t)min(max(set[i], 0), nsets - 1) * kNP;  // clamped like k_embed
+#pragma unroll
+            for (int k = 0; k < kNP; ++k) v[k] = pp[k];
+            qx = q[2 * (size_t)i];
+            qy = q[2 * (size_t)i + 1];
+        }
+        const float qa = sqrtf(qx * qx + qy * qy);
+        if (h == 0)
+#pragma unroll
+            for (int k = 0; k < kNP; ++k) dst[k] = (bf16)v[k];
+#pragma unroll
+        for (int f0 = 0; f0 < kNF / 4; ++f0) {
+            const int f = h * (kNF / 4) + f0;
+            float sn = 0.0f, cs = 0.0f;
+            if (live) {
+                const int j = ft.pidx[f];
+                float pv = 1.0f;
+#pragma unroll
+                for (int k = 0; k < kNP; ++k) pv = (k == j) ? ft.off[k] + ft.sc[k] * v[k] : pv;
+                sincosf(6.283185307f * (ft.f[f][0] * qx + ft.f[f][1] * qy) + ft.f[f][2] * qa * pv, &sn, &cs);
+            }
+            dst[kNP + f] = (bf16)sn;
+            dst[kNP + kNF + f] = (bf16)cs;
+        }
+        for (int k = kUsed + h; k < kIn; k += 4) dst[k] = (bf16)0.0f;
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const int wn = wave * 64;  // this wave's 64 output columns
+    for (int l = 0; l < kL; ++l) {
+        const int K = in_of(l);
+        const bf16* W = wb + (l == 0 ? 0 : (int64_t)kH * kIn + (int64_t)(l - 1) * kH * kH);
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < K; ks += 32) {
+            bf16x8 af[4], bfg[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bfg[j] = *(const bf16x8*)&W[(size_t)(wn + j * 16 + fr) * K + ks + fk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8*)&act[(i * 16 + fr) * kPLd + ks + fk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfg[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();  // every wave has read the layer's input
+        const float* b = p + b_off(l);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float bj = b[wn + j * 16 + fr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float z = acc[i][j][r] + bj;
+                    act[(i * 16 + (lane >> 4) * 4 + r) * kPLd + wn + j * 16 + fr] = (bf16)(z * sigm(z));
+                }
+        }
+        __syncthreads();
+    }
+    const float* w = p + kOutW;
+    for (int rr = wave; rr < kPM; rr += 4) {
+        const bf16* row = act + rr * kPLd + lane * 4;
+        float sum = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sum = fmaf((float)row[c], w[lane * 4 + c], sum);
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        if (lane == 0 && r0 + rr < n) y[r0 + rr] = sum + p[kOutB];
+    }
+}
+
 // Loss 1/n sum (y - t)^2 and the output layer's backward for a block of kHeadRows rows:
 // dy = 2 (y - t) / n (0 for rows >= n) into dy_out, dZ4[m][c] = dy * w[c] * silu'(Z4[m][c])
 // written in both layouts, and per-block partials of db = sum dy and of the loss.
@@ -700,6 +803,16 @@ int csx_mlp_forward(csx_mlp* m, int n, int nsets, const float* d_par, const int3
     rc = forward(m, n, nsets, d_par, d_set, d_q);
     if (rc) return rc;
     MCHK(m, hipMemcpyAsync(d_y, m->y, (size_t)n * 4, hipMemcpyDeviceToDevice, m->stream));
+    return CSX_OK;
+}
+
+int csx_mlp_predict(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                    float* d_y) {
+    if (!m || !d_par || !d_set || !d_q || !d_y || n < 1 || nsets < 1) return CSX_E_ARG;
+    MCHK(m, hipSetDevice(m->device));
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + kPM - 1) / kPM)), dim3(256), 0, m->stream, n, nsets, d_par,
+                       d_set, d_q, m->feat, m->wb, m->p, d_y);
+    MCHK(m, hipGetLastError());
     return CSX_OK;
 }
 

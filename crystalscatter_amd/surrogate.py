@@ -112,6 +112,25 @@ class MLP:
         torch.cuda.current_stream(self.device).wait_stream(self._stream)
         return out
 
+    def predict(self, par, sets, q, out=None):
+        """forward() bit for bit from the fused inference kernel, any number of rows."""
+        import torch
+        n = int(q.shape[0])
+        if n < 1:
+            raise ValueError("at least one row")
+        for t, dt in ((par, "float32"), (sets, "int32"), (q, "float32")):
+            if t.device != self.device or str(t.dtype) != "torch." + dt or not t.is_contiguous():
+                raise ValueError("inputs must be contiguous %s tensors on %s" % (dt, self.device))
+        if par.dim() != 2 or par.shape[1] != self.NPAR or q.shape[-1] != 2 or sets.shape[0] != n:
+            raise ValueError("par is (nsets, 6), sets (n,), q (n, 2)")
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=self.device)
+        self._stream.wait_stream(torch.cuda.current_stream(self.device))
+        self._in_use(par, sets, q, out)
+        self._chk(self._L.csx_mlp_predict(self._h, n, int(par.shape[0]), _p(par), _p(sets), _p(q), _p(out)), "predict")
+        torch.cuda.current_stream(self.device).wait_stream(self._stream)
+        return out
+
     def _target(self, target, n):
         if (target.device != self.device or str(target.dtype) != "torch.float32" or not target.is_contiguous()
                 or target.numel() != n):
@@ -302,14 +321,8 @@ def train(net, data, steps, batch=65536, lr=2e-3, lr_end=1e-4, seed=0, log_every
 
 
 def predict_log10(net, data_par, sets, q, mu, sigma):
-    """log10 I predicted for rows (device tensors), in chunks of the network's batch."""
-    import torch
-    n = int(q.shape[0])
-    out = torch.empty(n, dtype=torch.float32, device="cuda")
-    for a in range(0, n, net.max_batch):
-        b = min(n, a + net.max_batch)
-        net.forward(data_par, sets[a:b].contiguous(), q[a:b].contiguous(), out=out[a:b])
-    return out * sigma + mu
+    """log10 I predicted for rows (device tensors), in one fused inference launch."""
+    return net.predict(data_par, sets.contiguous(), q.contiguous()) * sigma + mu
 
 
 class FitPixels:

@@ -452,6 +452,11 @@ int csx_mlp_set_features(csx_mlp* m, const float* h_freq, const int32_t* h_pidx,
 /* d_y[i] = network(row i), n rows (asynchronous). */
 int csx_mlp_forward(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
                     float* d_y);
+/* The same d_y as csx_mlp_forward, bit for bit, from one fused kernel that keeps each block
+ * of rows' activations on chip across the layers (inference only; any n >= 1, no max_batch
+ * limit; asynchronous). */
+int csx_mlp_predict(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,
+                    float* d_y);
 /* Gradient of the loss 1/n sum_i (y_i - d_t[i])^2 into the model's gradient buffer;
  * *h_loss (optional, synchronous) = the loss. */
 int csx_mlp_gradient(csx_mlp* m, int n, int nsets, const float* d_par, const int32_t* d_set, const float* d_q,

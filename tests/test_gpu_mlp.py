@@ -49,6 +49,24 @@ def test_forward_matches_oracle(n, physics):
     assert np.sqrt(np.mean((y - y64) ** 2)) <= 2e-2 * scale
 
 
+@pytest.mark.parametrize("n,physics", [(1, False), (1000, True), (4096, False), (70001, True)])
+def test_fused_inference_equals_forward(n, physics):
+    """csx_mlp_predict (one fused kernel, activations on chip) gives forward's y bit for bit,
+    also beyond max_batch rows (forward then runs in chunks) and after a training step."""
+    import torch
+    net = _net(physics=physics)
+    (_, _, _, _), (dpar, dsets, dq, dt) = _inputs(n, seed=3)
+    for step in range(2):
+        y = torch.empty(n, dtype=torch.float32, device="cuda")
+        for a in range(0, n, net.max_batch):
+            b = min(n, a + net.max_batch)
+            net.forward(dpar, dsets[a:b].contiguous(), dq[a:b].contiguous(), out=y[a:b])
+        yp = net.predict(dpar, dsets, dq)
+        assert torch.equal(y, yp), (n, step, float((y - yp).abs().max()))
+        m = min(n, net.max_batch)
+        net.train_step(dpar, dsets[:m].contiguous(), dq[:m].contiguous(), dt[:m].contiguous(), 1e-3)
+
+
 def test_gradient_matches_oracle():
     n = 1500
     net = _net(physics=True)
