@@ -234,6 +234,9 @@ def run_surrogate(args, probs=None):
     cands, sval = sg.propose_starts(net, fpx, data.mu, data.sigma, [centre] * len(probs), rng, top=args.rescore)
     torch.cuda.synchronize()
     t_prop = time.perf_counter() - t0
+    # network rows of the proposal: x_init + 128 global draws, then 4 best + 4 x 32 local
+    # draws per fit (propose_starts' defaults), surrogate_px pixels each
+    prop_rows = len(probs) * (1 + 128 + 4 + 4 * 32) * args.surrogate_px
     # the exact objective scores the surrogate's best candidates, in one batched launch
     t0 = time.perf_counter()
     exact = nm_eval(ctx, fits, FIT_KEYS, cands, precision=args.precision, threads=args.cpu_threads)
@@ -282,6 +285,10 @@ def run_surrogate(args, probs=None):
         "train_loss_history": hist, "training_rows": len(data), "training_rows_per_s": rows_per_s,
         "training_s": t_train, "data_generation_s": t_data, "proposal_s": t_prop, "exact_rescore_s": t_rescore,
         "rescored_candidates_per_fit": args.rescore, "surrogate_px_per_fit": args.surrogate_px,
+        "proposal_network_rows": prop_rows,
+        "proposal_inference_tflops": prop_rows * MLP_FWD_FLOP / t_prop / 1e12,
+        "proposal_inference": "fused inference kernel (csx_mlp_predict); the rate includes the pixel "
+                              "subsampling and candidate draws on the host",
         "start_chi2_median": float(np.median(exact.min(axis=1))),
         "roofline": {"bound": "mfma", "achieved": rows_per_s * MLP_TRAIN_FLOP / 1e12, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": rows_per_s * MLP_TRAIN_FLOP / 1e12 / PEAK_BF16_TFLOPS,
@@ -363,7 +370,8 @@ def bench_aidata(args):
     from crystalscatter_amd import Context, Random, SasCalc, _lib, aidata
     L = _lib.lib()
     n = args.nimages
-    p = dict(CFG3, GridPoints=64, BeamPosX=0, BeamPosY=0)
+    bx, by = (int(v) for v in args.beam.split(","))
+    p = dict(CFG3, GridPoints=64, BeamPosX=bx, BeamPosY=by)
     opts = aidata.AIOptions(noise=True, bs=(0.1, 0.1, 0.5, 0.5), rphi_size=128, fft_size=128, fft_type=2,
                             fft_swap=True, fft_scale=True)
     ctx = Context(0)
@@ -399,7 +407,7 @@ def bench_aidata(args):
     t_seq = time.perf_counter() - t2
     ref_last = imgs[n - 1]
     single_vs_batch = float(((buf - ref_last).abs() / ref_last.abs().clamp_min(1e-300)).max())
-    cpu = None
+    cpu = check = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # checker only: the CPU baseline leg
@@ -409,6 +417,7 @@ def bench_aidata(args):
         e2.set("ignore_cbs", 1)
         rng2 = Random(1)
         keys = ["EditRadius", "EditSigma", "uca", "ucb", "ucc", "I0", "width_zuf", "dw_zuf"]
+        res = []
         tc = time.perf_counter()
         for k in range(m):
             e2.tpvPerformRandom(AI_IDS, rng2)
@@ -416,8 +425,26 @@ def bench_aidata(args):
             q.update({key: e2.tpv_value(key) for key in keys})
             o = oracle.OracleEngine(q)
             o.prepare()
-            AO.post_image(o.image(nthreads=args.cpu_threads), o.bounds(), (0, 0), rng2, opts)
+            res.append(AO.post_image(o.image(nthreads=args.cpu_threads), o.bounds(), (bx, by), rng2, opts))
         tcpu = time.perf_counter() - tc
+        # the timed call's first images against the same CPU sequence (scattering image after
+        # noise and beam stop, r,phi and FFT images)
+        sc = out["scat"][:m].cpu().numpy()
+        rp = out["rphi"][:m].cpu().numpy()
+        ff = out["fft"][:m].cpu().numpy()
+        e_sc = e_sp = e_rp = e_ff = 0.0
+        for k in range(min(m, n)):
+            img = res[k][0]
+            ok = np.isfinite(img) & (img != 0)
+            e_sc = max(e_sc, float((np.abs(sc[k][ok] - img[ok]) / np.abs(img[ok])).max()))
+            e_sp = max(e_sp, float(np.abs(sc[k][ok] - img[ok]).max() / np.abs(img[ok]).max()))
+            e_rp = max(e_rp, float(np.abs(rp[k] - res[k][2]).max() / max(1e-300, np.abs(res[k][2]).max())))
+            e_ff = max(e_ff, float(np.abs(ff[k] - res[k][3]).max() / max(1e-300, np.abs(res[k][3]).max())))
+        check = {"images": min(m, n), "scat_max_rel": e_sc, "scat_max_rel_to_peak": e_sp,
+                 "rphi_max_rel_to_peak": e_rp, "fft_max_rel_to_peak": e_ff,
+                 "note": "per-pixel relative errors of the noisy image include pixels that the noise "
+                         "draws close to zero",
+                 "against": "oracle/aidata_oracle.post_image on the oracle image, same rand() sequence"}
         cpu = {"value": m / tcpu, "unit": "images/s", "cores": args.cpu_threads, "kind": "port",
                "sample": "%d images of the same workload: FP64 oracle image (%d threads) and the CPU "
                          "post-processing restatement, %.2f s" % (m, args.cpu_threads, tcpu)}
@@ -426,10 +453,11 @@ def bench_aidata(args):
         "value": n / dt, "unit": "images/s", "n_gpus": 1, "steps": n, "warmup": n,
         "ms_per_step": 1e3 * dt / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.precision, "data": "synthetic",
-        "config": {"workload": "%d images, FCC spheres (key 0x00020500) 128x128, tpv %s, noise, beam stop, "
-                               "r,phi 128x128, FFT modulus 128x128" % (n, ";".join(AI_IDS))},
+        "config": {"workload": "%d images, FCC spheres (key 0x00020500) 128x128, beam at (%d, %d), tpv %s, noise, "
+                               "beam stop, r,phi 128x128, FFT modulus 128x128" % (n, bx, by, ";".join(AI_IDS))},
         "render_batch_images_per_s": n / t_batch, "render_batch_kernel_ms": k_batch_ms,
         "one_launch_per_image_images_per_s": n / t_seq, "single_vs_batched_max_rel": single_vs_batch,
+        "beam": [bx, by], "oracle_check": check,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -555,6 +583,7 @@ def main():
                          "bct: the BCT-cylinder kernel on random fit parameter sets")
     ap.add_argument("--nfits", type=int, default=512)
     ap.add_argument("--nimages", type=int, default=1024, help="images of the aidata workload")
+    ap.add_argument("--beam", default="0,0", help="aidata workload: BeamPosX,BeamPosY in pixels")
     ap.add_argument("--maxit", type=int, default=100)
     ap.add_argument("--maxit-surrogate", type=int, default=0,
                     help="exact simplex iterations after a surrogate start (0: the best re-scored candidate)")

@@ -101,6 +101,8 @@ def generate(ctx, engine, ids, n, rng, opts=None, precision="mixed", threads=16)
     for one call at a time)."""
     import torch
     opts = opts or AIOptions()
+    if not 1 <= n <= 65535:
+        raise ValueError("1..65535 images per call (the post-processing batch limit)")
     if opts.fft_size and opts.fft_size != opts.rphi_size:
         raise ValueError("the FFT image needs fft_size == rphi_size (the r,phi image is its input)")
     L = _lib.lib()

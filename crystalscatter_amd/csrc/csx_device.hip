@@ -1241,41 +1241,48 @@ __device__ __forceinline__ int tri_index(int a, int b) { return b * (b + 1) / 2 
 // Radial pre-pass (midpoint q grid): in that grid |q| of pixel (ihex, i) depends only on
 // {|i|, |ihex|} and is bitwise the same for all eight sign/swap images, so |q|, P(q), <F>^2
 // and the Debye-Waller factor are evaluated once per unordered pair and shared.  Entry
-// (a, b) uses pixel (ihex=b, i=a).  The grid has two halves of `half` threads each (a
-// multiple of the wave size, so no wave mixes them): the first evaluates q, P(q) and dwe,
-// the second <F>^2 -- independent FP64 chains, twice the waves in flight.
+// (a, b) uses pixel (ihex=b, i=a).  A block takes 128 entries with two halves of 128 threads
+// (two waves each, so no wave mixes them): the first evaluates q, P(q) and dwe, the second
+// <F>^2 -- independent FP64 chains, twice the waves in flight.  The halves meet in LDS and
+// the block stores whole 32-byte entries, 16 bytes per thread.
 template <int KIND, int PREC>
-__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, int half, Radial* __restrict__ tab) {
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    const bool second = g >= half;
-    const int n = second ? g - half : g;
+__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, Radial* __restrict__ tab) {
+    __shared__ double stage[128][4];
+    const int k = threadIdx.x & 127;
+    const bool second = threadIdx.x >= 128;
+    const int n = blockIdx.x * 128 + k;
     const int total = (m + 1) * (m + 2) / 2;
-    if (n >= total) return;
-    int b = (int)((sqrt(8.0 * n + 1.0) - 1.0) * 0.5);
-    while (b * (b + 1) / 2 > n) --b;
-    while ((b + 1) * (b + 2) / 2 <= n) ++b;
-    const int a = n - b * (b + 1) / 2;
-    double qx, qy, qz;
-    pixel_q(d.s, b, a, qx, qy, qz);
-    const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
-    double* e = reinterpret_cast<double*>(tab + n);
-    if (!second) {
-        double pq;
-        if constexpr (KIND == K_CUBE) pq = pq_cube(d, q);
-        else if constexpr (KIND == K_CYL_ISO) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
-        else if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
-        else pq = pq_sphere_fast(d, q);
-        e[0] = q;
-        e[1] = pq;
-        e[3] = exp(-d.s.dwfactor * q * q);
-    } else {
-        double fq = 0.0;
-        if constexpr (KIND == K_FCC) {
-            if constexpr (PREC == CSX_PREC_REF64) fq = fq_sphere(d, q);
-            else fq = fq_sphere_fast(d, q);
+    if (n < total) {
+        int b = (int)((sqrt(8.0 * n + 1.0) - 1.0) * 0.5);
+        while (b * (b + 1) / 2 > n) --b;
+        while ((b + 1) * (b + 2) / 2 <= n) ++b;
+        const int a = n - b * (b + 1) / 2;
+        double qx, qy, qz;
+        pixel_q(d.s, b, a, qx, qy, qz);
+        const double q = sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
+        if (!second) {
+            double pq;
+            if constexpr (KIND == K_CUBE) pq = pq_cube(d, q);
+            else if constexpr (KIND == K_CYL_ISO) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+            else if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
+            else pq = pq_sphere_fast(d, q);
+            stage[k][0] = q;
+            stage[k][1] = pq;
+            stage[k][3] = exp(-d.s.dwfactor * q * q);
+        } else {
+            double fq = 0.0;
+            if constexpr (KIND == K_FCC) {
+                if constexpr (PREC == CSX_PREC_REF64) fq = fq_sphere(d, q);
+                else fq = fq_sphere_fast(d, q);
+            }
+            stage[k][2] = fq;
         }
-        e[2] = fq;
     }
+    __syncthreads();
+    const int e = threadIdx.x >> 1, h = threadIdx.x & 1;
+    const int ne = blockIdx.x * 128 + e;
+    if (ne < total)
+        reinterpret_cast<double2*>(tab + ne)[h] = make_double2(stage[e][2 * h], stage[e][2 * h + 1]);
 }
 
 // Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.  rtab: radial
@@ -2152,8 +2159,7 @@ static void launch_radial(const DevSet& d, const csx_grid& g, Radial* rtab, hipS
         const int m = std::max(std::max(std::abs(g.xmin), std::abs(g.xmax - 1)),
                                std::max(std::abs(g.ymin), std::abs(g.ymax - 1)));
         const int total = (m + 1) * (m + 2) / 2;
-        const int half = (total + 255) / 256 * 256;
-        hipLaunchKernelGGL((k_radial<K, P>), dim3(2 * half / 256), dim3(256), 0, st, d, m, half, rtab);
+        hipLaunchKernelGGL((k_radial<K, P>), dim3((total + 127) / 128), dim3(256), 0, st, d, m, rtab);
     }
 }
 

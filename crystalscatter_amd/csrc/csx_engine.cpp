@@ -4,6 +4,7 @@
 // sc_simplexfit2d (Source/sc_calc_generic_gpu.h:108-598): parameters by key,
 // doCalculation, setFitData / setNoFitRect / doFitCalculation, prepared tables.
 // Host preparation runs here in C++; every pixel runs on the GPU.
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
@@ -397,11 +398,27 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
                          int64_t stride, double* d_noise, int nextra, double* h_extra, int threads) {
     if (!e || !ctx || !ids || !*ids || !rng || n <= 0 || !d_out || nextra < 0 || (nextra && !h_extra))
         return CSX_E_ARG;
+    // the post-processing calls take at most 65535 images per batch (grid y)
+    if (n > 65535) { e->err = "at most 65535 images per batch"; return CSX_E_ARG; }
     // the image size of the parameters as set (the random variation keeps the grid)
     int rc = e->prepared ? CSX_OK : engine_prepare_host(e);
     if (rc != CSX_OK) return rc;
     const int64_t npx = (int64_t)(e->P.s.xmax - e->P.s.xmin) * (e->P.s.ymax - e->P.s.ymin);
     if (npx > stride) { e->err = "image larger than its output stride"; return CSX_E_ARG; }
+    // On any failure below the generator and the random-variation state are put back as they
+    // were, so that a retry draws the reference's sequence from the same point.
+    const csr::State s0 = rng->s;
+    const csx::ParamSet pt0 = e->pt;
+    const bool on0 = e->tpv_on;
+    double old0[kNTpv];
+    std::copy(e->tpv_old, e->tpv_old + kNTpv, old0);
+    auto restore = [&](int code) {
+        rng->s = s0;
+        e->pt = pt0;
+        e->tpv_on = on0;
+        std::copy(old0, old0 + kNTpv, e->tpv_old);
+        return code;
+    };
     // draws in the AI-data loop's order, image after image: variation, noise, then the extra
     // values (beam stop, lines) of the caller's post-processing.  The noise is jumped over
     // here, its start state kept, and drawn on the device.
@@ -413,7 +430,7 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
         csr::jump_rows((uint64_t)npx, rows.data());
     }
     for (int k = 0; k < n; ++k) {
-        if ((rc = csx_engine_tpv(e, ids, rng, nullptr, 0)) != CSX_OK) return rc;
+        if ((rc = csx_engine_tpv(e, ids, rng, nullptr, 0)) != CSX_OK) return restore(rc);
         sets[(size_t)k] = e->pt;
         if (d_noise) {
             csr::window(rng->s, &wins[(size_t)k * csr::kDeg]);
@@ -449,7 +466,7 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
     work();
     for (auto& th : pool) th.join();
     for (int k = 0; k < n; ++k)
-        if (!errs[(size_t)k].empty()) { e->err = errs[(size_t)k]; return CSX_E_PARAM; }
+        if (!errs[(size_t)k].empty()) { e->err = errs[(size_t)k]; return restore(CSX_E_PARAM); }
     std::vector<const csx_staged*> built((size_t)n);
     for (int k = 0; k < n; ++k) built[(size_t)k] = st[(size_t)k].get();
     rc = csx_stage_commit_many(ctx, n, built.data(), nt);
@@ -458,6 +475,7 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
         char b[256];
         csx_last_error(ctx, b, sizeof b);
         e->err = b;
+        return restore(rc);
     }
     return rc;
 }

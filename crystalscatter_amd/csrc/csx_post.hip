@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "csx.h"
@@ -557,59 +558,69 @@ int csx_post_rphi_many(csx_ctx* c, const double* d_img, int x0, int x1, int y0, 
     auto sq = [](double v) { return v * v; };
     // the float radius loop and the cell each (step, row) writes, as the reference computes
     // them (a later write of a cell replaces an earlier one); it depends on irmax only, so one
-    // map serves every image with the same irmax
-    std::vector<int> writers, irmax_of;
-    std::vector<float> irss;
-    std::vector<int64_t> woff;
-    std::vector<int32_t> ioff;
-    std::vector<RphiImg> per((size_t)nimg);
-    for (int k = 0; k < nimg; ++k) {
-        const int xc = h_centres[2 * k] + dx, yc = h_centres[2 * k + 1] + dy;
-        double rmax = std::sqrt(sq(1. - xc) + sq(1. - yc));
-        rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq(1. - yc)));
-        rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq((sy - 1.) - yc)));
-        rmax = std::max(rmax, std::sqrt(sq(1. - xc) + sq((sy - 1.) - yc)));
-        const int irmax = (int)(rmax + 0.5);
-        size_t t = 0;
-        while (t < irmax_of.size() && irmax_of[t] != irmax) ++t;
-        if (t == irmax_of.size()) {
-            const float irstp = (float)irmax / (float)(s - 1);
-            const int64_t w0 = (int64_t)writers.size();
-            const int32_t i0 = (int32_t)irss.size();
-            writers.resize(writers.size() + (size_t)cells, -1);
-            int steps = 0;
-            for (float ir = 0; ir <= irmax; ir += irstp) {
-                irss.push_back(ir);
-                for (int i = 0; i <= s; ++i) {
-                    const int idx = (int)((float)(i * s) + ir / irstp);
-                    if (idx < cells && idx >= 0) writers[(size_t)(w0 + idx)] = steps * (s + 1) + i;
-                }
-                if (++steps > 4 * s + 16) return csx_ctx_fail(c, CSX_E_ARG, "r,phi radius loop does not end");
-            }
-            irmax_of.push_back(irmax);
-            woff.push_back(w0);
-            ioff.push_back(i0);
-        }
-        per[(size_t)k] = RphiImg{(double)xc, (double)yc, woff[t], ioff[t], 0};
-    }
-    const size_t wbytes = writers.size() * sizeof(int), ibytes = std::max<size_t>(irss.size(), 1) * sizeof(float),
-                 pbytes = per.size() * sizeof(RphiImg);
-    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    char* scr = (char*)csx_ctx_scratch(c, 0, up(wbytes) + up(ibytes) + pbytes);
-    if (!scr) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
-    int* d_writers = (int*)scr;
-    float* d_irs = (float*)(scr + up(wbytes));
-    RphiImg* d_per = (RphiImg*)(scr + up(wbytes) + up(ibytes));
+    // map serves every image with the same irmax (found by hash).  Images go in chunks whose
+    // maps stay within kRphiMapBytes (a chunk takes at least one map), so many distinct beam
+    // centres cost launches, not memory.
+    constexpr size_t kRphiMapBytes = size_t(256) << 20;
     hipStream_t st = (hipStream_t)csx_ctx_stream(c);
-    PCHK(c, hipMemcpyAsync(d_writers, writers.data(), wbytes, hipMemcpyHostToDevice, st));
-    if (!irss.empty()) PCHK(c, hipMemcpyAsync(d_irs, irss.data(), irss.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    PCHK(c, hipMemcpyAsync(d_per, per.data(), pbytes, hipMemcpyHostToDevice, st));
-    RphiArgs a{sx, sy, s, phiber, npx};
-    hipLaunchKernelGGL(k_rphi, dim3((unsigned)((cells + 255) / 256), nimg), dim3(256), 0, st, d_img, a, d_per, d_writers,
-                       d_irs, d_out);
-    PCHK(c, hipGetLastError());
-    // the host vectors must outlive the copies
-    PCHK(c, hipStreamSynchronize(st));
+    std::vector<int> writers;
+    std::vector<float> irss;
+    std::vector<RphiImg> per;
+    std::unordered_map<int, std::pair<int64_t, int32_t>> maps;  // irmax -> (writer, radius offsets)
+    for (int k0 = 0; k0 < nimg;) {
+        writers.clear();
+        irss.clear();
+        per.clear();
+        maps.clear();
+        int k = k0;
+        for (; k < nimg; ++k) {
+            const int xc = h_centres[2 * k] + dx, yc = h_centres[2 * k + 1] + dy;
+            double rmax = std::sqrt(sq(1. - xc) + sq(1. - yc));
+            rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq(1. - yc)));
+            rmax = std::max(rmax, std::sqrt(sq((sx - 1.) - xc) + sq((sy - 1.) - yc)));
+            rmax = std::max(rmax, std::sqrt(sq(1. - xc) + sq((sy - 1.) - yc)));
+            const int irmax = (int)(rmax + 0.5);
+            auto it = maps.find(irmax);
+            if (it == maps.end()) {
+                if (!maps.empty() && (writers.size() + (size_t)cells) * sizeof(int) > kRphiMapBytes) break;
+                const float irstp = (float)irmax / (float)(s - 1);
+                const int64_t w0 = (int64_t)writers.size();
+                const int32_t i0 = (int32_t)irss.size();
+                writers.resize(writers.size() + (size_t)cells, -1);
+                int steps = 0;
+                for (float ir = 0; ir <= irmax; ir += irstp) {
+                    irss.push_back(ir);
+                    for (int i = 0; i <= s; ++i) {
+                        const int idx = (int)((float)(i * s) + ir / irstp);
+                        if (idx < cells && idx >= 0) writers[(size_t)(w0 + idx)] = steps * (s + 1) + i;
+                    }
+                    if (++steps > 4 * s + 16) return csx_ctx_fail(c, CSX_E_ARG, "r,phi radius loop does not end");
+                }
+                it = maps.emplace(irmax, std::make_pair(w0, i0)).first;
+            }
+            per.push_back(RphiImg{(double)xc, (double)yc, it->second.first, it->second.second, 0});
+        }
+        const int nk = k - k0;
+        const size_t wbytes = writers.size() * sizeof(int), ibytes = std::max<size_t>(irss.size(), 1) * sizeof(float),
+                     pbytes = per.size() * sizeof(RphiImg);
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        char* scr = (char*)csx_ctx_scratch(c, 0, up(wbytes) + up(ibytes) + pbytes);
+        if (!scr) return csx_ctx_fail(c, CSX_E_NOMEM, "post-processing scratch");
+        int* d_writers = (int*)scr;
+        float* d_irs = (float*)(scr + up(wbytes));
+        RphiImg* d_per = (RphiImg*)(scr + up(wbytes) + up(ibytes));
+        PCHK(c, hipMemcpyAsync(d_writers, writers.data(), wbytes, hipMemcpyHostToDevice, st));
+        if (!irss.empty())
+            PCHK(c, hipMemcpyAsync(d_irs, irss.data(), irss.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        PCHK(c, hipMemcpyAsync(d_per, per.data(), pbytes, hipMemcpyHostToDevice, st));
+        RphiArgs a{sx, sy, s, phiber, npx};
+        hipLaunchKernelGGL(k_rphi, dim3((unsigned)((cells + 255) / 256), nk), dim3(256), 0, st, d_img + (int64_t)k0 * npx,
+                           a, d_per, d_writers, d_irs, d_out + (int64_t)k0 * cells);
+        PCHK(c, hipGetLastError());
+        // the host vectors must outlive the copies, and the scratch is reused by the next chunk
+        PCHK(c, hipStreamSynchronize(st));
+        k0 = k;
+    }
     return csx_post_scale_clip(c, d_out, cells, nimg, scale, clip, clip40, 0);
 }
 

@@ -112,6 +112,53 @@ def test_rphi_many_centres_against_the_oracle(oracle_lib):
         assert np.abs(got[k] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), k
 
 
+def test_rphi_many_distinct_radii_in_chunks():
+    """Many centres with distinct maximum radii (one cell map each): at s = 2048 the maps
+    (16 MiB each) no longer fit one launch's budget, so the images go in several launches;
+    every image equals its own one-centre call."""
+    import ctypes
+    import torch
+    from crystalscatter_amd import _lib
+    f, post, img, b = _post_case("fcc_rphi")
+    x0, x1, y0, y1 = (int(v) for v in b)
+    n, s = 20, 2048
+    centres = np.array([(2 * k - 19, (k * 7) % 13 - 6) for k in range(n)], dtype=np.int32)
+    imgs = torch.from_numpy(np.stack([img * (1.0 + 0.25 * k) for k in range(n)])).cuda()
+    out = torch.empty((n, s * s), dtype=torch.float64, device="cuda")
+    one = torch.empty((1, s * s), dtype=torch.float64, device="cuda")
+    ctx = Context(0)
+    L = _lib.lib()
+    torch.cuda.synchronize()
+    ctx._chk(L.csx_post_rphi_many(ctx._h, ctypes.c_void_p(imgs.data_ptr()), x0, x1, y0, y1,
+                                  centres.ctypes.data_as(ctypes.c_void_p), s, n, ctypes.c_void_p(out.data_ptr()),
+                                  0, 0, 0), "csx_post_rphi_many")
+    ctx.synchronize()
+    for k in range(n):
+        ctx._chk(L.csx_post_rphi(ctx._h, ctypes.c_void_p(imgs[k].data_ptr()), x0, x1, y0, y1, int(centres[k, 0]),
+                                 int(centres[k, 1]), s, 1, ctypes.c_void_p(one.data_ptr()), 0, 0, 0), "csx_post_rphi")
+        ctx.synchronize()
+        assert torch.equal(out[k], one[0]), k
+
+
+def test_tpv_batch_failure_leaves_the_generator_unchanged():
+    """A batch whose random variation produces an invalid set fails before any launch and puts
+    the generator (and the variation state) back, so a retry draws the same sequence."""
+    from crystalscatter_amd import CsxError
+    from crystalscatter_amd import aidata
+    p = dict(CFG3, GridPoints=16)
+    e = SasCalc(device=-1, params=p)
+    e.set("ignore_cbs", 1)
+    a, b = Random(4242), Random(4242)
+    a.zuf(5)
+    b.zuf(5)
+    # uca varied by up to +-5000 %: most draws are below the 0.01 the parameter check requires
+    with pytest.raises(CsxError):
+        aidata.generate(Context(0), e, ["uca=50"], 64, a, aidata.AIOptions(noise=False), "mixed")
+    assert [a.next() for _ in range(40)] == [b.next() for _ in range(40)]
+    with pytest.raises(ValueError):
+        aidata.generate(Context(0), e, ["uca=0.05"], 70000, a, aidata.AIOptions(noise=False), "mixed")
+
+
 @pytest.mark.parametrize("name", ["fcc_rphi", "fcc_rphi_off", "fcc_rphi_clip"])
 def test_fft_matches_the_reference(name):
     """The FFT image of the reference's r,phi image (calculateIFFT without FFTW)."""
