@@ -171,7 +171,8 @@ int csx_fit_fqs_noalloc(csx_ctx* ctx, int w, int h, const csx_mask* mask, int pr
                         double* h_sum, int64_t* h_cnt, int64_t* h_nancnt, double* d_fqs);
 
 /* Batched objective: many parameter sets, each with its own tables, detector size, data and
- * mask, evaluated in one persistent launch.  Sets are staged with csx_fit_batch_stage after
+ * mask, evaluated in one launch (one workgroup per 16x16 tile of the flat tile list of all
+ * staged sets).  Sets are staged with csx_fit_batch_stage after
  * csx_fit_batch_begin; csx_fit_batch_run evaluates all staged sets and writes h_sums[nsets]. */
 int csx_fit_batch_begin(csx_ctx* ctx);
 int csx_fit_batch_stage(csx_ctx* ctx, const csx_scalars* s, const double* h_carr,
