@@ -128,6 +128,10 @@ def run_fits(args, probs=None):
     fits = [dict(engine=p["engine"], d_data=p["data"].data_ptr(), w=p["w"], h=p["h"], x0=p["x0"], lo=FIT_LO,
                  hi=FIT_HI) for p in probs]
     ctx = Context(0)
+    ctx.set_device_prep(not args.host_prep)
+    if args.eager:
+        from crystalscatter_amd import _lib
+        _lib.lib().csx_set_batch_graph(ctx._h, 0)
     t0 = time.perf_counter()
     res, st = nm_batch(ctx, fits, FIT_KEYS, stp=0.05, maxit=args.maxit, ftol=1e-6, precision=args.precision, mode="corrected",
                        threads=args.cpu_threads)
@@ -156,6 +160,7 @@ def run_fits(args, probs=None):
                                "6 free parameters, detectors 128..512, Poisson targets, 8 px beam stop, "
                                "maxit %d, corrected simplex" % (len(probs), args.maxit),
                    "precision": args.precision, "pixels_per_round_mean": px,
+                   "lattice_tables": "host" if args.host_prep else "device",
                    "detectors": {str(n): int(sum(1 for p in probs if p["n"] == n)) for n in (128, 256, 384, 512)}},
         "rounds": int(rounds),
         "chi2_median": float(np.median(chi2)),
@@ -164,7 +169,8 @@ def run_fits(args, probs=None):
         "median_rel_param_err_start": float(np.median(rel0)),
         "median_rel_param_err_final": float(np.median(rel1)),
         "target_generation_s": t_gen,
-        "host_prep_ms": st["prep_ms"], "staging_ms": st["stage_ms"], "batched_run_ms": st["run_ms"],
+        "host_prep_ms": st["prep_ms"], "staging_ms": st["stage_ms"], "launch_ms": st["launch_ms"],
+        "batched_run_ms": st["run_ms"],
     }
     return line
 
@@ -585,6 +591,9 @@ def main():
     ap.add_argument("--nimages", type=int, default=1024, help="images of the aidata workload")
     ap.add_argument("--beam", default="0,0", help="aidata workload: BeamPosX,BeamPosY in pixels")
     ap.add_argument("--maxit", type=int, default=100)
+    ap.add_argument("--eager", action="store_true", help="fits: launch each round's steps one by one, not as a graph")
+    ap.add_argument("--host-prep", action="store_true",
+                    help="fits: build the lattice tables of every vertex on the host instead of the device")
     ap.add_argument("--maxit-surrogate", type=int, default=0,
                     help="exact simplex iterations after a surrogate start (0: the best re-scored candidate)")
     ap.add_argument("--train-patterns", type=int, default=4096)

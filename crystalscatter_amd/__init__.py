@@ -304,6 +304,31 @@ class Context:
     def synchronize(self):
         self._chk(self._L.csx_synchronize(self._h), "csx_synchronize")
 
+    def set_device_prep(self, on):
+        """Batched fits build lattice tables on the device (True, the default) or on the host."""
+        self._chk(self._L.csx_set_device_prep(self._h, 1 if on else 0), "csx_set_device_prep")
+
+    def device_prep_counts(self):
+        """(sets whose tables the device built, entries whose rounding was checked against the
+        C library's exp)."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self._L.csx_device_prep_counts(self._h, ctypes.byref(a), ctypes.byref(b)), "csx_device_prep_counts")
+        return a.value, b.value
+
+    _TABLE_DTYPES = {0: np.float32, 1: np.uint8, 2: np.uint8, 3: np.int32, 4: np.float32, 5: np.float64}
+
+    def fit_batch_table(self, set_index, which):
+        """Table `which` of set `set_index` of the last batch run (csx_fit_batch_table): 0
+        latpar3, 1 pair records (bytes), 2 FP64 pair records (bytes), 3 bin table, 4 isotropic
+        list, 5 (pair-record count, bin scale, |q| min, |q| max, weight bound)."""
+        need = ctypes.c_size_t()
+        self._chk(self._L.csx_fit_batch_table(self._h, int(set_index), int(which), None, 0, ctypes.byref(need)),
+                  "csx_fit_batch_table")
+        out = np.empty(need.value, dtype=np.uint8)
+        self._chk(self._L.csx_fit_batch_table(self._h, int(set_index), int(which), out.ctypes.data_as(ctypes.c_void_p),
+                                              need.value, ctypes.byref(need)), "csx_fit_batch_table")
+        return out.view(self._TABLE_DTYPES[which])
+
 
 def _check_out(out, n):
     if out.dtype != torch_float64() or not out.is_cuda or out.dim() != 2 or out.shape[0] < n or not out.is_contiguous():

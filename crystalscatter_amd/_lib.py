@@ -67,7 +67,7 @@ class NMStats(ctypes.Structure):
     """Mirror of csx_nm_stats."""
     _fields_ = [("rounds", ctypes.c_int64), ("evals", ctypes.c_int64), ("prep_ms", ctypes.c_double),
                 ("stage_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("pruned", ctypes.c_int64),
-                ("screen_ms", ctypes.c_double)]
+                ("screen_ms", ctypes.c_double), ("launch_ms", ctypes.c_double)]
 
 
 SCREEN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
@@ -114,6 +114,9 @@ SIGNATURES = [
     ("csx_tune_force", _I, [_VP, _I]),
     ("csx_set_batch_graph", _I, [_VP, _I]),
     ("csx_batch_graph_launches", ctypes.c_longlong, [_VP]),
+    ("csx_set_device_prep", _I, [_VP, _I]),
+    ("csx_device_prep_counts", _I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    ("csx_fit_batch_table", _I, [_VP, _I, _I, _VP, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("csx_nm_run_cb", _I, [_I, _I, _I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),
     ("csx_nm_batch", _I, [_VP, _I, ctypes.POINTER(NMFit), _I, _I, _I, ctypes.POINTER(_CP), _D, _I, _D, _I, _I,
                           ctypes.POINTER(NMResult), ctypes.POINTER(NMStats)]),

@@ -26,6 +26,8 @@
 #include "csx.h"
 #include "csx_physics.h"
 #include "csx_prep.h"
+#include "csx_ddexp.h"
+#include "csx_rows.h"
 
 namespace csx {
 
@@ -957,7 +959,7 @@ __device__ double romb_delta_cyl(const DevSet& d, double qx, double qy, int i1) 
 // the ratio of the smaller to the larger of X and Y (no power tables; the reference builds
 // X^m and Y^(n-m) as running products, so the two agree to rounding).
 __device__ double cyl_gauss_series(const DevSet& d, double qxs, double qys, int oc) {
-    const double* c1p = carr_of(d, 0);
+    const double* c1p = carr_of<0>(d);
     const double* c11 = d.carr11pm;  // [n][m], CSX_NIM2 columns
     const double X = (oc == 3) ? qys * qys : qxs * qxs, Y = (oc == 3) ? qxs * qxs : qys * qys;
     const bool xs = X <= Y;            // Horner variable r = small/large <= 1
@@ -1428,12 +1430,30 @@ __device__ __forceinline__ bool fit_masked(const csx_scalars& s, const csx_mask&
     return false;
 }
 
+// Device preparation of a batched set's lattice tables (k_prep_sets): inputs written at
+// staging, outputs written by the kernel before the set is evaluated.
+struct PrepIn {
+    double R[9], RT[9];       // rotated reciprocal bases (rows a*, b*, c*) and twinned ones
+    const int32_t* l2;        // latpar2 of the reflection list
+    const int2* pairs_t;      // Friedel pairs (ia, ib) of the list, in list order
+    const int2* iso_t;        // isotropic entries (latpar index, mhkl*fhkl)
+    float* l3;                // outputs: latpar3, pair records (one of pr, pr64), bin table,
+    PairRec* pr;              // isotropic list
+    PairRec64* pr64;
+    int32_t* lut;
+    float2* iso;
+    int32_t on, n2, np, niso;
+    int32_t flags;            // bit 0: a pair's members differ; bits 8+: ambiguous roundings
+    int32_t pad;
+};
+
 struct FitSet {
     DevSet d;
     csx_mask m;
     const double* data;
     int32_t w, h;
     int32_t tiles_x, first_tile;  // tile range of this set in the batch
+    PrepIn pin;
 };
 
 struct TileSum {
@@ -1583,6 +1603,7 @@ k_image_batch_lat(const FitSet* __restrict__ sets, const int2* __restrict__ tile
 struct SetSum {
     double sum;
     long long cnt, nan;
+    long long flags;  // batch: the set's device-preparation flags (PrepIn::flags)
 };
 __device__ __forceinline__ void block_sum_tiles(const TileSum* __restrict__ part, int n, SetSum* __restrict__ out) {
     __shared__ double s_s[kBlock];
@@ -1621,6 +1642,260 @@ __global__ void __launch_bounds__(kBlock) k_reduce_sets(const FitSet* __restrict
     const int s = blockIdx.x;
     if (s >= nsets) return;
     block_sum_tiles(part + sets[s].first_tile, ntiles_of[s], out + s);
+    if (threadIdx.x == 0) out[s].flags = sets[s].pin.on ? sets[s].pin.flags : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device preparation of batched lattice sets: what the host builds per parameter set from the
+// lattice constants (the latpar3 rows, Source/sc_calc_generic_cpu.cpp:838-954, and the
+// mixed path's Friedel-pair records, |u_z| segments and |q| bin table), built by one
+// workgroup per set from the set's rotated reciprocal bases and its lattice's structural
+// lists.  Every value is the host's, bit for bit: the same arithmetic (csx_rows.h), the pairs
+// in the host's stable |q| order, the weights summed in the host's order; the peak norm's
+// exponential is correctly rounded (csx_ddexp.h) and each entry where the C library's could
+// round the other way is checked and counted.
+constexpr int kPrepBlock = 256;
+constexpr int kMaxPairs = CSX_LATLEN / 2;
+constexpr int kLutBinsDev = 256;  // bins of the |q| table (kLutBins on the host side)
+
+// One latpar3 row: the 17 floats into T (if not null), and the values the pairs use.
+struct RowVals {
+    float qx, qy, qz, qh, t10, t11;
+    int amb;
+};
+__device__ inline void prep_row(const FitSet& f, int ii, float* T, RowVals& v) {
+    const csx_scalars& s = f.d.s;
+    const int32_t* L = f.pin.l2 + (size_t)ii * 6;
+    rows::Geometry g;
+    rows::geometry(f.pin.R, f.pin.RT, L[1], L[2], L[3], g);
+    const int shp = (int)(s.combo_key >> 16 & 0xff);
+    v.amb = 0;
+    v.t11 = 0.0f;
+    if (shp == 2 || shp == 7) {
+        const double a = rows::x2phi(g.qh, s.phiwidth);
+        double ex = 0.0;
+        if (rows::gauss_norm3_uses_exp(a)) {
+            bool near_mid;
+            double alt;
+            ex = ddexp::exp_rn(rows::gauss_norm3_exp_arg(a), near_mid, alt);
+            if (near_mid && (float)rows::gauss_norm3_from(a, alt) != (float)rows::gauss_norm3_from(a, ex)) v.amb = 1;
+        }
+        v.t11 = (float)rows::gauss_norm3_from(a, ex);
+    }
+    v.qx = (float)g.qx;
+    v.qy = (float)g.qy;
+    v.qz = (float)g.qz;
+    v.qh = (float)g.qh;
+    v.t10 = (float)rows::g3(g.qh);
+    if (T) {
+        T[0] = 0.0f;
+        T[1] = (float)(2 * kPiD / s.uca);
+        T[2] = v.qx; T[3] = v.qy; T[4] = v.qz; T[5] = v.qh;
+        T[6] = (float)(g.qh / (2.0 * kPiD));
+        T[7] = (float)g.tx; T[8] = (float)g.ty; T[9] = (float)g.tz;
+        T[10] = v.t10;
+        T[11] = v.t11;
+        T[12] = 0.0f;
+        T[13] = (float)g.qxy;
+        T[14] = 0.0f; T[15] = 0.0f; T[16] = 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kPrepBlock) k_prep_sets(FitSet* __restrict__ sets, int nsets,
+                                                          const int* __restrict__ ntiles_of, int2* __restrict__ tiles) {
+    if ((int)blockIdx.x >= nsets) return;
+    FitSet& f = sets[blockIdx.x];
+    // the set's entries of the batch's flat tile list: (set, tile within the set)
+    for (int k = threadIdx.x; k < ntiles_of[blockIdx.x]; k += kPrepBlock)
+        tiles[f.first_tile + k] = make_int2((int)blockIdx.x, k);
+    if (!f.pin.on) return;
+    __shared__ float s_qh[kMaxPairs], s_t10[kMaxPairs], s_t11[kMaxPairs];
+    __shared__ double s_A[kMaxPairs];
+    __shared__ int s_order[kMaxPairs];
+    __shared__ unsigned char s_seg[kMaxPairs];  // band | 0x10 if |u_z| <= kPlaneUz
+    __shared__ int s_cnt[kSegments][kPrepBlock];
+    __shared__ int s_hist[kSegments][kLutBinsDev + 1];
+    __shared__ int s_total[kSegments], s_start[kSegments + 1];
+    __shared__ int s_flags, s_amb;
+    __shared__ float s_qtop, s_qbot;
+    const int t = threadIdx.x;
+    const int n2 = f.pin.n2, np = f.pin.np;
+    if (t == 0) {
+        s_flags = 0;
+        s_amb = 0;
+    }
+    for (int i = t; i < kSegments * (kLutBinsDev + 1); i += kPrepBlock) (&s_hist[0][0])[i] = 0;
+    __syncthreads();
+    // latpar3 rows (row 0 zero)
+    for (int i = t; i < 17; i += kPrepBlock) f.pin.l3[i] = 0.0f;
+    int amb = 0;
+    for (int ii = 1 + t; ii <= n2; ii += kPrepBlock) {
+        RowVals v;
+        prep_row(f, ii, f.pin.l3 + (size_t)ii * 17, v);
+        amb += v.amb;
+    }
+    // pairs in list order: |q_hkl|, weight, segment; the partner must be the exact mirror
+    for (int k = t; k < np; k += kPrepBlock) {
+        const int2 ab = f.pin.pairs_t[k];
+        RowVals a, b;
+        prep_row(f, ab.x, nullptr, a);
+        prep_row(f, ab.y, nullptr, b);
+        const int32_t* La = f.pin.l2 + (size_t)ab.x * 6;
+        const int32_t* Lb = f.pin.l2 + (size_t)ab.y * 6;
+        const bool same = b.qh == a.qh && b.t10 == a.t10 && b.t11 == a.t11 && La[4] * La[5] == Lb[4] * Lb[5] &&
+                          b.qx == -a.qx && b.qy == -a.qy && b.qz == -a.qz && a.qh > 0.0f;
+        if (!same) atomicOr(&s_flags, 1);
+        const double A = (double)(La[4] * La[5]) * (double)a.t10 / (double)a.t11;
+        const double ux = (double)a.qx, uy = (double)a.qy, uz = (double)a.qz;
+        const double un = sqrt(ux * ux + uy * uy + uz * uz);
+        const int band = min(kBands - 1, (int)(fabs(uz / un) * kBands));
+        const bool plane = fabsf((float)(uz / un)) <= kPlaneUz;
+        s_qh[k] = a.qh;
+        s_t10[k] = a.t10;
+        s_t11[k] = a.t11;
+        s_A[k] = A;
+        s_seg[k] = (unsigned char)(band | (plane ? 0x10 : 0));
+    }
+    if (amb) atomicAdd(&s_amb, amb);
+    __syncthreads();
+    // the weights' bound in list order, the |q| range
+    if (t == 0) {
+        double asum = 0.0;
+        bool negative = false;
+        float qtop = 0.0f, qbot = np > 0 ? s_qh[0] : 0.0f;
+        for (int k = 0; k < np; ++k) {
+            asum += 2.0 * fabs(s_A[k]);
+            negative = negative || s_A[k] < 0;
+            qtop = fmaxf(qtop, s_qh[k]);
+            qbot = fminf(qbot, s_qh[k]);
+        }
+        f.d.asum2 = negative ? HUGE_VAL : asum;
+        s_qtop = qtop;
+        s_qbot = qbot;
+    }
+    // stable order by |q_hkl|
+    for (int k = t; k < np; k += kPrepBlock) {
+        const float q = s_qh[k];
+        int r = 0;
+        for (int j = 0; j < np; ++j) r += (s_qh[j] < q) || (s_qh[j] == q && j < k);
+        s_order[r] = k;
+    }
+    __syncthreads();
+    // segment positions: each lane takes a run of the sorted order
+    const int run = (np + kPrepBlock - 1) / kPrepBlock;
+    const int p0 = min(np, t * run), p1 = min(np, p0 + run);
+    int cnt[kSegments];
+    for (int b = 0; b < kSegments; ++b) cnt[b] = 0;
+    for (int p = p0; p < p1; ++p) {
+        const int sg = s_seg[s_order[p]];
+        ++cnt[sg & 0xf];
+        ++cnt[kBands];
+        if (sg & 0x10) ++cnt[kBands + 1];
+    }
+    for (int b = 0; b < kSegments; ++b) s_cnt[b][t] = cnt[b];
+    __syncthreads();
+    if (t < kSegments) {
+        int acc = 0;
+        for (int j = 0; j < kPrepBlock; ++j) {
+            const int c = s_cnt[t][j];
+            s_cnt[t][j] = acc;
+            acc += c;
+        }
+        s_total[t] = acc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        s_start[0] = 0;
+        for (int b = 0; b < kSegments; ++b) s_start[b + 1] = s_start[b] + ((s_total[b] + 3) & ~3);
+    }
+    __syncthreads();
+    int off[kSegments];
+    for (int b = 0; b < kSegments; ++b) off[b] = s_start[b] + s_cnt[b][t];
+    for (int p = p0; p < p1; ++p) {
+        const int k = s_order[p];
+        const int2 ab = f.pin.pairs_t[k];
+        const int32_t* La = f.pin.l2 + (size_t)ab.x * 6;
+        rows::Geometry g;
+        rows::geometry(f.pin.R, f.pin.RT, La[1], La[2], La[3], g);
+        const double ux = (double)(float)g.qx, uy = (double)(float)g.qy, uz = (double)(float)g.qz;
+        const double qh = (double)s_qh[k];
+        const double A = s_A[k];
+        const int sg = s_seg[k];
+        const int dst[3] = {off[sg & 0xf]++, off[kBands]++, (sg & 0x10) ? off[kBands + 1]++ : -1};
+        if (f.pin.pr64) {
+            PairRec64 r;
+            r.ux = ux / qh;
+            r.uy = uy / qh;
+            r.uz = uz / qh;
+            r.A = A;
+            r.qh = s_qh[k];
+            r.ia = ab.x;
+            r.ib = ab.y;
+            r.pad = 0.0f;
+            for (int j = 0; j < 3; ++j)
+                if (dst[j] >= 0) f.pin.pr64[dst[j]] = r;
+        } else {
+            const double un = sqrt(ux * ux + uy * uy + uz * uz);
+            PairRec r;
+            r.ux = (float)(ux / un);
+            r.uy = (float)(uy / un);
+            r.uz = (float)(uz / un);
+            r.qh = s_qh[k];
+            r.uz2 = (float)((uz / un) * (uz / un));
+            r.A = (float)A;
+            r.ia = ab.x;
+            r.ib = ab.y;
+            for (int j = 0; j < 3; ++j)
+                if (dst[j] >= 0) f.pin.pr[dst[j]] = r;
+        }
+    }
+    // inert padding to a multiple of 4 per segment
+    if (t < kSegments) {
+        for (int i = s_start[t] + s_total[t]; i < s_start[t + 1]; ++i) {
+            if (f.pin.pr64) f.pin.pr64[i] = PairRec64{0.0, 0.0, 0.0, 0.0, 1.0e30f, 0, 0, 0.0f};
+            else f.pin.pr[i] = PairRec{0.0f, 0.0f, 0.0f, 1.0e30f, 0.0f, 0.0f, 0, 0};
+        }
+    }
+    // bin table: lut[b][k] = first index of segment b with (double)|q| >= k dq
+    const double span = (double)s_qtop * 1.001 + 1e-3;
+    const float inv_dq = (float)(kLutBinsDev / span);
+    const double dqb = 1.0 / (double)inv_dq;
+    for (int k = t; k < np; k += kPrepBlock) {
+        const double q = (double)s_qh[k];
+        int kk = max(0, (int)(q / dqb) - 2);
+        while (kk < kLutBinsDev && !(q < kk * dqb)) ++kk;
+        if (kk < kLutBinsDev) {
+            const int sg = s_seg[k];
+            atomicAdd(&s_hist[sg & 0xf][kk], 1);
+            atomicAdd(&s_hist[kBands][kk], 1);
+            if (sg & 0x10) atomicAdd(&s_hist[kBands + 1][kk], 1);
+        }
+    }
+    __syncthreads();
+    if (t < kSegments) {
+        int32_t* lb = f.pin.lut + (size_t)t * (kLutBinsDev + 1);
+        int acc = 0;
+        for (int k = 0; k < kLutBinsDev; ++k) {
+            acc += s_hist[t][k];
+            lb[k] = s_start[t] + acc;
+        }
+        lb[kLutBinsDev] = s_start[t] + s_total[t];
+    }
+    // isotropic list: (|q_hkl| of latpar row ii, mhkl*fhkl)
+    for (int i = t; i < f.pin.niso; i += kPrepBlock) {
+        const int2 e = f.pin.iso_t[i];
+        const int32_t* L = f.pin.l2 + (size_t)e.x * 6;
+        rows::Geometry g;
+        rows::geometry(f.pin.R, f.pin.RT, L[1], L[2], L[3], g);
+        f.pin.iso[i] = make_float2((float)g.qh, (float)e.y);
+    }
+    if (t == 0) {
+        f.d.npairs = s_start[kSegments];
+        f.d.lut_inv_dq = inv_dq;
+        f.d.qh_min = s_qbot;
+        f.d.qh_max = s_qtop;
+        f.pin.flags = s_flags | (min(s_amb, 0x7fffff) << 8);
+    }
 }
 
 // ExpandImage (Source/sc_calc_generic_cpu.cpp:1246-1269): the whole image (-z..z)^2 from the
@@ -1700,7 +1975,7 @@ struct PinnedArena {
 struct BatchGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t h2d = nullptr, fit = nullptr, red = nullptr, d2h = nullptr;
+    hipGraphNode_t h2d = nullptr, prep = nullptr, fit = nullptr, red = nullptr, d2h = nullptr;
     const void* fit_fn = nullptr;
     long long launches = 0;
 };
@@ -1723,7 +1998,7 @@ struct DevBuf {
 // Base-2 exponent below which the mixed path drops a pair term (2^-32 ~ 2.3e-10 of the
 // pair's weight); each pixel's bound of the dropped mass enters its conditioning guard.
 constexpr float kDropExponent = -32.0f;
-constexpr int kLutBins = 256;
+constexpr int kLutBins = kLutBinsDev;
 
 // Host-side fast tables of one parameter set: Friedel pairs in |u_z| bands sorted by
 // |q_hkl|, the bin table, unpaired reflections and the isotropic list.
@@ -1897,7 +2172,8 @@ void set_consts(DevSet& d) {
     }
 }
 
-void set_fast(DevSet& d, const FastTables& ft) {
+// The per-set constants of the lattice pixel bodies that do not depend on the tables.
+void set_fast_consts(DevSet& d) {
     const csx_scalars& s = d.s;
     double width = 1;
     if (s.dsch) width = 4.0 / s.domainsize;
@@ -1922,15 +2198,85 @@ void set_fast(DevSet& d, const FastTables& ft) {
         d.sfq_c0 = 1 / (z0 * (z0 - 1));
         d.sfq_c2 = 1 / (z0 - 2);
     }
+    d.nlut = kLutBins;
+    d.cut = kDropExponent;
+}
+
+void set_fast(DevSet& d, const FastTables& ft) {
+    set_fast_consts(d);
     d.npairs = (int)ft.pairs.size();
     d.nsingle = (int)ft.singles.size();
     d.niso = (int)ft.iso.size();
-    d.nlut = kLutBins;
     d.lut_inv_dq = ft.lut_inv_dq;
-    d.cut = kDropExponent;
     d.qh_min = ft.qh_min;
     d.qh_max = ft.qh_max;
     d.asum2 = ft.asum2;
+}
+
+// The structural part of a lattice's fast tables, for the device preparation (k_prep_sets):
+// the Friedel pairs (ia, ib) in list order, the unpaired reflections and the isotropic
+// entries.  The host's pairing (build_fast_tables) takes for each valid reflection the first
+// unused valid reflection with the mirrored q_hkl, the same |q_hkl|, g3, peak norm and
+// mhkl*fhkl; for the lists of prepare, that is (-h, -k, -l) whenever it has the same
+// mhkl*fhkl, since q_hkl is linear in (h, k, l) with exactly mirrored rounding.  The device
+// checks every pair's values and flags a pair whose members differ.
+struct RowsTemplate {
+    std::shared_ptr<const csx::LatticeLists> lists;
+    std::vector<int2> pairs;
+    std::vector<int32_t> singles;
+    std::vector<int2> iso;  // (latpar index, mhkl*fhkl)
+};
+
+std::shared_ptr<const RowsTemplate> rows_template(const std::shared_ptr<const csx::LatticeLists>& L, int peakmax1) {
+    static std::mutex mu;
+    static std::vector<std::pair<int, std::shared_ptr<const RowsTemplate>>> cache;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const auto& c : cache)
+            if (c.second->lists == L && c.first == peakmax1) return c.second;
+    }
+    auto T = std::make_shared<RowsTemplate>();
+    T->lists = L;
+    const int n2 = L->n2;
+    const int32_t* L2 = L->l2.data();
+    std::vector<char> used(n2 + 1, 0);
+    int hm = 0;
+    for (int ii = 1; ii <= n2; ++ii)
+        for (int c = 1; c <= 3; ++c) hm = std::max(hm, std::abs(L2[ii * 6 + c]));
+    const int side = 2 * hm + 1;
+    std::unordered_map<int64_t, std::vector<int>> byhkl;
+    auto key = [&](int h, int k, int l) { return ((int64_t)(h + hm) * side + (k + hm)) * side + (l + hm); };
+    for (int ii = 1; ii <= n2; ++ii) byhkl[key(L2[ii * 6 + 1], L2[ii * 6 + 2], L2[ii * 6 + 3])].push_back(ii);
+    auto valid = [&](int ii) {
+        return (L2[ii * 6 + 1] | L2[ii * 6 + 2] | L2[ii * 6 + 3]) != 0 && L2[ii * 6 + 4] * L2[ii * 6 + 5] != 0;
+    };
+    for (int ii = 1; ii <= n2; ++ii) {
+        if (used[ii] || !valid(ii)) continue;
+        used[ii] = 1;
+        int jj = 0;
+        const auto it = byhkl.find(key(-L2[ii * 6 + 1], -L2[ii * 6 + 2], -L2[ii * 6 + 3]));
+        if (it != byhkl.end())
+            for (int c : it->second)
+                if (!used[c] && valid(c) && L2[c * 6 + 4] * L2[c * 6 + 5] == L2[ii * 6 + 4] * L2[ii * 6 + 5]) {
+                    jj = c;
+                    break;
+                }
+        if (!jj) {
+            T->singles.push_back(ii);
+            continue;
+        }
+        used[jj] = 1;
+        T->pairs.push_back(make_int2(ii, jj));
+    }
+    const int32_t* L1 = L->l1.data();
+    for (int ii = 1; ii <= peakmax1 && ii <= n2; ++ii) {
+        const int m = L1[ii * 6 + 4], f = L1[ii * 6 + 5];
+        if (m == 0 || f == 0) continue;
+        T->iso.push_back(make_int2(ii, m * f));
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    cache.emplace_back(peakmax1, T);
+    return T;
 }
 
 // Physics-mode orientation quadrature tables of Gaussian-distributed cylinders.
@@ -2033,6 +2379,7 @@ struct csx_ctx {
     std::string err;
     bool uploaded = false;
     DevSet set{};
+    double carr_h[4 * kCarrLen];  // the uploaded set's coefficient series (host copy)
     DevBuf carr, carr11pm, lat1, lat2, lat3, pairs, pairs64, lut, singles, iso, tiles, fqs_tmp, stats, cqdl, cqtt, cqtab;
     // Radial pre-pass: two tables used alternately, filled on a second stream so that a
     // render's pre-pass overlaps the previous render's image kernel.  ev_rad[s]: table s is
@@ -2049,11 +2396,26 @@ struct csx_ctx {
     // batch staging
     std::vector<FitSet> bsets;
     std::vector<int> bntiles;
-    std::vector<int2> btiles;
+    int bntiles_total = 0;  // tiles of the pending batch (its flat tile list is built on the device)
+    DevBuf btiles_d;
     PinnedArena barena;  // host image of the device arena (tables of all sets), pinned
     PinnedArena bres;    // sums and counts of the batch in flight (size = its set count)
-    struct Pending { size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso; };
+    struct Pending {
+        size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso;
+        size_t off_pairs_t, off_iso_t;             // device preparation: the lattice's template (arena)
+        size_t dev_l3, dev_pairs, dev_lut, dev_iso;  // and the set's tables (bdev); SIZE_MAX if none
+    };
     std::vector<Pending> bpend;
+    // device preparation (k_prep_sets): each template's arrays in this batch's arena (l1, l2,
+    // singles, pairs, iso), the device-built tables' buffer and its use, counters
+    struct TmplAt { const RowsTemplate* t; size_t off[5]; };
+    std::vector<TmplAt> btmpl;
+    DevBuf bdev;
+    size_t bdev_size = 0;
+    bool bprep = false;        // the pending batch has sets whose tables the device builds
+    bool device_prep = true;   // the batched fit driver stages lattice sets that way
+    long long prep_sets = 0, prep_amb = 0;
+    const FitSet* bsets_d = nullptr;  // device records of the last uploaded batch
     DevBuf barena_d, bpart_d, bsums_d;
     DevBuf post[2];  // scratch of the post-processing calls (csx_post.hip)
     int tune_force = -1;  // batched-objective variant for every batch of this context (-1: table)
@@ -2387,7 +2749,8 @@ void csx_destroy(csx_ctx* c) {
     c->bres.release();
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
-                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1], &c->coldq})
+                      &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1], &c->coldq,
+                      &c->bdev, &c->btiles_d})
         b->release();
     if (c->bgraph.exec) (void)hipGraphExecDestroy(c->bgraph.exec);
     if (c->bgraph.graph) (void)hipGraphDestroy(c->bgraph.graph);
@@ -2507,6 +2870,11 @@ static int check_scalars(csx_ctx* c, const csx_scalars* s) {
     return CSX_OK;
 }
 
+// The series the device reads (carr_of) from the host's 18 x CSX_NCOEF coefficient block.
+static void gather_carr(const double* full, double* out) {
+    for (int j = 0; j < 4; ++j) std::memcpy(out + j * kCarrLen, full + kCarrArrays[j] * CSX_NCOEF, kCarrLen * sizeof(double));
+}
+
 int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const double* h_carr11pm,
                const int32_t* h_l1, const int32_t* h_l2, const float* h_l3) {
     if (!c) return CSX_E_ARG;
@@ -2520,8 +2888,9 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
     // a radial pre-pass still running reads the tables about to be replaced
     HIPCHK(c, hipStreamSynchronize(c->side));
     const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
-    HIPCHK(c, c->carr.ensure(18 * CSX_NCOEF * sizeof(double)));
-    HIPCHK(c, hipMemcpyAsync(c->carr.p, h_carr, 18 * CSX_NCOEF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->carr.ensure(4 * kCarrLen * sizeof(double)));
+    gather_carr(h_carr, c->carr_h);
+    HIPCHK(c, hipMemcpyAsync(c->carr.p, c->carr_h, 4 * kCarrLen * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->set = DevSet{};
     c->set.s = *s;
     set_consts(c->set);
@@ -2742,22 +3111,23 @@ struct csx_staged {
     int ntiles = 0;
     int rc = CSX_OK;
     std::string err;
+    std::shared_ptr<const RowsTemplate> tmpl;  // tables built on the device (k_prep_sets)
 };
 csx_staged* csx_staged_new() { return new csx_staged(); }
 void csx_staged_free(csx_staged* s) { delete s; }
 
-// Host part of staging one set: checks, fast tables and the set's bytes, into `out`.
-// Touches no context state, so many sets can be built on different threads at once.
-void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr, const int32_t* h_l1,
-                     const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
-                     const double* d_data) {
-    csx_staged& st = *out;
+// The checks and the part of staging common to both kinds of staging: the set record and the
+// coefficient tables.
+static bool stage_begin(csx_staged& st, const csx_scalars* s, const double* h_carr, int w, int h,
+                        const csx_mask* mask, const double* d_data) {
     st.rc = CSX_OK;
     st.err.clear();
     st.arena.clear();
+    st.tmpl.reset();
     auto bad = [&](int code, const char* msg) {
         st.rc = code;
         st.err = msg;
+        return false;
     };
     if (!s || !h_carr || !mask || !d_data || w <= 0 || h <= 0) return bad(CSX_E_ARG, "null argument or empty fit array");
     if (kind_of(s->combo_key) < 0) return bad(CSX_E_ARG, "combination key not supported by the device path");
@@ -2773,8 +3143,6 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
         return bad(CSX_E_ARG, "fit array must match the image width and fit inside the image");
     if (s->iimin != s->xmin || s->iimin != s->ymin)
         return bad(CSX_E_ARG, "fit needs the full (Q4) or first-quadrant (Q1) image layout");
-    const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
-    if (lattice && (!h_l1 || !h_l2 || !h_l3)) return bad(CSX_E_ARG, "null reflection tables");
     FitSet& f = st.f;
     f = FitSet{};
     f.d.s = *s;
@@ -2786,24 +3154,72 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
     f.tiles_x = (w + kTile - 1) / kTile;
     st.ntiles = f.tiles_x * ((h + kTile - 1) / kTile);
     auto& pd = st.pd;
-    pd = csx_staged::Offsets{};
-    pd.off_carr = arena_put(st.arena, h_carr, 18 * CSX_NCOEF * sizeof(double));
-    if (lattice) {
-        pd.off_l1 = arena_put(st.arena, h_l1, (size_t)(s->peakmax1 + 1) * 6 * 4);
-        pd.off_l2 = arena_put(st.arena, h_l2, (size_t)(s->peakmax2 + 1) * 6 * 4);
-        pd.off_l3 = arena_put(st.arena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
-        FastTables ft;
-        build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
-        // kinds with the precise loop read only the FP64 records
-        pd.off_pairs = ft.pairs64.empty() ? arena_put(st.arena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec))
-                                          : SIZE_MAX;
-        pd.off_pairs64 = ft.pairs64.empty() ? SIZE_MAX
-                                            : arena_put(st.arena, ft.pairs64.data(), ft.pairs64.size() * sizeof(PairRec64));
-        pd.off_lut = arena_put(st.arena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
-        pd.off_singles = arena_put(st.arena, ft.singles.data(), ft.singles.size() * sizeof(int32_t));
-        pd.off_iso = arena_put(st.arena, ft.iso.data(), ft.iso.size() * sizeof(float2));
-        set_fast(f.d, ft);
+    pd = csx_staged::Offsets{SIZE_MAX, SIZE_MAX, SIZE_MAX, SIZE_MAX, SIZE_MAX, SIZE_MAX, SIZE_MAX, SIZE_MAX, SIZE_MAX};
+    double cc[4 * kCarrLen];
+    gather_carr(h_carr, cc);
+    pd.off_carr = arena_put(st.arena, cc, sizeof cc);
+    return true;
+}
+
+// Host part of staging one set: checks, fast tables and the set's bytes, into `out`.
+// Touches no context state, so many sets can be built on different threads at once.
+void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr, const int32_t* h_l1,
+                     const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
+                     const double* d_data) {
+    csx_staged& st = *out;
+    if (!stage_begin(st, s, h_carr, w, h, mask, d_data)) return;
+    const bool lattice = (s->combo_key >> 8 & 0xff) != 12;
+    if (!lattice) return;
+    if (!h_l1 || !h_l2 || !h_l3) {
+        st.rc = CSX_E_ARG;
+        st.err = "null reflection tables";
+        return;
     }
+    auto& pd = st.pd;
+    pd.off_l1 = arena_put(st.arena, h_l1, (size_t)(s->peakmax1 + 1) * 6 * 4);
+    pd.off_l2 = arena_put(st.arena, h_l2, (size_t)(s->peakmax2 + 1) * 6 * 4);
+    pd.off_l3 = arena_put(st.arena, h_l3, (size_t)(s->peakmax2 + 1) * 17 * 4);
+    FastTables ft;
+    build_fast_tables(*s, h_l1, h_l2, h_l3, ft);
+    // kinds with the precise loop read only the FP64 records
+    if (ft.pairs64.empty()) pd.off_pairs = arena_put(st.arena, ft.pairs.data(), ft.pairs.size() * sizeof(PairRec));
+    else pd.off_pairs64 = arena_put(st.arena, ft.pairs64.data(), ft.pairs64.size() * sizeof(PairRec64));
+    pd.off_lut = arena_put(st.arena, ft.lut.data(), ft.lut.size() * sizeof(int32_t));
+    pd.off_singles = arena_put(st.arena, ft.singles.data(), ft.singles.size() * sizeof(int32_t));
+    pd.off_iso = arena_put(st.arena, ft.iso.data(), ft.iso.size() * sizeof(float2));
+    set_fast(st.f.d, ft);
+}
+
+// Staging of a lattice set whose tables the device builds (k_prep_sets): the host part is the
+// scalars, the coefficient tables and the rotated reciprocal bases; the reflection lists and
+// their pairing are shared by every set of the lattice and go over once per batch.
+void csx_stage_build_rows(csx_staged* out, const csx::PreparedSet& P, int w, int h, const csx_mask* mask,
+                          const double* d_data) {
+    csx_staged& st = *out;
+    const csx_scalars* s = &P.s;
+    if (!stage_begin(st, s, P.carr.data(), w, h, mask, d_data)) return;
+    auto bad = [&](const char* msg) {
+        st.rc = CSX_E_ARG;
+        st.err = msg;
+        st.tmpl.reset();
+    };
+    if ((s->combo_key >> 8 & 0xff) == 12 || !P.lists)
+        return bad("device preparation needs a lattice set prepared with rows_on_device");
+    if (P.lists->n2 != s->peakmax2 || P.lists->n1 != s->peakmax1)
+        return bad("reflection lists do not match the set's peakmax");
+    st.tmpl = rows_template(P.lists, s->peakmax1);
+    if ((int)st.tmpl->pairs.size() > kMaxPairs) return bad("too many Friedel pairs for the device preparation");
+    FitSet& f = st.f;
+    set_fast_consts(f.d);
+    f.d.nsingle = (int)st.tmpl->singles.size();
+    f.d.niso = (int)st.tmpl->iso.size();
+    PrepIn& pin = f.pin;
+    std::memcpy(pin.R, P.rot, sizeof pin.R);
+    std::memcpy(pin.RT, P.rott, sizeof pin.RT);
+    pin.on = 1;
+    pin.n2 = s->peakmax2;
+    pin.np = (int)st.tmpl->pairs.size();
+    pin.niso = (int)st.tmpl->iso.size();
 }
 
 // Append built sets to the context's pending batch: offsets in order, then the byte copies
@@ -2837,11 +3253,70 @@ int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int t
         for (auto& th : pool) th.join();
     }
     c->barena.size = off;
+    // device preparation: each lattice template once per batch in the arena, each set's
+    // tables in the device buffer
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t o = (c->barena.size + 255) & ~size_t(255);
+        if (c->barena.reserve(o + bytes) != hipSuccess) return SIZE_MAX;
+        if (bytes) std::memcpy(c->barena.p + o, src, bytes);
+        c->barena.size = o + bytes;
+        return o;
+    };
+    for (int i = 0; i < n; ++i) {
+        const RowsTemplate* t = sts[i]->tmpl.get();
+        if (!t) continue;
+        bool found = false;
+        for (const auto& e : c->btmpl) found = found || e.t == t;
+        if (!found) {
+            csx_ctx::TmplAt e{t, {}};
+            e.off[0] = put(t->lists->l1.data(), t->lists->l1.size() * sizeof(int32_t));
+            e.off[1] = put(t->lists->l2.data(), t->lists->l2.size() * sizeof(int32_t));
+            e.off[2] = put(t->singles.data(), t->singles.size() * sizeof(int32_t));
+            e.off[3] = put(t->pairs.data(), t->pairs.size() * sizeof(int2));
+            e.off[4] = put(t->iso.data(), t->iso.size() * sizeof(int2));
+            for (size_t o : e.off)
+                if (o == SIZE_MAX) return fail(c, CSX_E_HIP, "pinned staging memory");
+            c->btmpl.push_back(e);
+        }
+    }
     for (int i = 0; i < n; ++i) {
         const csx_staged& st = *sts[i];
         const size_t b = base[i];
         auto at = [b](size_t o) { return o == SIZE_MAX ? SIZE_MAX : b + o; };
         csx_ctx::Pending pd{};
+        pd.off_pairs_t = pd.off_iso_t = SIZE_MAX;
+        pd.dev_l3 = pd.dev_pairs = pd.dev_lut = pd.dev_iso = SIZE_MAX;
+        if (st.tmpl) {
+            const RowsTemplate* t = st.tmpl.get();
+            const csx_ctx::TmplAt* e = nullptr;
+            for (const auto& x : c->btmpl)
+                if (x.t == t) e = &x;
+            auto dev = [&](size_t bytes) {
+                const size_t o = (c->bdev_size + 255) & ~size_t(255);
+                c->bdev_size = o + bytes;
+                return o;
+            };
+            const int np = (int)t->pairs.size();
+            const bool precise = precise_kind(kind_of(st.f.d.s.combo_key));
+            pd.dev_l3 = dev((size_t)(st.f.pin.n2 + 1) * 17 * sizeof(float));
+            pd.dev_pairs = dev((size_t)(3 * np + 4 * kSegments) * (precise ? sizeof(PairRec64) : sizeof(PairRec)));
+            pd.dev_lut = dev((size_t)kSegments * (kLutBins + 1) * sizeof(int32_t));
+            pd.dev_iso = dev((size_t)t->iso.size() * sizeof(float2));
+            pd.off_l1 = e->off[0];
+            pd.off_l2 = e->off[1];
+            pd.off_singles = e->off[2];
+            pd.off_pairs_t = e->off[3];
+            pd.off_iso_t = e->off[4];
+            pd.off_carr = at(st.pd.off_carr);
+            c->bprep = true;
+            FitSet f = st.f;
+            f.first_tile = c->bntiles_total;
+            c->bntiles_total += st.ntiles;
+            c->bsets.push_back(f);
+            c->bntiles.push_back(st.ntiles);
+            c->bpend.push_back(pd);
+            continue;
+        }
         pd.off_carr = at(st.pd.off_carr);
         pd.off_l1 = at(st.pd.off_l1);
         pd.off_l2 = at(st.pd.off_l2);
@@ -2852,8 +3327,8 @@ int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int t
         pd.off_singles = at(st.pd.off_singles);
         pd.off_iso = at(st.pd.off_iso);
         FitSet f = st.f;
-        f.first_tile = (int)c->btiles.size();
-        for (int k = 0; k < st.ntiles; ++k) c->btiles.push_back(make_int2((int)c->bsets.size(), k));
+        f.first_tile = c->bntiles_total;
+        c->bntiles_total += st.ntiles;
         c->bsets.push_back(f);
         c->bntiles.push_back(st.ntiles);
         c->bpend.push_back(pd);
@@ -2872,9 +3347,12 @@ int csx_fit_batch_begin(csx_ctx* c) {
     }
     c->bsets.clear();
     c->bntiles.clear();
-    c->btiles.clear();
+    c->bntiles_total = 0;
     c->barena.clear();
     c->bpend.clear();
+    c->btmpl.clear();
+    c->bdev_size = 0;
+    c->bprep = false;
     return CSX_OK;
 }
 
@@ -2896,30 +3374,49 @@ int csx_fit_batch_run(csx_ctx* c, int precision, double* h_sums, int64_t* h_cnts
 
 }  // extern "C"
 
-// Enqueue the staged batch on the context's stream without waiting: the set records, the
-// tile list and the per-set tile counts are appended to the pinned arena, which goes over in
-// one copy; then the batched kernel, the per-set reduction and the copy of the sums into
-// pinned host memory.  The arena must not be touched until csx_fit_batch_wait.
-// The staged batch's set records, tile list and per-set tile counts appended to the pinned
-// arena, with the device addresses of every set's tables filled in; one H2D copy on the
-// context's stream.  Returns the device addresses of the records and the tile list.
+// The staged batch's set records and per-set tile counts appended to the pinned arena, with
+// the device addresses of every set's tables filled in; one H2D copy on the context's stream
+// (or a node of the round's graph).  Returns the device addresses of the records, of the flat
+// tile list (written on the device by k_prep_sets) and of the tile counts.
 static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles, const int** d_ntiles,
                         size_t* bytes = nullptr) {
     const int nsets = (int)c->bsets.size();
-    const int ntiles = (int)c->btiles.size();
+    const int ntiles = c->bntiles_total;
     auto al = [](size_t o) { return (o + 255) & ~size_t(255); };
     const size_t o_sets = al(c->barena.size);
-    const size_t o_tiles = al(o_sets + nsets * sizeof(FitSet));
-    const size_t o_ntiles = al(o_tiles + ntiles * sizeof(int2));
+    const size_t o_ntiles = al(o_sets + nsets * sizeof(FitSet));
     const size_t total = o_ntiles + nsets * sizeof(int);
     HIPCHK(c, c->barena_d.ensure(total));
+    HIPCHK(c, c->btiles_d.ensure(std::max<size_t>(1, (size_t)ntiles) * sizeof(int2)));
     HIPCHK(c, c->barena.reserve(total));
+    if (c->bprep) HIPCHK(c, c->bdev.ensure(std::max<size_t>(c->bdev_size, 256)));
     char* base = (char*)c->barena_d.p;
+    char* dbase = (char*)c->bdev.p;
     for (int i = 0; i < nsets; ++i) {
         FitSet& f = c->bsets[i];
         const auto& pd = c->bpend[i];
         f.d.carr = (const double*)(base + pd.off_carr);
-        if ((f.d.s.combo_key >> 8 & 0xff) != 12) {
+        if (pd.dev_l3 != SIZE_MAX) {
+            // tables the device builds (k_prep_sets), lists of the lattice's template
+            PrepIn& pin = f.pin;
+            pin.l2 = (const int32_t*)(base + pd.off_l2);
+            pin.pairs_t = (const int2*)(base + pd.off_pairs_t);
+            pin.iso_t = (const int2*)(base + pd.off_iso_t);
+            pin.l3 = (float*)(dbase + pd.dev_l3);
+            const bool precise = precise_kind(kind_of(f.d.s.combo_key));
+            pin.pr = precise ? nullptr : (PairRec*)(dbase + pd.dev_pairs);
+            pin.pr64 = precise ? (PairRec64*)(dbase + pd.dev_pairs) : nullptr;
+            pin.lut = (int32_t*)(dbase + pd.dev_lut);
+            pin.iso = (float2*)(dbase + pd.dev_iso);
+            f.d.latpar1 = (const int32_t*)(base + pd.off_l1);
+            f.d.latpar2 = pin.l2;
+            f.d.latpar3 = pin.l3;
+            f.d.pairs = pin.pr;
+            f.d.pairs64 = pin.pr64;
+            f.d.lut = pin.lut;
+            f.d.singles = (const int32_t*)(base + pd.off_singles);
+            f.d.iso = pin.iso;
+        } else if ((f.d.s.combo_key >> 8 & 0xff) != 12) {
             f.d.latpar1 = (const int32_t*)(base + pd.off_l1);
             f.d.latpar2 = (const int32_t*)(base + pd.off_l2);
             f.d.latpar3 = (const float*)(base + pd.off_l3);
@@ -2931,14 +3428,14 @@ static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles,
         }
     }
     std::memcpy(c->barena.p + o_sets, c->bsets.data(), nsets * sizeof(FitSet));
-    std::memcpy(c->barena.p + o_tiles, c->btiles.data(), ntiles * sizeof(int2));
     std::memcpy(c->barena.p + o_ntiles, c->bntiles.data(), nsets * sizeof(int));
     c->barena.size = total;
     if (bytes) *bytes = total;  // the caller copies (a node of the round's graph)
     else HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
     *d_sets = (const FitSet*)(base + o_sets);
-    *d_tiles = (const int2*)(base + o_tiles);
+    *d_tiles = (const int2*)c->btiles_d.p;  // filled by k_prep_sets
     *d_ntiles = (const int*)(base + o_ntiles);
+    c->bsets_d = *d_sets;
     return CSX_OK;
 }
 
@@ -2951,8 +3448,17 @@ static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, cons
     BatchGraph& G = c->bgraph;
     TileSum* part = (TileSum*)c->bpart_d.p;
     SetSum* sums = (SetSum*)c->bsums_d.p;
+    FitSet* d_sets_w = const_cast<FitSet*>(d_sets);
+    int2* d_tiles_w = const_cast<int2*>(d_tiles);
+    void* prep_args[] = {(void*)&d_sets_w, (void*)&nsets, (void*)&d_ntiles, (void*)&d_tiles_w};
     void* fit_args[] = {(void*)&d_sets, (void*)&d_tiles, (void*)&ntiles, (void*)&part};
     void* red_args[] = {(void*)&d_sets, (void*)&nsets, (void*)&part, (void*)&d_ntiles, (void*)&sums};
+    // per set: its tile-list entries and, for sets staged with device preparation, its tables
+    hipKernelNodeParams kp{};
+    kp.func = (void*)k_prep_sets;
+    kp.gridDim = dim3(nsets);
+    kp.blockDim = dim3(kPrepBlock);
+    kp.kernelParams = prep_args;
     hipKernelNodeParams kf{};
     kf.func = const_cast<void*>(fit_fn);
     kf.gridDim = dim3(ntiles);
@@ -2972,7 +3478,8 @@ static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, cons
         HIPCHK(c, hipGraphCreate(&G.graph, 0));
         HIPCHK(c, hipGraphAddMemcpyNode1D(&G.h2d, G.graph, nullptr, 0, c->barena_d.p, c->barena.p, bytes,
                                           hipMemcpyHostToDevice));
-        HIPCHK(c, hipGraphAddKernelNode(&G.fit, G.graph, &G.h2d, 1, &kf));
+        HIPCHK(c, hipGraphAddKernelNode(&G.prep, G.graph, &G.h2d, 1, &kp));
+        HIPCHK(c, hipGraphAddKernelNode(&G.fit, G.graph, &G.prep, 1, &kf));
         HIPCHK(c, hipGraphAddKernelNode(&G.red, G.graph, &G.fit, 1, &kr));
         HIPCHK(c, hipGraphAddMemcpyNode1D(&G.d2h, G.graph, &G.red, 1, c->bres.p, sums, sbytes, hipMemcpyDeviceToHost));
         HIPCHK(c, hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
@@ -2980,6 +3487,7 @@ static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, cons
     } else {
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G.exec, G.h2d, c->barena_d.p, c->barena.p, bytes,
                                                     hipMemcpyHostToDevice));
+        HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.prep, &kp));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.fit, &kf));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.red, &kr));
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G.exec, G.d2h, c->bres.p, sums, sbytes, hipMemcpyDeviceToHost));
@@ -2994,7 +3502,7 @@ int csx_fit_batch_launch(csx_ctx* c, int precision) {
     const int nsets = (int)c->bsets.size();
     if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
     HIPCHK(c, hipSetDevice(c->device));
-    const int ntiles = (int)c->btiles.size();
+    const int ntiles = c->bntiles_total;
     const FitSet* d_sets;
     const int2* d_tiles;
     const int* d_ntiles;
@@ -3012,6 +3520,9 @@ int csx_fit_batch_launch(csx_ctx* c, int precision) {
         if (int rc = batch_graph_launch(c, fn, bytes, d_sets, d_tiles, d_ntiles, nsets, ntiles)) return rc;
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     } else {
+        hipLaunchKernelGGL(k_prep_sets, dim3(nsets), dim3(kPrepBlock), 0, c->stream, const_cast<FitSet*>(d_sets), nsets,
+                           d_ntiles, const_cast<int2*>(d_tiles));
+        HIPCHK(c, hipGetLastError());
         DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, d_tiles, ntiles, (TileSum*)c->bpart_d.p, variant,
                       c->stream);
         HIPCHK(c, hipGetLastError());
@@ -3036,12 +3547,70 @@ int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts) {
     for (int i = 0; i < nsets; ++i) h_sums[i] = r[i].sum;
     if (h_cnts) for (int i = 0; i < nsets; ++i) h_cnts[i] = r[i].cnt;
     c->bres.size = 0;
+    bool mismatch = false;
+    for (int i = 0; i < nsets; ++i)
+        if (c->bsets[i].pin.on) {
+            ++c->prep_sets;
+            c->prep_amb += r[i].flags >> 8;
+            mismatch = mismatch || (r[i].flags & 1);
+        }
+    if (mismatch) return fail(c, CSX_E_STATE, "device preparation: a Friedel pair's members differ");
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     c->timing_pending = false;
     return CSX_OK;
 }
+
+extern "C" {
+
+int csx_set_device_prep(csx_ctx* c, int on) {
+    if (!c || (on != 0 && on != 1)) return CSX_E_ARG;
+    c->device_prep = on != 0;
+    return CSX_OK;
+}
+
+int csx_device_prep_counts(const csx_ctx* c, long long* sets, long long* ambiguous) {
+    if (!c) return CSX_E_ARG;
+    if (sets) *sets = c->prep_sets;
+    if (ambiguous) *ambiguous = c->prep_amb;
+    return CSX_OK;
+}
+
+// A table of set `set` of the last uploaded batch, from device memory (after its run).
+int csx_fit_batch_table(csx_ctx* c, int set, int which, void* out, size_t bytes, size_t* needed) {
+    if (!c || set < 0 || set >= (int)c->bsets.size() || !c->bsets_d || which < 0 || which > 5) return CSX_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    FitSet f;
+    HIPCHK(c, hipMemcpy(&f, c->bsets_d + set, sizeof f, hipMemcpyDeviceToHost));
+    const DevSet& d = f.d;
+    if ((d.s.combo_key >> 8 & 0xff) == 12) return fail(c, CSX_E_ARG, "not a lattice set");
+    const void* src = nullptr;
+    size_t n = 0;
+    double scal[5];
+    switch (which) {
+    case 0: src = d.latpar3; n = (size_t)(d.s.peakmax2 + 1) * 17 * sizeof(float); break;
+    case 1: src = d.pairs; n = d.pairs ? (size_t)d.npairs * sizeof(PairRec) : 0; break;
+    case 2: src = d.pairs64; n = d.pairs64 ? (size_t)d.npairs * sizeof(PairRec64) : 0; break;
+    case 3: src = d.lut; n = (size_t)kSegments * (d.nlut + 1) * sizeof(int32_t); break;
+    case 4: src = d.iso; n = (size_t)d.niso * sizeof(float2); break;
+    case 5:
+        scal[0] = d.npairs; scal[1] = d.lut_inv_dq; scal[2] = d.qh_min; scal[3] = d.qh_max; scal[4] = d.asum2;
+        n = sizeof scal;
+        break;
+    }
+    if (needed) *needed = n;
+    if (!out) return CSX_OK;
+    if (bytes < n) return fail(c, CSX_E_ARG, "output buffer too small");
+    if (which == 5) std::memcpy(out, scal, n);
+    else if (n) HIPCHK(c, hipMemcpy(out, src, n, hipMemcpyDeviceToHost));
+    return CSX_OK;
+}
+
+}  // extern "C"
+
+int csx_ctx_device_prep(const csx_ctx* c) { return c && c->device_prep ? 1 : 0; }
 
 extern "C" int csx_render_batch_run(csx_ctx* c, int precision) {
     if (!c) return CSX_E_ARG;
@@ -3051,17 +3620,32 @@ extern "C" int csx_render_batch_run(csx_ctx* c, int precision) {
     for (const FitSet& f : c->bsets) {
         if (f.h != f.d.s.ymax - f.d.s.ymin) return fail(c, CSX_E_ARG, "a batched image must cover its whole grid");
     }
-    const int ntiles = (int)c->btiles.size();
+    const int ntiles = c->bntiles_total;
     const FitSet* d_sets;
     const int2* d_tiles;
     const int* d_ntiles;
     if (int rc = batch_upload(c, &d_sets, &d_tiles, &d_ntiles)) return rc;
     const int kind = kind_of(c->bsets[0].d.s.combo_key);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_prep_sets, dim3(nsets), dim3(kPrepBlock), 0, c->stream, const_cast<FitSet*>(d_sets), nsets,
+                       d_ntiles, const_cast<int2*>(d_tiles));
+    HIPCHK(c, hipGetLastError());
     DISPATCH_KIND(kind, precision, launch_image_batch, d_sets, d_tiles, ntiles, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->bprep) {
+        std::vector<FitSet> back(nsets);
+        HIPCHK(c, hipMemcpy(back.data(), d_sets, nsets * sizeof(FitSet), hipMemcpyDeviceToHost));
+        bool mismatch = false;
+        for (const FitSet& f : back)
+            if (f.pin.on) {
+                ++c->prep_sets;
+                c->prep_amb += f.pin.flags >> 8;
+                mismatch = mismatch || (f.pin.flags & 1);
+            }
+        if (mismatch) return fail(c, CSX_E_STATE, "device preparation: a Friedel pair's members differ");
+    }
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
@@ -3080,5 +3664,11 @@ void* csx_ctx_scratch(csx_ctx* c, int which, size_t bytes) {
 // A second context on the same device (its own stream and staging memory).
 int csx_create_like(const csx_ctx* c, csx_ctx** out) {
     if (!c || !out) return CSX_E_ARG;
-    return csx_create(c->device, out);
+    const int rc = csx_create(c->device, out);
+    if (rc == CSX_OK) {  // the same batch settings
+        (*out)->tune_force = c->tune_force;
+        (*out)->use_graph = c->use_graph;
+        (*out)->device_prep = c->device_prep;
+    }
+    return rc;
 }

@@ -447,14 +447,21 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
     csx_mask m{};
     for (int i = 0; i < 16; ++i) m.rects[i] = -1;
     std::atomic<int> next{0};
+    // lattice sets: the device builds the reflection tables (csx_set_device_prep)
+    const bool dev = csx_ctx_device_prep(ctx) != 0;
     auto work = [&] {
         for (int k = next++; k < n; k = next++) {
             const csx::ParamSet& ps = sets[(size_t)k];
+            const bool rows = dev && ps.ltype != 12;
             std::string msg = csx::check_parameters(ps, e->ignore_cbs);
-            if (msg.empty()) msg = csx::prepare(ps, preps[(size_t)k]);
+            if (msg.empty()) msg = csx::prepare(ps, preps[(size_t)k], rows);
             if (!msg.empty()) { errs[(size_t)k] = msg; continue; }
             const csx::PreparedSet& P = preps[(size_t)k];
             const int w = P.s.xmax - P.s.xmin, h = P.s.ymax - P.s.ymin;
+            if (rows) {
+                csx_stage_build_rows(st[(size_t)k].get(), P, w, h, &m, d_out + (size_t)k * stride);
+                continue;
+            }
             csx_stage_build(st[(size_t)k].get(), &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
                             P.latpar2.empty() ? nullptr : P.latpar2.data(),
                             P.latpar3.empty() ? nullptr : P.latpar3.data(), w, h, &m, d_out + (size_t)k * stride);

@@ -400,13 +400,20 @@ public:
         errs_.assign(n, std::string());
         preps_.resize(std::max(preps_.size(), (size_t)n));
         const auto t0 = clk::now();
+        // lattice sets: the device builds the reflection tables (csx_set_device_prep)
+        const bool dev = csx_ctx_device_prep(ctx_) != 0;
         parallel_for(n, threads_, [&](int r) {
             csx::ParamSet ps = *base_[pts[r].fit];
             for (int j = 0; j < npar_; ++j) set_tied(ps, keys_[j], pts[r].x[j]);
             csx::PreparedSet& P = preps_[r];
-            errs_[r] = csx::prepare(ps, P);
+            const bool rows = dev && ps.ltype != 12;
+            errs_[r] = csx::prepare(ps, P, rows);
             if (!errs_[r].empty()) return;
             const csx_nm_fit& F = fits_[pts[r].fit];
+            if (rows) {
+                csx_stage_build_rows(slots_[r]->p, P, F.w, F.h, &F.mask, F.d_data);
+                return;
+            }
             csx_stage_build(slots_[r]->p, &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
                             P.latpar2.empty() ? nullptr : P.latpar2.data(),
                             P.latpar3.empty() ? nullptr : P.latpar3.data(), F.w, F.h, &F.mask, F.d_data);
@@ -422,17 +429,20 @@ public:
                 ok_.push_back(r);
             }
         n_ = n;
+        auto t2 = clk::now();
         if (!built.empty()) {
             int rc = csx_fit_batch_begin(ctx_);
             if (rc) return rc;
             rc = csx_stage_commit_many(ctx_, (int)built.size(), built.data(), threads_);
             if (rc) return rc;
+            t2 = clk::now();
             rc = csx_fit_batch_launch(ctx_, prec_);
             if (rc) return rc;
         }
-        const auto t2 = clk::now();
+        const auto t3 = clk::now();
         t_prep += ms(t0, t1);
         t_stage += ms(t1, t2);
+        t_launch += ms(t2, t3);
         return CSX_OK;
     }
 
@@ -450,7 +460,7 @@ public:
         return rc;
     }
 
-    double t_prep = 0, t_stage = 0, t_run = 0;
+    double t_prep = 0, t_stage = 0, t_launch = 0, t_run = 0;
 
 private:
     struct Slot {
@@ -592,7 +602,7 @@ int csx_nm_batch_screened(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int n
             G.inflight = true;
         }
     }
-    double tp = 0, ts = 0, tr = 0;
+    double tp = 0, ts = 0, tl = 0, tr = 0;
     for (auto& G : gs) {
         if (G.inflight) {  // an error left a launch in flight: drain it
             std::vector<double> got(G.exact.size());
@@ -600,6 +610,7 @@ int csx_nm_batch_screened(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int n
         }
         tp += G.ev->t_prep;
         ts += G.ev->t_stage;
+        tl += G.ev->t_launch;
         tr += G.ev->t_run;
     }
     gs.clear();
@@ -614,6 +625,7 @@ int csx_nm_batch_screened(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int n
         stats->run_ms = tr;
         stats->pruned = pruned;
         stats->screen_ms = screen_ms;
+        stats->launch_ms = tl;
     }
     return CSX_OK;
 }

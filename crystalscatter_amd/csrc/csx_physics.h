@@ -57,7 +57,7 @@ struct alignas(32) Radial {
 
 struct DevSet {
     csx_scalars s;
-    const double* carr;      // 18 x CSX_NCOEF
+    const double* carr;      // 4 x kCarrLen: carr1p, carr4p, carr1f, carr4f (carr_of)
     const double* carr11pm;  // CSX_NIM2^2 or null
     const int32_t* latpar1;  // (peakmax1+1) x 6
     const int32_t* latpar2;  // (peakmax2+1) x 6
@@ -99,8 +99,15 @@ struct DevSet {
     float cq_lz;             // L/(z_L+1)
 };
 
-__device__ __forceinline__ const double* carr_of(const DevSet& d, int which) {
-    return d.carr + which * CSX_NCOEF;  // 0..8 = carr1p..9p, 9..17 = carr1f..9f
+// The coefficient series the pixel bodies read, carr1p, carr4p, carr1f and carr4f, entries
+// 0..120 (the longest series sums 120 terms), gathered from the host's 18 x CSX_NCOEF block
+// (carr1p..carr9p, carr1f..carr9f) when a set is uploaded or staged.
+constexpr int kCarrLen = 121;
+constexpr int kCarrArrays[4] = {0, 3, 9, 12};
+template <int W>
+__device__ __forceinline__ const double* carr_of(const DevSet& d) {
+    static_assert(W == 0 || W == 3 || W == 9 || W == 12, "coefficient series not kept on the device");
+    return d.carr + (W == 0 ? 0 : W == 3 ? 1 : W == 9 ? 2 : 3) * kCarrLen;
 }
 
 __device__ __forceinline__ double sqd(double x) { return x * x; }
@@ -159,7 +166,7 @@ __device__ __forceinline__ double series_proj(const double* c, double p, int nma
 __device__ inline double pq_sphere(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     double zr = (1 - sqd(s.sigma)) / (sqd(s.sigma));
-    if (q < 0.4 * s.limq[3]) return series_x2(carr_of(d, 3), q, 100);
+    if (q < 0.4 * s.limq[3]) return series_x2(carr_of<3>(d), q, 100);
     const double argq = q * s.radius / (zr + 1);
     if (zr * (zr - 1) * (zr - 2) * (zr - 3) * (zr - 4) * (zr - 5) == 0) zr += 0.001;
     const double pqr = (1 / (2.0 * zr * (zr - 1) * (zr - 2) * (zr - 3))) * pow(argq, -4);
@@ -178,7 +185,7 @@ __device__ inline double pq_sphere(const DevSet& d, double q) {
 __device__ inline double pq_cube(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     if (s.cs != 0) return 1.0;
-    if (q < 1.2 * s.limq[3]) return series_x2(carr_of(d, 3), q, 120);
+    if (q < 1.2 * s.limq[3]) return series_x2(carr_of<3>(d), q, 120);
     return s.por / (q * q * q * q);
 }
 
@@ -218,7 +225,7 @@ __device__ inline double pq_cube_zdir(const DevSet& d, double qxs, double qys, d
 __device__ inline double fq_sphere(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
-    if (q < (0.4 * s.limq[12])) return series_x2(carr_of(d, 12), q, 100);
+    if (q < (0.4 * s.limq[12])) return series_x2(carr_of<12>(d), q, 100);
     const double argq = q * s.radius / (zr + 1);
     const double pqr = (1 / (zr * (zr - 1)) * pow(argq, -2));
     const double at = atan(argq), b = 1.0 + argq * argq;
@@ -236,7 +243,7 @@ __device__ inline double fq_sphere(const DevSet& d, double q) {
 // the reference's evaluation are rounding-level (phases agree to ~1e-14 rad).
 __device__ inline double pq_sphere_fast(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
-    if (q < 0.4 * s.limq[3]) return series_x2(carr_of(d, 3), q, 100);
+    if (q < 0.4 * s.limq[3]) return series_x2(carr_of<3>(d), q, 100);
     const double zr = d.sph_zr;
     const double argq = q * d.sph_arg;
     const double ia = 1.0 / argq, ia2 = ia * ia;
@@ -260,7 +267,7 @@ __device__ inline double pq_sphere_fast(const DevSet& d, double q) {
 // at (zr-2)*at (at = atan(argq), cos(at) = 1/sqrt(b), sin(at) = argq/sqrt(b)).
 __device__ inline double fq_sphere_fast(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
-    if (q < (0.4 * s.limq[12])) return series_x2(carr_of(d, 12), q, 100);
+    if (q < (0.4 * s.limq[12])) return series_x2(carr_of<12>(d), q, 100);
     const double zr = d.sfq_zr;
     const double argq = q * d.sfq_arg;
     const double ia = 1.0 / argq;
@@ -292,7 +299,7 @@ __device__ inline double pq_cyl_transverse(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     const double zr = s.sigma == 1.0 ? 0.000000001 : (1 - sqd(s.sigma)) / (sqd(s.sigma));
     if ((s.length / s.radius) < 2) return 1;
-    if (q < (0.65 * s.limq[3])) return series_x2(carr_of(d, 3), q, 120);
+    if (q < (0.65 * s.limq[3])) return series_x2(carr_of<3>(d), q, 120);
     const double argpq = q * s.radius / (zr + 1);
     const double at = atan(2.0 * argpq), b = 1.0 + 4 * argpq * argpq;
     const double pqr1 = (1 / (zr * (zr - 1) * (zr - 2))) * pow(argpq, -3);
@@ -307,7 +314,7 @@ __device__ inline double pq_cyl_transverse(const DevSet& d, double q) {
 __device__ inline double pq_cyl_long_iso(const DevSet& d, double q) {
     const csx_scalars& s = d.s;
     const double zl = s.sigmal == 1.0 ? 0.000000001 : (1 - sqd(s.sigmal)) / sqd(s.sigmal);
-    if (q < (0.6 * s.limq[0])) return series_x2(carr_of(d, 0), q, 120);
+    if (q < (0.6 * s.limq[0])) return series_x2(carr_of<0>(d), q, 120);
     const double arglq = q * s.length / (zl + 1);
     double pql = (kPiD / (2.0 * zl)) * (1 / arglq);
     pql = pql - (1 / (2.0 * zl * (zl - 1) * arglq * arglq)) * cos((zl - 1) * atan(2.0 * arglq)) /
@@ -320,7 +327,7 @@ __device__ inline double pq_cyl_long_zdir(const DevSet& d, double qxs, double qy
     const csx_scalars& s = d.s;
     if (orc_cmp(s) == 4) return 1.0;
     const double zl = s.sigmal == 1.0 ? 0.000000001 : (1 - sqd(s.sigmal)) / sqd(s.sigmal);
-    if (q < (0.6 * s.limq[0])) return series_proj(carr_of(d, 0), qxs + qys, 120);
+    if (q < (0.6 * s.limq[0])) return series_proj(carr_of<0>(d), qxs + qys, 120);
     return rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
 }
 
@@ -331,11 +338,11 @@ __device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, 
     const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
     double pql;
     if (orc_cmp(s) == 4) pql = 1.0;
-    else if (limql < (0.6 * s.limq[9])) pql = series_proj(carr_of(d, 9), qxs + qys, 120);
+    else if (limql < (0.6 * s.limq[9])) pql = series_proj(carr_of<9>(d), qxs + qys, 120);
     else pql = rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
     double pqr;
     if (q < (1.0 * s.limq[12])) {
-        pqr = series_x2(carr_of(d, 12), q, 120);
+        pqr = series_x2(carr_of<12>(d), q, 120);
     } else {
         const double argpq = q * s.radius / (zr + 1);
         const double at = atan(argpq), b = 1.0 + argpq * argpq;

@@ -5,10 +5,12 @@
 // rotation entries, float reflection table); structure and names are this
 // project's own.
 #include "csx_prep.h"
+#include "csx_rows.h"
 
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 
 namespace csx {
 
@@ -690,21 +692,60 @@ double ref_gamma(double z) {
 }
 
 double gauss_norm3(double a) {
-    const double p2 = kPi * kPi;
-    if (a > 0.01) {
-        const double ex = std::exp(-a * kPi * kPi);
-        const double b0 = (1 - std::exp(-a * kPi * kPi)) / 2.;
-        const double b1 = (1 - ex * (1 + a * kPi * kPi)) / 12.;
-        const double b2 = (2 - ex * (2 + 2 * a * kPi * kPi + a * a * kPi * kPi * kPi * kPi)) / 240.;
-        const double b3 = (6 - ex * (6 + 6 * a * kPi * kPi + 3 * a * a * kPi * kPi * kPi * kPi +
-                                     a * a * a * kPi * kPi * kPi * kPi * kPi * kPi)) / 10080.;
-        return b0 / a - b1 / (a * a) + b2 / (a * a * a) - b3 / (a * a * a * a);
+    return rows::gauss_norm3_from(a, rows::gauss_norm3_uses_exp(a) ? std::exp(rows::gauss_norm3_exp_arg(a)) : 0.0);
+}
+
+// Reflection lists of one lattice type and HKLmax (ButtonHKLClick :8246-10296, fhkl_c
+// :10299-10500): every allowed (h, k, l) of the isotropic octant and of the whole box, its
+// multiplicity and rounded structure factor, in latpar1/latpar2 layout.  Built once per
+// (ltype, HKLmax) in the process and shared.
+std::shared_ptr<const LatticeLists> lattice_lists(int ltype, int hklmax) {
+    static std::mutex mu;
+    static std::vector<std::shared_ptr<const LatticeLists>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& c : cache)
+        if (c->ltype == ltype && c->hklmax == hklmax) return c;
+    auto L = std::make_shared<LatticeLists>();
+    L->ltype = ltype;
+    L->hklmax = hklmax;
+    const bool cubic = ltype != 8;
+    const int H = hklmax;
+    std::vector<int> v[2];
+    std::vector<double> f[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        const int lo = pass == 0 ? 0 : -H;
+        for (int h = lo; h <= H; ++h)
+            for (int k = lo; k <= H; ++k)
+                for (int l = lo; l <= H; ++l)
+                    if (allowed(ltype, h, k, l)) {
+                        const int pat = pattern(h, k, l);
+                        v[pass].push_back(h); v[pass].push_back(k); v[pass].push_back(l);
+                        v[pass].push_back(cubic ? kMultCubic[pat] : kMultTetra[pat]);
+                        double sph;
+                        f[pass].push_back(structure_factor(ltype, h, k, l, sph));
+                        (pass == 0 ? L->s1 : L->s2).push_back(sph);
+                    }
     }
-    (void)p2;
-    const double a1 = (kPi * kPi - 4);
-    const double a2 = (18 - 12 * kPi * kPi + kPi * kPi * kPi * kPi) / 2.;
-    const double a3 = (-1440 + 360 * kPi * kPi - 30 * kPi * kPi * kPi * kPi + kPi * kPi * kPi * kPi * kPi * kPi) / 6.;
-    return 2 - a1 * a + a2 * a * a - a3 * a * a * a;
+    const int n1 = static_cast<int>(v[0].size() / 4), n2 = static_cast<int>(v[1].size() / 4);
+    L->n1 = n1;
+    L->n2 = n2;
+    L->l1.assign(static_cast<size_t>(n1 + 1) * 6, 0);
+    L->l2.assign(static_cast<size_t>(n2 + 2) * 6, 0);
+    L->l1[6] = n1;
+    L->l2[6] = n2;
+    for (int i = 1; i <= n1; ++i) {
+        int32_t* r = &L->l1[static_cast<size_t>(i) * 6];
+        for (int c = 0; c < 4; ++c) r[1 + c] = v[0][4 * (i - 1) + c];
+        r[5] = static_cast<int32_t>(std::round(f[0][static_cast<size_t>(i - 1)]));
+    }
+    for (int i = 1; i <= n2; ++i) {
+        int32_t* r = &L->l2[static_cast<size_t>(i) * 6];
+        for (int c = 0; c < 3; ++c) r[1 + c] = v[1][4 * (i - 1) + c];
+        r[4] = 1;
+        r[5] = static_cast<int32_t>(std::round(f[1][static_cast<size_t>(i - 1)]));
+    }
+    cache.push_back(L);
+    return L;
 }
 
 std::string check_parameters(const ParamSet& p, bool ignore_cbs) {
@@ -726,7 +767,7 @@ std::string check_parameters(const ParamSet& p, bool ignore_cbs) {
     return "";
 }
 
-std::string prepare(const ParamSet& p, PreparedSet& P) {
+std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
     csx_scalars& s = P.s;
     std::memset(&s, 0, sizeof(s));
     s.combo_key = p.combo_key();
@@ -859,88 +900,49 @@ std::string prepare(const ParamSet& p, PreparedSet& P) {
         if (p.ltype != 4 && p.ltype != 5 && p.ltype != 7 && p.ltype != 8)
             return "lattice type not supported by the device path";
         // The reflection lists, multiplicities and structure factors depend only on the
-        // lattice type and HKLmax: built once per thread and kept while those stay the same
-        // (a fit's vertices and a training set's images change only the lattice constants).
-        struct HklLists {
-            int ltype = -1, H = -1;
-            std::vector<int> iso, ani;      // h, k, l, multiplicity
-            std::vector<double> f1, f2;     // structure factor per entry
-            std::vector<double> s1, s2;     // its sphno
-        };
-        static thread_local HklLists hl;
-        const int H = p.hklmax;
-        if (hl.ltype != p.ltype || hl.H != H) {
-            const bool cubic = p.ltype != 8;
-            hl = HklLists{};
-            for (int pass = 0; pass < 2; ++pass) {
-                const int lo = pass == 0 ? 0 : -H;
-                auto& v = pass == 0 ? hl.iso : hl.ani;
-                auto& f = pass == 0 ? hl.f1 : hl.f2;
-                auto& sp = pass == 0 ? hl.s1 : hl.s2;
-                for (int h = lo; h <= H; ++h)
-                    for (int k = lo; k <= H; ++k)
-                        for (int l = lo; l <= H; ++l)
-                            if (allowed(p.ltype, h, k, l)) {
-                                const int pat = pattern(h, k, l);
-                                v.push_back(h); v.push_back(k); v.push_back(l);
-                                v.push_back(cubic ? kMultCubic[pat] : kMultTetra[pat]);
-                                double sph;
-                                f.push_back(structure_factor(p.ltype, h, k, l, sph));
-                                sp.push_back(sph);
-                            }
-            }
-            hl.ltype = p.ltype;
-            hl.H = H;
-        }
-        const std::vector<int>& iso = hl.iso;
-        const std::vector<int>& ani = hl.ani;
-        const int n1 = static_cast<int>(iso.size() / 4), n2 = static_cast<int>(ani.size() / 4);
+        // lattice type and HKLmax: shared by every set of that lattice (a fit's vertices and a
+        // training set's images change only the lattice constants).
+        static thread_local std::shared_ptr<const LatticeLists> tl;
+        if (!tl || tl->ltype != p.ltype || tl->hklmax != p.hklmax) tl = lattice_lists(p.ltype, p.hklmax);
+        const LatticeLists& L = *tl;
+        const int n1 = L.n1, n2 = L.n2;
         if (n1 >= CSX_LATLEN || n2 >= CSX_LATLEN) return "reflection table exceeds latparlen";
         s.peakmax1 = n1;
         s.peakmax2 = n2;
-        P.latpar1.assign(static_cast<size_t>(n1 + 1) * 6, 0);
-        P.latpar2.assign(static_cast<size_t>(n2 + 2) * 6, 0);
-        P.latpar3.assign(static_cast<size_t>(n2 + 1) * 17, 0.f);
-        P.latpar1[6] = n1;
-        P.latpar2[6] = n2;
         double R[9], RT[9], vol;
         rotated_reciprocal_basis(p.uca, p.ucb, p.ucc, p.ucalpha, p.ucbeta, p.ucgamma, p.ucn1, p.ucn2,
                                  p.ucn3, p.ucpsi, R, RT, vol);
         P.ucvol = vol;
-        for (int i = 1; i <= n1; ++i) {
-            int* L = &P.latpar1[static_cast<size_t>(i) * 6];
-            L[1] = iso[4 * (i - 1)]; L[2] = iso[4 * (i - 1) + 1]; L[3] = iso[4 * (i - 1) + 2]; L[4] = iso[4 * (i - 1) + 3];
-            L[5] = static_cast<int>(std::round(hl.f1[static_cast<size_t>(i - 1)]));
-            lat_vol = P.ucvol;
-            lat_sph = hl.s1[static_cast<size_t>(i - 1)];
-        }
-        if (n1 == 0) { lat_vol = P.ucvol; lat_sph = 1; }
-        for (int i = 1; i <= n2; ++i) {
-            int* L = &P.latpar2[static_cast<size_t>(i) * 6];
-            const int h = ani[4 * (i - 1)], k = ani[4 * (i - 1) + 1], l = ani[4 * (i - 1) + 2];
-            L[1] = h; L[2] = k; L[3] = l;
-            L[5] = static_cast<int>(std::round(hl.f2[static_cast<size_t>(i - 1)]));
-            L[4] = 1;
-            P.sphno = hl.s2[static_cast<size_t>(i - 1)];
-            float* T = &P.latpar3[static_cast<size_t>(i) * 17];
-            T[1] = 2 * kPi / p.uca;
-            const double qx = (2 * kPi) * (R[0] * h + R[3] * k + R[6] * l);
-            const double qy = (2 * kPi) * (R[1] * h + R[4] * k + R[7] * l);
-            const double qz = (2 * kPi) * (R[2] * h + R[5] * k + R[8] * l);
-            const double qxy = (2 * kPi) * std::sqrt(h * h * (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]) +
-                                                    k * k * (R[3] * R[3] + R[4] * R[4] + R[5] * R[5]));
-            const double qh = std::sqrt(qx * qx + qy * qy + qz * qz);
-            T[2] = qx; T[3] = qy; T[4] = qz; T[5] = qh; T[6] = qh / (2.0 * kPi);
-            T[7] = (2 * kPi) * (RT[0] * h + RT[3] * k + RT[6] * l);
-            T[8] = (2 * kPi) * (RT[1] * h + RT[4] * k + RT[7] * l);
-            T[9] = (2 * kPi) * (RT[2] * h + RT[5] * k + RT[8] * l);
-            const double g3 = 4 * kPi * kPi / (2.0 * kPi * qh * qh);
-            const double x2phi = 4 * qh * qh / (kPi * s.phiwidth * s.phiwidth);
-            const int shp = p.peakidx + 1;
-            T[10] = g3;
-            T[11] = (shp == 2 || shp == 7) ? gauss_norm3(x2phi) : 0.0;
-            T[12] = 0;
-            T[13] = qxy;
+        lat_vol = P.ucvol;
+        lat_sph = n1 > 0 ? L.s1[static_cast<size_t>(n1 - 1)] : 1;
+        if (n2 > 0) P.sphno = L.s2[static_cast<size_t>(n2 - 1)];
+        P.lists = tl;
+        std::memcpy(P.rot, R, sizeof R);
+        std::memcpy(P.rott, RT, sizeof RT);
+        const int shp = p.peakidx + 1;
+        if (rows_on_device) {
+            // the caller's device builds latpar3 from rot/rott (csx_rows.h); latpar1/2 are
+            // P.lists' arrays
+            P.latpar1.clear();
+            P.latpar2.clear();
+            P.latpar3.clear();
+        } else {
+            P.latpar1 = L.l1;
+            P.latpar2 = L.l2;
+            P.latpar3.assign(static_cast<size_t>(n2 + 1) * 17, 0.f);
+            for (int i = 1; i <= n2; ++i) {
+                const int32_t* Lr = &L.l2[static_cast<size_t>(i) * 6];
+                float* T = &P.latpar3[static_cast<size_t>(i) * 17];
+                rows::Geometry g;
+                rows::geometry(R, RT, Lr[1], Lr[2], Lr[3], g);
+                T[1] = 2 * kPi / p.uca;
+                T[2] = g.qx; T[3] = g.qy; T[4] = g.qz; T[5] = g.qh; T[6] = g.qh / (2.0 * kPi);
+                T[7] = g.tx; T[8] = g.ty; T[9] = g.tz;
+                T[10] = rows::g3(g.qh);
+                T[11] = (shp == 2 || shp == 7) ? gauss_norm3(rows::x2phi(g.qh, s.phiwidth)) : 0.0;
+                T[12] = 0;
+                T[13] = g.qxy;
+            }
         }
         if (p.peakidx + 1 != 2) return "peak shape not supported by the device path";
         if (p.twinned) return "twinning not supported by the device path";

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <array>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,17 @@ struct ParamSet {
     uint32_t combo_key() const;
 };
 
+// Reflection lists of one lattice type and HKLmax, the part of the tables that does not
+// depend on the lattice constants: latpar1 ((n1+1) x 6) and latpar2 ((n2+2) x 6) exactly as
+// prepare fills them, and each entry's sphno.
+struct LatticeLists {
+    int ltype = -1, hklmax = -1, n1 = 0, n2 = 0;
+    std::vector<int32_t> l1, l2;
+    std::vector<double> s1, s2;
+};
+// Built once per (ltype, HKLmax) in the process; thread-safe.
+std::shared_ptr<const LatticeLists> lattice_lists(int ltype, int hklmax);
+
 struct PreparedSet {
     csx_scalars s{};
     std::array<double, 18 * CSX_NCOEF> carr{};  // carr1p..carr9p, carr1f..carr9f
@@ -54,13 +66,19 @@ struct PreparedSet {
     int xmin = 0, xmax = 0, ymin = 0, ymax = 0; // computed block (ihex, i)
     int axmin = 0, axmax = 0, aymin = 0, aymax = 0;  // image array (whole image when expanded)
     int cdim = 3;
+    // Lattice kinds: the reflection lists and the rotated reciprocal bases (rows a*, b*, c* of
+    // corotations' matrix and of its twinned counterpart) from which latpar3 is computed.
+    std::shared_ptr<const LatticeLists> lists;
+    double rot[9] = {}, rott[9] = {};
 };
 
 // checkAllParameters (Source/sc_calc_generic_cpu.cpp:198-317): empty string when valid.
 std::string check_parameters(const ParamSet& p, bool ignore_cbs);
 
-// prepareCalculation.  Returns empty string on success, else the reason.
-std::string prepare(const ParamSet& p, PreparedSet& out);
+// prepareCalculation.  Returns empty string on success, else the reason.  With
+// rows_on_device, lattice kinds leave latpar1/2/3 empty: the caller builds the tables on the
+// device from out.lists, out.rot and out.rott (csx_stage_build_rows).
+std::string prepare(const ParamSet& p, PreparedSet& out, bool rows_on_device = false);
 
 // Reference Gamma (Source/sc_calc_generic_gpu.cu:613-673), needed on host and device.
 double ref_gamma(double z);
@@ -84,6 +102,12 @@ void csx_stage_build(csx_staged* out, const csx_scalars* s, const double* h_carr
                      const int32_t* h_l2, const float* h_l3, int w, int h, const csx_mask* mask,
                      const double* d_data);
 int csx_stage_commit(csx_ctx* c, const csx_staged* st);
+// The same for a lattice set prepared with rows_on_device: its tables are built on the device
+// (k_prep_sets) from P.lists, P.rot and P.rott when the batch runs.
+void csx_stage_build_rows(csx_staged* out, const csx::PreparedSet& P, int w, int h, const csx_mask* mask,
+                          const double* d_data);
+// Whether the batched fit driver stages lattice sets for device preparation (csx_set_device_prep).
+int csx_ctx_device_prep(const csx_ctx* c);
 // n built sets at once, their bytes copied on up to `threads` threads.
 int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int threads);
 // csx_fit_batch_run in two halves: enqueue without waiting, then wait for the sums (the
@@ -93,7 +117,8 @@ int csx_fit_batch_wait(csx_ctx* c, double* h_sums, int64_t* h_cnts);
 // ExpandImage: the image array of the uploaded parameter set from its computed block
 // (csx_device.hip), ordered on the context's stream.
 int csx_expand_image(csx_ctx* c, const double* d_region, double* d_full);
-// A second context on the same device.
+// A second context on the same device, with the same batch settings (kernel variant, graphs,
+// device preparation).
 int csx_create_like(const csx_ctx* c, csx_ctx** out);
 // The context's device and stream, and its error message (for the post-processing kernels
 // of csx_post.hip, which live outside csx_device.hip).

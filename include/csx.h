@@ -209,6 +209,24 @@ int csx_tune_force(csx_ctx* ctx, int variant);
 int csx_set_batch_graph(csx_ctx* ctx, int enable);
 long long csx_batch_graph_launches(const csx_ctx* ctx);
 
+/* Device preparation of batched lattice sets (no reference counterpart; replaces the table
+ * fill of prepareCalculation, Source/sc_calc_generic_cpu.cpp:838-954, per simplex vertex).
+ * With it on (the default) the batched fit driver (csx_nm_batch, csx_nm_eval) prepares the
+ * lattice sets' scalars, coefficient tables and rotated reciprocal bases on the host, and
+ * one workgroup per set builds the reflection table (latpar3), the Friedel-pair records,
+ * their |q| bin table and the isotropic list on the device, as the first step of the round
+ * (a node of its graph).  The tables equal the host's bit for bit; the one value that could
+ * differ, the peak norm's exponential, is correctly rounded on the device and every entry
+ * where the C library's exp could round to a different float is counted
+ * (csx_device_prep_counts: sets built, entries counted).  Off: the host builds every table.
+ * csx_fit_batch_table copies a table of set `set` of the last batch run back to the host
+ * (which: 0 latpar3, 1 pair records, 2 FP64 pair records, 3 bin table, 4 isotropic list,
+ * 5 five doubles: pair-record count, bin scale, |q| min and max, weight bound), for either
+ * kind of staging; out = NULL returns the size in *needed. */
+int csx_set_device_prep(csx_ctx* ctx, int on);
+int csx_device_prep_counts(const csx_ctx* ctx, long long* sets, long long* ambiguous);
+int csx_fit_batch_table(csx_ctx* ctx, int set, int which, void* out, size_t bytes, size_t* needed);
+
 /* ---- Batched Nelder-Mead fits ------------------------------------------------------
  * The simplex of SasCalc_SimplexFit2D::doSimplexFit2D (Source/sc_simplexfit2d.cpp:254-611,
  * amotry :221-251) with the bounds rule of fehlerquadratsumme (:25-44: a point outside
@@ -263,6 +281,7 @@ typedef struct csx_nm_stats {
     double run_ms;            /* batched launch, reductions and the copy of the sums */
     int64_t pruned;           /* trial points screened out of the exact evaluation (csx_nm_batch_screened) */
     double screen_ms;         /* time in the screen callback */
+    double launch_ms;         /* enqueueing the rounds: set records, graph update and launch */
 } csx_nm_stats;
 int csx_nm_batch(csx_ctx* ctx, int nfits, const csx_nm_fit* fits, int npar, int nfree, int mode,
                  const char* const* keys, double stp, int maxit, double ftol, int precision, int threads,
