@@ -197,6 +197,19 @@ class SasCalc:
         if self._L.csx_engine_stage_image(self._h, ctx._h, ctypes.c_void_p(d_out_ptr), int(cap)) != 0:
             raise CsxError("stage_image: " + self._err())
 
+    def stage_points(self, ctx, keys, X, d_out_ptr, stride, threads=16):
+        """Stage len(X) images into ctx's pending batch (csx_engine_stage_points): image k has
+        this engine's parameters with keys[j] = X[k, j] ("a,b" sets both), output at
+        d_out_ptr + k*stride doubles.  Follow with csx_render_batch_run."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        n, npar = X.shape
+        if npar != len(keys):
+            raise ValueError("X must have one column per key")
+        ckeys = (ctypes.c_char_p * npar)(*[k.encode() for k in keys])
+        if self._L.csx_engine_stage_points(self._h, ctx._h, n, npar, ckeys, _dp(X), ctypes.c_void_p(d_out_ptr),
+                                           int(stride), int(threads)) != 0:
+            raise CsxError("stage_points: " + self._err())
+
 
 class Random:
     """The C library's rand() sequence after srand(seed), in a private state (csx_rand)."""

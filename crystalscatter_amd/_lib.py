@@ -115,6 +115,7 @@ SIGNATURES = [
     ("csx_set_batch_graph", _I, [_VP, _I]),
     ("csx_batch_graph_launches", ctypes.c_longlong, [_VP]),
     ("csx_set_device_prep", _I, [_VP, _I]),
+    ("csx_engine_stage_points", _I, [_VP, _VP, _I, _I, ctypes.POINTER(_CP), _PD, _VP, ctypes.c_int64, _I]),
     ("csx_device_prep_counts", _I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     ("csx_fit_batch_table", _I, [_VP, _I, _I, _VP, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("csx_nm_run_cb", _I, [_I, _I, _I, OBJECTIVE, _VP, _PD, _PD, _PD, _D, _I, _D, ctypes.POINTER(NMResult)]),

@@ -487,6 +487,70 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
     return rc;
 }
 
+int csx_engine_stage_points(csx_engine* e, csx_ctx* ctx, int n, int npar, const char* const* keys, const double* x,
+                            double* d_out, int64_t stride, int threads) {
+    if (!e || !ctx || n <= 0 || npar <= 0 || !keys || !x || !d_out || stride <= 0) return CSX_E_ARG;
+    for (int j = 0; j < npar; ++j)
+        if (!keys[j] || !*keys[j]) return CSX_E_ARG;
+    std::vector<csx::PreparedSet> preps((size_t)n);
+    std::vector<std::unique_ptr<csx_staged, void (*)(csx_staged*)>> st;
+    st.reserve((size_t)n);
+    for (int k = 0; k < n; ++k) st.emplace_back(csx_staged_new(), csx_staged_free);
+    std::vector<std::string> errs((size_t)n);
+    csx_mask m{};
+    for (int i = 0; i < 16; ++i) m.rects[i] = -1;
+    const bool dev = csx_ctx_device_prep(ctx) != 0;
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int k = next++; k < n; k = next++) {
+            csx::ParamSet ps = e->p;
+            for (int j = 0; j < npar; ++j) {  // a key "a,b" sets both names
+                std::string name;
+                for (const char* c = keys[j];; ++c) {
+                    if (*c == ',' || *c == 0) {
+                        if (!name.empty() && !ps.set(name, x[(size_t)k * npar + j])) errs[(size_t)k] = "unknown key " + name;
+                        name.clear();
+                        if (*c == 0) break;
+                    } else {
+                        name += *c;
+                    }
+                }
+            }
+            if (!errs[(size_t)k].empty()) continue;
+            const bool rows = dev && ps.ltype != 12;
+            std::string msg = csx::check_parameters(ps, e->ignore_cbs);
+            if (msg.empty()) msg = csx::prepare(ps, preps[(size_t)k], rows);
+            if (!msg.empty()) { errs[(size_t)k] = msg; continue; }
+            const csx::PreparedSet& P = preps[(size_t)k];
+            const int w = P.s.xmax - P.s.xmin, h = P.s.ymax - P.s.ymin;
+            if ((int64_t)w * h > stride) { errs[(size_t)k] = "image larger than its output stride"; continue; }
+            if (rows) {
+                csx_stage_build_rows(st[(size_t)k].get(), P, w, h, &m, d_out + (size_t)k * stride);
+                continue;
+            }
+            csx_stage_build(st[(size_t)k].get(), &P.s, P.carr.data(), P.latpar1.empty() ? nullptr : P.latpar1.data(),
+                            P.latpar2.empty() ? nullptr : P.latpar2.data(),
+                            P.latpar3.empty() ? nullptr : P.latpar3.data(), w, h, &m, d_out + (size_t)k * stride);
+        }
+    };
+    const int nt = std::max(1, std::min(threads > 0 ? threads : 1, n));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    for (int k = 0; k < n; ++k)
+        if (!errs[(size_t)k].empty()) { e->err = errs[(size_t)k]; return CSX_E_PARAM; }
+    std::vector<const csx_staged*> built((size_t)n);
+    for (int k = 0; k < n; ++k) built[(size_t)k] = st[(size_t)k].get();
+    const int rc = csx_stage_commit_many(ctx, n, built.data(), nt);
+    if (rc != CSX_OK) {
+        char b[256];
+        csx_last_error(ctx, b, sizeof b);
+        e->err = b;
+    }
+    return rc;
+}
+
 const csx_scalars* csx_engine_scalars(csx_engine* e) { return e ? &e->P.s : nullptr; }
 csx_ctx* csx_engine_ctx(csx_engine* e) { return e ? e->ctx : nullptr; }
 

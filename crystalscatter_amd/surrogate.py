@@ -246,11 +246,14 @@ class TrainingSet:
         return int(self.t.numel())
 
 
-def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, mu=None, sigma=None):
+def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, mu=None, sigma=None,
+             chunk_bytes=1 << 30):
     """Render npatterns BCT-cylinder patterns with the exact kernel (mixed precision, I0 = 1)
     at parameters drawn like the fit workload, and keep px_per_pattern random pixels of each
-    with a positive, finite intensity.  Targets are (max(log10 I, log10 I_max - FLOOR_DECADES)
-    - mu) / sigma; mu and sigma come from the data unless given."""
+    with a positive, finite intensity (the first that many of 2*px_per_pattern draws).
+    Targets are (max(log10 I, log10 I_max - FLOOR_DECADES) - mu) / sigma; mu and sigma come
+    from the data unless given.  Patterns go in chunks of chunk_bytes of images: one native
+    staging call (lattice tables built on the device) and one batched render per chunk."""
     import torch
     from . import Context, SasCalc, _lib, _prec
     rng = np.random.default_rng(seed)
@@ -258,38 +261,35 @@ def generate(npatterns, grid_points=128, px_per_pattern=2048, seed=7, device=0, 
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     sets, qs, ts, rs = [], [], [], []
-    # one host-only engine: each pattern's tables are staged into a batch, and every chunk of
-    # patterns is rendered in one launch (the images equal one render per pattern)
     e = SasCalc(device=-1, params=params_dict(xs[0], grid_points))
     e.set("ignore_cbs", 1)
     e.prepare()
     x0, x1, y0, y1 = e.block()
     npx = (x1 - x0) * (y1 - y0)
-    chunk = max(1, min(npatterns, (1 << 27) // (8 * npx)))
+    chunk = max(1, min(npatterns, chunk_bytes // (8 * npx)))
     buf = torch.empty((chunk, npx), dtype=torch.float64, device="cuda")
     ctx = Context(device)
     L = _lib.lib()
+    dev = torch.device("cuda")
     for k0 in range(0, npatterns, chunk):
         kn = min(chunk, npatterns - k0)
         ctx._chk(L.csx_fit_batch_begin(ctx._h), "csx_fit_batch_begin")
-        for j in range(kn):
-            e.set_params(params_dict(xs[k0 + j], grid_points))
-            e.stage_image(ctx, buf[j].data_ptr(), npx)
+        e.stage_points(ctx, FIT_KEYS, xs[k0:k0 + kn], buf.data_ptr(), npx)
         ctx._chk(L.csx_render_batch_run(ctx._h, _prec("mixed")), "csx_render_batch_run")
-        for j in range(kn):
-            k = k0 + j
-            img = buf[j]
-            top = torch.log10(torch.nan_to_num(img, nan=0.0, posinf=0.0).max())
-            idx = torch.randint(0, npx, (2 * px_per_pattern,), device="cuda", generator=gen)
-            v = img[idx]
-            ok = torch.isfinite(v) & (v > 0)
-            idx, v = idx[ok][:px_per_pattern], v[ok][:px_per_pattern]
-            qx, qy = pixel_q(e, idx.to(torch.float64))
-            sets.append(torch.full((idx.numel(),), k, dtype=torch.int32, device="cuda"))
-            qs.append(torch.stack([qx, qy], dim=1).to(torch.float32))
-            lv = torch.log10(v)
-            ts.append(torch.maximum(lv, top - FLOOR_DECADES))
-            rs.append(lv >= top - EVAL_DECADES)
+        img = buf[:kn]
+        top = torch.log10(torch.nan_to_num(img, nan=0.0, posinf=0.0).amax(dim=1))
+        idx = torch.randint(0, npx, (kn, 2 * px_per_pattern), device=dev, generator=gen)
+        v = torch.gather(img, 1, idx)
+        ok = torch.isfinite(v) & (v > 0)
+        keep = ok & (torch.cumsum(ok, dim=1) <= px_per_pattern)
+        row = torch.arange(kn, device=dev, dtype=torch.int64).unsqueeze(1).expand_as(idx)[keep]
+        idx, v = idx[keep], v[keep]
+        qx, qy = pixel_q(e, idx.to(torch.float64))
+        sets.append((row + k0).to(torch.int32))
+        qs.append(torch.stack([qx, qy], dim=1).to(torch.float32))
+        lv = torch.log10(v)
+        ts.append(torch.maximum(lv, top[row] - FLOOR_DECADES))
+        rs.append(lv >= top[row] - EVAL_DECADES)
         torch.cuda.current_stream().synchronize()  # the chunk buffer is reused by the next batch
     e.close()
     ctx.close()

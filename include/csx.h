@@ -353,6 +353,13 @@ int csx_engine_last_error(csx_engine* e, char* buf, int len);
  * for csx_render_batch_run, output d_out (room for cap doubles); the engine can be changed
  * right after. */
 int csx_engine_stage_image(csx_engine* e, csx_ctx* ctx, double* d_out, int64_t cap);
+/* n images staged at once (no reference counterpart; the training patterns of the surrogate):
+ * image k has the engine's parameters with keys[j] set to x[k*npar + j] (a key "a,b" sets
+ * both names), is checked, prepared on `threads` host threads (lattice tables on the device
+ * unless csx_set_device_prep(ctx, 0)) and staged into ctx's pending batch with its output at
+ * d_out + k*stride.  Follow with csx_render_batch_run.  The engine itself is not changed. */
+int csx_engine_stage_points(csx_engine* e, csx_ctx* ctx, int n, int npar, const char* const* keys, const double* x,
+                            double* d_out, int64_t stride, int threads);
 
 /* ---- Random variation for training data (tpvPerformRandom) ------------------------------
  * csx_rand: the C library's rand() sequence (srand(seed), then rand() values) in a private
