@@ -38,6 +38,8 @@ class Scalars(ctypes.Structure):
         ("ymax", ctypes.c_int32), ("iimin", ctypes.c_int32), ("cs", ctypes.c_int32),
         ("use1d", ctypes.c_int32), ("qmin", ctypes.c_double), ("axmin", ctypes.c_int32),
         ("axmax", ctypes.c_int32), ("aymin", ctypes.c_int32), ("aymax", ctypes.c_int32),
+        ("peak_eta", ctypes.c_double), ("peak_beta", ctypes.c_double), ("peak_c", ctypes.c_double * 5),
+        ("peak_gamma_b", ctypes.c_double), ("peak_gamma_bh", ctypes.c_double),
     ]
 
 

@@ -60,8 +60,22 @@ inline int kind_of(uint32_t key) {
     case CSX_KEY_CUBE_NONE_ISO: return K_CUBE;
     case CSX_KEY_CYL_NONE_ISO: return K_CYL_ISO;
     case CSX_KEY_CUBE_NONE_ZDIR: return K_CUBE_ZDIR;
-    default: return -1;
+    default: break;
     }
+    // spheres on the FCC, BCC, SC or BCT lattice, Gaussian or isotropic orientation, with the
+    // Lorentzian, mod. 1/2 Lorentzian, Pseudo-Voigt or Pearson VII peak (calc_GENERIC)
+    const uint32_t part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
+    if (part == 0 && (shp == 1 || (shp >= 3 && shp <= 6)) && (lt == 4 || lt == 5 || lt == 7 || lt == 8) &&
+        (ord == 0 || ord == 7))
+        return K_FCC;
+    return -1;
+}
+// Peak shape of a lattice key (2: Gaussian).
+inline int shape_of(uint32_t key) { return (int)(key >> 16 & 0xff); }
+// The arithmetic a request runs in: sets with a peak shape other than the Gaussian have only
+// the FP64 reference-order sum (the mixed path's windows assume the Gaussian's tails).
+inline int effective_precision(uint32_t key, int precision) {
+    return (precision == CSX_PREC_MIXED && kind_of(key) == K_FCC && shape_of(key) != 2) ? CSX_PREC_REF64 : precision;
 }
 // Kinds whose mixed peak sum runs the precise pair loop (FP64 exponents): their structure
 // factor 1 + (fq/pq)(2*pi*I/V - 1)*dwe cancels over much of the pattern (fq/pq reaches 1e8
@@ -132,9 +146,78 @@ __device__ __forceinline__ double near_pair_x(const NearCtx& nc, int ia) {
 // Reference-order FP64 sums: intensity over the anisotropic list (stopping at the first
 // inf, :131-140) and radintensity over the isotropic list (reading the anisotropic q_hkl
 // column, :69-83).
+// The other peak shapes of calc_GENERIC (Source/sc_gpu_generic.h:695-849): the isotropic
+// Gaussian sum and the anisotropic sum in the generic kernel's order (a reflection with
+// |q_hkl| <= 0 or NaN is skipped, the term is sq*mhkl*fhkl*psiord, a sum that turns inf or
+// NaN keeps its previous value and ends the loop).
+__device__ void lattice_ref64_shape(const DevSet& d, double qx, double qy, double qz, double q, double width,
+                                    double& inten, double& radint) {
+    const csx_scalars& s = d.s;
+    const double widthiso = 1.0 / s.uca;
+    radint = 0.0;
+    for (int ii = 1; ii <= s.peakmax1; ++ii) {
+        const int mhkl = d.latpar1[ii * 6 + 4], fhkl = d.latpar1[ii * 6 + 5];
+        const double qhkl = d.latpar3[ii * 17 + 5];
+        const double x2 = sqd(q - qhkl) / sqd(widthiso);
+        const double sq = exp(-4 * x2 / kPiD) / (kPiD * widthiso / 2.0);
+        const double sav = radint;
+        radint += sq * mhkl * fhkl;
+        if (isinf(radint) || isnan(radint)) { radint = sav; break; }
+    }
+    const int shp = (int)(s.combo_key >> 16 & 0xff);
+    const double eta = s.peak_eta, beta = s.peak_beta;
+    const double c0 = s.peak_c[0], c1 = s.peak_c[1], c2 = s.peak_c[2], c3 = s.peak_c[3], c4 = s.peak_c[4];
+    inten = 0.0;
+    for (int ii = 1; ii <= s.peakmax2; ++ii) {
+        const float* L3 = d.latpar3 + ii * 17;
+        const double qhkl = L3[5];
+        if (qhkl <= 0 || isnan(qhkl)) continue;
+        const int mhkl = d.latpar2[ii * 6 + 4], fhkl = d.latpar2[ii * 6 + 5];
+        const double g3 = L3[10], pn1 = L3[11], pn2 = L3[12];
+        const double yphi = ref_acos((qx * (double)L3[2] + qy * (double)L3[3] + qz * (double)L3[4]) / (q * qhkl));
+        const double x2 = (q - qhkl) * (q - qhkl) / (width * width);
+        const double x2phi = 4 * q * q / sqd(s.phiwidth);  // the Lorentzian shapes' factor
+        double sq = 0.0, psi = 1.0;
+        switch (shp) {
+        case 1:  // Lorentzian
+            sq = sqrt(c1) / (kPiD * width * (1.0 + c1 * x2));
+            psi = g3 / (pn1 * (1 + x2phi * yphi * yphi));
+            break;
+        case 3:  // mod. 1 Lorentzian
+            sq = 2 * sqrt(c2) / (kPiD * width * (1 + c1 * x2) * (1 + c1 * x2));
+            psi = g3 / (pn1 * (1 + x2phi * yphi * yphi) * (1 + x2phi * yphi * yphi));
+            break;
+        case 4:  // mod. 2 Lorentzian
+            sq = sqrt(c3) / (2 * width * exp(3 * log(1 + c1 * x2) / 2.0));
+            psi = g3 / (pn1 * exp(3 * log(1 + x2phi * yphi * yphi) / 2.0));
+            break;
+        case 5: {  // Pseudo-Voigt
+            sq = eta * sqrt(c1) / (kPiD * width * (1 + c1 * x2)) +
+                 (1 - eta) * sqrt(c0) * exp(-c0 * x2) / (sqrt(kPiD) * width);
+            const double x2psi = 4 * q * q / (kPiD * sqd(s.phiwidth));
+            psi = g3 * (eta * (1 / (1 + x2phi * yphi * yphi)) + (1 - eta) * exp(-x2psi * yphi * yphi)) /
+                  (eta * pn1 + (1 - eta) * pn2);
+            break;
+        }
+        case 6:  // Pearson VII
+            sq = s.peak_gamma_b * c4 * 2 / (s.peak_gamma_bh * kPiD * width * exp(beta * log(1 + 4 * c4 * x2)));
+            psi = g3 / (pn1 * exp(beta * log(1 + x2phi * yphi * yphi)));
+            break;
+        default: break;
+        }
+        const double sav = inten;
+        inten += sq * mhkl * fhkl * psi;
+        if (isinf(inten) || isnan(inten)) { inten = sav; break; }
+    }
+}
+
 __device__ void lattice_ref64(const DevSet& d, double qx, double qy, double qz, double q, double width,
                               double& inten, double& radint) {
     const csx_scalars& s = d.s;
+    if ((s.combo_key >> 16 & 0xff) != 2) {
+        lattice_ref64_shape(d, qx, qy, qz, q, width, inten, radint);
+        return;
+    }
     const double widthiso = 1.0 / s.uca;
     radint = 0.0;
     for (int ii = 1; ii <= s.peakmax1; ++ii) {
@@ -2950,6 +3033,7 @@ int csx_upload(csx_ctx* c, const csx_scalars* s, const double* h_carr, const dou
 
 
 int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
+    if (c) precision = effective_precision(c->set.s.combo_key, precision);
     if (!c || !g || !d_out) return CSX_E_ARG;
     if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_render before csx_upload");
     if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
@@ -2966,6 +3050,7 @@ int csx_render(csx_ctx* c, const csx_grid* g, int precision, double* d_out) {
 }
 
 int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out, uint64_t* h_stats) {
+    if (c) precision = effective_precision(c->set.s.combo_key, precision);
     if (!c || !g || !d_out || !h_stats) return CSX_E_ARG;
     if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_render_stats before csx_upload");
     if (g->xmax <= g->xmin || g->ymax <= g->ymin) return fail(c, CSX_E_ARG, "empty grid");
@@ -2982,11 +3067,12 @@ int csx_render_stats(csx_ctx* c, const csx_grid* g, int precision, double* d_out
 }
 
 
-int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, const double* d_data,
+int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision_in, const double* d_data,
                 double* h_sum, int64_t* h_cnt, int64_t* h_nan, double* d_fqs) {
     if (!c || !mask || !d_data || !h_sum || w <= 0 || h <= 0) return CSX_E_ARG;
     if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_fit_fqs before csx_upload");
     const csx_scalars& s = c->set.s;
+    const int precision = effective_precision(s.combo_key, precision_in);
     if ((long long)(s.xmax - s.xmin) * (s.ymax - s.ymin) < (long long)w * h || (s.xmax - s.xmin) != w)
         return fail(c, CSX_E_ARG, "fit array must match the image width and fit inside the image");
     if (s.iimin != s.xmin || s.iimin != s.ymin)
@@ -3026,11 +3112,12 @@ int csx_fit_fqs(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, c
 // The objective of csx_fit_fqs with its tile partials in the static scratch array instead of
 // context memory, and their sum taken on the device: nothing is allocated, on the device or
 // on the host.
-int csx_fit_fqs_noalloc(csx_ctx* c, int w, int h, const csx_mask* mask, int precision, const double* d_data,
+int csx_fit_fqs_noalloc(csx_ctx* c, int w, int h, const csx_mask* mask, int precision_in, const double* d_data,
                         double* h_sum, int64_t* h_cnt, int64_t* h_nan, double* d_fqs) {
     if (!c || !mask || !d_data || !h_sum || w <= 0 || h <= 0) return CSX_E_ARG;
     if (!c->uploaded) return fail(c, CSX_E_STATE, "csx_fit_fqs_noalloc before csx_upload");
     const csx_scalars& s = c->set.s;
+    const int precision = effective_precision(s.combo_key, precision_in);
     if ((long long)(s.xmax - s.xmin) * (s.ymax - s.ymin) < (long long)w * h || (s.xmax - s.xmin) != w)
         return fail(c, CSX_E_ARG, "fit array must match the image width and fit inside the image");
     if (s.iimin != s.xmin || s.iimin != s.ymin)
@@ -3205,6 +3292,7 @@ void csx_stage_build_rows(csx_staged* out, const csx::PreparedSet& P, int w, int
     };
     if ((s->combo_key >> 8 & 0xff) == 12 || !P.lists)
         return bad("device preparation needs a lattice set prepared with rows_on_device");
+    if ((s->combo_key >> 16 & 0xff) != 2) return bad("device preparation builds the Gaussian peak's tables only");
     if (P.lists->n2 != s->peakmax2 || P.lists->n1 != s->peakmax1)
         return bad("reflection lists do not match the set's peakmax");
     st.tmpl = rows_template(P.lists, s->peakmax1);
@@ -3501,6 +3589,7 @@ int csx_fit_batch_launch(csx_ctx* c, int precision) {
     if (!c) return CSX_E_ARG;
     const int nsets = (int)c->bsets.size();
     if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
+    precision = effective_precision(c->bsets[0].d.s.combo_key, precision);
     HIPCHK(c, hipSetDevice(c->device));
     const int ntiles = c->bntiles_total;
     const FitSet* d_sets;
@@ -3616,6 +3705,7 @@ extern "C" int csx_render_batch_run(csx_ctx* c, int precision) {
     if (!c) return CSX_E_ARG;
     const int nsets = (int)c->bsets.size();
     if (nsets == 0) return fail(c, CSX_E_STATE, "empty batch");
+    precision = effective_precision(c->bsets[0].d.s.combo_key, precision);
     HIPCHK(c, hipSetDevice(c->device));
     for (const FitSet& f : c->bsets) {
         if (f.h != f.d.s.ymax - f.d.s.ymin) return fail(c, CSX_E_ARG, "a batched image must cover its whole grid");

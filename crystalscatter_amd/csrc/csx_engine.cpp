@@ -452,7 +452,7 @@ int csx_engine_tpv_batch(csx_engine* e, csx_ctx* ctx, const char* ids, csx_rand*
     auto work = [&] {
         for (int k = next++; k < n; k = next++) {
             const csx::ParamSet& ps = sets[(size_t)k];
-            const bool rows = dev && ps.ltype != 12;
+            const bool rows = dev && ps.ltype != 12 && ps.peakidx + 1 == 2;  // Gaussian peaks
             std::string msg = csx::check_parameters(ps, e->ignore_cbs);
             if (msg.empty()) msg = csx::prepare(ps, preps[(size_t)k], rows);
             if (!msg.empty()) { errs[(size_t)k] = msg; continue; }
@@ -517,7 +517,7 @@ int csx_engine_stage_points(csx_engine* e, csx_ctx* ctx, int n, int npar, const 
                 }
             }
             if (!errs[(size_t)k].empty()) continue;
-            const bool rows = dev && ps.ltype != 12;
+            const bool rows = dev && ps.ltype != 12 && ps.peakidx + 1 == 2;  // Gaussian peaks
             std::string msg = csx::check_parameters(ps, e->ignore_cbs);
             if (msg.empty()) msg = csx::prepare(ps, preps[(size_t)k], rows);
             if (!msg.empty()) { errs[(size_t)k] = msg; continue; }

@@ -406,7 +406,7 @@ public:
             csx::ParamSet ps = *base_[pts[r].fit];
             for (int j = 0; j < npar_; ++j) set_tied(ps, keys_[j], pts[r].x[j]);
             csx::PreparedSet& P = preps_[r];
-            const bool rows = dev && ps.ltype != 12;
+            const bool rows = dev && ps.ltype != 12 && ps.peakidx + 1 == 2;  // Gaussian peaks
             errs_[r] = csx::prepare(ps, P, rows);
             if (!errs_[r].empty()) return;
             const csx_nm_fit& F = fits_[pts[r].fit];

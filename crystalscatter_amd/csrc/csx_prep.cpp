@@ -695,6 +695,68 @@ double gauss_norm3(double a) {
     return rows::gauss_norm3_from(a, rows::gauss_norm3_uses_exp(a) ? std::exp(rows::gauss_norm3_exp_arg(a)) : 0.0);
 }
 
+// lorentznorm3 (Source/sc_calc_generic_gpu.cu:809-826): the integral of sin x / (1 + a x^2) over
+// [0, pi], asymptotic form for a > 0.01, Taylor below.
+double lorentz_norm3(double a) {
+    if (a > 0.01) {
+        const double b0 = std::log(1 + a * kPi * kPi) / 2.;
+        const double b1 = (a * kPi * kPi - std::log(1 + a * kPi * kPi)) / 12.;
+        const double b2 = (-2 * a * kPi * kPi + a * a * kPi * kPi * kPi * kPi + 2 * std::log(1 + a * kPi * kPi)) / 480.;
+        const double b3 = (a * kPi * kPi * (6 - 3 * a * kPi * kPi + 2 * a * a * kPi * kPi * kPi * kPi) -
+                           6 * std::log(1 + a * kPi * kPi)) / 60480.;
+        return b0 / a - b1 / (a * a) + b2 / (a * a * a) - b3 / (a * a * a * a);
+    }
+    const double a1 = (kPi * kPi - 4);
+    const double a2 = (18 - 12 * kPi * kPi + kPi * kPi * kPi * kPi) / 2.;
+    const double a3 = (-1440 + 360 * kPi * kPi - 30 * kPi * kPi * kPi * kPi + kPi * kPi * kPi * kPi * kPi * kPi) / 6.;
+    return 2 - a1 * a + a2 * a * a - a3 * a * a * a;
+}
+
+// pearsonnorm3 (Source/sc_calc_generic_gpu.cu:868-946): the integral of sin x / (1 + a x^2)^b
+// over [0, pi] with b nudged by 1e-6: closed forms for a > 20 and a <= 0.01, else trapezoid
+// levels (end points, then the midpoints of the doubling intervals) until two agree to 1e-3,
+// at most 10.
+double pearson_norm3(double a, double b) {
+    b += 0.000001;
+    if (a > 20.) return (1 - std::exp((1 - b) * std::log(1 + a * kPi * kPi))) / (2 * (b - 1)) / a;
+    if (a <= 0.01) {
+        const double a1 = (kPi * kPi - 4) * b;
+        const double a2 = (18 - 12 * kPi * kPi + kPi * kPi * kPi * kPi) * b * (b + 1) / 2.;
+        const double a3 = (-1440 + 360 * kPi * kPi - 30 * kPi * kPi * kPi * kPi + kPi * kPi * kPi * kPi * kPi * kPi) *
+                          b * (b + 1) * (b + 2) / 6.;
+        return 2. - a1 * a + a2 * a * a - a3 * a * a * a;
+    }
+    auto f = [&](double x) { return std::sin(x) / std::exp(b * std::log(1 + a * x * x)); };
+    int n = 1;
+    double s = 0.5 * (kPi - 0) * (f(0) + f(kPi));
+    double old = s;
+    for (int level = 2; level <= 10; ++level) {
+        const double del = (kPi - 0) / n;
+        double x = 0 + 0.5 * del, sum = 0.0;
+        for (int j = 1; j <= n; ++j) {
+            sum += f(x);
+            x = x + del;
+        }
+        s = 0.5 * (s + (kPi - 0) * sum / n);
+        n = 2 * n;
+        if (std::fabs(1 - s / old) < 0.001) break;
+        old = s;
+    }
+    return s;
+}
+
+// The peak norm of latpar3 entry 11 for peak shape shp (Source/sc_calc_generic_cpu.cpp:899-928).
+double peak_norm(int shp, double x2phi, double beta) {
+    switch (shp) {
+    case 1: case 5: return lorentz_norm3(x2phi);
+    case 2: case 7: return gauss_norm3(x2phi);
+    case 3: return pearson_norm3(x2phi, 2);
+    case 4: return pearson_norm3(x2phi, 3 / 2.0);
+    case 6: return pearson_norm3(x2phi, beta);
+    default: return 0.0;
+    }
+}
+
 // Reflection lists of one lattice type and HKLmax (ButtonHKLClick :8246-10296, fhkl_c
 // :10299-10500): every allowed (h, k, l) of the isotropic octant and of the whole box, its
 // multiplicity and rounded structure factor, in latpar1/latpar2 layout.  Built once per
@@ -786,6 +848,17 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
     s.dwfactor = p.displacement * p.displacement / 3.0 * p.dw_zuf;
     s.uca = p.uca; s.ucb = p.ucb; s.domainsize = p.domainsize; s.width_zuf = p.width_zuf;
     s.phiwidth = 4. / p.azi;
+    // peak-shape parameters: setPeakPar (Source/sc_calc_generic_gpu.h:224) and the width
+    // constants of prepareCalculation (Source/sc_calc_generic_cpu.cpp:585-590)
+    s.peak_eta = p.peakpar / 100.;
+    s.peak_beta = p.peakpar;
+    s.peak_c[0] = 1;
+    s.peak_c[1] = 4;
+    s.peak_c[2] = 4 * (std::sqrt(2.0) - 1);
+    s.peak_c[3] = 4 * (std::exp(2 * std::log(2.0) / 3.0) - 1);
+    s.peak_c[4] = (s.peak_beta == 0) ? 1 : std::exp(std::log(2.) / s.peak_beta) - 1;
+    s.peak_gamma_b = ref_gamma(s.peak_beta);
+    s.peak_gamma_bh = ref_gamma(s.peak_beta - 0.5);
     s.ceff = p.ceff;
 
     // quadrant ranges (:373-394); 1D: one row of zmax pixels, never expanded
@@ -939,12 +1012,14 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
                 T[2] = g.qx; T[3] = g.qy; T[4] = g.qz; T[5] = g.qh; T[6] = g.qh / (2.0 * kPi);
                 T[7] = g.tx; T[8] = g.ty; T[9] = g.tz;
                 T[10] = rows::g3(g.qh);
-                T[11] = (shp == 2 || shp == 7) ? gauss_norm3(rows::x2phi(g.qh, s.phiwidth)) : 0.0;
+                T[11] = peak_norm(shp, rows::x2phi(g.qh, s.phiwidth), s.peak_beta);
                 T[12] = 0;
                 T[13] = g.qxy;
             }
         }
-        if (p.peakidx + 1 != 2) return "peak shape not supported by the device path";
+        if (shp < 1 || shp > 6) return "peak shape not supported by the device path";
+        if (shp != 2 && (p.particle != 0 || (p.ordis != 0 && p.ordis != 7)))
+            return "peak shapes other than the Gaussian are served for spheres with Gaussian or isotropic orientation";
         if (p.twinned) return "twinning not supported by the device path";
     }
     s.sphno_cubevol = lat_vol * lat_sph;

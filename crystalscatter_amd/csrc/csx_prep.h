@@ -84,6 +84,11 @@ std::string prepare(const ParamSet& p, PreparedSet& out, bool rows_on_device = f
 double ref_gamma(double z);
 // gaussnorm3 (Source/sc_calc_generic_gpu.cu:838-854).
 double gauss_norm3(double a);
+// lorentznorm3 (.cu:809-826), pearsonnorm3 (.cu:868-946), and the peak norm of latpar3 entry 11
+// for peak shape shp (Source/sc_calc_generic_cpu.cpp:899-928).
+double lorentz_norm3(double a);
+double pearson_norm3(double a, double b);
+double peak_norm(int shp, double x2phi, double beta);
 
 }  // namespace csx
 

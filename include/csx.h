@@ -104,6 +104,12 @@ typedef struct csx_scalars {
     int32_t axmin, axmax, aymin, aymax; /* bounds of the image array (data()): the computed
                                            block, or the whole image when ExpandImage fills it
                                            from a computed quadrant or half */
+    /* peak shapes other than the Gaussian (setPeakPar, prepareCalculation cpu:585-590):
+     * eta = EditPeakPar/100 (Pseudo-Voigt), beta = EditPeakPar (Pearson VII), the width
+     * constants c0..c4, and Gamma(beta), Gamma(beta - 1/2) of the reference's Gamma */
+    double peak_eta, peak_beta;
+    double peak_c[5];
+    double peak_gamma_b, peak_gamma_bh;
 } csx_scalars;
 
 /* Block of pixels to evaluate: ihex in [xmin,xmax), i in [ymin,ymax); output in the

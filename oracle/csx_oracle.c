@@ -76,6 +76,7 @@ struct csxo_engine {
     float latpar3[LATLEN][17];
     double ucvol, sphno, lat_ucvol, lat_sphno;
     double ri[9], rt[9];
+    double eta, beta, c0, c1, c2, c3, c4; /* peak-shape parameters (setPeakShapeParams, cpu:585-590) */
 };
 
 /* ========================================================================== */
@@ -232,6 +233,79 @@ double csxo_gaussnorm3(double a)
     const double a3 = (-1440 + 360 * M_PI * M_PI - 30 * M_PI * M_PI * M_PI * M_PI +
                        M_PI * M_PI * M_PI * M_PI * M_PI * M_PI) / 6.;
     return a0 - a1 * a + a2 * a * a - a3 * a * a * a;
+}
+
+/* Integral of sin x / (1 + a x^2) over [0,pi] as the reference approximates it,
+ * Source/sc_calc_generic_gpu.cu:809-826 (asymptotic form for a>0.01, Taylor below). */
+double csxo_lorentznorm3(double a)
+{
+    if (a > 0.01) {
+        const double b0 = log(1 + a * M_PI * M_PI) / 2.;
+        const double b1 = (a * M_PI * M_PI - log(1 + a * M_PI * M_PI)) / 12.;
+        const double b2 = (-2 * a * M_PI * M_PI + a * a * M_PI * M_PI * M_PI * M_PI + 2 * log(1 + a * M_PI * M_PI)) / 480.;
+        const double b3 = (a * M_PI * M_PI * (6 - 3 * a * M_PI * M_PI + 2 * a * a * M_PI * M_PI * M_PI * M_PI) -
+                           6 * log(1 + a * M_PI * M_PI)) / 60480.;
+        return b0 / a - b1 / (a * a) + b2 / (a * a * a) - b3 / (a * a * a * a);
+    }
+    const double a0 = 2;
+    const double a1 = (M_PI * M_PI - 4);
+    const double a2 = (18 - 12 * M_PI * M_PI + M_PI * M_PI * M_PI * M_PI) / 2.;
+    const double a3 = (-1440 + 360 * M_PI * M_PI - 30 * M_PI * M_PI * M_PI * M_PI +
+                       M_PI * M_PI * M_PI * M_PI * M_PI * M_PI) / 6.;
+    return a0 - a1 * a + a2 * a * a - a3 * a * a * a;
+}
+
+/* One trapezoid refinement of the integral of sin x / (1 + a x^2)^b over [at, bt]
+ * (pearsonintegral3, Source/sc_calc_generic_gpu.cu:918-946): level 1 the end points, then
+ * the midpoints of the *it current intervals, which double. */
+static void pearson_trapz(double at, double bt, double a, double b, double* sx, int nx, int* it)
+{
+    if (nx == 1) {
+        const double fa = sin(at) / exp(b * log(1 + a * at * at));
+        const double fb = sin(bt) / exp(b * log(1 + a * bt * bt));
+        *sx = 0.5 * (bt - at) * (fa + fb);
+        *it = 1;
+        return;
+    }
+    const double tnm = *it;
+    const double del = (bt - at) / tnm;
+    double xt = at + 0.5 * del, sum = 0.0;
+    for (int j = 1; j <= *it; ++j) {
+        sum += sin(xt) / exp(b * log(1 + a * xt * xt));
+        xt = xt + del;
+    }
+    *sx = 0.5 * (*sx + (bt - at) * sum / tnm);
+    *it = 2 * *it;
+}
+
+/* Integral of sin x / (1 + a x^2)^b over [0,pi] (pearsonnorm3, Source/sc_calc_generic_gpu.cu:
+ * 868-899): b nudged by 1e-6, closed forms for a > 20 and a <= 0.01, else trapezoid levels
+ * until two agree to 1e-3 (at most 10). */
+double csxo_pearsonnorm3(double a, double b)
+{
+    b += 0.000001;
+    if (a > 20.) {
+        const double b0 = (1 - exp((1 - b) * log(1 + a * M_PI * M_PI))) / (2 * (b - 1));
+        return b0 / a;
+    }
+    if (a <= 0.01) {
+        const double a0 = 2.;
+        const double a1 = (M_PI * M_PI - 4) * b;
+        const double a2 = (18 - 12 * M_PI * M_PI + M_PI * M_PI * M_PI * M_PI) * b * (b + 1) / 2.;
+        const double a3 = (-1440 + 360 * M_PI * M_PI - 30 * M_PI * M_PI * M_PI * M_PI +
+                           M_PI * M_PI * M_PI * M_PI * M_PI * M_PI) * b * (b + 1) * (b + 2) / 6.;
+        return a0 - a1 * a + a2 * a * a - a3 * a * a * a;
+    }
+    int it = 0;
+    double s = 0;
+    pearson_trapz(0, M_PI, a, b, &s, 1, &it);
+    double old = s;
+    for (int itf = 2; itf <= 10; ++itf) {
+        pearson_trapz(0, M_PI, a, b, &s, itf, &it);
+        if (fabs(1 - s / old) < kEps3) break;
+        old = s;
+    }
+    return s;
 }
 
 /* Neville extrapolation to h=0 over k points, 1-based arrays, as polint
@@ -998,6 +1072,14 @@ int csxo_prepare(csxo_engine* e)
     e->iimin = (quad == 1) ? 0 : -e->zmax;
     e->iimax = e->zmax;
     e->phiwidth = 4. / e->azi;
+    /* peak-shape parameters: setPeakPar (.h:224), prepareCalculation cpu:585-590 */
+    e->eta = e->peakpar / 100.;
+    e->beta = e->peakpar;
+    e->c0 = 1;
+    e->c1 = 4;
+    e->c2 = 4 * (sqrt(2.0) - 1);
+    e->c3 = 4 * (exp(2 * log(2.0) / 3.0) - 1);
+    e->c4 = (e->beta == 0) ? 1 : exp(log(2.) / e->beta) - 1;
     e->p1 = e->radius / e->radiusi;
     e->cosphi = cos(e->polPhi * M_PI / 180.0); e->sinphi = sin(e->polPhi * M_PI / 180.0);
     e->costheta = cos(e->polTheta * M_PI / 180.0); e->sintheta = sin(e->polTheta * M_PI / 180.0);
@@ -1061,8 +1143,15 @@ int csxo_prepare(csxo_engine* e)
             L3[9] = (2 * M_PI) * (T[2] * h + T[5] * k + T[8] * l);
             const double g3 = 4 * M_PI * M_PI / (2.0 * M_PI * qh * qh);
             const double x2phi = 4 * qh * qh / (M_PI * e->phiwidth * e->phiwidth);
-            double pn1 = 0;
-            if (e->shp == 2 || e->shp == 7) pn1 = csxo_gaussnorm3(x2phi);
+            double pn1 = 0; /* the peak norm of each shape (cpu:899-928) */
+            switch (e->shp) {
+            case 1: case 5: pn1 = csxo_lorentznorm3(x2phi); break;
+            case 2: case 7: pn1 = csxo_gaussnorm3(x2phi); break;
+            case 3: pn1 = csxo_pearsonnorm3(x2phi, 2); break;
+            case 4: pn1 = csxo_pearsonnorm3(x2phi, 3 / 2.0); break;
+            case 6: pn1 = csxo_pearsonnorm3(x2phi, e->beta); break;
+            default: break;
+            }
             L3[10] = g3; L3[11] = pn1; L3[12] = 0; L3[13] = qxy;
         }
         e->sphno = sphno;
@@ -1345,6 +1434,72 @@ static double fq_cylinder(const csxo_engine* e, double limql, double qxs, double
  * map to 0 for x<-1 and to pi for x>1. */
 static inline double ref_acos(double x) { return (x < -1) ? 0.0 : ((x > +1) ? M_PI : acos(x)); }
 
+/* Lattice factor of calc_GENERIC for the other peak shapes (Source/sc_gpu_generic.h:743-880):
+ * the isotropic Gaussian sum and the anisotropic sum in the generic kernel's order (a
+ * reflection with |q_hkl| <= 0 or NaN is skipped, the term is sq*mhkl*fhkl*psiord, a sum that
+ * turns inf or NaN keeps its previous value and ends the loop).  Twinning is not served. */
+static void lattice_sums_shape(const csxo_engine* e, double qx, double qy, double qz, double q, double width,
+                               double* szq_factor_I, double* szqiso)
+{
+    const double widthiso = 1.0 / e->uca;
+    const double sphno_cubevol = e->lat_ucvol * e->lat_sphno;
+    double radint = 0.0, inten = 0.0;
+    for (int ii = 1; ii <= e->peakmax1; ++ii) {
+        const int mhkl = e->latpar1[ii][4], fhkl = e->latpar1[ii][5];
+        const double qhkl = e->latpar3[ii][5];
+        const double x2 = sq(q - qhkl) / sq(widthiso);
+        const double s = exp(-4 * x2 / M_PI) / (M_PI * widthiso / 2.0);
+        const double sav = radint;
+        radint += s * mhkl * fhkl;
+        if (isinf(radint) || isnan(radint)) { radint = sav; break; }
+    }
+    for (int ii = 1; ii <= e->peakmax2; ++ii) {
+        const double qhkl = e->latpar3[ii][5];
+        if (qhkl <= 0 || isnan(qhkl)) continue;
+        const int mhkl = e->latpar2[ii][4], fhkl = e->latpar2[ii][5];
+        const double qxh = e->latpar3[ii][2], qyh = e->latpar3[ii][3], qzh = e->latpar3[ii][4];
+        const double g3 = e->latpar3[ii][10], pn1 = e->latpar3[ii][11], pn2 = e->latpar3[ii][12];
+        const double yphi = ref_acos((qx * qxh + qy * qyh + qz * qzh) / (q * qhkl));
+        const double x2 = (q - qhkl) * (q - qhkl) / (width * width);
+        const double x2phi = 4 * q * q / sq(e->phiwidth); /* the Lorentzian shapes' b/c factor */
+        double s = 0, psi = 1;
+        switch (e->shp) {
+        case 1: /* Lorentzian */
+            s = sqrt(e->c1) / (M_PI * width * (1.0 + e->c1 * x2));
+            psi = g3 / (pn1 * (1 + x2phi * yphi * yphi));
+            break;
+        case 3: /* mod. 1 Lorentzian */
+            s = 2 * sqrt(e->c2) / (M_PI * width * (1 + e->c1 * x2) * (1 + e->c1 * x2));
+            psi = g3 / (pn1 * (1 + x2phi * yphi * yphi) * (1 + x2phi * yphi * yphi));
+            break;
+        case 4: /* mod. 2 Lorentzian */
+            s = sqrt(e->c3) / (2 * width * exp(3 * log(1 + e->c1 * x2) / 2.0));
+            psi = g3 / (pn1 * exp(3 * log(1 + x2phi * yphi * yphi) / 2.0));
+            break;
+        case 5: { /* Pseudo-Voigt */
+            s = e->eta * sqrt(e->c1) / (M_PI * width * (1 + e->c1 * x2)) +
+                (1 - e->eta) * sqrt(e->c0) * exp(-e->c0 * x2) / (sqrt(M_PI) * width);
+            const double x2psi = 4 * q * q / (M_PI * sq(e->phiwidth));
+            psi = g3 * (e->eta * (1 / (1 + x2phi * yphi * yphi)) + (1 - e->eta) * exp(-x2psi * yphi * yphi)) /
+                  (e->eta * pn1 + (1 - e->eta) * pn2);
+            break;
+        }
+        case 6: /* Pearson VII */
+            s = csxo_gamma(e->beta) * e->c4 * 2 /
+                (csxo_gamma(e->beta - 0.5) * M_PI * width * exp(e->beta * log(1 + 4 * e->c4 * x2)));
+            psi = g3 / (pn1 * exp(e->beta * log(1 + x2phi * yphi * yphi)));
+            break;
+        default:
+            break;
+        }
+        const double sav = inten;
+        inten += s * mhkl * fhkl * psi;
+        if (isinf(inten) || isnan(inten)) { inten = sav; break; }
+    }
+    *szqiso = (1 + (1) * (8 * M_PI * M_PI * M_PI * radint / (4 * M_PI * sphno_cubevol) - 1) * exp(-e->ucb * q * q));
+    *szq_factor_I = 2 * M_PI * inten / sphno_cubevol;
+}
+
 /* Lattice factor of the Gaussian-peak kernels: isotropic and anisotropic peak sums
  * (Source/sc_gpu_pSph_pGau_lFCC_oGau.h:52-146; identical loops in
  * Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h:100-217).  Returns szq and szqiso. */
@@ -1358,6 +1513,10 @@ static void lattice_sums(const csxo_engine* e, double qx, double qy, double qz, 
     const double widthiso = 1.0 / e->uca;
     const double sphno_cubevol = e->lat_ucvol * e->lat_sphno;
     double radint = 0.0, inten = 0.0;
+    if (e->shp != 2) {
+        lattice_sums_shape(e, qx, qy, qz, q, width, szq_factor_I, szqiso);
+        return;
+    }
     for (int ii = 1; ii <= e->peakmax1; ++ii) {
         const int mhkl = e->latpar1[ii][4], fhkl = e->latpar1[ii][5];
         if (mhkl == 0 || fhkl == 0) continue;
@@ -1392,10 +1551,36 @@ static void lattice_sums(const csxo_engine* e, double qx, double qy, double qz, 
     *szq_factor_I = 2 * M_PI * inten / sphno_cubevol;
 }
 
+/* Spheres on a lattice: P(q), <F>^2 and the lattice factor (Source/sc_gpu_pSph_pGau_lFCC_oGau.h:
+ * 15-171; calc_GENERIC's sphere and peak branches, Source/sc_gpu_generic.h:40-52, 590-880). */
+static double sphere_lattice_pixel(const csxo_engine* e, double qx, double qy, double qz, double q)
+{
+    const double pq = pq_sphere(e, q);
+    const double fq = fq_sphere(e, q);
+    double I, szqiso;
+    lattice_sums(e, qx, qy, qz, q, &I, &szqiso);
+    const double szq = (1 + (fq / pq) * (I - 1) * exp(-e->dwfactor * q * q));
+    return e->base + e->i0 * (szq * pq + e->iso * szqiso * pq) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+}
+
+/* Spheres on the FCC, BCC, SC or BCT lattice, Gaussian or isotropic orientation, with the
+ * Lorentzian, mod. 1/2 Lorentzian, Pseudo-Voigt or Pearson VII peak (calc_GENERIC). */
+static int sphere_lattice_shape_key(unsigned key)
+{
+    const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
+    return part == 0 && (shp == 1 || (shp >= 3 && shp <= 6)) && (lt == 4 || lt == 5 || lt == 7 || lt == 8) &&
+           (ord == 0 || ord == 7);
+}
+
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)
 {
     const double q = sqrt(qx * qx + qy * qy + qz * qz) + kEps9;
     const unsigned key = csxo_combo_key(e);
+    if (sphere_lattice_shape_key(key)) {
+        if (e->twin) return -1;
+        *out = sphere_lattice_pixel(e, qx, qy, qz, q);
+        return 0;
+    }
     switch (key) {
     case 0x00000C00u: /* spheres without lattice, Gaussian or z-oriented distribution: the generic */
     case 0x00000C06u: /* kernel's sphere branch (Source/sc_gpu_generic.h:40-52, 868-880), the same value */
@@ -1421,15 +1606,9 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
     case 0x00020700u:
     case 0x00020707u:
     case 0x00020800u: /* spheres on the BCT lattice (generic kernel) */
-    case 0x00020807u: {
-        const double pq = pq_sphere(e, q);
-        const double fq = fq_sphere(e, q);
-        double I, szqiso;
-        lattice_sums(e, qx, qy, qz, q, &I, &szqiso);
-        const double szq = (1 + (fq / pq) * (I - 1) * exp(-e->dwfactor * q * q));
-        *out = e->base + e->i0 * (szq * pq + e->iso * szqiso * pq) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+    case 0x00020807u:
+        *out = sphere_lattice_pixel(e, qx, qy, qz, q);
         return 0;
-    }
     case 0x01000C00u: { /* Source/sc_gpu_pCyl_lNon_oGau.h:15-100 (fq unused without lattice) */
         double pq = 0;
         switch (e->orcase) {

@@ -63,6 +63,8 @@ double csxo_scalar(const csxo_engine* e, const char* name);
 /* Reference Gamma function (Source/sc_calc_generic_gpu.cu:613-673) and gaussnorm3 (.cu:838-854). */
 double csxo_gamma(double z);
 double csxo_gaussnorm3(double a);
+double csxo_lorentznorm3(double a);
+double csxo_pearsonnorm3(double a, double b);
 
 /* Nelder-Mead over an objective callback, restating doSimplexFit2D
  * (Source/sc_simplexfit2d.cpp:254-611) with amotry (:221-251) and the bound rule of

@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv post lattices fits qmodes  # only these
+    python tests/golden/make_golden.py cube tpv post lattices peaks fits qmodes  # only these
 """
 import json
 import os
@@ -123,6 +123,29 @@ def write_lattices(manifest):
             extra["carr11pm"] = r["carr11pm"].reshape(131, 131)[:121, :121].copy()
         np.savez_compressed(os.path.join(HERE, name + ".npz"), image=r["image"], bounds=r["bounds"],
                             **table_dict(r, True), **extra)
+        manifest["fixtures"][name + ".npz"] = {"params": p}
+
+
+# Spheres on lattices with the other peak shapes of the generic kernel (Source/sc_gpu_generic.h:
+# 695-834): Lorentzian (ComboBoxPeak 0), mod. 1 and mod. 2 Lorentzian (2, 3), Pseudo-Voigt (4,
+# EditPeakPar = 100 eta) and Pearson VII (5, EditPeakPar = beta; its peak norm is the
+# reference's trapezoid iteration), on the four cubic/tetragonal lattices, both orientations.
+PEAKS = [("fcc_sphere_lorentz", dict(CFG3, ComboBoxPeak=0, GridPoints=32)),
+         ("bcc_sphere_mod1lor", dict(CFG3, LType=4, ComboBoxPeak=2, GridPoints=32)),
+         ("sc_sphere_mod2lor_iso", dict(CFG3, LType=7, uca=25, ucb=25, ucc=25, Ordis=7, ComboBoxPeak=3,
+                                        GridPoints=32)),
+         ("fcc_sphere_pvoigt", dict(CFG3, ComboBoxPeak=4, EditPeakPar=30, GridPoints=32)),
+         ("bct_sphere_pearson", dict(CFG3, LType=8, uca=30, ucb=30, ucc=45, ComboBoxPeak=5, EditPeakPar=1.5,
+                                     GridPoints=32)),
+         ("fcc_sphere_pearson_iso", dict(CFG3, Ordis=7, ComboBoxPeak=5, EditPeakPar=3, EditAzi=20,
+                                         GridPoints=32, iso=0.3))]
+
+
+def write_peaks(manifest):
+    for name, p in PEAKS:
+        r = run_ref(p)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), image=r["image"], bounds=r["bounds"],
+                            **table_dict(r, True))
         manifest["fixtures"][name + ".npz"] = {"params": p}
 
 
@@ -271,6 +294,8 @@ def main():
             write_post(manifest)
         if "lattices" in only:
             write_lattices(manifest)
+        if "peaks" in only:
+            write_peaks(manifest)
         if "fits" in only:
             write_fits(manifest)
         if "qmodes" in only:
@@ -352,6 +377,7 @@ def main():
     write_tpv(manifest)
     write_post(manifest)
     write_lattices(manifest)
+    write_peaks(manifest)
     write_fits(manifest)
     write_qmodes(manifest)
 

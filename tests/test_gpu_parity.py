@@ -321,15 +321,35 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
 @pytest.mark.parametrize("prec,tol", [("ref64", 1e-13), ("mixed", 1e-4)])
 @pytest.mark.parametrize("name", ["bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz",
                                   "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz",
-                                  "cube_zdir_b.npz"])
+                                  "cube_zdir_b.npz", "fcc_sphere_lorentz.npz", "bcc_sphere_mod1lor.npz",
+                                  "sc_sphere_mod2lor_iso.npz", "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz",
+                                  "fcc_sphere_pearson_iso.npz"])
 def test_generic_kernel_keys(name, prec, tol):
     """Combinations of the reference's generic kernel served here (spheres on BCC, SC and BCT
-    lattices, other orientation distributions, isotropic cylinders) against its own output."""
+    lattices, other orientation distributions, isotropic cylinders, the other peak shapes on
+    the four lattices) against its own output.  The peak shapes other than the Gaussian run
+    the FP64 sums in either precision."""
     f = fixture(name)
     e, img = _calc(params(name), prec)
     r = rel_err(img, f["image"])
     _report("%s %s" % (name, prec), r)
     assert r.max() < tol
+
+
+@pytest.mark.parametrize("name", ["fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz", "sc_sphere_mod2lor_iso.npz"])
+def test_peak_shapes_256_against_oracle_random_pixels(oracle_lib, name):
+    """The other peak shapes at 256x256 (the FP64 sums): 2,000 random pixels against the
+    oracle, which equals the reference bit for bit on the 64x64 fixtures."""
+    p = dict(params(name), GridPoints=128)
+    e, img = _calc(p, "mixed")
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    rng = np.random.default_rng(4)
+    idx = rng.choice(img.size, 2000, replace=False)
+    ref = o.image_points(idx % 256 - 128, idx // 256 - 128, nthreads=16)
+    r = rel_err(img[idx], ref)
+    _report("%s 256^2 vs oracle" % name, r)
+    assert r.max() < 1e-12
 
 
 def test_bcc_512_against_oracle_random_pixels(oracle_lib):

@@ -19,7 +19,9 @@ def _host(p, ignore_cbs=True):
 
 @pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "bcc_sphere.npz",
                                   "sc_sphere_iso.npz", "fcc_sphere_iso.npz", "bct_sphere.npz", "sphere_zdir.npz",
-                                  "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz", "cube_zdir_b.npz"])
+                                  "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz", "cube_zdir_b.npz",
+                                  "fcc_sphere_lorentz.npz", "bcc_sphere_mod1lor.npz", "sc_sphere_mod2lor_iso.npz",
+                                  "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz", "fcc_sphere_pearson_iso.npz"])
 def test_tables_match_reference(name):
     f = fixture(name)
     e = _host(params(name))
