@@ -1207,17 +1207,22 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 fq = fq_sphere(d, q);
             }
         } else {
+            // the transverse P(q) and the radial factor of <F>^2 depend on |q| only: from the
+            // radial table when there is one (the same functions of the same |q|)
+            const double pt = rv ? rv->pq : pq_cyl_transverse(d, q);
+            const double fr = rv ? rv->fq : fq_cyl_radial(d, q);
+            double pl, fl;
             switch (s.orcase) {
             case 0:
-                pq = pq_cyl_long_zdir(d, qx * s.cosphi * s.sintheta, qy * s.sinphi * s.sintheta, q);
-                fq = fq_cyl_zdir(d, sqrt(s.cosphi * qx * qx + s.sinphi * qy * qy + 0.000000001),
-                                 qx * s.cosphi * s.sintheta, qy * s.sinphi * s.sintheta, q);
+                cyl_long_both(d, q, sqrt(s.cosphi * qx * qx + s.sinphi * qy * qy + 0.000000001),
+                              qx * s.cosphi * s.sintheta, qy * s.sinphi * s.sintheta, pl, fl);
                 break;
-            case 1: pq = pq_cyl_long_zdir(d, qx, 0, q); fq = fq_cyl_zdir(d, fabs(qx), qx, 0, q); break;
-            case 2: pq = pq_cyl_long_zdir(d, 0, qy, q); fq = fq_cyl_zdir(d, fabs(qy), 0, qy, q); break;
-            default: pq = pq_cyl_long_zdir(d, qx, qy, q); fq = fq_cyl_zdir(d, q, qx, qy, q); break;
+            case 1: cyl_long_both(d, q, fabs(qx), qx, 0, pl, fl); break;
+            case 2: cyl_long_both(d, q, fabs(qy), 0, qy, pl, fl); break;
+            default: cyl_long_both(d, q, q, qx, qy, pl, fl); break;
             }
-            pq *= pq_cyl_transverse(d, q);
+            pq = pl * pt;
+            fq = fl * fr;
         }
         double width = 1;
         if (s.dsch) width = 4.0 / s.domainsize;
@@ -1318,7 +1323,10 @@ __device__ __forceinline__ void tile_pixel(int& px, int& py) {
 }
 
 // Kinds whose form factors depend on |q| only.
-constexpr bool radial_kind(int kind) { return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE || kind == K_CYL_ISO; }
+// (BCT cylinders: the transverse P(q) and the radial factor of <F>^2.)
+constexpr bool radial_kind(int kind) {
+    return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE || kind == K_CYL_ISO || kind == K_BCT;
+}
 
 // Index of (a, b), 0 <= a <= b, in the triangle table of radial values.
 __device__ __forceinline__ int tri_index(int a, int b) { return b * (b + 1) / 2 + a; }
@@ -1331,11 +1339,11 @@ __device__ __forceinline__ int tri_index(int a, int b) { return b * (b + 1) / 2 
 // <F>^2 -- independent FP64 chains, twice the waves in flight.  The halves meet in LDS and
 // the block stores whole 32-byte entries, 16 bytes per thread.
 template <int KIND, int PREC>
-__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, Radial* __restrict__ tab) {
+__device__ __forceinline__ void radial_block(const DevSet& d, int m, Radial* __restrict__ tab, int blk) {
     __shared__ double stage[128][4];
     const int k = threadIdx.x & 127;
     const bool second = threadIdx.x >= 128;
-    const int n = blockIdx.x * 128 + k;
+    const int n = blk * 128 + k;
     const int total = (m + 1) * (m + 2) / 2;
     if (n < total) {
         int b = (int)((sqrt(8.0 * n + 1.0) - 1.0) * 0.5);
@@ -1349,6 +1357,7 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, Radial* _
             double pq;
             if constexpr (KIND == K_CUBE) pq = pq_cube(d, q);
             else if constexpr (KIND == K_CYL_ISO) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+            else if constexpr (KIND == K_BCT) pq = pq_cyl_transverse(d, q);
             else if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
             else pq = pq_sphere_fast(d, q);
             stage[k][0] = q;
@@ -1359,15 +1368,21 @@ __global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, Radial* _
             if constexpr (KIND == K_FCC) {
                 if constexpr (PREC == CSX_PREC_REF64) fq = fq_sphere(d, q);
                 else fq = fq_sphere_fast(d, q);
+            } else if constexpr (KIND == K_BCT) {
+                fq = fq_cyl_radial(d, q);
             }
             stage[k][2] = fq;
         }
     }
     __syncthreads();
     const int e = threadIdx.x >> 1, h = threadIdx.x & 1;
-    const int ne = blockIdx.x * 128 + e;
+    const int ne = blk * 128 + e;
     if (ne < total)
         reinterpret_cast<double2*>(tab + ne)[h] = make_double2(stage[e][2 * h], stage[e][2 * h + 1]);
+}
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(256) k_radial(const DevSet d, int m, Radial* __restrict__ tab) {
+    radial_block<KIND, PREC>(d, m, tab, blockIdx.x);
 }
 
 // Image: tile (blockIdx.x, blockIdx.y) of the grid box; store in IDX order.  rtab: radial
@@ -1536,6 +1551,8 @@ struct FitSet {
     const double* data;
     int32_t w, h;
     int32_t tiles_x, first_tile;  // tile range of this set in the batch
+    const Radial* rtab;           // radial table of the set (k_batch_radial), or null
+    int32_t rad_m, rad_pad;       // its largest |index|
     PrepIn pin;
 };
 
@@ -1565,7 +1582,12 @@ __device__ __forceinline__ void fit_tile(const FitSet& f, int tile, double* __re
     const bool masked = !inside || fit_masked(s, f.m, f.data, f.w, xc, yc);
     const int ihex = xc + s.iimin, i = yc + s.iimin;
     PixQ P3{ihex, i, 0.0, 0.0, 0.0, false};
-    const double v = pixel_value<KIND, PREC>(f.d, P3);
+    const Radial* rv = nullptr;
+    if (f.rtab) {
+        const int a = abs(i), b = abs(ihex);
+        rv = f.rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
+    }
+    const double v = pixel_value<KIND, PREC>(f.d, P3, nullptr, rv);
     if (inside) {
         const size_t idx = (size_t)(ihex - s.xmin) + (size_t)(s.xmax - s.xmin) * (size_t)(i - s.ymin);
         if (!masked) {
@@ -1660,7 +1682,12 @@ __device__ __forceinline__ void image_batch_tiles(const FitSet* __restrict__ set
     const csx_scalars& s = f.d.s;
     const int x = min(tx * kTile + px, f.w - 1), y = min(ty * kTile + py, f.h - 1);
     PixQ P3{s.xmin + x, s.ymin + y, 0.0, 0.0, 0.0, false};
-    const double v = pixel_value<KIND, PREC>(f.d, P3);
+    const Radial* rv = nullptr;
+    if (f.rtab) {
+        const int a = abs(P3.i), b = abs(P3.ihex);
+        rv = f.rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
+    }
+    const double v = pixel_value<KIND, PREC>(f.d, P3, nullptr, rv);
     __shared__ double tile[kTile][kTile + 1];
     tile[py][px] = v;
     __syncthreads();
@@ -1726,6 +1753,23 @@ __global__ void __launch_bounds__(kBlock) k_reduce_sets(const FitSet* __restrict
     if (s >= nsets) return;
     block_sum_tiles(part + sets[s].first_tile, ntiles_of[s], out + s);
     if (threadIdx.x == 0) out[s].flags = sets[s].pin.on ? sets[s].pin.flags : 0;
+}
+
+// Radial tables of a batch's sets (the radial kinds in the midpoint grid): block b fills 128
+// entries of the set s with rblock0[s] <= b < rblock0[s + 1], as k_radial does for one render.
+template <int KIND, int PREC>
+__global__ void __launch_bounds__(256) k_batch_radial(const FitSet* __restrict__ sets, int nsets,
+                                                      const int* __restrict__ rblock0) {
+    const int b = blockIdx.x;
+    if (b >= rblock0[nsets]) return;
+    int lo = 0, hi = nsets;  // the last set whose first block is <= b
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (rblock0[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const FitSet& f = sets[lo];
+    radial_block<KIND, PREC>(f.d, f.rad_m, const_cast<Radial*>(f.rtab), b - rblock0[lo]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2058,7 +2102,7 @@ struct PinnedArena {
 struct BatchGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t h2d = nullptr, prep = nullptr, fit = nullptr, red = nullptr, d2h = nullptr;
+    hipGraphNode_t h2d = nullptr, prep = nullptr, rad = nullptr, fit = nullptr, red = nullptr, d2h = nullptr;
     const void* fit_fn = nullptr;
     long long launches = 0;
 };
@@ -2487,6 +2531,7 @@ struct csx_ctx {
         size_t off_carr, off_l1, off_l2, off_l3, off_pairs, off_pairs64, off_lut, off_singles, off_iso;
         size_t off_pairs_t, off_iso_t;             // device preparation: the lattice's template (arena)
         size_t dev_l3, dev_pairs, dev_lut, dev_iso;  // and the set's tables (bdev); SIZE_MAX if none
+        size_t dev_rad;                              // the set's radial table (brad); SIZE_MAX if none
     };
     std::vector<Pending> bpend;
     // device preparation (k_prep_sets): each template's arrays in this batch's arena (l1, l2,
@@ -2495,6 +2540,12 @@ struct csx_ctx {
     std::vector<TmplAt> btmpl;
     DevBuf bdev;
     size_t bdev_size = 0;
+    // radial tables of the batch's sets (k_batch_radial): their buffer and use, and the first
+    // block of each set (nsets + 1 entries)
+    DevBuf brad;
+    size_t brad_size = 0;
+    std::vector<int> brblock0;
+    const int* brblock0_d = nullptr;
     bool bprep = false;        // the pending batch has sets whose tables the device builds
     bool device_prep = true;   // the batched fit driver stages lattice sets that way
     long long prep_sets = 0, prep_amb = 0;
@@ -2662,6 +2713,19 @@ static const void* fit_batch_kernel(int variant) {
     } else {
         return (const void*)k_fit_batch<K, P>;
     }
+}
+// Kernel of the batch's radial tables for kind K, precision P (kinds without radial tables
+// plan no blocks; their launch returns at once).
+template <int K, int P>
+static const void* batch_radial_kernel() {
+    if constexpr (radial_kind(K)) return (const void*)k_batch_radial<K, P>;
+    else return (const void*)k_batch_radial<K_SPHERE, P>;
+}
+template <int K, int P>
+static void launch_batch_radial(const FitSet* sets, int nsets, const int* rblock0, int blocks, hipStream_t st) {
+    if (blocks <= 0) return;
+    void* args[] = {(void*)&sets, (void*)&nsets, (void*)&rblock0};
+    (void)hipLaunchKernel(batch_radial_kernel<K, P>(), dim3(blocks), dim3(256), args, 0, st);
 }
 template <int K, int P>
 static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, TileSum* part, int variant,
@@ -2833,7 +2897,7 @@ void csx_destroy(csx_ctx* c) {
     for (DevBuf* b : {&c->carr, &c->carr11pm, &c->lat1, &c->lat2, &c->lat3, &c->pairs, &c->pairs64, &c->lut, &c->singles,
                       &c->iso, &c->stats, &c->radial[0], &c->radial[1], &c->cqdl, &c->cqtt, &c->cqtab, &c->tiles,
                       &c->fqs_tmp, &c->barena_d, &c->bpart_d, &c->bsums_d, &c->post[0], &c->post[1], &c->coldq,
-                      &c->bdev, &c->btiles_d})
+                      &c->bdev, &c->btiles_d, &c->brad})
         b->release();
     if (c->bgraph.exec) (void)hipGraphExecDestroy(c->bgraph.exec);
     if (c->bgraph.graph) (void)hipGraphDestroy(c->bgraph.graph);
@@ -3367,6 +3431,28 @@ int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int t
             c->btmpl.push_back(e);
         }
     }
+    // radial table of a set of a radial kind in the midpoint grid: its largest |index| over
+    // the set's pixels, its entries in the batch's radial buffer and its blocks
+    auto radial_of = [&](FitSet& f, csx_ctx::Pending& pd) {
+        pd.dev_rad = SIZE_MAX;
+        f.rtab = nullptr;
+        f.rad_m = 0;
+        const csx_scalars& ss = f.d.s;
+        if (c->brblock0.empty()) c->brblock0.push_back(0);
+        int blocks = 0;
+        if (radial_kind(kind_of(ss.combo_key)) && !ss.centerbeam && !ss.use1d) {
+            const long long m = std::max(std::max(std::llabs(ss.xmin), std::llabs((long long)ss.xmin + f.w - 1)),
+                                         std::max(std::llabs(ss.ymin), std::llabs((long long)ss.ymin + f.h - 1)));
+            if (m <= 46000) {  // the triangle index stays an int
+                const size_t entries = (size_t)((m + 1) * (m + 2) / 2);
+                pd.dev_rad = (c->brad_size + 255) & ~size_t(255);
+                c->brad_size = pd.dev_rad + entries * sizeof(Radial);
+                f.rad_m = (int)m;
+                blocks = (int)((entries + 127) / 128);
+            }
+        }
+        c->brblock0.push_back(c->brblock0.back() + blocks);
+    };
     for (int i = 0; i < n; ++i) {
         const csx_staged& st = *sts[i];
         const size_t b = base[i];
@@ -3398,6 +3484,7 @@ int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int t
             pd.off_carr = at(st.pd.off_carr);
             c->bprep = true;
             FitSet f = st.f;
+            radial_of(f, pd);
             f.first_tile = c->bntiles_total;
             c->bntiles_total += st.ntiles;
             c->bsets.push_back(f);
@@ -3415,6 +3502,7 @@ int csx_stage_commit_many(csx_ctx* c, int n, const csx_staged* const* sts, int t
         pd.off_singles = at(st.pd.off_singles);
         pd.off_iso = at(st.pd.off_iso);
         FitSet f = st.f;
+        radial_of(f, pd);
         f.first_tile = c->bntiles_total;
         c->bntiles_total += st.ntiles;
         c->bsets.push_back(f);
@@ -3441,6 +3529,8 @@ int csx_fit_batch_begin(csx_ctx* c) {
     c->btmpl.clear();
     c->bdev_size = 0;
     c->bprep = false;
+    c->brad_size = 0;
+    c->brblock0.clear();
     return CSX_OK;
 }
 
@@ -3471,10 +3561,13 @@ static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles,
     const int nsets = (int)c->bsets.size();
     const int ntiles = c->bntiles_total;
     auto al = [](size_t o) { return (o + 255) & ~size_t(255); };
+    if (c->brblock0.size() != (size_t)nsets + 1) return fail(c, CSX_E_STATE, "batch radial plan out of step");
     const size_t o_sets = al(c->barena.size);
     const size_t o_ntiles = al(o_sets + nsets * sizeof(FitSet));
-    const size_t total = o_ntiles + nsets * sizeof(int);
+    const size_t o_rblock = al(o_ntiles + nsets * sizeof(int));
+    const size_t total = o_rblock + (nsets + 1) * sizeof(int);
     HIPCHK(c, c->barena_d.ensure(total));
+    if (c->brad_size) HIPCHK(c, c->brad.ensure(c->brad_size));
     HIPCHK(c, c->btiles_d.ensure(std::max<size_t>(1, (size_t)ntiles) * sizeof(int2)));
     HIPCHK(c, c->barena.reserve(total));
     if (c->bprep) HIPCHK(c, c->bdev.ensure(std::max<size_t>(c->bdev_size, 256)));
@@ -3484,6 +3577,7 @@ static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles,
         FitSet& f = c->bsets[i];
         const auto& pd = c->bpend[i];
         f.d.carr = (const double*)(base + pd.off_carr);
+        f.rtab = pd.dev_rad == SIZE_MAX ? nullptr : (const Radial*)((char*)c->brad.p + pd.dev_rad);
         if (pd.dev_l3 != SIZE_MAX) {
             // tables the device builds (k_prep_sets), lists of the lattice's template
             PrepIn& pin = f.pin;
@@ -3517,7 +3611,9 @@ static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles,
     }
     std::memcpy(c->barena.p + o_sets, c->bsets.data(), nsets * sizeof(FitSet));
     std::memcpy(c->barena.p + o_ntiles, c->bntiles.data(), nsets * sizeof(int));
+    std::memcpy(c->barena.p + o_rblock, c->brblock0.data(), (nsets + 1) * sizeof(int));
     c->barena.size = total;
+    c->brblock0_d = (const int*)(base + o_rblock);
     if (bytes) *bytes = total;  // the caller copies (a node of the round's graph)
     else HIPCHK(c, hipMemcpyAsync(c->barena_d.p, c->barena.p, total, hipMemcpyHostToDevice, c->stream));
     *d_sets = (const FitSet*)(base + o_sets);
@@ -3531,7 +3627,7 @@ static int batch_upload(csx_ctx* c, const FitSet** d_sets, const int2** d_tiles,
 // kernel over the flat tile list, the per-set reduction and the copy of the sums to pinned
 // memory.  The graph is built once per context and kernel; every later round updates its
 // four nodes in place (sizes, pointers, grid) and launches it with one call.
-static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, const FitSet* d_sets,
+static int batch_graph_launch(csx_ctx* c, const void* fit_fn, const void* rad_fn, size_t bytes, const FitSet* d_sets,
                               const int2* d_tiles, const int* d_ntiles, int nsets, int ntiles) {
     BatchGraph& G = c->bgraph;
     TileSum* part = (TileSum*)c->bpart_d.p;
@@ -3539,6 +3635,14 @@ static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, cons
     FitSet* d_sets_w = const_cast<FitSet*>(d_sets);
     int2* d_tiles_w = const_cast<int2*>(d_tiles);
     void* prep_args[] = {(void*)&d_sets_w, (void*)&nsets, (void*)&d_ntiles, (void*)&d_tiles_w};
+    const int* d_rblock0 = c->brblock0_d;
+    void* rad_args[] = {(void*)&d_sets, (void*)&nsets, (void*)&d_rblock0};
+    // the sets' radial tables (a grid of at least one block: the kernel returns past the plan)
+    hipKernelNodeParams kq{};
+    kq.func = const_cast<void*>(rad_fn);
+    kq.gridDim = dim3(std::max(1, c->brblock0.back()));
+    kq.blockDim = dim3(256);
+    kq.kernelParams = rad_args;
     void* fit_args[] = {(void*)&d_sets, (void*)&d_tiles, (void*)&ntiles, (void*)&part};
     void* red_args[] = {(void*)&d_sets, (void*)&nsets, (void*)&part, (void*)&d_ntiles, (void*)&sums};
     // per set: its tile-list entries and, for sets staged with device preparation, its tables
@@ -3567,7 +3671,8 @@ static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, cons
         HIPCHK(c, hipGraphAddMemcpyNode1D(&G.h2d, G.graph, nullptr, 0, c->barena_d.p, c->barena.p, bytes,
                                           hipMemcpyHostToDevice));
         HIPCHK(c, hipGraphAddKernelNode(&G.prep, G.graph, &G.h2d, 1, &kp));
-        HIPCHK(c, hipGraphAddKernelNode(&G.fit, G.graph, &G.prep, 1, &kf));
+        HIPCHK(c, hipGraphAddKernelNode(&G.rad, G.graph, &G.prep, 1, &kq));
+        HIPCHK(c, hipGraphAddKernelNode(&G.fit, G.graph, &G.rad, 1, &kf));
         HIPCHK(c, hipGraphAddKernelNode(&G.red, G.graph, &G.fit, 1, &kr));
         HIPCHK(c, hipGraphAddMemcpyNode1D(&G.d2h, G.graph, &G.red, 1, c->bres.p, sums, sbytes, hipMemcpyDeviceToHost));
         HIPCHK(c, hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
@@ -3576,6 +3681,7 @@ static int batch_graph_launch(csx_ctx* c, const void* fit_fn, size_t bytes, cons
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G.exec, G.h2d, c->barena_d.p, c->barena.p, bytes,
                                                     hipMemcpyHostToDevice));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.prep, &kp));
+        HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.rad, &kq));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.fit, &kf));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G.exec, G.red, &kr));
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G.exec, G.d2h, c->bres.p, sums, sbytes, hipMemcpyDeviceToHost));
@@ -3606,11 +3712,15 @@ int csx_fit_batch_launch(csx_ctx* c, int precision) {
     if (c->use_graph) {
         const void* fn = nullptr;
         DISPATCH_KIND_FN(kind, precision, fit_batch_kernel, fn, variant);
-        if (int rc = batch_graph_launch(c, fn, bytes, d_sets, d_tiles, d_ntiles, nsets, ntiles)) return rc;
+        const void* rfn = nullptr;
+        DISPATCH_KIND_FN(kind, precision, batch_radial_kernel, rfn);
+        if (int rc = batch_graph_launch(c, fn, rfn, bytes, d_sets, d_tiles, d_ntiles, nsets, ntiles)) return rc;
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     } else {
         hipLaunchKernelGGL(k_prep_sets, dim3(nsets), dim3(kPrepBlock), 0, c->stream, const_cast<FitSet*>(d_sets), nsets,
                            d_ntiles, const_cast<int2*>(d_tiles));
+        HIPCHK(c, hipGetLastError());
+        DISPATCH_KIND(kind, precision, launch_batch_radial, d_sets, nsets, c->brblock0_d, c->brblock0.back(), c->stream);
         HIPCHK(c, hipGetLastError());
         DISPATCH_KIND(kind, precision, launch_fit_batch, d_sets, d_tiles, ntiles, (TileSum*)c->bpart_d.p, variant,
                       c->stream);
@@ -3719,6 +3829,8 @@ extern "C" int csx_render_batch_run(csx_ctx* c, int precision) {
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_prep_sets, dim3(nsets), dim3(kPrepBlock), 0, c->stream, const_cast<FitSet*>(d_sets), nsets,
                        d_ntiles, const_cast<int2*>(d_tiles));
+    HIPCHK(c, hipGetLastError());
+    DISPATCH_KIND(kind, precision, launch_batch_radial, d_sets, nsets, c->brblock0_d, c->brblock0.back(), c->stream);
     HIPCHK(c, hipGetLastError());
     DISPATCH_KIND(kind, precision, launch_image_batch, d_sets, d_tiles, ntiles, c->stream);
     HIPCHK(c, hipGetLastError());

@@ -331,15 +331,19 @@ __device__ inline double pq_cyl_long_zdir(const DevSet& d, double qxs, double qy
     return rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
 }
 
-// formfq_partCylinder, ordis 6, cs 0 (Source/sc_lib_formfq_partCylinder.h:8-206).
-__device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, double qys, double q) {
+// formfq_partCylinder, ordis 6, cs 0 (Source/sc_lib_formfq_partCylinder.h:8-206): the
+// longitudinal factor (projection qxs + qys, series below 0.6 limq1f of limql) times the radial
+// one (a function of |q| only, kept apart for the radial tables).
+__device__ inline double fq_cyl_long(const DevSet& d, double limql, double qxs, double qys) {
     const csx_scalars& s = d.s;
     const double zl = (1 - sqd(s.sigmal)) / sqd(s.sigmal);
+    if (orc_cmp(s) == 4) return 1.0;
+    if (limql < (0.6 * s.limq[9])) return series_proj(carr_of<9>(d), qxs + qys, 120);
+    return rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
+}
+__device__ inline double fq_cyl_radial(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
     const double zr = (1 - sqd(s.sigma)) / sqd(s.sigma);
-    double pql;
-    if (orc_cmp(s) == 4) pql = 1.0;
-    else if (limql < (0.6 * s.limq[9])) pql = series_proj(carr_of<9>(d), qxs + qys, 120);
-    else pql = rod_p(zl, (qxs + qys + 0.000000001) * s.length / (zl + 1));
     double pqr;
     if (q < (1.0 * s.limq[12])) {
         pqr = series_x2(carr_of<12>(d), q, 120);
@@ -354,7 +358,32 @@ __device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, 
         const double pqr3 = (2 / sqrt(kPiD)) * (pqr1 + (9 / 16.0) * pqr2);
         pqr = pqr3 * pqr3;
     }
-    return pql * pqr;
+    return pqr;
+}
+__device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, double qys, double q) {
+    return fq_cyl_long(d, limql, qxs, qys) * fq_cyl_radial(d, q);
+}
+
+// pq_cyl_long_zdir and fq_cyl_long at one pixel: the same values, with the rod average
+// evaluated once when both take it at the same argument (z_L agrees unless sigma_L = 1).
+__device__ inline void cyl_long_both(const DevSet& d, double q, double limql, double qxs, double qys, double& pl,
+                                     double& fl) {
+    const csx_scalars& s = d.s;
+    if (orc_cmp(s) == 4) {
+        pl = 1.0;
+        fl = 1.0;
+        return;
+    }
+    const bool pser = q < (0.6 * s.limq[0]);
+    const bool fser = limql < (0.6 * s.limq[9]);
+    const double zlf = (1 - sqd(s.sigmal)) / sqd(s.sigmal);
+    const double zlp = s.sigmal == 1.0 ? 0.000000001 : zlf;
+    const bool same = zlp == zlf;
+    double rod = 0.0;
+    if (!fser || (!pser && same)) rod = rod_p(zlf, (qxs + qys + 0.000000001) * s.length / (zlf + 1));
+    if (pser) pl = series_proj(carr_of<0>(d), qxs + qys, 120);
+    else pl = same ? rod : rod_p(zlp, (qxs + qys + 0.000000001) * s.length / (zlp + 1));
+    fl = fser ? series_proj(carr_of<9>(d), qxs + qys, 120) : rod;
 }
 
 }  // namespace csx
