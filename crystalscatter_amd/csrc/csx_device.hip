@@ -62,12 +62,12 @@ inline int kind_of(uint32_t key) {
     case CSX_KEY_CUBE_NONE_ZDIR: return K_CUBE_ZDIR;
     default: break;
     }
-    // spheres on the FCC, BCC, SC or BCT lattice, Gaussian or isotropic orientation, with the
-    // Lorentzian, mod. 1/2 Lorentzian, Pseudo-Voigt or Pearson VII peak (calc_GENERIC)
+    // spheres on the cubic lattices (BCC, FCC, SC, Ia3d, Pn3m, Im3m, Fd3m) or BCT, Gaussian or
+    // isotropic orientation, with the Lorentzian, Gaussian, mod. 1/2 Lorentzian, Pseudo-Voigt or
+    // Pearson VII peak (calc_GENERIC)
     const uint32_t part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    if (part == 0 && (shp == 1 || (shp >= 3 && shp <= 6)) && (lt == 4 || lt == 5 || lt == 7 || lt == 8) &&
-        (ord == 0 || ord == 7))
-        return K_FCC;
+    const bool lattice3d = (lt >= 4 && lt <= 11 && lt != 6) || lt == 17;
+    if (part == 0 && shp >= 1 && shp <= 6 && lattice3d && (ord == 0 || ord == 7)) return K_FCC;
     return -1;
 }
 // Peak shape of a lattice key (2: Gaussian).

@@ -487,23 +487,75 @@ int pattern(int h, int k, int l) {
 const int kMultCubic[15] = {0, 24, 24, 24, 6, 6, 6, 24, 24, 24, 12, 12, 12, 8, 48};
 const int kMultTetra[15] = {0, 8, 8, 8, 4, 4, 2, 8, 8, 8, 4, 8, 8, 8, 16};
 
-// Reflection conditions: FCC = cubic F--- (hkl: h+k,h+l,k+l; 0kl: k,l; hhl: h+l; 00l: l);
-// BCC = cubic I--- (hkl: h+k+l, which implies its zonal and serial conditions); SC = cubic
-// P--- (none); BCT = tetragonal I--- (hkl: h+k+l; 0kl: k+l; hk0: h+k; hhl: l; 00l: l; 0k0: k).
+// Cubic space groups by their reflection conditions (ButtonHKLClick, Source/sc_calc_generic_cpu.cpp:
+// 8444-8500 picks the group of each lattice type; the codes are SETCX's c1, c2, c7, c10 at
+// :8529-8600; the conditions are :9941-9998):
+//   integral  -1 none, 0 h+k+l even (I), 6 h+k, h+l, k+l even (F);
+//   zonal     0kl with k, l the two indices after the zero one:  0 k+l even, 3 k+l = 4n with k
+//             and l even, 4 k and l even, 5 the index that follows the zero one cyclically even;
+//   serial    00l: 0 l even, 1 l = 4n;
+//   hhl       two equal indices r and a third u != 0:  0 u even, 2 r+u even, 4 2r+u = 4n with u
+//             even, 5 r and u even;
+// -1 leaves a class unconditioned.  Every condition applies to each cyclic permutation.
+struct CubicGroup {
+    int integral, zonal, serial, hhl;
+};
+
+bool cubic_group(int ltype, CubicGroup& g) {
+    switch (ltype) {
+    case 4: case 11: g = {0, 0, 0, 0}; return true;  // BCC, Im3m: I---
+    case 5: g = {6, 4, 0, 2}; return true;           // FCC: F---
+    case 7: g = {-1, -1, -1, -1}; return true;       // SC: P---
+    case 9: g = {0, 4, 1, 4}; return true;           // Ia3d: Ia-d
+    case 10: g = {-1, 0, 0, -1}; return true;        // Pn3m: Pn--
+    case 17: g = {6, 3, 1, 2}; return true;          // Fd3m: Fd--
+    default: return false;
+    }
+}
+
+bool cubic_allowed(const CubicGroup& g, int h, int k, int l) {
+    const int v[3] = {h, k, l};
+    const bool integral = g.integral == -1 || (g.integral == 0 && ev(h + k + l)) ||
+                          (g.integral == 6 && ev(h + k) && ev(h + l) && ev(k + l));
+    if (!integral) return false;
+    for (int z = 0; z < 3; ++z) {
+        // zonal: v[z] == 0, the others in index order (0kl: k,l; h0l: h,l; hk0: h,k)
+        if (v[z] == 0) {
+            const int a = v[z == 0 ? 1 : 0], b = v[z == 2 ? 1 : 2], next = v[(z + 1) % 3];
+            switch (g.zonal) {
+            case 0: if (!ev(a + b)) return false; break;
+            case 3: if (!((a + b) % 4 == 0 && ev(a) && ev(b))) return false; break;
+            case 4: if (!(ev(a) && ev(b))) return false; break;
+            case 5: if (!ev(next)) return false; break;
+            default: break;
+            }
+        }
+        // serial: the two indices other than v[z] are zero
+        if (v[(z + 1) % 3] == 0 && v[(z + 2) % 3] == 0) {
+            if (g.serial == 0 && !ev(v[z])) return false;
+            if (g.serial == 1 && v[z] % 4 != 0) return false;
+        }
+        // hhl: the two indices other than v[z] are equal, v[z] != 0
+        const int r = v[(z + 1) % 3], u = v[z];
+        if (r == v[(z + 2) % 3] && u != 0) {
+            switch (g.hhl) {
+            case 0: if (!ev(u)) return false; break;
+            case 2: if (!ev(r + u)) return false; break;
+            case 4: if (!((2 * r + u) % 4 == 0 && ev(u))) return false; break;
+            case 5: if (!(ev(r) && ev(u))) return false; break;
+            default: break;
+            }
+        }
+    }
+    return true;
+}
+
+// Reflection conditions of a lattice type: the cubic groups above; BCT = tetragonal I---
+// (hkl: h+k+l; 0kl: k+l; hk0: h+k; hhl: l; 00l: l; 0k0: k).
 bool allowed(int ltype, int h, int k, int l) {
     if (h == 0 && k == 0 && l == 0) return false;
-    if (ltype == 7) return true;
-    if (ltype == 4) return ev(h + k + l);
-    if (ltype == 5) {
-        if (!(ev(h + k) && ev(h + l) && ev(k + l))) return false;
-        if ((h == 0 && !(ev(k) && ev(l))) || (k == 0 && !(ev(h) && ev(l))) || (l == 0 && !(ev(h) && ev(k))))
-            return false;
-        if ((h == k && l != 0 && !ev(h + l)) || (h == l && k != 0 && !ev(h + k)) || (k == l && h != 0 && !ev(h + k)))
-            return false;
-        if ((h == 0 && k == 0 && !ev(l)) || (h == 0 && l == 0 && !ev(k)) || (k == 0 && l == 0 && !ev(h)))
-            return false;
-        return true;
-    }
+    CubicGroup g;
+    if (cubic_group(ltype, g)) return cubic_allowed(g, h, k, l);
     if (!ev(h + k + l)) return false;
     if ((h == 0 && !ev(k + l)) || (k == 0 && !ev(h + l))) return false;
     if (l == 0 && !ev(h + k)) return false;
@@ -513,17 +565,20 @@ bool allowed(int ltype, int h, int k, int l) {
     return true;
 }
 
-// |F(hkl)|^2 of the FCC (4 atoms) / BCC and BCT (2 atoms) basis, float basis coordinates;
-// SC: 1 (fhkl_c, Source/sc_calc_generic_cpu.cpp:10374-10478).
+// |F(hkl)|^2 of the lattice's basis with float coordinates: FCC 4 atoms, Fd3m 8 (the FCC four
+// and the four at quarter positions), BCC, BCT, Ia3d and Im3m 2 (body-centred); SC and Pn3m 1
+// (fhkl_c, Source/sc_calc_generic_cpu.cpp:10374-10563).
 double structure_factor(int ltype, int h, int k, int l, double& sphno) {
-    if (ltype == 7) {
+    if (ltype == 7 || ltype == 10) {
         sphno = 1;
         return 1;
     }
-    static const float fcc[4][3] = {{0, 0, 0}, {0.5f, 0.5f, 0}, {0, 0.5f, 0.5f}, {0.5f, 0, 0.5f}};
-    static const float bct[2][3] = {{0, 0, 0}, {0.5f, 0.5f, 0.5f}};
-    const float(*r)[3] = ltype == 5 ? fcc : bct;
-    const int n = ltype == 5 ? 4 : 2;
+    static const float diamond[8][3] = {{0, 0, 0}, {0.5f, 0.5f, 0}, {0, 0.5f, 0.5f}, {0.5f, 0, 0.5f},
+                                        {0.25f, 0.25f, 0.25f}, {0.75f, 0.75f, 0.25f},
+                                        {0.25f, 0.75f, 0.75f}, {0.75f, 0.25f, 0.75f}};
+    static const float body[2][3] = {{0, 0, 0}, {0.5f, 0.5f, 0.5f}};
+    const float(*r)[3] = (ltype == 5 || ltype == 17) ? diamond : body;
+    const int n = ltype == 17 ? 8 : ltype == 5 ? 4 : 2;
     sphno = n;
     double c = 0.0, s = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -770,7 +825,8 @@ std::shared_ptr<const LatticeLists> lattice_lists(int ltype, int hklmax) {
     auto L = std::make_shared<LatticeLists>();
     L->ltype = ltype;
     L->hklmax = hklmax;
-    const bool cubic = ltype != 8;
+    CubicGroup g;
+    const bool cubic = cubic_group(ltype, g);
     const int H = hklmax;
     std::vector<int> v[2];
     std::vector<double> f[2];
@@ -970,8 +1026,8 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
     P.latpar2.assign(2 * 6, 0);
     P.latpar3.assign(2 * 17, 0.f);
     if (p.ltype != 12) {
-        if (p.ltype != 4 && p.ltype != 5 && p.ltype != 7 && p.ltype != 8)
-            return "lattice type not supported by the device path";
+        CubicGroup g;
+        if (!cubic_group(p.ltype, g) && p.ltype != 8) return "lattice type not supported by the device path";
         // The reflection lists, multiplicities and structure factors depend only on the
         // lattice type and HKLmax: shared by every set of that lattice (a fit's vertices and a
         // training set's images change only the lattice constants).

@@ -857,24 +857,6 @@ static int mult_tetragonal(int pat)
 
 static int even(int v) { return (v % 2) == 0; }
 
-/* FCC, cubic group F--- (ComboBoxCubic 12: c1=6, c2=4, c7=0, c10=2;
- * Source/sc_calc_generic_cpu.cpp:8352, :8593-8595, conditions :9936-9990). */
-static int allowed_fcc(int h, int k, int l)
-{
-    int c1 = 0, c2 = 1, c7 = 1, c10 = 1;
-    if (even(h + k) && even(h + l) && even(k + l)) c1 = 1;
-    if (h == 0 && !(even(k) && even(l))) c2 = 0;
-    if (k == 0 && !(even(h) && even(l))) c2 = 0;
-    if (l == 0 && !(even(h) && even(k))) c2 = 0;
-    if (h == k && l != 0 && !even(h + l)) c10 = 0;
-    if (h == l && k != 0 && !even(h + k)) c10 = 0;
-    if (k == l && h != 0 && !even(h + k)) c10 = 0;
-    if (h == 0 && k == 0 && !even(l)) c7 = 0;
-    if (h == 0 && l == 0 && !even(k)) c7 = 0;
-    if (k == 0 && l == 0 && !even(h)) c7 = 0;
-    return c1 * c2 * c7 * c10;
-}
-
 /* BCT, tetragonal group I--- (ComboBoxTetragonal 23: c1=0, c2=0, c4=0, c6=0, c7=0,
  * c8=-1, c10=0; Source/sc_calc_generic_cpu.cpp:8367, :8833-8835, conditions :9992-10056). */
 static int allowed_bct(int h, int k, int l)
@@ -891,22 +873,63 @@ static int allowed_bct(int h, int k, int l)
     return c1 * c2 * c4 * c6 * c7 * c10;
 }
 
-/* BCC, cubic group I--- (ComboBoxCubic 7: c1=0, c2=0, c7=0, c10=0; :8564-8567) and SC, P---
- * (ComboBoxCubic 0: no condition; :8534-8537), with the cubic checks of :9941-9998. */
-static int allowed_bcc(int h, int k, int l)
+/* The cubic system's reflection conditions (Source/sc_calc_generic_cpu.cpp:9941-9990) with the
+ * codes of the space group ButtonHKLClick selects for the lattice (SETCX, :8529-8600):
+ * c1 integral (-1 none, 0 h+k+l, 6 h+k,h+l,k+l), c2 zonal (0 k+l, 3 k+l=4n and k,l, 4 k,l,
+ * 5 k cyclic), c7 serial (0 l, 1 l=4n), c10 hhl (0 l, 2 h+l, 4 2h+l=4n and l, 5 h,l). */
+static int allowed_cubic(int c1, int c2, int c7, int c10, int h, int k, int l)
 {
-    int c1 = 0, c2 = 1, c7 = 1, c10 = 1;
-    if (even(h + k + l)) c1 = 1;
-    if (h == 0 && !even(k + l)) c2 = 0;
-    if (k == 0 && !even(h + l)) c2 = 0;
-    if (l == 0 && !even(h + k)) c2 = 0;
-    if (h == k && l != 0 && !even(l)) c10 = 0;
-    if (h == l && k != 0 && !even(k)) c10 = 0;
-    if (k == l && h != 0 && !even(h)) c10 = 0;
-    if (h == 0 && k == 0 && !even(l)) c7 = 0;
-    if (h == 0 && l == 0 && !even(k)) c7 = 0;
-    if (k == 0 && l == 0 && !even(h)) c7 = 0;
-    return c1 * c2 * c7 * c10;
+    int c1case = 0, c2case = 1, c7case = 1, c10case = 1;
+    if (c1 == -1) c1case = 1;
+    if (c1 == 0 && even(h + k + l)) c1case = 1;
+    if (c1 == 6 && even(h + k) && even(h + l) && even(k + l)) c1case = 1;
+    if (c2 == 0 && h == 0 && !even(k + l)) c2case = 0;
+    if (c2 == 0 && k == 0 && !even(h + l)) c2case = 0;
+    if (c2 == 0 && l == 0 && !even(h + k)) c2case = 0;
+    if (c2 == 3 && h == 0 && !(((k + l) % 4) == 0 && even(k) && even(l))) c2case = 0;
+    if (c2 == 3 && k == 0 && !(((h + l) % 4) == 0 && even(h) && even(l))) c2case = 0;
+    if (c2 == 3 && l == 0 && !(((h + k) % 4) == 0 && even(h) && even(k))) c2case = 0;
+    if (c2 == 4 && h == 0 && !(even(k) && even(l))) c2case = 0;
+    if (c2 == 4 && k == 0 && !(even(h) && even(l))) c2case = 0;
+    if (c2 == 4 && l == 0 && !(even(h) && even(k))) c2case = 0;
+    if (c2 == 5 && h == 0 && !even(k)) c2case = 0;
+    if (c2 == 5 && k == 0 && !even(l)) c2case = 0;
+    if (c2 == 5 && l == 0 && !even(h)) c2case = 0;
+    if (c10 == 0 && h == k && l != 0 && !even(l)) c10case = 0;
+    if (c10 == 0 && h == l && k != 0 && !even(k)) c10case = 0;
+    if (c10 == 0 && k == l && h != 0 && !even(h)) c10case = 0;
+    if (c10 == 2 && h == k && l != 0 && !even(h + l)) c10case = 0;
+    if (c10 == 2 && h == l && k != 0 && !even(h + k)) c10case = 0;
+    if (c10 == 2 && k == l && h != 0 && !even(h + k)) c10case = 0;
+    if (c10 == 4 && h == k && l != 0 && !(((2 * h + l) % 4) == 0 && even(l))) c10case = 0;
+    if (c10 == 4 && h == l && k != 0 && !(((2 * h + k) % 4) == 0 && even(k))) c10case = 0;
+    if (c10 == 4 && k == l && h != 0 && !(((2 * k + h) % 4) == 0 && even(h))) c10case = 0;
+    if (c10 == 5 && h == k && l != 0 && !(even(h) && even(l))) c10case = 0;
+    if (c10 == 5 && h == l && k != 0 && !(even(h) && even(k))) c10case = 0;
+    if (c10 == 5 && k == l && h != 0 && !(even(k) && even(h))) c10case = 0;
+    if (c7 == 0 && h == 0 && k == 0 && !even(l)) c7case = 0;
+    if (c7 == 0 && h == 0 && l == 0 && !even(k)) c7case = 0;
+    if (c7 == 0 && k == 0 && l == 0 && !even(h)) c7case = 0;
+    if (c7 == 1 && h == 0 && k == 0 && !((l % 4) == 0)) c7case = 0;
+    if (c7 == 1 && h == 0 && l == 0 && !((k % 4) == 0)) c7case = 0;
+    if (c7 == 1 && k == 0 && l == 0 && !((h % 4) == 0)) c7case = 0;
+    return c1case * c2case * c7case * c10case;
+}
+
+/* The cubic lattices' space-group codes (ButtonHKLClick :8444-8500 picks ComboBoxCubic per
+ * lattice type; SETCX :8529-8600): BCC and Im3m I--- (7), FCC F--- (12), SC P--- (0), Ia3d
+ * Ia-d (11), Pn3m Pn-- (5), Fd3m Fd-- (15).  Returns 0 for a lattice that is not cubic. */
+static int cubic_codes(int ltype, int* c)
+{
+    switch (ltype) {
+    case 4: case 11: c[0] = 0; c[1] = 0; c[2] = 0; c[3] = 0; return 1;
+    case 5: c[0] = 6; c[1] = 4; c[2] = 0; c[3] = 2; return 1;
+    case 7: c[0] = -1; c[1] = -1; c[2] = -1; c[3] = -1; return 1;
+    case 9: c[0] = 0; c[1] = 4; c[2] = 1; c[3] = 4; return 1;
+    case 10: c[0] = -1; c[1] = 0; c[2] = 0; c[3] = -1; return 1;
+    case 17: c[0] = 6; c[1] = 3; c[2] = 1; c[3] = 2; return 1;
+    default: return 0;
+    }
 }
 
 /* hkl loops (Source/sc_calc_generic_cpu.cpp:9712-10291): pass 1 (isotropic) over
@@ -914,7 +937,8 @@ static int allowed_bcc(int h, int k, int l)
 static void build_reflections(csxo_engine* e)
 {
     const int H = e->hklmax;
-    const int cubic = (e->ltype == 4 || e->ltype == 5 || e->ltype == 7);
+    int cc[4];
+    const int cubic = cubic_codes(e->ltype, cc);
     for (int pass = 1; pass <= 2; ++pass) {
         const int lo = (pass == 1) ? 0 : -H;
         int ct = 1;
@@ -925,13 +949,7 @@ static void build_reflections(csxo_engine* e)
                 for (int l = lo; l <= H; ++l) {
                     const int pat = hkl_pattern(h, k, l);
                     const int mult = cubic ? mult_cubic(pat) : mult_tetragonal(pat);
-                    int ok;
-                    switch (e->ltype) {
-                    case 4: ok = allowed_bcc(h, k, l); break;
-                    case 5: ok = allowed_fcc(h, k, l); break;
-                    case 7: ok = 1; break;
-                    default: ok = allowed_bct(h, k, l); break;
-                    }
+                    int ok = cubic ? allowed_cubic(cc[0], cc[1], cc[2], cc[3], h, k, l) : allowed_bct(h, k, l);
                     if (h == 0 && k == 0 && l == 0) ok = 0;
                     if (ok) {
                         lp[1][0] = ct;
@@ -943,20 +961,29 @@ static void build_reflections(csxo_engine* e)
 }
 
 /* Structure factor |sum exp(2 pi i h.r)|^2 over the basis (fhkl_c, BCC :10374-10393, FCC
- * :10397-10420, SC :10449-10455, BCT :10457-10478; basis coordinates in float as in the
- * reference). */
+ * :10397-10420, SC :10449-10455, BCT :10457-10478, Ia3d :10480-10501, Pn3m :10503-10509, Im3m
+ * :10511-10532, Fd3m :10534-10563; basis coordinates in float as in the reference). */
 static double fhkl_value(int ltype, int h, int k, int l, double* sphno)
 {
-    float r[5][4];
+    float r[9][4];
     int nn;
     memset(r, 0, sizeof(r));
-    if (ltype == 7) { *sphno = 1; return 1; }
+    if (ltype == 7 || ltype == 10) { *sphno = 1; return 1; } /* SC, Pn3m (:10449, :10509) */
     if (ltype == 5) {
         *sphno = 4; nn = 4;
         r[2][1] = 1 / 2.; r[2][2] = 1 / 2.;
         r[3][2] = 1 / 2.; r[3][3] = 1 / 2.;
         r[4][1] = 1 / 2.; r[4][3] = 1 / 2.;
-    } else {
+    } else if (ltype == 17) { /* Fd3m, 8 atoms (:10537-10548) */
+        *sphno = 8; nn = 8;
+        r[2][1] = 1 / 2.; r[2][2] = 1 / 2.;
+        r[3][2] = 1 / 2.; r[3][3] = 1 / 2.;
+        r[4][1] = 1 / 2.; r[4][3] = 1 / 2.;
+        r[5][1] = 1 / 4.; r[5][2] = 1 / 4.; r[5][3] = 1 / 4.;
+        r[6][1] = 3 / 4.; r[6][2] = 3 / 4.; r[6][3] = 1 / 4.;
+        r[7][1] = 1 / 4.; r[7][2] = 3 / 4.; r[7][3] = 3 / 4.;
+        r[8][1] = 3 / 4.; r[8][2] = 1 / 4.; r[8][3] = 3 / 4.;
+    } else { /* BCC, BCT, Ia3d, Im3m: 2 atoms */
         *sphno = 2; nn = 2;
         r[2][1] = 1 / 2.; r[2][2] = 1 / 2.; r[2][3] = 1 / 2.;
     }
@@ -1106,7 +1133,10 @@ int csxo_prepare(csxo_engine* e)
     e->peakmax1 = e->peakmax2 = 0;
     e->lat_ucvol = e->ucvol; e->lat_sphno = 1;
     if (e->lattice) {
-        if (e->ltype != 4 && e->ltype != 5 && e->ltype != 7 && e->ltype != 8) return -3; /* BCC, FCC, SC, BCT */
+        {
+            int cc[4];
+            if (!cubic_codes(e->ltype, cc) && e->ltype != 8) return -3; /* cubic lattices and BCT */
+        }
         build_reflections(e);
         e->peakmax1 = e->latpar1[1][0];
         e->peakmax2 = e->latpar2[1][0];
@@ -1568,8 +1598,8 @@ static double sphere_lattice_pixel(const csxo_engine* e, double qx, double qy, d
 static int sphere_lattice_shape_key(unsigned key)
 {
     const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    return part == 0 && (shp == 1 || (shp >= 3 && shp <= 6)) && (lt == 4 || lt == 5 || lt == 7 || lt == 8) &&
-           (ord == 0 || ord == 7);
+    int cc[4];
+    return part == 0 && shp >= 1 && shp <= 6 && (cubic_codes((int)lt, cc) || lt == 8) && (ord == 0 || ord == 7);
 }
 
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)

@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv post lattices peaks fits qmodes  # only these
+    python tests/golden/make_golden.py cube tpv post lattices peaks cubics fits qmodes  # only these
 """
 import json
 import os
@@ -143,6 +143,26 @@ PEAKS = [("fcc_sphere_lorentz", dict(CFG3, ComboBoxPeak=0, GridPoints=32)),
 
 def write_peaks(manifest):
     for name, p in PEAKS:
+        r = run_ref(p)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), image=r["image"], bounds=r["bounds"],
+                            **table_dict(r, True))
+        manifest["fixtures"][name + ".npz"] = {"params": p}
+
+
+# Spheres on the other cubic lattices of ButtonHKLClick: Ia3d (cubic group Ia-d), Pn3m (Pn--),
+# Im3m (I---) and Fd3m (Fd--, the 8-atom diamond basis), Gaussian and isotropic orientation.
+# The reference's parameter check fixes vesicles on Ia3d, Pn3m and Im3m; the driver's calculation
+# ignores that rule (bIgnoreCbs), as the fits do.
+CUBICS = [("ia3d_sphere", dict(CFG3, LType=9, uca=40, ucb=40, ucc=40, EditRadius=6, GridPoints=32)),
+          ("pn3m_sphere_iso", dict(CFG3, LType=10, uca=25, ucb=25, ucc=25, EditRadius=5, Ordis=7, GridPoints=32)),
+          ("im3m_sphere_lorentz", dict(CFG3, LType=11, uca=28, ucb=28, ucc=28, ComboBoxPeak=0, GridPoints=32)),
+          ("fd3m_sphere", dict(CFG3, LType=17, uca=45, ucb=45, ucc=45, EditRadius=7, GridPoints=32)),
+          ("fd3m_sphere_pvoigt_iso", dict(CFG3, LType=17, uca=45, ucb=45, ucc=45, EditRadius=7, Ordis=7,
+                                          ComboBoxPeak=4, EditPeakPar=60, HKLmax=7, GridPoints=32))]
+
+
+def write_cubics(manifest):
+    for name, p in CUBICS:
         r = run_ref(p)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), image=r["image"], bounds=r["bounds"],
                             **table_dict(r, True))
@@ -296,6 +316,8 @@ def main():
             write_lattices(manifest)
         if "peaks" in only:
             write_peaks(manifest)
+        if "cubics" in only:
+            write_cubics(manifest)
         if "fits" in only:
             write_fits(manifest)
         if "qmodes" in only:
@@ -378,6 +400,7 @@ def main():
     write_post(manifest)
     write_lattices(manifest)
     write_peaks(manifest)
+    write_cubics(manifest)
     write_fits(manifest)
     write_qmodes(manifest)
 

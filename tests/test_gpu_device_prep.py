@@ -64,6 +64,8 @@ CASES = [
     ("sc_sphere_iso.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
     ("fcc_sphere_iso.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
     ("bct_sphere.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
+    ("fd3m_sphere.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
+    ("ia3d_sphere.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
 ]
 
 

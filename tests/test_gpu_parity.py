@@ -323,11 +323,12 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
                                   "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz",
                                   "cube_zdir_b.npz", "fcc_sphere_lorentz.npz", "bcc_sphere_mod1lor.npz",
                                   "sc_sphere_mod2lor_iso.npz", "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz",
-                                  "fcc_sphere_pearson_iso.npz"])
+                                  "fcc_sphere_pearson_iso.npz", "ia3d_sphere.npz", "pn3m_sphere_iso.npz",
+                                  "im3m_sphere_lorentz.npz", "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz"])
 def test_generic_kernel_keys(name, prec, tol):
-    """Combinations of the reference's generic kernel served here (spheres on BCC, SC and BCT
-    lattices, other orientation distributions, isotropic cylinders, the other peak shapes on
-    the four lattices) against its own output.  The peak shapes other than the Gaussian run
+    """Combinations of the reference's generic kernel served here (spheres on the BCC, SC, BCT,
+    Ia3d, Pn3m, Im3m and Fd3m lattices, other orientation distributions, isotropic cylinders,
+    the other peak shapes) against its own output.  The peak shapes other than the Gaussian run
     the FP64 sums in either precision."""
     f = fixture(name)
     e, img = _calc(params(name), prec)
@@ -350,6 +351,22 @@ def test_peak_shapes_256_against_oracle_random_pixels(oracle_lib, name):
     r = rel_err(img[idx], ref)
     _report("%s 256^2 vs oracle" % name, r)
     assert r.max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["fd3m_sphere.npz", "ia3d_sphere.npz", "pn3m_sphere_iso.npz"])
+def test_cubic_lattices_256_against_oracle_random_pixels(oracle_lib, name):
+    """Gaussian peaks on the Fd3m, Ia3d and Pn3m lattices at 256x256 in mixed precision: 2,000
+    random pixels against the oracle (bit-exact with the reference on the 64x64 fixtures)."""
+    p = dict(params(name), GridPoints=128)
+    e, img = _calc(p, "mixed")
+    o = oracle_lib.OracleEngine(p)
+    o.prepare()
+    rng = np.random.default_rng(8)
+    idx = rng.choice(img.size, 2000, replace=False)
+    ref = o.image_points(idx % 256 - 128, idx // 256 - 128, nthreads=16)
+    r = rel_err(img[idx], ref)
+    _report("%s 256^2 vs oracle" % name, r)
+    assert r.max() < 1e-4
 
 
 def test_bcc_512_against_oracle_random_pixels(oracle_lib):
