@@ -62,11 +62,12 @@ inline int kind_of(uint32_t key) {
     case CSX_KEY_CUBE_NONE_ZDIR: return K_CUBE_ZDIR;
     default: break;
     }
-    // spheres on the cubic lattices (BCC, FCC, SC, Ia3d, Pn3m, Im3m, Fd3m) or BCT, Gaussian or
-    // isotropic orientation, with the Lorentzian, Gaussian, mod. 1/2 Lorentzian, Pseudo-Voigt or
-    // Pearson VII peak (calc_GENERIC)
+    // spheres on the cubic lattices (BCC, FCC, SC, Ia3d, Pn3m, Im3m, Fd3m), BCT, HCP or the
+    // orthorhombic lattice (the last two without reflections), Gaussian or isotropic
+    // orientation, with the Lorentzian, Gaussian, mod. 1/2 Lorentzian, Pseudo-Voigt or Pearson
+    // VII peak (calc_GENERIC)
     const uint32_t part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    const bool lattice3d = (lt >= 4 && lt <= 11 && lt != 6) || lt == 17;
+    const bool lattice3d = (lt >= 4 && lt <= 11) || lt == 17 || lt == 18;
     if (part == 0 && shp >= 1 && shp <= 6 && lattice3d && (ord == 0 || ord == 7)) return K_FCC;
     return -1;
 }
@@ -3348,6 +3349,12 @@ void csx_stage_build_rows(csx_staged* out, const csx::PreparedSet& P, int w, int
                           const double* d_data) {
     csx_staged& st = *out;
     const csx_scalars* s = &P.s;
+    if ((s->combo_key >> 8 & 0xff) != 12 && P.lists && P.lists->n2 == 0 && s->peakmax2 == 0 && s->peakmax1 == 0) {
+        // no reflection (HKLmax 0, HCP, orthorhombic): the empty tables go over as they are
+        static const float l3[17] = {};
+        return csx_stage_build(out, s, P.carr.data(), P.lists->l1.data(), P.lists->l2.data(), l3, w, h, mask,
+                               d_data);
+    }
     if (!stage_begin(st, s, P.carr.data(), w, h, mask, d_data)) return;
     auto bad = [&](const char* msg) {
         st.rc = CSX_E_ARG;

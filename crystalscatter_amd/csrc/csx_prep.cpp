@@ -551,9 +551,13 @@ bool cubic_allowed(const CubicGroup& g, int h, int k, int l) {
 }
 
 // Reflection conditions of a lattice type: the cubic groups above; BCT = tetragonal I---
-// (hkl: h+k+l; 0kl: k+l; hk0: h+k; hhl: l; 00l: l; 0k0: k).
+// (hkl: h+k+l; 0kl: k+l; hk0: h+k; hhl: l; 00l: l; 0k0: k).  HCP (hexagonal system) and the
+// orthorhombic lattice admit no reflection: the reference's condition blocks of those systems
+// are commented out, so its acceptance flag keeps its initial 0 (ButtonHKLClick,
+// Source/sc_calc_generic_cpu.cpp:8256, :10059-10145, :10249).
 bool allowed(int ltype, int h, int k, int l) {
     if (h == 0 && k == 0 && l == 0) return false;
+    if (ltype == 6 || ltype == 18) return false;
     CubicGroup g;
     if (cubic_group(ltype, g)) return cubic_allowed(g, h, k, l);
     if (!ev(h + k + l)) return false;
@@ -847,7 +851,7 @@ std::shared_ptr<const LatticeLists> lattice_lists(int ltype, int hklmax) {
     const int n1 = static_cast<int>(v[0].size() / 4), n2 = static_cast<int>(v[1].size() / 4);
     L->n1 = n1;
     L->n2 = n2;
-    L->l1.assign(static_cast<size_t>(n1 + 1) * 6, 0);
+    L->l1.assign(static_cast<size_t>(std::max(n1, 1) + 1) * 6, 0);  // row 1 holds the count
     L->l2.assign(static_cast<size_t>(n2 + 2) * 6, 0);
     L->l1[6] = n1;
     L->l2[6] = n2;
@@ -1027,7 +1031,8 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
     P.latpar3.assign(2 * 17, 0.f);
     if (p.ltype != 12) {
         CubicGroup g;
-        if (!cubic_group(p.ltype, g) && p.ltype != 8) return "lattice type not supported by the device path";
+        if (!cubic_group(p.ltype, g) && p.ltype != 6 && p.ltype != 8 && p.ltype != 18)
+            return "lattice type not supported by the device path";
         // The reflection lists, multiplicities and structure factors depend only on the
         // lattice type and HKLmax: shared by every set of that lattice (a fit's vertices and a
         // training set's images change only the lattice constants).

@@ -949,7 +949,10 @@ static void build_reflections(csxo_engine* e)
                 for (int l = lo; l <= H; ++l) {
                     const int pat = hkl_pattern(h, k, l);
                     const int mult = cubic ? mult_cubic(pat) : mult_tetragonal(pat);
-                    int ok = cubic ? allowed_cubic(cc[0], cc[1], cc[2], cc[3], h, k, l) : allowed_bct(h, k, l);
+                    /* HCP and orthorhombic: their systems' condition blocks are commented out in
+                     * the reference, so ccase keeps its initial 0 (:8256, :10059-10145) */
+                    int ok = cubic ? allowed_cubic(cc[0], cc[1], cc[2], cc[3], h, k, l)
+                                   : (e->ltype == 6 || e->ltype == 18) ? 0 : allowed_bct(h, k, l);
                     if (h == 0 && k == 0 && l == 0) ok = 0;
                     if (ok) {
                         lp[1][0] = ct;
@@ -1135,7 +1138,8 @@ int csxo_prepare(csxo_engine* e)
     if (e->lattice) {
         {
             int cc[4];
-            if (!cubic_codes(e->ltype, cc) && e->ltype != 8) return -3; /* cubic lattices and BCT */
+            if (!cubic_codes(e->ltype, cc) && e->ltype != 6 && e->ltype != 8 && e->ltype != 18)
+                return -3; /* cubic lattices, HCP, BCT, orthorhombic */
         }
         build_reflections(e);
         e->peakmax1 = e->latpar1[1][0];
@@ -1599,7 +1603,8 @@ static int sphere_lattice_shape_key(unsigned key)
 {
     const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
     int cc[4];
-    return part == 0 && shp >= 1 && shp <= 6 && (cubic_codes((int)lt, cc) || lt == 8) && (ord == 0 || ord == 7);
+    return part == 0 && shp >= 1 && shp <= 6 && (cubic_codes((int)lt, cc) || lt == 6 || lt == 8 || lt == 18) &&
+           (ord == 0 || ord == 7);
 }
 
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)

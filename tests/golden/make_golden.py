@@ -8,7 +8,7 @@ Needs the reference checkout at /root/reference; the fixtures themselves are
 self-contained and are what the tests read.
 
     make -C oracle ref && python tests/golden/make_golden.py          # all fixtures
-    python tests/golden/make_golden.py cube tpv post lattices peaks cubics fits qmodes  # only these
+    python tests/golden/make_golden.py cube tpv post lattices peaks lattices3d fits qmodes  # only these
 """
 import json
 import os
@@ -149,7 +149,7 @@ def write_peaks(manifest):
         manifest["fixtures"][name + ".npz"] = {"params": p}
 
 
-# Spheres on the other cubic lattices of ButtonHKLClick: Ia3d (cubic group Ia-d), Pn3m (Pn--),
+# Spheres on the other 3D lattices of ButtonHKLClick: Ia3d (cubic group Ia-d), Pn3m (Pn--),
 # Im3m (I---) and Fd3m (Fd--, the 8-atom diamond basis), Gaussian and isotropic orientation.
 # The reference's parameter check fixes vesicles on Ia3d, Pn3m and Im3m; the driver's calculation
 # ignores that rule (bIgnoreCbs), as the fits do.
@@ -158,7 +158,13 @@ CUBICS = [("ia3d_sphere", dict(CFG3, LType=9, uca=40, ucb=40, ucc=40, EditRadius
           ("im3m_sphere_lorentz", dict(CFG3, LType=11, uca=28, ucb=28, ucc=28, ComboBoxPeak=0, GridPoints=32)),
           ("fd3m_sphere", dict(CFG3, LType=17, uca=45, ucb=45, ucc=45, EditRadius=7, GridPoints=32)),
           ("fd3m_sphere_pvoigt_iso", dict(CFG3, LType=17, uca=45, ucb=45, ucc=45, EditRadius=7, Ordis=7,
-                                          ComboBoxPeak=4, EditPeakPar=60, HKLmax=7, GridPoints=32))]
+                                          ComboBoxPeak=4, EditPeakPar=60, HKLmax=7, GridPoints=32)),
+          # no reflection: HKLmax 0, and HCP and the orthorhombic lattice, whose systems' condition
+          # blocks the reference has commented out (ButtonHKLClick accepts nothing there)
+          ("fcc_sphere_hkl0", dict(CFG3, HKLmax=0, GridPoints=32)),
+          ("hcp_sphere", dict(CFG3, LType=6, uca=30, ucb=30, ucc=49, ucgamma=120, GridPoints=32)),
+          ("ortho_sphere_iso", dict(CFG3, LType=18, uca=30, ucb=35, ucc=40, Ordis=7, ComboBoxPeak=3,
+                                    GridPoints=32))]
 
 
 def write_cubics(manifest):
@@ -316,7 +322,7 @@ def main():
             write_lattices(manifest)
         if "peaks" in only:
             write_peaks(manifest)
-        if "cubics" in only:
+        if "lattices3d" in only:
             write_cubics(manifest)
         if "fits" in only:
             write_fits(manifest)

@@ -97,6 +97,26 @@ def test_device_tables_equal_host_tables(name, extra, keys, lo, hi, precision):
     assert sc[0] > 12 and 0 < sc[2] < sc[3]
 
 
+@pytest.mark.parametrize("name,extra,keys,lo,hi", [("fcc_sphere_hkl0.npz", {}, SPH_KEYS, SPH_LO, SPH_HI),
+                                                   ("cfg4_bct.npz", {"HKLmax": 0}, BCT_KEYS, BCT_LO, BCT_HI)],
+                         ids=["fcc-hkl0", "bct-hkl0"])
+def test_sets_without_reflections(name, extra, keys, lo, hi):
+    """A lattice set without reflections (HKLmax 0) asked for device preparation stages its
+    empty tables from the host; its sums equal those of the host path bit for bit."""
+    rng = np.random.default_rng(6)
+    params = dict(_base(name), **extra)
+    fits, keep = _fits(params, lo, hi, 4, 16, rng)
+    X = lo + (hi - lo) * rng.uniform(0.0, 1.0, (4, 8, len(lo)))
+    s_dev, t_dev, c_dev = _run(fits, keys, X, "mixed", True)
+    s_host, t_host, c_host = _run(fits, keys, X, "mixed", False)
+    assert c_dev == (0, 0) and c_host == (0, 0)
+    assert np.all(np.isfinite(s_host)) and np.all(s_host > 0)
+    np.testing.assert_array_equal(s_dev.view(np.uint64), s_host.view(np.uint64))
+    for s in range(X.shape[0] * X.shape[1]):
+        for w in range(6):
+            assert np.array_equal(t_dev[s][w].view(np.uint8), t_host[s][w].view(np.uint8)), (s, w)
+
+
 def test_batched_fits_identical_with_device_preparation():
     """Whole batched fits (BCT cylinders, mixed precision, corrected simplex): every iterate,
     hence the final vertex, value, iteration and evaluation counts, is the same whether the

@@ -324,11 +324,12 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
                                   "cube_zdir_b.npz", "fcc_sphere_lorentz.npz", "bcc_sphere_mod1lor.npz",
                                   "sc_sphere_mod2lor_iso.npz", "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz",
                                   "fcc_sphere_pearson_iso.npz", "ia3d_sphere.npz", "pn3m_sphere_iso.npz",
-                                  "im3m_sphere_lorentz.npz", "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz"])
+                                  "im3m_sphere_lorentz.npz", "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz",
+                                  "fcc_sphere_hkl0.npz", "hcp_sphere.npz", "ortho_sphere_iso.npz"])
 def test_generic_kernel_keys(name, prec, tol):
     """Combinations of the reference's generic kernel served here (spheres on the BCC, SC, BCT,
-    Ia3d, Pn3m, Im3m and Fd3m lattices, other orientation distributions, isotropic cylinders,
-    the other peak shapes) against its own output.  The peak shapes other than the Gaussian run
+    Ia3d, Pn3m, Im3m and Fd3m lattices, lattices without reflections, other orientation
+    distributions, isotropic cylinders, the other peak shapes) against its own output.  The peak shapes other than the Gaussian run
     the FP64 sums in either precision."""
     f = fixture(name)
     e, img = _calc(params(name), prec)

@@ -26,11 +26,12 @@ NO_LATTICE = [
 LATTICE = [("sphere lt%d ord%d shp%d" % (lt, od, pk + 1),
             dict(CFG3, LType=lt, Ordis=od, ComboBoxPeak=pk, uca=28, ucb=28, ucc=28 if lt != 8 else 40,
                  EditPeakPar={4: 30, 5: 1.5}.get(pk, 0), EditAzi=7))
-           for lt in (4, 5, 7, 8, 9, 10, 11, 17) for od in (0, 7) for pk in range(6)]
+           for lt in (4, 5, 6, 7, 8, 9, 10, 11, 17, 18) for od in (0, 7) for pk in range(6)]
 BCT_CYL = [("cylinder bct zdir", dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, theta=0, phi=0,
                                        EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1,
                                        Base=0, HKLmax=5, EditRadius=4, Length=30, uca=25, ucb=25, ucc=40,
                                        EditDomainSize=250, EditAzi=5))]
+BCT_CYL.append(("cylinder bct zdir hkl0", dict(BCT_CYL[0][1], HKLmax=0)))
 CASES = NO_LATTICE + LATTICE + BCT_CYL
 
 

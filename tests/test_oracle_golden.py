@@ -48,10 +48,13 @@ def test_vitess_golden_image(oracle_lib):
 
 @pytest.mark.parametrize("name", ["cfg1_sphere.npz", "cfg2_cylgauss.npz", "cfg3_fcc_g32.npz", "cube_a.npz",
                                   "cube_b.npz", "bcc_sphere.npz", "sc_sphere_iso.npz", "fcc_sphere_iso.npz",
-                                  "bct_sphere.npz", "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz", "cube_zdir.npz",
-                                  "cube_zdir_b.npz", "fcc_sphere_lorentz.npz", "bcc_sphere_mod1lor.npz",
-                                  "sc_sphere_mod2lor_iso.npz", "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz",
-                                  "fcc_sphere_pearson_iso.npz", "ia3d_sphere.npz", "pn3m_sphere_iso.npz", "im3m_sphere_lorentz.npz", "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz"])
+                                  "bct_sphere.npz", "sphere_zdir.npz", "cyl_iso.npz", "cyl_iso_b.npz",
+                                  "cube_zdir.npz", "cube_zdir_b.npz", "fcc_sphere_lorentz.npz",
+                                  "bcc_sphere_mod1lor.npz", "sc_sphere_mod2lor_iso.npz", "fcc_sphere_pvoigt.npz",
+                                  "bct_sphere_pearson.npz", "fcc_sphere_pearson_iso.npz", "ia3d_sphere.npz",
+                                  "pn3m_sphere_iso.npz", "im3m_sphere_lorentz.npz", "fd3m_sphere.npz",
+                                  "fd3m_sphere_pvoigt_iso.npz", "fcc_sphere_hkl0.npz", "hcp_sphere.npz",
+                                  "ortho_sphere_iso.npz"])
 def test_tables_and_image_bitexact(oracle_lib, name):
     f = fixture(name)
     e = _engine(oracle_lib, params(name))
