@@ -62,13 +62,16 @@ inline int kind_of(uint32_t key) {
     case CSX_KEY_CUBE_NONE_ZDIR: return K_CUBE_ZDIR;
     default: break;
     }
-    // spheres on the cubic lattices (BCC, FCC, SC, Ia3d, Pn3m, Im3m, Fd3m), BCT, HCP or the
-    // orthorhombic lattice (the last two without reflections), Gaussian or isotropic
-    // orientation, with the Lorentzian, Gaussian, mod. 1/2 Lorentzian, Pseudo-Voigt or Pearson
-    // VII peak (calc_GENERIC)
+    // The lattices served: the 2D ones (hexagonal, square, centred rectangular), the cubic ones
+    // (BCC, FCC, SC, Ia3d, Pn3m, Im3m, Fd3m), HCP, BCT and the orthorhombic one (HCP and
+    // orthorhombic without reflections).  Spheres on them with the Lorentzian, Gaussian, mod.
+    // 1/2 Lorentzian, Pseudo-Voigt or Pearson VII peak, any orientation distribution (which
+    // enters neither the sphere's factors nor these sums); perfectly oriented cylinders on
+    // them with the Gaussian peak (calc_GENERIC, and the BCT key's own kernel).
     const uint32_t part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    const bool lattice3d = (lt >= 4 && lt <= 11) || lt == 17 || lt == 18;
-    if (part == 0 && shp >= 1 && shp <= 6 && lattice3d && (ord == 0 || ord == 7)) return K_FCC;
+    const bool lattice = (lt >= 1 && lt <= 11) || lt == 17 || lt == 18;
+    if (part == 0 && shp >= 1 && shp <= 6 && lattice && ord <= 13) return K_FCC;
+    if (part == 1 && shp == 2 && lattice && ord == 6) return K_BCT;
     return -1;
 }
 // Peak shape of a lattice key (2: Gaussian).

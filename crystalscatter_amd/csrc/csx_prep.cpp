@@ -557,6 +557,7 @@ bool cubic_allowed(const CubicGroup& g, int h, int k, int l) {
 // Source/sc_calc_generic_cpu.cpp:8256, :10059-10145, :10249).
 bool allowed(int ltype, int h, int k, int l) {
     if (h == 0 && k == 0 && l == 0) return false;
+    if (ltype >= 1 && ltype <= 3) return true;  // 2D systems accept every reflection (:9936)
     if (ltype == 6 || ltype == 18) return false;
     CubicGroup g;
     if (cubic_group(ltype, g)) return cubic_allowed(g, h, k, l);
@@ -570,12 +571,20 @@ bool allowed(int ltype, int h, int k, int l) {
 }
 
 // |F(hkl)|^2 of the lattice's basis with float coordinates: FCC 4 atoms, Fd3m 8 (the FCC four
-// and the four at quarter positions), BCC, BCT, Ia3d and Im3m 2 (body-centred); SC and Pn3m 1
-// (fhkl_c, Source/sc_calc_generic_cpu.cpp:10374-10563).
+// and the four at quarter positions), BCC, BCT, Ia3d and Im3m 2 (body-centred); SC, Pn3m and
+// the hexagonal and square 2D lattices 1; the centred rectangular 2D lattice 2, in the plane
+// (fhkl_c, Source/sc_calc_generic_cpu.cpp:10330-10371, 10374-10563).
 double structure_factor(int ltype, int h, int k, int l, double& sphno) {
-    if (ltype == 7 || ltype == 10) {
+    if (ltype == 7 || ltype == 10 || ltype == 1 || ltype == 2) {
         sphno = 1;
         return 1;
+    }
+    if (ltype == 3) {
+        const float r = 0.5f;
+        const double arg = 2 * kPi * (h * r + k * r);
+        const double c = 1.0 + std::cos(arg), sn = 0.0 + std::sin(arg);
+        sphno = 2;
+        return std::round(c * c + sn * sn);
     }
     static const float diamond[8][3] = {{0, 0, 0}, {0.5f, 0.5f, 0}, {0, 0.5f, 0.5f}, {0.5f, 0, 0.5f},
                                         {0.25f, 0.25f, 0.25f}, {0.75f, 0.75f, 0.25f},
@@ -596,8 +605,11 @@ double structure_factor(int ltype, int h, int k, int l, double& sphno) {
 // Reciprocal basis rotated so the crystal direction (u,v,w) is along the beam, with the
 // reference's float intermediates (corotations, Source/sc_calc_generic_cpu.cpp:1487-2098).
 // Out: R (row-major reciprocal vectors a*,b*,c*), RT (twinned cell), vol.
+// lat2d (the 2D lattices): R is the rotated in-plane reciprocal basis of the cell (a, b,
+// gamma) with c* = (0, 0, 1), and vol the cell area (:1613-1615, :1676-1729, :1935-1943,
+// :2060-2066); RT stays the 3D twin.
 void rotated_reciprocal_basis(double a, double b, double c, double al, double be, double ga,
-                              double u, double v, double w, double psi, double R[9], double RT[9],
+                              double u, double v, double w, double psi, bool lat2d, double R[9], double RT[9],
                               double& vol) {
     al *= kPi / 180.0; be *= kPi / 180.0; ga *= kPi / 180.0; psi *= kPi / 180.0;
     const double ca = std::cos(al), cb = std::cos(be), cg = std::cos(ga), sb = std::sin(be);
@@ -678,6 +690,22 @@ void rotated_reciprocal_basis(double a, double b, double c, double al, double be
     RT[4] = (adt[0] * cdt[2] - adt[2] * cdt[0]) / vol; RT[5] = (adt[1] * cdt[0] - adt[0] * cdt[1]) / vol;
     RT[6] = (adt[1] * bdt[2] - adt[2] * bdt[1]) / vol; RT[7] = (adt[2] * bdt[0] - adt[0] * bdt[2]) / vol;
     RT[8] = (adt[0] * bdt[1] - adt[1] * bdt[0]) / vol;
+    if (lat2d) {
+        // the plane cell's vectors (a, 0) and (b cos g, b sin g) through the unit axes (float),
+        // their reciprocals over the area, rotated by q
+        const double sg = std::sin(ga);
+        const float one = 1.0f, zero = 0.0f;
+        const double aex = a * one + (b * cg) * zero + 0.0 * zero, aey = 0.0 * one + (b * sg) * zero + 0.0 * zero;
+        const double bex = a * zero + (b * cg) * one + 0.0 * zero, bey = 0.0 * zero + (b * sg) * one + 0.0 * zero;
+        const double area = a * b * sg;
+        const double as[3] = {bey / area, -bex / area, 0.0}, bs[3] = {-aey / area, aex / area, 0.0},
+                     cs[3] = {0.0, 0.0, 1.0};
+        const double* rs[3] = {as, bs, cs};
+        for (int r3 = 0; r3 < 3; ++r3)
+            for (int i = 0; i < 3; ++i)
+                R[3 * r3 + i] = q[3 * i] * rs[r3][0] + q[3 * i + 1] * rs[r3][1] + q[3 * i + 2] * rs[r3][2];
+        vol = area;
+    }
 }
 
 }  // namespace
@@ -831,18 +859,21 @@ std::shared_ptr<const LatticeLists> lattice_lists(int ltype, int hklmax) {
     L->hklmax = hklmax;
     CubicGroup g;
     const bool cubic = cubic_group(ltype, g);
+    // 2D lattices: l runs to 0 (pass 0: l = 0; pass 1: l in [-H, 0]), multiplicity 1
+    // (ButtonHKLClick :9585-9593, :9720-9722, :9812)
+    const bool sys2d = ltype >= 1 && ltype <= 3;
     const int H = hklmax;
     std::vector<int> v[2];
     std::vector<double> f[2];
     for (int pass = 0; pass < 2; ++pass) {
-        const int lo = pass == 0 ? 0 : -H;
+        const int lo = pass == 0 ? 0 : -H, lmax = sys2d ? 0 : H;
         for (int h = lo; h <= H; ++h)
             for (int k = lo; k <= H; ++k)
-                for (int l = lo; l <= H; ++l)
+                for (int l = lo; l <= lmax; ++l)
                     if (allowed(ltype, h, k, l)) {
                         const int pat = pattern(h, k, l);
                         v[pass].push_back(h); v[pass].push_back(k); v[pass].push_back(l);
-                        v[pass].push_back(cubic ? kMultCubic[pat] : kMultTetra[pat]);
+                        v[pass].push_back(sys2d ? 1 : cubic ? kMultCubic[pat] : kMultTetra[pat]);
                         double sph;
                         f[pass].push_back(structure_factor(ltype, h, k, l, sph));
                         (pass == 0 ? L->s1 : L->s2).push_back(sph);
@@ -1031,7 +1062,8 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
     P.latpar3.assign(2 * 17, 0.f);
     if (p.ltype != 12) {
         CubicGroup g;
-        if (!cubic_group(p.ltype, g) && p.ltype != 6 && p.ltype != 8 && p.ltype != 18)
+        if (!cubic_group(p.ltype, g) && !(p.ltype >= 1 && p.ltype <= 3) && p.ltype != 6 && p.ltype != 8 &&
+            p.ltype != 18)
             return "lattice type not supported by the device path";
         // The reflection lists, multiplicities and structure factors depend only on the
         // lattice type and HKLmax: shared by every set of that lattice (a fit's vertices and a
@@ -1045,7 +1077,7 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
         s.peakmax2 = n2;
         double R[9], RT[9], vol;
         rotated_reciprocal_basis(p.uca, p.ucb, p.ucc, p.ucalpha, p.ucbeta, p.ucgamma, p.ucn1, p.ucn2,
-                                 p.ucn3, p.ucpsi, R, RT, vol);
+                                 p.ucn3, p.ucpsi, p.ltype >= 1 && p.ltype <= 3, R, RT, vol);
         P.ucvol = vol;
         lat_vol = P.ucvol;
         lat_sph = n1 > 0 ? L.s1[static_cast<size_t>(n1 - 1)] : 1;
@@ -1079,8 +1111,7 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
             }
         }
         if (shp < 1 || shp > 6) return "peak shape not supported by the device path";
-        if (shp != 2 && (p.particle != 0 || (p.ordis != 0 && p.ordis != 7)))
-            return "peak shapes other than the Gaussian are served for spheres with Gaussian or isotropic orientation";
+        if (shp != 2 && p.particle != 0) return "peak shapes other than the Gaussian are served for spheres";
         if (p.twinned) return "twinning not supported by the device path";
     }
     s.sphno_cubevol = lat_vol * lat_sph;

@@ -939,20 +939,25 @@ static void build_reflections(csxo_engine* e)
     const int H = e->hklmax;
     int cc[4];
     const int cubic = cubic_codes(e->ltype, cc);
+    /* 2D lattices (:9585-9593, :9720-9722, :9812, :9936): l runs to lmax = 0 (pass 1: l = 0;
+     * pass 2: l in [-H, 0]), every (h, k, l) but (0, 0, 0) is accepted, multiplicity 1 */
+    const int sys2d = e->ltype >= 1 && e->ltype <= 3;
     for (int pass = 1; pass <= 2; ++pass) {
         const int lo = (pass == 1) ? 0 : -H;
+        const int lmax = sys2d ? 0 : H;
         int ct = 1;
         int(*lp)[6] = (pass == 1) ? e->latpar1 : e->latpar2;
         lp[1][0] = 0;
         for (int h = lo; h <= H; ++h)
             for (int k = lo; k <= H; ++k)
-                for (int l = lo; l <= H; ++l) {
+                for (int l = lo; l <= lmax; ++l) {
                     const int pat = hkl_pattern(h, k, l);
-                    const int mult = cubic ? mult_cubic(pat) : mult_tetragonal(pat);
+                    const int mult = sys2d ? 1 : cubic ? mult_cubic(pat) : mult_tetragonal(pat);
                     /* HCP and orthorhombic: their systems' condition blocks are commented out in
                      * the reference, so ccase keeps its initial 0 (:8256, :10059-10145) */
-                    int ok = cubic ? allowed_cubic(cc[0], cc[1], cc[2], cc[3], h, k, l)
-                                   : (e->ltype == 6 || e->ltype == 18) ? 0 : allowed_bct(h, k, l);
+                    int ok = sys2d ? 1
+                             : cubic ? allowed_cubic(cc[0], cc[1], cc[2], cc[3], h, k, l)
+                                     : (e->ltype == 6 || e->ltype == 18) ? 0 : allowed_bct(h, k, l);
                     if (h == 0 && k == 0 && l == 0) ok = 0;
                     if (ok) {
                         lp[1][0] = ct;
@@ -971,7 +976,19 @@ static double fhkl_value(int ltype, int h, int k, int l, double* sphno)
     float r[9][4];
     int nn;
     memset(r, 0, sizeof(r));
-    if (ltype == 7 || ltype == 10) { *sphno = 1; return 1; } /* SC, Pn3m (:10449, :10509) */
+    if (ltype == 7 || ltype == 10 || ltype == 1 || ltype == 2) { /* SC, Pn3m, hex and square 2D */
+        *sphno = 1;
+        return 1;
+    }
+    if (ltype == 3) { /* centred rectangular 2D: (0,0), (1/2,1/2) in the plane (:10352-10371) */
+        const float r2 = 1 / 2.;
+        double sc = 1.0, ss = 0.0; /* atom 1 at the origin: cos 0, sin 0 */
+        sc = sc + cos(2 * M_PI * (h * r2 + k * r2));
+        ss = ss + sin(2 * M_PI * (h * r2 + k * r2));
+        *sphno = 2;
+        (void)l;
+        return round(sc * sc + ss * ss);
+    }
     if (ltype == 5) {
         *sphno = 4; nn = 4;
         r[2][1] = 1 / 2.; r[2][2] = 1 / 2.;
@@ -999,12 +1016,14 @@ static double fhkl_value(int ltype, int h, int k, int l, double* sphno)
     return round(sc * sc + ss * ss);
 }
 
-/* corotations (Source/sc_calc_generic_cpu.cpp:1487-2098) for 3D lattices: the
- * reciprocal basis rotated so the crystal axis (u,v,w) points along the beam,
- * with the reference's float intermediates.  Outputs ri (row-major m11..m33),
- * rt (twin) and the unit-cell volume vvol. */
+/* corotations (Source/sc_calc_generic_cpu.cpp:1487-2098): the reciprocal basis rotated so
+ * the crystal axis (u,v,w) points along the beam, with the reference's float intermediates.
+ * Outputs ri (row-major m11..m33), rt (twin) and the unit-cell volume vvol.  For the 2D
+ * lattices (lat2d) ri is the rotated in-plane reciprocal basis a*, b* of the cell (a, b,
+ * gamma) and c* = (0, 0, 1), and vvol the cell area (:1613-1615, :1676-1729, :1935-1943,
+ * :2060-2066); rt stays the 3D twin. */
 static void corotations(double a, double b, double c, double alpha, double beta, double gamma,
-                        double u, double v, double w, double ephi, double* M, double* MT, double* vvol)
+                        double u, double v, double w, double ephi, int lat2d, double* M, double* MT, double* vvol)
 {
     alpha = alpha * M_PI / 180.0; beta = beta * M_PI / 180.0;
     gamma = gamma * M_PI / 180.0; ephi = ephi * M_PI / 180.0;
@@ -1078,6 +1097,23 @@ static void corotations(double a, double b, double c, double alpha, double beta,
     MT[3] = (adzt * cdyt - adyt * cdzt) / vol; MT[4] = (adxt * cdzt - adzt * cdxt) / vol; MT[5] = (adyt * cdxt - adxt * cdyt) / vol;
     MT[6] = (adyt * bdzt - adzt * bdyt) / vol; MT[7] = (adzt * bdxt - adxt * bdzt) / vol; MT[8] = (adxt * bdyt - adyt * bdxt) / vol;
     *vvol = vol;
+    if (lat2d) {
+        const double sg = sin(gamma);
+        const double m11 = a, m12 = b * cg, m13 = 0, m21 = 0, m22 = b * sg, m23 = 0;
+        const double aexd = m11 * aax + m12 * aay + m13 * aaz, aeyd = m21 * aax + m22 * aay + m23 * aaz;
+        const double bexd = m11 * bax + m12 * bay + m13 * baz, beyd = m21 * bax + m22 * bay + m23 * baz;
+        const double area = a * b * sg;
+        const double asx = beyd / area, asy = -bexd / area, asz = 0;
+        const double bsx = -aeyd / area, bsy = aexd / area, bsz = 0;
+        const double csx = 0, csy = 0, csz = 1;
+        M[0] = q11 * asx + q12 * asy + q13 * asz; M[1] = q21 * asx + q22 * asy + q23 * asz;
+        M[2] = q31 * asx + q32 * asy + q33 * asz;
+        M[3] = q11 * bsx + q12 * bsy + q13 * bsz; M[4] = q21 * bsx + q22 * bsy + q23 * bsz;
+        M[5] = q31 * bsx + q32 * bsy + q33 * bsz;
+        M[6] = q11 * csx + q12 * csy + q13 * csz; M[7] = q21 * csx + q22 * csy + q23 * csz;
+        M[8] = q31 * csx + q32 * csy + q33 * csz;
+        *vvol = area;
+    }
 }
 
 /* ========================================================================== */
@@ -1138,15 +1174,16 @@ int csxo_prepare(csxo_engine* e)
     if (e->lattice) {
         {
             int cc[4];
-            if (!cubic_codes(e->ltype, cc) && e->ltype != 6 && e->ltype != 8 && e->ltype != 18)
-                return -3; /* cubic lattices, HCP, BCT, orthorhombic */
+            if (!cubic_codes(e->ltype, cc) && !(e->ltype >= 1 && e->ltype <= 3) && e->ltype != 6 &&
+                e->ltype != 8 && e->ltype != 18)
+                return -3; /* 2D lattices, cubic lattices, HCP, BCT, orthorhombic */
         }
         build_reflections(e);
         e->peakmax1 = e->latpar1[1][0];
         e->peakmax2 = e->latpar2[1][0];
         double vv;
         corotations(e->uca, e->ucb, e->ucc, e->ucalpha, e->ucbeta, e->ucgamma, e->ucn1, e->ucn2,
-                    e->ucn3, e->ucpsi, e->ri, e->rt, &vv);
+                    e->ucn3, e->ucpsi, e->ltype >= 1 && e->ltype <= 3, e->ri, e->rt, &vv);
         e->ucvol = vv; /* corotations writes the cell volume back (:1049-1052) */
         double sphno = 1;
         for (int p = 1; p <= e->peakmax1; ++p) {
@@ -1597,14 +1634,30 @@ static double sphere_lattice_pixel(const csxo_engine* e, double qx, double qy, d
     return e->base + e->i0 * (szq * pq + e->iso * szqiso * pq) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
 }
 
-/* Spheres on the FCC, BCC, SC or BCT lattice, Gaussian or isotropic orientation, with the
- * Lorentzian, mod. 1/2 Lorentzian, Pseudo-Voigt or Pearson VII peak (calc_GENERIC). */
+/* The 2D lattices (hexagonal, square, centred rectangular) and the 3D ones served: the cubic
+ * lattices, HCP, BCT, orthorhombic. */
+static int served_lattice(unsigned lt)
+{
+    int cc[4];
+    return (lt >= 1 && lt <= 3) || cubic_codes((int)lt, cc) || lt == 6 || lt == 8 || lt == 18;
+}
+
+/* Spheres on a lattice with the Lorentzian, Gaussian, mod. 1/2 Lorentzian, Pseudo-Voigt or
+ * Pearson VII peak (calc_GENERIC; the FCC Gaussian key has its own kernel with the same sums):
+ * neither the sphere's factors nor these peak sums depend on the orientation distribution. */
 static int sphere_lattice_shape_key(unsigned key)
 {
     const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    int cc[4];
-    return part == 0 && shp >= 1 && shp <= 6 && (cubic_codes((int)lt, cc) || lt == 6 || lt == 8 || lt == 18) &&
-           (ord == 0 || ord == 7);
+    return part == 0 && shp >= 1 && shp <= 6 && served_lattice(lt) && ord <= 13;
+}
+
+/* Perfectly oriented cylinders on a lattice with the Gaussian peak: the BCT key's own kernel
+ * (Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h) and, for the other lattices, calc_GENERIC's cylinder
+ * z-dir branch (Source/sc_gpu_generic.h:76-121) with the same Gaussian sums. */
+static int cylinder_lattice_key(unsigned key)
+{
+    const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
+    return part == 1 && shp == 2 && served_lattice(lt) && ord == 6;
 }
 
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)
@@ -1634,16 +1687,6 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }
-    case 0x00020500u: /* Source/sc_gpu_pSph_pGau_lFCC_oGau.h:15-171 */
-    case 0x00020507u: /* spheres with Gaussian peaks on FCC, BCC, SC, Gaussian or isotropic orientation: */
-    case 0x00020400u: /* calc_GENERIC's sphere and Gaussian-peak branches (Source/sc_gpu_generic.h:40-52, */
-    case 0x00020407u: /* 691-880), the same sums; the orientation distribution does not enter them */
-    case 0x00020700u:
-    case 0x00020707u:
-    case 0x00020800u: /* spheres on the BCT lattice (generic kernel) */
-    case 0x00020807u:
-        *out = sphere_lattice_pixel(e, qx, qy, qz, q);
-        return 0;
     case 0x01000C00u: { /* Source/sc_gpu_pCyl_lNon_oGau.h:15-100 (fq unused without lattice) */
         double pq = 0;
         switch (e->orcase) {
@@ -1669,7 +1712,9 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         *out = e->base + e->i0 * (pq + e->iso) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }
-    case 0x01020806u: { /* Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h:16-245 */
+    default: break;
+    }
+    if (cylinder_lattice_key(key)) { /* Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h:16-245 */
         double pq = 0, fq = 0;
         switch (e->orcase) {
         case ORC_GENERAL:
@@ -1687,8 +1732,7 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         *out = e->base + e->i0 * (szq * pq + e->iso * szqiso * 1.0) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
         return 0;
     }
-    default: return -1;
-    }
+    return -1;
 }
 
 /* convert_idx2q (Source/sc_calc_generic_gpu.cu:249-287). */

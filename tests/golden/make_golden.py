@@ -164,7 +164,22 @@ CUBICS = [("ia3d_sphere", dict(CFG3, LType=9, uca=40, ucb=40, ucc=40, EditRadius
           ("fcc_sphere_hkl0", dict(CFG3, HKLmax=0, GridPoints=32)),
           ("hcp_sphere", dict(CFG3, LType=6, uca=30, ucb=30, ucc=49, ucgamma=120, GridPoints=32)),
           ("ortho_sphere_iso", dict(CFG3, LType=18, uca=30, ucb=35, ucc=40, Ordis=7, ComboBoxPeak=3,
-                                    GridPoints=32))]
+                                    GridPoints=32)),
+          # spheres under other orientation distributions (they enter neither the sphere's
+          # factors nor the peak sums), spheres on a 2D lattice
+          ("fcc_sphere_ordis1", dict(CFG3, Ordis=1, GridPoints=32)),
+          ("bcc_sphere_zdir_lorentz", dict(CFG3, LType=4, Ordis=6, ComboBoxPeak=0, GridPoints=32)),
+          ("hex_sphere", dict(CFG3, LType=1, uca=30, ucb=30, ucgamma=120, GridPoints=32)),
+          # perfectly oriented cylinders with Gaussian peaks on the 2D lattices (hexagonal, square,
+          # centred rectangular: the cell's plane basis, l in [-H, 0], multiplicity 1) and on FCC
+          ("hex_cyl_zdir", dict(CFG4_BASE, LType=1, EditRadius=4, Length=40, uca=20, ucb=20, ucc=20, ucgamma=120,
+                                EditDomainSize=250, EditAzi=5, GridPoints=32)),
+          ("sq_cyl_zdir", dict(CFG4_BASE, LType=2, EditRadius=3, Length=30, uca=18, ucb=18, ucc=18, theta=90,
+                               EditDomainSize=200, EditAzi=8, GridPoints=32)),
+          ("rect_cyl_zdir", dict(CFG4_BASE, LType=3, EditRadius=3, Length=50, uca=16, ucb=26, ucc=20,
+                                 EditDomainSize=300, EditAzi=4, HKLmax=4, GridPoints=32)),
+          ("fcc_cyl_zdir", dict(CFG4_BASE, LType=5, EditRadius=4, Length=30, uca=30, ucb=30, ucc=30,
+                                EditDomainSize=250, EditAzi=5, GridPoints=32))]
 
 
 def write_cubics(manifest):

@@ -325,7 +325,8 @@ def test_back_to_back_renders_keep_their_own_radial_tables():
                                   "sc_sphere_mod2lor_iso.npz", "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz",
                                   "fcc_sphere_pearson_iso.npz", "ia3d_sphere.npz", "pn3m_sphere_iso.npz",
                                   "im3m_sphere_lorentz.npz", "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz",
-                                  "fcc_sphere_hkl0.npz", "hcp_sphere.npz", "ortho_sphere_iso.npz"])
+                                  "fcc_sphere_hkl0.npz", "hcp_sphere.npz", "ortho_sphere_iso.npz",
+                                  "fcc_sphere_ordis1.npz", "bcc_sphere_zdir_lorentz.npz", "hex_sphere.npz"])
 def test_generic_kernel_keys(name, prec, tol):
     """Combinations of the reference's generic kernel served here (spheres on the BCC, SC, BCT,
     Ia3d, Pn3m, Im3m and Fd3m lattices, lattices without reflections, other orientation
@@ -352,6 +353,19 @@ def test_peak_shapes_256_against_oracle_random_pixels(oracle_lib, name):
     r = rel_err(img[idx], ref)
     _report("%s 256^2 vs oracle" % name, r)
     assert r.max() < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("ref64", 1e-11), ("mixed", 1e-4)])
+@pytest.mark.parametrize("name", ["hex_cyl_zdir.npz", "sq_cyl_zdir.npz", "rect_cyl_zdir.npz", "fcc_cyl_zdir.npz"])
+def test_cylinders_on_lattices(name, prec, tol):
+    """Perfectly oriented cylinders with Gaussian peaks on the hexagonal, square and centred
+    rectangular 2D lattices and on FCC (the generic kernel's cylinder z-dir branch, served by
+    the BCT kernel) against the reference's own images; tolerances as for the BCT sets."""
+    f = fixture(name)
+    e, img = _calc(params(name), prec)
+    r = rel_err(img, f["image"])
+    _report("%s %s" % (name, prec), r)
+    assert r.max() < tol
 
 
 @pytest.mark.parametrize("name", ["fd3m_sphere.npz", "ia3d_sphere.npz", "pn3m_sphere_iso.npz"])

@@ -26,12 +26,19 @@ NO_LATTICE = [
 LATTICE = [("sphere lt%d ord%d shp%d" % (lt, od, pk + 1),
             dict(CFG3, LType=lt, Ordis=od, ComboBoxPeak=pk, uca=28, ucb=28, ucc=28 if lt != 8 else 40,
                  EditPeakPar={4: 30, 5: 1.5}.get(pk, 0), EditAzi=7))
-           for lt in (4, 5, 6, 7, 8, 9, 10, 11, 17, 18) for od in (0, 7) for pk in range(6)]
+           for lt in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 17, 18) for od in (0, 7) for pk in range(6)]
+# spheres under the other orientation distributions (which enter neither the sphere's factors
+# nor the peak sums)
+LATTICE += [("sphere lt5 ord%d shp2" % od, dict(CFG3, Ordis=od, uca=28, ucb=28, ucc=28, EditAzi=7))
+            for od in (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13)]
 BCT_CYL = [("cylinder bct zdir", dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, theta=0, phi=0,
                                        EditSigma=0.06, SigmaL=0.06, RadButDebyeScherrer=1, EditQmax=1, I0=1,
                                        Base=0, HKLmax=5, EditRadius=4, Length=30, uca=25, ucb=25, ucc=40,
                                        EditDomainSize=250, EditAzi=5))]
 BCT_CYL.append(("cylinder bct zdir hkl0", dict(BCT_CYL[0][1], HKLmax=0)))
+# perfectly oriented cylinders on the other lattices (calc_GENERIC's cylinder z-dir branch)
+BCT_CYL += [("cylinder lt%d zdir" % lt, dict(BCT_CYL[0][1], LType=lt, ucc=25, ucgamma=120 if lt == 1 else 90))
+            for lt in (1, 2, 3, 4, 5, 7, 9, 10, 11, 17)]
 CASES = NO_LATTICE + LATTICE + BCT_CYL
 
 
@@ -53,7 +60,7 @@ def test_served_key_matches_oracle(oracle_lib, name, p):
 
 
 @pytest.mark.parametrize("p", [dict(CFG3, ComboBoxPeak=6), dict(CFG3, ComboBoxPeak=7),
-                               dict(CFG3, LType=1), dict(CFG3, ComboBoxParticle=2),
+                               dict(CFG3, LType=0), dict(CFG3, ComboBoxParticle=2),
                                dict(CFG3, CheckBoxTwinned=1)],
                          ids=["gamma-peak", "anisotropic-gaussian-peak", "lamellar-lattice", "disk", "twinned"])
 def test_unserved_combinations_are_refused(p):

@@ -24,7 +24,9 @@ def _host(p, ignore_cbs=True):
                                   "fcc_sphere_pvoigt.npz", "bct_sphere_pearson.npz", "fcc_sphere_pearson_iso.npz",
                                   "ia3d_sphere.npz", "pn3m_sphere_iso.npz", "im3m_sphere_lorentz.npz",
                                   "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz", "fcc_sphere_hkl0.npz",
-                                  "hcp_sphere.npz", "ortho_sphere_iso.npz"])
+                                  "hcp_sphere.npz", "ortho_sphere_iso.npz", "fcc_sphere_ordis1.npz",
+                                  "bcc_sphere_zdir_lorentz.npz", "hex_sphere.npz", "hex_cyl_zdir.npz",
+                                  "sq_cyl_zdir.npz", "rect_cyl_zdir.npz", "fcc_cyl_zdir.npz"])
 def test_tables_match_reference(name):
     f = fixture(name)
     e = _host(params(name))
@@ -38,7 +40,7 @@ def test_tables_match_reference(name):
         np.testing.assert_array_equal(e.table("latpar1")[6:], f["latpar1"][6:])
         np.testing.assert_array_equal(e.table("latpar2")[6:], f["latpar2"][6:])
         np.testing.assert_array_equal(e.table("latpar3").reshape(-1, 17)[1:, 1:14], f["latpar3"][1:, 1:14])
-        sphno = {4: 2, 5: 4, 7: 1, 8: 2, 9: 2, 10: 1, 11: 2, 17: 8}.get(int(params(name)["LType"]), 1) if s.peakmax1 else 1
+        sphno = {3: 2, 4: 2, 5: 4, 7: 1, 8: 2, 9: 2, 10: 1, 11: 2, 17: 8}.get(int(params(name)["LType"]), 1) if s.peakmax1 else 1
         assert s.sphno_cubevol == f["ucvol"][0] * sphno
     if "carr11pm" in f:
         np.testing.assert_array_equal(e.table("carr11pm").reshape(131, 131)[:121, :121], f["carr11pm"])

@@ -54,7 +54,9 @@ def test_vitess_golden_image(oracle_lib):
                                   "bct_sphere_pearson.npz", "fcc_sphere_pearson_iso.npz", "ia3d_sphere.npz",
                                   "pn3m_sphere_iso.npz", "im3m_sphere_lorentz.npz", "fd3m_sphere.npz",
                                   "fd3m_sphere_pvoigt_iso.npz", "fcc_sphere_hkl0.npz", "hcp_sphere.npz",
-                                  "ortho_sphere_iso.npz"])
+                                  "ortho_sphere_iso.npz", "fcc_sphere_ordis1.npz", "bcc_sphere_zdir_lorentz.npz",
+                                  "hex_sphere.npz", "hex_cyl_zdir.npz", "sq_cyl_zdir.npz", "rect_cyl_zdir.npz",
+                                  "fcc_cyl_zdir.npz"])
 def test_tables_and_image_bitexact(oracle_lib, name):
     f = fixture(name)
     e = _engine(oracle_lib, params(name))
