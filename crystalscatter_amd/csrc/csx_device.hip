@@ -39,7 +39,10 @@ constexpr double kLog2e = 1.4426950408889634074;
 // K_CYL_GAUSS_Q: Gaussian cylinders in physics mode with the FP32 orientation quadrature
 // (mixed precision only; chosen by csx_render, never by the key alone).
 enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4, K_CYL_GAUSS_Q = 5, K_CUBE = 6,
-            K_CYL_ISO = 7, K_CUBE_ZDIR = 8 };
+            K_CYL_ISO = 7, K_CUBE_ZDIR = 8, K_CYL_LAT = 9 };
+// Kinds with a lattice sum: spheres (and every key whose factors behave alike) on lattices,
+// isotropic cylinders on lattices, perfectly oriented cylinders on lattices.
+constexpr bool lattice_kind(int kind) { return kind == K_FCC || kind == K_CYL_LAT || kind == K_BCT; }
 
 inline int kind_of(uint32_t key) {
     switch (key) {
@@ -72,6 +75,9 @@ inline int kind_of(uint32_t key) {
     const bool lattice = (lt >= 1 && lt <= 11) || lt == 17 || lt == 18;
     if (part == 0 && shp >= 1 && shp <= 6 && lattice && ord <= 13) return K_FCC;
     if (part == 1 && shp == 2 && lattice && ord == 6) return K_BCT;
+    // isotropic cylinders on them with the Gaussian peak: P(q) and <F>^2 depend on |q| only,
+    // so they take the sums of the sphere lattices with the cylinder's factors
+    if (part == 1 && shp == 2 && lattice && ord == 7) return K_CYL_LAT;
     return -1;
 }
 // Peak shape of a lattice key (2: Gaussian).
@@ -1146,7 +1152,7 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                                               ColdPix* cp = nullptr) {
     const csx_scalars& s = d.s;
     // The mixed FCC path with a radial entry needs only |q| and the pixel indices.
-    if (!(rv && KIND == K_FCC && PREC == CSX_PREC_MIXED)) need_q(s, P3);
+    if (!(rv && (KIND == K_FCC || KIND == K_CYL_LAT) && PREC == CSX_PREC_MIXED)) need_q(s, P3);
     const double qx = P3.qx, qy = P3.qy, qz = P3.qz;
     const double q = rv ? rv->q : sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
     if constexpr (KIND == K_SPHERE) {
@@ -1210,6 +1216,11 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 pq = pq_sphere(d, q);
                 fq = fq_sphere(d, q);
             }
+        } else if constexpr (KIND == K_CYL_LAT) {
+            // calc_GENERIC, cylinders, isotropic (Source/sc_gpu_generic.h:64-74): formpq and
+            // formfq of the cylinder at |q|
+            pq = rv ? rv->pq : pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+            fq = rv ? rv->fq : fq_cyl_long_iso(d, q) * fq_cyl_radial(d, q);
         } else {
             // the transverse P(q) and the radial factor of <F>^2 depend on |q| only: from the
             // radial table when there is one (the same functions of the same |q|)
@@ -1329,7 +1340,8 @@ __device__ __forceinline__ void tile_pixel(int& px, int& py) {
 // Kinds whose form factors depend on |q| only.
 // (BCT cylinders: the transverse P(q) and the radial factor of <F>^2.)
 constexpr bool radial_kind(int kind) {
-    return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE || kind == K_CYL_ISO || kind == K_BCT;
+    return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE || kind == K_CYL_ISO || kind == K_BCT ||
+           kind == K_CYL_LAT;
 }
 
 // Index of (a, b), 0 <= a <= b, in the triangle table of radial values.
@@ -1360,7 +1372,7 @@ __device__ __forceinline__ void radial_block(const DevSet& d, int m, Radial* __r
         if (!second) {
             double pq;
             if constexpr (KIND == K_CUBE) pq = pq_cube(d, q);
-            else if constexpr (KIND == K_CYL_ISO) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+            else if constexpr (KIND == K_CYL_ISO || KIND == K_CYL_LAT) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
             else if constexpr (KIND == K_BCT) pq = pq_cyl_transverse(d, q);
             else if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
             else pq = pq_sphere_fast(d, q);
@@ -1374,6 +1386,8 @@ __device__ __forceinline__ void radial_block(const DevSet& d, int m, Radial* __r
                 else fq = fq_sphere_fast(d, q);
             } else if constexpr (KIND == K_BCT) {
                 fq = fq_cyl_radial(d, q);
+            } else if constexpr (KIND == K_CYL_LAT) {
+                fq = fq_cyl_long_iso(d, q) * fq_cyl_radial(d, q);
             }
             stage[k][2] = fq;
         }
@@ -1566,6 +1580,19 @@ struct TileSum {
 };
 
 
+// Pixel value of a set of a batch.  The kinds without a lattice read the set's constants from a
+// private copy: read through the reference into the sets array, the isotropic cylinder's P(q)
+// came out wrong above its series radius.  The lattice kinds read them in place.
+template <int KIND, int PREC>
+__device__ __forceinline__ double set_pixel(const FitSet& f, PixQ& P3, const Radial* rv) {
+    if constexpr (lattice_kind(KIND)) {
+        return pixel_value<KIND, PREC>(f.d, P3, nullptr, rv);
+    } else {
+        const DevSet d = f.d;
+        return pixel_value<KIND, PREC>(d, P3, nullptr, rv);
+    }
+}
+
 // One fit tile: per-pixel FQSVERGL terms, NaN counted, block-reduced in FP64.
 template <int KIND, int PREC>
 __device__ __forceinline__ void fit_tile(const FitSet& f, int tile, double* __restrict__ fqs_out, TileSum* __restrict__ out) {
@@ -1591,7 +1618,7 @@ __device__ __forceinline__ void fit_tile(const FitSet& f, int tile, double* __re
         const int a = abs(i), b = abs(ihex);
         rv = f.rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
     }
-    const double v = pixel_value<KIND, PREC>(f.d, P3, nullptr, rv);
+    const double v = set_pixel<KIND, PREC>(f, P3, rv);
     if (inside) {
         const size_t idx = (size_t)(ihex - s.xmin) + (size_t)(s.xmax - s.xmin) * (size_t)(i - s.ymin);
         if (!masked) {
@@ -1691,7 +1718,7 @@ __device__ __forceinline__ void image_batch_tiles(const FitSet* __restrict__ set
         const int a = abs(P3.i), b = abs(P3.ihex);
         rv = f.rtab + (a <= b ? tri_index(a, b) : tri_index(b, a));
     }
-    const double v = pixel_value<KIND, PREC>(f.d, P3, nullptr, rv);
+    const double v = set_pixel<KIND, PREC>(f, P3, rv);
     __shared__ double tile[kTile][kTile + 1];
     tile[py][px] = v;
     __syncthreads();
@@ -1773,7 +1800,10 @@ __global__ void __launch_bounds__(256) k_batch_radial(const FitSet* __restrict__
         else hi = mid;
     }
     const FitSet& f = sets[lo];
-    radial_block<KIND, PREC>(f.d, f.rad_m, const_cast<Radial*>(f.rtab), b - rblock0[lo]);
+    // the set's constants as a private copy: read through the reference into the sets array,
+    // the isotropic cylinders' entries came out wrong (zero P(q) above the series radius)
+    const DevSet d = f.d;
+    radial_block<KIND, PREC>(d, f.rad_m, const_cast<Radial*>(f.rtab), b - rblock0[lo]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2296,7 +2326,7 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
 void set_consts(DevSet& d) {
     const csx_scalars& s = d.s;
     const double zr = (1 - s.sigma * s.sigma) / (s.sigma * s.sigma);
-    if (kind_of(s.combo_key) == K_BCT) {
+    if (kind_of(s.combo_key) == K_BCT || kind_of(s.combo_key) == K_CYL_LAT) {
         d.cyl_g0 = csx::ref_gamma(zr + 1);
         d.cyl_g1 = csx::ref_gamma(zr - 1 / 2.0);
         d.cyl_g2 = csx::ref_gamma(zr - 3 / 2.0);
@@ -2672,7 +2702,7 @@ static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsign
                          const Radial* rtab, const ColdQueue* cq, hipEvent_t* ev, hipStream_t st) {
     dim3 grid((g.xmax - g.xmin + kTile - 1) / kTile, (g.ymax - g.ymin + kTile - 1) / kTile);
     if (ev) (void)hipEventRecord(ev[0], st);
-    if constexpr ((K == K_FCC || K == K_BCT) && P == CSX_PREC_MIXED) {
+    if constexpr (lattice_kind(K) && P == CSX_PREC_MIXED) {
         if (cq) {
             hipLaunchKernelGGL((k_image_lat_hot<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab, *cq);
             if (ev) (void)hipEventRecord(ev[1], st);
@@ -2680,7 +2710,7 @@ static void launch_image(const DevSet& d, const csx_grid& g, double* out, unsign
             return;
         }
     }
-    if constexpr (K == K_FCC || K == K_BCT)
+    if constexpr (lattice_kind(K))
         hipLaunchKernelGGL((k_image_lat<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
     else
         hipLaunchKernelGGL((k_image<K, P>), grid, dim3(kBlock), 0, st, d, g, out, stats, rtab);
@@ -2707,12 +2737,12 @@ static void launch_fit_one(const FitSet& f, int ntiles, double* fqs, TileSum* pa
 // Kernel of the batched objective for kind K, precision P, variant v (one workgroup per tile).
 template <int K, int P>
 static const void* fit_batch_kernel(int variant) {
-    if constexpr ((K == K_FCC || K == K_BCT) && P == CSX_PREC_MIXED) {
+    if constexpr (lattice_kind(K) && P == CSX_PREC_MIXED) {
         static const void* const tab[kTuneVariants] = {
             (const void*)k_fit_batch_lat<K, P>, (const void*)k_fit_batch_free<K, P>,
             (const void*)k_fit_batch_cap<K, P, 5>, (const void*)k_fit_batch_cap<K, P, 6>};
         return tab[(variant >= 0 && variant < kTuneVariants) ? variant : 0];
-    } else if constexpr (K == K_FCC || K == K_BCT) {
+    } else if constexpr (lattice_kind(K)) {
         return (const void*)k_fit_batch_lat<K, P>;
     } else {
         return (const void*)k_fit_batch<K, P>;
@@ -2740,7 +2770,7 @@ static void launch_fit_batch(const FitSet* sets, const int2* tiles, int ntiles, 
 
 template <int K, int P>
 static void launch_image_batch(const FitSet* sets, const int2* tiles, int ntiles, hipStream_t st) {
-    if constexpr (K == K_FCC || K == K_BCT)
+    if constexpr (lattice_kind(K))
         hipLaunchKernelGGL((k_image_batch_lat<K, P>), dim3(ntiles), dim3(kBlock), 0, st, sets, tiles, ntiles);
     else
         hipLaunchKernelGGL((k_image_batch<K, P>), dim3(ntiles), dim3(kBlock), 0, st, sets, tiles, ntiles);
@@ -2794,6 +2824,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     case K_CUBE: if (prec) FN<K_CUBE, 1>(__VA_ARGS__); else FN<K_CUBE, 0>(__VA_ARGS__); break; \
     case K_CYL_ISO: if (prec) FN<K_CYL_ISO, 1>(__VA_ARGS__); else FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
     case K_CUBE_ZDIR: if (prec) FN<K_CUBE_ZDIR, 1>(__VA_ARGS__); else FN<K_CUBE_ZDIR, 0>(__VA_ARGS__); break; \
+    case K_CYL_LAT: if (prec) FN<K_CYL_LAT, 1>(__VA_ARGS__); else FN<K_CYL_LAT, 0>(__VA_ARGS__); break; \
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
@@ -2808,6 +2839,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     case K_CUBE: out = (prec) ? FN<K_CUBE, 1>(__VA_ARGS__) : FN<K_CUBE, 0>(__VA_ARGS__); break; \
     case K_CYL_ISO: out = (prec) ? FN<K_CYL_ISO, 1>(__VA_ARGS__) : FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
     case K_CUBE_ZDIR: out = (prec) ? FN<K_CUBE_ZDIR, 1>(__VA_ARGS__) : FN<K_CUBE_ZDIR, 0>(__VA_ARGS__); break; \
+    case K_CYL_LAT: out = (prec) ? FN<K_CYL_LAT, 1>(__VA_ARGS__) : FN<K_CYL_LAT, 0>(__VA_ARGS__); break; \
     default: out = (prec) ? FN<K_BCT, 1>(__VA_ARGS__) : FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
@@ -2830,7 +2862,7 @@ static int render_launch(csx_ctx* c, int kind, int precision, const csx_grid& g,
     // most one slot per wave), unless unpaired reflections send every pixel to FP64.
     ColdQueue cq{};
     const ColdQueue* cqp = nullptr;
-    if ((kind == K_FCC || kind == K_BCT) && precision == CSX_PREC_MIXED && c->set.nsingle == 0) {
+    if (lattice_kind(kind) && precision == CSX_PREC_MIXED && c->set.nsingle == 0) {
         const size_t nwaves = 4 * (size_t)((g.xmax - g.xmin + kTile - 1) / kTile) *
                               (size_t)((g.ymax - g.ymin + kTile - 1) / kTile);
         const size_t need = 256 + nwaves * (sizeof(unsigned long long) + 64 * sizeof(ColdPix));

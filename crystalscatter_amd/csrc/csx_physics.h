@@ -360,6 +360,18 @@ __device__ inline double fq_cyl_radial(const DevSet& d, double q) {
     }
     return pqr;
 }
+// Longitudinal part of <F>^2 for isotropic orientation (formfq_partCylinder ordis 7,
+// Source/sc_lib_formfq_partCylinder.h:27-53): the series below 0.6 limq1f, else the closed form.
+__device__ inline double fq_cyl_long_iso(const DevSet& d, double q) {
+    const csx_scalars& s = d.s;
+    const double zl = (1 - sqd(s.sigmal)) / sqd(s.sigmal);
+    if (q < (0.6 * s.limq[9])) return series_x2(carr_of<9>(d), q, 120);
+    const double arglq = q * s.length / (zl + 1);
+    double pql = (kPiD / (2.0 * zl)) * (1 / arglq);
+    pql = pql - (1 / (2.0 * zl * (zl - 1) * arglq * arglq)) * cos((zl - 1) * atan(2.0 * arglq)) /
+                    pow(1.0 + 4 * arglq * arglq, (zl - 1) / 2.0);
+    return pql;
+}
 __device__ inline double fq_cyl_zdir(const DevSet& d, double limql, double qxs, double qys, double q) {
     return fq_cyl_long(d, limql, qxs, qys) * fq_cyl_radial(d, q);
 }

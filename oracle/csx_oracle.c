@@ -1459,6 +1459,24 @@ static double fq_cylinder(const csxo_engine* e, double limql, double qxs, double
     const double zl = (1 - sq(e->sigmal)) / sq(e->sigmal);
     const double zr = (1 - sq(e->sigma)) / sq(e->sigma);
     double pql = 1.0, pqr;
+    if (e->ordis == 7) { /* isotropic (Source/sc_lib_formfq_partCylinder.h:27-53) */
+        if (q < (0.6 * e->limq[9])) {
+            double s = 1.0, old = 0.0, qq = 1.0;
+            for (int n = 1; n <= 120; ++n) {
+                qq = qq * q * q;
+                s += e->C.c1f[n] * qq;
+                const double d = fabs((s - old) / s);
+                if (d < 0.0001) break;
+                old = s;
+            }
+            pql = s;
+        } else {
+            const double arglq = q * e->length / (zl + 1);
+            pql = (M_PI / (2.0 * zl)) * (1 / arglq);
+            pql = pql - (1 / (2.0 * zl * (zl - 1) * arglq * arglq)) * cos((zl - 1) * atan(2.0 * arglq)) /
+                            pow(1.0 + 4 * arglq * arglq, (zl - 1) / 2.0);
+        }
+    }
     if (e->ordis == 6) {
         if (orc_is(e, 4)) pql = 1.0;
         else if (limql < (0.6 * e->limq[9])) {
@@ -1651,13 +1669,13 @@ static int sphere_lattice_shape_key(unsigned key)
     return part == 0 && shp >= 1 && shp <= 6 && served_lattice(lt) && ord <= 13;
 }
 
-/* Perfectly oriented cylinders on a lattice with the Gaussian peak: the BCT key's own kernel
- * (Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h) and, for the other lattices, calc_GENERIC's cylinder
- * z-dir branch (Source/sc_gpu_generic.h:76-121) with the same Gaussian sums. */
+/* Perfectly oriented (ordis 6) or isotropic (7) cylinders on a lattice with the Gaussian peak:
+ * the BCT z-dir key's own kernel (Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h) and calc_GENERIC's
+ * cylinder branches (Source/sc_gpu_generic.h:62-121) with the same Gaussian sums. */
 static int cylinder_lattice_key(unsigned key)
 {
     const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    return part == 1 && shp == 2 && served_lattice(lt) && ord == 6;
+    return part == 1 && shp == 2 && served_lattice(lt) && (ord == 6 || ord == 7);
 }
 
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)
@@ -1713,6 +1731,15 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         return 0;
     }
     default: break;
+    }
+    if (cylinder_lattice_key(key) && e->ordis == 7) { /* Source/sc_gpu_generic.h:64-74, 536-880 */
+        const double pq = pq_cylinder(e, qx, qy, q);
+        const double fq = fq_cylinder(e, q, qx, qy, q);
+        double I, szqiso;
+        lattice_sums(e, qx, qy, qz, q, &I, &szqiso);
+        const double szq = (1 + (fq / pq) * (I - 1) * exp(-e->dwfactor * q * q));
+        *out = e->base + e->i0 * (szq * pq + e->iso * szqiso * 1.0) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+        return 0;
     }
     if (cylinder_lattice_key(key)) { /* Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h:16-245 */
         double pq = 0, fq = 0;

@@ -179,7 +179,12 @@ CUBICS = [("ia3d_sphere", dict(CFG3, LType=9, uca=40, ucb=40, ucc=40, EditRadius
           ("rect_cyl_zdir", dict(CFG4_BASE, LType=3, EditRadius=3, Length=50, uca=16, ucb=26, ucc=20,
                                  EditDomainSize=300, EditAzi=4, HKLmax=4, GridPoints=32)),
           ("fcc_cyl_zdir", dict(CFG4_BASE, LType=5, EditRadius=4, Length=30, uca=30, ucb=30, ucc=30,
-                                EditDomainSize=250, EditAzi=5, GridPoints=32))]
+                                EditDomainSize=250, EditAzi=5, GridPoints=32)),
+          # isotropic cylinders (L/R >= 2) with Gaussian peaks on FCC and on the hexagonal 2D lattice
+          ("fcc_cyl_iso", dict(CFG4_BASE, LType=5, Ordis=7, EditRadius=3, Length=30, uca=30, ucb=30, ucc=30,
+                               EditDomainSize=250, EditAzi=5, GridPoints=32)),
+          ("hex_cyl_iso", dict(CFG4_BASE, LType=1, Ordis=7, EditRadius=4, Length=40, uca=20, ucb=20, ucc=20,
+                               ucgamma=120, EditDomainSize=200, EditAzi=6, iso=0.2, GridPoints=32))]
 
 
 def write_cubics(manifest):

@@ -214,3 +214,50 @@ def test_tuning_variants_give_the_same_sums(key_params):
         np.testing.assert_array_equal(got[0], ref[0])
         np.testing.assert_array_equal(got[1], ref[1])
     assert L.csx_tune_force(ctx._h, -1) == 0
+
+
+BATCH_KINDS = {
+    "sphere-iso": dict(LType=12, ComboBoxParticle=0, Ordis=7, EditRadius=5, EditSigma=0.1, EditQmax=2, I0=1, Base=0),
+    "sphere-zdir": dict(LType=12, ComboBoxParticle=0, Ordis=6, EditRadius=5, EditSigma=0.1, EditQmax=2, I0=1, Base=0),
+    "cube-iso": dict(LType=12, ComboBoxParticle=4, Ordis=7, EditRadius=10, EditSigma=0.1, EditQmax=2, I0=1, Base=0),
+    "cylinder-iso": dict(LType=12, ComboBoxParticle=1, Ordis=7, EditRadius=3, Length=30, EditSigma=0.06, SigmaL=0.06,
+                         EditQmax=1, I0=1, Base=0),
+    "cylinder-zdir": dict(LType=12, ComboBoxParticle=1, Ordis=6, EditRadius=3, Length=30, EditSigma=0.06,
+                          SigmaL=0.06, theta=30, phi=45, EditQmax=1, I0=1, Base=0),
+    "cylinder-iso-fcc": dict(LType=5, ComboBoxParticle=1, Ordis=7, ComboBoxPeak=1, EditRadius=3, Length=30,
+                             EditSigma=0.06, SigmaL=0.06, uca=30, ucb=30, ucc=30, EditDomainSize=250, EditAzi=5,
+                             RadButDebyeScherrer=1, HKLmax=5, EditQmax=1, I0=1, Base=0),
+}
+
+
+@pytest.mark.parametrize("prec", ["ref64", "mixed"])
+@pytest.mark.parametrize("kind", sorted(BATCH_KINDS))
+def test_batched_images_equal_single_renders(kind, prec):
+    """Every kind's images staged into one csx_render_batch_run (three parameter sets, radius
+    and cell varied) equal the sets' own renders -- the batch reads each set's constants from
+    the sets array, the single render from its kernel arguments: bit for bit, except the mixed
+    lattice sums, which the batch evaluates in one pass (within 1e-6, as for the BCT sets)."""
+    import torch
+    from crystalscatter_amd import render_batch
+    base = dict(BATCH_KINDS[kind], GridPoints=16)
+    sets = [dict(base, EditRadius=base["EditRadius"] * f, uca=30 * f, ucb=30 * f, ucc=30 * f) for f in (1.0, 1.1, 0.9)]
+    engines, singles = [], []
+    for p in sets:
+        e = SasCalc(device=0, params=p)
+        e.set("ignore_cbs", 1)
+        singles.append(torch.tensor(e.doCalculation(prec).ravel()))
+        b = SasCalc(device=-1, params=p)
+        b.set("ignore_cbs", 1)
+        engines.append(b)
+    ctx = Context(0)
+    out = torch.full((len(sets), singles[0].numel()), -7.0, dtype=torch.float64, device="cuda")
+    render_batch(ctx, engines, precision=prec, out=out)
+    torch.cuda.synchronize()
+    got = out.cpu()
+    ctx.close()
+    for k, ref in enumerate(singles):
+        assert torch.isfinite(ref).all() and (ref != 0).any(), (kind, k)
+        if prec == "mixed" and base["LType"] != 12:
+            assert rel_err(got[k].numpy(), ref.numpy()).max() <= 1e-6, (kind, k)
+        else:
+            assert torch.equal(got[k], ref), (kind, prec, k, int((got[k] != ref).sum()))

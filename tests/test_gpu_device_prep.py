@@ -66,6 +66,8 @@ CASES = [
     ("bct_sphere.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
     ("fd3m_sphere.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
     ("ia3d_sphere.npz", {}, SPH_KEYS, SPH_LO, SPH_HI, "mixed"),
+    ("fcc_cyl_iso.npz", {}, BCT_KEYS, np.array([2, 20, 20, 20, 100, 2.0]), np.array([4, 60, 40, 50, 400, 10.0]),
+     "mixed"),
 ]
 
 

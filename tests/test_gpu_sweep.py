@@ -39,6 +39,9 @@ BCT_CYL.append(("cylinder bct zdir hkl0", dict(BCT_CYL[0][1], HKLmax=0)))
 # perfectly oriented cylinders on the other lattices (calc_GENERIC's cylinder z-dir branch)
 BCT_CYL += [("cylinder lt%d zdir" % lt, dict(BCT_CYL[0][1], LType=lt, ucc=25, ucgamma=120 if lt == 1 else 90))
             for lt in (1, 2, 3, 4, 5, 7, 9, 10, 11, 17)]
+# isotropic cylinders (L/R >= 2) on lattices
+BCT_CYL += [("cylinder lt%d iso" % lt, dict(BCT_CYL[0][1], LType=lt, Ordis=7, ucc=25, ucgamma=120 if lt == 1 else 90))
+            for lt in (1, 3, 4, 5, 7, 8, 17)]
 CASES = NO_LATTICE + LATTICE + BCT_CYL
 
 

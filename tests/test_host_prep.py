@@ -26,7 +26,8 @@ def _host(p, ignore_cbs=True):
                                   "fd3m_sphere.npz", "fd3m_sphere_pvoigt_iso.npz", "fcc_sphere_hkl0.npz",
                                   "hcp_sphere.npz", "ortho_sphere_iso.npz", "fcc_sphere_ordis1.npz",
                                   "bcc_sphere_zdir_lorentz.npz", "hex_sphere.npz", "hex_cyl_zdir.npz",
-                                  "sq_cyl_zdir.npz", "rect_cyl_zdir.npz", "fcc_cyl_zdir.npz"])
+                                  "sq_cyl_zdir.npz", "rect_cyl_zdir.npz", "fcc_cyl_zdir.npz", "fcc_cyl_iso.npz",
+                                  "hex_cyl_iso.npz"])
 def test_tables_match_reference(name):
     f = fixture(name)
     e = _host(params(name))
