@@ -39,10 +39,10 @@ constexpr double kLog2e = 1.4426950408889634074;
 // K_CYL_GAUSS_Q: Gaussian cylinders in physics mode with the FP32 orientation quadrature
 // (mixed precision only; chosen by csx_render, never by the key alone).
 enum Kind { K_SPHERE = 0, K_FCC = 1, K_CYL_GAUSS = 2, K_CYL_ZDIR = 3, K_BCT = 4, K_CYL_GAUSS_Q = 5, K_CUBE = 6,
-            K_CYL_ISO = 7, K_CUBE_ZDIR = 8, K_CYL_LAT = 9 };
+            K_CYL_ISO = 7, K_CUBE_ZDIR = 8, K_ISO_LAT = 9 };
 // Kinds with a lattice sum: spheres (and every key whose factors behave alike) on lattices,
 // isotropic cylinders on lattices, perfectly oriented cylinders on lattices.
-constexpr bool lattice_kind(int kind) { return kind == K_FCC || kind == K_CYL_LAT || kind == K_BCT; }
+constexpr bool lattice_kind(int kind) { return kind == K_FCC || kind == K_ISO_LAT || kind == K_BCT; }
 
 inline int kind_of(uint32_t key) {
     switch (key) {
@@ -75,11 +75,13 @@ inline int kind_of(uint32_t key) {
     const bool lattice = (lt >= 1 && lt <= 11) || lt == 17 || lt == 18;
     if (part == 0 && shp >= 1 && shp <= 6 && lattice && ord <= 13) return K_FCC;
     if (part == 1 && shp == 2 && lattice && ord == 6) return K_BCT;
-    // isotropic cylinders on them with the Gaussian peak: P(q) and <F>^2 depend on |q| only,
-    // so they take the sums of the sphere lattices with the cylinder's factors
-    if (part == 1 && shp == 2 && lattice && ord == 7) return K_CYL_LAT;
+    // isotropic cylinders and cubes on them with the Gaussian peak: P(q) and <F>^2 depend on
+    // |q| only, so they take the sums of the sphere lattices with the particle's factors
+    if ((part == 1 || part == 4) && shp == 2 && lattice && ord == 7) return K_ISO_LAT;
     return -1;
 }
+// Isotropic cubes on a lattice (kind K_ISO_LAT; the other particle of that kind is the cylinder).
+__host__ __device__ inline bool lattice_cube(uint32_t key) { return (key >> 24) == 4; }
 // Peak shape of a lattice key (2: Gaussian).
 inline int shape_of(uint32_t key) { return (int)(key >> 16 & 0xff); }
 // The arithmetic a request runs in: sets with a peak shape other than the Gaussian have only
@@ -1152,7 +1154,7 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                                               ColdPix* cp = nullptr) {
     const csx_scalars& s = d.s;
     // The mixed FCC path with a radial entry needs only |q| and the pixel indices.
-    if (!(rv && (KIND == K_FCC || KIND == K_CYL_LAT) && PREC == CSX_PREC_MIXED)) need_q(s, P3);
+    if (!(rv && (KIND == K_FCC || KIND == K_ISO_LAT) && PREC == CSX_PREC_MIXED)) need_q(s, P3);
     const double qx = P3.qx, qy = P3.qy, qz = P3.qz;
     const double q = rv ? rv->q : sqrt(qx * qx + qy * qy + qz * qz) + 0.000000001;
     if constexpr (KIND == K_SPHERE) {
@@ -1216,11 +1218,19 @@ __device__ __forceinline__ double pixel_value(const DevSet& d, PixQ& P3, unsigne
                 pq = pq_sphere(d, q);
                 fq = fq_sphere(d, q);
             }
-        } else if constexpr (KIND == K_CYL_LAT) {
-            // calc_GENERIC, cylinders, isotropic (Source/sc_gpu_generic.h:64-74): formpq and
-            // formfq of the cylinder at |q|
-            pq = rv ? rv->pq : pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
-            fq = rv ? rv->fq : fq_cyl_long_iso(d, q) * fq_cyl_radial(d, q);
+        } else if constexpr (KIND == K_ISO_LAT) {
+            // calc_GENERIC, isotropic cylinders (Source/sc_gpu_generic.h:64-74: formpq and formfq
+            // of the cylinder at |q|) and homogeneous cubes (:295-302: formpq, and fq = pq)
+            if (rv) {
+                pq = rv->pq;
+                fq = rv->fq;
+            } else if (lattice_cube(s.combo_key)) {
+                pq = pq_cube(d, q);
+                fq = pq;
+            } else {
+                pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+                fq = fq_cyl_long_iso(d, q) * fq_cyl_radial(d, q);
+            }
         } else {
             // the transverse P(q) and the radial factor of <F>^2 depend on |q| only: from the
             // radial table when there is one (the same functions of the same |q|)
@@ -1341,7 +1351,7 @@ __device__ __forceinline__ void tile_pixel(int& px, int& py) {
 // (BCT cylinders: the transverse P(q) and the radial factor of <F>^2.)
 constexpr bool radial_kind(int kind) {
     return kind == K_SPHERE || kind == K_FCC || kind == K_CUBE || kind == K_CYL_ISO || kind == K_BCT ||
-           kind == K_CYL_LAT;
+           kind == K_ISO_LAT;
 }
 
 // Index of (a, b), 0 <= a <= b, in the triangle table of radial values.
@@ -1372,7 +1382,9 @@ __device__ __forceinline__ void radial_block(const DevSet& d, int m, Radial* __r
         if (!second) {
             double pq;
             if constexpr (KIND == K_CUBE) pq = pq_cube(d, q);
-            else if constexpr (KIND == K_CYL_ISO || KIND == K_CYL_LAT) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+            else if constexpr (KIND == K_CYL_ISO) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
+            else if (KIND == K_ISO_LAT && lattice_cube(d.s.combo_key)) pq = pq_cube(d, q);
+            else if constexpr (KIND == K_ISO_LAT) pq = pq_cyl_long_iso(d, q) * pq_cyl_transverse(d, q);
             else if constexpr (KIND == K_BCT) pq = pq_cyl_transverse(d, q);
             else if constexpr (KIND == K_SPHERE || PREC == CSX_PREC_REF64) pq = pq_sphere(d, q);
             else pq = pq_sphere_fast(d, q);
@@ -1386,8 +1398,8 @@ __device__ __forceinline__ void radial_block(const DevSet& d, int m, Radial* __r
                 else fq = fq_sphere_fast(d, q);
             } else if constexpr (KIND == K_BCT) {
                 fq = fq_cyl_radial(d, q);
-            } else if constexpr (KIND == K_CYL_LAT) {
-                fq = fq_cyl_long_iso(d, q) * fq_cyl_radial(d, q);
+            } else if constexpr (KIND == K_ISO_LAT) {
+                fq = lattice_cube(d.s.combo_key) ? pq_cube(d, q) : fq_cyl_long_iso(d, q) * fq_cyl_radial(d, q);
             }
             stage[k][2] = fq;
         }
@@ -2326,7 +2338,7 @@ void build_fast_tables(const csx_scalars& s, const int32_t* L1, const int32_t* L
 void set_consts(DevSet& d) {
     const csx_scalars& s = d.s;
     const double zr = (1 - s.sigma * s.sigma) / (s.sigma * s.sigma);
-    if (kind_of(s.combo_key) == K_BCT || kind_of(s.combo_key) == K_CYL_LAT) {
+    if (kind_of(s.combo_key) == K_BCT || kind_of(s.combo_key) == K_ISO_LAT) {
         d.cyl_g0 = csx::ref_gamma(zr + 1);
         d.cyl_g1 = csx::ref_gamma(zr - 1 / 2.0);
         d.cyl_g2 = csx::ref_gamma(zr - 3 / 2.0);
@@ -2824,7 +2836,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     case K_CUBE: if (prec) FN<K_CUBE, 1>(__VA_ARGS__); else FN<K_CUBE, 0>(__VA_ARGS__); break; \
     case K_CYL_ISO: if (prec) FN<K_CYL_ISO, 1>(__VA_ARGS__); else FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
     case K_CUBE_ZDIR: if (prec) FN<K_CUBE_ZDIR, 1>(__VA_ARGS__); else FN<K_CUBE_ZDIR, 0>(__VA_ARGS__); break; \
-    case K_CYL_LAT: if (prec) FN<K_CYL_LAT, 1>(__VA_ARGS__); else FN<K_CYL_LAT, 0>(__VA_ARGS__); break; \
+    case K_ISO_LAT: if (prec) FN<K_ISO_LAT, 1>(__VA_ARGS__); else FN<K_ISO_LAT, 0>(__VA_ARGS__); break; \
     default: if (prec) FN<K_BCT, 1>(__VA_ARGS__); else FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 
@@ -2839,7 +2851,7 @@ static int radial_table(csx_ctx* c, const csx_grid& g, Radial** out, int* slot) 
     case K_CUBE: out = (prec) ? FN<K_CUBE, 1>(__VA_ARGS__) : FN<K_CUBE, 0>(__VA_ARGS__); break; \
     case K_CYL_ISO: out = (prec) ? FN<K_CYL_ISO, 1>(__VA_ARGS__) : FN<K_CYL_ISO, 0>(__VA_ARGS__); break; \
     case K_CUBE_ZDIR: out = (prec) ? FN<K_CUBE_ZDIR, 1>(__VA_ARGS__) : FN<K_CUBE_ZDIR, 0>(__VA_ARGS__); break; \
-    case K_CYL_LAT: out = (prec) ? FN<K_CYL_LAT, 1>(__VA_ARGS__) : FN<K_CYL_LAT, 0>(__VA_ARGS__); break; \
+    case K_ISO_LAT: out = (prec) ? FN<K_ISO_LAT, 1>(__VA_ARGS__) : FN<K_ISO_LAT, 0>(__VA_ARGS__); break; \
     default: out = (prec) ? FN<K_BCT, 1>(__VA_ARGS__) : FN<K_BCT, 0>(__VA_ARGS__); break; \
     }
 

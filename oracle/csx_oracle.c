@@ -1678,6 +1678,14 @@ static int cylinder_lattice_key(unsigned key)
     return part == 1 && shp == 2 && served_lattice(lt) && (ord == 6 || ord == 7);
 }
 
+/* Isotropic homogeneous cubes on a lattice with the Gaussian peak (calc_GENERIC's cube branch
+ * sets fq = pq). */
+static int cube_lattice_key(unsigned key)
+{
+    const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
+    return part == 4 && shp == 2 && served_lattice(lt) && ord == 7;
+}
+
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)
 {
     const double q = sqrt(qx * qx + qy * qy + qz * qz) + kEps9;
@@ -1731,6 +1739,15 @@ int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* ou
         return 0;
     }
     default: break;
+    }
+    if (cube_lattice_key(key)) { /* Source/sc_gpu_generic.h:290-302, 536-880: fq = pq */
+        if (e->cs != 0) return -1;
+        const double pq = pq_cube(e, q);
+        double I, szqiso;
+        lattice_sums(e, qx, qy, qz, q, &I, &szqiso);
+        const double szq = (1 + (pq / pq) * (I - 1) * exp(-e->dwfactor * q * q));
+        *out = e->base + e->i0 * (szq * pq + e->iso * szqiso * 1.0) + e->ifluc / (1 + q * q * e->rfluc * e->rfluc);
+        return 0;
     }
     if (cylinder_lattice_key(key) && e->ordis == 7) { /* Source/sc_gpu_generic.h:64-74, 536-880 */
         const double pq = pq_cylinder(e, qx, qy, q);

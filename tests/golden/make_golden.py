@@ -184,7 +184,12 @@ CUBICS = [("ia3d_sphere", dict(CFG3, LType=9, uca=40, ucb=40, ucc=40, EditRadius
           ("fcc_cyl_iso", dict(CFG4_BASE, LType=5, Ordis=7, EditRadius=3, Length=30, uca=30, ucb=30, ucc=30,
                                EditDomainSize=250, EditAzi=5, GridPoints=32)),
           ("hex_cyl_iso", dict(CFG4_BASE, LType=1, Ordis=7, EditRadius=4, Length=40, uca=20, ucb=20, ucc=20,
-                               ucgamma=120, EditDomainSize=200, EditAzi=6, iso=0.2, GridPoints=32))]
+                               ucgamma=120, EditDomainSize=200, EditAzi=6, iso=0.2, GridPoints=32)),
+          # isotropic homogeneous cubes with Gaussian peaks (calc_GENERIC sets fq = pq)
+          ("sc_cube_iso", dict(CFG3, LType=7, ComboBoxParticle=4, Ordis=7, EditRadius=10, EditSigma=0.1, uca=25,
+                               ucb=25, ucc=25, GridPoints=32)),
+          ("bcc_cube_iso", dict(CFG3, LType=4, ComboBoxParticle=4, Ordis=7, EditRadius=8, EditSigma=0.15, uca=30,
+                                ucb=30, ucc=30, EditQmax=2, iso=0.3, GridPoints=32))]
 
 
 def write_cubics(manifest):

@@ -227,6 +227,9 @@ BATCH_KINDS = {
     "cylinder-iso-fcc": dict(LType=5, ComboBoxParticle=1, Ordis=7, ComboBoxPeak=1, EditRadius=3, Length=30,
                              EditSigma=0.06, SigmaL=0.06, uca=30, ucb=30, ucc=30, EditDomainSize=250, EditAzi=5,
                              RadButDebyeScherrer=1, HKLmax=5, EditQmax=1, I0=1, Base=0),
+    "cube-iso-sc": dict(LType=7, ComboBoxParticle=4, Ordis=7, ComboBoxPeak=1, EditRadius=10, EditSigma=0.1, uca=25,
+                        ucb=25, ucc=25, EditDomainSize=200, EditAzi=5, RadButDebyeScherrer=1, HKLmax=5, EditQmax=1,
+                        I0=1, Base=0),
 }
 
 

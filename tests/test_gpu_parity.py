@@ -357,13 +357,14 @@ def test_peak_shapes_256_against_oracle_random_pixels(oracle_lib, name):
 
 @pytest.mark.parametrize("prec,tol", [("ref64", 1e-11), ("mixed", 1e-4)])
 @pytest.mark.parametrize("name", ["hex_cyl_zdir.npz", "sq_cyl_zdir.npz", "rect_cyl_zdir.npz", "fcc_cyl_zdir.npz",
-                                  "fcc_cyl_iso.npz", "hex_cyl_iso.npz"])
+                                  "fcc_cyl_iso.npz", "hex_cyl_iso.npz", "sc_cube_iso.npz", "bcc_cube_iso.npz"])
 def test_cylinders_on_lattices(name, prec, tol):
     """Perfectly oriented cylinders with Gaussian peaks on the hexagonal, square and centred
     rectangular 2D lattices and on FCC (the generic kernel's cylinder z-dir branch, served by
     the BCT kernel), and isotropic cylinders on FCC and the hexagonal lattice (the lattice
-    sums of the spheres with the cylinder's P(q) and <F>^2), against the reference's own
-    images; tolerances as for the BCT sets."""
+    sums of the spheres with the cylinder's P(q) and <F>^2), and isotropic homogeneous cubes
+    on SC and BCC (<F>^2 = P(q)), against the reference's own images; tolerances as for the
+    BCT sets."""
     f = fixture(name)
     e, img = _calc(params(name), prec)
     r = rel_err(img, f["image"])

@@ -42,6 +42,10 @@ BCT_CYL += [("cylinder lt%d zdir" % lt, dict(BCT_CYL[0][1], LType=lt, ucc=25, uc
 # isotropic cylinders (L/R >= 2) on lattices
 BCT_CYL += [("cylinder lt%d iso" % lt, dict(BCT_CYL[0][1], LType=lt, Ordis=7, ucc=25, ucgamma=120 if lt == 1 else 90))
             for lt in (1, 3, 4, 5, 7, 8, 17)]
+# isotropic homogeneous cubes on lattices
+BCT_CYL += [("cube lt%d iso" % lt, dict(CFG3, LType=lt, ComboBoxParticle=4, Ordis=7, EditRadius=10, EditSigma=0.1,
+                                        uca=28, ucb=28, ucc=28, EditQmax=1.5, ucgamma=120 if lt == 1 else 90))
+            for lt in (1, 4, 5, 7, 9, 17)]
 CASES = NO_LATTICE + LATTICE + BCT_CYL
 
 
@@ -64,8 +68,10 @@ def test_served_key_matches_oracle(oracle_lib, name, p):
 
 @pytest.mark.parametrize("p", [dict(CFG3, ComboBoxPeak=6), dict(CFG3, ComboBoxPeak=7),
                                dict(CFG3, LType=0), dict(CFG3, ComboBoxParticle=2),
-                               dict(CFG3, CheckBoxTwinned=1)],
-                         ids=["gamma-peak", "anisotropic-gaussian-peak", "lamellar-lattice", "disk", "twinned"])
+                               dict(CFG3, CheckBoxTwinned=1),
+                               dict(CFG3, ComboBoxParticle=4, Ordis=7, ComboBoxInterior=1)],
+                         ids=["gamma-peak", "anisotropic-gaussian-peak", "lamellar-lattice", "disk", "twinned",
+                              "core-shell-cube-lattice"])
 def test_unserved_combinations_are_refused(p):
     e = SasCalc(device=0, params=dict(p, GridPoints=G))
     with pytest.raises(CsxError):
