@@ -68,6 +68,13 @@ enum {
 #define CSX_KEY_SPHERE_BCT_ISO    0x00020807u
 #define CSX_KEY_SPHERE_NONE_GAUSS 0x00000C00u  /* no lattice: the isotropic sphere's value */
 #define CSX_KEY_SPHERE_NONE_ZDIR  0x00000C06u
+/* The other lattice keys served (the generic kernel), without names: on the 2D lattices
+ * (LType 1-3), the cubic ones (4, 5, 7, 9, 10, 11, 17), HCP (6), BCT (8) and orthorhombic (18)
+ *   spheres, peak shapes 1-6, any orientation distribution: (shp<<16) | (LType<<8) | ordis;
+ *   perfectly oriented cylinders, Gaussian peak:            0x01020006 | (LType<<8);
+ *   isotropic cylinders (L/R >= 2), Gaussian peak:          0x01020007 | (LType<<8);
+ *   isotropic homogeneous cubes, Gaussian peak:             0x04020007 | (LType<<8).
+ * Any other key is refused by csx_upload / preparation with CSX_E_ARG. */
 
 #define CSX_NCOEF 151   /* coeffarray_len */
 #define CSX_NIM2 131    /* imax2d_len */
