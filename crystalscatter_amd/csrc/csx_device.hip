@@ -70,14 +70,14 @@ inline int kind_of(uint32_t key) {
     // orthorhombic without reflections).  Spheres on them with the Lorentzian, Gaussian, mod.
     // 1/2 Lorentzian, Pseudo-Voigt or Pearson VII peak, any orientation distribution (which
     // enters neither the sphere's factors nor these sums); perfectly oriented cylinders on
-    // them with the Gaussian peak (calc_GENERIC, and the BCT key's own kernel).
+    // them (calc_GENERIC, and the BCT Gaussian key's own kernel).
     const uint32_t part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
     const bool lattice = (lt >= 1 && lt <= 11) || lt == 17 || lt == 18;
     if (part == 0 && shp >= 1 && shp <= 6 && lattice && ord <= 13) return K_FCC;
-    if (part == 1 && shp == 2 && lattice && ord == 6) return K_BCT;
-    // isotropic cylinders and cubes on them with the Gaussian peak: P(q) and <F>^2 depend on
-    // |q| only, so they take the sums of the sphere lattices with the particle's factors
-    if ((part == 1 || part == 4) && shp == 2 && lattice && ord == 7) return K_ISO_LAT;
+    if (part == 1 && shp >= 1 && shp <= 6 && lattice && ord == 6) return K_BCT;
+    // isotropic cylinders and cubes on them: P(q) and <F>^2 depend on |q| only, so they take
+    // the sums of the sphere lattices with the particle's factors (peak shapes 1-6 for all)
+    if ((part == 1 || part == 4) && shp >= 1 && shp <= 6 && lattice && ord == 7) return K_ISO_LAT;
     return -1;
 }
 // Isotropic cubes on a lattice (kind K_ISO_LAT; the other particle of that kind is the cylinder).
@@ -87,7 +87,8 @@ inline int shape_of(uint32_t key) { return (int)(key >> 16 & 0xff); }
 // The arithmetic a request runs in: sets with a peak shape other than the Gaussian have only
 // the FP64 reference-order sum (the mixed path's windows assume the Gaussian's tails).
 inline int effective_precision(uint32_t key, int precision) {
-    return (precision == CSX_PREC_MIXED && kind_of(key) == K_FCC && shape_of(key) != 2) ? CSX_PREC_REF64 : precision;
+    return (precision == CSX_PREC_MIXED && lattice_kind(kind_of(key)) && shape_of(key) != 2) ? CSX_PREC_REF64
+                                                                                            : precision;
 }
 // Kinds whose mixed peak sum runs the precise pair loop (FP64 exponents): their structure
 // factor 1 + (fq/pq)(2*pi*I/V - 1)*dwe cancels over much of the pattern (fq/pq reaches 1e8

@@ -1111,7 +1111,6 @@ std::string prepare(const ParamSet& p, PreparedSet& P, bool rows_on_device) {
             }
         }
         if (shp < 1 || shp > 6) return "peak shape not supported by the device path";
-        if (shp != 2 && p.particle != 0) return "peak shapes other than the Gaussian are served for spheres";
         // the generic kernel leaves P(q) of a cube on a lattice unset for the other interiors
         if (p.particle == 4 && p.interior != 0) return "cubes on a lattice need a homogeneous interior";
         if (p.twinned) return "twinning not supported by the device path";

@@ -71,9 +71,10 @@ enum {
 /* The other lattice keys served (the generic kernel), without names: on the 2D lattices
  * (LType 1-3), the cubic ones (4, 5, 7, 9, 10, 11, 17), HCP (6), BCT (8) and orthorhombic (18)
  *   spheres, peak shapes 1-6, any orientation distribution: (shp<<16) | (LType<<8) | ordis;
- *   perfectly oriented cylinders, Gaussian peak:            0x01020006 | (LType<<8);
- *   isotropic cylinders (L/R >= 2), Gaussian peak:          0x01020007 | (LType<<8);
- *   isotropic homogeneous cubes, Gaussian peak:             0x04020007 | (LType<<8).
+ *   perfectly oriented cylinders, peak shapes 1-6:          0x01000006 | (shp<<16) | (LType<<8);
+ *   isotropic cylinders (L/R >= 2), peak shapes 1-6:        0x01000007 | (shp<<16) | (LType<<8);
+ *   isotropic homogeneous cubes, peak shapes 1-6:           0x04000007 | (shp<<16) | (LType<<8).
+ * Peak shapes other than the Gaussian (2) run the FP64 sums in either precision.
  * Any other key is refused by csx_upload / preparation with CSX_E_ARG. */
 
 #define CSX_NCOEF 151   /* coeffarray_len */

@@ -1669,21 +1669,21 @@ static int sphere_lattice_shape_key(unsigned key)
     return part == 0 && shp >= 1 && shp <= 6 && served_lattice(lt) && ord <= 13;
 }
 
-/* Perfectly oriented (ordis 6) or isotropic (7) cylinders on a lattice with the Gaussian peak:
- * the BCT z-dir key's own kernel (Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h) and calc_GENERIC's
- * cylinder branches (Source/sc_gpu_generic.h:62-121) with the same Gaussian sums. */
+/* Perfectly oriented (ordis 6) or isotropic (7) cylinders on a lattice with peak shapes 1-6:
+ * the BCT z-dir Gaussian key's own kernel (Source/sc_gpu_pCyl_pGau_lBCT_oZDi.h) and
+ * calc_GENERIC's cylinder branches (Source/sc_gpu_generic.h:62-121) with the same sums. */
 static int cylinder_lattice_key(unsigned key)
 {
     const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    return part == 1 && shp == 2 && served_lattice(lt) && (ord == 6 || ord == 7);
+    return part == 1 && shp >= 1 && shp <= 6 && served_lattice(lt) && (ord == 6 || ord == 7);
 }
 
-/* Isotropic homogeneous cubes on a lattice with the Gaussian peak (calc_GENERIC's cube branch
+/* Isotropic homogeneous cubes on a lattice with peak shapes 1-6 (calc_GENERIC's cube branch
  * sets fq = pq). */
 static int cube_lattice_key(unsigned key)
 {
     const unsigned part = key >> 24, shp = key >> 16 & 0xff, lt = key >> 8 & 0xff, ord = key & 0xff;
-    return part == 4 && shp == 2 && served_lattice(lt) && ord == 7;
+    return part == 4 && shp >= 1 && shp <= 6 && served_lattice(lt) && ord == 7;
 }
 
 int csxo_pixel(const csxo_engine* e, double qx, double qy, double qz, double* out)

@@ -189,7 +189,15 @@ CUBICS = [("ia3d_sphere", dict(CFG3, LType=9, uca=40, ucb=40, ucc=40, EditRadius
           ("sc_cube_iso", dict(CFG3, LType=7, ComboBoxParticle=4, Ordis=7, EditRadius=10, EditSigma=0.1, uca=25,
                                ucb=25, ucc=25, GridPoints=32)),
           ("bcc_cube_iso", dict(CFG3, LType=4, ComboBoxParticle=4, Ordis=7, EditRadius=8, EditSigma=0.15, uca=30,
-                                ucb=30, ucc=30, EditQmax=2, iso=0.3, GridPoints=32))]
+                                ucb=30, ucc=30, EditQmax=2, iso=0.3, GridPoints=32)),
+          # the other peak shapes with cylinders and cubes (FP64 sums)
+          ("bct_cyl_zdir_lorentz", dict(CFG4_BASE, ComboBoxPeak=0, EditRadius=4, Length=30, uca=25, ucb=25, ucc=40,
+                                        EditDomainSize=250, EditAzi=5, GridPoints=32)),
+          ("fcc_cyl_iso_pearson", dict(CFG4_BASE, LType=5, Ordis=7, ComboBoxPeak=5, EditPeakPar=2, EditRadius=3,
+                                       Length=30, uca=30, ucb=30, ucc=30, EditDomainSize=250, EditAzi=5,
+                                       GridPoints=32)),
+          ("sc_cube_iso_pvoigt", dict(CFG3, LType=7, ComboBoxParticle=4, Ordis=7, ComboBoxPeak=4, EditPeakPar=40,
+                                      EditRadius=10, EditSigma=0.1, uca=25, ucb=25, ucc=25, GridPoints=32))]
 
 
 def write_cubics(manifest):

@@ -357,7 +357,8 @@ def test_peak_shapes_256_against_oracle_random_pixels(oracle_lib, name):
 
 @pytest.mark.parametrize("prec,tol", [("ref64", 1e-11), ("mixed", 1e-4)])
 @pytest.mark.parametrize("name", ["hex_cyl_zdir.npz", "sq_cyl_zdir.npz", "rect_cyl_zdir.npz", "fcc_cyl_zdir.npz",
-                                  "fcc_cyl_iso.npz", "hex_cyl_iso.npz", "sc_cube_iso.npz", "bcc_cube_iso.npz"])
+                                  "fcc_cyl_iso.npz", "hex_cyl_iso.npz", "sc_cube_iso.npz", "bcc_cube_iso.npz",
+                                  "bct_cyl_zdir_lorentz.npz", "fcc_cyl_iso_pearson.npz", "sc_cube_iso_pvoigt.npz"])
 def test_cylinders_on_lattices(name, prec, tol):
     """Perfectly oriented cylinders with Gaussian peaks on the hexagonal, square and centred
     rectangular 2D lattices and on FCC (the generic kernel's cylinder z-dir branch, served by

@@ -46,6 +46,10 @@ BCT_CYL += [("cylinder lt%d iso" % lt, dict(BCT_CYL[0][1], LType=lt, Ordis=7, uc
 BCT_CYL += [("cube lt%d iso" % lt, dict(CFG3, LType=lt, ComboBoxParticle=4, Ordis=7, EditRadius=10, EditSigma=0.1,
                                         uca=28, ucb=28, ucc=28, EditQmax=1.5, ucgamma=120 if lt == 1 else 90))
             for lt in (1, 4, 5, 7, 9, 17)]
+# the other peak shapes with cylinders (both orientations) and cubes
+BCT_CYL += [("%s shp%d" % (name, pk + 1), dict(p, ComboBoxPeak=pk, EditPeakPar={4: 30, 5: 1.5}.get(pk, 0)))
+            for name, p in (BCT_CYL[0], BCT_CYL[-1], [c for c in BCT_CYL if c[0] == "cylinder lt5 iso"][0])
+            for pk in (0, 2, 3, 4, 5)]
 CASES = NO_LATTICE + LATTICE + BCT_CYL
 
 

@@ -57,7 +57,8 @@ def test_vitess_golden_image(oracle_lib):
                                   "ortho_sphere_iso.npz", "fcc_sphere_ordis1.npz", "bcc_sphere_zdir_lorentz.npz",
                                   "hex_sphere.npz", "hex_cyl_zdir.npz", "sq_cyl_zdir.npz", "rect_cyl_zdir.npz",
                                   "fcc_cyl_zdir.npz", "fcc_cyl_iso.npz", "hex_cyl_iso.npz", "sc_cube_iso.npz",
-                                  "bcc_cube_iso.npz"])
+                                  "bcc_cube_iso.npz", "bct_cyl_zdir_lorentz.npz", "fcc_cyl_iso_pearson.npz",
+                                  "sc_cube_iso_pvoigt.npz"])
 def test_tables_and_image_bitexact(oracle_lib, name):
     f = fixture(name)
     e = _engine(oracle_lib, params(name))
