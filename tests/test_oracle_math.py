@@ -183,6 +183,74 @@ def test_bct_reflection_lists_brute_force(oracle_lib):
     assert (L2[:, 5] == 4).all()
 
 
+def _cubic_engine(oracle_lib, ltype, H=6):
+    p = dict(LType=ltype, ComboBoxParticle=0, Ordis=0, ComboBoxPeak=1, EditRadius=4, EditSigma=0.08, uca=30,
+             ucb=30, ucc=30, RadButDebyeScherrer=1, EditDomainSize=200, EditAzi=5, HKLmax=H, GridPoints=8)
+    e = oracle_lib.OracleEngine(p)
+    e.prepare()
+    return e
+
+
+def test_fd3m_reflections_are_those_of_the_diamond_structure(oracle_lib):
+    """Fd3m (cubic group Fd--): the accepted reflections with a nonzero structure factor are
+    exactly those where the diamond basis' |F|^2 = |F_fcc|^2 |1 + i^(h+k+l)|^2 is nonzero (all
+    odd, or all even with h+k+l = 4n), and |F|^2 is 32 (all odd) or 64."""
+    e = _cubic_engine(oracle_lib, 17)
+    L2 = e.table("latpar2").reshape(-1, 6)[1:]
+    got = {tuple(x[1:4]) for x in L2 if x[5] != 0}
+    r = range(-6, 7)
+    want = {(h, k, l) for h in r for k in r for l in r
+            if (h % 2 == k % 2 == l % 2) and (h % 2 == 1 or (h + k + l) % 4 == 0) and (h, k, l) != (0, 0, 0)}
+    assert got == want
+    for x in L2:
+        if x[5]:
+            assert x[5] == (32 if x[1] % 2 else 64)
+
+
+def test_ia3d_reflection_conditions(oracle_lib):
+    """Ia3d (group 230): hkl: h+k+l = 2n; 0kl: k, l = 2n; hhl: 2h+l = 4n; h00: h = 4n
+    (International Tables, with the cyclic permutations), checked by brute force."""
+    e = _cubic_engine(oracle_lib, 9)
+    L2 = e.table("latpar2").reshape(-1, 6)[1:]
+    r = range(-6, 7)
+
+    def ok(h, k, l):
+        if (h + k + l) % 2:
+            return False
+        v = (h, k, l)
+        for z in range(3):
+            a, b, c = v[z], v[(z + 1) % 3], v[(z + 2) % 3]
+            if a == 0 and (b % 2 or c % 2):
+                return False
+            if b == c and a != 0 and ((2 * b + a) % 4 or a % 2):
+                return False
+            if b == 0 and c == 0 and a % 4:
+                return False
+        return True
+
+    want = {(h, k, l) for h in r for k in r for l in r if (h, k, l) != (0, 0, 0) and ok(h, k, l)}
+    assert set(map(tuple, L2[:, 1:4])) == want
+    assert (L2[:, 5] == 4).all()   # body-centred two-atom basis, h+k+l even
+
+
+def test_2d_hexagonal_lattice_reciprocal_vectors(oracle_lib):
+    """Hexagonal 2D lattice (a = b, gamma = 120 deg): every (h, k) with l = 0 is accepted with
+    multiplicity 1, and |q_hk0| = (4 pi / (sqrt(3) a)) sqrt(h^2 + k^2 + h k) -- the hexagonal
+    d-spacing, which a sign slip in the plane cell's reciprocal basis would change to h^2 + k^2 - h k."""
+    p = dict(LType=1, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, EditRadius=3, Length=30, EditSigma=0.06,
+             SigmaL=0.06, uca=20, ucb=20, ucc=20, ucgamma=120, RadButDebyeScherrer=1, EditDomainSize=200,
+             EditAzi=5, HKLmax=4, GridPoints=8, theta=0, phi=0)
+    e = oracle_lib.OracleEngine(p)
+    e.prepare()
+    L2 = e.table("latpar2").reshape(-1, 6)[1:]
+    L3 = e.table("latpar3").reshape(-1, 17)[1:]
+    assert len(L2) == 9 * 9 * 5 - 1 and (L2[:, 4] == 1).all() and (L2[:, 5] == 1).all()
+    sel = L2[:, 3] == 0
+    h, k = L2[sel, 1].astype(float), L2[sel, 2].astype(float)
+    expect = 4 * np.pi / (np.sqrt(3) * 20) * np.sqrt(h * h + k * k + h * k)
+    assert rel_err(L3[sel, 5], expect).max() < 2e-6
+
+
 @pytest.mark.parametrize("ltype,a,c", [(5, 30.0, 30.0), (8, 25.0, 40.0)])
 def test_reflection_vectors_have_lattice_lengths(oracle_lib, ltype, a, c):
     e = _lattice_engine(oracle_lib, ltype, a, c)
