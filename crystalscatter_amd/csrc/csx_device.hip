@@ -1593,12 +1593,13 @@ struct TileSum {
 };
 
 
-// Pixel value of a set of a batch.  The kinds without a lattice read the set's constants from a
-// private copy: read through the reference into the sets array, the isotropic cylinder's P(q)
-// came out wrong above its series radius.  The lattice kinds read them in place.
+// Pixel value of a set of a batch.  The kinds without a lattice and the isotropic particles on
+// lattices read the set's constants from a private copy: read through the reference into the
+// sets array, the isotropic cylinder's P(q) came out wrong above its series radius (with and
+// without a lattice).  The sphere and z-oriented cylinder lattice kinds read them in place.
 template <int KIND, int PREC>
 __device__ __forceinline__ double set_pixel(const FitSet& f, PixQ& P3, const Radial* rv) {
-    if constexpr (lattice_kind(KIND)) {
+    if constexpr (KIND == K_FCC || KIND == K_BCT) {
         return pixel_value<KIND, PREC>(f.d, P3, nullptr, rv);
     } else {
         const DevSet d = f.d;

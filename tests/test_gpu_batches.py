@@ -231,6 +231,13 @@ BATCH_KINDS = {
                         ucb=25, ucc=25, EditDomainSize=200, EditAzi=5, RadButDebyeScherrer=1, HKLmax=5, EditQmax=1,
                         I0=1, Base=0),
 }
+# the same lattice kinds with the q grid from the detector geometry (no radial table: the
+# particle's factors are evaluated per pixel)
+_BEAM = dict(CenterBeam=1, EditPixelX=2.5, EditPixelY=2.5, EditPixelNoX=64, EditPixelNoY=64, EditDet=2.0,
+             EditWavelength=0.154)
+BATCH_KINDS["cylinder-iso-fcc-beam"] = dict(BATCH_KINDS["cylinder-iso-fcc"], **_BEAM)
+BATCH_KINDS["cube-iso-sc-beam"] = dict(BATCH_KINDS["cube-iso-sc"], **_BEAM)
+BATCH_KINDS["cylinder-iso-beam"] = dict(BATCH_KINDS["cylinder-iso"], **_BEAM)
 
 
 @pytest.mark.parametrize("prec", ["ref64", "mixed"])
