@@ -238,6 +238,14 @@ _BEAM = dict(CenterBeam=1, EditPixelX=2.5, EditPixelY=2.5, EditPixelNoX=64, Edit
 BATCH_KINDS["cylinder-iso-fcc-beam"] = dict(BATCH_KINDS["cylinder-iso-fcc"], **_BEAM)
 BATCH_KINDS["cube-iso-sc-beam"] = dict(BATCH_KINDS["cube-iso-sc"], **_BEAM)
 BATCH_KINDS["cylinder-iso-beam"] = dict(BATCH_KINDS["cylinder-iso"], **_BEAM)
+BATCH_KINDS["sphere-fcc-beam"] = dict(LType=5, ComboBoxParticle=0, Ordis=0, ComboBoxPeak=1, EditRadius=8,
+                                      EditSigma=0.08, uca=30, ucb=30, ucc=30, EditDomainSize=200,
+                                      RadButDebyeScherrer=1, EditDebyeWaller=1.5, EditAzi=5, HKLmax=5, EditQmax=1,
+                                      I0=1, Base=0, **_BEAM)
+BATCH_KINDS["cylinder-zdir-bct-beam"] = dict(LType=8, ComboBoxParticle=1, Ordis=6, ComboBoxPeak=1, theta=0, phi=0,
+                                             EditRadius=4, Length=30, EditSigma=0.06, SigmaL=0.06, uca=25, ucb=25,
+                                             ucc=40, EditDomainSize=250, EditAzi=5, RadButDebyeScherrer=1, HKLmax=5,
+                                             EditQmax=1, I0=1, Base=0, **_BEAM)
 
 
 @pytest.mark.parametrize("prec", ["ref64", "mixed"])
